@@ -1,0 +1,237 @@
+"""ctypes mirror of include/bd.h and the loader of the in-tree libbd.so.
+
+`SceneBuffers` marshals a CompiledScene into a `bd_scene` struct and keeps the numpy arrays it
+points at alive for as long as the struct is used.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent.parent          # break-that-denim_amd/
+LIB_PATH = PKG_DIR / "libbd.so"
+
+BD_OK = 0
+BD_MAX_LINE = 1024
+BD_MAX_LEVELS = 12
+BD_MAX_NEWTON = 8
+BD_MAX_GRIPS = 16
+BD_PRECISION_FP32 = 0
+
+(BD_BUF_X, BD_BUF_V, BD_BUF_DX, BD_BUF_PRED, BD_BUF_DELTA, BD_BUF_RHS, BD_BUF_LEVEL_X, BD_BUF_LEVEL_B,
+ BD_BUF_LEVEL_R, BD_BUF_STENCIL, BD_BUF_FIXED, BD_BUF_PROLONG, BD_BUF_REST, BD_BUF_INTACT, BD_BUF_CUTMASK,
+ BD_BUF_SEGFLAGS, BD_BUF_FAILSITE, BD_BUF_TAILS, BD_BUF_FIELD, BD_BUF_LAST_UPDATE, BD_BUF_ENERGY) = range(21)
+
+
+class bd_yarn(C.Structure):
+    _fields_ = [("linear_density", C.c_double), ("ea_low", C.c_double), ("ea_high", C.c_double),
+                ("crimp_strain", C.c_double), ("ea_compress", C.c_double), ("axial_viscosity", C.c_double),
+                ("bend_rigidity", C.c_double), ("bend_viscosity", C.c_double), ("strength", C.c_double),
+                ("softening_opening", C.c_double), ("yarns_per_span", C.c_int32), ("pad_", C.c_int32)]
+
+
+class bd_material(C.Structure):
+    _fields_ = [("pitch_u", C.c_double), ("pitch_v", C.c_double), ("yarns", bd_yarn * 2),
+                ("shear_stiffness", C.c_double), ("crossing_friction", C.c_double),
+                ("crossing_normal0", C.c_double), ("crossing_wrap", C.c_double), ("air_drag", C.c_double)]
+
+
+class bd_layer(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("node_offset", C.c_int32), ("bundle_u", C.c_int32),
+                ("bundle_v", C.c_int32), ("field_offset", C.c_int32), ("field_k", C.c_int32),
+                ("field_m", C.c_int32), ("weave_pitch_u", C.c_double), ("weave_pitch_v", C.c_double),
+                ("material", bd_material)]
+
+
+class bd_scene(C.Structure):
+    _fields_ = [("node_count", C.c_int32), ("layer_count", C.c_int32), ("layers", C.POINTER(bd_layer)),
+                ("position", C.POINTER(C.c_double)), ("mass", C.POINTER(C.c_double)),
+                ("pinned", C.POINTER(C.c_uint8)), ("rest_length", C.POINTER(C.c_double)),
+                ("softening", C.POINTER(C.c_double)), ("seg_flags", C.POINTER(C.c_uint16)),
+                ("intact", C.POINTER(C.c_double)), ("cut_mask", C.POINTER(C.c_uint32)),
+                ("fail_site", C.POINTER(C.c_double)), ("crossing_normal0", C.POINTER(C.c_double)),
+                ("tails", C.POINTER(C.c_double)), ("crossing_count", C.c_int32), ("pad_", C.c_int32),
+                ("field_strength", C.POINTER(C.c_float)), ("field_damage", C.POINTER(C.c_uint8)),
+                ("field_area", C.POINTER(C.c_uint8))]
+
+
+class bd_params(C.Structure):
+    _fields_ = [("h", C.c_double), ("gravity", C.c_double * 3), ("newton_iterations", C.c_int32),
+                ("newton_max", C.c_int32), ("newton_tolerance", C.c_double), ("pcg_iterations", C.c_int32),
+                ("pcg_max_iterations", C.c_int32), ("pcg_rel_tol", C.c_double), ("coarse_sweeps", C.c_int32),
+                ("max_coarsest_nodes", C.c_int32), ("lag_coarse", C.c_int32), ("line_search", C.c_int32),
+                ("history", C.c_int32), ("precision", C.c_int32)]
+
+
+class bd_grips(C.Structure):
+    _fields_ = [("grip_count", C.c_int32), ("entry_count", C.c_int32), ("node", C.POINTER(C.c_int32)),
+                ("grip", C.POINTER(C.c_int32)), ("weight", C.POINTER(C.c_double)),
+                ("offset", C.POINTER(C.c_double)), ("goal", C.POINTER(C.c_double)),
+                ("stiffness", C.POINTER(C.c_double))]
+
+
+class bd_stats(C.Structure):
+    _fields_ = [("newton_iterations", C.c_int32), ("pcg_iterations", C.c_int32 * BD_MAX_NEWTON),
+                ("rel_residual", C.c_double * BD_MAX_NEWTON), ("alpha", C.c_double * BD_MAX_NEWTON),
+                ("max_update", C.c_double * BD_MAX_NEWTON), ("rejections", C.c_int32),
+                ("ruptures", C.c_int32), ("pulled_out", C.c_int32), ("level_count", C.c_int32),
+                ("level_nodes", C.c_int32 * BD_MAX_LEVELS)]
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _yarn_struct(y) -> bd_yarn:
+    return bd_yarn(y["linearDensity"], y["eaLow"], y["eaHigh"], y["crimpStrain"], y["eaCompress"],
+                   y["axialViscosity"], y["bendRigidity"], y["bendViscosity"], y["strength"],
+                   y["softeningOpening"], int(y.get("yarnsPerSpan", 1)), 0)
+
+
+def layer_struct(L) -> bd_layer:
+    m = L.material
+    mat = bd_material(m["pitchU"], m["pitchV"], (bd_yarn * 2)(_yarn_struct(m["yarns"][0]), _yarn_struct(m["yarns"][1])),
+                      m["shearStiffness"], m["crossingFriction"], m["crossingNormal0"], m["crossingWrap"], m["airDrag"])
+    return bd_layer(L.nx, L.ny, L.node_offset, L.bu, L.bv, L.field_offset, L.K, L.M,
+                    L.weave["pitchU"], L.weave["pitchV"], mat)
+
+
+class SceneBuffers:
+    """A `bd_scene` pointing into contiguous copies of a CompiledScene's arrays."""
+
+    def __init__(self, S):
+        self.N = S.node_count
+        self.keep = {
+            "position": np.ascontiguousarray(S.position, dtype=np.float64).ravel(),
+            "mass": np.ascontiguousarray(S.mass, dtype=np.float64),
+            "pinned": np.ascontiguousarray(S.pinned, dtype=np.uint8),
+            "rest_length": np.ascontiguousarray(S.rest_length, dtype=np.float64),
+            "softening": np.ascontiguousarray(S.softening, dtype=np.float64),
+            "seg_flags": np.ascontiguousarray(S.seg_flags, dtype=np.uint16),
+            "intact": np.ascontiguousarray(S.intact, dtype=np.float64),
+            "cut_mask": np.ascontiguousarray(S.cut_mask, dtype=np.uint32),
+            "fail_site": np.ascontiguousarray(S.fail_site, dtype=np.float64),
+            "crossing_normal0": np.ascontiguousarray(S.crossing_normal0, dtype=np.float64),
+            "tails": np.ascontiguousarray(S.tails, dtype=np.float64),
+            "field_strength": np.ascontiguousarray(S.field_strength, dtype=np.float32),
+            "field_damage": np.ascontiguousarray(S.field_damage, dtype=np.uint8),
+            "field_area": np.ascontiguousarray(S.field_area, dtype=np.uint8),
+        }
+        self.layers = (bd_layer * len(S.layers))(*[layer_struct(L) for L in S.layers])
+        k = self.keep
+        self.scene = bd_scene(
+            S.node_count, len(S.layers), C.cast(self.layers, C.POINTER(bd_layer)),
+            _ptr(k["position"], C.c_double), _ptr(k["mass"], C.c_double), _ptr(k["pinned"], C.c_uint8),
+            _ptr(k["rest_length"], C.c_double), _ptr(k["softening"], C.c_double),
+            _ptr(k["seg_flags"], C.c_uint16), _ptr(k["intact"], C.c_double), _ptr(k["cut_mask"], C.c_uint32),
+            _ptr(k["fail_site"], C.c_double), _ptr(k["crossing_normal0"], C.c_double),
+            _ptr(k["tails"], C.c_double), S.crossing_count, 0, _ptr(k["field_strength"], C.c_float),
+            _ptr(k["field_damage"], C.c_uint8), _ptr(k["field_area"], C.c_uint8))
+
+
+def make_params(h=1 / 120, gravity=(0.0, 0.0, -9.81), newton_iterations=2, newton_max=None,
+                newton_tolerance=1e-5, pcg_iterations=2, pcg_rel_tol=0.0, pcg_max_iterations=200,
+                coarse_sweeps=2, max_coarsest_nodes=400, lag_coarse=True, line_search=True, history=True,
+                precision=BD_PRECISION_FP32) -> bd_params:
+    """Solver parameters; the defaults are the reference's production schedule (GPU_DEFAULTS,
+    gpu/solver.ts:61-72: h 1/120, 2 Newton x 2 PCG, 2 coarse sweeps, coarsest <= 400 nodes)."""
+    p = bd_params()
+    p.h = h
+    p.gravity[:] = list(gravity)
+    p.newton_iterations = newton_iterations
+    p.newton_max = newton_iterations if newton_max is None else newton_max
+    p.newton_tolerance = newton_tolerance
+    p.pcg_iterations = pcg_iterations
+    p.pcg_max_iterations = pcg_max_iterations
+    p.pcg_rel_tol = pcg_rel_tol
+    p.coarse_sweeps = coarse_sweeps
+    p.max_coarsest_nodes = max_coarsest_nodes
+    p.lag_coarse = int(bool(lag_coarse))
+    p.line_search = int(bool(line_search))
+    p.history = int(bool(history))
+    p.precision = precision
+    return p
+
+
+class GripBuffers:
+    """A `bd_grips` struct for a list of grips, each a dict with nodes, weights, offsets (E, 3),
+    goal (3,) and stiffness k (N/m per unit weight)."""
+
+    def __init__(self, grips):
+        nodes, gid, w, off = [], [], [], []
+        goals, ks = [], []
+        for g_i, g in enumerate(grips):
+            nodes.extend(int(n) for n in g["nodes"])
+            gid.extend([g_i] * len(g["nodes"]))
+            w.extend(float(x) for x in g["weights"])
+            off.append(np.asarray(g["offsets"], dtype=np.float64).reshape(-1, 3))
+            goals.append(np.asarray(g["goal"], dtype=np.float64))
+            ks.append(float(g["k"]))
+        self.node = np.asarray(nodes, dtype=np.int32)
+        self.grip = np.asarray(gid, dtype=np.int32)
+        self.weight = np.asarray(w, dtype=np.float64)
+        self.offset = np.ascontiguousarray(np.concatenate(off) if off else np.zeros((0, 3))).ravel()
+        self.goal = np.ascontiguousarray(np.concatenate(goals) if goals else np.zeros(0))
+        self.k = np.asarray(ks, dtype=np.float64)
+        self.struct = bd_grips(len(grips), len(self.node), _ptr(self.node, C.c_int32), _ptr(self.grip, C.c_int32),
+                               _ptr(self.weight, C.c_double), _ptr(self.offset, C.c_double),
+                               _ptr(self.goal, C.c_double), _ptr(self.k, C.c_double))
+
+
+_lib = None
+
+
+def load_library(path: os.PathLike | None = None):
+    """Load libbd.so (built in-tree by the package's build step). Raises if it is missing: there
+    is no fallback path for the solve."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    lib = C.CDLL(str(p))
+    P = C.POINTER
+    vp = C.c_void_p
+    lib.bd_last_error.restype = C.c_char_p
+    lib.bd_last_error.argtypes = [vp]
+    lib.bd_create.argtypes = [P(bd_scene), P(bd_params), P(vp)]
+    lib.bd_destroy.argtypes = [vp]
+    lib.bd_set_state.argtypes = [vp, P(C.c_double), P(C.c_double)]
+    lib.bd_set_segments.argtypes = [vp, P(C.c_double), P(C.c_double), P(C.c_uint32), P(C.c_uint16),
+                                    P(C.c_double), P(C.c_double)]
+    lib.bd_set_field.argtypes = [vp, P(C.c_uint8)]
+    lib.bd_set_grips.argtypes = [vp, P(bd_grips)]
+    lib.bd_set_timestep.argtypes = [vp, C.c_double]
+    lib.bd_set_history.argtypes = [vp, C.c_int]
+    lib.bd_set_newton_tolerance.argtypes = [vp, C.c_double]
+    lib.bd_set_pcg.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+    lib.bd_substep.argtypes = [vp, C.c_int, C.c_int]
+    for name in ("bd_predict", "bd_vcycle", "bd_pcg", "bd_line_search", "bd_finalize", "bd_damage",
+                 "bd_slip", "bd_sync"):
+        getattr(lib, name).argtypes = [vp]
+    lib.bd_assemble.argtypes = [vp, C.c_int]
+    lib.bd_smooth.argtypes = [vp, C.c_int, C.c_int]
+    lib.bd_residual.argtypes = [vp, C.c_int]
+    lib.bd_newton_apply.argtypes = [vp, C.c_int]
+    lib.bd_read.argtypes = [vp, C.c_int, C.c_int, vp, C.c_size_t]
+    lib.bd_write.argtypes = [vp, C.c_int, C.c_int, vp, C.c_size_t]
+    lib.bd_buffer_bytes.argtypes = [vp, C.c_int, C.c_int]
+    lib.bd_buffer_bytes.restype = C.c_size_t
+    lib.bd_get_stats.argtypes = [vp, P(bd_stats)]
+    if path is None:
+        _lib = lib
+    return lib
+
+
+# Every entry point include/bd.h declares (checked by the CPU test that loads the library).
+EXPORTED = (
+    "bd_last_error", "bd_create", "bd_destroy", "bd_set_state", "bd_set_segments", "bd_set_field",
+    "bd_set_grips", "bd_set_timestep", "bd_set_history", "bd_set_newton_tolerance", "bd_set_pcg",
+    "bd_substep", "bd_predict", "bd_assemble", "bd_vcycle", "bd_smooth", "bd_residual", "bd_pcg",
+    "bd_line_search", "bd_newton_apply", "bd_finalize", "bd_damage", "bd_slip", "bd_sync", "bd_read",
+    "bd_write", "bd_buffer_bytes", "bd_get_stats",
+)

@@ -1,0 +1,233 @@
+/*
+ * bd.h -- C ABI of the MI355X implicit cloth solve (libbd.so).
+ *
+ * One projected-Newton substep of a woven yarn-crossing lattice: inertial prediction, grip
+ * springs, assembly of the clamp-projected Hessian as a radius-2 block stencil, Galerkin
+ * multigrid hierarchy, PCG preconditioned by one symmetric V-cycle with block-tridiagonal
+ * yarn-line smoothers, line search over {1, 1/2, 1/4, 1/8}, commit, then the yarn damage field
+ * (cohesive damage, pull-out, span rupture) and crossing slip.
+ *
+ * The entry points replace the WebGPU `GpuSolver` of the reference
+ * (src/simulation/gpu/solver.ts:194-838); each declaration cites the member it stands for.
+ * Plain C types only: the host owns every input array for the duration of a call (the library
+ * copies what it keeps to device memory, as queue.writeBuffer does) and readbacks copy into
+ * caller memory. Work is enqueued on one HIP stream per solver handle; calls that return data
+ * synchronise that stream. A handle is not thread-safe.
+ *
+ * Every call returns BD_OK (0) or a negative BD_ERR_* code; bd_last_error() gives the message.
+ */
+#ifndef BD_H
+#define BD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BD_OK 0
+#define BD_ERR_INVALID (-1)   /* bad argument or inconsistent scene */
+#define BD_ERR_HIP (-2)       /* HIP runtime failure (message has the HIP error string) */
+#define BD_ERR_LIMIT (-3)     /* size beyond a documented limit (line length, bundle, nodes) */
+#define BD_ERR_NOMEM (-4)     /* device allocation failed */
+#define BD_ERR_STATE (-5)     /* call order (e.g. a V-cycle before any assembly) */
+
+/* Limits (the reference's own are MAX_LINE 384, 2^20 contact nodes, 3 layers). */
+#define BD_MAX_LINE 1024      /* longest yarn line solved in one workgroup's LDS */
+#define BD_MAX_LEVELS 12
+#define BD_MAX_NEWTON 8
+#define BD_MAX_GRIPS 16
+
+/* Yarn family law and constants (src/simulation/material.ts:18-54). */
+typedef struct bd_yarn {
+  double linear_density;    /* kg/m */
+  double ea_low, ea_high;   /* axial tangent stiffness while decrimping / after, N */
+  double crimp_strain;
+  double ea_compress;       /* compressive stiffness, N */
+  double axial_viscosity;   /* N s */
+  double bend_rigidity;     /* N m^2 */
+  double bend_viscosity;    /* s */
+  double strength;          /* intact peak tension, N */
+  double softening_opening; /* m */
+  int32_t yarns_per_span;   /* parallel yarns per lattice span (1 unbundled) */
+  int32_t pad_;
+} bd_yarn;
+
+/* Mechanical (bundled) material of a layer (material.ts:56-85, 198-226). */
+typedef struct bd_material {
+  double pitch_u, pitch_v;          /* rest lengths of u / v spans, m */
+  bd_yarn yarns[2];                 /* [0] u family (picks), [1] v family (warps) */
+  double shear_stiffness;           /* diagonal pair stiffness, N/m */
+  double crossing_friction;
+  double crossing_normal0;          /* N, per bundled crossing */
+  double crossing_wrap;
+  double air_drag;                  /* N s / m^3 */
+} bd_material;
+
+/* One lattice layer (scene.ts:73-89) and its yarn damage field layout (yarnField.ts:47-66). */
+typedef struct bd_layer {
+  int32_t nx, ny, node_offset;
+  int32_t bundle_u, bundle_v;       /* warps per column, picks per row */
+  int32_t field_offset, field_k, field_m;  /* crossings: K = nx*bu warps, M = ny*bv picks */
+  double weave_pitch_u, weave_pitch_v;    /* individual-yarn pitches */
+  bd_material material;
+} bd_layer;
+
+/*
+ * Compiled scene (scene.ts:91-125, CompiledScene). Node n = offset + j*nx + i owns span 2n
+ * (to i+1, u family) and 2n+1 (to j+1, v family); tails are [u-, u+, v-, v+] per node; the
+ * damage field holds per crossing c and family f entry 2c+f.
+ */
+typedef struct bd_scene {
+  int32_t node_count;
+  int32_t layer_count;
+  const bd_layer* layers;
+  const double* position;          /* 3N */
+  const double* mass;              /* N */
+  const uint8_t* pinned;           /* N */
+  const double* rest_length;       /* 2N */
+  const double* softening;         /* 2N */
+  const uint16_t* seg_flags;       /* 2N: 1 exists, 2 broken, 4 initial defect, 8 slipped */
+  const double* intact;            /* 2N */
+  const uint32_t* cut_mask;        /* 2N */
+  const double* fail_site;         /* 2N */
+  const double* crossing_normal0;  /* N */
+  const double* tails;             /* 4N */
+  int32_t crossing_count;          /* C */
+  int32_t pad_;
+  const float* field_strength;     /* 2C, peak tension per yarn segment, N */
+  const uint8_t* field_damage;     /* 2C, 255 cut, 254 pulled out */
+  const uint8_t* field_area;       /* 2C, cross-section code */
+} bd_scene;
+
+/* Solver parameters (gpu/solver.ts:26-72 GpuSolverParams; newtonMultigrid.ts:20-36 MGConfig). */
+typedef struct bd_params {
+  double h;                    /* timestep, s */
+  double gravity[3];
+  int32_t newton_iterations;   /* always performed */
+  int32_t newton_max;          /* gated extra iterations while max |update| > newton_tolerance */
+  double newton_tolerance;     /* m */
+  int32_t pcg_iterations;      /* fixed-count mode (pcg_rel_tol == 0) */
+  int32_t pcg_max_iterations;  /* cap in tolerance mode */
+  double pcg_rel_tol;          /* > 0: iterate until ||b - A d|| / ||b|| <= tol (FP64 norms) */
+  int32_t coarse_sweeps;       /* line sweeps on the coarsest level */
+  int32_t max_coarsest_nodes;  /* coarsen while a level has more nodes */
+  int32_t lag_coarse;          /* reuse the Galerkin levels of Newton iteration 0 */
+  int32_t line_search;         /* choose alpha in {1, 1/2, 1/4, 1/8} by the potential */
+  int32_t history;             /* damage field + crossing slip after commit */
+  int32_t precision;           /* BD_PRECISION_* */
+} bd_params;
+
+#define BD_PRECISION_FP32 0    /* FP32 operators and smoothers, FP64 dot products */
+
+/* Quadratic grip springs (solver.ts:25-37 ReferenceGrip, interact.wgsl:159-181 gripTerms). */
+typedef struct bd_grips {
+  int32_t grip_count;           /* <= BD_MAX_GRIPS */
+  int32_t entry_count;
+  const int32_t* node;          /* per entry */
+  const int32_t* grip;          /* per entry: owning grip */
+  const double* weight;         /* per entry */
+  const double* offset;         /* 3 per entry: rest offset from the goal */
+  const double* goal;           /* 3 per grip */
+  const double* stiffness;      /* per grip, N/m per unit weight */
+} bd_grips;
+
+/* Per-substep statistics (gpu/solver.ts:218-225 counters). */
+typedef struct bd_stats {
+  int32_t newton_iterations;              /* performed in the last substep */
+  int32_t pcg_iterations[BD_MAX_NEWTON];  /* per Newton iteration */
+  double rel_residual[BD_MAX_NEWTON];     /* final ||b - A d|| / ||b|| (FP64) */
+  double alpha[BD_MAX_NEWTON];            /* line-search step */
+  double max_update[BD_MAX_NEWTON];       /* largest nodal |update|, m */
+  int32_t rejections;                     /* line searches that returned alpha = 0 */
+  int32_t ruptures;                       /* spans separated by the damage field (total) */
+  int32_t pulled_out;                     /* spans released by crossing slip (total) */
+  int32_t level_count;
+  int32_t level_nodes[BD_MAX_LEVELS];
+} bd_stats;
+
+/* Buffers for bd_read / bd_write. `level` selects the multigrid level where it applies. */
+typedef enum bd_buffer {
+  BD_BUF_X = 0,        /* positions, 3N doubles (device FP32 widened) */
+  BD_BUF_V,            /* velocities, 3N doubles */
+  BD_BUF_DX,           /* current step increment, 3N doubles */
+  BD_BUF_PRED,         /* inertial prediction, 3N doubles */
+  BD_BUF_DELTA,        /* Newton direction of the last PCG, 3N doubles */
+  BD_BUF_RHS,          /* Newton rhs -grad, 3N doubles */
+  BD_BUF_LEVEL_X,      /* level solution vector, 3 N_l doubles */
+  BD_BUF_LEVEL_B,      /* level rhs vector, 3 N_l doubles */
+  BD_BUF_LEVEL_R,      /* level residual vector, 3 N_l doubles */
+  BD_BUF_STENCIL,      /* level operator expanded to N_l x 25 x 9 doubles (row-major 3x3) */
+  BD_BUF_FIXED,        /* level fixed flags, N_l bytes */
+  BD_BUF_PROLONG,      /* prolongation weights of level l -> l+1, 4 N_l doubles */
+  BD_BUF_REST,         /* span rest lengths, 2N doubles */
+  BD_BUF_INTACT,       /* 2N doubles */
+  BD_BUF_CUTMASK,      /* 2N uint32 */
+  BD_BUF_SEGFLAGS,     /* 2N uint16 */
+  BD_BUF_FAILSITE,     /* 2N doubles */
+  BD_BUF_TAILS,        /* 4N doubles */
+  BD_BUF_FIELD,        /* 2C uint8 damage codes */
+  BD_BUF_LAST_UPDATE,  /* N doubles, norm of each node's last Newton update */
+  BD_BUF_ENERGY,       /* 8 doubles: energy changes [4] and magnitudes [4] of the last line search */
+  BD_BUF_COUNT_
+} bd_buffer;
+
+typedef struct bd_solver bd_solver;
+
+/* Message of the last failing call on `s` (or of bd_create when s is NULL). */
+const char* bd_last_error(const bd_solver* s);
+
+/* new GpuSolver(device, scene, params) (gpu/solver.ts:275). Uploads the scene and sizes every
+ * level; fails with BD_ERR_LIMIT beyond BD_MAX_LINE or bundles > 32. */
+int bd_create(const bd_scene* scene, const bd_params* params, bd_solver** out);
+int bd_destroy(bd_solver* s);
+
+/* setState(position, velocity?) (gpu/solver.ts:831-836): 3N doubles each; v may be NULL (zero). */
+int bd_set_state(bd_solver* s, const double* x, const double* v);
+/* setSegments(spans, tails) (gpu/solver.ts:815-818); any pointer may be NULL (unchanged). */
+int bd_set_segments(bd_solver* s, const double* rest_length, const double* intact, const uint32_t* cut_mask,
+                    const uint16_t* seg_flags, const double* fail_site, const double* tails);
+/* setField(damage) (gpu/solver.ts:821-824): 2C codes. */
+int bd_set_field(bd_solver* s, const uint8_t* damage);
+/* Grip springs for the following substeps (gripTerms inputs, interact.wgsl:159-181); NULL clears. */
+int bd_set_grips(bd_solver* s, const bd_grips* grips);
+/* setTimestep (gpu/solver.ts:796-799), setHistoryEnabled (:766-768), setNewtonTolerance (:771-774). */
+int bd_set_timestep(bd_solver* s, double h);
+int bd_set_history(bd_solver* s, int on);
+int bd_set_newton_tolerance(bd_solver* s, double tol);
+/* PCG schedule: fixed `iterations` when rel_tol == 0, else to rel_tol with at most max_iterations. */
+int bd_set_pcg(bd_solver* s, int iterations, double rel_tol, int max_iterations);
+
+/* encodeSubstep(encoder, minIterations, maxIterations) (gpu/solver.ts:737-760). */
+int bd_substep(bd_solver* s, int min_iterations, int max_iterations);
+
+/* Stage entry points (gpu/solver.ts:574-728 encodePredict, encodeAssemble, encodeVCycle,
+ * encodePCG, encodeLineSearch, encodeSmooth, encodeFinalize, encodeDamage). */
+int bd_predict(bd_solver* s);
+int bd_assemble(bd_solver* s, int coarse);
+/* level-0 V-cycle: LEVEL_X(0) = V(LEVEL_B(0)) from a zero start */
+int bd_vcycle(bd_solver* s);
+/* one colour-ordered line sweep of `level` on LEVEL_X/LEVEL_B of that level */
+int bd_smooth(bd_solver* s, int level, int reverse);
+/* LEVEL_R(level) = LEVEL_B - A LEVEL_X (fixed rows 0) */
+int bd_residual(bd_solver* s, int level);
+/* DELTA = PCG solve of the assembled system with RHS */
+int bd_pcg(bd_solver* s);
+int bd_line_search(bd_solver* s);
+int bd_newton_apply(bd_solver* s, int iteration);
+int bd_finalize(bd_solver* s);
+int bd_damage(bd_solver* s);
+int bd_slip(bd_solver* s);
+
+int bd_sync(bd_solver* s);
+int bd_read(bd_solver* s, bd_buffer buf, int level, void* dst, size_t bytes);
+int bd_write(bd_solver* s, bd_buffer buf, int level, const void* src, size_t bytes);
+size_t bd_buffer_bytes(const bd_solver* s, bd_buffer buf, int level);
+int bd_get_stats(bd_solver* s, bd_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BD_H */

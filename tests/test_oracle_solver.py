@@ -1,0 +1,318 @@
+"""Oracle multigrid and solver pinned by the reference's unit tests
+(tests/unit/newtonMultigrid.test.ts, referenceSolver.test.ts) and brute force on tiny inputs."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from bd.capi import make_params
+from bd.material import DENIM
+from bd.scene import compile_scene, coupon, SEG_BROKEN
+from conftest import lcg
+
+
+def ref_params(h=1 / 240, gravity=(0, 0, -9.81), newton=2, pcg=4):
+    """SolverParams with MG_DEFAULTS (newtonMultigrid.ts:36): no lagging, no line search."""
+    return make_params(h=h, gravity=gravity, newton_iterations=newton, pcg_iterations=pcg, coarse_sweeps=6,
+                       max_coarsest_nodes=48, lag_coarse=False, line_search=False, history=True)
+
+
+def mg_coupon(nu, nv):  # newtonMultigrid.test.ts:27-43
+    return coupon(nu, nv, angle=0.3, compressU=0.02, compressV=0.01, shear=0.01, collarRows=2, seed=3,
+                  perturbation=0.3e-3)
+
+
+def rs_coupon(nu, nv, **kw):  # referenceSolver.test.ts:7-24
+    spec = dict(angle=0.3, compressU=0.01, compressV=0.005, shear=0.01, collarRows=1, seed=7, perturbation=0.2e-3)
+    spec.update(kw)
+    return coupon(nu, nv, **spec)
+
+
+def to_dense(A, nx, ny):
+    N = A.shape[0]
+    M = np.zeros((3 * N, 3 * N))
+    for n in range(N):
+        i, j = n % nx, n // nx
+        for dj in range(-2, 3):
+            for di in range(-2, 3):
+                if 0 <= i + di < nx and 0 <= j + dj < ny:
+                    m = n + dj * nx + di
+                    M[3 * n:3 * n + 3, 3 * m:3 * m + 3] = A[n, (dj + 2) * 5 + di + 2]
+    return M
+
+
+def test_solve_penta_residual():  # newtonMultigrid.test.ts:62-93
+    r = lcg(9)
+    n = 9
+    D = np.zeros((n, 3, 3))
+    E = np.zeros((n, 3, 3))
+    F = np.zeros((n, 3, 3))
+    for k in range(n):
+        for q in range(9):
+            E[k].flat[q] = r()
+            F[k].flat[q] = 0.5 * r()
+        for a in range(3):
+            for b in range(3):
+                D[k, a, b] = 12 if a == b else 0.3 * (a + b - 2)
+    rhs = np.array([r() for _ in range(3 * n)])
+    x = O.solve_penta(D, E, F, rhs).reshape(n, 3)
+    for k in range(n):
+        s = D[k] @ x[k]
+        if k + 1 < n: s += E[k] @ x[k + 1]
+        if k + 2 < n: s += F[k] @ x[k + 2]
+        if k >= 1: s += E[k - 1].T @ x[k - 1]
+        if k >= 2: s += F[k - 2].T @ x[k - 2]
+        assert np.max(np.abs(s - rhs[3 * k:3 * k + 3])) < 1e-12
+
+
+def test_operator_symmetric_and_galerkin_equals_ptap():  # newtonMultigrid.test.ts:95-151
+    S = compile_scene([mg_coupon(11, 9)])
+    L = S.layers[0]
+    o = O.Oracle(S, ref_params())
+    o.predict()
+    o.assemble()
+    o.build_hierarchy()
+    A = o.stencil(0)
+    M = to_dense(A, L.nx, L.ny)
+    assert np.max(np.abs(M - M.T)) / np.max(np.abs(M)) < 1e-12
+    assert o.level_count() >= 2
+    fixed = o.fixed(0)
+    idx, w = o.prolongation(0)
+    nc = o.level_nodes(1)
+    Pd = np.zeros((3 * A.shape[0], 3 * nc))
+    for f in range(A.shape[0]):
+        for k in range(4):
+            if idx[f, k] >= 0:
+                for q in range(3):
+                    Pd[3 * f + q, 3 * idx[f, k] + q] = w[f, k]
+    Mf = M.copy()
+    for n in np.nonzero(fixed)[0]:
+        Mf[3 * n:3 * n + 3, :] = 0
+        Mf[:, 3 * n:3 * n + 3] = 0
+    G = Pd.T @ Mf @ Pd
+    cx, cy = (L.nx + 1) // 2 + (0 if L.nx % 2 else 1), None
+    Cst = o.stencil(1)
+    cfixed = o.fixed(1)
+    cnx = math.ceil((L.nx - 1) / 2) + 1
+    cny = math.ceil((L.ny - 1) / 2) + 1
+    Mc = to_dense(Cst, cnx, cny)
+    free = np.repeat(cfixed == 0, 3)
+    err = np.max(np.abs((G - Mc)[np.ix_(free, free)]))
+    assert err / np.max(np.abs(G[np.ix_(free, free)])) < 1e-10
+
+
+def test_smoothing_vcycle_pcg_reduce_residual():  # newtonMultigrid.test.ts:153-194
+    S = compile_scene([mg_coupon(40, 30)])
+    o = O.Oracle(S, ref_params())
+    o.predict()
+    o.assemble()
+    o.build_hierarchy()
+    fixed = o.fixed(0)
+    N = S.node_count
+    r = lcg(4)
+    b = np.array([r() for _ in range(3 * N)])
+    b.reshape(N, 3)[fixed == 1] = 0
+    norm = lambda x: np.linalg.norm(b - o.apply(0, x))
+    r0 = norm(np.zeros(3 * N))
+    x = o.smooth(0, np.zeros(3 * N), b)
+    assert norm(x) < r0
+    assert o.level_count() > 2
+    y = o.vcycle(b)
+    assert norm(y) < 0.5 * r0
+    xs = np.array([1e-5 * r() for _ in range(3 * N)])
+    xs.reshape(N, 3)[fixed == 1] = 0
+    bs = o.apply(0, xs)
+
+    def enorm(x):
+        e = x - xs
+        return math.sqrt(e @ o.apply(0, e))
+
+    e0 = enorm(np.zeros(3 * N))
+    z = o.pcg(bs, 3)
+    assert enorm(z) / e0 < 5e-3
+
+
+def dense_smooth(A, fixed, nx, ny, x0, b, reverse=False, colors=3):
+    """smoothLines (newtonMultigrid.ts:659-733) restated with a dense solve of every line system."""
+    N = nx * ny
+    x = x0.reshape(N, 3).copy()
+    order = [(f, c) for f in (1, 0) for c in range(colors)]
+    if reverse:
+        order.reverse()
+    for family, color in order:
+        xo = x.copy()
+        lines = ny if family == 0 else nx
+        for line in range(color, lines, colors):
+            nodes = [line * nx + k for k in range(nx)] if family == 0 else [k * nx + line for k in range(ny)]
+            n3 = 3 * len(nodes)
+            M = np.zeros((n3, n3))
+            rhs = np.zeros(n3)
+            for k, n in enumerate(nodes):
+                rhs[3 * k:3 * k + 3] = b[3 * n:3 * n + 3]
+                if fixed[n]:
+                    M[3 * k:3 * k + 3, 3 * k:3 * k + 3] = np.eye(3)
+                    continue
+                maj = 0.0
+                i, j = n % nx, n // nx
+                for dj in range(-2, 3):
+                    for di in range(-2, 3):
+                        if not (0 <= i + di < nx and 0 <= j + dj < ny):
+                            continue
+                        m = n + dj * nx + di
+                        if fixed[m] and not (di == 0 and dj == 0):
+                            continue
+                        B = A[n, (dj + 2) * 5 + di + 2]
+                        along, across = (di, dj) if family == 0 else (dj, di)
+                        if across == 0 and abs(along) <= 1:
+                            if along == 1 or along == 0:
+                                M[3 * k:3 * k + 3, 3 * (k + along):3 * (k + along) + 3] += B
+                            if along == 1:
+                                M[3 * (k + 1):3 * (k + 1) + 3, 3 * k:3 * k + 3] += B.T
+                            continue
+                        if across == 0 or abs(across) == colors:
+                            maj += np.linalg.norm(B)
+                        rhs[3 * k:3 * k + 3] -= B @ xo[m]
+                M[3 * k:3 * k + 3, 3 * k:3 * k + 3] += maj * np.eye(3)
+                rhs[3 * k:3 * k + 3] += maj * xo[n]
+            sol = np.linalg.solve(M, rhs).reshape(-1, 3)
+            for k, n in enumerate(nodes):
+                if not fixed[n]:
+                    x[n] = sol[k]
+    return x.ravel()
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+def test_smoother_matches_dense_line_solves(reverse):
+    """Brute force on a small coupon: a full colour-ordered sweep equals dense solves of every line's
+    block-tridiagonal system (lower blocks = transposes, +-2 along-line couplings majorised by their
+    Frobenius norms), up to the 2e-6 relative pivot shift."""
+    S = compile_scene([mg_coupon(10, 8)])
+    L = S.layers[0]
+    o = O.Oracle(S, ref_params())
+    o.predict()
+    o.assemble()
+    A = o.stencil(0)
+    fixed = o.fixed(0)
+    N = S.node_count
+    r = lcg(11)
+    b = np.array([r() for _ in range(3 * N)])
+    x0 = np.array([1e-3 * r() for _ in range(3 * N)])
+    x0.reshape(N, 3)[fixed == 1] = 0
+    x = o.smooth(0, x0, b, reverse)
+    ref = dense_smooth(A, fixed, L.nx, L.ny, x0, b, reverse)
+    assert np.max(np.abs(x - ref)) <= 1e-4 * np.max(np.abs(ref))
+    # a plausible slip (dropping the majorisation) is caught at this tolerance
+    assert np.max(np.abs(x - x0)) > 1e-3 * np.max(np.abs(ref))
+
+
+def test_fd_gradient():  # referenceSolver.test.ts:39-64
+    S = compile_scene([rs_coupon(9, 8)])
+    o = O.Oracle(S, ref_params())
+    v = o.v
+    v[:] = 0.01 * np.sin(np.arange(len(v)) * 1.7)
+    o.predict()
+    rnd = lcg(1)
+    dx = o.dx.reshape(-1, 3)
+    for n in range(S.node_count):
+        if S.pinned[n]:
+            continue
+        for q in range(3):
+            dx[n, q] = 2e-5 * rnd()
+    free = [n for n in range(S.node_count) if not S.pinned[n]][:20]
+    dxf = o.dx
+    for n in free:
+        g = o.node_gradient(n)
+        for q in range(3):
+            h = 1e-9
+            k = 3 * n + q
+            x0 = dxf[k]
+            dxf[k] = x0 + h
+            ep = o.potential()
+            dxf[k] = x0 - h
+            em = o.potential()
+            dxf[k] = x0
+            fdv = (ep - em) / (2 * h)
+            scale = max(1e-6, abs(fdv), abs(g[q]))
+            assert abs(fdv - g[q]) / scale < 2e-4
+
+
+def test_rigid_motion_invariance():  # referenceSolver.test.ts:66-100
+    S = compile_scene([rs_coupon(8, 7, compressU=0.0, compressV=0.0, shear=0)])
+    o = O.Oracle(S, ref_params())
+    rnd = lcg(5)
+    dx = o.dx.reshape(-1, 3)
+    for n in range(S.node_count):
+        if S.pinned[n]:
+            continue
+        for q in range(3):
+            dx[n, q] = 3e-5 * rnd()
+    o.pred[:] = o.dx
+    e0 = o.potential()
+    ax = np.array([0.3, -0.5, 0.81])
+    kx, ky, kz = ax / np.linalg.norm(ax)
+    th = 0.7
+    c, sn = math.cos(th), math.sin(th)
+    R = np.array([[c + kx * kx * (1 - c), kx * ky * (1 - c) - kz * sn, kx * kz * (1 - c) + ky * sn],
+                  [ky * kx * (1 - c) + kz * sn, c + ky * ky * (1 - c), ky * kz * (1 - c) - kx * sn],
+                  [kz * kx * (1 - c) - ky * sn, kz * ky * (1 - c) + kx * sn, c + kz * kz * (1 - c)]])
+    t = np.array([0.01, -0.02, 0.005])
+    x = o.x.reshape(-1, 3)
+    y = x + dx
+    rx = x @ R.T + t
+    ry = y @ R.T + t
+    x[:] = rx
+    dx[:] = ry - rx
+    o.pred[:] = o.dx
+    e1 = o.potential()
+    assert abs(e1 - e0) / max(e0, 1e-12) < 1e-7
+
+
+def test_newton_decreases_potential():  # referenceSolver.test.ts:103-118
+    S = compile_scene([rs_coupon(30, 24)])
+    o = O.Oracle(S, ref_params())
+    for _ in range(10):
+        o.substep()
+    o.predict()
+    e0 = o.potential()
+    prev = e0
+    for it in range(4):
+        o.newton_step(True)
+        e = o.potential()
+        assert e <= prev * 1.02
+        prev = e
+    assert prev < e0
+
+
+def test_settles_under_gravity():  # referenceSolver.test.ts:120-135
+    S = compile_scene([rs_coupon(20, 16)])
+    o = O.Oracle(S, ref_params())
+    for _ in range(240):
+        o.substep()
+    v = o.v.reshape(-1, 3)
+    x = o.x.reshape(-1, 3)
+    assert np.all(np.isfinite(x))
+    assert np.max(np.linalg.norm(v, axis=1)) < 0.02
+    assert np.max(np.abs(x[:, 2])) < 0.005
+    assert o.broken_count == 0
+
+
+def test_pull_ruptures_and_conserves_yarn_length():  # referenceSolver.test.ts:137-160
+    S = compile_scene([rs_coupon(14, 10, compressU=0, compressV=0, shear=0, perturbation=0)])
+    o = O.Oracle(S, ref_params(gravity=(0, 0, 0)))
+    L0 = o.yarn_material_length()
+    L = S.layers[0]
+    top = [L.node_offset + (L.ny - 1) * L.nx + i for i in range(L.nx)]
+    vdir = np.array([L.mapA[1], L.mapA[3]])
+    vdir /= np.linalg.norm(vdir)
+    x = o.x.reshape(-1, 3)
+    k = 0
+    while k < 400 and o.broken_count < 6:
+        for n in top:
+            x[n, 0] += (0.02 / 240) * vdir[0]
+            x[n, 1] += (0.02 / 240) * vdir[1]
+        o.substep()
+        k += 1
+    assert o.broken_count > 0
+    assert int(np.count_nonzero(o.seg_flags & SEG_BROKEN)) >= o.broken_count
+    assert abs(o.yarn_material_length() - L0) / L0 < 1e-9
