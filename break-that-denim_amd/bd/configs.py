@@ -1,0 +1,129 @@
+"""Workload scenes: hanging sheets with pinned top rows and moving grips.
+
+The sheet configs express their physics through the reference's lattice model (the reference has no
+StVK triangle membrane, Poisson coupling or dihedral bending): per-family linear axial laws with
+E * pitch per span (warp : weft stiffness 1.6 : 1), bending rigidity k * pitch per span, lumped
+mass areal_density * pitch^2 per crossing, and diagonal shear springs equal to the membrane shear
+modulus E / (2 (1 + nu)) (a square cell's diagonal pair under simple shear stores
+1/2 ks gamma^2 pitch^2). A breakable central region draws one break strain per yarn from
+lognormal(ln median, sigma) with a fixed seed; its peak tension is the yarn law at that strain.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .capi import make_params
+from .material import crossing_mass, linear_sheet_material
+from .noise import fbm
+from .scene import CompiledScene, Layer, SEG_EXISTS, field_layout
+
+UNBREAKABLE = 1.0e9
+
+
+def _axial_tension(y, eps):
+    eps = np.asarray(eps, dtype=np.float64)
+    ec = y["crimpStrain"]
+    dk = y["eaHigh"] - y["eaLow"]
+    lo = y["eaLow"] * eps + dk * eps * eps / (2 * ec)
+    hi = y["eaLow"] * ec + dk * ec / 2 + y["eaHigh"] * (eps - ec)
+    return np.where(eps <= 0, y["eaCompress"] * eps, np.where(eps <= ec, lo, hi))
+
+
+def sheet_scene(n, pitch, material, perturbation=0.5e-3, seed=101, pin_top=True, yarn_region=0,
+                break_median=0.12, break_sigma=0.3, break_seed=7):
+    """n x n hanging sheet in the x-z plane (u -> x, v -> z, top row j = n-1 pinned), with a small
+    out-of-plane perturbation along y from fbm noise to seed buckling."""
+    nx = ny = n
+    N = nx * ny
+    L = Layer(0, material, material, (1, 1), nx, ny, 0, N, (1.0, 0.0, 0.0, 1.0), (0.0, 0.0, 0.0),
+              ((nx - 1) * pitch / 2, (ny - 1) * pitch / 2))
+    C = field_layout([L])
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    i = i.ravel()
+    j = j.ravel()
+    u = i * pitch - L.restCenter[0]
+    v = j * pitch - L.restCenter[1]
+    pinned = (j == ny - 1) if pin_top else np.zeros(N, bool)
+    py = np.where(pinned, 0.0, perturbation * fbm(seed, u / 0.04, v / 0.04, 3))
+    S = CompiledScene(
+        layers=[L], node_count=N, rest_uv=np.stack([i * pitch, j * pitch], 1), position=np.stack([u, py, v], 1),
+        mass=np.full(N, crossing_mass(material)), pinned=pinned.astype(np.uint8), wear=np.zeros(N, np.float32),
+        structural_wear=np.zeros(N, np.float32), rest_length=np.zeros(2 * N), softening=np.zeros(2 * N),
+        seg_flags=np.zeros(2 * N, np.uint16), intact=np.zeros(2 * N), cut_mask=np.zeros(2 * N, np.uint32),
+        fail_site=np.zeros(2 * N), crossing_count=C, field_strength=np.full(2 * C, UNBREAKABLE, np.float32),
+        field_damage=np.zeros(2 * C, np.uint8), field_area=np.full(2 * C, 255, np.uint8),
+        crossing_normal0=np.full(N, material["crossingNormal0"]), tails=np.zeros(4 * N))
+    n_idx = np.arange(N)
+    for f in (0, 1):
+        ex = (i < nx - 1) if f == 0 else (j < ny - 1)
+        s = 2 * n_idx[ex] + f
+        S.seg_flags[s] = SEG_EXISTS
+        S.rest_length[s] = pitch
+        S.intact[s] = 1.0
+        S.softening[s] = material["yarns"][f]["softeningOpening"]
+        S.fail_site[s] = 0.5
+    if yarn_region > 0:
+        rng = np.random.default_rng(break_seed)
+        c0 = (n - yarn_region) // 2
+        idx = np.arange(c0, c0 + yarn_region)
+        # one break strain per yarn: warps (v family) per column, wefts (u family) per row
+        eb_warp = np.exp(math.log(break_median) + break_sigma * rng.standard_normal(yarn_region))
+        eb_weft = np.exp(math.log(break_median) + break_sigma * rng.standard_normal(yarn_region))
+        kk, mm = np.meshgrid(idx, idx, indexing="xy")   # kk: column (warp), mm: row (weft)
+        c = (mm * nx + kk).ravel()
+        t_warp = _axial_tension(material["yarns"][1], eb_warp)[np.searchsorted(idx, kk.ravel())]
+        t_weft = _axial_tension(material["yarns"][0], eb_weft)[np.searchsorted(idx, mm.ravel())]
+        S.field_strength[2 * c + 1] = t_warp.astype(np.float32)
+        S.field_strength[2 * c + 0] = t_weft.astype(np.float32)
+        S.extra["yarn_region"] = (c0, yarn_region)
+    return S
+
+
+def corner_grip(S, corner_i, corner_j, k, size=3):
+    """A size x size patch grip at a lattice corner: equal weights summing to 1, goal at the patch
+    centroid, offsets from it (the ReferenceGrip quadratic spring, solver.ts:25-37)."""
+    L = S.layers[0]
+    i0 = 0 if corner_i == 0 else L.nx - size
+    j0 = 0 if corner_j == 0 else L.ny - size
+    nodes = [(j0 + b) * L.nx + (i0 + a) for b in range(size) for a in range(size)]
+    pos = S.position[nodes]
+    goal = pos.mean(0)
+    return {"nodes": nodes, "weights": [1.0 / len(nodes)] * len(nodes), "offsets": pos - goal, "goal": goal.copy(), "k": k}
+
+
+def config1():
+    """64 x 64 sheet, 0.5 m, warp/weft 1.6:1 with E_warp = 2e4 N/m, nu = 0.2 (shear fit), bending
+    k = 1e-5 N m, 0.35 kg/m^2, top row pinned, one bottom corner gripped and moved +x 2 mm/step,
+    h = 1/120, 3 Newton iterations, 3-level MG (coarsest <= 400 nodes), rel tol 1e-5, no history."""
+    n = 64
+    pitch = 0.5 / 63
+    mat = linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / (2 * 1.2))
+    S = sheet_scene(n, pitch, mat)
+    grip = corner_grip(S, 1, 0, k=2e4)
+    params = make_params(h=1 / 120, newton_iterations=3, pcg_rel_tol=1e-5, pcg_max_iterations=100,
+                         max_coarsest_nodes=400, coarse_sweeps=2, lag_coarse=True, line_search=True, history=False)
+    return {"name": "sheet64-plain", "scene": S, "params": params, "grips": [grip], "velocity": [np.array([2e-3, 0, 0])],
+            "steps": 120}
+
+
+def config_sheet(n=1024, pitch=1e-3, region=256, rel_tol=1e-6, history=True, newton=3):
+    """n x n sheet at 1 mm pitch with a central breakable woven region (per-yarn lognormal break
+    strains, median 0.12, sigma 0.3, seed 7) and two bottom-corner grips pulled apart at 5 mm/step;
+    plain FP32 MG-PCG to rel-res `rel_tol`."""
+    mat = linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / (2 * 1.2))
+    S = sheet_scene(n, pitch, mat, yarn_region=region)
+    g0 = corner_grip(S, 0, 0, k=2e4)
+    g1 = corner_grip(S, 1, 0, k=2e4)
+    params = make_params(h=1 / 120, newton_iterations=newton, pcg_rel_tol=rel_tol, pcg_max_iterations=200,
+                         max_coarsest_nodes=400, coarse_sweeps=2, lag_coarse=True, line_search=True, history=history)
+    return {"name": f"sheet{n}-yarns{region}", "scene": S, "params": params, "grips": [g0, g1],
+            "velocity": [np.array([-5e-3, 0, 0]), np.array([5e-3, 0, 0])], "steps": 240}
+
+
+def advance_grips(grips, velocity):
+    """Move every grip goal by its per-step velocity (the caller's handle motion)."""
+    for g, v in zip(grips, velocity):
+        g["goal"] = np.asarray(g["goal"], dtype=np.float64) + v
+    return grips

@@ -1,0 +1,190 @@
+"""Host-side mirror of the reference's GpuSolver (src/simulation/gpu/solver.ts:194-838) over libbd.so.
+
+Method names follow the reference's (encodeSubstep -> encode_substep, setState -> set_state, ...);
+the stage encoders enqueue on the solver's HIP stream and return, readbacks synchronise. Failing
+calls raise RuntimeError with the library's message, as the reference throws JS Errors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import (BD_BUF_CUTMASK, BD_BUF_DELTA, BD_BUF_DX, BD_BUF_ENERGY, BD_BUF_FAILSITE, BD_BUF_FIELD,
+                   BD_BUF_FIXED, BD_BUF_INTACT, BD_BUF_LAST_UPDATE, BD_BUF_LEVEL_B, BD_BUF_LEVEL_R, BD_BUF_LEVEL_X,
+                   BD_BUF_PRED, BD_BUF_PROLONG, BD_BUF_REST, BD_BUF_RHS, BD_BUF_SEGFLAGS, BD_BUF_STENCIL,
+                   BD_BUF_TAILS, BD_BUF_V, BD_BUF_X, GripBuffers, SceneBuffers, bd_stats, make_params)
+
+_DTYPES = {
+    BD_BUF_FIXED: np.uint8, BD_BUF_CUTMASK: np.uint32, BD_BUF_SEGFLAGS: np.uint16, BD_BUF_FIELD: np.uint8,
+}
+
+
+class GpuSolver:
+    """One implicit-solve handle on the GPU (gpu/solver.ts:275 constructor)."""
+
+    def __init__(self, scene, params=None, grips=None):
+        self.lib = capi.load_library()
+        self.scene = scene
+        self.N = scene.node_count
+        self.params = params if params is not None else make_params()
+        self._sb = SceneBuffers(scene)
+        h = C.c_void_p()
+        rc = self.lib.bd_create(C.byref(self._sb.scene), C.byref(self.params), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"bd_create failed ({rc}): {self.lib.bd_last_error(None).decode()}")
+        self.h = h
+        self._grips = None
+        if grips:
+            self.set_grips(grips)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.bd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {self.lib.bd_last_error(self.h).decode()}")
+
+    # ---- state upload (gpu/solver.ts:801-837)
+    def set_state(self, position, velocity=None):
+        x = np.ascontiguousarray(position, dtype=np.float64).ravel()
+        v = None if velocity is None else np.ascontiguousarray(velocity, dtype=np.float64).ravel()
+        self._check(self.lib.bd_set_state(self.h, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                          None if v is None else v.ctypes.data_as(C.POINTER(C.c_double))), "bd_set_state")
+
+    def set_segments(self, rest=None, intact=None, cut_mask=None, seg_flags=None, fail_site=None, tails=None):
+        def p(a, dt, ct):
+            if a is None:
+                return None, None
+            arr = np.ascontiguousarray(a, dtype=dt)
+            return arr, arr.ctypes.data_as(C.POINTER(ct))
+        keep = [p(rest, np.float64, C.c_double), p(intact, np.float64, C.c_double), p(cut_mask, np.uint32, C.c_uint32),
+                p(seg_flags, np.uint16, C.c_uint16), p(fail_site, np.float64, C.c_double), p(tails, np.float64, C.c_double)]
+        self._check(self.lib.bd_set_segments(self.h, *[k[1] for k in keep]), "bd_set_segments")
+
+    def set_field(self, damage):
+        d = np.ascontiguousarray(damage, dtype=np.uint8)
+        self._check(self.lib.bd_set_field(self.h, d.ctypes.data_as(C.POINTER(C.c_uint8))), "bd_set_field")
+
+    def set_grips(self, grips):
+        self._grips = GripBuffers(grips) if grips else None
+        self._check(self.lib.bd_set_grips(self.h, C.byref(self._grips.struct) if self._grips else None), "bd_set_grips")
+
+    def set_timestep(self, h):
+        self._check(self.lib.bd_set_timestep(self.h, h), "bd_set_timestep")
+
+    def set_history_enabled(self, on):
+        self._check(self.lib.bd_set_history(self.h, int(bool(on))), "bd_set_history")
+
+    def set_newton_tolerance(self, tol):
+        self._check(self.lib.bd_set_newton_tolerance(self.h, tol), "bd_set_newton_tolerance")
+
+    def set_pcg(self, iterations=2, rel_tol=0.0, max_iterations=200):
+        self._check(self.lib.bd_set_pcg(self.h, iterations, rel_tol, max_iterations), "bd_set_pcg")
+
+    # ---- schedule (gpu/solver.ts:574-760)
+    def encode_substep(self, min_iterations=0, max_iterations=0):
+        self._check(self.lib.bd_substep(self.h, min_iterations, max_iterations), "bd_substep")
+
+    substep = encode_substep
+
+    def encode_predict(self):
+        self._check(self.lib.bd_predict(self.h), "bd_predict")
+
+    def encode_assemble(self, coarse=True):
+        self._check(self.lib.bd_assemble(self.h, int(bool(coarse))), "bd_assemble")
+
+    def encode_vcycle(self):
+        self._check(self.lib.bd_vcycle(self.h), "bd_vcycle")
+
+    def encode_smooth(self, level, reverse=False):
+        self._check(self.lib.bd_smooth(self.h, level, int(bool(reverse))), "bd_smooth")
+
+    def encode_residual(self, level):
+        self._check(self.lib.bd_residual(self.h, level), "bd_residual")
+
+    def encode_pcg(self):
+        self._check(self.lib.bd_pcg(self.h), "bd_pcg")
+
+    def encode_line_search(self):
+        self._check(self.lib.bd_line_search(self.h), "bd_line_search")
+
+    def encode_newton_apply(self, iteration=0):
+        self._check(self.lib.bd_newton_apply(self.h, iteration), "bd_newton_apply")
+
+    def encode_finalize(self):
+        self._check(self.lib.bd_finalize(self.h), "bd_finalize")
+
+    def encode_damage(self):
+        self._check(self.lib.bd_damage(self.h), "bd_damage")
+
+    def encode_slip(self):
+        self._check(self.lib.bd_slip(self.h), "bd_slip")
+
+    def sync(self):
+        self._check(self.lib.bd_sync(self.h), "bd_sync")
+
+    # ---- readback
+    def read(self, buf, level=0):
+        nbytes = self.lib.bd_buffer_bytes(self.h, buf, level)
+        if nbytes == 0:
+            raise RuntimeError(f"buffer {buf} level {level} is empty")
+        dt = _DTYPES.get(buf, np.float64)
+        out = np.empty(nbytes // np.dtype(dt).itemsize, dtype=dt)
+        self._check(self.lib.bd_read(self.h, buf, level, out.ctypes.data, nbytes), "bd_read")
+        return out
+
+    def write(self, buf, data, level=0):
+        nbytes = self.lib.bd_buffer_bytes(self.h, buf, level)
+        dt = _DTYPES.get(buf, np.float64)
+        a = np.ascontiguousarray(data, dtype=dt).ravel()
+        if a.nbytes < nbytes:
+            raise ValueError(f"need {nbytes} bytes for buffer {buf}, got {a.nbytes}")
+        self._check(self.lib.bd_write(self.h, buf, level, a.ctypes.data, a.nbytes), "bd_write")
+
+    def vec(self, buf, level=0):
+        return self.read(buf, level).reshape(-1, 3)
+
+    @property
+    def x(self): return self.vec(BD_BUF_X)
+
+    @property
+    def v(self): return self.vec(BD_BUF_V)
+
+    @property
+    def dx(self): return self.vec(BD_BUF_DX)
+
+    def stencil(self, level=0):
+        return self.read(BD_BUF_STENCIL, level).reshape(-1, 25, 3, 3)
+
+    def fixed(self, level=0):
+        return self.read(BD_BUF_FIXED, level)
+
+    def prolongation(self, level=0):
+        return self.read(BD_BUF_PROLONG, level).reshape(-1, 4)
+
+    def energy(self):
+        e = self.read(BD_BUF_ENERGY)
+        return e[:4], e[4:]
+
+    def stats(self):
+        st = bd_stats()
+        self._check(self.lib.bd_get_stats(self.h, C.byref(st)), "bd_get_stats")
+        n = st.newton_iterations
+        return {
+            "newton_iterations": n,
+            "pcg_iterations": list(st.pcg_iterations[:max(n, 1)]),
+            "rel_residual": list(st.rel_residual[:max(n, 1)]),
+            "alpha": list(st.alpha[:max(n, 1)]),
+            "max_update": list(st.max_update[:max(n, 1)]),
+            "rejections": st.rejections,
+            "ruptures": st.ruptures,
+            "pulled_out": st.pulled_out,
+            "level_nodes": list(st.level_nodes[:st.level_count]),
+        }

@@ -1,0 +1,153 @@
+// Internal layout of a solver handle and the launchers each kernel file provides.
+//
+// HBM layout (structure of arrays; every plane padded to a multiple of 64 entries):
+//   node vectors      3 planes of float  [x | y | z], e.g. x[c * Np + n]
+//   fine operator     32 planes of float per node (128 B): the symmetric centre block (6), the
+//                     forward symmetric blocks E (+1,0), N (0,+1), NE (+1,+1), NW (-1,+1) (6 each)
+//                     and the along-line scalar couplings EE (+2,0), NN (0,+2). Every block of the
+//                     clamp-projected Hessian is symmetric and A(m,n) = A(n,m), so backward
+//                     couplings are read from the neighbour's forward planes.
+//   coarse operators  13 symmetric blocks per node (78 planes, 312 B): centre plus the 12 forward
+//                     offsets of the radius-2 stencil (Galerkin blocks stay symmetric).
+//   prolongation      4 float weights per fine node, in bracket order (j outer, i inner).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/bd.h"
+#include "bd_math.h"
+
+namespace bd {
+
+constexpr int kFineComps = 32;
+constexpr int kCoarseBlocks = 13;
+constexpr int kCoarseComps = 6 * kCoarseBlocks;
+constexpr int kDotBlocks = 512;
+constexpr int kMaxLevels = BD_MAX_LEVELS;
+
+// forward offsets of the coarse stencil, block b = 1..12 (block 0 is the centre)
+__host__ __device__ constexpr int coarse_di(int b) {
+  return b == 1 ? 1 : b == 2 ? 2 : b == 3 ? -2 : b == 4 ? -1 : b == 5 ? 0 : b == 6 ? 1 : b == 7 ? 2 : b == 8 ? -2 : b == 9 ? -1 : b == 10 ? 0 : b == 11 ? 1 : b == 12 ? 2 : 0;
+}
+__host__ __device__ constexpr int coarse_dj(int b) { return b == 0 ? 0 : b <= 2 ? 0 : b <= 7 ? 1 : 2; }
+// block index of forward offset (di, dj), or -1 when (di, dj) is not a forward offset
+__host__ __device__ inline int coarse_block(int di, int dj) {
+  if (di < -2 || di > 2) return -1;
+  if (dj == 0) return di == 0 ? 0 : di == 1 ? 1 : di == 2 ? 2 : -1;
+  if (dj == 1) return 5 + di;   // di in [-2, 2] -> 3..7
+  if (dj == 2) return 10 + di;  // -> 8..12
+  return -1;
+}
+
+// fine forward-plane bases
+constexpr int F_C = 0, F_E = 6, F_N = 12, F_NE = 18, F_NW = 24, F_EE = 30, F_NN = 31;
+
+struct Level {
+  int N = 0, Np = 0;
+  Geo geo{};
+  bool fine = false;
+  float* op = nullptr;
+  uint8_t* fixed = nullptr;
+  float* x = nullptr;   // 3 planes
+  float* b = nullptr;
+  float* r = nullptr;
+  float* pw = nullptr;  // 4 planes: prolongation weights to the next level
+  // line tables: per colour (6: v-lines 0..2, u-lines 0..2) a list of int4 (start, stride, len, family)
+  int4* lines = nullptr;
+  int color_start[7] = {0};
+  int max_len = 0;
+};
+
+struct Scal {
+  // PCG and line-search scalars kept on the device
+  double bb, rz, rz_new, pap, alpha, beta, rr, relres;
+  double e[4], m[4];
+  double line_alpha;
+  double tol;
+  int done, iters, max_iters, fixed_iters;
+  int rejections, pad;
+  unsigned int ticket;       // last-block-done counter of the current reduction
+  unsigned int max_update[BD_MAX_NEWTON];  // float bits
+  // per Newton iteration record of the substep
+  int rec_iters[BD_MAX_NEWTON];
+  double rec_relres[BD_MAX_NEWTON];
+  double rec_alpha[BD_MAX_NEWTON];
+};
+
+struct Solver {
+  bd_params P{};
+  int N = 0, Np = 0, C = 0;
+  int nl = 0;
+  DevLayer layers_h[8];
+  DevLayer* layers = nullptr;  // device copy
+  Geo geo{};
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // node state (3 planes each)
+  float *x = nullptr, *v = nullptr, *dx = nullptr, *pred = nullptr, *delta = nullptr;
+  float *rhs = nullptr, *p = nullptr, *ap = nullptr;   // PCG; r and z are level 0's b and x
+  float* last_update = nullptr;
+  float* mass = nullptr;
+  double* cn0 = nullptr;
+  uint8_t* pinned = nullptr;
+  uint8_t* layer_of = nullptr;
+  // spans
+  double *rest = nullptr, *intact = nullptr, *fail_site = nullptr, *tails = nullptr, *softening = nullptr;
+  uint32_t* cut_mask = nullptr;
+  uint16_t* flags = nullptr;
+  double* strain = nullptr;
+  // damage field
+  uint8_t *damage = nullptr, *area = nullptr, *sep = nullptr, *pullmark = nullptr;
+  float* strength = nullptr;
+  // grips
+  int grip_count = 0;
+  int* grip_of = nullptr;      // per node
+  float* grip_w = nullptr;     // per node
+  double* grip_off = nullptr;  // 3 planes
+  double* grip_goal_k = nullptr;  // per grip: goal xyz, k
+  // reductions
+  double* partials = nullptr;  // kDotBlocks * 8
+  Scal* scal = nullptr;
+  Scal* scal_h = nullptr;      // pinned host mirror
+  int* counters = nullptr;     // [0] ruptures, [1] pulled out
+  // multigrid
+  int nlev = 0;
+  Level lv[kMaxLevels];
+  bool assembled = false, hierarchy = false;
+  bd_stats stats{};
+  int coarse_valid_iteration = -1;
+};
+
+// launchers (assemble.hip)
+void launch_predict(Solver& s);
+void launch_assemble(Solver& s);
+void launch_newton_apply(Solver& s, int it);
+void launch_commit(Solver& s);
+// multigrid.hip
+void launch_prolong_weights(Solver& s, int k);
+void launch_rap(Solver& s, int k);
+void launch_residual(Solver& s, int k, const float* x, const float* b, float* r);
+void launch_restrict(Solver& s, int k);
+void launch_prolong_add(Solver& s, int k);
+void launch_smooth(Solver& s, int k, int reverse);
+void launch_zero(Solver& s, float* p, size_t n);
+void vcycle(Solver& s, int k);
+// pcg.hip
+void pcg_solve(Solver& s);
+void launch_energy(Solver& s);
+// history.hip
+void launch_damage(Solver& s);
+void launch_slip(Solver& s);
+
+#define BD_HIP(call)                                                     \
+  do {                                                                   \
+    hipError_t e_ = (call);                                              \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+inline int grid_for(int n, int block) { return (n + block - 1) / block; }
+
+}  // namespace bd

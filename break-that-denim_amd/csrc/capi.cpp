@@ -1,0 +1,793 @@
+// C-ABI of libbd.so (include/bd.h): solver handles, uploads, the substep schedule, stage entry
+// points and readbacks. Stands in for the reference's GpuSolver (src/simulation/gpu/solver.ts).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bd_internal.h"
+
+struct bd_solver {
+  bd::Solver s;
+};
+
+namespace bd {
+void smooth_set_attributes();
+void launch_record(Solver& s, int it);
+void launch_reset_counters(Solver& s);
+}  // namespace bd
+
+using namespace bd;
+
+static thread_local std::string g_create_error;
+
+namespace {
+
+template <class T>
+T* dalloc(size_t count) {
+  if (count == 0) count = 1;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess) throw std::runtime_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+  BD_HIP(hipMemset(p, 0, count * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+int pad64(int n) { return (n + 63) / 64 * 64; }
+
+// interleaved xyz doubles -> 3 float planes
+void upload_vec3(Solver& s, float* dst, int N, int Np, const double* src) {
+  std::vector<float> h((size_t)3 * Np, 0.f);
+  for (int n = 0; n < N; n++)
+    for (int q = 0; q < 3; q++) h[(size_t)q * Np + n] = (float)src[3 * n + q];
+  BD_HIP(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, s.stream));
+  BD_HIP(hipStreamSynchronize(s.stream));
+}
+
+void download_vec3(Solver& s, const float* src, int N, int Np, double* dst) {
+  std::vector<float> h((size_t)3 * Np);
+  BD_HIP(hipMemcpyAsync(h.data(), src, h.size() * sizeof(float), hipMemcpyDeviceToHost, s.stream));
+  BD_HIP(hipStreamSynchronize(s.stream));
+  for (int n = 0; n < N; n++)
+    for (int q = 0; q < 3; q++) dst[3 * n + q] = h[(size_t)q * Np + n];
+}
+
+template <class T>
+void upload(Solver& s, T* dst, const T* src, size_t count) {
+  BD_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s.stream));
+  BD_HIP(hipStreamSynchronize(s.stream));
+}
+
+template <class T>
+void download(Solver& s, T* dst, const T* src, size_t count) {
+  BD_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, s.stream));
+  BD_HIP(hipStreamSynchronize(s.stream));
+}
+
+DevLayer dev_layer(const bd_layer& L) {
+  DevLayer d{};
+  d.nx = L.nx;
+  d.ny = L.ny;
+  d.off = L.node_offset;
+  d.bu = L.bundle_u;
+  d.bv = L.bundle_v;
+  d.foff = L.field_offset;
+  d.fk = L.field_k;
+  d.fm = L.field_m;
+  d.wpu = L.weave_pitch_u;
+  d.wpv = L.weave_pitch_v;
+  d.pu = L.material.pitch_u;
+  d.pv = L.material.pitch_v;
+  for (int f = 0; f < 2; f++) {
+    const bd_yarn& y = L.material.yarns[f];
+    d.y[f] = DevYarn{y.linear_density, y.ea_low, y.ea_high, y.crimp_strain, y.ea_compress, y.axial_viscosity, y.bend_rigidity,
+                     y.bend_viscosity, y.strength, y.softening_opening, y.yarns_per_span, 0};
+  }
+  d.shear_k = L.material.shear_stiffness;
+  d.mu = L.material.crossing_friction;
+  d.cn0 = L.material.crossing_normal0;
+  d.wrap = L.material.crossing_wrap;
+  d.air_drag = L.material.air_drag;
+  return d;
+}
+
+// level layouts (lattice.ts:43-66 coarsenLayers / levelLayers, newtonMultigrid.ts:480-491)
+std::vector<Geo> level_layouts(const Geo& fine, int max_coarsest) {
+  std::vector<Geo> out{fine};
+  for (;;) {
+    const Geo& cur = out.back();
+    int N = 0;
+    bool can = true;
+    for (int l = 0; l < cur.nl; l++) {
+      N += cur.nx[l] * cur.ny[l];
+      if (!(cur.nx[l] * cur.ny[l] > 48 && cur.nx[l] >= 5 && cur.ny[l] >= 5)) can = false;
+    }
+    if (N <= max_coarsest || !can || (int)out.size() >= kMaxLevels) break;
+    Geo c{};
+    c.nl = cur.nl;
+    int off = 0;
+    for (int l = 0; l < cur.nl; l++) {
+      c.nx[l] = cur.nx[l] / 2 + 1;  // ceil((n - 1) / 2) + 1
+      c.ny[l] = cur.ny[l] / 2 + 1;
+      c.off[l] = off;
+      off += c.nx[l] * c.ny[l];
+    }
+    out.push_back(c);
+  }
+  return out;
+}
+
+// lineTable (lattice.ts:86-106): v-lines (family 1) colours 0..2, then u-lines colours 0..2
+void build_lines(Solver& s, Level& L) {
+  std::vector<std::vector<int4>> by(6);
+  int maxlen = 0;
+  for (int l = 0; l < L.geo.nl; l++) {
+    int nx = L.geo.nx[l], ny = L.geo.ny[l], off = L.geo.off[l];
+    for (int i = 0; i < nx; i++) by[i % 3].push_back(make_int4(off + i, nx, ny, 1));
+    for (int j = 0; j < ny; j++) by[3 + j % 3].push_back(make_int4(off + j * nx, 1, nx, 0));
+    maxlen = std::max(maxlen, std::max(nx, ny));
+  }
+  std::vector<int4> flat;
+  L.color_start[0] = 0;
+  for (int c = 0; c < 6; c++) {
+    flat.insert(flat.end(), by[c].begin(), by[c].end());
+    L.color_start[c + 1] = (int)flat.size();
+  }
+  L.max_len = maxlen;
+  L.lines = dalloc<int4>(flat.size());
+  upload(s, L.lines, flat.data(), flat.size());
+}
+
+void destroy_all(Solver& s) {
+  void* ptrs[] = {s.layers, s.x, s.v, s.dx, s.pred, s.delta, s.rhs, s.p, s.ap, s.last_update, s.mass, s.cn0, s.pinned,
+                  s.layer_of, s.rest, s.intact, s.fail_site, s.tails, s.softening, s.cut_mask, s.flags, s.strain,
+                  s.damage, s.area, s.sep, s.pullmark, s.strength, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k,
+                  s.partials, s.scal, s.counters};
+  for (void* p : ptrs) dfree(p);
+  for (int k = 0; k < s.nlev; k++) {
+    Level& L = s.lv[k];
+    dfree(L.op);
+    if (k > 0) dfree(L.fixed);
+    if (k > 0) dfree(L.x);
+    if (k > 0) dfree(L.b);
+    dfree(L.r);
+    dfree(L.pw);
+    dfree(L.lines);
+  }
+  if (s.scal_h) (void)hipHostFree(s.scal_h);
+  if (s.stream) (void)hipStreamDestroy(s.stream);
+}
+
+int fail(bd_solver* h, int code, const std::string& msg) {
+  if (h) h->s.err = msg;
+  return code;
+}
+
+template <class F>
+int guarded(bd_solver* h, F&& f) {
+  if (!h) return BD_ERR_INVALID;
+  try {
+    return f(h->s);
+  } catch (const std::exception& e) {
+    return fail(h, BD_ERR_HIP, e.what());
+  }
+}
+
+void build_hierarchy(Solver& s) {
+  for (int k = 0; k + 1 < s.nlev; k++) {
+    launch_prolong_weights(s, k);
+    launch_rap(s, k);
+  }
+  s.hierarchy = true;
+}
+
+void newton_iteration(Solver& s, int it, bool coarse) {
+  launch_assemble(s);
+  s.assembled = true;
+  if (coarse || !s.hierarchy) build_hierarchy(s);
+  pcg_solve(s);
+  launch_energy(s);
+  launch_newton_apply(s, it);
+  launch_record(s, it);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bd_last_error(const bd_solver* h) { return h ? h->s.err.c_str() : g_create_error.c_str(); }
+
+int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
+  g_create_error.clear();
+  if (!sc || !prm || !out) {
+    g_create_error = "bd_create: null argument";
+    return BD_ERR_INVALID;
+  }
+  *out = nullptr;
+  if (sc->node_count <= 0 || sc->layer_count <= 0 || sc->layer_count > 8) {
+    g_create_error = "bd_create: need 1..8 layers and a positive node count";
+    return BD_ERR_INVALID;
+  }
+  int total = 0;
+  for (int l = 0; l < sc->layer_count; l++) {
+    const bd_layer& L = sc->layers[l];
+    if (L.nx < 2 || L.ny < 2 || L.node_offset != total) {
+      g_create_error = "bd_create: layers must be at least 2x2 and stored contiguously";
+      return BD_ERR_INVALID;
+    }
+    if (L.nx > BD_MAX_LINE || L.ny > BD_MAX_LINE) {
+      g_create_error = "bd_create: yarn line longer than BD_MAX_LINE (" + std::to_string(BD_MAX_LINE) + ")";
+      return BD_ERR_LIMIT;
+    }
+    if (L.bundle_u < 1 || L.bundle_v < 1 || L.bundle_u > 32 || L.bundle_v > 32) {
+      g_create_error = "bd_create: bundle too large for the cut masks";
+      return BD_ERR_LIMIT;
+    }
+    total += L.nx * L.ny;
+  }
+  if (total != sc->node_count) {
+    g_create_error = "bd_create: node_count does not match the layers";
+    return BD_ERR_INVALID;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    g_create_error = "bd_create: no HIP device";
+    return BD_ERR_HIP;
+  }
+  bd_solver* h = new bd_solver();
+  Solver& s = h->s;
+  try {
+    s.P = *prm;
+    s.N = sc->node_count;
+    s.Np = pad64(s.N);
+    s.C = sc->crossing_count;
+    s.nl = sc->layer_count;
+    BD_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    s.geo.nl = s.nl;
+    for (int l = 0; l < s.nl; l++) {
+      s.layers_h[l] = dev_layer(sc->layers[l]);
+      s.geo.nx[l] = sc->layers[l].nx;
+      s.geo.ny[l] = sc->layers[l].ny;
+      s.geo.off[l] = sc->layers[l].node_offset;
+    }
+    s.layers = dalloc<DevLayer>(s.nl);
+    upload(s, s.layers, s.layers_h, s.nl);
+    const int N = s.N, Np = s.Np;
+    s.x = dalloc<float>(3 * (size_t)Np);
+    s.v = dalloc<float>(3 * (size_t)Np);
+    s.dx = dalloc<float>(3 * (size_t)Np);
+    s.pred = dalloc<float>(3 * (size_t)Np);
+    s.delta = dalloc<float>(3 * (size_t)Np);
+    s.rhs = dalloc<float>(3 * (size_t)Np);
+    s.p = dalloc<float>(3 * (size_t)Np);
+    s.ap = dalloc<float>(3 * (size_t)Np);
+    s.last_update = dalloc<float>(Np);
+    s.mass = dalloc<float>(Np);
+    s.cn0 = dalloc<double>(Np);
+    s.pinned = dalloc<uint8_t>(Np);
+    s.rest = dalloc<double>(2 * (size_t)N);
+    s.intact = dalloc<double>(2 * (size_t)N);
+    s.fail_site = dalloc<double>(2 * (size_t)N);
+    s.tails = dalloc<double>(4 * (size_t)N);
+    s.softening = dalloc<double>(2 * (size_t)N);
+    s.strain = dalloc<double>(2 * (size_t)N);
+    s.cut_mask = dalloc<uint32_t>(2 * (size_t)N);
+    s.flags = dalloc<uint16_t>(2 * (size_t)N);
+    s.damage = dalloc<uint8_t>(2 * (size_t)s.C);
+    s.area = dalloc<uint8_t>(2 * (size_t)s.C);
+    s.sep = dalloc<uint8_t>(2 * (size_t)s.C);
+    s.pullmark = dalloc<uint8_t>(2 * (size_t)s.C);
+    s.strength = dalloc<float>(2 * (size_t)s.C);
+    s.grip_of = dalloc<int>(Np);
+    s.grip_w = dalloc<float>(Np);
+    s.grip_off = dalloc<double>(3 * (size_t)Np);
+    s.grip_goal_k = dalloc<double>(4 * BD_MAX_GRIPS);
+    s.partials = dalloc<double>(kDotBlocks * 8);
+    s.scal = dalloc<Scal>(1);
+    s.counters = dalloc<int>(8);
+    BD_HIP(hipHostMalloc((void**)&s.scal_h, sizeof(Scal), hipHostMallocDefault));
+    std::memset(s.scal_h, 0, sizeof(Scal));
+
+    std::vector<float> f(Np, 0.f);
+    for (int n = 0; n < N; n++) f[n] = (float)sc->mass[n];
+    upload(s, s.mass, f.data(), Np);
+    upload(s, s.cn0, sc->crossing_normal0, N);
+    upload(s, s.pinned, sc->pinned, N);
+    std::vector<int> gof(Np, -1);
+    upload(s, s.grip_of, gof.data(), Np);
+    upload(s, s.rest, sc->rest_length, 2 * (size_t)N);
+    upload(s, s.intact, sc->intact, 2 * (size_t)N);
+    upload(s, s.fail_site, sc->fail_site, 2 * (size_t)N);
+    upload(s, s.tails, sc->tails, 4 * (size_t)N);
+    upload(s, s.softening, sc->softening, 2 * (size_t)N);
+    upload(s, s.cut_mask, sc->cut_mask, 2 * (size_t)N);
+    upload(s, s.flags, sc->seg_flags, 2 * (size_t)N);
+    if (s.C > 0) {
+      upload(s, s.damage, sc->field_damage, 2 * (size_t)s.C);
+      upload(s, s.area, sc->field_area, 2 * (size_t)s.C);
+      upload(s, s.strength, sc->field_strength, 2 * (size_t)s.C);
+    }
+    upload_vec3(s, s.x, N, Np, sc->position);
+
+    std::vector<Geo> geos = level_layouts(s.geo, prm->max_coarsest_nodes);
+    s.nlev = (int)geos.size();
+    for (int k = 0; k < s.nlev; k++) {
+      Level& L = s.lv[k];
+      L.geo = geos[k];
+      L.N = 0;
+      for (int l = 0; l < L.geo.nl; l++) L.N += L.geo.nx[l] * L.geo.ny[l];
+      L.Np = pad64(L.N);
+      L.fine = k == 0;
+      L.op = dalloc<float>((size_t)(L.fine ? kFineComps : kCoarseComps) * L.Np);
+      if (k == 0) {
+        L.fixed = s.pinned;
+        L.x = dalloc<float>(3 * (size_t)L.Np);  // PCG z
+        L.b = dalloc<float>(3 * (size_t)L.Np);  // PCG r
+      } else {
+        L.fixed = dalloc<uint8_t>(L.Np);
+        L.x = dalloc<float>(3 * (size_t)L.Np);
+        L.b = dalloc<float>(3 * (size_t)L.Np);
+      }
+      L.r = dalloc<float>(3 * (size_t)L.Np);
+      if (k + 1 < s.nlev) L.pw = dalloc<float>(4 * (size_t)L.Np);
+      build_lines(s, L);
+    }
+    smooth_set_attributes();
+    s.stats.level_count = s.nlev;
+    for (int k = 0; k < s.nlev && k < BD_MAX_LEVELS; k++) s.stats.level_nodes[k] = s.lv[k].N;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    destroy_all(s);
+    delete h;
+    return BD_ERR_HIP;
+  }
+  *out = h;
+  return BD_OK;
+}
+
+int bd_destroy(bd_solver* h) {
+  if (!h) return BD_OK;
+  (void)hipStreamSynchronize(h->s.stream);
+  destroy_all(h->s);
+  delete h;
+  return BD_OK;
+}
+
+int bd_set_state(bd_solver* h, const double* x, const double* v) {
+  return guarded(h, [&](Solver& s) {
+    if (!x) return fail(h, BD_ERR_INVALID, "bd_set_state: null positions");
+    upload_vec3(s, s.x, s.N, s.Np, x);
+    if (v)
+      upload_vec3(s, s.v, s.N, s.Np, v);
+    else
+      BD_HIP(hipMemsetAsync(s.v, 0, 3 * (size_t)s.Np * sizeof(float), s.stream));
+    BD_HIP(hipStreamSynchronize(s.stream));
+    return BD_OK;
+  });
+}
+
+int bd_set_segments(bd_solver* h, const double* rest, const double* intact, const uint32_t* cut, const uint16_t* flags,
+                    const double* fail_site, const double* tails) {
+  return guarded(h, [&](Solver& s) {
+    size_t n2 = 2 * (size_t)s.N;
+    if (rest) upload(s, s.rest, rest, n2);
+    if (intact) upload(s, s.intact, intact, n2);
+    if (cut) upload(s, s.cut_mask, cut, n2);
+    if (flags) upload(s, s.flags, flags, n2);
+    if (fail_site) upload(s, s.fail_site, fail_site, n2);
+    if (tails) upload(s, s.tails, tails, 2 * n2);
+    return BD_OK;
+  });
+}
+
+int bd_set_field(bd_solver* h, const uint8_t* damage) {
+  return guarded(h, [&](Solver& s) {
+    if (!damage) return fail(h, BD_ERR_INVALID, "bd_set_field: null damage");
+    if (s.C > 0) upload(s, s.damage, damage, 2 * (size_t)s.C);
+    return BD_OK;
+  });
+}
+
+int bd_set_grips(bd_solver* h, const bd_grips* g) {
+  return guarded(h, [&](Solver& s) {
+    std::vector<int> gof(s.Np, -1);
+    std::vector<float> w(s.Np, 0.f);
+    std::vector<double> off(3 * (size_t)s.Np, 0.0), gk(4 * BD_MAX_GRIPS, 0.0);
+    int count = 0;
+    if (g) {
+      if (g->grip_count < 0 || g->grip_count > BD_MAX_GRIPS) return fail(h, BD_ERR_LIMIT, "bd_set_grips: more than BD_MAX_GRIPS grips");
+      count = g->grip_count;
+      for (int k = 0; k < count; k++) {
+        for (int q = 0; q < 3; q++) gk[4 * k + q] = g->goal[3 * k + q];
+        gk[4 * k + 3] = g->stiffness[k];
+      }
+      for (int e = 0; e < g->entry_count; e++) {
+        int n = g->node[e];
+        if (n < 0 || n >= s.N || g->grip[e] < 0 || g->grip[e] >= count)
+          return fail(h, BD_ERR_INVALID, "bd_set_grips: entry out of range");
+        gof[n] = g->grip[e];
+        w[n] = (float)g->weight[e];
+        for (int q = 0; q < 3; q++) off[(size_t)q * s.Np + n] = g->offset[3 * e + q];
+      }
+    }
+    s.grip_count = count;
+    upload(s, s.grip_of, gof.data(), gof.size());
+    upload(s, s.grip_w, w.data(), w.size());
+    upload(s, s.grip_off, off.data(), off.size());
+    upload(s, s.grip_goal_k, gk.data(), gk.size());
+    return BD_OK;
+  });
+}
+
+int bd_set_timestep(bd_solver* h, double dt) {
+  return guarded(h, [&](Solver& s) {
+    if (!(dt > 0)) return fail(h, BD_ERR_INVALID, "bd_set_timestep: h must be positive");
+    s.P.h = dt;
+    return BD_OK;
+  });
+}
+
+int bd_set_history(bd_solver* h, int on) {
+  return guarded(h, [&](Solver& s) {
+    s.P.history = on ? 1 : 0;
+    return BD_OK;
+  });
+}
+
+int bd_set_newton_tolerance(bd_solver* h, double tol) {
+  return guarded(h, [&](Solver& s) {
+    s.P.newton_tolerance = tol;
+    return BD_OK;
+  });
+}
+
+int bd_set_pcg(bd_solver* h, int iterations, double rel_tol, int max_iterations) {
+  return guarded(h, [&](Solver& s) {
+    if (iterations < 0 || rel_tol < 0 || (rel_tol > 0 && max_iterations < 1)) return fail(h, BD_ERR_INVALID, "bd_set_pcg: bad schedule");
+    s.P.pcg_iterations = iterations;
+    s.P.pcg_rel_tol = rel_tol;
+    s.P.pcg_max_iterations = max_iterations;
+    return BD_OK;
+  });
+}
+
+int bd_predict(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    launch_predict(s);
+    return BD_OK;
+  });
+}
+
+int bd_assemble(bd_solver* h, int coarse) {
+  return guarded(h, [&](Solver& s) {
+    launch_assemble(s);
+    s.assembled = true;
+    if (coarse || !s.hierarchy) build_hierarchy(s);
+    return BD_OK;
+  });
+}
+
+int bd_vcycle(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    if (!s.hierarchy) return fail(h, BD_ERR_STATE, "bd_vcycle: assemble first");
+    BD_HIP(hipMemsetAsync(&s.scal->done, 0, sizeof(int), s.stream));
+    launch_zero(s, s.lv[0].x, 3 * (size_t)s.lv[0].Np);
+    vcycle(s, 0);
+    return BD_OK;
+  });
+}
+
+int bd_smooth(bd_solver* h, int level, int reverse) {
+  return guarded(h, [&](Solver& s) {
+    if (!s.hierarchy) return fail(h, BD_ERR_STATE, "bd_smooth: assemble first");
+    if (level < 0 || level >= s.nlev) return fail(h, BD_ERR_INVALID, "bd_smooth: level out of range");
+    BD_HIP(hipMemsetAsync(&s.scal->done, 0, sizeof(int), s.stream));
+    launch_smooth(s, level, reverse);
+    return BD_OK;
+  });
+}
+
+int bd_residual(bd_solver* h, int level) {
+  return guarded(h, [&](Solver& s) {
+    if (!s.hierarchy) return fail(h, BD_ERR_STATE, "bd_residual: assemble first");
+    if (level < 0 || level >= s.nlev) return fail(h, BD_ERR_INVALID, "bd_residual: level out of range");
+    BD_HIP(hipMemsetAsync(&s.scal->done, 0, sizeof(int), s.stream));
+    Level& L = s.lv[level];
+    launch_residual(s, level, L.x, L.b, L.r);
+    return BD_OK;
+  });
+}
+
+int bd_pcg(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    if (!s.hierarchy) return fail(h, BD_ERR_STATE, "bd_pcg: assemble first");
+    pcg_solve(s);
+    launch_record(s, 0);
+    return BD_OK;
+  });
+}
+
+int bd_line_search(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    launch_energy(s);
+    return BD_OK;
+  });
+}
+
+int bd_newton_apply(bd_solver* h, int it) {
+  return guarded(h, [&](Solver& s) {
+    if (it < 0 || it >= BD_MAX_NEWTON) return fail(h, BD_ERR_INVALID, "bd_newton_apply: iteration out of range");
+    launch_newton_apply(s, it);
+    return BD_OK;
+  });
+}
+
+int bd_finalize(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    launch_commit(s);
+    if (s.P.history) {
+      launch_damage(s);
+      launch_slip(s);
+    }
+    return BD_OK;
+  });
+}
+
+int bd_damage(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    launch_damage(s);
+    return BD_OK;
+  });
+}
+
+int bd_slip(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    launch_slip(s);
+    return BD_OK;
+  });
+}
+
+int bd_substep(bd_solver* h, int min_it, int max_it) {
+  return guarded(h, [&](Solver& s) {
+    const int n_it = std::min(BD_MAX_NEWTON, std::max(s.P.newton_iterations, min_it));
+    const int n_max = std::min(BD_MAX_NEWTON, std::max(std::max(s.P.newton_max, n_it), max_it));
+    launch_reset_counters(s);
+    launch_predict(s);
+    int done = 0;
+    for (int it = 0; it < n_max; it++) {
+      if (it >= n_it) {
+        // gated optional iteration (gate.wgsl): only while the last update moved a node > tolerance
+        BD_HIP(hipMemcpyAsync(s.scal_h->max_update, s.scal->max_update, sizeof(s.scal_h->max_update), hipMemcpyDeviceToHost, s.stream));
+        BD_HIP(hipStreamSynchronize(s.stream));
+        float moved;
+        std::memcpy(&moved, &s.scal_h->max_update[it - 1], sizeof(float));
+        if (!(moved > (float)s.P.newton_tolerance)) break;
+      }
+      newton_iteration(s, it, it == 0 || !s.P.lag_coarse);
+      done = it + 1;
+    }
+    s.stats.newton_iterations = done;
+    launch_commit(s);
+    if (s.P.history) {
+      launch_damage(s);
+      launch_slip(s);
+    }
+    return BD_OK;
+  });
+}
+
+int bd_sync(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    BD_HIP(hipStreamSynchronize(s.stream));
+    return BD_OK;
+  });
+}
+
+size_t bd_buffer_bytes(const bd_solver* h, bd_buffer buf, int level) {
+  if (!h) return 0;
+  const Solver& s = h->s;
+  int Nl = (level >= 0 && level < s.nlev) ? s.lv[level].N : 0;
+  switch (buf) {
+    case BD_BUF_X: case BD_BUF_V: case BD_BUF_DX: case BD_BUF_PRED: case BD_BUF_DELTA: case BD_BUF_RHS:
+      return 3 * (size_t)s.N * sizeof(double);
+    case BD_BUF_LEVEL_X: case BD_BUF_LEVEL_B: case BD_BUF_LEVEL_R:
+      return 3 * (size_t)Nl * sizeof(double);
+    case BD_BUF_STENCIL: return (size_t)Nl * 25 * 9 * sizeof(double);
+    case BD_BUF_FIXED: return (size_t)Nl;
+    case BD_BUF_PROLONG: return (level + 1 < s.nlev) ? 4 * (size_t)Nl * sizeof(double) : 0;
+    case BD_BUF_REST: case BD_BUF_INTACT: case BD_BUF_FAILSITE: return 2 * (size_t)s.N * sizeof(double);
+    case BD_BUF_CUTMASK: return 2 * (size_t)s.N * sizeof(uint32_t);
+    case BD_BUF_SEGFLAGS: return 2 * (size_t)s.N * sizeof(uint16_t);
+    case BD_BUF_TAILS: return 4 * (size_t)s.N * sizeof(double);
+    case BD_BUF_FIELD: return 2 * (size_t)s.C;
+    case BD_BUF_LAST_UPDATE: return (size_t)s.N * sizeof(double);
+    case BD_BUF_ENERGY: return 8 * sizeof(double);
+    default: return 0;
+  }
+}
+
+// expand a level operator to N x 25 x 9 (row-major 3x3 per stencil offset (dj+2)*5 + di+2)
+static void expand_stencil(const Solver& s, int k, const std::vector<float>& op, double* out) {
+  const Level& L = s.lv[k];
+  const int Np = L.Np;
+  std::memset(out, 0, (size_t)L.N * 225 * sizeof(double));
+  auto put = [&](int n, int di, int dj, const float* b6) {
+    double* d = out + ((size_t)n * 25 + (dj + 2) * 5 + (di + 2)) * 9;
+    const float full[9] = {b6[0], b6[1], b6[2], b6[1], b6[3], b6[4], b6[2], b6[4], b6[5]};
+    for (int q = 0; q < 9; q++) d[q] = full[q];
+  };
+  auto blk = [&](int base, int row, float* o) {
+    for (int q = 0; q < 6; q++) o[q] = op[(size_t)(base + q) * Np + row];
+  };
+  for (int l = 0; l < L.geo.nl; l++) {
+    int nx = L.geo.nx[l], ny = L.geo.ny[l], off = L.geo.off[l];
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        int n = off + j * nx + i;
+        float b[6];
+        blk(0, n, b);
+        put(n, 0, 0, b);
+        if (L.fine) {
+          const int fwd[4][3] = {{F_E, 1, 0}, {F_N, 0, 1}, {F_NE, 1, 1}, {F_NW, -1, 1}};
+          for (auto& f : fwd) {
+            int di = f[1], dj = f[2];
+            if (i + di >= 0 && i + di < nx && j + dj < ny) {
+              blk(f[0], n, b);
+              put(n, di, dj, b);
+            }
+            if (i - di >= 0 && i - di < nx && j - dj >= 0) {
+              blk(f[0], n - dj * nx - di, b);
+              put(n, -di, -dj, b);
+            }
+          }
+          const int sc[2][3] = {{F_EE, 2, 0}, {F_NN, 0, 2}};
+          for (auto& f : sc) {
+            int di = f[1], dj = f[2];
+            float v;
+            if (i + di < nx && j + dj < ny) {
+              v = op[(size_t)f[0] * Np + n];
+              float s6[6] = {v, 0, 0, v, 0, v};
+              put(n, di, dj, s6);
+            }
+            if (i - di >= 0 && j - dj >= 0) {
+              v = op[(size_t)f[0] * Np + n - dj * nx - di];
+              float s6[6] = {v, 0, 0, v, 0, v};
+              put(n, -di, -dj, s6);
+            }
+          }
+        } else {
+          for (int kb = 1; kb < kCoarseBlocks; kb++) {
+            int di = coarse_di(kb), dj = coarse_dj(kb);
+            if (i + di >= 0 && i + di < nx && j + dj < ny) {
+              blk(6 * kb, n, b);
+              put(n, di, dj, b);
+            }
+            if (i - di >= 0 && i - di < nx && j - dj >= 0) {
+              blk(6 * kb, n - dj * nx - di, b);
+              put(n, -di, -dj, b);
+            }
+          }
+        }
+      }
+  }
+}
+
+int bd_read(bd_solver* h, bd_buffer buf, int level, void* dst, size_t bytes) {
+  return guarded(h, [&](Solver& s) {
+    size_t need = bd_buffer_bytes(h, buf, level);
+    if (need == 0) return fail(h, BD_ERR_INVALID, "bd_read: unknown buffer or level");
+    if (!dst || bytes < need) return fail(h, BD_ERR_INVALID, "bd_read: destination too small");
+    double* d = static_cast<double*>(dst);
+    switch (buf) {
+      case BD_BUF_X: download_vec3(s, s.x, s.N, s.Np, d); break;
+      case BD_BUF_V: download_vec3(s, s.v, s.N, s.Np, d); break;
+      case BD_BUF_DX: download_vec3(s, s.dx, s.N, s.Np, d); break;
+      case BD_BUF_PRED: download_vec3(s, s.pred, s.N, s.Np, d); break;
+      case BD_BUF_DELTA: download_vec3(s, s.delta, s.N, s.Np, d); break;
+      case BD_BUF_RHS: download_vec3(s, s.rhs, s.N, s.Np, d); break;
+      case BD_BUF_LEVEL_X: download_vec3(s, s.lv[level].x, s.lv[level].N, s.lv[level].Np, d); break;
+      case BD_BUF_LEVEL_B: download_vec3(s, s.lv[level].b, s.lv[level].N, s.lv[level].Np, d); break;
+      case BD_BUF_LEVEL_R: download_vec3(s, s.lv[level].r, s.lv[level].N, s.lv[level].Np, d); break;
+      case BD_BUF_STENCIL: {
+        const Level& L = s.lv[level];
+        std::vector<float> op((size_t)(L.fine ? kFineComps : kCoarseComps) * L.Np);
+        download(s, op.data(), L.op, op.size());
+        expand_stencil(s, level, op, d);
+        break;
+      }
+      case BD_BUF_FIXED: download(s, static_cast<uint8_t*>(dst), s.lv[level].fixed, (size_t)s.lv[level].N); break;
+      case BD_BUF_PROLONG: {
+        const Level& L = s.lv[level];
+        std::vector<float> pw(4 * (size_t)L.Np);
+        download(s, pw.data(), L.pw, pw.size());
+        for (int n = 0; n < L.N; n++)
+          for (int q = 0; q < 4; q++) d[4 * n + q] = pw[(size_t)q * L.Np + n];
+        break;
+      }
+      case BD_BUF_REST: download(s, d, s.rest, 2 * (size_t)s.N); break;
+      case BD_BUF_INTACT: download(s, d, s.intact, 2 * (size_t)s.N); break;
+      case BD_BUF_FAILSITE: download(s, d, s.fail_site, 2 * (size_t)s.N); break;
+      case BD_BUF_TAILS: download(s, d, s.tails, 4 * (size_t)s.N); break;
+      case BD_BUF_CUTMASK: download(s, static_cast<uint32_t*>(dst), s.cut_mask, 2 * (size_t)s.N); break;
+      case BD_BUF_SEGFLAGS: download(s, static_cast<uint16_t*>(dst), s.flags, 2 * (size_t)s.N); break;
+      case BD_BUF_FIELD: download(s, static_cast<uint8_t*>(dst), s.damage, 2 * (size_t)s.C); break;
+      case BD_BUF_LAST_UPDATE: {
+        std::vector<float> t(s.N);
+        download(s, t.data(), s.last_update, (size_t)s.N);
+        for (int n = 0; n < s.N; n++) d[n] = t[n];
+        break;
+      }
+      case BD_BUF_ENERGY: {
+        download(s, s.scal_h, s.scal, 1);
+        for (int k = 0; k < 4; k++) {
+          d[k] = s.scal_h->e[k];
+          d[4 + k] = s.scal_h->m[k];
+        }
+        break;
+      }
+      default: return fail(h, BD_ERR_INVALID, "bd_read: unsupported buffer");
+    }
+    return BD_OK;
+  });
+}
+
+int bd_write(bd_solver* h, bd_buffer buf, int level, const void* src, size_t bytes) {
+  return guarded(h, [&](Solver& s) {
+    size_t need = bd_buffer_bytes(h, buf, level);
+    if (need == 0) return fail(h, BD_ERR_INVALID, "bd_write: unknown buffer or level");
+    if (!src || bytes < need) return fail(h, BD_ERR_INVALID, "bd_write: source too small");
+    const double* d = static_cast<const double*>(src);
+    switch (buf) {
+      case BD_BUF_X: upload_vec3(s, s.x, s.N, s.Np, d); break;
+      case BD_BUF_V: upload_vec3(s, s.v, s.N, s.Np, d); break;
+      case BD_BUF_DX: upload_vec3(s, s.dx, s.N, s.Np, d); break;
+      case BD_BUF_PRED: upload_vec3(s, s.pred, s.N, s.Np, d); break;
+      case BD_BUF_DELTA: upload_vec3(s, s.delta, s.N, s.Np, d); break;
+      case BD_BUF_RHS: upload_vec3(s, s.rhs, s.N, s.Np, d); break;
+      case BD_BUF_LEVEL_X: upload_vec3(s, s.lv[level].x, s.lv[level].N, s.lv[level].Np, d); break;
+      case BD_BUF_LEVEL_B: upload_vec3(s, s.lv[level].b, s.lv[level].N, s.lv[level].Np, d); break;
+      case BD_BUF_REST: upload(s, s.rest, d, 2 * (size_t)s.N); break;
+      case BD_BUF_INTACT: upload(s, s.intact, d, 2 * (size_t)s.N); break;
+      case BD_BUF_FAILSITE: upload(s, s.fail_site, d, 2 * (size_t)s.N); break;
+      case BD_BUF_TAILS: upload(s, s.tails, d, 4 * (size_t)s.N); break;
+      case BD_BUF_CUTMASK: upload(s, s.cut_mask, static_cast<const uint32_t*>(src), 2 * (size_t)s.N); break;
+      case BD_BUF_SEGFLAGS: upload(s, s.flags, static_cast<const uint16_t*>(src), 2 * (size_t)s.N); break;
+      case BD_BUF_FIELD: upload(s, s.damage, static_cast<const uint8_t*>(src), 2 * (size_t)s.C); break;
+      default: return fail(h, BD_ERR_INVALID, "bd_write: buffer is read-only");
+    }
+    return BD_OK;
+  });
+}
+
+int bd_get_stats(bd_solver* h, bd_stats* out) {
+  return guarded(h, [&](Solver& s) {
+    if (!out) return fail(h, BD_ERR_INVALID, "bd_get_stats: null output");
+    download(s, s.scal_h, s.scal, 1);
+    int counters[8];
+    download(s, counters, s.counters, 8);
+    bd_stats st = s.stats;
+    for (int k = 0; k < BD_MAX_NEWTON; k++) {
+      st.pcg_iterations[k] = s.scal_h->rec_iters[k];
+      st.rel_residual[k] = s.scal_h->rec_relres[k];
+      st.alpha[k] = s.scal_h->rec_alpha[k];
+      float mu;
+      std::memcpy(&mu, &s.scal_h->max_update[k], sizeof(float));
+      st.max_update[k] = mu;
+    }
+    st.rejections = s.scal_h->rejections;
+    st.ruptures = counters[0];
+    st.pulled_out = counters[1];
+    *out = st;
+    return BD_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,540 @@
+// PCG with the symmetric V-cycle preconditioner, and the line-search energy changes.
+//
+// pcg restates newtonMultigrid.ts:537-576: r = b - A d (d = 0), z = V(r), p = z, then per
+// iteration Ap = A p, alpha = r.z / p.Ap, d += alpha p, r -= alpha Ap, z = V(r), beta, p = z + beta p;
+// the last fixed-count iteration skips its V-cycle. Extension: with a relative tolerance it stops
+// once ||r|| / ||b|| <= tol. Dot products accumulate in FP64 with a fixed grid and a fixed summation
+// order (deterministic); scalars never leave the device, and a `done` flag gates every kernel of the
+// loop so that iterations queued past convergence return at once.
+//
+// k_energy restates energyChange / chooseStep (solver.ts:430-534, newtonMultigrid.ts:582-603) in
+// FP64: per node-owned term, the change of the incremental potential at alpha in {1, 1/2, 1/4, 1/8}
+// by per-term differences and their magnitude sums, reduced deterministically, then the step rule.
+#include <stdexcept>
+
+#include "bd_internal.h"
+
+namespace bd {
+
+namespace {
+
+__device__ __forceinline__ void store_agent(double* p, double v) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __double_as_longlong(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_agent(const double* p) {
+  return __longlong_as_double(__hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Grid-wide deterministic sum of NV doubles per thread. Returns true in thread 0 of the last block
+// to finish, with the totals in tot (summed in block order).
+template <int NV>
+__device__ bool grid_sum(double (&v)[NV], double* partials, unsigned int* ticket, double (&tot)[NV]) {
+  __shared__ double sh[NV][16];
+  __shared__ int last;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    double x = v[q];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) sh[q][wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < NV; q++) {
+      double s = 0;
+      for (int w = 0; w < nw; w++) s += sh[q][w];
+      store_agent(&partials[blockIdx.x * NV + q], s);
+    }
+    __threadfence();
+    unsigned int t = atomicAdd(ticket, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last || threadIdx.x != 0) return false;
+  __threadfence();
+  for (int q = 0; q < NV; q++) tot[q] = 0;
+  for (unsigned int b = 0; b < gridDim.x; b++)
+    for (int q = 0; q < NV; q++) tot[q] += load_agent(&partials[b * NV + q]);
+  *ticket = 0u;
+  return true;
+}
+
+struct PcgArgs {
+  int N, Np;
+  const uint8_t* fixed;
+  Scal* sc;
+  double* partials;
+  float *r, *z, *p, *ap, *delta;
+  const float* rhs;
+};
+
+__global__ __launch_bounds__(256) void k_pcg_start(PcgArgs A) {
+  double v[1] = {0};
+  const int Np = A.Np;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < A.N; n += gridDim.x * blockDim.x) {
+    bool fx = A.fixed[n];
+    for (int q = 0; q < 3; q++) {
+      float b = fx ? 0.f : A.rhs[q * Np + n];
+      A.r[q * Np + n] = b;
+      A.z[q * Np + n] = 0.f;
+      A.delta[q * Np + n] = 0.f;
+      A.p[q * Np + n] = 0.f;
+      A.ap[q * Np + n] = 0.f;
+      v[0] += (double)b * b;
+    }
+  }
+  double tot[1];
+  if (grid_sum<1>(v, A.partials, &A.sc->ticket, tot)) {
+    Scal* s = A.sc;
+    s->bb = tot[0];
+    s->relres = tot[0] > 0 ? 1.0 : 0.0;
+    s->iters = 0;
+    s->rz = 0;
+    s->beta = 0;
+    s->done = (s->tol > 0 && !(tot[0] > 0)) ? 1 : 0;
+  }
+}
+
+// rz = r . z ; beta = rz / rz_prev (first: beta = 0)
+__global__ __launch_bounds__(256) void k_dot_rz(PcgArgs A, int first) {
+  if (A.sc->done) return;
+  double v[1] = {0};
+  const int Np = A.Np;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < A.N; n += gridDim.x * blockDim.x)
+    for (int q = 0; q < 3; q++) v[0] += (double)A.r[q * Np + n] * (double)A.z[q * Np + n];
+  double tot[1];
+  if (grid_sum<1>(v, A.partials, &A.sc->ticket, tot)) {
+    Scal* s = A.sc;
+    s->beta = first ? 0.0 : tot[0] / s->rz;
+    s->rz = tot[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_update_p(PcgArgs A) {
+  if (A.sc->done) return;
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= A.N) return;
+  const int Np = A.Np;
+  float beta = (float)A.sc->beta;
+  for (int q = 0; q < 3; q++) A.p[q * Np + n] = A.z[q * Np + n] + beta * A.p[q * Np + n];
+}
+
+__global__ __launch_bounds__(256) void k_update_xr(PcgArgs A) {
+  if (A.sc->done) return;
+  const int Np = A.Np;
+  float alpha = (float)A.sc->alpha;
+  double v[1] = {0};
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < A.N; n += gridDim.x * blockDim.x)
+    for (int q = 0; q < 3; q++) {
+      A.delta[q * Np + n] += alpha * A.p[q * Np + n];
+      float r = A.r[q * Np + n] - alpha * A.ap[q * Np + n];
+      A.r[q * Np + n] = r;
+      A.z[q * Np + n] = 0.f;
+      v[0] += (double)r * r;
+    }
+  double tot[1];
+  if (grid_sum<1>(v, A.partials, &A.sc->ticket, tot)) {
+    Scal* s = A.sc;
+    s->rr = tot[0];
+    s->relres = s->bb > 0 ? sqrt(tot[0] / s->bb) : 0.0;
+    s->iters += 1;
+    int limit = s->tol > 0 ? s->max_iters : s->fixed_iters;
+    if ((s->tol > 0 && s->relres <= s->tol) || s->iters >= limit) s->done = 1;
+  }
+}
+
+// Ap = A p (fixed rows 0) and p . Ap ; alpha = rz / pAp, stop when pAp <= 0
+__device__ __forceinline__ void blk_load6(const float* op, int Np, int base, int row, float o[6]) {
+#pragma unroll
+  for (int q = 0; q < 6; q++) o[q] = op[(base + q) * Np + row];
+}
+
+__global__ __launch_bounds__(256) void k_matvec_dot_fine(PcgArgs A, const float* op, Geo geo) {
+  if (A.sc->done) return;
+  const int Np = A.Np;
+  double v[1] = {0};
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < A.N; n += gridDim.x * blockDim.x) {
+    float y[3] = {0.f, 0.f, 0.f};
+    if (!A.fixed[n]) {
+      int l = geo_layer(geo, n);
+      int r0 = n - geo.off[l], nx = geo.nx[l], ny = geo.ny[l];
+      int j = r0 / nx, i = r0 - j * nx;
+      const float* p = A.p;
+      float b[6], xm[3], t[3];
+      auto acc = [&](int m) {
+        xm[0] = p[m];
+        xm[1] = p[Np + m];
+        xm[2] = p[2 * Np + m];
+        sym_mul(b, xm, t);
+        y[0] += t[0];
+        y[1] += t[1];
+        y[2] += t[2];
+      };
+      auto accs = [&](float s, int m) {
+        y[0] += s * p[m];
+        y[1] += s * p[Np + m];
+        y[2] += s * p[2 * Np + m];
+      };
+      blk_load6(op, Np, F_C, n, b);
+      acc(n);
+      const bool e = i + 1 < nx, w = i > 0, no = j + 1 < ny, so = j > 0;
+      if (e) { blk_load6(op, Np, F_E, n, b); acc(n + 1); }
+      if (w) { blk_load6(op, Np, F_E, n - 1, b); acc(n - 1); }
+      if (no) { blk_load6(op, Np, F_N, n, b); acc(n + nx); }
+      if (so) { blk_load6(op, Np, F_N, n - nx, b); acc(n - nx); }
+      if (e && no) { blk_load6(op, Np, F_NE, n, b); acc(n + nx + 1); }
+      if (w && so) { blk_load6(op, Np, F_NE, n - nx - 1, b); acc(n - nx - 1); }
+      if (w && no) { blk_load6(op, Np, F_NW, n, b); acc(n + nx - 1); }
+      if (e && so) { blk_load6(op, Np, F_NW, n - nx + 1, b); acc(n - nx + 1); }
+      if (i + 2 < nx) accs(op[F_EE * Np + n], n + 2);
+      if (i >= 2) accs(op[F_EE * Np + n - 2], n - 2);
+      if (j + 2 < ny) accs(op[F_NN * Np + n], n + 2 * nx);
+      if (j >= 2) accs(op[F_NN * Np + n - 2 * nx], n - 2 * nx);
+    }
+    for (int q = 0; q < 3; q++) {
+      A.ap[q * Np + n] = y[q];
+      v[0] += (double)A.p[q * Np + n] * (double)y[q];
+    }
+  }
+  double tot[1];
+  if (grid_sum<1>(v, A.partials, &A.sc->ticket, tot)) {
+    Scal* s = A.sc;
+    s->pap = tot[0];
+    if (!(tot[0] > 0)) {
+      s->done = 1;
+      s->alpha = 0;
+    } else {
+      s->alpha = s->rz / tot[0];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- line-search energy (FP64)
+struct EnergyArgs {
+  int N, Np;
+  Geo geo;
+  const DevLayer* layers;
+  double h;
+  const float *x, *dx, *pred, *d, *mass;
+  const double* cn0;
+  const uint8_t* pinned;
+  const double *rest, *intact;
+  const uint32_t* cut;
+  const uint16_t* flags;
+  const int* grip_of;
+  const float* grip_w;
+  const double* grip_off;
+  const double* grip_goal_k;
+  Scal* sc;
+  double* partials;
+};
+
+__device__ __forceinline__ bool act(const EnergyArgs& E, int s) { return (E.flags[s] & (SEG_EXISTS | SEG_BROKEN)) == SEG_EXISTS; }
+
+__device__ __forceinline__ void ld3d(const float* p, int Np, int n, double o[3]) {
+  o[0] = p[n];
+  o[1] = p[Np + n];
+  o[2] = p[2 * Np + n];
+}
+
+__device__ __forceinline__ double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// lenChange (solver.ts:447-451)
+__device__ __forceinline__ void len_change(const double* e0, const double* de, double a, double l0, double& dl, double& la) {
+  double ea[3] = {e0[0] + a * de[0], e0[1] + a * de[1], e0[2] + a * de[2]};
+  la = sqrt(dot3(ea, ea));
+  dl = (2 * a * dot3(e0, de) + a * a * dot3(de, de)) / fmax(la + l0, 1e-12);
+}
+
+__global__ __launch_bounds__(256) void k_energy(EnergyArgs E) {
+  const double al[4] = {1.0, 0.5, 0.25, 0.125};
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // e[4], m[4]
+  const int Np = E.Np;
+  const double h = E.h;
+  auto add = [&](int k, double v) {
+    acc[k] += v;
+    acc[4 + k] += fabs(v);
+  };
+  auto quad = [&](double gd, double q) {
+    for (int k = 0; k < 4; k++) {
+      acc[k] += al[k] * gd + 0.5 * al[k] * al[k] * q;
+      acc[4 + k] += fabs(al[k] * gd) + fabs(0.5 * al[k] * al[k] * q);
+    }
+  };
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < E.N; n += gridDim.x * blockDim.x) {
+    const int l = geo_layer(E.geo, n);
+    const DevLayer& L = E.layers[l];
+    const int nx = E.geo.nx[l], ny = E.geo.ny[l], r0 = n - E.geo.off[l];
+    const int j = r0 / nx, i = r0 - j * nx;
+    auto dir = [&](int m, double o[3]) {
+      if (E.pinned[m]) {
+        o[0] = o[1] = o[2] = 0;
+      } else {
+        ld3d(E.d, Np, m, o);
+      }
+    };
+    double dn[3], d0[3], xn[3];
+    dir(n, dn);
+    ld3d(E.dx, Np, n, d0);
+    ld3d(E.x, Np, n, xn);
+    if (!E.pinned[n]) {
+      double w = (double)E.mass[n] / (h * h);
+      double pr[3];
+      ld3d(E.pred, Np, n, pr);
+      quad(w * (dn[0] * (d0[0] - pr[0]) + dn[1] * (d0[1] - pr[1]) + dn[2] * (d0[2] - pr[2])), w * dot3(dn, dn));
+      int gi = E.grip_of[n];
+      if (gi >= 0) {
+        const double* G = E.grip_goal_k + 4 * gi;
+        double kg = G[3] * (double)E.grip_w[n];
+        double g[3];
+        for (int q = 0; q < 3; q++) g[q] = kg * ((xn[q] - G[q]) + d0[q] - E.grip_off[q * Np + n]);
+        if (kg > 0) quad(dot3(g, dn), kg * dot3(dn, dn));
+      }
+      if (L.air_drag > 0) {
+        double k = (L.air_drag * L.pu * L.pv) / h;
+        double t[2][3], ll[2];
+        for (int f = 0; f < 2; f++) {
+          int pv = f == 0 ? (i > 0 ? n - 1 : -1) : (j > 0 ? n - nx : -1);
+          int nv = f == 0 ? (i + 1 < nx ? n + 1 : -1) : (j + 1 < ny ? n + nx : -1);
+          int a = (pv >= 0 && act(E, 2 * pv + f)) ? pv : n;
+          int b = (nv >= 0 && act(E, 2 * n + f)) ? nv : n;
+          double xa[3], xb[3];
+          ld3d(E.x, Np, a, xa);
+          ld3d(E.x, Np, b, xb);
+          for (int q = 0; q < 3; q++) t[f][q] = xb[q] - xa[q];
+          ll[f] = sqrt(dot3(t[f], t[f]));
+        }
+        double e[3], kpar, kperp;
+        bool done = false;
+        if (ll[0] > 0 && ll[1] > 0) {
+          double c[3] = {t[0][1] * t[1][2] - t[0][2] * t[1][1], t[0][2] * t[1][0] - t[0][0] * t[1][2], t[0][0] * t[1][1] - t[0][1] * t[1][0]};
+          double lc = sqrt(dot3(c, c));
+          if (lc > 1e-3 * ll[0] * ll[1]) {
+            for (int q = 0; q < 3; q++) e[q] = c[q] / lc;
+            kpar = k;
+            kperp = 0;
+            done = true;
+          }
+        }
+        if (!done) {
+          if (ll[0] > 0 || ll[1] > 0) {
+            int f = ll[0] >= ll[1] ? 0 : 1;
+            for (int q = 0; q < 3; q++) e[q] = t[f][q] / ll[f];
+            kpar = 0;
+            kperp = k;
+          } else {
+            e[0] = 1;
+            e[1] = 0;
+            e[2] = 0;
+            kpar = k;
+            kperp = k;
+          }
+        }
+        double D[9];
+        for (int r = 0; r < 3; r++)
+          for (int c = 0; c < 3; c++) D[3 * r + c] = (kpar - kperp) * e[r] * e[c] + (r == c ? kperp : 0);
+        double t1[3], t2[3];
+        for (int r = 0; r < 3; r++) {
+          t1[r] = D[3 * r] * d0[0] + D[3 * r + 1] * d0[1] + D[3 * r + 2] * d0[2];
+          t2[r] = D[3 * r] * dn[0] + D[3 * r + 1] * dn[1] + D[3 * r + 2] * dn[2];
+        }
+        quad(dot3(t1, dn), dot3(dn, t2));
+      }
+    }
+    for (int f = 0; f < 2; f++) {
+      const DevYarn& y = L.y[f];
+      int next = f == 0 ? (i + 1 < nx ? n + 1 : -1) : (j + 1 < ny ? n + nx : -1);
+      int prev = f == 0 ? (i > 0 ? n - 1 : -1) : (j > 0 ? n - nx : -1);
+      int s = 2 * n + f;
+      if (next >= 0 && act(E, s)) {
+        double Lr = E.rest[s], it = E.intact[s];
+        double xb[3], db[3], dd[3], en[3], ec[3], de[3];
+        ld3d(E.x, Np, next, xb);
+        ld3d(E.dx, Np, next, db);
+        dir(next, dd);
+        for (int q = 0; q < 3; q++) {
+          en[q] = xb[q] - xn[q];
+          ec[q] = en[q] + db[q] - d0[q];
+          de[q] = dd[q] - dn[q];
+        }
+        double l0 = sqrt(dot3(ec, ec)), ln = sqrt(dot3(en, en));
+        double T0 = it * axial_tension<double>(y, l0 / Lr - 1);
+        double c = y.visc / (Lr * h);
+        for (int k = 0; k < 4; k++) {
+          double dl, la;
+          len_change(ec, de, al[k], l0, dl, la);
+          double Ta = it * axial_tension<double>(y, la / Lr - 1);
+          add(k, 0.5 * (T0 + Ta) * dl);
+          add(k, 0.5 * c * dl * (la + l0 - 2 * ln));
+        }
+      }
+      if (prev >= 0 && next >= 0 && act(E, 2 * prev + f) && act(E, s)) {
+        int s0 = 2 * prev + f;
+        double wa, wc, kperb, fold;
+        bend_geometry<double>(E.rest[s0], E.rest[s], wa, wc, kperb, fold);
+        double sc = bend_scale<double>(E.intact[s0], E.intact[s], E.cut[s0], E.cut[s], y.B);
+        double kb = y.bend_rig * sc * kperb;
+        double kd = (kb * y.bend_visc) / h;
+        double xa[3], xc[3], da[3], dc[3], dp[3], dq[3], rn[3], r0[3], dr[3];
+        ld3d(E.x, Np, prev, xa);
+        ld3d(E.x, Np, next, xc);
+        ld3d(E.dx, Np, prev, da);
+        ld3d(E.dx, Np, next, dc);
+        dir(prev, dp);
+        dir(next, dq);
+        for (int q = 0; q < 3; q++) {
+          rn[q] = wa * (xa[q] - xn[q]) + wc * (xc[q] - xn[q]);
+          r0[q] = rn[q] + wa * (da[q] - d0[q]) + wc * (dc[q] - d0[q]);
+          dr[q] = wa * dp[q] - 2 * dn[q] + wc * dq[q];
+        }
+        double q0 = dot3(r0, r0), l0 = sqrt(q0), lrn = sqrt(dot3(rn, rn));
+        for (int k = 0; k < 4; k++) {
+          double a = al[k];
+          double dqa = 2 * a * dot3(r0, dr) + a * a * dot3(dr, dr);
+          add(k, 0.5 * kb * bend_psi_delta(q0, q0 + dqa, dqa, fold));
+          double dl, la;
+          len_change(r0, dr, a, l0, dl, la);
+          add(k, 0.5 * kd * dl * (la + l0 - 2 * lrn));
+        }
+      }
+    }
+    if (i + 1 < nx && j + 1 < ny) {
+      int r = n + 1, t = n + nx, rt = r + nx;
+      auto io = [&](int s) { return act(E, s) ? E.intact[s] : 0.0; };
+      double ks = (L.shear_k * (double)E.cn0[n]) / L.cn0;
+      double jam = shear_jam<double>(cell_area<double>(io(2 * n), io(2 * t), io(2 * n + 1), io(2 * r + 1)));
+      double Ld = sqrt(E.rest[2 * n] * E.rest[2 * n] + E.rest[2 * n + 1] * E.rest[2 * n + 1]);
+      bool cu = act(E, 2 * n), cv = act(E, 2 * n + 1), rv = act(E, 2 * r + 1), tu = act(E, 2 * t);
+      for (int dg = 0; dg < 2; dg++) {
+        bool on = dg == 0 ? ((cu && rv) || (cv && tu)) : ((cu && cv) || (rv && tu));
+        if (!on) continue;
+        int p = dg == 0 ? n : r, q2 = dg == 0 ? rt : t;
+        double xp[3], xq[3], ddp[3], ddq[3], vp[3], vq[3], ec[3], de[3];
+        ld3d(E.x, Np, p, xp);
+        ld3d(E.x, Np, q2, xq);
+        ld3d(E.dx, Np, p, ddp);
+        ld3d(E.dx, Np, q2, ddq);
+        dir(p, vp);
+        dir(q2, vq);
+        for (int z = 0; z < 3; z++) {
+          ec[z] = (xq[z] - xp[z]) + (ddq[z] - ddp[z]);
+          de[z] = vq[z] - vp[z];
+        }
+        double l0 = sqrt(dot3(ec, ec));
+        double f0, k0, e0;
+        shear_law<double>(ks, Ld, l0, jam, f0, k0, e0);
+        for (int k = 0; k < 4; k++) {
+          double dl, la, fa, ka, ea;
+          len_change(ec, de, al[k], l0, dl, la);
+          shear_law<double>(ks, Ld, la, jam, fa, ka, ea);
+          add(k, 0.5 * (f0 + fa) * dl);
+        }
+      }
+    }
+  }
+  double tot[8];
+  if (grid_sum<8>(acc, E.partials, &E.sc->ticket, tot)) {
+    Scal* s = E.sc;
+    for (int k = 0; k < 4; k++) {
+      s->e[k] = tot[k];
+      s->m[k] = tot[4 + k];
+    }
+    // chooseStep (newtonMultigrid.ts:597-603)
+    int best = 0;
+    for (int k = 1; k < 4; k++)
+      if (tot[k] < tot[best]) best = k;
+    double alpha;
+    bool all_up = true;
+    for (int k = 0; k < 4; k++)
+      if (!(tot[k] > kLineNoise * tot[4 + k])) all_up = false;
+    if (tot[best] < -kLineNoise * tot[4 + best])
+      alpha = al[best];
+    else if (all_up)
+      alpha = 0;
+    else
+      alpha = 1;
+    if (alpha == 0) s->rejections += 1;
+    s->line_alpha = alpha;
+  }
+}
+
+__global__ void k_set_alpha(Scal* sc, double a) { sc->line_alpha = a; }
+
+__global__ void k_record(Scal* sc, int it) {
+  sc->rec_iters[it] = sc->iters;
+  sc->rec_relres[it] = sc->relres;
+  sc->rec_alpha[it] = sc->line_alpha;
+}
+
+__global__ void k_reset_counters(Scal* sc) {
+  for (int k = 0; k < BD_MAX_NEWTON; k++) {
+    sc->max_update[k] = 0u;
+    sc->rec_iters[k] = 0;
+    sc->rec_relres[k] = 0;
+    sc->rec_alpha[k] = 0;
+  }
+}
+
+__global__ void k_pcg_setup(Scal* sc, double tol, int max_iters, int fixed_iters) {
+  sc->tol = tol;
+  sc->max_iters = max_iters;
+  sc->fixed_iters = fixed_iters;
+  sc->done = 0;
+  sc->ticket = 0u;
+}
+
+PcgArgs pcg_args(Solver& s) {
+  Level& L = s.lv[0];
+  return PcgArgs{s.N, s.Np, L.fixed, s.scal, s.partials, L.b, L.x, s.p, s.ap, s.delta, s.rhs};
+}
+
+int dot_grid(int N) {
+  int g = (N + 255) / 256;
+  return g < kDotBlocks ? g : kDotBlocks;
+}
+
+}  // namespace
+
+// One PCG solve on level 0 (newtonMultigrid.ts:537-576), leaving the direction in delta.
+void pcg_solve(Solver& s) {
+  PcgArgs A = pcg_args(s);
+  const bool tol_mode = s.P.pcg_rel_tol > 0;
+  const int K = tol_mode ? std::max(1, s.P.pcg_max_iterations) : std::max(0, s.P.pcg_iterations);
+  const int G = dot_grid(s.N), B = grid_for(s.N, 256);
+  k_pcg_setup<<<1, 1, 0, s.stream>>>(s.scal, tol_mode ? s.P.pcg_rel_tol : 0.0, K, K);
+  k_pcg_start<<<G, 256, 0, s.stream>>>(A);
+  if (K == 0) return;
+  vcycle(s, 0);
+  k_dot_rz<<<G, 256, 0, s.stream>>>(A, 1);
+  k_update_p<<<B, 256, 0, s.stream>>>(A);
+  for (int it = 0; it < K; it++) {
+    k_matvec_dot_fine<<<G, 256, 0, s.stream>>>(A, s.lv[0].op, s.lv[0].geo);
+    k_update_xr<<<G, 256, 0, s.stream>>>(A);
+    if (it + 1 == K) break;
+    if (tol_mode) {
+      // the device decides convergence; check it before queueing the next V-cycle
+      BD_HIP(hipMemcpyAsync(&s.scal_h->done, &s.scal->done, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+      BD_HIP(hipStreamSynchronize(s.stream));
+      if (s.scal_h->done) break;
+    }
+    vcycle(s, 0);
+    k_dot_rz<<<G, 256, 0, s.stream>>>(A, 0);
+    k_update_p<<<B, 256, 0, s.stream>>>(A);
+  }
+}
+
+void launch_record(Solver& s, int it) { k_record<<<1, 1, 0, s.stream>>>(s.scal, it); }
+
+void launch_reset_counters(Solver& s) { k_reset_counters<<<1, 1, 0, s.stream>>>(s.scal); }
+
+void launch_energy(Solver& s) {
+  if (!s.P.line_search) {
+    k_set_alpha<<<1, 1, 0, s.stream>>>(s.scal, 1.0);
+    return;
+  }
+  EnergyArgs E{s.N, s.Np, s.geo, s.layers, s.P.h, s.x, s.dx, s.pred, s.delta, s.mass, s.cn0, s.pinned, s.rest, s.intact,
+               s.cut_mask, s.flags, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k, s.scal, s.partials};
+  k_energy<<<dot_grid(s.N), 256, 0, s.stream>>>(E);
+}
+
+}  // namespace bd

@@ -1,0 +1,66 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol include/bd.h declares, and
+validates its inputs before touching a device (limits of the reference's GpuSolver constructor,
+gpu/solver.ts:298-363, as error codes)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from bd import capi
+from bd.configs import sheet_scene
+from bd.material import linear_sheet_material
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "bd.h").read_text()
+    return sorted(set(re.findall(r"\b(bd_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = capi.load_library()
+    names = declared_symbols()
+    assert len(names) >= 28
+    for name in names:
+        assert hasattr(lib, name), name
+    assert sorted(capi.EXPORTED) == names
+
+
+def test_struct_sizes_match_header_layout():
+    # bd_yarn: 10 doubles + 2 ints; bd_material: 2 + 2 yarns + 5 doubles
+    assert C.sizeof(capi.bd_yarn) == 88
+    assert C.sizeof(capi.bd_material) == 2 * 8 + 2 * 88 + 5 * 8
+    assert C.sizeof(capi.bd_params) == 8 * 4 + 4 * 2 + 8 + 4 * 2 + 8 + 4 * 6
+
+
+def _create(scene, params=None):
+    lib = capi.load_library()
+    sb = capi.SceneBuffers(scene)
+    h = C.c_void_p()
+    p = params or capi.make_params()
+    rc = lib.bd_create(C.byref(sb.scene), C.byref(p), C.byref(h))
+    return rc, lib.bd_last_error(None).decode(), h
+
+
+def test_create_rejects_lines_beyond_the_limit():
+    mat = linear_sheet_material(1e-3, 2e4, 1.6, 1e-5, 0.35, 8e3)
+    S = sheet_scene(capi.BD_MAX_LINE + 1, 1e-3, mat)
+    rc, msg, _ = _create(S)
+    assert rc == -3 and "BD_MAX_LINE" in msg
+
+
+def test_create_rejects_inconsistent_scene():
+    mat = linear_sheet_material(1e-3, 2e4, 1.6, 1e-5, 0.35, 8e3)
+    S = sheet_scene(8, 1e-3, mat)
+    S.node_count = 63
+    rc, msg, _ = _create(S)
+    assert rc == -1 and "node_count" in msg
+
+
+def test_null_handle_is_an_error():
+    lib = capi.load_library()
+    assert lib.bd_substep(None, 0, 0) == -1
+    assert lib.bd_read(None, 0, 0, None, 0) == -1

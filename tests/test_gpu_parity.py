@@ -1,0 +1,211 @@
+"""GPU vs FP64 oracle, stage by stage, through the C-ABI (mirrors the reference's
+tests/gpu/correctness.ts:165-275 on its worn 48x40 coupon with 3x2 bundles).
+
+Tolerances are the reference's own GPU-vs-f64 bars (relative to each quantity's max magnitude):
+fine operator 5e-5, rhs 1e-3, Galerkin levels 1e-5, one smoother sweep 1e-3, V-cycle 5e-3,
+PCG step 2e-2, line-search energies 1e-2, 60-substep dynamics 0.25 of the motion.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from bd.capi import (make_params, BD_BUF_RHS, BD_BUF_LEVEL_B, BD_BUF_LEVEL_X, BD_BUF_DELTA)
+from bd.material import DENIM
+from bd.scene import compile_scene, coupon
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def corr_coupon(nu=48, nv=40, **kw):  # correctness.ts:26-45
+    spec = dict(bundle=[3, 2], angle=0.3, compressU=0.03, compressV=0.02, shear=0.02, collarRows=2,
+                wear=dict(center=[0, 0], axis=[1, 0], radii=[0.03, 0.03], amount=1, cellSize=0, featureScale=0.01,
+                          defectThreshold=2, defectDensity=0, sparseFrom=2),
+                wearStrengthLoss=[0.45, 0.9], seed=7, perturbation=0.4e-3)
+    spec.update(kw)
+    return coupon(nu, nv, **spec)
+
+
+PARAMS = make_params()  # GPU_DEFAULTS (gpu/solver.ts:61-72), self-contact off
+
+
+def rel_to_scale(a, b, skip_rows=None):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if skip_rows is not None and len(skip_rows):
+        keep = np.ones(a.shape[0], bool)
+        keep[list(skip_rows)] = False
+        a, b = a[keep], b[keep]
+    scale = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / scale) if scale > 0 else float(np.max(np.abs(a)))
+
+
+def kink_rows(cpu, S):
+    """Rows touched by a span whose strain is within 1e-6 of the axial law's kink (correctness.ts:55-68)."""
+    rows = set()
+    x = cpu.x.reshape(-1, 3) + cpu.dx.reshape(-1, 3)
+    L = S.layers[0]
+    for s in np.nonzero((cpu.seg_flags & 3) == 1)[0]:
+        a = s >> 1
+        b = a + 1 if s % 2 == 0 else a + L.nx
+        if abs(np.linalg.norm(x[b] - x[a]) / cpu.rest[s] - 1) < 1e-6:
+            rows.update((a, b))
+    return rows
+
+
+def copy_state(cpu, gpu):
+    """Round the oracle's state to FP32 so both sides start identically (correctness.ts:146-163)."""
+    xs = cpu.x.astype(np.float32).astype(np.float64)
+    vs = cpu.v.astype(np.float32).astype(np.float64)
+    cpu.x[:] = xs
+    cpu.v[:] = vs
+    gpu.set_state(xs, vs)
+    gpu.set_segments(cpu.rest, cpu.intact, cpu.cut_mask, cpu.seg_flags, cpu.fail_site, cpu.tails)
+    gpu.set_field(cpu.damage)
+    return xs, vs
+
+
+@pytest.fixture(scope="module")
+def coupon_pair():
+    from bd.solver import GpuSolver
+    S = compile_scene([corr_coupon()])
+    cpu = O.Oracle(S, PARAMS)
+    for _ in range(12):
+        cpu.substep()
+    say("gpu: create solver on the 48x40 worn coupon")
+    gpu = GpuSolver(S, PARAMS)
+    copy_state(cpu, gpu)
+    return S, cpu, gpu
+
+
+def test_assemble_and_hierarchy(coupon_pair):
+    S, cpu, gpu = coupon_pair
+    copy_state(cpu, gpu)
+    cpu.predict()
+    cpu.assemble()
+    say("gpu: predict + assemble")
+    gpu.encode_predict()
+    gpu.encode_assemble(True)
+    Ag = gpu.stencil(0)
+    Ac = cpu.stencil(0)
+    skip = kink_rows(cpu, S)
+    e = rel_to_scale(Ag.reshape(len(Ag), -1), Ac.reshape(len(Ac), -1), skip)
+    say(f"fine operator maxAbs/scale = {e:.2e}")
+    assert e < 5e-5
+    bg = gpu.read(BD_BUF_RHS)
+    bc = cpu.rhs()
+    e = rel_to_scale(bg, bc)
+    say(f"newton rhs maxAbs/scale = {e:.2e}")
+    assert e < 1e-3
+    assert np.array_equal(gpu.fixed(0), cpu.fixed(0))
+    # Galerkin levels from the GPU fine operator (isolates RAP, correctness.ts:199-206)
+    cpu.load_fine_operator(Ag, gpu.fixed(0))
+    assert cpu.level_count() == len(gpu.stats()["level_nodes"])
+    for k in range(cpu.level_count() - 1):
+        pg = gpu.prolongation(k)
+        _, pc = cpu.prolongation(k)
+        assert np.max(np.abs(pg - pc)) < 1e-6, f"prolongation weights level {k}"
+    for k in range(1, cpu.level_count()):
+        Ak = gpu.stencil(k)
+        Ck = cpu.stencil(k)
+        assert np.array_equal(gpu.fixed(k), cpu.fixed(k)), f"fixed flags level {k}"
+        e = rel_to_scale(Ak.reshape(len(Ak), -1), Ck.reshape(len(Ck), -1))
+        say(f"galerkin level {k} maxAbs/scale = {e:.2e}")
+        assert e < 1e-5
+
+
+def test_smoother_vcycle_pcg(coupon_pair):
+    S, cpu, gpu = coupon_pair
+    copy_state(cpu, gpu)
+    cpu.predict()
+    cpu.assemble()
+    gpu.encode_predict()
+    gpu.encode_assemble(True)
+    cpu.load_fine_operator(gpu.stencil(0), gpu.fixed(0))
+    b = gpu.read(BD_BUF_RHS)
+    N = S.node_count
+    # one forward sweep from zero (correctness.ts:464 tolerance 1e-3)
+    say("gpu: one smoother sweep")
+    gpu.write(BD_BUF_LEVEL_B, b, 0)
+    gpu.write(BD_BUF_LEVEL_X, np.zeros(3 * N), 0)
+    gpu.encode_smooth(0, False)
+    xg = gpu.read(BD_BUF_LEVEL_X, 0)
+    xc = cpu.smooth(0, np.zeros(3 * N), b, False)
+    e = rel_to_scale(xg, xc)
+    say(f"smoother sweep maxAbs/scale = {e:.2e}")
+    assert e < 1e-3
+    say("gpu: V-cycle")
+    gpu.write(BD_BUF_LEVEL_B, b, 0)
+    gpu.encode_vcycle()
+    zg = gpu.read(BD_BUF_LEVEL_X, 0)
+    zc = cpu.vcycle(b)
+    e = rel_to_scale(zg, zc)
+    say(f"V-cycle maxAbs/scale = {e:.2e}")
+    assert e < 5e-3
+    say("gpu: PCG (2 iterations)")
+    gpu.write(BD_BUF_RHS, b)
+    gpu.encode_pcg()
+    dg = gpu.read(BD_BUF_DELTA)
+    dc = cpu.pcg(b, PARAMS.pcg_iterations)
+    e = rel_to_scale(dg, dc)
+    say(f"PCG step maxAbs/scale = {e:.2e}")
+    assert e < 2e-2
+    # line-search energy changes along the GPU direction (correctness.ts:223-231)
+    say("gpu: line search")
+    gpu.encode_line_search()
+    eg, mg = gpu.energy()
+    ec, mc = cpu.energy_change(dg)
+    err = np.max(np.abs(eg - ec)) / np.max(np.abs(ec))
+    say(f"line-search energies gpu {eg} cpu {ec} rel {err:.2e}")
+    assert err < 1e-2
+    assert np.max(np.abs(mg - mc) / np.abs(mc)) < 1e-2
+
+
+def test_pcg_relative_tolerance(coupon_pair):
+    """Tolerance mode: the FP64-accumulated residual the GPU reports is the true residual of its
+    FP32 iterate, recomputed here in FP64 with the oracle's operator."""
+    S, cpu, gpu = coupon_pair
+    copy_state(cpu, gpu)
+    cpu.predict()
+    cpu.assemble()
+    gpu.encode_predict()
+    gpu.encode_assemble(True)
+    cpu.load_fine_operator(gpu.stencil(0), gpu.fixed(0))
+    gpu.set_pcg(2, 1e-6, 200)
+    say("gpu: PCG to rel-res 1e-6")
+    gpu.encode_pcg()
+    st = gpu.stats()
+    d = gpu.read(BD_BUF_DELTA)
+    b = gpu.read(BD_BUF_RHS)
+    fixed = np.repeat(cpu.fixed(0) == 1, 3)
+    b[fixed] = 0
+    r = b - cpu.apply(0, d)
+    r[fixed] = 0
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    say(f"pcg iterations {st['pcg_iterations'][0]} reported {st['rel_residual'][0]:.2e} recomputed {rel:.2e}")
+    gpu.set_pcg(PARAMS.pcg_iterations, 0.0, 200)
+    assert st["rel_residual"][0] <= 1e-6
+    assert rel < 3e-6
+    assert 1 <= st["pcg_iterations"][0] < 200
+
+
+def test_dynamics_60_substeps(coupon_pair):
+    S, cpu, gpu = coupon_pair
+    xs, _ = copy_state(cpu, gpu)
+    steps = 60
+    for _ in range(steps):
+        cpu.substep()
+    say("gpu: 60 substeps")
+    for _ in range(steps):
+        gpu.encode_substep()
+    xg = gpu.read(0)
+    moved = np.max(np.abs(cpu.x - xs))
+    err = np.max(np.abs(xg - cpu.x))
+    say(f"dynamics: max |x_gpu - x_cpu| = {err * 1e6:.2f} um, max motion {moved * 1e6:.1f} um")
+    assert err < 0.25 * moved
