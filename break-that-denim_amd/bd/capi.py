@@ -81,6 +81,11 @@ class bd_stats(C.Structure):
                 ("level_nodes", C.c_int32 * BD_MAX_LEVELS)]
 
 
+class bd_kernel_profile(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_int32), ("pad_", C.c_int32), ("total_ms", C.c_double),
+                ("bytes", C.c_double)]
+
+
 def _ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
@@ -222,6 +227,9 @@ def load_library(path: os.PathLike | None = None):
     lib.bd_buffer_bytes.argtypes = [vp, C.c_int, C.c_int]
     lib.bd_buffer_bytes.restype = C.c_size_t
     lib.bd_get_stats.argtypes = [vp, P(bd_stats)]
+    lib.bd_set_grip_goals.argtypes = [vp, C.c_int, P(C.c_double)]
+    lib.bd_profile_enable.argtypes = [vp, C.c_int]
+    lib.bd_profile_read.argtypes = [vp, P(bd_kernel_profile), C.c_int, P(C.c_int)]
     if path is None:
         _lib = lib
     return lib
@@ -233,5 +241,5 @@ EXPORTED = (
     "bd_set_grips", "bd_set_timestep", "bd_set_history", "bd_set_newton_tolerance", "bd_set_pcg",
     "bd_substep", "bd_predict", "bd_assemble", "bd_vcycle", "bd_smooth", "bd_residual", "bd_pcg",
     "bd_line_search", "bd_newton_apply", "bd_finalize", "bd_damage", "bd_slip", "bd_sync", "bd_read",
-    "bd_write", "bd_buffer_bytes", "bd_get_stats",
+    "bd_write", "bd_buffer_bytes", "bd_get_stats", "bd_set_grip_goals", "bd_profile_enable", "bd_profile_read",
 )

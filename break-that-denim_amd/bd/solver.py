@@ -76,6 +76,22 @@ class GpuSolver:
         self._grips = GripBuffers(grips) if grips else None
         self._check(self.lib.bd_set_grips(self.h, C.byref(self._grips.struct) if self._grips else None), "bd_set_grips")
 
+    def set_grip_goals(self, goals):
+        """Move the current grips' goals (one xyz per grip) without re-uploading the patches."""
+        g = np.ascontiguousarray(goals, dtype=np.float64).reshape(-1, 3)
+        self._check(self.lib.bd_set_grip_goals(self.h, len(g), g.ctypes.data_as(C.POINTER(C.c_double))),
+                    "bd_set_grip_goals")
+
+    def profile_enable(self, on=True):
+        self._check(self.lib.bd_profile_enable(self.h, int(bool(on))), "bd_profile_enable")
+
+    def profile_read(self):
+        """{kernel class: (launches, total ms, algorithmic bytes)} since the last read."""
+        arr = (capi.bd_kernel_profile * 32)()
+        n = C.c_int()
+        self._check(self.lib.bd_profile_read(self.h, arr, 32, C.byref(n)), "bd_profile_read")
+        return {arr[i].name.decode(): (arr[i].launches, arr[i].total_ms, arr[i].bytes) for i in range(n.value)}
+
     def set_timestep(self, h):
         self._check(self.lib.bd_set_timestep(self.h, h), "bd_set_timestep")
 

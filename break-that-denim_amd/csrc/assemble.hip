@@ -430,7 +430,7 @@ __global__ __launch_bounds__(256) void k_predict(int N, int Np, float h, float g
 
 // newtonApply (step.wgsl:97-112 / newtonMultigrid.ts:637-644): dx += alpha * delta with a
 // non-finite firewall, per-node update norm and the iteration's largest update
-__global__ __launch_bounds__(256) void k_newton_apply(int N, int Np, const uint8_t* pinned, const float* delta, float* dx,
+__global__ __launch_bounds__(256) void k_newton_apply(int N, int Np, const uint8_t* pinned, const double* delta, float* dx,
                                                        float* last_update, const Scal* scal, unsigned int* max_update) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= N) return;
@@ -438,16 +438,16 @@ __global__ __launch_bounds__(256) void k_newton_apply(int N, int Np, const uint8
     last_update[n] = 0.f;
     return;
   }
-  float a = (float)scal->line_alpha;
-  float d[3] = {delta[n], delta[Np + n], delta[2 * Np + n]};
+  double a = scal->line_alpha;
+  double d[3] = {delta[n], delta[Np + n], delta[2 * Np + n]};
   bool ok = isfinite(d[0]) && isfinite(d[1]) && isfinite(d[2]) && isfinite(a);
-  float s2 = 0.f;
+  double s2 = 0.0;
   for (int q = 0; q < 3; q++) {
-    float st = ok ? a * d[q] : 0.f;
-    dx[q * Np + n] += st;
+    double st = ok ? a * d[q] : 0.0;
+    dx[q * Np + n] = (float)((double)dx[q * Np + n] + st);
     s2 += st * st;
   }
-  float lu = sqrtf(s2);
+  float lu = (float)sqrt(s2);
   last_update[n] = lu;
   atomicMax(max_update, __float_as_uint(lu));
 }
@@ -467,11 +467,15 @@ __global__ __launch_bounds__(256) void k_commit(int N, int Np, float inv_h, floa
 }  // namespace
 
 void launch_predict(Solver& s) {
+  ProfScope ps(s, PC_PREDICT_COMMIT, s.N * 37.0);
   k_predict<<<grid_for(s.N, 256), 256, 0, s.stream>>>(s.N, s.Np, (float)s.P.h, (float)s.P.gravity[0], (float)s.P.gravity[1],
                                                       (float)s.P.gravity[2], s.pinned, s.v, s.pred, s.dx);
 }
 
 void launch_assemble(Solver& s) {
+  // reads x, dx, pred, mass, cn0, pinned, spans (rest, intact, cut, flags), grip id: 97 B;
+  // writes the 32-plane row and the rhs: 140 B
+  ProfScope ps(s, PC_ASSEMBLE, s.N * 237.0);
   AsmArgs A;
   A.c = NodeCtx{s.x, s.dx, s.pred, s.mass, s.cn0, s.pinned, s.rest, s.intact, s.cut_mask, s.flags, s.Np};
   A.layers = s.layers;
@@ -488,11 +492,13 @@ void launch_assemble(Solver& s) {
 }
 
 void launch_newton_apply(Solver& s, int it) {
+  ProfScope ps(s, PC_PREDICT_COMMIT, s.N * 52.0);
   k_newton_apply<<<grid_for(s.N, 256), 256, 0, s.stream>>>(s.N, s.Np, s.pinned, s.delta, s.dx, s.last_update, s.scal,
                                                            &s.scal->max_update[it]);
 }
 
 void launch_commit(Solver& s) {
+  ProfScope ps(s, PC_PREDICT_COMMIT, s.N * 60.0);
   k_commit<<<grid_for(s.N, 256), 256, 0, s.stream>>>(s.N, s.Np, (float)(1.0 / s.P.h), s.x, s.v, s.dx);
 }
 

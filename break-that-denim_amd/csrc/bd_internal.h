@@ -13,6 +13,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <cstdio>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -68,6 +70,7 @@ struct Scal {
   double tol;
   int done, iters, max_iters, fixed_iters;
   int rejections, pad;
+  int need_true, restart, last, replacements;
   unsigned int ticket;       // last-block-done counter of the current reduction
   unsigned int max_update[BD_MAX_NEWTON];  // float bits
   // per Newton iteration record of the substep
@@ -76,8 +79,31 @@ struct Scal {
   double rec_alpha[BD_MAX_NEWTON];
 };
 
+// kernel classes for bd_profile_read
+enum ProfClass {
+  PC_PREDICT_COMMIT, PC_ASSEMBLE, PC_PROLONG_WEIGHTS, PC_RAP, PC_SMOOTH_FINE, PC_SMOOTH_COARSE, PC_RESIDUAL_FINE,
+  PC_RESIDUAL_COARSE, PC_RESTRICT, PC_PROLONG, PC_MATVEC, PC_PCG_VECTOR, PC_TRUE_RESIDUAL, PC_ENERGY, PC_DAMAGE, PC_SLIP,
+  PC_COUNT
+};
+inline const char* prof_name(int c) {
+  static const char* n[PC_COUNT] = {"predict_commit", "assemble", "prolong_weights", "rap", "smooth_fine", "smooth_coarse",
+                                    "residual_fine", "residual_coarse", "restrict", "prolong", "matvec_dot", "pcg_vector",
+                                    "true_residual", "energy", "damage", "slip"};
+  return n[c];
+}
+
+struct ProfRec {
+  int cls;
+  int ev;  // index of the start event in the pool (stop = ev + 1)
+  double bytes;
+};
+
 struct Solver {
   bd_params P{};
+  // profiling
+  bool prof_on = false;
+  std::vector<hipEvent_t> prof_pool;
+  std::vector<ProfRec> prof_recs;
   int N = 0, Np = 0, C = 0;
   int nl = 0;
   DevLayer layers_h[8];
@@ -87,7 +113,8 @@ struct Solver {
   std::string err;
 
   // node state (3 planes each)
-  float *x = nullptr, *v = nullptr, *dx = nullptr, *pred = nullptr, *delta = nullptr;
+  float *x = nullptr, *v = nullptr, *dx = nullptr, *pred = nullptr;
+  double* delta = nullptr;  // Newton direction accumulated in FP64 (iterative refinement)
   float *rhs = nullptr, *p = nullptr, *ap = nullptr;   // PCG; r and z are level 0's b and x
   float* last_update = nullptr;
   float* mass = nullptr;
@@ -108,6 +135,12 @@ struct Solver {
   float* grip_w = nullptr;     // per node
   double* grip_off = nullptr;  // 3 planes
   double* grip_goal_k = nullptr;  // per grip: goal xyz, k
+  // pinned staging ring for per-substep goal updates (a slot is reused after its copy completed)
+  static constexpr int kGoalSlots = 16;
+  double* grip_goal_h = nullptr;     // kGoalSlots x 4 * BD_MAX_GRIPS
+  hipEvent_t goal_ev[kGoalSlots] = {};
+  bool goal_ev_used[kGoalSlots] = {};
+  int goal_slot = 0;
   // reductions
   double* partials = nullptr;  // kDotBlocks * 8
   Scal* scal = nullptr;
@@ -149,5 +182,26 @@ void launch_slip(Solver& s);
   } while (0)
 
 inline int grid_for(int n, int block) { return (n + block - 1) / block; }
+
+// Brackets the launches in its scope with HIP events while profiling is on.
+struct ProfScope {
+  Solver& s;
+  int rec = -1;
+  ProfScope(Solver& s_, int cls, double bytes) : s(s_) {
+    if (!s.prof_on) return;
+    int ev = (int)s.prof_recs.size() * 2;
+    while ((int)s.prof_pool.size() < ev + 2) {
+      hipEvent_t e;
+      BD_HIP(hipEventCreate(&e));
+      s.prof_pool.push_back(e);
+    }
+    BD_HIP(hipEventRecord(s.prof_pool[ev], s.stream));
+    s.prof_recs.push_back(ProfRec{cls, ev, bytes});
+    rec = (int)s.prof_recs.size() - 1;
+  }
+  ~ProfScope() {
+    if (rec >= 0) (void)hipEventRecord(s.prof_pool[s.prof_recs[rec].ev + 1], s.stream);
+  }
+};
 
 }  // namespace bd

@@ -163,6 +163,9 @@ void destroy_all(Solver& s) {
     dfree(L.lines);
   }
   if (s.scal_h) (void)hipHostFree(s.scal_h);
+  if (s.grip_goal_h) (void)hipHostFree(s.grip_goal_h);
+  for (int q = 0; q < Solver::kGoalSlots; q++)
+    if (s.goal_ev[q]) (void)hipEventDestroy(s.goal_ev[q]);
   if (s.stream) (void)hipStreamDestroy(s.stream);
 }
 
@@ -265,7 +268,7 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
     s.v = dalloc<float>(3 * (size_t)Np);
     s.dx = dalloc<float>(3 * (size_t)Np);
     s.pred = dalloc<float>(3 * (size_t)Np);
-    s.delta = dalloc<float>(3 * (size_t)Np);
+    s.delta = dalloc<double>(3 * (size_t)Np);
     s.rhs = dalloc<float>(3 * (size_t)Np);
     s.p = dalloc<float>(3 * (size_t)Np);
     s.ap = dalloc<float>(3 * (size_t)Np);
@@ -295,6 +298,10 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
     s.counters = dalloc<int>(8);
     BD_HIP(hipHostMalloc((void**)&s.scal_h, sizeof(Scal), hipHostMallocDefault));
     std::memset(s.scal_h, 0, sizeof(Scal));
+    const size_t goal_bytes = sizeof(double) * 4 * BD_MAX_GRIPS * (Solver::kGoalSlots + 1);
+    BD_HIP(hipHostMalloc((void**)&s.grip_goal_h, goal_bytes, hipHostMallocDefault));
+    std::memset(s.grip_goal_h, 0, goal_bytes);
+    for (int q = 0; q < Solver::kGoalSlots; q++) BD_HIP(hipEventCreateWithFlags(&s.goal_ev[q], hipEventDisableTiming));
 
     std::vector<float> f(Np, 0.f);
     for (int n = 0; n < N; n++) f[n] = (float)sc->mass[n];
@@ -419,10 +426,29 @@ int bd_set_grips(bd_solver* h, const bd_grips* g) {
       }
     }
     s.grip_count = count;
+    std::memcpy(s.grip_goal_h + (size_t)Solver::kGoalSlots * 4 * BD_MAX_GRIPS, gk.data(), sizeof(double) * 4 * BD_MAX_GRIPS);
     upload(s, s.grip_of, gof.data(), gof.size());
     upload(s, s.grip_w, w.data(), w.size());
     upload(s, s.grip_off, off.data(), off.size());
     upload(s, s.grip_goal_k, gk.data(), gk.size());
+    return BD_OK;
+  });
+}
+
+int bd_set_grip_goals(bd_solver* h, int count, const double* goals) {
+  return guarded(h, [&](Solver& s) {
+    if (count != s.grip_count || (count > 0 && !goals)) return fail(h, BD_ERR_INVALID, "bd_set_grip_goals: grip count differs from bd_set_grips");
+    const int slot = s.goal_slot;
+    s.goal_slot = (slot + 1) % Solver::kGoalSlots;
+    if (s.goal_ev_used[slot]) BD_HIP(hipEventSynchronize(s.goal_ev[slot]));
+    double* hbuf = s.grip_goal_h + (size_t)slot * 4 * BD_MAX_GRIPS;
+    std::memcpy(hbuf, s.grip_goal_h + (size_t)Solver::kGoalSlots * 4 * BD_MAX_GRIPS, sizeof(double) * 4 * BD_MAX_GRIPS);
+    for (int k = 0; k < count; k++)
+      for (int q = 0; q < 3; q++) hbuf[4 * k + q] = goals[3 * k + q];
+    std::memcpy(s.grip_goal_h + (size_t)Solver::kGoalSlots * 4 * BD_MAX_GRIPS, hbuf, sizeof(double) * 4 * BD_MAX_GRIPS);
+    BD_HIP(hipMemcpyAsync(s.grip_goal_k, hbuf, sizeof(double) * 4 * BD_MAX_GRIPS, hipMemcpyHostToDevice, s.stream));
+    BD_HIP(hipEventRecord(s.goal_ev[slot], s.stream));
+    s.goal_ev_used[slot] = true;
     return BD_OK;
   });
 }
@@ -584,6 +610,41 @@ int bd_substep(bd_solver* h, int min_it, int max_it) {
   });
 }
 
+int bd_profile_enable(bd_solver* h, int on) {
+  return guarded(h, [&](Solver& s) {
+    BD_HIP(hipStreamSynchronize(s.stream));
+    s.prof_on = on != 0;
+    s.prof_recs.clear();
+    return BD_OK;
+  });
+}
+
+int bd_profile_read(bd_solver* h, bd_kernel_profile* out, int max_entries, int* count) {
+  return guarded(h, [&](Solver& s) {
+    if (!out || !count) return fail(h, BD_ERR_INVALID, "bd_profile_read: null output");
+    BD_HIP(hipStreamSynchronize(s.stream));
+    bd_kernel_profile acc[PC_COUNT];
+    std::memset(acc, 0, sizeof(acc));
+    for (const ProfRec& r : s.prof_recs) {
+      float ms = 0.f;
+      BD_HIP(hipEventElapsedTime(&ms, s.prof_pool[r.ev], s.prof_pool[r.ev + 1]));
+      acc[r.cls].launches += 1;
+      acc[r.cls].total_ms += ms;
+      acc[r.cls].bytes += r.bytes;
+    }
+    s.prof_recs.clear();
+    int n = 0;
+    for (int c = 0; c < PC_COUNT && n < max_entries; c++) {
+      if (acc[c].launches == 0) continue;
+      out[n] = acc[c];
+      std::snprintf(out[n].name, sizeof(out[n].name), "%s", prof_name(c));
+      n++;
+    }
+    *count = n;
+    return BD_OK;
+  });
+}
+
 int bd_sync(bd_solver* h) {
   return guarded(h, [&](Solver& s) {
     BD_HIP(hipStreamSynchronize(s.stream));
@@ -691,7 +752,13 @@ int bd_read(bd_solver* h, bd_buffer buf, int level, void* dst, size_t bytes) {
       case BD_BUF_V: download_vec3(s, s.v, s.N, s.Np, d); break;
       case BD_BUF_DX: download_vec3(s, s.dx, s.N, s.Np, d); break;
       case BD_BUF_PRED: download_vec3(s, s.pred, s.N, s.Np, d); break;
-      case BD_BUF_DELTA: download_vec3(s, s.delta, s.N, s.Np, d); break;
+      case BD_BUF_DELTA: {
+        std::vector<double> t(3 * (size_t)s.Np);
+        download(s, t.data(), s.delta, t.size());
+        for (int n = 0; n < s.N; n++)
+          for (int q = 0; q < 3; q++) d[3 * n + q] = t[(size_t)q * s.Np + n];
+        break;
+      }
       case BD_BUF_RHS: download_vec3(s, s.rhs, s.N, s.Np, d); break;
       case BD_BUF_LEVEL_X: download_vec3(s, s.lv[level].x, s.lv[level].N, s.lv[level].Np, d); break;
       case BD_BUF_LEVEL_B: download_vec3(s, s.lv[level].b, s.lv[level].N, s.lv[level].Np, d); break;
@@ -750,7 +817,13 @@ int bd_write(bd_solver* h, bd_buffer buf, int level, const void* src, size_t byt
       case BD_BUF_V: upload_vec3(s, s.v, s.N, s.Np, d); break;
       case BD_BUF_DX: upload_vec3(s, s.dx, s.N, s.Np, d); break;
       case BD_BUF_PRED: upload_vec3(s, s.pred, s.N, s.Np, d); break;
-      case BD_BUF_DELTA: upload_vec3(s, s.delta, s.N, s.Np, d); break;
+      case BD_BUF_DELTA: {
+        std::vector<double> t(3 * (size_t)s.Np, 0.0);
+        for (int n = 0; n < s.N; n++)
+          for (int q = 0; q < 3; q++) t[(size_t)q * s.Np + n] = d[3 * n + q];
+        upload(s, s.delta, t.data(), t.size());
+        break;
+      }
       case BD_BUF_RHS: upload_vec3(s, s.rhs, s.N, s.Np, d); break;
       case BD_BUF_LEVEL_X: upload_vec3(s, s.lv[level].x, s.lv[level].N, s.lv[level].Np, d); break;
       case BD_BUF_LEVEL_B: upload_vec3(s, s.lv[level].b, s.lv[level].N, s.lv[level].Np, d); break;

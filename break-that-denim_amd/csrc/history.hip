@@ -431,6 +431,7 @@ HistArgs hist_args(Solver& s) {
 // updateDamage (solver.ts:659-666): strain snapshot, field update, pull-out, span coupling
 void launch_damage(Solver& s) {
   HistArgs A = hist_args(s);
+  ProfScope ps(s, PC_DAMAGE, s.N * 100.0 + s.C * 24.0);
   k_strain<<<grid_for(2 * s.N, 256), 256, 0, s.stream>>>(A);
   if (s.C > 0) {
     k_sep<<<grid_for(2 * s.C, 256), 256, 0, s.stream>>>(A);
@@ -446,6 +447,7 @@ void launch_damage(Solver& s) {
 // updateSlip (solver.ts:669-683): 2 families x 2 parities
 void launch_slip(Solver& s) {
   HistArgs A = hist_args(s);
+  ProfScope ps(s, PC_SLIP, s.N * 4 * 60.0);
   for (int f = 0; f < 2; f++)
     for (int par = 0; par < 2; par++) k_slip<<<grid_for(s.N, 256), 256, 0, s.stream>>>(A, f, par);
 }

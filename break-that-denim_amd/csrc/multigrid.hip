@@ -595,6 +595,7 @@ void launch_zero(Solver& s, float* p, size_t n) {
 void launch_prolong_weights(Solver& s, int k) {
   Level& L = s.lv[k];
   PwArgs P{lv_args(L), s.lv[k + 1].geo, L.pw, L.fine, s.flags, s.intact};
+  ProfScope ps(s, PC_PROLONG_WEIGHTS, L.N * (L.fine ? 41.0 : 89.0));
   k_prolong_weights<<<grid_for(L.N, 256), 256, 0, s.stream>>>(P);
 }
 
@@ -602,6 +603,7 @@ void launch_rap(Solver& s, int k) {
   Level& L = s.lv[k];
   Level& Cl = s.lv[k + 1];
   RapArgs R{lv_args(L), Cl.geo, Cl.N, Cl.Np, L.pw, Cl.op, Cl.fixed};
+  ProfScope ps(s, PC_RAP, L.N * ((L.fine ? 232.0 : 600.0) + 17.0) + Cl.N * 313.0);
   if (L.fine)
     k_rap<true><<<grid_for(Cl.N, 64), 64, 0, s.stream>>>(R);
   else
@@ -610,6 +612,7 @@ void launch_rap(Solver& s, int k) {
 
 void launch_residual(Solver& s, int k, const float* x, const float* b, float* r) {
   Level& L = s.lv[k];
+  ProfScope ps(s, L.fine ? PC_RESIDUAL_FINE : PC_RESIDUAL_COARSE, L.N * (L.fine ? 268.0 : 636.0));
   if (L.fine)
     k_residual<true><<<grid_for(L.N, 256), 256, 0, s.stream>>>(lv_args(L), x, b, r, pcg_gate(s));
   else
@@ -620,6 +623,7 @@ void launch_restrict(Solver& s, int k) {
   Level& L = s.lv[k];
   Level& Cl = s.lv[k + 1];
   XferArgs X{lv_args(L), Cl.geo, Cl.N, Cl.Np, L.pw, Cl.fixed, L.r, Cl.b, Cl.x, L.x, pcg_gate(s)};
+  ProfScope ps(s, PC_RESTRICT, L.N * 29.0 + Cl.N * 25.0);
   k_restrict<<<grid_for(Cl.N, 256), 256, 0, s.stream>>>(X);
 }
 
@@ -627,6 +631,7 @@ void launch_prolong_add(Solver& s, int k) {
   Level& L = s.lv[k];
   Level& Cl = s.lv[k + 1];
   XferArgs X{lv_args(L), Cl.geo, Cl.N, Cl.Np, L.pw, Cl.fixed, L.r, Cl.b, Cl.x, L.x, pcg_gate(s)};
+  ProfScope ps(s, PC_PROLONG, L.N * 41.0 + Cl.N * 12.0);
   k_prolong_add<<<grid_for(L.N, 256), 256, 0, s.stream>>>(X);
 }
 
@@ -643,6 +648,9 @@ void launch_smooth(Solver& s, int k, int reverse) {
     int block = ((maxlen + 63) / 64) * 64;
     size_t shm = (size_t)27 * block * sizeof(float);
     SmoothArgs S{lv_args(L), L.lines + L.color_start[c], L.x, L.b, pcg_gate(s)};
+    // each node of the colour's lines: its full row (fine 58 floats, coarse 150) + b, x read, x write
+    double nodes = (double)count * (c < 3 ? L.geo.ny[0] : L.geo.nx[0]);
+    ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * (L.fine ? 268.0 : 636.0));
     if (L.fine)
       k_smooth<true><<<count, block, shm, s.stream>>>(S);
     else

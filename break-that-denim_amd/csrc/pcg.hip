@@ -64,7 +64,8 @@ struct PcgArgs {
   const uint8_t* fixed;
   Scal* sc;
   double* partials;
-  float *r, *z, *p, *ap, *delta;
+  float *r, *z, *p, *ap;
+  double* delta;
   const float* rhs;
 };
 
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(256) void k_pcg_start(PcgArgs A) {
       float b = fx ? 0.f : A.rhs[q * Np + n];
       A.r[q * Np + n] = b;
       A.z[q * Np + n] = 0.f;
-      A.delta[q * Np + n] = 0.f;
+      A.delta[q * Np + n] = 0.0;
       A.p[q * Np + n] = 0.f;
       A.ap[q * Np + n] = 0.f;
       v[0] += (double)b * b;
@@ -91,11 +92,15 @@ __global__ __launch_bounds__(256) void k_pcg_start(PcgArgs A) {
     s->iters = 0;
     s->rz = 0;
     s->beta = 0;
+    s->need_true = 0;
+    s->restart = 0;
+    s->last = 0;
+    s->replacements = 0;
     s->done = (s->tol > 0 && !(tot[0] > 0)) ? 1 : 0;
   }
 }
 
-// rz = r . z ; beta = rz / rz_prev (first: beta = 0)
+// rz = r . z ; beta = rz / rz_prev (first, or after a residual replacement: beta = 0)
 __global__ __launch_bounds__(256) void k_dot_rz(PcgArgs A, int first) {
   if (A.sc->done) return;
   double v[1] = {0};
@@ -105,7 +110,8 @@ __global__ __launch_bounds__(256) void k_dot_rz(PcgArgs A, int first) {
   double tot[1];
   if (grid_sum<1>(v, A.partials, &A.sc->ticket, tot)) {
     Scal* s = A.sc;
-    s->beta = first ? 0.0 : tot[0] / s->rz;
+    s->beta = (first || s->restart) ? 0.0 : tot[0] / s->rz;
+    s->restart = 0;
     s->rz = tot[0];
   }
 }
@@ -122,11 +128,12 @@ __global__ __launch_bounds__(256) void k_update_p(PcgArgs A) {
 __global__ __launch_bounds__(256) void k_update_xr(PcgArgs A) {
   if (A.sc->done) return;
   const int Np = A.Np;
-  float alpha = (float)A.sc->alpha;
+  const double alpha_d = A.sc->alpha;
+  const float alpha = (float)alpha_d;
   double v[1] = {0};
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < A.N; n += gridDim.x * blockDim.x)
     for (int q = 0; q < 3; q++) {
-      A.delta[q * Np + n] += alpha * A.p[q * Np + n];
+      A.delta[q * Np + n] += alpha_d * (double)A.p[q * Np + n];
       float r = A.r[q * Np + n] - alpha * A.ap[q * Np + n];
       A.r[q * Np + n] = r;
       A.z[q * Np + n] = 0.f;
@@ -138,8 +145,76 @@ __global__ __launch_bounds__(256) void k_update_xr(PcgArgs A) {
     s->rr = tot[0];
     s->relres = s->bb > 0 ? sqrt(tot[0] / s->bb) : 0.0;
     s->iters += 1;
-    int limit = s->tol > 0 ? s->max_iters : s->fixed_iters;
-    if ((s->tol > 0 && s->relres <= s->tol) || s->iters >= limit) s->done = 1;
+    if (s->tol > 0) {
+      // the recursive FP32 residual drifts from the true one: confirm with an FP64 recomputation
+      s->last = s->iters >= s->max_iters;
+      if (s->relres <= s->tol || s->last) s->need_true = 1;
+    } else if (s->iters >= s->fixed_iters) {
+      s->done = 1;
+    }
+  }
+}
+
+// Residual replacement (tolerance mode): r = b - A d with d in FP64 and FP64 accumulation over the
+// FP32 operator; stop when the true relative residual meets the tolerance, otherwise continue from
+// the true residual with a restarted search direction (iterative refinement of the FP32 inner solve).
+__global__ __launch_bounds__(256) void k_true_residual(PcgArgs A, const float* op, Geo geo) {
+  if (A.sc->done || !A.sc->need_true) return;
+  const int Np = A.Np;
+  double v[1] = {0};
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < A.N; n += gridDim.x * blockDim.x) {
+    double y[3] = {0, 0, 0};
+    if (!A.fixed[n]) {
+      int l = geo_layer(geo, n);
+      int r0 = n - geo.off[l], nx = geo.nx[l], ny = geo.ny[l];
+      int j = r0 / nx, i = r0 - j * nx;
+      const double* d = A.delta;
+      auto acc = [&](int base, int row, int m) {
+        double s0 = op[(base + 0) * Np + row], s1 = op[(base + 1) * Np + row], s2 = op[(base + 2) * Np + row];
+        double s3 = op[(base + 3) * Np + row], s4 = op[(base + 4) * Np + row], s5 = op[(base + 5) * Np + row];
+        double x0 = d[m], x1 = d[Np + m], x2 = d[2 * Np + m];
+        y[0] += s0 * x0 + s1 * x1 + s2 * x2;
+        y[1] += s1 * x0 + s3 * x1 + s4 * x2;
+        y[2] += s2 * x0 + s4 * x1 + s5 * x2;
+      };
+      auto accs = [&](double s, int m) {
+        y[0] += s * d[m];
+        y[1] += s * d[Np + m];
+        y[2] += s * d[2 * Np + m];
+      };
+      acc(F_C, n, n);
+      const bool e = i + 1 < nx, w = i > 0, no = j + 1 < ny, so = j > 0;
+      if (e) acc(F_E, n, n + 1);
+      if (w) acc(F_E, n - 1, n - 1);
+      if (no) acc(F_N, n, n + nx);
+      if (so) acc(F_N, n - nx, n - nx);
+      if (e && no) acc(F_NE, n, n + nx + 1);
+      if (w && so) acc(F_NE, n - nx - 1, n - nx - 1);
+      if (w && no) acc(F_NW, n, n + nx - 1);
+      if (e && so) acc(F_NW, n - nx + 1, n - nx + 1);
+      if (i + 2 < nx) accs(op[F_EE * Np + n], n + 2);
+      if (i >= 2) accs(op[F_EE * Np + n - 2], n - 2);
+      if (j + 2 < ny) accs(op[F_NN * Np + n], n + 2 * nx);
+      if (j >= 2) accs(op[F_NN * Np + n - 2 * nx], n - 2 * nx);
+    }
+    for (int q = 0; q < 3; q++) {
+      double r = A.fixed[n] ? 0.0 : (double)A.rhs[q * Np + n] - y[q];
+      A.r[q * Np + n] = (float)r;
+      v[0] += r * r;
+    }
+  }
+  double tot[1];
+  if (grid_sum<1>(v, A.partials, &A.sc->ticket, tot)) {
+    Scal* s = A.sc;
+    s->rr = tot[0];
+    s->relres = s->bb > 0 ? sqrt(tot[0] / s->bb) : 0.0;
+    s->need_true = 0;
+    s->replacements += 1;
+    if (s->relres <= s->tol || s->last) {
+      s->done = 1;
+    } else {
+      s->restart = 1;
+    }
   }
 }
 
@@ -215,7 +290,8 @@ struct EnergyArgs {
   Geo geo;
   const DevLayer* layers;
   double h;
-  const float *x, *dx, *pred, *d, *mass;
+  const float *x, *dx, *pred, *mass;
+  const double* d;
   const double* cn0;
   const uint8_t* pinned;
   const double *rest, *intact;
@@ -232,6 +308,11 @@ struct EnergyArgs {
 __device__ __forceinline__ bool act(const EnergyArgs& E, int s) { return (E.flags[s] & (SEG_EXISTS | SEG_BROKEN)) == SEG_EXISTS; }
 
 __device__ __forceinline__ void ld3d(const float* p, int Np, int n, double o[3]) {
+  o[0] = p[n];
+  o[1] = p[Np + n];
+  o[2] = p[2 * Np + n];
+}
+__device__ __forceinline__ void ld3d(const double* p, int Np, int n, double o[3]) {
   o[0] = p[n];
   o[1] = p[Np + n];
   o[2] = p[2 * Np + n];
@@ -501,15 +582,36 @@ void pcg_solve(Solver& s) {
   const bool tol_mode = s.P.pcg_rel_tol > 0;
   const int K = tol_mode ? std::max(1, s.P.pcg_max_iterations) : std::max(0, s.P.pcg_iterations);
   const int G = dot_grid(s.N), B = grid_for(s.N, 256);
+  const double N = s.N;
   k_pcg_setup<<<1, 1, 0, s.stream>>>(s.scal, tol_mode ? s.P.pcg_rel_tol : 0.0, K, K);
-  k_pcg_start<<<G, 256, 0, s.stream>>>(A);
+  {
+    ProfScope ps(s, PC_PCG_VECTOR, N * 85.0);
+    k_pcg_start<<<G, 256, 0, s.stream>>>(A);
+  }
   if (K == 0) return;
   vcycle(s, 0);
-  k_dot_rz<<<G, 256, 0, s.stream>>>(A, 1);
-  k_update_p<<<B, 256, 0, s.stream>>>(A);
+  {
+    ProfScope ps(s, PC_PCG_VECTOR, N * 24.0);
+    k_dot_rz<<<G, 256, 0, s.stream>>>(A, 1);
+  }
+  {
+    ProfScope ps(s, PC_PCG_VECTOR, N * 36.0);
+    k_update_p<<<B, 256, 0, s.stream>>>(A);
+  }
   for (int it = 0; it < K; it++) {
-    k_matvec_dot_fine<<<G, 256, 0, s.stream>>>(A, s.lv[0].op, s.lv[0].geo);
-    k_update_xr<<<G, 256, 0, s.stream>>>(A);
+    {
+      // fine row (58 floats) + p read + Ap write
+      ProfScope ps(s, PC_MATVEC, N * 256.0);
+      k_matvec_dot_fine<<<G, 256, 0, s.stream>>>(A, s.lv[0].op, s.lv[0].geo);
+    }
+    {
+      ProfScope ps(s, PC_PCG_VECTOR, N * 108.0);
+      k_update_xr<<<G, 256, 0, s.stream>>>(A);
+    }
+    if (tol_mode) {
+      ProfScope ps(s, PC_TRUE_RESIDUAL, 0.0);
+      k_true_residual<<<G, 256, 0, s.stream>>>(A, s.lv[0].op, s.lv[0].geo);
+    }
     if (it + 1 == K) break;
     if (tol_mode) {
       // the device decides convergence; check it before queueing the next V-cycle
@@ -518,8 +620,14 @@ void pcg_solve(Solver& s) {
       if (s.scal_h->done) break;
     }
     vcycle(s, 0);
-    k_dot_rz<<<G, 256, 0, s.stream>>>(A, 0);
-    k_update_p<<<B, 256, 0, s.stream>>>(A);
+    {
+      ProfScope ps(s, PC_PCG_VECTOR, N * 24.0);
+      k_dot_rz<<<G, 256, 0, s.stream>>>(A, 0);
+    }
+    {
+      ProfScope ps(s, PC_PCG_VECTOR, N * 36.0);
+      k_update_p<<<B, 256, 0, s.stream>>>(A);
+    }
   }
 }
 
@@ -532,8 +640,9 @@ void launch_energy(Solver& s) {
     k_set_alpha<<<1, 1, 0, s.stream>>>(s.scal, 1.0);
     return;
   }
-  EnergyArgs E{s.N, s.Np, s.geo, s.layers, s.P.h, s.x, s.dx, s.pred, s.delta, s.mass, s.cn0, s.pinned, s.rest, s.intact,
+  EnergyArgs E{s.N, s.Np, s.geo, s.layers, s.P.h, s.x, s.dx, s.pred, s.mass, s.delta, s.cn0, s.pinned, s.rest, s.intact,
                s.cut_mask, s.flags, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k, s.scal, s.partials};
+  ProfScope ps(s, PC_ENERGY, s.N * 161.0);
   k_energy<<<dot_grid(s.N), 256, 0, s.stream>>>(E);
 }
 

@@ -192,6 +192,9 @@ int bd_set_segments(bd_solver* s, const double* rest_length, const double* intac
 int bd_set_field(bd_solver* s, const uint8_t* damage);
 /* Grip springs for the following substeps (gripTerms inputs, interact.wgsl:159-181); NULL clears. */
 int bd_set_grips(bd_solver* s, const bd_grips* grips);
+/* Move the goals of the current grips (3 doubles per grip), the per-substep handle update of
+ * Interaction.stepHandles / gripLead (interaction.ts, interact.wgsl:96-157); copies 32 B per grip. */
+int bd_set_grip_goals(bd_solver* s, int grip_count, const double* goals);
 /* setTimestep (gpu/solver.ts:796-799), setHistoryEnabled (:766-768), setNewtonTolerance (:771-774). */
 int bd_set_timestep(bd_solver* s, double h);
 int bd_set_history(bd_solver* s, int on);
@@ -219,6 +222,20 @@ int bd_newton_apply(bd_solver* s, int iteration);
 int bd_finalize(bd_solver* s);
 int bd_damage(bd_solver* s);
 int bd_slip(bd_solver* s);
+
+/* Per-kernel timing (the reference's timestamp-query profiling, tools/gpu/scripts/profile.ts):
+ * while enabled, every launch on the solver's stream is bracketed by HIP events. bd_profile_read
+ * synchronises, sums per kernel class since the last read and clears. `bytes` is the algorithmic
+ * HBM traffic of the launches (minimal unique bytes by the layout's byte model). */
+typedef struct bd_kernel_profile {
+  char name[32];
+  int32_t launches;
+  int32_t pad_;
+  double total_ms;
+  double bytes;
+} bd_kernel_profile;
+int bd_profile_enable(bd_solver* s, int on);
+int bd_profile_read(bd_solver* s, bd_kernel_profile* out, int max_entries, int* count);
 
 int bd_sync(bd_solver* s);
 int bd_read(bd_solver* s, bd_buffer buf, int level, void* dst, size_t bytes);
