@@ -109,6 +109,9 @@ def main():
     prof = gpu.profile_read()
     gpu.profile_enable(False)
     hbm = {k: v for k, v in prof.items() if v[2] > 0}
+    if not hbm:
+        print(json.dumps({"note": "no profiled frames"}), flush=True)
+        return
     dom = max(hbm, key=lambda k: hbm[k][1])
     launches, tot_ms, nbytes = hbm[dom]
     achieved = nbytes / (tot_ms * 1e-3) / 1e9

@@ -1,12 +1,9 @@
 // Galerkin multigrid on the half-stencil operators: connectivity-masked prolongation weights,
-// the Galerkin triple product P^T A P, residual / restriction / prolongation, and the
-// colour-ordered block-tridiagonal yarn-line smoother.
+// the Galerkin triple product P^T A P, residual / restriction / prolongation and the V-cycle
+// (the line smoother is in smooth.hip).
 //
-// Restates buildProlongation / galerkin / vcycle / smoothLines / solvePenta
-// (src/simulation/reference/newtonMultigrid.ts:287-528, 647-733) and lattice.ts:13-41, 86-106.
-// The smoother solves each line exactly like solvePenta's banded elimination, but by odd-even
-// block cyclic reduction with the whole line resident in LDS: one workgroup per line, one row per
-// thread, log2(len) reduction levels, every pivot shifted by PIVOT_SHIFT of its scale.
+// Restates buildProlongation / galerkin / vcycle (src/simulation/reference/newtonMultigrid.ts:
+// 287-387, 495-528) and bracket / coarseToFine (lattice.ts:13-41).
 #include <stdexcept>
 
 #include "bd_internal.h"
@@ -348,234 +345,6 @@ __global__ __launch_bounds__(256) void k_prolong_add(XferArgs X) {
   for (int q = 0; q < 3; q++) X.xf[q * Np + f] += s[q];
 }
 
-// ---------------------------------------------------------------- line smoother
-struct M3 {
-  float m[9];
-};
-
-__device__ __forceinline__ void sym_to_full(const float* s, float* o) {
-  o[0] = s[0]; o[1] = s[1]; o[2] = s[2];
-  o[3] = s[1]; o[4] = s[3]; o[5] = s[4];
-  o[6] = s[2]; o[7] = s[4]; o[8] = s[5];
-}
-
-// inverse of the symmetric block shifted by PIVOT_SHIFT * max|diag| (newtonMultigrid.ts:414-423)
-__device__ __forceinline__ void pivot_inv(const float* s, float* o) {
-  float e = (float)kPivotShift * fmaxf(fabsf(s[0]), fmaxf(fabsf(s[3]), fabsf(s[5])));
-  float a00 = s[0] + e, a01 = s[1], a02 = s[2], a11 = s[3] + e, a12 = s[4], a22 = s[5] + e;
-  float c00 = a11 * a22 - a12 * a12, c01 = a12 * a02 - a01 * a22, c02 = a01 * a12 - a11 * a02;
-  float det = a00 * c00 + a01 * c01 + a02 * c02;
-  float id = 1.f / det;
-  o[0] = c00 * id;
-  o[1] = c01 * id;
-  o[2] = c02 * id;
-  o[3] = (a00 * a22 - a02 * a02) * id;
-  o[4] = (a02 * a01 - a00 * a12) * id;
-  o[5] = (a00 * a11 - a01 * a01) * id;
-}
-
-__device__ __forceinline__ void mm(const float* a, const float* b, float* o) {
-#pragma unroll
-  for (int r = 0; r < 3; r++)
-#pragma unroll
-    for (int c = 0; c < 3; c++) o[r * 3 + c] = a[r * 3] * b[c] + a[r * 3 + 1] * b[3 + c] + a[r * 3 + 2] * b[6 + c];
-}
-// a^T b
-__device__ __forceinline__ void mtm(const float* a, const float* b, float* o) {
-#pragma unroll
-  for (int r = 0; r < 3; r++)
-#pragma unroll
-    for (int c = 0; c < 3; c++) o[r * 3 + c] = a[r] * b[c] + a[3 + r] * b[3 + c] + a[6 + r] * b[6 + c];
-}
-// a b^T
-__device__ __forceinline__ void mmt(const float* a, const float* b, float* o) {
-#pragma unroll
-  for (int r = 0; r < 3; r++)
-#pragma unroll
-    for (int c = 0; c < 3; c++) o[r * 3 + c] = a[r * 3] * b[c * 3] + a[r * 3 + 1] * b[c * 3 + 1] + a[r * 3 + 2] * b[c * 3 + 2];
-}
-__device__ __forceinline__ void mv(const float* a, const float* v, float* o) {
-  o[0] = a[0] * v[0] + a[1] * v[1] + a[2] * v[2];
-  o[1] = a[3] * v[0] + a[4] * v[1] + a[5] * v[2];
-  o[2] = a[6] * v[0] + a[7] * v[1] + a[8] * v[2];
-}
-__device__ __forceinline__ void mtv(const float* a, const float* v, float* o) {
-  o[0] = a[0] * v[0] + a[3] * v[1] + a[6] * v[2];
-  o[1] = a[1] * v[0] + a[4] * v[1] + a[7] * v[2];
-  o[2] = a[2] * v[0] + a[5] * v[1] + a[8] * v[2];
-}
-
-struct SmoothArgs {
-  LvArgs a;
-  const int4* lines;
-  float* x;
-  const float* b;
-  const int* gate;
-};
-
-// one colour of lines: block = one line, thread k = row k (smoothLines, newtonMultigrid.ts:659-733)
-template <bool FINE>
-__global__ __launch_bounds__(1024) void k_smooth(SmoothArgs S) {
-  if (S.gate && *S.gate) return;
-  extern __shared__ float sm[];
-  const int4 ln = S.lines[blockIdx.x];
-  const int len = ln.z, fam = ln.w;
-  const int Lr = blockDim.x;
-  float* sD = sm;           // 6 planes (sym D, later its shifted inverse)
-  float* sU = sD + 6 * Lr;  // 9 planes: coupling to the next kept row
-  float* sL = sU + 9 * Lr;  // 9 planes: coupling to the previous kept row (eliminated rows)
-  float* sR = sL + 9 * Lr;  // 3 planes: rhs, then solution
-  const int k = threadIdx.x;
-  const int Np = S.a.Np;
-  const bool row = k < len;
-  int n = row ? ln.x + k * ln.y : 0;
-  bool fixed_n = row ? S.a.fixed[n] != 0 : true;
-  if (row) {
-    float D[6], U[9], r[3];
-    r[0] = S.b[n];
-    r[1] = S.b[Np + n];
-    r[2] = S.b[2 * Np + n];
-    for (int q = 0; q < 9; q++) U[q] = 0.f;
-    if (fixed_n) {
-      D[0] = 1.f; D[1] = 0.f; D[2] = 0.f; D[3] = 1.f; D[4] = 0.f; D[5] = 1.f;
-    } else {
-      int l, i, j;
-      node_ij(S.a.geo, n, l, i, j);
-      blk_load(S.a.op, Np, 0, n, D);
-      float maj = 0.f;
-      for_couplings<FINE>(S.a.op, Np, n, i, j, S.a.geo.nx[l], S.a.geo.ny[l], [&](int di, int dj, int m, const float* blk) {
-        if (S.a.fixed[m]) return;
-        const int along = fam == 0 ? di : dj;
-        const int across = fam == 0 ? dj : di;
-        if (across == 0 && along == 1) {
-          sym_to_full(blk, U);
-          return;
-        }
-        if (across == 0 && along == -1) return;
-        if (across == 0) maj += sym_frob(blk);  // +-2 along the line, majorised on the diagonal
-        float xm[3] = {S.x[m], S.x[Np + m], S.x[2 * Np + m]}, t[3];
-        sym_mul(blk, xm, t);
-        r[0] -= t[0];
-        r[1] -= t[1];
-        r[2] -= t[2];
-      });
-      D[0] += maj;
-      D[3] += maj;
-      D[5] += maj;
-      r[0] += maj * S.x[n];
-      r[1] += maj * S.x[Np + n];
-      r[2] += maj * S.x[2 * Np + n];
-    }
-    for (int q = 0; q < 6; q++) sD[q * Lr + k] = D[q];
-    for (int q = 0; q < 9; q++) sU[q * Lr + k] = U[q];
-    for (int q = 0; q < 3; q++) sR[q * Lr + k] = r[q];
-  }
-  __syncthreads();
-  // odd-even block cyclic reduction
-  int top = 1;
-  while (2 * top < len) top <<= 1;
-  for (int d = 1; d < len; d <<= 1) {
-    const int d2 = 2 * d;
-    if (row && (k & (d2 - 1)) == d) {  // eliminated at this level
-      float s[6], inv[6];
-      for (int q = 0; q < 6; q++) s[q] = sD[q * Lr + k];
-      pivot_inv(s, inv);
-      for (int q = 0; q < 6; q++) sD[q * Lr + k] = inv[q];
-      // coupling to the previous kept row: transpose of its forward block
-      float Up[9];
-      for (int q = 0; q < 9; q++) Up[q] = sU[q * Lr + k - d];
-      sL[0 * Lr + k] = Up[0]; sL[1 * Lr + k] = Up[3]; sL[2 * Lr + k] = Up[6];
-      sL[3 * Lr + k] = Up[1]; sL[4 * Lr + k] = Up[4]; sL[5 * Lr + k] = Up[7];
-      sL[6 * Lr + k] = Up[2]; sL[7 * Lr + k] = Up[5]; sL[8 * Lr + k] = Up[8];
-    }
-    __syncthreads();
-    if (row && (k & (d2 - 1)) == 0) {  // kept
-      float D[9], R[3], Unew[9];
-      float Ds[6];
-      for (int q = 0; q < 6; q++) Ds[q] = sD[q * Lr + k];
-      sym_to_full(Ds, D);
-      for (int q = 0; q < 3; q++) R[q] = sR[q * Lr + k];
-      for (int q = 0; q < 9; q++) Unew[q] = 0.f;
-      const int j1 = k - d, j2 = k + d;
-      if (j1 >= 0) {
-        float Di[6], Dif[9], U1[9], T[9], W[9], r1[3], y1[3], t[3];
-        for (int q = 0; q < 6; q++) Di[q] = sD[q * Lr + j1];
-        sym_to_full(Di, Dif);
-        for (int q = 0; q < 9; q++) U1[q] = sU[q * Lr + j1];
-        for (int q = 0; q < 3; q++) r1[q] = sR[q * Lr + j1];
-        mm(Dif, U1, T);
-        mtm(U1, T, W);
-        for (int q = 0; q < 9; q++) D[q] -= W[q];
-        mv(Dif, r1, y1);
-        mtv(U1, y1, t);
-        for (int q = 0; q < 3; q++) R[q] -= t[q];
-      }
-      if (j2 < len) {
-        float Di[6], Dif[9], Uk[9], U2[9], T2[9], W[9], r2[3], t[3];
-        for (int q = 0; q < 6; q++) Di[q] = sD[q * Lr + j2];
-        sym_to_full(Di, Dif);
-        for (int q = 0; q < 9; q++) Uk[q] = sU[q * Lr + k];
-        for (int q = 0; q < 9; q++) U2[q] = sU[q * Lr + j2];
-        for (int q = 0; q < 3; q++) r2[q] = sR[q * Lr + j2];
-        mm(Uk, Dif, T2);
-        mmt(T2, Uk, W);
-        for (int q = 0; q < 9; q++) D[q] -= W[q];
-        mv(T2, r2, t);
-        for (int q = 0; q < 3; q++) R[q] -= t[q];
-        mm(T2, U2, W);
-        for (int q = 0; q < 9; q++) Unew[q] = -W[q];
-      }
-      // keep the symmetric part of the Schur complement
-      sD[0 * Lr + k] = D[0];
-      sD[1 * Lr + k] = 0.5f * (D[1] + D[3]);
-      sD[2 * Lr + k] = 0.5f * (D[2] + D[6]);
-      sD[3 * Lr + k] = D[4];
-      sD[4 * Lr + k] = 0.5f * (D[5] + D[7]);
-      sD[5 * Lr + k] = D[8];
-      for (int q = 0; q < 3; q++) sR[q * Lr + k] = R[q];
-      for (int q = 0; q < 9; q++) sU[q * Lr + k] = Unew[q];
-    }
-    __syncthreads();
-  }
-  if (k == 0 && len > 0) {
-    float s[6], inv[6], Dif[9], R[3], x0[3];
-    for (int q = 0; q < 6; q++) s[q] = sD[q * Lr];
-    pivot_inv(s, inv);
-    sym_to_full(inv, Dif);
-    for (int q = 0; q < 3; q++) R[q] = sR[q * Lr];
-    mv(Dif, R, x0);
-    for (int q = 0; q < 3; q++) sR[q * Lr] = x0[q];
-  }
-  __syncthreads();
-  for (int d = top; d >= 1; d >>= 1) {
-    if (row && (k & (2 * d - 1)) == d) {
-      float r[3], xl[3], t[3], Lk[9], Uk[9], Dif[9], Di[6], z[3];
-      for (int q = 0; q < 3; q++) r[q] = sR[q * Lr + k];
-      for (int q = 0; q < 3; q++) xl[q] = sR[q * Lr + k - d];
-      for (int q = 0; q < 9; q++) Lk[q] = sL[q * Lr + k];
-      mv(Lk, xl, t);
-      for (int q = 0; q < 3; q++) r[q] -= t[q];
-      if (k + d < len) {
-        float xr[3];
-        for (int q = 0; q < 3; q++) xr[q] = sR[q * Lr + k + d];
-        for (int q = 0; q < 9; q++) Uk[q] = sU[q * Lr + k];
-        mv(Uk, xr, t);
-        for (int q = 0; q < 3; q++) r[q] -= t[q];
-      }
-      for (int q = 0; q < 6; q++) Di[q] = sD[q * Lr + k];
-      sym_to_full(Di, Dif);
-      mv(Dif, r, z);
-      for (int q = 0; q < 3; q++) sR[q * Lr + k] = z[q];
-    }
-    __syncthreads();
-  }
-  if (row && !fixed_n) {
-    S.x[n] = sR[k];
-    S.x[Np + n] = sR[Lr + k];
-    S.x[2 * Np + n] = sR[2 * Lr + k];
-  }
-}
-
 __global__ void k_zero(float* p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.f;
@@ -633,35 +402,6 @@ void launch_prolong_add(Solver& s, int k) {
   XferArgs X{lv_args(L), Cl.geo, Cl.N, Cl.Np, L.pw, Cl.fixed, L.r, Cl.b, Cl.x, L.x, pcg_gate(s)};
   ProfScope ps(s, PC_PROLONG, L.N * 41.0 + Cl.N * 12.0);
   k_prolong_add<<<grid_for(L.N, 256), 256, 0, s.stream>>>(X);
-}
-
-void launch_smooth(Solver& s, int k, int reverse) {
-  Level& L = s.lv[k];
-  for (int q = 0; q < 6; q++) {
-    int c = reverse ? 5 - q : q;
-    int count = L.color_start[c + 1] - L.color_start[c];
-    if (count == 0) continue;
-    int maxlen = 0;
-    // lines of a colour share one family, so one length bound per colour
-    maxlen = c < 3 ? 0 : 0;
-    for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, c < 3 ? L.geo.ny[l] : L.geo.nx[l]);
-    int block = ((maxlen + 63) / 64) * 64;
-    size_t shm = (size_t)27 * block * sizeof(float);
-    SmoothArgs S{lv_args(L), L.lines + L.color_start[c], L.x, L.b, pcg_gate(s)};
-    // each node of the colour's lines: its full row (fine 58 floats, coarse 150) + b, x read, x write
-    double nodes = (double)count * (c < 3 ? L.geo.ny[0] : L.geo.nx[0]);
-    ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * (L.fine ? 268.0 : 636.0));
-    if (L.fine)
-      k_smooth<true><<<count, block, shm, s.stream>>>(S);
-    else
-      k_smooth<false><<<count, block, shm, s.stream>>>(S);
-  }
-}
-
-void smooth_set_attributes() {
-  int shm = 27 * BD_MAX_LINE * (int)sizeof(float);
-  BD_HIP(hipFuncSetAttribute((const void*)k_smooth<true>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
-  BD_HIP(hipFuncSetAttribute((const void*)k_smooth<false>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
 }
 
 // vcycle (newtonMultigrid.ts:495-528) on level k's x / b buffers, x zero on entry

@@ -1,0 +1,686 @@
+// Colour-ordered block-tridiagonal yarn-line smoother (smoothLines / solvePenta,
+// src/simulation/reference/newtonMultigrid.ts:435-473, 659-733; line table lattice.ts:86-106).
+//
+// One colour pass solves every line of that colour exactly: D_k x_k + U_k x_{k+1} + U_{k-1}^T x_{k-1}
+// = r_k, where D_k is the node's centre block (plus the Frobenius norms of its +-2 along-line
+// couplings, majorised onto the diagonal), U_k its +1 along-line coupling, and r_k its rhs minus
+// every other coupling times the lagged iterate. Lines of one colour share no coupling (3 colours),
+// so reading x in place equals the reference's per-colour snapshot.
+//
+// MI355X mapping: one workgroup per line, two rows per thread. The row gather is branch-free (all
+// ~110 loads of a row issue at once, clamped indices with masks) because a colour pass has only a
+// few hundred lines in flight and is latency-bound. The first odd-even reduction level runs in
+// registers (the thread owns both rows of a pair; neighbours exchange one 9-float term via LDS), the
+// remaining log2(len) - 1 levels by block cyclic reduction over the kept rows in LDS (27 floats per
+// kept row: 55 KB for a 1024-node line, two workgroups per CU). Every pivot is shifted by
+// PIVOT_SHIFT of its largest diagonal entry, as in the reference.
+#include "bd_internal.h"
+
+namespace bd {
+
+namespace {
+
+struct SmoothArgs {
+  const float* op;
+  const uint8_t* fixed;
+  int Np;
+  const int4* lines;  // (start, stride, length, family)
+  float* x;
+  const float* b;
+  const int* gate;
+  Geo geo;
+};
+
+__device__ __forceinline__ void sym_full(const float* s, float* o) {
+  o[0] = s[0]; o[1] = s[1]; o[2] = s[2];
+  o[3] = s[1]; o[4] = s[3]; o[5] = s[4];
+  o[6] = s[2]; o[7] = s[4]; o[8] = s[5];
+}
+
+// inverse of the symmetric block shifted by PIVOT_SHIFT * max|diag| (newtonMultigrid.ts:414-423)
+__device__ __forceinline__ void pivot_inv(const float* s, float* o) {
+  float e = (float)kPivotShift * fmaxf(fabsf(s[0]), fmaxf(fabsf(s[3]), fabsf(s[5])));
+  float a00 = s[0] + e, a01 = s[1], a02 = s[2], a11 = s[3] + e, a12 = s[4], a22 = s[5] + e;
+  float c00 = a11 * a22 - a12 * a12, c01 = a12 * a02 - a01 * a22, c02 = a01 * a12 - a11 * a02;
+  float id = 1.f / (a00 * c00 + a01 * c01 + a02 * c02);
+  o[0] = c00 * id;
+  o[1] = c01 * id;
+  o[2] = c02 * id;
+  o[3] = (a00 * a22 - a02 * a02) * id;
+  o[4] = (a02 * a01 - a00 * a12) * id;
+  o[5] = (a00 * a11 - a01 * a01) * id;
+}
+
+__device__ __forceinline__ void mm(const float* a, const float* b, float* o) {
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) o[r * 3 + c] = a[r * 3] * b[c] + a[r * 3 + 1] * b[3 + c] + a[r * 3 + 2] * b[6 + c];
+}
+__device__ __forceinline__ void mtm(const float* a, const float* b, float* o) {  // a^T b
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) o[r * 3 + c] = a[r] * b[c] + a[3 + r] * b[3 + c] + a[6 + r] * b[6 + c];
+}
+__device__ __forceinline__ void mmt(const float* a, const float* b, float* o) {  // a b^T
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) o[r * 3 + c] = a[r * 3] * b[c * 3] + a[r * 3 + 1] * b[c * 3 + 1] + a[r * 3 + 2] * b[c * 3 + 2];
+}
+__device__ __forceinline__ void mv(const float* a, const float* v, float* o) {
+  o[0] = a[0] * v[0] + a[1] * v[1] + a[2] * v[2];
+  o[1] = a[3] * v[0] + a[4] * v[1] + a[5] * v[2];
+  o[2] = a[6] * v[0] + a[7] * v[1] + a[8] * v[2];
+}
+__device__ __forceinline__ void mtv(const float* a, const float* v, float* o) {
+  o[0] = a[0] * v[0] + a[3] * v[1] + a[6] * v[2];
+  o[1] = a[1] * v[0] + a[4] * v[1] + a[7] * v[2];
+  o[2] = a[2] * v[0] + a[5] * v[1] + a[8] * v[2];
+}
+__device__ __forceinline__ void smv(const float* s, const float* v, float* o) {
+  o[0] = s[0] * v[0] + s[1] * v[1] + s[2] * v[2];
+  o[1] = s[1] * v[0] + s[3] * v[1] + s[4] * v[2];
+  o[2] = s[2] * v[0] + s[4] * v[1] + s[5] * v[2];
+}
+
+// one coupling of the row gather: source row and plane base of its symmetric block (or scalar)
+struct Cpl {
+  int di, dj;   // offset of the neighbour
+  int sdi, sdj; // offset of the row that stores the block (0,0: own forward planes)
+  int base;     // first plane
+  bool scalar;
+};
+
+// fine stencil: 4 forward blocks, their 4 mirrors, 2 forward scalars and their mirrors
+__device__ constexpr Cpl kFine[12] = {
+    {1, 0, 0, 0, F_E, false},   {-1, 0, -1, 0, F_E, false}, {0, 1, 0, 0, F_N, false},    {0, -1, 0, -1, F_N, false},
+    {1, 1, 0, 0, F_NE, false},  {-1, -1, -1, -1, F_NE, false}, {-1, 1, 0, 0, F_NW, false}, {1, -1, 1, -1, F_NW, false},
+    {2, 0, 0, 0, F_EE, true},   {-2, 0, -2, 0, F_EE, true}, {0, 2, 0, 0, F_NN, true},    {0, -2, 0, -2, F_NN, true}};
+
+// Gathers D (6), U (9: +1 along coupling, full), r (3) of row n on a line of family FAM.
+// All loads use clamped in-bounds indices and are masked afterwards, so they issue together.
+template <bool FINE, int FAM>
+__device__ __forceinline__ void gather_row(const SmoothArgs& S, int n, int i, int j, int nx, int ny, float D[6], float U[9],
+                                           float r[3], bool& fixed_n) {
+  const int Np = S.Np;
+  const float* op = S.op;
+  const float* x = S.x;
+  fixed_n = S.fixed[n] != 0;
+  float c[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) c[q] = op[q * Np + n];
+  const float xn0 = x[n], xn1 = x[Np + n], xn2 = x[2 * Np + n];
+  r[0] = S.b[n];
+  r[1] = S.b[Np + n];
+  r[2] = S.b[2 * Np + n];
+#pragma unroll
+  for (int q = 0; q < 9; q++) U[q] = 0.f;
+  float maj = 0.f;
+  auto one = [&](int di, int dj, int sdi, int sdj, int base, bool scalar, bool neg_src) {
+    const bool in = i + di >= 0 && i + di < nx && j + dj >= 0 && j + dj < ny;
+    const int m = in ? n + dj * nx + di : n;
+    const int src = in ? n + sdj * nx + sdi : n;
+    float blk[6];
+    if (scalar) {
+      float s = op[base * Np + src];
+      blk[0] = s; blk[1] = 0.f; blk[2] = 0.f; blk[3] = s; blk[4] = 0.f; blk[5] = s;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 6; q++) blk[q] = op[(base + q) * Np + src];
+    }
+    const float xm[3] = {x[m], x[Np + m], x[2 * Np + m]};
+    const bool use = in && S.fixed[m] == 0;
+    const int along = FAM == 0 ? di : dj;
+    const int across = FAM == 0 ? dj : di;
+    if (across == 0 && along == 1) {
+      if (use) sym_full(blk, U);
+    } else if (!(across == 0 && along == -1)) {
+      float t[3];
+      smv(blk, xm, t);
+      const float w = use ? 1.f : 0.f;
+      r[0] -= w * t[0];
+      r[1] -= w * t[1];
+      r[2] -= w * t[2];
+      if (across == 0) maj += w * sym_frob(blk);  // +-2 along the line
+    }
+  };
+  if (FINE) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) one(kFine[k].di, kFine[k].dj, kFine[k].sdi, kFine[k].sdj, kFine[k].base, kFine[k].scalar, false);
+  } else {
+    // 24 couplings of 6-plane blocks: batches of 6 keep the loads in flight within the VGPR budget
+#pragma unroll 3
+    for (int k = 1; k < kCoarseBlocks; k++) {
+      one(coarse_di(k), coarse_dj(k), 0, 0, 6 * k, false, false);
+      one(-coarse_di(k), -coarse_dj(k), -coarse_di(k), -coarse_dj(k), 6 * k, false, true);
+    }
+  }
+  if (fixed_n) {
+    D[0] = 1.f; D[1] = 0.f; D[2] = 0.f; D[3] = 1.f; D[4] = 0.f; D[5] = 1.f;
+#pragma unroll
+    for (int q = 0; q < 9; q++) U[q] = 0.f;
+    r[0] = S.b[n];
+    r[1] = S.b[Np + n];
+    r[2] = S.b[2 * Np + n];
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < 6; q++) D[q] = c[q];
+  D[0] += maj;
+  D[3] += maj;
+  D[5] += maj;
+  r[0] += maj * xn0;
+  r[1] += maj * xn1;
+  r[2] += maj * xn2;
+}
+
+// Partial row gather of one (row, g) task: the couplings c with c % G == g. Writes the partial
+// rhs correction and majorisation to LDS; the task owning the +1 along-line coupling writes U.
+template <bool FINE, int FAM, int G>
+__device__ __forceinline__ void gather_task(const SmoothArgs& S, int n, int i, int j, int nx, int ny, int g, float* pr, float* pm,
+                                            float* gU, int row, int len) {
+  const int Np = S.Np;
+  const float* op = S.op;
+  const float* x = S.x;
+  float r0 = 0.f, r1 = 0.f, r2 = 0.f, maj = 0.f;
+  auto one = [&](int di, int dj, int sdi, int sdj, int base, bool scalar) {
+    const bool in = i + di >= 0 && i + di < nx && j + dj >= 0 && j + dj < ny;
+    const int m = in ? n + dj * nx + di : n;
+    const int src = in ? n + sdj * nx + sdi : n;
+    float blk[6];
+    if (scalar) {
+      float s = op[base * Np + src];
+      blk[0] = s; blk[1] = 0.f; blk[2] = 0.f; blk[3] = s; blk[4] = 0.f; blk[5] = s;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 6; q++) blk[q] = op[(base + q) * Np + src];
+    }
+    const float xm[3] = {x[m], x[Np + m], x[2 * Np + m]};
+    const bool use = in && S.fixed[m] == 0;
+    const int along = FAM == 0 ? di : dj;
+    const int across = FAM == 0 ? dj : di;
+    if (across == 0 && along == 1) {
+      float full[9];
+      sym_full(blk, full);
+#pragma unroll
+      for (int q = 0; q < 9; q++) gU[q * len + row] = use ? full[q] : 0.f;
+    } else if (!(across == 0 && along == -1)) {
+      float t[3];
+      smv(blk, xm, t);
+      const float w = use ? 1.f : 0.f;
+      r0 += w * t[0];
+      r1 += w * t[1];
+      r2 += w * t[2];
+      if (across == 0) maj += w * sym_frob(blk);
+    }
+  };
+  if (FINE) {
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+      if (k % G == g) one(kFine[k].di, kFine[k].dj, kFine[k].sdi, kFine[k].sdj, kFine[k].base, kFine[k].scalar);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 24; k++) {
+      if (k % G != g) continue;
+      const int b = 1 + (k >> 1);
+      if (k & 1)
+        one(-coarse_di(b), -coarse_dj(b), -coarse_di(b), -coarse_dj(b), 6 * b, false);
+      else
+        one(coarse_di(b), coarse_dj(b), 0, 0, 6 * b, false);
+    }
+  }
+  pr[(0 * G + g) * len + row] = r0;
+  pr[(1 * G + g) * len + row] = r1;
+  pr[(2 * G + g) * len + row] = r2;
+  pm[g * len + row] = maj;
+}
+
+// Block tridiagonal solve of one line by two-rows-per-thread odd-even cyclic reduction.
+// Thread t holds rows a = 2t and b = 2t+1 (D sym, U full coupling to the next row, r) and
+// receives their solutions. cr: 27 * Lr floats of LDS. One barrier per reduction level: a kept
+// row recomputes its eliminated neighbours' shifted pivots itself.
+__device__ __forceinline__ void line_solve(float* cr, int Lr, int t, int len, const float* Da_in, const float* Ua,
+                                           const float* ra_in, const float* Db, const float* Ub, const float* rb,
+                                           float xa[3], float xb[3]) {
+  float* sD = cr;           // 6 planes
+  float* sU = sD + 6 * Lr;  // 9 planes
+  float* sL = sU + 9 * Lr;  // 9 planes
+  float* sR = sL + 9 * Lr;  // 3 planes
+  const int R2 = (len + 1) >> 1;
+  const bool has_a = 2 * t < len, has_b = 2 * t + 1 < len;
+  float Da[6], ra[3];
+#pragma unroll
+  for (int q = 0; q < 6; q++) Da[q] = Da_in[q];
+#pragma unroll
+  for (int q = 0; q < 3; q++) ra[q] = ra_in[q];
+
+  // register level: eliminate row b (U_a couples a -> b, U_b couples b -> 2t+2)
+  float Dib[6], Dibf[9], yb[3];
+  pivot_inv(Db, Dib);
+  sym_full(Dib, Dibf);
+  smv(Dib, rb, yb);
+  {
+    float T[9], P[9], p[3];
+    mm(Dibf, Ub, T);
+    mtm(Ub, T, P);
+    mtv(Ub, yb, p);
+    if (has_b && t + 1 < R2) {
+      sL[0 * Lr + t] = P[0];
+      sL[1 * Lr + t] = 0.5f * (P[1] + P[3]);
+      sL[2 * Lr + t] = 0.5f * (P[2] + P[6]);
+      sL[3 * Lr + t] = P[4];
+      sL[4 * Lr + t] = 0.5f * (P[5] + P[7]);
+      sL[5 * Lr + t] = P[8];
+      sL[6 * Lr + t] = p[0];
+      sL[7 * Lr + t] = p[1];
+      sL[8 * Lr + t] = p[2];
+    }
+  }
+  float Unew[9];
+  {
+    float T2[9], W[9], t3[3];
+    mm(Ua, Dibf, T2);
+    mmt(T2, Ua, W);
+    Da[0] -= W[0];
+    Da[1] -= 0.5f * (W[1] + W[3]);
+    Da[2] -= 0.5f * (W[2] + W[6]);
+    Da[3] -= W[4];
+    Da[4] -= 0.5f * (W[5] + W[7]);
+    Da[5] -= W[8];
+    mv(T2, rb, t3);
+    ra[0] -= t3[0];
+    ra[1] -= t3[1];
+    ra[2] -= t3[2];
+    mm(T2, Ub, W);
+#pragma unroll
+    for (int q = 0; q < 9; q++) Unew[q] = -W[q];
+  }
+  __syncthreads();
+  if (has_a && t >= 1) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) Da[q] -= sL[q * Lr + t - 1];
+#pragma unroll
+    for (int q = 0; q < 3; q++) ra[q] -= sL[(6 + q) * Lr + t - 1];
+  }
+  __syncthreads();
+  if (t < R2) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) sD[q * Lr + t] = Da[q];
+#pragma unroll
+    for (int q = 0; q < 9; q++) sU[q * Lr + t] = Unew[q];
+#pragma unroll
+    for (int q = 0; q < 3; q++) sR[q * Lr + t] = ra[q];
+  }
+  __syncthreads();
+
+  const int k = t;
+  const bool row = k < R2;
+  int top = 1;
+  while (2 * top < R2) top <<= 1;
+  for (int d = 1; d < R2; d <<= 1) {
+    if (row && (k & (2 * d - 1)) == 0) {
+      float D[9], Ds[6], R[3], Un[9];
+#pragma unroll
+      for (int q = 0; q < 6; q++) Ds[q] = sD[q * Lr + k];
+      sym_full(Ds, D);
+#pragma unroll
+      for (int q = 0; q < 3; q++) R[q] = sR[q * Lr + k];
+#pragma unroll
+      for (int q = 0; q < 9; q++) Un[q] = 0.f;
+      const int j1 = k - d, j2 = k + d;
+      if (j1 >= 0) {
+        float Ds1[6], Di[6], Dif[9], U1[9], T[9], W[9], r1[3], y1[3], tt[3];
+#pragma unroll
+        for (int q = 0; q < 6; q++) Ds1[q] = sD[q * Lr + j1];
+        pivot_inv(Ds1, Di);
+        sym_full(Di, Dif);
+#pragma unroll
+        for (int q = 0; q < 9; q++) U1[q] = sU[q * Lr + j1];
+#pragma unroll
+        for (int q = 0; q < 3; q++) r1[q] = sR[q * Lr + j1];
+        mm(Dif, U1, T);
+        mtm(U1, T, W);
+#pragma unroll
+        for (int q = 0; q < 9; q++) D[q] -= W[q];
+        mv(Dif, r1, y1);
+        mtv(U1, y1, tt);
+#pragma unroll
+        for (int q = 0; q < 3; q++) R[q] -= tt[q];
+      }
+      if (j2 < R2) {
+        float Ds2[6], Di[6], Dif[9], Uk[9], U2[9], T2[9], W[9], r2[3], tt[3];
+#pragma unroll
+        for (int q = 0; q < 6; q++) Ds2[q] = sD[q * Lr + j2];
+        pivot_inv(Ds2, Di);
+        sym_full(Di, Dif);
+#pragma unroll
+        for (int q = 0; q < 9; q++) Uk[q] = sU[q * Lr + k];
+#pragma unroll
+        for (int q = 0; q < 9; q++) U2[q] = sU[q * Lr + j2];
+#pragma unroll
+        for (int q = 0; q < 3; q++) r2[q] = sR[q * Lr + j2];
+        // the eliminated row's coupling to this row, kept for its back-substitution
+        sL[0 * Lr + j2] = Uk[0]; sL[1 * Lr + j2] = Uk[3]; sL[2 * Lr + j2] = Uk[6];
+        sL[3 * Lr + j2] = Uk[1]; sL[4 * Lr + j2] = Uk[4]; sL[5 * Lr + j2] = Uk[7];
+        sL[6 * Lr + j2] = Uk[2]; sL[7 * Lr + j2] = Uk[5]; sL[8 * Lr + j2] = Uk[8];
+        mm(Uk, Dif, T2);
+        mmt(T2, Uk, W);
+#pragma unroll
+        for (int q = 0; q < 9; q++) D[q] -= W[q];
+        mv(T2, r2, tt);
+#pragma unroll
+        for (int q = 0; q < 3; q++) R[q] -= tt[q];
+        mm(T2, U2, W);
+#pragma unroll
+        for (int q = 0; q < 9; q++) Un[q] = -W[q];
+      }
+      sD[0 * Lr + k] = D[0];
+      sD[1 * Lr + k] = 0.5f * (D[1] + D[3]);
+      sD[2 * Lr + k] = 0.5f * (D[2] + D[6]);
+      sD[3 * Lr + k] = D[4];
+      sD[4 * Lr + k] = 0.5f * (D[5] + D[7]);
+      sD[5 * Lr + k] = D[8];
+#pragma unroll
+      for (int q = 0; q < 3; q++) sR[q * Lr + k] = R[q];
+#pragma unroll
+      for (int q = 0; q < 9; q++) sU[q * Lr + k] = Un[q];
+    }
+    __syncthreads();
+  }
+  if (k == 0) {
+    float s[6], inv[6], R[3], x0[3];
+#pragma unroll
+    for (int q = 0; q < 6; q++) s[q] = sD[q * Lr];
+    pivot_inv(s, inv);
+#pragma unroll
+    for (int q = 0; q < 3; q++) R[q] = sR[q * Lr];
+    smv(inv, R, x0);
+#pragma unroll
+    for (int q = 0; q < 3; q++) sR[q * Lr] = x0[q];
+  }
+  __syncthreads();
+  for (int d = top; d >= 1; d >>= 1) {
+    if (row && (k & (2 * d - 1)) == d) {
+      float rr[3], xl[3], tt[3], Lk[9], Uk[9], Ds[6], Di[6], z[3];
+#pragma unroll
+      for (int q = 0; q < 3; q++) rr[q] = sR[q * Lr + k];
+#pragma unroll
+      for (int q = 0; q < 3; q++) xl[q] = sR[q * Lr + k - d];
+#pragma unroll
+      for (int q = 0; q < 9; q++) Lk[q] = sL[q * Lr + k];
+      mv(Lk, xl, tt);
+#pragma unroll
+      for (int q = 0; q < 3; q++) rr[q] -= tt[q];
+      if (k + d < R2) {
+        float xr[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) xr[q] = sR[q * Lr + k + d];
+#pragma unroll
+        for (int q = 0; q < 9; q++) Uk[q] = sU[q * Lr + k];
+        mv(Uk, xr, tt);
+#pragma unroll
+        for (int q = 0; q < 3; q++) rr[q] -= tt[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 6; q++) Ds[q] = sD[q * Lr + k];
+      pivot_inv(Ds, Di);
+      smv(Di, rr, z);
+#pragma unroll
+      for (int q = 0; q < 3; q++) sR[q * Lr + k] = z[q];
+    }
+    __syncthreads();
+  }
+  float xn[3] = {0.f, 0.f, 0.f};
+  xa[0] = xa[1] = xa[2] = 0.f;
+  if (t < R2) {
+    xa[0] = sR[t];
+    xa[1] = sR[Lr + t];
+    xa[2] = sR[2 * Lr + t];
+  }
+  if (t + 1 < R2) {
+    xn[0] = sR[t + 1];
+    xn[1] = sR[Lr + t + 1];
+    xn[2] = sR[2 * Lr + t + 1];
+  }
+  float tt[3], q1[3];
+  mtv(Ua, xa, tt);
+  mv(Ub, xn, q1);
+  float rr[3] = {rb[0] - tt[0] - q1[0], rb[1] - tt[1] - q1[1], rb[2] - tt[2] - q1[2]};
+  smv(Dib, rr, xb);
+}
+
+__device__ __forceinline__ void set_identity(float D[6], float U[9], float r[3]) {
+  D[0] = 1.f; D[1] = 0.f; D[2] = 0.f; D[3] = 1.f; D[4] = 0.f; D[5] = 1.f;
+#pragma unroll
+  for (int q = 0; q < 9; q++) U[q] = 0.f;
+  r[0] = r[1] = r[2] = 0.f;
+}
+
+struct LineGeo {
+  int4 ln;
+  int nx, ny, i0, j0;
+};
+
+__device__ __forceinline__ LineGeo line_geo(const SmoothArgs& S) {
+  LineGeo g;
+  g.ln = S.lines[xcd_block(blockIdx.x, gridDim.x)];
+  int l = geo_layer(S.geo, g.ln.x);
+  g.nx = S.geo.nx[l];
+  g.ny = S.geo.ny[l];
+  int r0 = g.ln.x - S.geo.off[l];
+  g.j0 = r0 / g.nx;
+  g.i0 = r0 - g.j0 * g.nx;
+  return g;
+}
+
+// Long lines: each thread gathers its two rows straight into registers.
+template <bool FINE, int FAM>
+__global__ __launch_bounds__(512, 4) void k_smooth_direct(SmoothArgs S) {
+  if (S.gate && *S.gate) return;
+  extern __shared__ float sm[];
+  const LineGeo L = line_geo(S);
+  const int len = L.ln.z;
+  const int t = threadIdx.x;
+  const int ka = 2 * t, kb = 2 * t + 1;
+  const int na = L.ln.x + ka * L.ln.y, nb = L.ln.x + kb * L.ln.y;
+  float Da[6], Ua[9], ra[3], Db[6], Ub[9], rb[3];
+  bool fa = true, fb = true;
+  if (ka < len)
+    gather_row<FINE, FAM>(S, na, FAM == 0 ? L.i0 + ka : L.i0, FAM == 0 ? L.j0 : L.j0 + ka, L.nx, L.ny, Da, Ua, ra, fa);
+  else
+    set_identity(Da, Ua, ra);
+  if (kb < len)
+    gather_row<FINE, FAM>(S, nb, FAM == 0 ? L.i0 + kb : L.i0, FAM == 0 ? L.j0 : L.j0 + kb, L.nx, L.ny, Db, Ub, rb, fb);
+  else
+    set_identity(Db, Ub, rb);
+  float xa[3], xb[3];
+  line_solve(sm, blockDim.x, t, len, Da, Ua, ra, Db, Ub, rb, xa, xb);
+  const int Np = S.Np;
+  if (ka < len && !fa) {
+    S.x[na] = xa[0];
+    S.x[Np + na] = xa[1];
+    S.x[2 * Np + na] = xa[2];
+  }
+  if (kb < len && !fb) {
+    S.x[nb] = xb[0];
+    S.x[Np + nb] = xb[1];
+    S.x[2 * Np + nb] = xb[2];
+  }
+}
+
+// Short lines: every row's couplings are split over G threads so that each thread issues about
+// one round trip of loads; rows are assembled in LDS, then solved by the first ceil(len/2) threads.
+template <bool FINE, int FAM, int G>
+__global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S) {
+  if (S.gate && *S.gate) return;
+  extern __shared__ float sm[];
+  const LineGeo L = line_geo(S);
+  const int len = L.ln.z;
+  const int R2 = (len + 1) >> 1;
+  const int Lr = ((R2 + 63) / 64) * 64;
+  float* cr = sm;              // 27 * Lr: reduction workspace
+  float* gD = cr + 27 * Lr;    // 6 * len: centre blocks, then D
+  float* gU = gD + 6 * len;    // 9 * len
+  float* gR = gU + 9 * len;    // 3 * len: b, then r
+  float* gX = gR + 3 * len;    // 3 * len: own x
+  float* pr = gX + 3 * len;    // 3 * G * len
+  float* pm = pr + 3 * G * len;  // G * len
+  uint8_t* gF = reinterpret_cast<uint8_t*>(pm + G * len);
+  const int T = blockDim.x;
+  const int Np = S.Np;
+  for (int task = threadIdx.x; task < len * G; task += T) {
+    const int row = task / G, g = task - row * G;
+    const int n = L.ln.x + row * L.ln.y;
+    const int i = FAM == 0 ? L.i0 + row : L.i0, j = FAM == 0 ? L.j0 : L.j0 + row;
+    gather_task<FINE, FAM, G>(S, n, i, j, L.nx, L.ny, g, pr, pm, gU, row, len);
+    if (g == 0) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) gD[q * len + row] = S.op[q * Np + n];
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        gR[q * len + row] = S.b[q * Np + n];
+        gX[q * len + row] = S.x[q * Np + n];
+      }
+      gF[row] = S.fixed[n];
+    }
+  }
+  __syncthreads();
+  for (int row = threadIdx.x; row < len; row += T) {
+    if (gF[row]) {
+      gD[0 * len + row] = 1.f; gD[1 * len + row] = 0.f; gD[2 * len + row] = 0.f;
+      gD[3 * len + row] = 1.f; gD[4 * len + row] = 0.f; gD[5 * len + row] = 1.f;
+#pragma unroll
+      for (int q = 0; q < 9; q++) gU[q * len + row] = 0.f;
+      continue;
+    }
+    float maj = 0.f, s0 = 0.f, s1 = 0.f, s2 = 0.f;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      maj += pm[g * len + row];
+      s0 += pr[(0 * G + g) * len + row];
+      s1 += pr[(1 * G + g) * len + row];
+      s2 += pr[(2 * G + g) * len + row];
+    }
+    gD[0 * len + row] += maj;
+    gD[3 * len + row] += maj;
+    gD[5 * len + row] += maj;
+    gR[0 * len + row] += maj * gX[0 * len + row] - s0;
+    gR[1 * len + row] += maj * gX[1 * len + row] - s1;
+    gR[2 * len + row] += maj * gX[2 * len + row] - s2;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t >= Lr) return;  // the solve uses the first Lr threads only (no barrier below involves the rest)
+  float Da[6], Ua[9], ra[3], Db[6], Ub[9], rb[3];
+  const int ka = 2 * t, kb = 2 * t + 1;
+  if (ka < len) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) Da[q] = gD[q * len + ka];
+#pragma unroll
+    for (int q = 0; q < 9; q++) Ua[q] = gU[q * len + ka];
+#pragma unroll
+    for (int q = 0; q < 3; q++) ra[q] = gR[q * len + ka];
+  } else {
+    set_identity(Da, Ua, ra);
+  }
+  if (kb < len) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) Db[q] = gD[q * len + kb];
+#pragma unroll
+    for (int q = 0; q < 9; q++) Ub[q] = gU[q * len + kb];
+#pragma unroll
+    for (int q = 0; q < 3; q++) rb[q] = gR[q * len + kb];
+  } else {
+    set_identity(Db, Ub, rb);
+  }
+  float xa[3], xb[3];
+  line_solve(cr, Lr, t, len, Da, Ua, ra, Db, Ub, rb, xa, xb);
+  if (ka < len && !gF[ka]) {
+    const int n = L.ln.x + ka * L.ln.y;
+    S.x[n] = xa[0];
+    S.x[Np + n] = xa[1];
+    S.x[2 * Np + n] = xa[2];
+  }
+  if (kb < len && !gF[kb]) {
+    const int n = L.ln.x + kb * L.ln.y;
+    S.x[n] = xb[0];
+    S.x[Np + n] = xb[1];
+    S.x[2 * Np + n] = xb[2];
+  }
+}
+
+// rows per staged line up to which the staged kernel is used, and its gather width
+constexpr int kStagedMaxLen = 257;
+
+template <bool FINE, int FAM, int G>
+size_t staged_shm(int len) {
+  int R2 = (len + 1) / 2;
+  int Lr = ((R2 + 63) / 64) * 64;
+  return sizeof(float) * ((size_t)27 * Lr + (size_t)(6 + 9 + 3 + 3 + 4 * G) * len) + (size_t)len + 16;
+}
+
+template <bool FINE, int FAM>
+void launch_one(const SmoothArgs& S, int count, int len, hipStream_t st) {
+  if (len <= kStagedMaxLen) {
+    // widest gather that fits 1024 threads: couplings per thread ~ (FINE ? 12 : 24) / G
+    const int G = len <= 128 ? 8 : 4;
+    int threads = std::min(1024, ((len * G + 63) / 64) * 64);
+    const int R2 = (len + 1) / 2;
+    threads = std::max(threads, ((R2 + 63) / 64) * 64);
+    if (G == 8)
+      k_smooth_staged<FINE, FAM, 8><<<count, threads, staged_shm<FINE, FAM, 8>(len), st>>>(S);
+    else
+      k_smooth_staged<FINE, FAM, 4><<<count, threads, staged_shm<FINE, FAM, 4>(len), st>>>(S);
+    return;
+  }
+  int R2 = (len + 1) / 2;
+  int block = ((R2 + 63) / 64) * 64;
+  size_t shm = (size_t)27 * block * sizeof(float);
+  k_smooth_direct<FINE, FAM><<<count, block, shm, st>>>(S);
+}
+
+}  // namespace
+
+void smooth_set_attributes() {
+  int shm = 27 * ((BD_MAX_LINE / 2 + 63) / 64 * 64) * (int)sizeof(float);
+  int shm_staged = (int)staged_shm<false, 0, 8>(128);
+  shm_staged = std::max(shm_staged, (int)staged_shm<false, 0, 4>(kStagedMaxLen));
+  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_direct<true, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_direct<true, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_direct<false, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_direct<false, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+#define BD_STAGED_ATTR(F, A, G) \
+  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_staged<F, A, G>, hipFuncAttributeMaxDynamicSharedMemorySize, shm_staged))
+  BD_STAGED_ATTR(true, 0, 4); BD_STAGED_ATTR(true, 1, 4); BD_STAGED_ATTR(true, 0, 8); BD_STAGED_ATTR(true, 1, 8);
+  BD_STAGED_ATTR(false, 0, 4); BD_STAGED_ATTR(false, 1, 4); BD_STAGED_ATTR(false, 0, 8); BD_STAGED_ATTR(false, 1, 8);
+#undef BD_STAGED_ATTR
+}
+
+// Smooth all six colours of level k (v-lines colours 0..2 then u-lines 0..2; reversed for the
+// post-smoothing sweep so the V-cycle stays symmetric).
+void launch_smooth(Solver& s, int k, int reverse) {
+  Level& L = s.lv[k];
+  for (int q = 0; q < 6; q++) {
+    int c = reverse ? 5 - q : q;
+    int count = L.color_start[c + 1] - L.color_start[c];
+    if (count == 0) continue;
+    const int fam = c < 3 ? 1 : 0;
+    int maxlen = 0;
+    for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
+    SmoothArgs S{L.op, L.fixed, L.Np, L.lines + L.color_start[c], L.x, L.b, &s.scal->done, L.geo};
+    // each node of the colour's lines: its full row (fine 58 floats, coarse 150) + b, x read, x write
+    double nodes = (double)count * maxlen;
+    ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * (L.fine ? 268.0 : 636.0));
+    if (L.fine) {
+      if (fam == 0) launch_one<true, 0>(S, count, maxlen, s.stream);
+      else launch_one<true, 1>(S, count, maxlen, s.stream);
+    } else {
+      if (fam == 0) launch_one<false, 0>(S, count, maxlen, s.stream);
+      else launch_one<false, 1>(S, count, maxlen, s.stream);
+    }
+  }
+}
+
+}  // namespace bd
