@@ -23,7 +23,13 @@ BD_PRECISION_FP32 = 0
 
 (BD_BUF_X, BD_BUF_V, BD_BUF_DX, BD_BUF_PRED, BD_BUF_DELTA, BD_BUF_RHS, BD_BUF_LEVEL_X, BD_BUF_LEVEL_B,
  BD_BUF_LEVEL_R, BD_BUF_STENCIL, BD_BUF_FIXED, BD_BUF_PROLONG, BD_BUF_REST, BD_BUF_INTACT, BD_BUF_CUTMASK,
- BD_BUF_SEGFLAGS, BD_BUF_FAILSITE, BD_BUF_TAILS, BD_BUF_FIELD, BD_BUF_LAST_UPDATE, BD_BUF_ENERGY) = range(21)
+ BD_BUF_SEGFLAGS, BD_BUF_FAILSITE, BD_BUF_TAILS, BD_BUF_FIELD, BD_BUF_LAST_UPDATE, BD_BUF_ENERGY,
+ BD_BUF_EXT, BD_BUF_CMETA, BD_BUF_PAIRS, BD_BUF_CELLS, BD_BUF_ITEMS) = range(26)
+
+BD_CONTACT_K = 4
+BD_CONTACT_TABLE_BITS = 17
+BD_CONTACT_NONE = 0xFFFFFFFF
+BD_CONTACT_MAX_NODES = (1 << 20) - 1
 
 
 class bd_yarn(C.Structure):
@@ -36,7 +42,8 @@ class bd_yarn(C.Structure):
 class bd_material(C.Structure):
     _fields_ = [("pitch_u", C.c_double), ("pitch_v", C.c_double), ("yarns", bd_yarn * 2),
                 ("shear_stiffness", C.c_double), ("crossing_friction", C.c_double),
-                ("crossing_normal0", C.c_double), ("crossing_wrap", C.c_double), ("air_drag", C.c_double)]
+                ("crossing_normal0", C.c_double), ("crossing_wrap", C.c_double), ("air_drag", C.c_double),
+                ("thickness", C.c_double)]
 
 
 class bd_layer(C.Structure):
@@ -63,7 +70,7 @@ class bd_params(C.Structure):
                 ("newton_max", C.c_int32), ("newton_tolerance", C.c_double), ("pcg_iterations", C.c_int32),
                 ("pcg_max_iterations", C.c_int32), ("pcg_rel_tol", C.c_double), ("coarse_sweeps", C.c_int32),
                 ("max_coarsest_nodes", C.c_int32), ("lag_coarse", C.c_int32), ("line_search", C.c_int32),
-                ("history", C.c_int32), ("precision", C.c_int32)]
+                ("history", C.c_int32), ("precision", C.c_int32), ("self_contact", C.c_int32), ("pad_", C.c_int32)]
 
 
 class bd_grips(C.Structure):
@@ -99,7 +106,8 @@ def _yarn_struct(y) -> bd_yarn:
 def layer_struct(L) -> bd_layer:
     m = L.material
     mat = bd_material(m["pitchU"], m["pitchV"], (bd_yarn * 2)(_yarn_struct(m["yarns"][0]), _yarn_struct(m["yarns"][1])),
-                      m["shearStiffness"], m["crossingFriction"], m["crossingNormal0"], m["crossingWrap"], m["airDrag"])
+                      m["shearStiffness"], m["crossingFriction"], m["crossingNormal0"], m["crossingWrap"], m["airDrag"],
+                      m.get("thickness", 0.0))
     return bd_layer(L.nx, L.ny, L.node_offset, L.bu, L.bv, L.field_offset, L.K, L.M,
                     L.weave["pitchU"], L.weave["pitchV"], mat)
 
@@ -140,7 +148,7 @@ class SceneBuffers:
 def make_params(h=1 / 120, gravity=(0.0, 0.0, -9.81), newton_iterations=2, newton_max=None,
                 newton_tolerance=1e-5, pcg_iterations=2, pcg_rel_tol=0.0, pcg_max_iterations=200,
                 coarse_sweeps=2, max_coarsest_nodes=400, lag_coarse=True, line_search=True, history=True,
-                precision=BD_PRECISION_FP32) -> bd_params:
+                precision=BD_PRECISION_FP32, self_contact=False) -> bd_params:
     """Solver parameters; the defaults are the reference's production schedule (GPU_DEFAULTS,
     gpu/solver.ts:61-72: h 1/120, 2 Newton x 2 PCG, 2 coarse sweeps, coarsest <= 400 nodes)."""
     p = bd_params()
@@ -158,6 +166,7 @@ def make_params(h=1 / 120, gravity=(0.0, 0.0, -9.81), newton_iterations=2, newto
     p.line_search = int(bool(line_search))
     p.history = int(bool(history))
     p.precision = precision
+    p.self_contact = int(bool(self_contact))
     return p
 
 
@@ -230,6 +239,9 @@ def load_library(path: os.PathLike | None = None):
     lib.bd_set_grip_goals.argtypes = [vp, C.c_int, P(C.c_double)]
     lib.bd_profile_enable.argtypes = [vp, C.c_int]
     lib.bd_profile_read.argtypes = [vp, P(bd_kernel_profile), C.c_int, P(C.c_int)]
+    for name in ("bd_contact_detect", "bd_contact_terms", "bd_project_start"):
+        getattr(lib, name).argtypes = [vp]
+    lib.bd_set_floor.argtypes = [vp, C.c_int, C.c_double]
     if path is None:
         _lib = lib
     return lib
@@ -242,4 +254,5 @@ EXPORTED = (
     "bd_substep", "bd_predict", "bd_assemble", "bd_vcycle", "bd_smooth", "bd_residual", "bd_pcg",
     "bd_line_search", "bd_newton_apply", "bd_finalize", "bd_damage", "bd_slip", "bd_sync", "bd_read",
     "bd_write", "bd_buffer_bytes", "bd_get_stats", "bd_set_grip_goals", "bd_profile_enable", "bd_profile_read",
+    "bd_contact_detect", "bd_contact_terms", "bd_project_start", "bd_set_floor",
 )

@@ -127,3 +127,109 @@ def advance_grips(grips, velocity):
     for g, v in zip(grips, velocity):
         g["goal"] = np.asarray(g["goal"], dtype=np.float64) + v
     return grips
+
+
+def fold_scene(nx=48, ny=64, pitch=1e-3, radius=1e-3, material=None):
+    """A horizontal nx x ny sheet folded over itself along v: the lower layer at z = 0, a semicircular
+    fold of the given radius, the upper layer at z = 2 radius (the folded-coupon workload's shape).
+    Nodes follow the fold by arc length, so spans keep their rest length up to the chord of the arc."""
+    mat = material or linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / (2 * 1.2))
+    N = nx * ny
+    L = Layer(0, mat, mat, (1, 1), nx, ny, 0, N, (1.0, 0.0, 0.0, 1.0), (0.0, 0.0, 0.0),
+              ((nx - 1) * pitch / 2, (ny - 1) * pitch / 2))
+    C = field_layout([L])
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    i = i.ravel()
+    j = j.ravel()
+    s = j * pitch
+    arc = math.pi * radius
+    L1 = ((ny - 1) * pitch - arc) / 2
+    th = np.clip((s - L1) / radius, 0.0, math.pi)
+    y = np.where(s <= L1, s, np.where(s >= L1 + arc, L1 - (s - L1 - arc), L1 + radius * np.sin(th)))
+    z = np.where(s <= L1, 0.0, np.where(s >= L1 + arc, 2 * radius, radius * (1 - np.cos(th))))
+    x = i * pitch - (nx - 1) * pitch / 2
+    S = CompiledScene(
+        layers=[L], node_count=N, rest_uv=np.stack([i * pitch, j * pitch], 1), position=np.stack([x, y - L1 / 2, z], 1),
+        mass=np.full(N, crossing_mass(mat)), pinned=np.zeros(N, np.uint8), wear=np.zeros(N, np.float32),
+        structural_wear=np.zeros(N, np.float32), rest_length=np.zeros(2 * N), softening=np.zeros(2 * N),
+        seg_flags=np.zeros(2 * N, np.uint16), intact=np.zeros(2 * N), cut_mask=np.zeros(2 * N, np.uint32),
+        fail_site=np.zeros(2 * N), crossing_count=C, field_strength=np.full(2 * C, UNBREAKABLE, np.float32),
+        field_damage=np.zeros(2 * C, np.uint8), field_area=np.full(2 * C, 255, np.uint8),
+        crossing_normal0=np.full(N, mat["crossingNormal0"]), tails=np.zeros(4 * N))
+    n_idx = np.arange(N)
+    for f in (0, 1):
+        ex = (i < nx - 1) if f == 0 else (j < ny - 1)
+        sg = 2 * n_idx[ex] + f
+        S.seg_flags[sg] = SEG_EXISTS
+        S.rest_length[sg] = pitch
+        S.intact[sg] = 1.0
+        S.softening[sg] = mat["yarns"][f]["softeningOpening"]
+        S.fail_site[sg] = 0.5
+    S.extra["fold"] = {"L1": L1, "radius": radius, "j_lower": int(np.floor(L1 / pitch)),
+                       "j_upper": int(np.ceil((L1 + arc) / pitch))}
+    return S
+
+
+def config_fold(nx=48, ny=64, pitch=1e-3, radius=1e-3, floor_gap=0.05e-3, newton=3, rel_tol=1e-5):
+    """The fold dropped under gravity onto a floor just below the lower layer, with self-contact:
+    the upper layer lands on the lower one."""
+    S = fold_scene(nx, ny, pitch, radius)
+    half = 0.5 * S.layers[0].material["thickness"]
+    params = make_params(h=1 / 120, newton_iterations=newton, pcg_rel_tol=rel_tol, pcg_max_iterations=100,
+                         max_coarsest_nodes=400, coarse_sweeps=2, lag_coarse=True, line_search=True, history=False,
+                         self_contact=True)
+    return {"name": f"fold{nx}x{ny}", "scene": S, "params": params, "grips": [], "floor_z": -(half + floor_gap),
+            "steps": 24}
+
+
+def flap_scene(nx=48, ny=64, pitch=1e-3, gap=2e-3, material=None):
+    """One horizontal lattice torn across its middle row into two flaps, the upper flap (rows
+    j >= ny/2) turned over onto the lower one: reflected in v, shifted by half a pitch in both
+    directions and lifted by `gap`. The spans across the tear are broken (SEG_BROKEN)."""
+    mat = material or linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / (2 * 1.2))
+    N = nx * ny
+    L = Layer(0, mat, mat, (1, 1), nx, ny, 0, N, (1.0, 0.0, 0.0, 1.0), (0.0, 0.0, 0.0),
+              ((nx - 1) * pitch / 2, (ny - 1) * pitch / 2))
+    C = field_layout([L])
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    i = i.ravel()
+    j = j.ravel()
+    half = ny // 2
+    up = j >= half
+    x = np.where(up, (i + 0.5) * pitch, i * pitch) - (nx - 1) * pitch / 2
+    y = np.where(up, (ny - 1 - j + 0.5) * pitch, j * pitch) - (half - 1) * pitch / 2
+    z = np.where(up, gap, 0.0)
+    S = CompiledScene(
+        layers=[L], node_count=N, rest_uv=np.stack([i * pitch, j * pitch], 1), position=np.stack([x, y, z], 1),
+        mass=np.full(N, crossing_mass(mat)), pinned=np.zeros(N, np.uint8), wear=np.zeros(N, np.float32),
+        structural_wear=np.zeros(N, np.float32), rest_length=np.zeros(2 * N), softening=np.zeros(2 * N),
+        seg_flags=np.zeros(2 * N, np.uint16), intact=np.zeros(2 * N), cut_mask=np.zeros(2 * N, np.uint32),
+        fail_site=np.zeros(2 * N), crossing_count=C, field_strength=np.full(2 * C, UNBREAKABLE, np.float32),
+        field_damage=np.zeros(2 * C, np.uint8), field_area=np.full(2 * C, 255, np.uint8),
+        crossing_normal0=np.full(N, mat["crossingNormal0"]), tails=np.zeros(4 * N))
+    n_idx = np.arange(N)
+    for f in (0, 1):
+        ex = (i < nx - 1) if f == 0 else (j < ny - 1)
+        sg = 2 * n_idx[ex] + f
+        S.seg_flags[sg] = SEG_EXISTS
+        S.rest_length[sg] = pitch
+        S.intact[sg] = 1.0
+        S.softening[sg] = mat["yarns"][f]["softeningOpening"]
+        S.fail_site[sg] = 0.5
+    tear = 2 * n_idx[j == half - 1] + 1
+    S.seg_flags[tear] |= 2
+    S.intact[tear] = 0.0
+    S.extra["flaps"] = {"half": half, "gap": gap}
+    return S
+
+
+def config_flaps(nx=48, ny=64, pitch=1e-3, gap=2e-3, floor_gap=0.05e-3, newton=3, rel_tol=1e-5):
+    """The upper flap falls under gravity onto the lower one, which lies on a floor just below it,
+    with self-contact (the workload of a turned-over flap landing on the cloth)."""
+    S = flap_scene(nx, ny, pitch, gap)
+    half = 0.5 * S.layers[0].material["thickness"]
+    params = make_params(h=1 / 120, newton_iterations=newton, pcg_rel_tol=rel_tol, pcg_max_iterations=100,
+                         max_coarsest_nodes=400, coarse_sweeps=2, lag_coarse=True, line_search=True, history=False,
+                         self_contact=True)
+    return {"name": f"flaps{nx}x{ny}", "scene": S, "params": params, "grips": [], "floor_z": -(half + floor_gap),
+            "steps": 24}

@@ -11,13 +11,16 @@ import ctypes as C
 import numpy as np
 
 from . import capi
-from .capi import (BD_BUF_CUTMASK, BD_BUF_DELTA, BD_BUF_DX, BD_BUF_ENERGY, BD_BUF_FAILSITE, BD_BUF_FIELD,
+from .capi import (BD_BUF_CELLS, BD_BUF_CMETA, BD_BUF_EXT, BD_BUF_ITEMS, BD_BUF_PAIRS, BD_CONTACT_K,
+                   BD_BUF_CUTMASK, BD_BUF_DELTA, BD_BUF_DX, BD_BUF_ENERGY, BD_BUF_FAILSITE, BD_BUF_FIELD,
                    BD_BUF_FIXED, BD_BUF_INTACT, BD_BUF_LAST_UPDATE, BD_BUF_LEVEL_B, BD_BUF_LEVEL_R, BD_BUF_LEVEL_X,
                    BD_BUF_PRED, BD_BUF_PROLONG, BD_BUF_REST, BD_BUF_RHS, BD_BUF_SEGFLAGS, BD_BUF_STENCIL,
                    BD_BUF_TAILS, BD_BUF_V, BD_BUF_X, GripBuffers, SceneBuffers, bd_stats, make_params)
 
 _DTYPES = {
     BD_BUF_FIXED: np.uint8, BD_BUF_CUTMASK: np.uint32, BD_BUF_SEGFLAGS: np.uint16, BD_BUF_FIELD: np.uint8,
+    BD_BUF_EXT: np.float32, BD_BUF_CMETA: np.float32, BD_BUF_PAIRS: np.uint32, BD_BUF_CELLS: np.uint32,
+    BD_BUF_ITEMS: np.float32,
 }
 
 
@@ -143,6 +146,22 @@ class GpuSolver:
     def encode_slip(self):
         self._check(self.lib.bd_slip(self.h), "bd_slip")
 
+    def encode_contact_detect(self):
+        """SelfContact.encodeDetect (gpu/selfcontact.ts:139-150)."""
+        self._check(self.lib.bd_contact_detect(self.h), "bd_contact_detect")
+
+    def encode_contact_terms(self):
+        """SelfContact.encodeTerms (gpu/selfcontact.ts:153-157)."""
+        self._check(self.lib.bd_contact_terms(self.h), "bd_contact_terms")
+
+    def encode_project_start(self):
+        """projectStart (step.wgsl:85-95)."""
+        self._check(self.lib.bd_project_start(self.h), "bd_project_start")
+
+    def set_floor(self, z):
+        """setFloor(z | null) (gpu/solver.ts:796-801)."""
+        self._check(self.lib.bd_set_floor(self.h, 0 if z is None else 1, 0.0 if z is None else float(z)), "bd_set_floor")
+
     def sync(self):
         self._check(self.lib.bd_sync(self.h), "bd_sync")
 
@@ -166,6 +185,26 @@ class GpuSolver:
 
     def vec(self, buf, level=0):
         return self.read(buf, level).reshape(-1, 3)
+
+    # ---- contact buffers
+    def ext(self):
+        """(N, 9) external terms: gradient xyz, Hessian xx xy xz yy yz zz."""
+        return self.read(BD_BUF_EXT).reshape(-1, 9)
+
+    def cmeta(self):
+        """(N, 2, 4) bounds: own (normal, share) and reciprocal (direction, share)."""
+        return self.read(BD_BUF_CMETA).reshape(-1, 2, 4)
+
+    def pairs(self):
+        """(N, K + 2): K candidate faces, reciprocal key, row mark."""
+        return self.read(BD_BUF_PAIRS).reshape(-1, BD_CONTACT_K + 2)
+
+    def cells(self):
+        return self.read(BD_BUF_CELLS)
+
+    def items(self):
+        """(2N, 8) face records: centroid now, radius, centroid at start, face id bits."""
+        return self.read(BD_BUF_ITEMS).reshape(-1, 8)
 
     @property
     def x(self): return self.vec(BD_BUF_X)

@@ -11,6 +11,7 @@
 #include <stdexcept>
 
 #include "bd_internal.h"
+#include "bounds.h"
 
 namespace bd {
 
@@ -171,6 +172,10 @@ struct AsmArgs {
   const float* grip_w;
   const double* grip_off;
   const double* grip_goal_k;
+  const float* ext;      // external terms (self-contact) or null
+  const float4* cmeta;   // contact bounds or null
+  int floor_on;
+  float floor_z;
 };
 
 __device__ __forceinline__ void add_neg_axial(float* blk, float kpar, float kperp, const float e[3]) {
@@ -402,6 +407,50 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs A) {
     }
   }
 
+  // external node terms (self-contact and friction, gathered per node): gradient and symmetric
+  // Hessian (assemble.wgsl:183-190)
+  if (A.ext) {
+    for (int q = 0; q < 3; q++) g[q] += A.ext[q * Np + n];
+    for (int q = 0; q < 6; q++) C.a[q] += A.ext[(3 + q) * Np + n];
+  }
+  // floor contact: cubic penalty along +z and friction on the step (assemble.wgsl:191-209)
+  bool onFloorCore = false;
+  if (A.floor_on) {
+    float half = 0.5f * (float)L.thickness;
+    float delta = half + kFloorBulge;
+    float zc = c.x[2 * Np + n] + dn.z;
+    float s = delta - (zc - A.floor_z);
+    if (s > 0.f) {
+      float lam = kFloorK * s * s / delta;
+      float ut = sqrtf(dn.x * dn.x + dn.y * dn.y);
+      float kf = kFloorMu * lam / fmaxf(ut, kFloorSpeed * h);
+      g[0] += kf * dn.x;
+      g[1] += kf * dn.y;
+      g[2] -= lam;
+      C.a[0] += kf;
+      C.a[3] += kf;
+      C.a[5] += 2.f * kFloorK * s / delta;
+      onFloorCore = zc <= A.floor_z + half * (1.f + 1e-3f);
+    }
+  }
+  // active contact bounds: a node at its bound whose net force still drives it on is held along the
+  // bound normal in the Newton direction (assemble.wgsl:305-322)
+  if (A.cmeta || onFloorCore) {
+    float kAct = kActiveBound * (C.a[0] + C.a[3] + C.a[5]);
+    if (A.cmeta) {
+      for (int b = 0; b < 2; b++) {
+        float4 m = A.cmeta[2 * n + b];
+        if (m.w >= 1e8f || (m.x == 0.f && m.y == 0.f && m.z == 0.f)) continue;
+        float approach = -(dn.x * m.x + dn.y * m.y + dn.z * m.z);
+        if (approach >= m.w - 1e-3f * fabsf(m.w) && g[0] * m.x + g[1] * m.y + g[2] * m.z > 0.f) {
+          float e[3] = {m.x, m.y, m.z};
+          sym_add_axial(C, 1.f, kAct, 0.f, e);
+        }
+      }
+    }
+    if (onFloorCore && g[2] > 0.f) C.a[5] += kAct;
+  }
+
   for (int q = 0; q < 6; q++) {
     op[(F_C + q) * Np + n] = C.a[q];
     op[(F_E + q) * Np + n] = blk[0][q];
@@ -430,26 +479,62 @@ __global__ __launch_bounds__(256) void k_predict(int N, int Np, float h, float g
 
 // newtonApply (step.wgsl:97-112 / newtonMultigrid.ts:637-644): dx += alpha * delta with a
 // non-finite firewall, per-node update norm and the iteration's largest update
-__global__ __launch_bounds__(256) void k_newton_apply(int N, int Np, const uint8_t* pinned, const double* delta, float* dx,
-                                                       float* last_update, const Scal* scal, unsigned int* max_update) {
+struct ApplyArgs {
+  int N, Np;
+  const uint8_t* pinned;
+  const double* delta;
+  float* dx;
+  float* last_update;
+  const Scal* scal;
+  unsigned int* max_update;
+  // bounds: contact (cmeta, or null) and the floor
+  const float4* cmeta;
+  int floor_on;
+  float floor_z, gnorm;
+  const float* x;
+  const float* mass;
+  const uint32_t* ijl;
+  const DevLayer* layers;
+};
+
+__global__ __launch_bounds__(256) void k_newton_apply(ApplyArgs A) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  if (pinned[n]) {
-    last_update[n] = 0.f;
+  if (n >= A.N) return;
+  const int Np = A.Np;
+  if (A.pinned[n]) {
+    A.last_update[n] = 0.f;
     return;
   }
-  double a = scal->line_alpha;
-  double d[3] = {delta[n], delta[Np + n], delta[2 * Np + n]};
+  double a = A.scal->line_alpha;
+  double d[3] = {A.delta[n], A.delta[Np + n], A.delta[2 * Np + n]};
   bool ok = isfinite(d[0]) && isfinite(d[1]) && isfinite(d[2]) && isfinite(a);
-  double s2 = 0.0;
-  for (int q = 0; q < 3; q++) {
-    double st = ok ? a * d[q] : 0.0;
-    dx[q * Np + n] = (float)((double)dx[q * Np + n] + st);
-    s2 += st * st;
+  float lu;
+  if (!A.cmeta && !A.floor_on) {
+    double s2 = 0.0;
+    for (int q = 0; q < 3; q++) {
+      double st = ok ? a * d[q] : 0.0;
+      A.dx[q * Np + n] = (float)((double)A.dx[q * Np + n] + st);
+      s2 += st * st;
+    }
+    lu = (float)sqrt(s2);
+  } else {
+    // the updated step made feasible for the contact bounds and the floor (step.wgsl:97-112)
+    float old[3], nd[3];
+    for (int q = 0; q < 3; q++) {
+      old[q] = A.dx[q * Np + n];
+      nd[q] = (float)((double)old[q] + (ok ? a * d[q] : 0.0));
+    }
+    if (A.cmeta) feasible_step(nd, A.cmeta[2 * n], A.cmeta[2 * n + 1]);
+    if (A.floor_on) {
+      float half = 0.5f * (float)A.layers[A.ijl[n] >> 24].thickness;
+      nd[2] = floor_bound_z(A.x[2 * Np + n], old[2], nd[2], half, A.mass[n], A.gnorm, A.floor_z);
+    }
+    float e0 = nd[0] - old[0], e1 = nd[1] - old[1], e2 = nd[2] - old[2];
+    lu = sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+    for (int q = 0; q < 3; q++) A.dx[q * Np + n] = nd[q];
   }
-  float lu = (float)sqrt(s2);
-  last_update[n] = lu;
-  atomicMax(max_update, __float_as_uint(lu));
+  A.last_update[n] = lu;
+  atomicMax(A.max_update, __float_as_uint(lu));
 }
 
 // commit (solver.ts:622-628)
@@ -488,13 +573,34 @@ void launch_assemble(Solver& s) {
   A.grip_w = s.grip_w;
   A.grip_off = s.grip_off;
   A.grip_goal_k = s.grip_goal_k;
+  A.ext = s.P.self_contact ? s.ext : nullptr;
+  A.cmeta = s.P.self_contact ? s.cmeta : nullptr;
+  A.floor_on = s.floor_on ? 1 : 0;
+  A.floor_z = s.floor_z;
   k_assemble<<<grid_for(s.N, 256), 256, 0, s.stream>>>(A);
 }
 
 void launch_newton_apply(Solver& s, int it) {
   ProfScope ps(s, PC_PREDICT_COMMIT, s.N * 52.0);
-  k_newton_apply<<<grid_for(s.N, 256), 256, 0, s.stream>>>(s.N, s.Np, s.pinned, s.delta, s.dx, s.last_update, s.scal,
-                                                           &s.scal->max_update[it]);
+  ApplyArgs A;
+  A.N = s.N;
+  A.Np = s.Np;
+  A.pinned = s.pinned;
+  A.delta = s.delta;
+  A.dx = s.dx;
+  A.last_update = s.last_update;
+  A.scal = s.scal;
+  A.max_update = &s.scal->max_update[it];
+  A.cmeta = s.P.self_contact ? s.cmeta : nullptr;
+  A.floor_on = s.floor_on ? 1 : 0;
+  A.floor_z = s.floor_z;
+  float gx = (float)s.P.gravity[0], gy = (float)s.P.gravity[1], gz = (float)s.P.gravity[2];
+  A.gnorm = sqrtf((gx * gx + gy * gy) + gz * gz);
+  A.x = s.x;
+  A.mass = s.mass;
+  A.ijl = s.ijl;
+  A.layers = s.layers;
+  k_newton_apply<<<grid_for(s.N, 256), 256, 0, s.stream>>>(A);
 }
 
 void launch_commit(Solver& s) {

@@ -83,12 +83,13 @@ struct Scal {
 enum ProfClass {
   PC_PREDICT_COMMIT, PC_ASSEMBLE, PC_PROLONG_WEIGHTS, PC_RAP, PC_SMOOTH_FINE, PC_SMOOTH_COARSE, PC_RESIDUAL_FINE,
   PC_RESIDUAL_COARSE, PC_RESTRICT, PC_PROLONG, PC_MATVEC, PC_PCG_VECTOR, PC_TRUE_RESIDUAL, PC_ENERGY, PC_DAMAGE, PC_SLIP,
-  PC_COUNT
+  PC_CONTACT_HASH, PC_CONTACT_DETECT, PC_CONTACT_TERMS, PC_COUNT
 };
 inline const char* prof_name(int c) {
   static const char* n[PC_COUNT] = {"predict_commit", "assemble", "prolong_weights", "rap", "smooth_fine", "smooth_coarse",
                                     "residual_fine", "residual_coarse", "restrict", "prolong", "matvec_dot", "pcg_vector",
-                                    "true_residual", "energy", "damage", "slip"};
+                                    "true_residual", "energy", "damage", "slip", "contact_hash", "contact_detect",
+                                    "contact_terms"};
   return n[c];
 }
 
@@ -152,7 +153,26 @@ struct Solver {
   bool assembled = false, hierarchy = false;
   bd_stats stats{};
   int coarse_valid_iteration = -1;
+
+  // contact: self-contact (node against intact cell) and the floor
+  bool floor_on = false;
+  float floor_z = 0.f;
+  uint32_t* ijl = nullptr;      // per node: i | j << 12 | layer << 24
+  uint32_t* cells = nullptr;    // 2 * kTable: counts scanned into starts, then fill cursors
+  uint32_t* scan_sums = nullptr;
+  float4* items = nullptr;      // 2 per record, at most 2 records per face
+  uint32_t* pairs = nullptr;    // kContactK planes of Np: candidate faces
+  uint32_t* keys = nullptr;     // reciprocal-bound keys (cleared by resolve)
+  uint32_t* marks = nullptr;    // damaged-row marks
+  float4* cmeta = nullptr;      // 2 per node: own bound, reciprocal bound
+  int* acc = nullptr;           // 9 planes of fixed-point accumulators
+  float* ext = nullptr;         // 9 planes: gradient xyz, Hessian xx xy xz yy yz zz
+  bool contact_detected = false;
 };
+
+constexpr int kContactK = 4;
+constexpr int kTableBits = 17;
+constexpr int kTable = 1 << kTableBits;
 
 // launchers (assemble.hip)
 void launch_predict(Solver& s);
@@ -174,6 +194,12 @@ void launch_energy(Solver& s);
 // history.hip
 void launch_damage(Solver& s);
 void launch_slip(Solver& s);
+// contact.hip
+void contact_alloc(Solver& s);
+void contact_reset_bounds(Solver& s);
+void launch_contact_detect(Solver& s);
+void launch_contact_terms(Solver& s);
+void launch_project_start(Solver& s);
 
 #define BD_HIP(call)                                                     \
   do {                                                                   \

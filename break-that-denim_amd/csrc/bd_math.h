@@ -37,7 +37,7 @@ struct DevLayer {
   double wpu, wpv;   // individual-yarn pitches
   double pu, pv;     // mechanical pitches
   DevYarn y[2];
-  double shear_k, mu, cn0, wrap, air_drag;
+  double shear_k, mu, cn0, wrap, air_drag, thickness;
 };
 
 // ---------------------------------------------------------------- symmetric 3x3 blocks

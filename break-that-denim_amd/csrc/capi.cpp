@@ -96,6 +96,7 @@ DevLayer dev_layer(const bd_layer& L) {
   d.cn0 = L.material.crossing_normal0;
   d.wrap = L.material.crossing_wrap;
   d.air_drag = L.material.air_drag;
+  d.thickness = L.material.thickness;
   return d;
 }
 
@@ -150,7 +151,8 @@ void destroy_all(Solver& s) {
   void* ptrs[] = {s.layers, s.x, s.v, s.dx, s.pred, s.delta, s.rhs, s.p, s.ap, s.last_update, s.mass, s.cn0, s.pinned,
                   s.layer_of, s.rest, s.intact, s.fail_site, s.tails, s.softening, s.cut_mask, s.flags, s.strain,
                   s.damage, s.area, s.sep, s.pullmark, s.strength, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k,
-                  s.partials, s.scal, s.counters};
+                  s.partials, s.scal, s.counters, s.ijl, s.cells, s.scan_sums, s.items, s.pairs, s.keys,
+                  s.marks, s.cmeta, s.acc, s.ext};
   for (void* p : ptrs) dfree(p);
   for (int k = 0; k < s.nlev; k++) {
     Level& L = s.lv[k];
@@ -193,6 +195,7 @@ void build_hierarchy(Solver& s) {
 }
 
 void newton_iteration(Solver& s, int it, bool coarse) {
+  if (s.P.self_contact) launch_contact_terms(s);
   launch_assemble(s);
   s.assembled = true;
   if (coarse || !s.hierarchy) build_hierarchy(s);
@@ -230,6 +233,10 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
       g_create_error = "bd_create: yarn line longer than BD_MAX_LINE (" + std::to_string(BD_MAX_LINE) + ")";
       return BD_ERR_LIMIT;
     }
+    if (L.nx > 4096 || L.ny > 4096) {
+      g_create_error = "bd_create: lattice index beyond 12 bits";
+      return BD_ERR_LIMIT;
+    }
     if (L.bundle_u < 1 || L.bundle_v < 1 || L.bundle_u > 32 || L.bundle_v > 32) {
       g_create_error = "bd_create: bundle too large for the cut masks";
       return BD_ERR_LIMIT;
@@ -239,6 +246,10 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
   if (total != sc->node_count) {
     g_create_error = "bd_create: node_count does not match the layers";
     return BD_ERR_INVALID;
+  }
+  if (prm->self_contact && sc->node_count > BD_CONTACT_MAX_NODES) {
+    g_create_error = "bd_create: self-contact bound keys hold at most 2^20 - 1 nodes";
+    return BD_ERR_LIMIT;
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
@@ -323,6 +334,24 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
       upload(s, s.strength, sc->field_strength, 2 * (size_t)s.C);
     }
     upload_vec3(s, s.x, N, Np, sc->position);
+
+    // contact: lattice coordinates, external terms (zero unless self-contact writes them), bounds
+    {
+      std::vector<uint32_t> ijl(Np, 0u);
+      for (int l = 0; l < s.nl; l++) {
+        const bd_layer& L = sc->layers[l];
+        for (int j = 0; j < L.ny; j++)
+          for (int i = 0; i < L.nx; i++) ijl[L.node_offset + j * L.nx + i] = (uint32_t)i | ((uint32_t)j << 12) | ((uint32_t)l << 24);
+      }
+      s.ijl = dalloc<uint32_t>(Np);
+      upload(s, s.ijl, ijl.data(), Np);
+      s.ext = dalloc<float>(9 * (size_t)Np);
+      BD_HIP(hipMemsetAsync(s.ext, 0, 9 * (size_t)Np * sizeof(float), s.stream));
+      s.cmeta = dalloc<float4>(2 * (size_t)Np);
+      contact_reset_bounds(s);
+      if (s.P.self_contact) contact_alloc(s);
+      BD_HIP(hipStreamSynchronize(s.stream));
+    }
 
     std::vector<Geo> geos = level_layouts(s.geo, prm->max_coarsest_nodes);
     s.nlev = (int)geos.size();
@@ -581,12 +610,46 @@ int bd_slip(bd_solver* h) {
   });
 }
 
+int bd_contact_detect(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    if (!s.P.self_contact) return fail(h, BD_ERR_STATE, "bd_contact_detect: created without self_contact");
+    launch_contact_detect(s);
+    return BD_OK;
+  });
+}
+
+int bd_contact_terms(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    if (!s.P.self_contact) return fail(h, BD_ERR_STATE, "bd_contact_terms: created without self_contact");
+    launch_contact_terms(s);
+    return BD_OK;
+  });
+}
+
+int bd_project_start(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    launch_project_start(s);
+    return BD_OK;
+  });
+}
+
+int bd_set_floor(bd_solver* h, int enabled, double z) {
+  return guarded(h, [&](Solver& s) {
+    BD_HIP(hipStreamSynchronize(s.stream));
+    s.floor_on = enabled != 0;
+    s.floor_z = enabled ? (float)z : 0.f;
+    return BD_OK;
+  });
+}
+
 int bd_substep(bd_solver* h, int min_it, int max_it) {
   return guarded(h, [&](Solver& s) {
     const int n_it = std::min(BD_MAX_NEWTON, std::max(s.P.newton_iterations, min_it));
     const int n_max = std::min(BD_MAX_NEWTON, std::max(std::max(s.P.newton_max, n_it), max_it));
     launch_reset_counters(s);
     launch_predict(s);
+    if (s.P.self_contact) launch_contact_detect(s);
+    if (s.P.self_contact || s.floor_on) launch_project_start(s);
     int done = 0;
     for (int it = 0; it < n_max; it++) {
       if (it >= n_it) {
@@ -671,6 +734,11 @@ size_t bd_buffer_bytes(const bd_solver* h, bd_buffer buf, int level) {
     case BD_BUF_FIELD: return 2 * (size_t)s.C;
     case BD_BUF_LAST_UPDATE: return (size_t)s.N * sizeof(double);
     case BD_BUF_ENERGY: return 8 * sizeof(double);
+    case BD_BUF_EXT: return 9 * (size_t)s.N * sizeof(float);
+    case BD_BUF_CMETA: return 8 * (size_t)s.N * sizeof(float);
+    case BD_BUF_PAIRS: return s.cells ? (kContactK + 2) * (size_t)s.N * sizeof(uint32_t) : 0;
+    case BD_BUF_CELLS: return s.cells ? 2 * (size_t)kTable * sizeof(uint32_t) : 0;
+    case BD_BUF_ITEMS: return s.cells ? 16 * (size_t)s.N * sizeof(float) : 0;
     default: return 0;
   }
 }
@@ -800,6 +868,30 @@ int bd_read(bd_solver* h, bd_buffer buf, int level, void* dst, size_t bytes) {
         }
         break;
       }
+      case BD_BUF_EXT: {
+        std::vector<float> t(9 * (size_t)s.Np);
+        download(s, t.data(), s.ext, t.size());
+        float* o = static_cast<float*>(dst);
+        for (int n = 0; n < s.N; n++)
+          for (int q = 0; q < 9; q++) o[9 * (size_t)n + q] = t[(size_t)q * s.Np + n];
+        break;
+      }
+      case BD_BUF_CMETA: download(s, static_cast<float4*>(dst), s.cmeta, 2 * (size_t)s.N); break;
+      case BD_BUF_PAIRS: {
+        std::vector<uint32_t> t(kContactK * (size_t)s.Np), k(s.N), m(s.N);
+        download(s, t.data(), s.pairs, t.size());
+        download(s, k.data(), s.keys, (size_t)s.N);
+        download(s, m.data(), s.marks, (size_t)s.N);
+        uint32_t* o = static_cast<uint32_t*>(dst);
+        for (int n = 0; n < s.N; n++) {
+          for (int q = 0; q < kContactK; q++) o[(kContactK + 2) * (size_t)n + q] = t[(size_t)q * s.Np + n];
+          o[(kContactK + 2) * (size_t)n + kContactK] = k[n];
+          o[(kContactK + 2) * (size_t)n + kContactK + 1] = m[n];
+        }
+        break;
+      }
+      case BD_BUF_CELLS: download(s, static_cast<uint32_t*>(dst), s.cells, 2 * (size_t)kTable); break;
+      case BD_BUF_ITEMS: download(s, static_cast<float4*>(dst), s.items, 4 * (size_t)s.N); break;
       default: return fail(h, BD_ERR_INVALID, "bd_read: unsupported buffer");
     }
     return BD_OK;
@@ -834,6 +926,15 @@ int bd_write(bd_solver* h, bd_buffer buf, int level, const void* src, size_t byt
       case BD_BUF_CUTMASK: upload(s, s.cut_mask, static_cast<const uint32_t*>(src), 2 * (size_t)s.N); break;
       case BD_BUF_SEGFLAGS: upload(s, s.flags, static_cast<const uint16_t*>(src), 2 * (size_t)s.N); break;
       case BD_BUF_FIELD: upload(s, s.damage, static_cast<const uint8_t*>(src), 2 * (size_t)s.C); break;
+      case BD_BUF_EXT: {
+        const float* in = static_cast<const float*>(src);
+        std::vector<float> t(9 * (size_t)s.Np, 0.f);
+        for (int n = 0; n < s.N; n++)
+          for (int q = 0; q < 9; q++) t[(size_t)q * s.Np + n] = in[9 * (size_t)n + q];
+        upload(s, s.ext, t.data(), t.size());
+        break;
+      }
+      case BD_BUF_CMETA: upload(s, s.cmeta, static_cast<const float4*>(src), 2 * (size_t)s.N); break;
       default: return fail(h, BD_ERR_INVALID, "bd_write: buffer is read-only");
     }
     return BD_OK;

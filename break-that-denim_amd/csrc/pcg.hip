@@ -13,6 +13,7 @@
 #include <stdexcept>
 
 #include "bd_internal.h"
+#include "bounds.h"
 
 namespace bd {
 
@@ -303,6 +304,9 @@ struct EnergyArgs {
   const double* grip_goal_k;
   Scal* sc;
   double* partials;
+  const float* ext;  // external terms (self-contact) or null
+  int floor_on;
+  double floor_z;
 };
 
 __device__ __forceinline__ bool act(const EnergyArgs& E, int s) { return (E.flags[s] & (SEG_EXISTS | SEG_BROKEN)) == SEG_EXISTS; }
@@ -370,6 +374,28 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs E) {
         double g[3];
         for (int q = 0; q < 3; q++) g[q] = kg * ((xn[q] - G[q]) + d0[q] - E.grip_off[q * Np + n]);
         if (kg > 0) quad(dot3(g, dn), kg * dot3(dn, dn));
+      }
+      // external terms (contact): quadratic model from their gradient and Hessian at this iterate
+      // (energy.wgsl:106-110)
+      if (E.ext) {
+        double ge[3], He[6];
+        for (int q = 0; q < 3; q++) ge[q] = E.ext[q * Np + n];
+        for (int q = 0; q < 6; q++) He[q] = E.ext[(3 + q) * Np + n];
+        double Hd0 = He[0] * dn[0] + He[1] * dn[1] + He[2] * dn[2];
+        double Hd1 = He[1] * dn[0] + He[3] * dn[1] + He[4] * dn[2];
+        double Hd2 = He[2] * dn[0] + He[4] * dn[1] + He[5] * dn[2];
+        quad(dot3(ge, dn), dn[0] * Hd0 + dn[1] * Hd1 + dn[2] * Hd2);
+      }
+      // floor contact: its quadratic model at this iterate (energy.wgsl:112-126)
+      if (E.floor_on) {
+        double reach = 0.5 * L.thickness + (double)kFloorBulge;
+        double s0 = reach - (xn[2] + d0[2] - E.floor_z);
+        if (s0 > 0) {
+          double kc = kFloorK;
+          double lam = kc * s0 * s0 / reach;
+          double kf = (double)kFloorMu * lam / fmax(sqrt(d0[0] * d0[0] + d0[1] * d0[1]), (double)kFloorSpeed * h);
+          quad(kf * (d0[0] * dn[0] + d0[1] * dn[1]) - lam * dn[2], kf * (dn[0] * dn[0] + dn[1] * dn[1]) + (2.0 * kc * s0 / reach) * dn[2] * dn[2]);
+        }
       }
       if (L.air_drag > 0) {
         double k = (L.air_drag * L.pu * L.pv) / h;
@@ -641,7 +667,8 @@ void launch_energy(Solver& s) {
     return;
   }
   EnergyArgs E{s.N, s.Np, s.geo, s.layers, s.P.h, s.x, s.dx, s.pred, s.mass, s.delta, s.cn0, s.pinned, s.rest, s.intact,
-               s.cut_mask, s.flags, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k, s.scal, s.partials};
+               s.cut_mask, s.flags, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k, s.scal, s.partials,
+               s.P.self_contact ? s.ext : nullptr, s.floor_on ? 1 : 0, (double)s.floor_z};
   ProfScope ps(s, PC_ENERGY, s.N * 161.0);
   k_energy<<<dot_grid(s.N), 256, 0, s.stream>>>(E);
 }

@@ -39,6 +39,15 @@ extern "C" {
 #define BD_MAX_NEWTON 8
 #define BD_MAX_GRIPS 16
 
+/* Self-contact (gpu/selfcontact.ts:7-33 SELF_CONTACT, selfcontact.wgsl:22-30): K candidate faces
+ * per node, a spatial hash of 2^17 cells of 12 mm, reach 3 mm, contact distance 1 mm, cubic
+ * penalty k = 500 N/m, friction 0.45 regularised below 2 mm/s, bound share 0.45 above a 0.3 mm
+ * hard core. Floor contact (floor.ts FLOOR_CONTACT): bulge 0.3 mm, k = 1300 N/m, friction 0.35. */
+#define BD_CONTACT_K 4
+#define BD_CONTACT_TABLE_BITS 17
+#define BD_CONTACT_NONE 0xFFFFFFFFu
+#define BD_CONTACT_MAX_NODES ((1 << 20) - 1)  /* the reciprocal-bound key holds node + 1 in 20 bits */
+
 /* Yarn family law and constants (src/simulation/material.ts:18-54). */
 typedef struct bd_yarn {
   double linear_density;    /* kg/m */
@@ -63,6 +72,7 @@ typedef struct bd_material {
   double crossing_normal0;          /* N, per bundled crossing */
   double crossing_wrap;
   double air_drag;                  /* N s / m^3 */
+  double thickness;                 /* apparent thickness for contact (material.ts:76-77), m */
 } bd_material;
 
 /* One lattice layer (scene.ts:73-89) and its yarn damage field layout (yarnField.ts:47-66). */
@@ -117,6 +127,8 @@ typedef struct bd_params {
   int32_t line_search;         /* choose alpha in {1, 1/2, 1/4, 1/8} by the potential */
   int32_t history;             /* damage field + crossing slip after commit */
   int32_t precision;           /* BD_PRECISION_* */
+  int32_t self_contact;        /* node-face self-contact (GpuSolverParams.selfContact, gpu/solver.ts:49-50) */
+  int32_t pad_;
 } bd_params;
 
 #define BD_PRECISION_FP32 0    /* FP32 operators and smoothers, FP64 dot products */
@@ -170,6 +182,11 @@ typedef enum bd_buffer {
   BD_BUF_FIELD,        /* 2C uint8 damage codes */
   BD_BUF_LAST_UPDATE,  /* N doubles, norm of each node's last Newton update */
   BD_BUF_ENERGY,       /* 8 doubles: energy changes [4] and magnitudes [4] of the last line search */
+  BD_BUF_EXT,          /* external terms, 9N floats per node: gradient xyz, Hessian xx xy xz yy yz zz */
+  BD_BUF_CMETA,        /* contact bounds, 8N floats per node: own (normal xyz, share), reciprocal (dir xyz, share) */
+  BD_BUF_PAIRS,        /* (K + 2) N uint32 per node: K candidate faces (NONE = empty), bound key, row mark */
+  BD_BUF_CELLS,        /* 2 x 2^17 uint32: cell start offsets, then fill counts */
+  BD_BUF_ITEMS,        /* 2N records x 8 floats: centroid now, radius, centroid at start, face (uint32 bits) */
   BD_BUF_COUNT_
 } bd_buffer;
 
@@ -222,6 +239,17 @@ int bd_newton_apply(bd_solver* s, int iteration);
 int bd_finalize(bd_solver* s);
 int bd_damage(bd_solver* s);
 int bd_slip(bd_solver* s);
+
+/* SelfContact.encodeDetect (gpu/selfcontact.ts:139-150): hash the intact faces at the start and
+ * predicted positions, K nearest faces per node, directional and reciprocal bounds (CMETA). */
+int bd_contact_detect(bd_solver* s);
+/* SelfContact.encodeTerms (gpu/selfcontact.ts:153-157): penalty and friction terms at the current
+ * iterate into EXT (node and face corners), trust-region tightening of the node's own bound. */
+int bd_contact_terms(bd_solver* s);
+/* projectStart (step.wgsl:85-95): the predicted step made feasible for the bounds and the floor. */
+int bd_project_start(bd_solver* s);
+/* setFloor(z | null) (gpu/solver.ts:796-801): floor plane at height z when `enabled`, else none. */
+int bd_set_floor(bd_solver* s, int enabled, double z);
 
 /* Per-kernel timing (the reference's timestamp-query profiling, tools/gpu/scripts/profile.ts):
  * while enabled, every launch on the solver's stream is bracketed by HIP events. bd_profile_read

@@ -296,9 +296,80 @@ typedef struct {
   int pcg_iters;
   double pcg_relres;
   double last_alpha, last_e[4], last_m[4];
+  /* external node terms and contact bounds, all off by default (assemble.wgsl:183-215,305-322;
+     energy.wgsl:106-126; step.wgsl:61-112): ext 9 per node (gradient xyz, Hessian xx xy xz yy yz zz),
+     cmeta 8 per node (own and reciprocal bound: direction xyz, largest approach), the floor plane */
+  double* ext;
+  double* cmeta;
+  int floor_on;
+  double floor_z;
 } or_solver;
 
 static inline const bd_material* mat_of(const or_solver* S, int n) { return &S->lay[S->layer_of[n]].material; }
+
+/* ---------------------------------------------------------------- external terms, floor, bounds */
+
+#define FLOOR_BULGE 0.3e-3
+#define FLOOR_K 1300.0
+#define FLOOR_MU 0.35
+#define FLOOR_SPEED 0.002
+#define ACTIVE_BOUND 100.0
+
+/* ext and floor terms of node n at the iterate: adds the gradient to g and the symmetric Hessian
+   [xx xy xz yy yz zz] to H; returns whether the node sits on the floor's hard limit */
+static int external_terms(const or_solver* S, int n, double* g, double* H) {
+  if (S->ext) {
+    for (int q = 0; q < 3; q++) g[q] += S->ext[9 * n + q];
+    for (int q = 0; q < 6; q++) H[q] += S->ext[9 * n + 3 + q];
+  }
+  int core = 0;
+  if (S->floor_on) {
+    double half = 0.5 * mat_of(S, n)->thickness, delta = half + FLOOR_BULGE;
+    double zc = S->x[3 * n + 2] + S->dx[3 * n + 2];
+    double s = delta - (zc - S->floor_z);
+    if (s > 0) {
+      double lam = FLOOR_K * s * s / delta;
+      double ux = S->dx[3 * n], uy = S->dx[3 * n + 1];
+      double kf = FLOOR_MU * lam / fmax(sqrt(ux * ux + uy * uy), FLOOR_SPEED * S->P.h);
+      g[0] += kf * ux;
+      g[1] += kf * uy;
+      g[2] -= lam;
+      H[0] += kf;
+      H[3] += kf;
+      H[5] += 2 * FLOOR_K * s / delta;
+      core = zc <= S->floor_z + half * (1 + 1e-3);
+    }
+  }
+  return core;
+}
+
+/* bound / feasible (step.wgsl:61-77) in FP64 */
+static void bound_d(double* d, const double* m) {
+  double approach = -(d[0] * m[0] + d[1] * m[1] + d[2] * m[2]);
+  if (approach > m[3])
+    for (int q = 0; q < 3; q++) d[q] += (approach - m[3]) * m[q];
+}
+
+/* floorBound (step.wgsl:79-83 with common.wgsl floorLowest, floorWeightDepth) */
+static double floor_bound_z(const or_solver* S, int n, double linz, double dz) {
+  double half = 0.5 * mat_of(S, n)->thickness, delta = half + FLOOR_BULGE;
+  double g = sqrt(S->P.gravity[0] * S->P.gravity[0] + S->P.gravity[1] * S->P.gravity[1] + S->P.gravity[2] * S->P.gravity[2]);
+  double sW = sqrt(S->mass[n] * g * delta / FLOOR_K);
+  double z0 = S->x[3 * n + 2], zl = z0 + linz;
+  double s = fmax(delta - (zl - S->floor_z), 0);
+  double lowest = fmin(fmax(S->floor_z + delta - fmax(sW, 2 * s), S->floor_z + half), zl);
+  return fmax(dz, lowest - z0);
+}
+
+/* the step d of node n (from the iterate step lin) made feasible for its bounds and the floor */
+static void feasible_d(const or_solver* S, int n, const double* lin, double* d) {
+  if (S->cmeta) {
+    bound_d(d, &S->cmeta[8 * n]);
+    bound_d(d, &S->cmeta[8 * n + 4]);
+    bound_d(d, &S->cmeta[8 * n]);
+  }
+  if (S->floor_on) d[2] = floor_bound_z(S, n, lin[2], d[2]);
+}
 
 static inline int seg_of(const or_solver* S, int n, int f, int dir) {
   if (dir > 0) return S->nbr[4 * n + 2 * f + 1] >= 0 ? 2 * n + f : -1;
@@ -518,6 +589,8 @@ void or_node_gradient(or_solver* S, int n, double* g) {
   double w = S->mass[n] / (h * h);
   for (int q = 0; q < 3; q++) g[q] += w * (S->dx[3 * n + q] - S->pred[3 * n + q]);
   grip_term(S, n, g);
+  double Hx[6] = {0, 0, 0, 0, 0, 0};
+  external_terms(S, n, g, Hx);
   double D[9], Dg[3];
   air_drag(S, n, D);
   drag_mul(D, &S->dx[3 * n], Dg);
@@ -592,6 +665,14 @@ void or_energy_change(or_solver* S, const double* d, double* out, double* mag) {
       drag_mul(D, d0, t1);
       drag_mul(D, dn, t2);
       QUAD(dot3(t1, dn), dot3(dn, t2));
+      if (S->ext || S->floor_on) {
+        /* external terms and the floor: quadratic models at this iterate (energy.wgsl:106-126) */
+        double ge[3] = {0, 0, 0}, h6[6] = {0, 0, 0, 0, 0, 0};
+        external_terms(S, n, ge, h6);
+        double Hd[3] = {h6[0] * dn[0] + h6[1] * dn[1] + h6[2] * dn[2], h6[1] * dn[0] + h6[3] * dn[1] + h6[4] * dn[2],
+                        h6[2] * dn[0] + h6[4] * dn[1] + h6[5] * dn[2]};
+        QUAD(dot3(ge, dn), dot3(dn, Hd));
+      }
     }
     const bd_material* mat = mat_of(S, n);
     for (int f = 0; f < 2; f++) {
@@ -859,6 +940,40 @@ static void assemble_fine(or_solver* S, op_t* A) {
       scatter_pair(A, p, q, H);
     }
   }
+  if (!S->ext && !S->cmeta && !S->floor_on) return;
+  /* external node terms and the floor into the node's own block; then active bounds: a node at
+     its bound whose net force still drives it on is held along the bound normal with ACTIVE_BOUND
+     times the trace of its own block (assemble.wgsl:183-215, 305-322) */
+  uint8_t* core = calloc((size_t)(S->N > 0 ? S->N : 1), 1);
+  for (int n = 0; n < S->N; n++) {
+    if (A->fixed[n]) continue;
+    double g[3] = {0, 0, 0}, h6[6] = {0, 0, 0, 0, 0, 0};
+    core[n] = (uint8_t)external_terms(S, n, g, h6);
+    double B[9] = {h6[0], h6[1], h6[2], h6[1], h6[3], h6[4], h6[2], h6[4], h6[5]};
+    add_block(A, n, CENTER, 1, B);
+  }
+  for (int n = 0; n < S->N; n++) {
+    if (A->fixed[n]) continue;
+    const double* D = A->A + ((size_t)n * STENCIL + CENTER) * 9;
+    double kAct = ACTIVE_BOUND * (D[0] + D[4] + D[8]);
+    double g[3];
+    or_node_gradient(S, n, g);
+    for (int b = 0; b < 2 && S->cmeta; b++) {
+      const double* m = &S->cmeta[8 * n + 4 * b];
+      if (m[3] >= 1e8 || (m[0] == 0 && m[1] == 0 && m[2] == 0)) continue;
+      double approach = -dot3(&S->dx[3 * n], m);
+      if (approach >= m[3] - 1e-3 * fabs(m[3]) && dot3(g, m) > 0) {
+        axial_block(kAct, 0, m, H);
+        add_block(A, n, CENTER, 1, H);
+      }
+    }
+    if (core[n] && g[2] > 0) {
+      static const double ez[3] = {0, 0, 1};
+      axial_block(kAct, 0, ez, H);
+      add_block(A, n, CENTER, 1, H);
+    }
+  }
+  free(core);
 }
 
 /* ---------------------------------------------------------------- hierarchy */
@@ -1375,6 +1490,7 @@ void or_destroy(or_solver* S) {
   free(S->x); free(S->v); free(S->dx); free(S->pred); free(S->rest); free(S->softening); free(S->intact);
   free(S->cut_mask); free(S->fail_site); free(S->seg_flags); free(S->tails); free(S->strain);
   free(S->last_update); free(S->nbr); free(S->layer_of); free(S->grip_of); free(S->grip_w); free(S->grip_off);
+  free(S->ext); free(S->cmeta);
   free(S);
 }
 
@@ -1393,6 +1509,33 @@ void or_set_grips(or_solver* S, const bd_grips* G) {
     S->grip_w[n] = G->weight[e];
     for (int q = 0; q < 3; q++) S->grip_off[3 * n + q] = G->offset[3 * e + q];
   }
+}
+
+/* external terms (9 per node, NULL clears), contact bounds (8 per node, NULL clears), the floor */
+void or_set_ext(or_solver* S, const double* ext) {
+  free(S->ext);
+  S->ext = NULL;
+  if (!ext) return;
+  S->ext = malloc(sizeof(double) * 9 * (size_t)S->N);
+  memcpy(S->ext, ext, sizeof(double) * 9 * (size_t)S->N);
+}
+void or_set_cmeta(or_solver* S, const double* cmeta) {
+  free(S->cmeta);
+  S->cmeta = NULL;
+  if (!cmeta) return;
+  S->cmeta = malloc(sizeof(double) * 8 * (size_t)S->N);
+  memcpy(S->cmeta, cmeta, sizeof(double) * 8 * (size_t)S->N);
+}
+void or_set_floor(or_solver* S, int on, double z) {
+  S->floor_on = on;
+  S->floor_z = on ? z : 0;
+}
+
+/* projectStart (step.wgsl:85-95) */
+void or_project_start(or_solver* S) {
+  static const double zero[3] = {0, 0, 0};
+  for (int n = 0; n < S->N; n++)
+    if (!S->pinned[n]) feasible_d(S, n, zero, &S->dx[3 * n]);
 }
 
 /* predict (solver.ts:602-613) */
@@ -1476,10 +1619,28 @@ double or_newton_step(or_solver* S, int first) {
   }
   S->last_alpha = alpha;
   double m = 0;
-  for (int q = 0; q < 3 * N; q++) {
-    d[q] *= alpha;
-    S->dx[q] += d[q];
-    if (fabs(d[q]) > m) m = fabs(d[q]);
+  if (S->cmeta || S->floor_on) {
+    /* newtonApply (step.wgsl:97-112): the updated step made feasible for the bounds and the floor */
+    for (int n = 0; n < N; n++) {
+      if (S->pinned[n]) {
+        d[3 * n] = d[3 * n + 1] = d[3 * n + 2] = 0;
+        continue;
+      }
+      double old[3] = {S->dx[3 * n], S->dx[3 * n + 1], S->dx[3 * n + 2]}, nd[3];
+      for (int q = 0; q < 3; q++) nd[q] = old[q] + alpha * d[3 * n + q];
+      feasible_d(S, n, old, nd);
+      for (int q = 0; q < 3; q++) {
+        d[3 * n + q] = nd[q] - old[q];
+        S->dx[3 * n + q] = nd[q];
+        if (fabs(d[3 * n + q]) > m) m = fabs(d[3 * n + q]);
+      }
+    }
+  } else {
+    for (int q = 0; q < 3 * N; q++) {
+      d[q] *= alpha;
+      S->dx[q] += d[q];
+      if (fabs(d[q]) > m) m = fabs(d[q]);
+    }
   }
   for (int n = 0; n < N; n++) S->last_update[n] = norm3(&d[3 * n]);
   free(b);

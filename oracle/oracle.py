@@ -31,6 +31,13 @@ STENCIL = 25
 CENTER = 12
 
 
+class OcScene(C.Structure):
+    """oc_scene of contact_oracle.c: node-major FP32 arrays of one scene state."""
+    _fields_ = [("N", C.c_int), ("nl", C.c_int), ("nx", C.c_int * 8), ("ny", C.c_int * 8), ("off", C.c_int * 8),
+                ("x", C.POINTER(C.c_float)), ("dx", C.POINTER(C.c_float)), ("flags", C.POINTER(C.c_uint16)),
+                ("pinned", C.POINTER(C.c_uint8)), ("mass", C.POINTER(C.c_float))]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
 
@@ -55,6 +62,32 @@ def lib():
                      "or_pcg_iterations"):
             getattr(L, name).argtypes = [vp]
             getattr(L, name).restype = C.c_int
+        L.or_set_ext.argtypes = [vp, dp]
+        L.or_set_cmeta.argtypes = [vp, dp]
+        L.or_set_floor.argtypes = [vp, C.c_int, C.c_double]
+        L.or_project_start.argtypes = [vp]
+        # FP32 self-contact restatement (contact_oracle.c)
+        fp = P(C.c_float)
+        up = P(C.c_uint32)
+        L.oc_detect.argtypes = [P(OcScene), up, fp, up, fp]
+        L.oc_detect_brute.argtypes = [P(OcScene), up, fp]
+        L.oc_terms.argtypes = [P(OcScene), up, fp, C.c_float, C.c_float, P(C.c_int32), fp]
+        L.oc_closest_tri.argtypes = [fp, fp, fp, fp, fp]
+        L.oc_closest_tri.restype = C.c_float
+        L.oc_face_hit.argtypes = [fp, fp, fp, fp, fp, fp, fp]
+        L.oc_face_hit.restype = C.c_float
+        L.oc_seg_tri.argtypes = [fp, fp, fp, fp, fp]
+        L.oc_seg_tri.restype = C.c_float
+        L.oc_seg_dist.argtypes = [fp, fp, fp]
+        L.oc_seg_dist.restype = C.c_float
+        L.oc_hash_cell.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.oc_hash_cell.restype = C.c_uint32
+        L.oc_face_valid.argtypes = [P(OcScene), C.c_uint32]
+        L.oc_face_valid.restype = C.c_int
+        L.oc_feasible.argtypes = [fp, fp]
+        L.oc_floor_bound_z.argtypes = [C.c_float] * 7
+        L.oc_floor_bound_z.restype = C.c_float
+        L.oc_project_start.argtypes = [P(OcScene), fp, fp, C.c_int, C.c_float, fp, C.c_float]
         L.or_rhs.argtypes = [vp, dp]
         L.or_node_gradient.argtypes = [vp, C.c_int, dp]
         L.or_potential.argtypes = [vp]
@@ -254,6 +287,23 @@ class Oracle:
     @property
     def pcg_relres(self): return lib().or_pcg_relres(self.h)
 
+    # ---- external terms, contact bounds, floor (FP64 restatement of their use)
+    def set_ext(self, ext):
+        """(N, 9) gradient + Hessian [xx xy xz yy yz zz] per node, or None."""
+        a = None if ext is None else np.ascontiguousarray(ext, dtype=np.float64).ravel()
+        lib().or_set_ext(self.h, _dp(a) if a is not None else None)
+
+    def set_cmeta(self, cmeta):
+        """(N, 2, 4) own and reciprocal bounds, or None."""
+        a = None if cmeta is None else np.ascontiguousarray(cmeta, dtype=np.float64).ravel()
+        lib().or_set_cmeta(self.h, _dp(a) if a is not None else None)
+
+    def set_floor(self, z):
+        lib().or_set_floor(self.h, 0 if z is None else 1, 0.0 if z is None else float(z))
+
+    def project_start(self):
+        lib().or_project_start(self.h)
+
     def rhs(self):
         b = np.zeros(3 * self.N)
         lib().or_rhs(self.h, _dp(b))
@@ -335,3 +385,119 @@ class Oracle:
         x = np.zeros_like(b) if x is None else np.ascontiguousarray(x, dtype=np.float64).ravel().copy()
         lib().or_pcg(self.h, _dp(b), _dp(x), iterations, rel_tol, max_iter)
         return x
+
+
+class ContactTwin:
+    """The FP32 restatement of the reference's self-contact (contact_oracle.c) on one scene state:
+    x, dx (N, 3) float32 and the span flags; K = 4 candidates per node."""
+
+    K = 4
+    NONE = 0xFFFFFFFF
+    TABLE = 1 << 17
+
+    def __init__(self, scene, x, dx, flags=None, pinned=None, mass=None):
+        self.N = scene.node_count
+        L = scene.layers
+        self.keep = {
+            "x": np.ascontiguousarray(x, dtype=np.float32).reshape(-1),
+            "dx": np.ascontiguousarray(dx, dtype=np.float32).reshape(-1),
+            "flags": np.ascontiguousarray(scene.seg_flags if flags is None else flags, dtype=np.uint16),
+            "pinned": np.ascontiguousarray(scene.pinned if pinned is None else pinned, dtype=np.uint8),
+            "mass": np.ascontiguousarray(scene.mass if mass is None else mass, dtype=np.float32),
+        }
+        k = self.keep
+        S = OcScene()
+        S.N = self.N
+        S.nl = len(L)
+        for i, lay in enumerate(L):
+            S.nx[i], S.ny[i], S.off[i] = lay.nx, lay.ny, lay.node_offset
+        S.x = k["x"].ctypes.data_as(C.POINTER(C.c_float))
+        S.dx = k["dx"].ctypes.data_as(C.POINTER(C.c_float))
+        S.flags = k["flags"].ctypes.data_as(C.POINTER(C.c_uint16))
+        S.pinned = k["pinned"].ctypes.data_as(C.POINTER(C.c_uint8))
+        S.mass = k["mass"].ctypes.data_as(C.POINTER(C.c_float))
+        self.S = S
+        self.half = np.array([0.5 * lay.material.get("thickness", 0.0) for lay in L] + [0.0] * (8 - len(L)), dtype=np.float32)
+
+    def detect(self):
+        """faceCount .. resolveBounds: returns cells (2T,), items (2N, 8), pairs (N, K+2), cmeta (N, 2, 4)."""
+        cells = np.zeros(2 * self.TABLE, dtype=np.uint32)
+        items = np.zeros((2 * self.N, 8), dtype=np.float32)
+        pairs = np.zeros((self.N, self.K + 2), dtype=np.uint32)
+        cmeta = np.zeros((self.N, 2, 4), dtype=np.float32)
+        lib().oc_detect(C.byref(self.S), _up(cells), _fp(items), _up(pairs), _fp(cmeta))
+        return cells, items, pairs, cmeta
+
+    def detect_brute(self):
+        pairs = np.zeros((self.N, self.K + 2), dtype=np.uint32)
+        cmeta = np.zeros((self.N, 2, 4), dtype=np.float32)
+        lib().oc_detect_brute(C.byref(self.S), _up(pairs), _fp(cmeta))
+        return pairs, cmeta
+
+    def terms(self, pairs, cmeta, h, gnorm):
+        """terms + gather: returns (acc (N, 9) int32, ext (N, 9) float32); cmeta own bounds tightened in place."""
+        acc = np.zeros((self.N, 9), dtype=np.int32)
+        ext = np.zeros((self.N, 9), dtype=np.float32)
+        pr = np.ascontiguousarray(pairs, dtype=np.uint32)
+        assert cmeta.dtype == np.float32 and cmeta.flags.c_contiguous
+        lib().oc_terms(C.byref(self.S), _up(pr), _fp(cmeta), C.c_float(h), C.c_float(gnorm),
+                       acc.ctypes.data_as(C.POINTER(C.c_int32)), _fp(ext))
+        return acc, ext
+
+    def project_start(self, cmeta, floor_z=None, gnorm=9.81):
+        """projectStart on a copy of dx: returns the feasible predicted step (N, 3)."""
+        dx = self.keep["dx"].copy()
+        lib().oc_project_start(C.byref(self.S), _fp(np.ascontiguousarray(cmeta, dtype=np.float32)), _fp(dx),
+                               0 if floor_z is None else 1, C.c_float(0.0 if floor_z is None else floor_z),
+                               _fp(self.half), C.c_float(gnorm))
+        return dx.reshape(-1, 3)
+
+    def face_valid(self, f):
+        return bool(lib().oc_face_valid(C.byref(self.S), int(f)))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _up(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _f3(v):
+    return np.ascontiguousarray(v, dtype=np.float32)
+
+
+def closest_tri(p, a, b, c):
+    w = np.zeros(3, dtype=np.float32)
+    d2 = lib().oc_closest_tri(_fp(_f3(p)), _fp(_f3(a)), _fp(_f3(b)), _fp(_f3(c)), _fp(w))
+    return w, d2
+
+
+def face_hit(p, p00, p10, p01, p11):
+    w = np.zeros(4, dtype=np.float32)
+    q = np.zeros(3, dtype=np.float32)
+    d = lib().oc_face_hit(_fp(_f3(p)), _fp(_f3(p00)), _fp(_f3(p10)), _fp(_f3(p01)), _fp(_f3(p11)), _fp(w), _fp(q))
+    return w, q, d
+
+
+def seg_tri(o, d, a, b, c):
+    return lib().oc_seg_tri(_fp(_f3(o)), _fp(_f3(d)), _fp(_f3(a)), _fp(_f3(b)), _fp(_f3(c)))
+
+
+def seg_dist(q, a, b):
+    return lib().oc_seg_dist(_fp(_f3(q)), _fp(_f3(a)), _fp(_f3(b)))
+
+
+def hash_cell(cx, cy, cz):
+    return lib().oc_hash_cell(cx, cy, cz)
+
+
+def feasible(cmeta_n, d):
+    dd = _f3(d).copy()
+    lib().oc_feasible(_fp(np.ascontiguousarray(cmeta_n, dtype=np.float32).ravel()), _fp(dd))
+    return dd
+
+
+def floor_bound_z(z0, linz, dz, half, mass, gnorm, fz):
+    return lib().oc_floor_bound_z(z0, linz, dz, half, mass, gnorm, fz)

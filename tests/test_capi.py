@@ -30,10 +30,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_sizes_match_header_layout():
-    # bd_yarn: 10 doubles + 2 ints; bd_material: 2 + 2 yarns + 5 doubles
+    # bd_yarn: 10 doubles + 2 ints; bd_material: 2 + 2 yarns + 6 doubles
     assert C.sizeof(capi.bd_yarn) == 88
-    assert C.sizeof(capi.bd_material) == 2 * 8 + 2 * 88 + 5 * 8
-    assert C.sizeof(capi.bd_params) == 8 * 4 + 4 * 2 + 8 + 4 * 2 + 8 + 4 * 6
+    assert C.sizeof(capi.bd_material) == 2 * 8 + 2 * 88 + 6 * 8
+    assert C.sizeof(capi.bd_params) == 8 * 4 + 4 * 2 + 8 + 4 * 2 + 8 + 4 * 8
 
 
 def _create(scene, params=None):
@@ -50,6 +50,15 @@ def test_create_rejects_lines_beyond_the_limit():
     S = sheet_scene(capi.BD_MAX_LINE + 1, 1e-3, mat)
     rc, msg, _ = _create(S)
     assert rc == -3 and "BD_MAX_LINE" in msg
+
+
+def test_create_rejects_self_contact_beyond_the_key_range():
+    # the reciprocal-bound key holds node + 1 in 20 bits (gpu/selfcontact.ts:84)
+    mat = linear_sheet_material(1e-3, 2e4, 1.6, 1e-5, 0.35, 8e3)
+    S = sheet_scene(1024, 1e-3, mat, perturbation=0.0)
+    assert S.node_count == capi.BD_CONTACT_MAX_NODES + 1
+    rc, msg, _ = _create(S, capi.make_params(self_contact=True))
+    assert rc == -3 and "2^20" in msg
 
 
 def test_create_rejects_inconsistent_scene():
