@@ -560,7 +560,6 @@ void launch_predict(Solver& s) {
 void launch_assemble(Solver& s) {
   // reads x, dx, pred, mass, cn0, pinned, spans (rest, intact, cut, flags), grip id: 97 B;
   // writes the 32-plane row and the rhs: 140 B
-  ProfScope ps(s, PC_ASSEMBLE, s.N * 237.0);
   AsmArgs A;
   A.c = NodeCtx{s.x, s.dx, s.pred, s.mass, s.cn0, s.pinned, s.rest, s.intact, s.cut_mask, s.flags, s.Np};
   A.layers = s.layers;
@@ -577,7 +576,11 @@ void launch_assemble(Solver& s) {
   A.cmeta = s.P.self_contact ? s.cmeta : nullptr;
   A.floor_on = s.floor_on ? 1 : 0;
   A.floor_z = s.floor_z;
-  k_assemble<<<grid_for(s.N, 256), 256, 0, s.stream>>>(A);
+  {
+    ProfScope ps(s, PC_ASSEMBLE, s.N * 237.0);
+    k_assemble<<<grid_for(s.N, 256), 256, 0, s.stream>>>(A);
+  }
+  launch_colmajor_op(s, 0);
 }
 
 void launch_newton_apply(Solver& s, int it) {

@@ -60,7 +60,16 @@ struct Level {
   int4* lines = nullptr;
   int color_start[7] = {0};
   int max_len = 0;
+  // dual layout (lines longer than 128 nodes): column-major copies (node (i, j) of a layer at
+  // off + i * ny + j) of the operator, fixed flags and iterate for the v-line passes; the v-line
+  // entries of `lines` then start in that layout with stride 1
+  bool dual = false;
+  float* opT = nullptr;
+  uint8_t* fixedT = nullptr;
+  float* xT = nullptr;
 };
+
+constexpr int kDualMinLine = 129;
 
 struct Scal {
   // PCG and line-search scalars kept on the device
@@ -83,13 +92,13 @@ struct Scal {
 enum ProfClass {
   PC_PREDICT_COMMIT, PC_ASSEMBLE, PC_PROLONG_WEIGHTS, PC_RAP, PC_SMOOTH_FINE, PC_SMOOTH_COARSE, PC_RESIDUAL_FINE,
   PC_RESIDUAL_COARSE, PC_RESTRICT, PC_PROLONG, PC_MATVEC, PC_PCG_VECTOR, PC_TRUE_RESIDUAL, PC_ENERGY, PC_DAMAGE, PC_SLIP,
-  PC_CONTACT_HASH, PC_CONTACT_DETECT, PC_CONTACT_TERMS, PC_COUNT
+  PC_CONTACT_HASH, PC_CONTACT_DETECT, PC_CONTACT_TERMS, PC_TRANSPOSE, PC_COUNT
 };
 inline const char* prof_name(int c) {
   static const char* n[PC_COUNT] = {"predict_commit", "assemble", "prolong_weights", "rap", "smooth_fine", "smooth_coarse",
                                     "residual_fine", "residual_coarse", "restrict", "prolong", "matvec_dot", "pcg_vector",
                                     "true_residual", "energy", "damage", "slip", "contact_hash", "contact_detect",
-                                    "contact_terms"};
+                                    "contact_terms", "transpose"};
   return n[c];
 }
 
@@ -187,6 +196,9 @@ void launch_restrict(Solver& s, int k);
 void launch_prolong_add(Solver& s, int k);
 void launch_smooth(Solver& s, int k, int reverse);
 void launch_zero(Solver& s, float* p, size_t n);
+// column-major copies of a dual-layout level: operator and fixed flags, or the iterate
+void launch_colmajor_op(Solver& s, int k);
+void launch_colmajor_x(Solver& s, int k);
 void vcycle(Solver& s, int k);
 // pcg.hip
 void pcg_solve(Solver& s);

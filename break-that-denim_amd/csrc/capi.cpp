@@ -132,7 +132,8 @@ void build_lines(Solver& s, Level& L) {
   int maxlen = 0;
   for (int l = 0; l < L.geo.nl; l++) {
     int nx = L.geo.nx[l], ny = L.geo.ny[l], off = L.geo.off[l];
-    for (int i = 0; i < nx; i++) by[i % 3].push_back(make_int4(off + i, nx, ny, 1));
+    for (int i = 0; i < nx; i++)
+      by[i % 3].push_back(L.dual ? make_int4(off + i * ny, 1, ny, 1) : make_int4(off + i, nx, ny, 1));
     for (int j = 0; j < ny; j++) by[3 + j % 3].push_back(make_int4(off + j * nx, 1, nx, 0));
     maxlen = std::max(maxlen, std::max(nx, ny));
   }
@@ -163,6 +164,9 @@ void destroy_all(Solver& s) {
     dfree(L.r);
     dfree(L.pw);
     dfree(L.lines);
+    dfree(L.opT);
+    dfree(L.fixedT);
+    dfree(L.xT);
   }
   if (s.scal_h) (void)hipHostFree(s.scal_h);
   if (s.grip_goal_h) (void)hipHostFree(s.grip_goal_h);
@@ -190,6 +194,7 @@ void build_hierarchy(Solver& s) {
   for (int k = 0; k + 1 < s.nlev; k++) {
     launch_prolong_weights(s, k);
     launch_rap(s, k);
+    launch_colmajor_op(s, k + 1);
   }
   s.hierarchy = true;
 }
@@ -374,6 +379,15 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
       }
       L.r = dalloc<float>(3 * (size_t)L.Np);
       if (k + 1 < s.nlev) L.pw = dalloc<float>(4 * (size_t)L.Np);
+      int longest = 0;
+      for (int l = 0; l < L.geo.nl; l++) longest = std::max(longest, std::max(L.geo.nx[l], L.geo.ny[l]));
+      L.dual = longest >= kDualMinLine;
+      if (L.dual) {
+        L.opT = dalloc<float>((size_t)(L.fine ? kFineComps : kCoarseComps) * L.Np);
+        L.fixedT = dalloc<uint8_t>(L.Np);
+        L.xT = dalloc<float>(3 * (size_t)L.Np);
+        BD_HIP(hipMemsetAsync(L.xT, 0, 3 * (size_t)L.Np * sizeof(float), s.stream));
+      }
       build_lines(s, L);
     }
     smooth_set_attributes();
@@ -535,6 +549,7 @@ int bd_vcycle(bd_solver* h) {
     if (!s.hierarchy) return fail(h, BD_ERR_STATE, "bd_vcycle: assemble first");
     BD_HIP(hipMemsetAsync(&s.scal->done, 0, sizeof(int), s.stream));
     launch_zero(s, s.lv[0].x, 3 * (size_t)s.lv[0].Np);
+    if (s.lv[0].dual) launch_zero(s, s.lv[0].xT, 3 * (size_t)s.lv[0].Np);
     vcycle(s, 0);
     return BD_OK;
   });
@@ -545,6 +560,7 @@ int bd_smooth(bd_solver* h, int level, int reverse) {
     if (!s.hierarchy) return fail(h, BD_ERR_STATE, "bd_smooth: assemble first");
     if (level < 0 || level >= s.nlev) return fail(h, BD_ERR_INVALID, "bd_smooth: level out of range");
     BD_HIP(hipMemsetAsync(&s.scal->done, 0, sizeof(int), s.stream));
+    launch_colmajor_x(s, level);
     launch_smooth(s, level, reverse);
     return BD_OK;
   });
@@ -917,7 +933,10 @@ int bd_write(bd_solver* h, bd_buffer buf, int level, const void* src, size_t byt
         break;
       }
       case BD_BUF_RHS: upload_vec3(s, s.rhs, s.N, s.Np, d); break;
-      case BD_BUF_LEVEL_X: upload_vec3(s, s.lv[level].x, s.lv[level].N, s.lv[level].Np, d); break;
+      case BD_BUF_LEVEL_X:
+        upload_vec3(s, s.lv[level].x, s.lv[level].N, s.lv[level].Np, d);
+        launch_colmajor_x(s, level);
+        break;
       case BD_BUF_LEVEL_B: upload_vec3(s, s.lv[level].b, s.lv[level].N, s.lv[level].Np, d); break;
       case BD_BUF_REST: upload(s, s.rest, d, 2 * (size_t)s.N); break;
       case BD_BUF_INTACT: upload(s, s.intact, d, 2 * (size_t)s.N); break;

@@ -286,6 +286,8 @@ struct XferArgs {
   float* xc;       // coarse solution
   float* xf;       // fine solution
   const int* gate;
+  float* xcT;      // column-major coarse solution (dual-layout coarse level) or null
+  float* xfT;      // column-major fine solution (dual-layout fine level) or null
 };
 
 __global__ __launch_bounds__(256) void k_restrict(XferArgs X) {
@@ -316,6 +318,7 @@ __global__ __launch_bounds__(256) void k_restrict(XferArgs X) {
   for (int q = 0; q < 3; q++) {
     X.bc[q * Npc + I] = s[q];
     X.xc[q * Npc + I] = 0.f;
+    if (X.xcT) X.xcT[q * Npc + I] = 0.f;  // every entry of the permuted copy is cleared as well
   }
 }
 
@@ -342,12 +345,46 @@ __global__ __launch_bounds__(256) void k_prolong_add(XferArgs X) {
       s[1] += w * X.xc[Npc + c];
       s[2] += w * X.xc[2 * Npc + c];
     }
-  for (int q = 0; q < 3; q++) X.xf[q * Np + f] += s[q];
+  const int fT = X.a.geo.off[l] + i * ny + j;
+  for (int q = 0; q < 3; q++) {
+    const float v = X.xf[q * Np + f] + s[q];
+    X.xf[q * Np + f] = v;
+    if (X.xfT) X.xfT[q * Np + fT] = v;
+  }
 }
 
 __global__ void k_zero(float* p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.f;
+}
+
+// row-major (off + j nx + i) -> column-major (off + i ny + j) copy of `planes` planes of one layer,
+// through 32 x 32 tiles in LDS so both sides are coalesced
+template <class T>
+__global__ __launch_bounds__(256) void k_to_colmajor(const T* in, T* out, int Np, int off, int nx, int ny) {
+  __shared__ T tile[32][33];
+  const size_t q = blockIdx.z;
+  const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const T* src = in + q * Np + off;
+  T* dst = out + q * Np + off;
+  for (int r = threadIdx.y; r < 32; r += 8) {
+    const int i = i0 + threadIdx.x, j = j0 + r;
+    if (i < nx && j < ny) tile[r][threadIdx.x] = src[j * nx + i];
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += 8) {
+    const int j = j0 + threadIdx.x, i = i0 + r;
+    if (i < nx && j < ny) dst[i * ny + j] = tile[threadIdx.x][r];
+  }
+}
+
+template <class T>
+void to_colmajor(Solver& s, const Level& L, const T* in, T* out, int planes) {
+  for (int l = 0; l < L.geo.nl; l++) {
+    const int nx = L.geo.nx[l], ny = L.geo.ny[l];
+    dim3 grid((nx + 31) / 32, (ny + 31) / 32, planes);
+    k_to_colmajor<T><<<grid, dim3(32, 8), 0, s.stream>>>(in, out, L.Np, L.geo.off[l], nx, ny);
+  }
 }
 
 LvArgs lv_args(const Level& L) { return LvArgs{L.op, L.fixed, L.N, L.Np, L.geo}; }
@@ -359,6 +396,21 @@ const int* pcg_gate(Solver& s) { return &s.scal->done; }
 void launch_zero(Solver& s, float* p, size_t n) {
   if (n == 0) return;
   k_zero<<<(unsigned)((n + 255) / 256), 256, 0, s.stream>>>(p, n);
+}
+
+void launch_colmajor_op(Solver& s, int k) {
+  Level& L = s.lv[k];
+  if (!L.dual) return;
+  const int comps = L.fine ? kFineComps : kCoarseComps;
+  ProfScope ps(s, PC_TRANSPOSE, 2.0 * L.N * (4.0 * comps + 1.0));
+  to_colmajor<float>(s, L, L.op, L.opT, comps);
+  to_colmajor<uint8_t>(s, L, L.fixed, L.fixedT, 1);
+}
+
+void launch_colmajor_x(Solver& s, int k) {
+  Level& L = s.lv[k];
+  if (!L.dual) return;
+  to_colmajor<float>(s, L, L.x, L.xT, 3);
 }
 
 void launch_prolong_weights(Solver& s, int k) {
@@ -391,16 +443,18 @@ void launch_residual(Solver& s, int k, const float* x, const float* b, float* r)
 void launch_restrict(Solver& s, int k) {
   Level& L = s.lv[k];
   Level& Cl = s.lv[k + 1];
-  XferArgs X{lv_args(L), Cl.geo, Cl.N, Cl.Np, L.pw, Cl.fixed, L.r, Cl.b, Cl.x, L.x, pcg_gate(s)};
-  ProfScope ps(s, PC_RESTRICT, L.N * 29.0 + Cl.N * 25.0);
+  XferArgs X{lv_args(L), Cl.geo, Cl.N, Cl.Np, L.pw, Cl.fixed, L.r, Cl.b, Cl.x, L.x, pcg_gate(s),
+             Cl.dual ? Cl.xT : nullptr, nullptr};
+  ProfScope ps(s, PC_RESTRICT, L.N * 29.0 + Cl.N * (Cl.dual ? 37.0 : 25.0));
   k_restrict<<<grid_for(Cl.N, 256), 256, 0, s.stream>>>(X);
 }
 
 void launch_prolong_add(Solver& s, int k) {
   Level& L = s.lv[k];
   Level& Cl = s.lv[k + 1];
-  XferArgs X{lv_args(L), Cl.geo, Cl.N, Cl.Np, L.pw, Cl.fixed, L.r, Cl.b, Cl.x, L.x, pcg_gate(s)};
-  ProfScope ps(s, PC_PROLONG, L.N * 41.0 + Cl.N * 12.0);
+  XferArgs X{lv_args(L), Cl.geo, Cl.N, Cl.Np, L.pw, Cl.fixed, L.r, Cl.b, Cl.x, L.x, pcg_gate(s),
+             nullptr, L.dual ? L.xT : nullptr};
+  ProfScope ps(s, PC_PROLONG, L.N * (L.dual ? 53.0 : 41.0) + Cl.N * 12.0);
   k_prolong_add<<<grid_for(L.N, 256), 256, 0, s.stream>>>(X);
 }
 

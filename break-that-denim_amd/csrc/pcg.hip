@@ -68,6 +68,7 @@ struct PcgArgs {
   float *r, *z, *p, *ap;
   double* delta;
   const float* rhs;
+  float* zT;  // column-major copy of z (dual-layout fine level) or null
 };
 
 __global__ __launch_bounds__(256) void k_pcg_start(PcgArgs A) {
@@ -79,6 +80,7 @@ __global__ __launch_bounds__(256) void k_pcg_start(PcgArgs A) {
       float b = fx ? 0.f : A.rhs[q * Np + n];
       A.r[q * Np + n] = b;
       A.z[q * Np + n] = 0.f;
+      if (A.zT) A.zT[q * Np + n] = 0.f;
       A.delta[q * Np + n] = 0.0;
       A.p[q * Np + n] = 0.f;
       A.ap[q * Np + n] = 0.f;
@@ -138,6 +140,7 @@ __global__ __launch_bounds__(256) void k_update_xr(PcgArgs A) {
       float r = A.r[q * Np + n] - alpha * A.ap[q * Np + n];
       A.r[q * Np + n] = r;
       A.z[q * Np + n] = 0.f;
+      if (A.zT) A.zT[q * Np + n] = 0.f;
       v[0] += (double)r * r;
     }
   double tot[1];
@@ -592,7 +595,7 @@ __global__ void k_pcg_setup(Scal* sc, double tol, int max_iters, int fixed_iters
 
 PcgArgs pcg_args(Solver& s) {
   Level& L = s.lv[0];
-  return PcgArgs{s.N, s.Np, L.fixed, s.scal, s.partials, L.b, L.x, s.p, s.ap, s.delta, s.rhs};
+  return PcgArgs{s.N, s.Np, L.fixed, s.scal, s.partials, L.b, L.x, s.p, s.ap, s.delta, s.rhs, L.dual ? L.xT : nullptr};
 }
 
 int dot_grid(int N) {

@@ -14,6 +14,12 @@
 // remaining log2(len) - 1 levels by block cyclic reduction over the kept rows in LDS (27 floats per
 // kept row: 55 KB for a 1024-node line, two workgroups per CU). Every pivot is shifted by
 // PIVOT_SHIFT of its largest diagonal entry, as in the reference.
+//
+// Levels with lines longer than 128 nodes also keep their operator, fixed flags and iterate in a
+// column-major copy (node (i, j) of a layer at off + i * ny + j): v-line passes read that copy, so
+// consecutive rows of a v-line are consecutive in memory and the gathers coalesce as they do for
+// u-lines. Each pass writes its solution to both copies (the other one with a 12-byte strided
+// store per row) and reads b from the row-major vector.
 #include "bd_internal.h"
 
 namespace bd {
@@ -21,15 +27,22 @@ namespace bd {
 namespace {
 
 struct SmoothArgs {
-  const float* op;
-  const uint8_t* fixed;
+  const float* op;       // in the pass's layout (column-major for v-lines of a dual-layout level)
+  const uint8_t* fixed;  // same layout
   int Np;
-  const int4* lines;  // (start, stride, length, family)
-  float* x;
-  const float* b;
+  const int4* lines;     // (start, stride, length, family), start in the pass's layout
+  float* x;              // same layout
+  const float* b;        // row-major
+  float* xo;             // the other layout's copy of x (dual-layout levels), or null
   const int* gate;
   Geo geo;
 };
+
+// index strides of lattice offsets (di, dj) in the pass's layout
+template <int FAM, bool DUAL>
+__device__ __forceinline__ int stride_i(int nx, int ny) { return (FAM == 1 && DUAL) ? ny : 1; }
+template <int FAM, bool DUAL>
+__device__ __forceinline__ int stride_j(int nx, int ny) { return (FAM == 1 && DUAL) ? 1 : nx; }
 
 __device__ __forceinline__ void sym_full(const float* s, float* o) {
   o[0] = s[0]; o[1] = s[1]; o[2] = s[2];
@@ -101,27 +114,28 @@ __device__ constexpr Cpl kFine[12] = {
 
 // Gathers D (6), U (9: +1 along coupling, full), r (3) of row n on a line of family FAM.
 // All loads use clamped in-bounds indices and are masked afterwards, so they issue together.
-template <bool FINE, int FAM>
-__device__ __forceinline__ void gather_row(const SmoothArgs& S, int n, int i, int j, int nx, int ny, float D[6], float U[9],
-                                           float r[3], bool& fixed_n) {
+template <bool FINE, int FAM, bool DUAL>
+__device__ __forceinline__ void gather_row(const SmoothArgs& S, int n, int nb, int i, int j, int nx, int ny, float D[6],
+                                           float U[9], float r[3], bool& fixed_n) {
   const int Np = S.Np;
   const float* op = S.op;
   const float* x = S.x;
+  const int si = stride_i<FAM, DUAL>(nx, ny), sj = stride_j<FAM, DUAL>(nx, ny);
   fixed_n = S.fixed[n] != 0;
   float c[6];
 #pragma unroll
   for (int q = 0; q < 6; q++) c[q] = op[q * Np + n];
   const float xn0 = x[n], xn1 = x[Np + n], xn2 = x[2 * Np + n];
-  r[0] = S.b[n];
-  r[1] = S.b[Np + n];
-  r[2] = S.b[2 * Np + n];
+  r[0] = S.b[nb];
+  r[1] = S.b[Np + nb];
+  r[2] = S.b[2 * Np + nb];
 #pragma unroll
   for (int q = 0; q < 9; q++) U[q] = 0.f;
   float maj = 0.f;
   auto one = [&](int di, int dj, int sdi, int sdj, int base, bool scalar, bool neg_src) {
     const bool in = i + di >= 0 && i + di < nx && j + dj >= 0 && j + dj < ny;
-    const int m = in ? n + dj * nx + di : n;
-    const int src = in ? n + sdj * nx + sdi : n;
+    const int m = in ? n + dj * sj + di * si : n;
+    const int src = in ? n + sdj * sj + sdi * si : n;
     float blk[6];
     if (scalar) {
       float s = op[base * Np + src];
@@ -161,9 +175,9 @@ __device__ __forceinline__ void gather_row(const SmoothArgs& S, int n, int i, in
     D[0] = 1.f; D[1] = 0.f; D[2] = 0.f; D[3] = 1.f; D[4] = 0.f; D[5] = 1.f;
 #pragma unroll
     for (int q = 0; q < 9; q++) U[q] = 0.f;
-    r[0] = S.b[n];
-    r[1] = S.b[Np + n];
-    r[2] = S.b[2 * Np + n];
+    r[0] = S.b[nb];
+    r[1] = S.b[Np + nb];
+    r[2] = S.b[2 * Np + nb];
     return;
   }
 #pragma unroll
@@ -178,17 +192,18 @@ __device__ __forceinline__ void gather_row(const SmoothArgs& S, int n, int i, in
 
 // Partial row gather of one (row, g) task: the couplings c with c % G == g. Writes the partial
 // rhs correction and majorisation to LDS; the task owning the +1 along-line coupling writes U.
-template <bool FINE, int FAM, int G>
+template <bool FINE, int FAM, int G, bool DUAL>
 __device__ __forceinline__ void gather_task(const SmoothArgs& S, int n, int i, int j, int nx, int ny, int g, float* pr, float* pm,
                                             float* gU, int row, int len) {
   const int Np = S.Np;
   const float* op = S.op;
   const float* x = S.x;
+  const int si = stride_i<FAM, DUAL>(nx, ny), sj = stride_j<FAM, DUAL>(nx, ny);
   float r0 = 0.f, r1 = 0.f, r2 = 0.f, maj = 0.f;
   auto one = [&](int di, int dj, int sdi, int sdj, int base, bool scalar) {
     const bool in = i + di >= 0 && i + di < nx && j + dj >= 0 && j + dj < ny;
-    const int m = in ? n + dj * nx + di : n;
-    const int src = in ? n + sdj * nx + sdi : n;
+    const int m = in ? n + dj * sj + di * si : n;
+    const int src = in ? n + sdj * sj + sdi * si : n;
     float blk[6];
     if (scalar) {
       float s = op[base * Np + src];
@@ -460,63 +475,87 @@ __device__ __forceinline__ void set_identity(float D[6], float U[9], float r[3])
 
 struct LineGeo {
   int4 ln;
-  int nx, ny, i0, j0;
+  int nx, ny, off, i0, j0;
 };
 
+template <int FAM, bool DUAL>
 __device__ __forceinline__ LineGeo line_geo(const SmoothArgs& S) {
   LineGeo g;
   g.ln = S.lines[xcd_block(blockIdx.x, gridDim.x)];
   int l = geo_layer(S.geo, g.ln.x);
   g.nx = S.geo.nx[l];
   g.ny = S.geo.ny[l];
-  int r0 = g.ln.x - S.geo.off[l];
-  g.j0 = r0 / g.nx;
-  g.i0 = r0 - g.j0 * g.nx;
+  g.off = S.geo.off[l];
+  int r0 = g.ln.x - g.off;
+  if (FAM == 1 && DUAL) {
+    g.i0 = r0 / g.ny;
+    g.j0 = r0 - g.i0 * g.ny;
+  } else {
+    g.j0 = r0 / g.nx;
+    g.i0 = r0 - g.j0 * g.nx;
+  }
   return g;
 }
 
+// row k of the line: its index in the pass's layout, in the row-major layout (b), and in the other
+// layout (the second copy of x on dual-layout levels)
+template <int FAM, bool DUAL>
+__device__ __forceinline__ void row_index(const LineGeo& L, int k, int& n, int& nb, int& no, int& i, int& j) {
+  n = L.ln.x + k * L.ln.y;
+  i = FAM == 0 ? L.i0 + k : L.i0;
+  j = FAM == 0 ? L.j0 : L.j0 + k;
+  const int rm = L.off + j * L.nx + i, cm = L.off + i * L.ny + j;
+  nb = (FAM == 1 && DUAL) ? rm : n;
+  no = DUAL ? (FAM == 1 ? rm : cm) : n;
+}
+
+__device__ __forceinline__ void store_x(const SmoothArgs& S, int n, int no, const float v[3], bool dual) {
+  const int Np = S.Np;
+  S.x[n] = v[0];
+  S.x[Np + n] = v[1];
+  S.x[2 * Np + n] = v[2];
+  if (dual) {
+    S.xo[no] = v[0];
+    S.xo[Np + no] = v[1];
+    S.xo[2 * Np + no] = v[2];
+  }
+}
+
 // Long lines: each thread gathers its two rows straight into registers.
-template <bool FINE, int FAM>
+template <bool FINE, int FAM, bool DUAL>
 __global__ __launch_bounds__(512, 4) void k_smooth_direct(SmoothArgs S) {
   if (S.gate && *S.gate) return;
   extern __shared__ float sm[];
-  const LineGeo L = line_geo(S);
+  const LineGeo L = line_geo<FAM, DUAL>(S);
   const int len = L.ln.z;
   const int t = threadIdx.x;
   const int ka = 2 * t, kb = 2 * t + 1;
-  const int na = L.ln.x + ka * L.ln.y, nb = L.ln.x + kb * L.ln.y;
+  int na, nba, noa, ia, ja, nb, nbb, nob, ib, jb;
+  row_index<FAM, DUAL>(L, ka, na, nba, noa, ia, ja);
+  row_index<FAM, DUAL>(L, kb, nb, nbb, nob, ib, jb);
   float Da[6], Ua[9], ra[3], Db[6], Ub[9], rb[3];
   bool fa = true, fb = true;
   if (ka < len)
-    gather_row<FINE, FAM>(S, na, FAM == 0 ? L.i0 + ka : L.i0, FAM == 0 ? L.j0 : L.j0 + ka, L.nx, L.ny, Da, Ua, ra, fa);
+    gather_row<FINE, FAM, DUAL>(S, na, nba, ia, ja, L.nx, L.ny, Da, Ua, ra, fa);
   else
     set_identity(Da, Ua, ra);
   if (kb < len)
-    gather_row<FINE, FAM>(S, nb, FAM == 0 ? L.i0 + kb : L.i0, FAM == 0 ? L.j0 : L.j0 + kb, L.nx, L.ny, Db, Ub, rb, fb);
+    gather_row<FINE, FAM, DUAL>(S, nb, nbb, ib, jb, L.nx, L.ny, Db, Ub, rb, fb);
   else
     set_identity(Db, Ub, rb);
   float xa[3], xb[3];
   line_solve(sm, blockDim.x, t, len, Da, Ua, ra, Db, Ub, rb, xa, xb);
-  const int Np = S.Np;
-  if (ka < len && !fa) {
-    S.x[na] = xa[0];
-    S.x[Np + na] = xa[1];
-    S.x[2 * Np + na] = xa[2];
-  }
-  if (kb < len && !fb) {
-    S.x[nb] = xb[0];
-    S.x[Np + nb] = xb[1];
-    S.x[2 * Np + nb] = xb[2];
-  }
+  if (ka < len && !fa) store_x(S, na, noa, xa, DUAL);
+  if (kb < len && !fb) store_x(S, nb, nob, xb, DUAL);
 }
 
 // Short lines: every row's couplings are split over G threads so that each thread issues about
 // one round trip of loads; rows are assembled in LDS, then solved by the first ceil(len/2) threads.
-template <bool FINE, int FAM, int G>
+template <bool FINE, int FAM, int G, bool DUAL>
 __global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S) {
   if (S.gate && *S.gate) return;
   extern __shared__ float sm[];
-  const LineGeo L = line_geo(S);
+  const LineGeo L = line_geo<FAM, DUAL>(S);
   const int len = L.ln.z;
   const int R2 = (len + 1) >> 1;
   const int Lr = ((R2 + 63) / 64) * 64;
@@ -532,15 +571,15 @@ __global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S) {
   const int Np = S.Np;
   for (int task = threadIdx.x; task < len * G; task += T) {
     const int row = task / G, g = task - row * G;
-    const int n = L.ln.x + row * L.ln.y;
-    const int i = FAM == 0 ? L.i0 + row : L.i0, j = FAM == 0 ? L.j0 : L.j0 + row;
-    gather_task<FINE, FAM, G>(S, n, i, j, L.nx, L.ny, g, pr, pm, gU, row, len);
+    int n, nb, no, i, j;
+    row_index<FAM, DUAL>(L, row, n, nb, no, i, j);
+    gather_task<FINE, FAM, G, DUAL>(S, n, i, j, L.nx, L.ny, g, pr, pm, gU, row, len);
     if (g == 0) {
 #pragma unroll
       for (int q = 0; q < 6; q++) gD[q * len + row] = S.op[q * Np + n];
 #pragma unroll
       for (int q = 0; q < 3; q++) {
-        gR[q * len + row] = S.b[q * Np + n];
+        gR[q * len + row] = S.b[q * Np + nb];
         gX[q * len + row] = S.x[q * Np + n];
       }
       gF[row] = S.fixed[n];
@@ -598,16 +637,14 @@ __global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S) {
   float xa[3], xb[3];
   line_solve(cr, Lr, t, len, Da, Ua, ra, Db, Ub, rb, xa, xb);
   if (ka < len && !gF[ka]) {
-    const int n = L.ln.x + ka * L.ln.y;
-    S.x[n] = xa[0];
-    S.x[Np + n] = xa[1];
-    S.x[2 * Np + n] = xa[2];
+    int n, nb, no, i, j;
+    row_index<FAM, DUAL>(L, ka, n, nb, no, i, j);
+    store_x(S, n, no, xa, DUAL);
   }
   if (kb < len && !gF[kb]) {
-    const int n = L.ln.x + kb * L.ln.y;
-    S.x[n] = xb[0];
-    S.x[Np + n] = xb[1];
-    S.x[2 * Np + n] = xb[2];
+    int n, nb, no, i, j;
+    row_index<FAM, DUAL>(L, kb, n, nb, no, i, j);
+    store_x(S, n, no, xb, DUAL);
   }
 }
 
@@ -621,7 +658,7 @@ size_t staged_shm(int len) {
   return sizeof(float) * ((size_t)27 * Lr + (size_t)(6 + 9 + 3 + 3 + 4 * G) * len) + (size_t)len + 16;
 }
 
-template <bool FINE, int FAM>
+template <bool FINE, int FAM, bool DUAL>
 void launch_one(const SmoothArgs& S, int count, int len, hipStream_t st) {
   if (len <= kStagedMaxLen) {
     // widest gather that fits 1024 threads: couplings per thread ~ (FINE ? 12 : 24) / G
@@ -630,15 +667,39 @@ void launch_one(const SmoothArgs& S, int count, int len, hipStream_t st) {
     const int R2 = (len + 1) / 2;
     threads = std::max(threads, ((R2 + 63) / 64) * 64);
     if (G == 8)
-      k_smooth_staged<FINE, FAM, 8><<<count, threads, staged_shm<FINE, FAM, 8>(len), st>>>(S);
+      k_smooth_staged<FINE, FAM, 8, DUAL><<<count, threads, staged_shm<FINE, FAM, 8>(len), st>>>(S);
     else
-      k_smooth_staged<FINE, FAM, 4><<<count, threads, staged_shm<FINE, FAM, 4>(len), st>>>(S);
+      k_smooth_staged<FINE, FAM, 4, DUAL><<<count, threads, staged_shm<FINE, FAM, 4>(len), st>>>(S);
     return;
   }
   int R2 = (len + 1) / 2;
   int block = ((R2 + 63) / 64) * 64;
   size_t shm = (size_t)27 * block * sizeof(float);
-  k_smooth_direct<FINE, FAM><<<count, block, shm, st>>>(S);
+  k_smooth_direct<FINE, FAM, DUAL><<<count, block, shm, st>>>(S);
+}
+
+template <bool F, int A, bool D>
+void set_direct_attr(int shm) {
+  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_direct<F, A, D>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+}
+template <bool F, int A, int G, bool D>
+void set_staged_attr(int shm) {
+  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_staged<F, A, G, D>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+}
+template <bool D>
+void set_attrs(int shm, int shm_staged) {
+  set_direct_attr<true, 0, D>(shm);
+  set_direct_attr<true, 1, D>(shm);
+  set_direct_attr<false, 0, D>(shm);
+  set_direct_attr<false, 1, D>(shm);
+  set_staged_attr<true, 0, 4, D>(shm_staged);
+  set_staged_attr<true, 1, 4, D>(shm_staged);
+  set_staged_attr<true, 0, 8, D>(shm_staged);
+  set_staged_attr<true, 1, 8, D>(shm_staged);
+  set_staged_attr<false, 0, 4, D>(shm_staged);
+  set_staged_attr<false, 1, 4, D>(shm_staged);
+  set_staged_attr<false, 0, 8, D>(shm_staged);
+  set_staged_attr<false, 1, 8, D>(shm_staged);
 }
 
 }  // namespace
@@ -647,15 +708,8 @@ void smooth_set_attributes() {
   int shm = 27 * ((BD_MAX_LINE / 2 + 63) / 64 * 64) * (int)sizeof(float);
   int shm_staged = (int)staged_shm<false, 0, 8>(128);
   shm_staged = std::max(shm_staged, (int)staged_shm<false, 0, 4>(kStagedMaxLen));
-  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_direct<true, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
-  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_direct<true, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
-  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_direct<false, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
-  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_direct<false, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
-#define BD_STAGED_ATTR(F, A, G) \
-  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_staged<F, A, G>, hipFuncAttributeMaxDynamicSharedMemorySize, shm_staged))
-  BD_STAGED_ATTR(true, 0, 4); BD_STAGED_ATTR(true, 1, 4); BD_STAGED_ATTR(true, 0, 8); BD_STAGED_ATTR(true, 1, 8);
-  BD_STAGED_ATTR(false, 0, 4); BD_STAGED_ATTR(false, 1, 4); BD_STAGED_ATTR(false, 0, 8); BD_STAGED_ATTR(false, 1, 8);
-#undef BD_STAGED_ATTR
+  set_attrs<false>(shm, shm_staged);
+  set_attrs<true>(shm, shm_staged);
 }
 
 // Smooth all six colours of level k (v-lines colours 0..2 then u-lines 0..2; reversed for the
@@ -669,16 +723,29 @@ void launch_smooth(Solver& s, int k, int reverse) {
     const int fam = c < 3 ? 1 : 0;
     int maxlen = 0;
     for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
-    SmoothArgs S{L.op, L.fixed, L.Np, L.lines + L.color_start[c], L.x, L.b, &s.scal->done, L.geo};
+    const bool colmajor = L.dual && fam == 1;
+    SmoothArgs S{colmajor ? L.opT : L.op, colmajor ? L.fixedT : L.fixed, L.Np, L.lines + L.color_start[c],
+                 colmajor ? L.xT : L.x, L.b, L.dual ? (colmajor ? L.x : L.xT) : nullptr, &s.scal->done, L.geo};
     // each node of the colour's lines: its full row (fine 58 floats, coarse 150) + b, x read, x write
+    // (dual-layout levels write x twice)
     double nodes = (double)count * maxlen;
-    ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * (L.fine ? 268.0 : 636.0));
-    if (L.fine) {
-      if (fam == 0) launch_one<true, 0>(S, count, maxlen, s.stream);
-      else launch_one<true, 1>(S, count, maxlen, s.stream);
+    ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * ((L.fine ? 268.0 : 636.0) + (L.dual ? 12.0 : 0.0)));
+    if (L.dual) {
+      if (L.fine) {
+        if (fam == 0) launch_one<true, 0, true>(S, count, maxlen, s.stream);
+        else launch_one<true, 1, true>(S, count, maxlen, s.stream);
+      } else {
+        if (fam == 0) launch_one<false, 0, true>(S, count, maxlen, s.stream);
+        else launch_one<false, 1, true>(S, count, maxlen, s.stream);
+      }
     } else {
-      if (fam == 0) launch_one<false, 0>(S, count, maxlen, s.stream);
-      else launch_one<false, 1>(S, count, maxlen, s.stream);
+      if (L.fine) {
+        if (fam == 0) launch_one<true, 0, false>(S, count, maxlen, s.stream);
+        else launch_one<true, 1, false>(S, count, maxlen, s.stream);
+      } else {
+        if (fam == 0) launch_one<false, 0, false>(S, count, maxlen, s.stream);
+        else launch_one<false, 1, false>(S, count, maxlen, s.stream);
+      }
     }
   }
 }

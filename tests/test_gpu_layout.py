@@ -1,0 +1,81 @@
+"""The column-major copy used by the v-line passes of long-line levels (lines of 129+ nodes),
+against the FP64 oracle with the reference's bars (tests/gpu/correctness.ts): one forward and one
+reverse smoother sweep on the fine level (1e-3), the V-cycle (5e-3) and a 2-iteration PCG (2e-2),
+on a 136 x 136 hanging sheet whose fine level keeps both layouts and whose coarser levels do not."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from bd.capi import BD_BUF_DELTA, BD_BUF_LEVEL_B, BD_BUF_LEVEL_X, BD_BUF_RHS, make_params
+from bd.configs import sheet_scene
+from bd.material import linear_sheet_material
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_to_scale(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def test_dual_layout_smoother_vcycle_pcg():
+    from bd.solver import GpuSolver
+    n = 136
+    pitch = 1e-3
+    mat = linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / 2.4)
+    S = sheet_scene(n, pitch, mat)
+    params = make_params(history=False)
+    cpu = O.Oracle(S, params)
+    for _ in range(4):
+        cpu.substep()
+    xs = cpu.x.astype(np.float32).astype(np.float64)
+    vs = cpu.v.astype(np.float32).astype(np.float64)
+    cpu.x[:] = xs
+    cpu.v[:] = vs
+    say("gpu: create the 136x136 sheet (fine level in both layouts)")
+    gpu = GpuSolver(S, params)
+    gpu.set_state(xs, vs)
+    cpu.predict()
+    cpu.assemble()
+    gpu.encode_predict()
+    gpu.encode_assemble(True)
+    assert len(gpu.stats()["level_nodes"]) >= 3
+    e = rel_to_scale(gpu.stencil(0).reshape(n * n, -1), cpu.stencil(0).reshape(n * n, -1))
+    say(f"fine operator maxAbs/scale = {e:.2e}")
+    assert e < 5e-5
+    cpu.load_fine_operator(gpu.stencil(0), gpu.fixed(0))
+    b = gpu.read(BD_BUF_RHS)
+    N = S.node_count
+    rng = np.random.default_rng(4)
+    x0 = rng.standard_normal(3 * N) * 1e-4
+    x0[np.repeat(cpu.fixed(0) == 1, 3)] = 0
+    for reverse in (False, True):
+        say(f"gpu: one {'reverse' if reverse else 'forward'} smoother sweep from a random iterate")
+        gpu.write(BD_BUF_LEVEL_B, b, 0)
+        gpu.write(BD_BUF_LEVEL_X, x0, 0)
+        gpu.encode_smooth(0, reverse)
+        xg = gpu.read(BD_BUF_LEVEL_X, 0)
+        xc = cpu.smooth(0, x0.copy(), b, reverse)
+        e = rel_to_scale(xg, xc)
+        say(f"  sweep maxAbs/scale = {e:.2e}")
+        assert e < 1e-3
+    say("gpu: V-cycle")
+    gpu.write(BD_BUF_LEVEL_B, b, 0)
+    gpu.encode_vcycle()
+    e = rel_to_scale(gpu.read(BD_BUF_LEVEL_X, 0), cpu.vcycle(b))
+    say(f"V-cycle maxAbs/scale = {e:.2e}")
+    assert e < 5e-3
+    say("gpu: PCG (2 iterations)")
+    gpu.write(BD_BUF_RHS, b)
+    gpu.encode_pcg()
+    e = rel_to_scale(gpu.read(BD_BUF_DELTA), cpu.pcg(b, params.pcg_iterations))
+    say(f"PCG step maxAbs/scale = {e:.2e}")
+    assert e < 2e-2
