@@ -9,6 +9,11 @@ line smoothers) to rel-res 1e-6 with FP64 residual replacement, FP64 line search
 damage field and crossing slip. All state lives in HBM before the timed region; per frame the host
 only moves the grip goals (64 bytes).
 
+Two preconditioners are timed on the same frames: "plain", the reference's V-cycle (line sweeps on
+every level), and "mixed", the same V-cycle with the coarsest level solved exactly in FP64 (banded
+Cholesky per hierarchy build, dense inverse applied). `value` is the mixed method's ms/frame; the
+plain method's numbers and the iteration ratio are reported beside it.
+
 Prints one JSON line: the metric, the GPU roofline of the dominant kernel class (HIP-event timed in
 an instrumented pass over the same frames) and a CPU baseline (the FP64 C oracle, single thread, on
 a bounded sample of the same frame, extrapolated with the GPU's measured iteration counts).
@@ -34,6 +39,7 @@ def parse():
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--profile-frames", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--method", choices=("both", "mixed", "plain"), default="both")
     return ap.parse_args()
 
 
@@ -66,19 +72,14 @@ def cpu_baseline(cfg, gpu_iters_per_newton, newton_per_frame):
                       f"iterations per frame"}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU strip decomposition is "
-              f"a design note, not part of this build", file=sys.stderr)
-        sys.exit(2)
+def run_method(args, cfg, coarse_solve, profile_frames):
+    """Warm-up, then exactly `args.steps` timed frames of one preconditioner, then an instrumented
+    pass of `profile_frames` frames (per-kernel-class HIP event times)."""
     import numpy as np
-    from bd.configs import config_sheet, advance_grips
     from bd.solver import GpuSolver
-
-    cfg = config_sheet(args.size)
-    S = cfg["scene"]
-    gpu = GpuSolver(S, cfg["params"], cfg["grips"])
+    params = cfg["params"]
+    params.coarse_solve = coarse_solve
+    gpu = GpuSolver(cfg["scene"], params, cfg["grips"])
     goals = np.array([g["goal"] for g in cfg["grips"]], dtype=np.float64)
     vel = np.array(cfg["velocity"], dtype=np.float64)
 
@@ -100,17 +101,41 @@ def main():
         newton.append(st["newton_iterations"])
     gpu.sync()
     t1 = time.perf_counter()
-    ms = 1e3 * (t1 - t0) / args.steps
-
-    # instrumented pass: per-kernel-class HIP event times over the same kind of frames
+    out = {"ms": 1e3 * (t1 - t0) / args.steps, "iters": float(np.mean(iters)), "newton": float(np.mean(newton)),
+           "levels": gpu.stats()["level_nodes"]}
     gpu.profile_enable(True)
-    for _ in range(args.profile_frames):
+    for _ in range(profile_frames):
         frame()
-    prof = gpu.profile_read()
+    out["prof"] = gpu.profile_read()
     gpu.profile_enable(False)
+    out["finite"] = bool(np.isfinite(gpu.read(0)).all())
+    gpu.close()
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU strip decomposition is "
+              f"a design note, not part of this build", file=sys.stderr)
+        sys.exit(2)
+    from bd.capi import BD_COARSE_EXACT, BD_COARSE_SWEEPS
+    from bd.configs import config_sheet
+
+    cfg = config_sheet(args.size)
+    S = cfg["scene"]
+    runs = {}
+    if args.method in ("both", "plain"):
+        runs["plain"] = run_method(args, cfg, BD_COARSE_SWEEPS, args.profile_frames if args.method == "plain" else 0)
+    if args.method in ("both", "mixed"):
+        runs["mixed"] = run_method(args, cfg, BD_COARSE_EXACT, args.profile_frames)
+    main_name = "mixed" if "mixed" in runs else "plain"
+    R = runs[main_name]
+    ms = R["ms"]
+    prof = R["prof"]
     hbm = {k: v for k, v in prof.items() if v[2] > 0}
     if not hbm:
-        print(json.dumps({"note": "no profiled frames"}), flush=True)
+        print(json.dumps({"note": "no profiled frames", "ms_per_frame": {k: v["ms"] for k, v in runs.items()}}), flush=True)
         return
     dom = max(hbm, key=lambda k: hbm[k][1])
     launches, tot_ms, nbytes = hbm[dom]
@@ -121,14 +146,13 @@ def main():
     kernels = {k: {"launches": v[0], "ms_per_frame": round(v[1] / args.profile_frames, 3),
                    "GBps": round(v[2] / (v[1] * 1e-3) / 1e9, 1) if v[1] > 0 and v[2] > 0 else None}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
-
-    x = gpu.read(0)
-    finite = bool(np.isfinite(x).all())
-    iters_per_frame = float(np.mean(iters))
-    newton_per_frame = float(np.mean(newton))
+    iters_per_frame = R["iters"]
+    newton_per_frame = R["newton"]
     cpu = None
     if not args.no_cpu_baseline and os.environ.get("BD_BENCH_NO_CPU") is None:
-        cpu = cpu_baseline(cfg, iters_per_frame / max(newton_per_frame, 1), newton_per_frame)
+        P = runs.get("plain", R)
+        cpu = cpu_baseline(cfg, P["iters"] / max(P["newton"], 1), P["newton"])
+    both = len(runs) == 2
     out = {
         "metric": "ms/frame & PCG iters to rel-res 1e-6: learned+mixed vs plain MG-PCG",
         "value": round(ms, 3),
@@ -140,15 +164,20 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (FP64 dots, residual replacement, line search)",
+        "dtype": "fp32 (FP64 dots, residual replacement, line search" +
+                 (", coarsest solve)" if main_name == "mixed" else ")"),
         "data": "synthetic",
-        "config": {"workload": f"sheet{args.size}x{args.size} plain FP32 MG-PCG to rel-res 1e-6, "
-                               f"central 256x256 breakable yarn region, 2 grips at 5 mm/step",
-                   "nodes": S.node_count, "levels": gpu.stats()["level_nodes"], "newton_per_frame": newton_per_frame,
-                   "method": "plain"},
+        "config": {"workload": f"sheet{args.size}x{args.size} FP32 MG-PCG to rel-res 1e-6, "
+                               f"central {args.size // 4}x{args.size // 4} breakable yarn region, 2 grips at 5 mm/step",
+                   "nodes": S.node_count, "levels": R["levels"], "newton_per_frame": newton_per_frame,
+                   "method": main_name + (" (FP64 exact coarsest solve)" if main_name == "mixed" else " (line sweeps)")},
         "pcg_iterations_per_frame": iters_per_frame,
         "pcg_iterations_per_newton": round(iters_per_frame / max(newton_per_frame, 1), 2),
-        "finite": finite,
+        "methods": {k: {"ms_per_frame": round(v["ms"], 3), "pcg_iterations_per_frame": v["iters"]}
+                    for k, v in runs.items()},
+        "iteration_ratio_plain_over_mixed": round(runs["plain"]["iters"] / runs["mixed"]["iters"], 3) if both else None,
+        "speedup_plain_over_mixed": round(runs["plain"]["ms"] / runs["mixed"]["ms"], 3) if both else None,
+        "finite": all(r["finite"] for r in runs.values()),
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,

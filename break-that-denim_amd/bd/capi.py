@@ -20,6 +20,8 @@ BD_MAX_LEVELS = 12
 BD_MAX_NEWTON = 8
 BD_MAX_GRIPS = 16
 BD_PRECISION_FP32 = 0
+BD_COARSE_SWEEPS = 0
+BD_COARSE_EXACT = 1
 
 (BD_BUF_X, BD_BUF_V, BD_BUF_DX, BD_BUF_PRED, BD_BUF_DELTA, BD_BUF_RHS, BD_BUF_LEVEL_X, BD_BUF_LEVEL_B,
  BD_BUF_LEVEL_R, BD_BUF_STENCIL, BD_BUF_FIXED, BD_BUF_PROLONG, BD_BUF_REST, BD_BUF_INTACT, BD_BUF_CUTMASK,
@@ -70,7 +72,7 @@ class bd_params(C.Structure):
                 ("newton_max", C.c_int32), ("newton_tolerance", C.c_double), ("pcg_iterations", C.c_int32),
                 ("pcg_max_iterations", C.c_int32), ("pcg_rel_tol", C.c_double), ("coarse_sweeps", C.c_int32),
                 ("max_coarsest_nodes", C.c_int32), ("lag_coarse", C.c_int32), ("line_search", C.c_int32),
-                ("history", C.c_int32), ("precision", C.c_int32), ("self_contact", C.c_int32), ("pad_", C.c_int32)]
+                ("history", C.c_int32), ("precision", C.c_int32), ("self_contact", C.c_int32), ("coarse_solve", C.c_int32)]
 
 
 class bd_grips(C.Structure):
@@ -148,7 +150,7 @@ class SceneBuffers:
 def make_params(h=1 / 120, gravity=(0.0, 0.0, -9.81), newton_iterations=2, newton_max=None,
                 newton_tolerance=1e-5, pcg_iterations=2, pcg_rel_tol=0.0, pcg_max_iterations=200,
                 coarse_sweeps=2, max_coarsest_nodes=400, lag_coarse=True, line_search=True, history=True,
-                precision=BD_PRECISION_FP32, self_contact=False) -> bd_params:
+                precision=BD_PRECISION_FP32, self_contact=False, coarse_solve=BD_COARSE_SWEEPS) -> bd_params:
     """Solver parameters; the defaults are the reference's production schedule (GPU_DEFAULTS,
     gpu/solver.ts:61-72: h 1/120, 2 Newton x 2 PCG, 2 coarse sweeps, coarsest <= 400 nodes)."""
     p = bd_params()
@@ -167,6 +169,7 @@ def make_params(h=1 / 120, gravity=(0.0, 0.0, -9.81), newton_iterations=2, newto
     p.history = int(bool(history))
     p.precision = precision
     p.self_contact = int(bool(self_contact))
+    p.coarse_solve = int(coarse_solve)
     return p
 
 

@@ -26,7 +26,7 @@ namespace bd {
 constexpr int kFineComps = 32;
 constexpr int kCoarseBlocks = 13;
 constexpr int kCoarseComps = 6 * kCoarseBlocks;
-constexpr int kDotBlocks = 512;
+constexpr int kDotBlocks = 2048;
 constexpr int kMaxLevels = BD_MAX_LEVELS;
 
 // forward offsets of the coarse stencil, block b = 1..12 (block 0 is the centre)
@@ -92,13 +92,13 @@ struct Scal {
 enum ProfClass {
   PC_PREDICT_COMMIT, PC_ASSEMBLE, PC_PROLONG_WEIGHTS, PC_RAP, PC_SMOOTH_FINE, PC_SMOOTH_COARSE, PC_RESIDUAL_FINE,
   PC_RESIDUAL_COARSE, PC_RESTRICT, PC_PROLONG, PC_MATVEC, PC_PCG_VECTOR, PC_TRUE_RESIDUAL, PC_ENERGY, PC_DAMAGE, PC_SLIP,
-  PC_CONTACT_HASH, PC_CONTACT_DETECT, PC_CONTACT_TERMS, PC_TRANSPOSE, PC_COUNT
+  PC_CONTACT_HASH, PC_CONTACT_DETECT, PC_CONTACT_TERMS, PC_TRANSPOSE, PC_COARSE_FACTOR, PC_COARSE_APPLY, PC_COUNT
 };
 inline const char* prof_name(int c) {
   static const char* n[PC_COUNT] = {"predict_commit", "assemble", "prolong_weights", "rap", "smooth_fine", "smooth_coarse",
                                     "residual_fine", "residual_coarse", "restrict", "prolong", "matvec_dot", "pcg_vector",
                                     "true_residual", "energy", "damage", "slip", "contact_hash", "contact_detect",
-                                    "contact_terms", "transpose"};
+                                    "contact_terms", "transpose", "coarse_factor", "coarse_apply"};
   return n[c];
 }
 
@@ -177,6 +177,11 @@ struct Solver {
   int* acc = nullptr;           // 9 planes of fixed-point accumulators
   float* ext = nullptr;         // 9 planes: gradient xyz, Hessian xx xy xz yy yz zz
   bool contact_detected = false;
+
+  // exact coarsest solve (BD_COARSE_EXACT): banded Cholesky factor and dense inverse, FP64
+  int cn = 0, cbw = 0;
+  double* cband = nullptr;
+  double* cinv = nullptr;
 };
 
 constexpr int kContactK = 4;
@@ -212,6 +217,10 @@ void contact_reset_bounds(Solver& s);
 void launch_contact_detect(Solver& s);
 void launch_contact_terms(Solver& s);
 void launch_project_start(Solver& s);
+// coarse.hip
+void coarse_exact_alloc(Solver& s);
+void launch_coarse_factor(Solver& s);
+void launch_coarse_apply(Solver& s);
 
 #define BD_HIP(call)                                                     \
   do {                                                                   \

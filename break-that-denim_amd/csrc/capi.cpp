@@ -153,7 +153,7 @@ void destroy_all(Solver& s) {
                   s.layer_of, s.rest, s.intact, s.fail_site, s.tails, s.softening, s.cut_mask, s.flags, s.strain,
                   s.damage, s.area, s.sep, s.pullmark, s.strength, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k,
                   s.partials, s.scal, s.counters, s.ijl, s.cells, s.scan_sums, s.items, s.pairs, s.keys,
-                  s.marks, s.cmeta, s.acc, s.ext};
+                  s.marks, s.cmeta, s.acc, s.ext, s.cband, s.cinv};
   for (void* p : ptrs) dfree(p);
   for (int k = 0; k < s.nlev; k++) {
     Level& L = s.lv[k];
@@ -196,6 +196,7 @@ void build_hierarchy(Solver& s) {
     launch_rap(s, k);
     launch_colmajor_op(s, k + 1);
   }
+  if (s.P.coarse_solve == BD_COARSE_EXACT) launch_coarse_factor(s);
   s.hierarchy = true;
 }
 
@@ -250,6 +251,10 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
   }
   if (total != sc->node_count) {
     g_create_error = "bd_create: node_count does not match the layers";
+    return BD_ERR_INVALID;
+  }
+  if (prm->coarse_solve != BD_COARSE_SWEEPS && prm->coarse_solve != BD_COARSE_EXACT) {
+    g_create_error = "bd_create: unknown coarse_solve";
     return BD_ERR_INVALID;
   }
   if (prm->self_contact && sc->node_count > BD_CONTACT_MAX_NODES) {
@@ -391,6 +396,7 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
       build_lines(s, L);
     }
     smooth_set_attributes();
+    if (s.P.coarse_solve == BD_COARSE_EXACT) coarse_exact_alloc(s);
     s.stats.level_count = s.nlev;
     for (int k = 0; k < s.nlev && k < BD_MAX_LEVELS; k++) s.stats.level_nodes[k] = s.lv[k].N;
   } catch (const std::exception& e) {

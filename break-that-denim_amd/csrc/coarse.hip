@@ -1,0 +1,211 @@
+// Exact solve of the coarsest multigrid level (BD_COARSE_EXACT). The reference smooths its coarsest
+// level with coarse_sweeps colour-ordered line sweeps (newtonMultigrid.ts:495-528, gpu/solver.ts
+// encodeVCycle); this option replaces them by A_c^-1 in FP64, the mixed-precision coarse solve.
+//
+// The coarsest level's DOF matrix (DOF 3 n + c, nodes row-major per layer) is banded with half
+// bandwidth bw = 3 (2 nx + 2) + 2 (867 DOFs and bw = 110 for 17 x 17). Per hierarchy build:
+//   k_band_load      the lower band from the symmetric coarse blocks (FP32 -> FP64);
+//   k_band_cholesky  right-looking banded Cholesky in one workgroup, the active (bw+1)^2 window in
+//                    LDS as a ring of rows (98.6 KB at bw = 110); rows leave the window final;
+//   k_band_inverse   one wave per column c: L y = e_c, then L^T x = y, written as row c of the
+//                    symmetric inverse (coalesced);
+// and per V-cycle k_coarse_apply: one wave per row of A_c^-1 (FP64 dot with b, 6 MB at 867 DOFs).
+#include <stdexcept>
+
+#include "bd_internal.h"
+
+namespace bd {
+
+namespace {
+
+constexpr int kCholThreads = 1024;
+
+__device__ __forceinline__ int sym_index(int r, int c) {
+  // [xx, xy, xz, yy, yz, zz]
+  const int lo = r < c ? r : c, hi = r < c ? c : r;
+  return lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+}
+
+struct BandArgs {
+  const float* op;
+  int N, Np;
+  Geo geo;
+  int n, bw;
+  double* L;    // n x (bw + 1): L(i, j) at i * (bw + 1) + (i - j)
+  double* inv;  // n x n
+};
+
+// lower band of the DOF matrix from the coarse blocks (centre + 12 forward blocks per row; a
+// backward block is the transposed forward block of the other row, and the blocks are symmetric)
+__global__ __launch_bounds__(256) void k_band_load(BandArgs B) {
+  const int w = B.bw + 1;
+  const size_t total = (size_t)B.n * w;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / w), o = (int)(e - (size_t)i * w);
+    const int j = i - o;
+    double v = 0.0;
+    if (j >= 0) {
+      const int a = i / 3, r = i - 3 * a, b = j / 3, c = j - 3 * b;
+      const int la = geo_layer(B.geo, a), lb = geo_layer(B.geo, b);
+      if (la == lb) {
+        const int nx = B.geo.nx[la];
+        const int ra = a - B.geo.off[la], rb = b - B.geo.off[la];
+        const int ja = ra / nx, ia = ra - ja * nx, jb = rb / nx, ib = rb - jb * nx;
+        const int di = ia - ib, dj = ja - jb;  // a relative to b (a follows b: dj > 0 or dj == 0, di >= 0)
+        if (a == b) {
+          v = B.op[sym_index(r, c) * B.Np + a];
+        } else {
+          const int k = coarse_block(di, dj);
+          if (k > 0) v = B.op[(6 * k + sym_index(r, c)) * B.Np + b];
+        }
+      }
+    }
+    B.L[e] = v;
+  }
+}
+
+// Right-looking banded Cholesky in one workgroup. The window holds rows k .. k + bw of the band
+// (row i in slot i mod (bw + 1)); after step k row k is final and is replaced by row k + bw + 1.
+__global__ __launch_bounds__(kCholThreads) void k_band_cholesky(BandArgs B) {
+  extern __shared__ double win[];  // (bw + 1) x (bw + 1)
+  __shared__ double piv;
+  const int n = B.n, bw = B.bw, w = bw + 1;
+  const int t = threadIdx.x, T = blockDim.x;
+  for (int e = t; e < w * w; e += T) {
+    const int row = e / w;
+    win[e] = row < n ? B.L[(size_t)row * w + (e - row * w)] : 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < n; k++) {
+    double* rk = win + (k % w) * w;
+    if (t == 0) {
+      double d = rk[0];
+      // an SPD operator has positive pivots; a non-positive one (rounding in a near-singular
+      // level) is replaced by a tiny positive value so the factor stays finite
+      d = d > 0.0 ? sqrt(d) : 1e-150;
+      rk[0] = d;
+      piv = d;
+    }
+    __syncthreads();
+    const int m = min(bw, n - 1 - k);
+    // column k below the diagonal: L(i, k) at row i, offset i - k
+    for (int q = t; q < m; q += T) {
+      const int i = k + 1 + q;
+      win[(i % w) * w + (i - k)] /= piv;
+    }
+    __syncthreads();
+    // trailing update of rows k+1 .. k+m, columns k+1 .. i
+    const int pairs = m * (m + 1) / 2;
+    for (int p = t; p < pairs; p += T) {
+      int a = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+      while ((a + 1) * (a + 2) / 2 <= p) a++;
+      while (a * (a + 1) / 2 > p) a--;
+      const int b = p - a * (a + 1) / 2;  // 0 <= b <= a < m
+      const int i = k + 1 + a, j = k + 1 + b;
+      const double lik = win[(i % w) * w + (i - k)];
+      const double ljk = win[(j % w) * w + (j - k)];
+      win[(i % w) * w + (i - j)] -= lik * ljk;
+    }
+    __syncthreads();
+    // row k is final: store it, load row k + w into its slot
+    const int nk = k + w;
+    for (int o = t; o < w; o += T) {
+      B.L[(size_t)k * w + o] = rk[o];
+      rk[o] = nk < n ? B.L[(size_t)nk * w + o] : 0.0;
+    }
+    __syncthreads();
+  }
+}
+
+// one wave per column c of the inverse: L y = e_c, L^T x = y; x is row c of the symmetric inverse
+__global__ __launch_bounds__(64) void k_band_inverse(BandArgs B) {
+  extern __shared__ double y[];  // n
+  const int n = B.n, bw = B.bw, w = bw + 1;
+  const int c = blockIdx.x, lane = threadIdx.x;
+  const double* L = B.L;
+  for (int i = lane; i < n; i += 64) y[i] = 0.0;
+  __syncthreads();
+  for (int i = c; i < n; i++) {
+    double s = 0.0;
+    const int k0 = max(c, i - bw);
+    for (int k = k0 + lane; k < i; k += 64) s += L[(size_t)i * w + (i - k)] * y[k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) y[i] = ((i == c ? 1.0 : 0.0) - s) / L[(size_t)i * w];
+    __syncthreads();
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double s = 0.0;
+    const int k1 = min(n - 1, i + bw);
+    for (int k = i + 1 + lane; k <= k1; k += 64) s += L[(size_t)k * w + (k - i)] * y[k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) y[i] = (y[i] - s) / L[(size_t)i * w];
+    __syncthreads();
+  }
+  for (int i = lane; i < n; i += 64) B.inv[(size_t)c * n + i] = y[i];
+}
+
+// x = A_c^-1 b on the coarsest level: one wave per row, FP64 products and sums
+__global__ __launch_bounds__(256) void k_coarse_apply(const double* inv, int n, int Np, const float* b, float* x,
+                                                      const int* gate) {
+  if (gate && *gate) return;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const double* r = inv + (size_t)row * n;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) {
+    const int node = j / 3, c = j - 3 * node;
+    s += r[j] * (double)b[c * Np + node];
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) {
+    const int node = row / 3, c = row - 3 * node;
+    x[c * Np + node] = (float)s;
+  }
+}
+
+int band_width(const Geo& g) {
+  int bw = 2;
+  for (int l = 0; l < g.nl; l++) bw = std::max(bw, 3 * (2 * g.nx[l] + 2) + 2);
+  return bw;
+}
+
+}  // namespace
+
+size_t coarse_exact_bytes(const Level& L) {
+  const size_t n = 3 * (size_t)L.N, w = band_width(L.geo) + 1;
+  return sizeof(double) * (n * w + n * n);
+}
+
+void coarse_exact_alloc(Solver& s) {
+  Level& L = s.lv[s.nlev - 1];
+  const int n = 3 * L.N, bw = band_width(L.geo);
+  const size_t lds = sizeof(double) * (size_t)(bw + 1) * (bw + 1);
+  if (lds > 150 * 1024)
+    throw std::runtime_error("exact coarse solve: coarsest level too wide for the LDS window (bandwidth " +
+                             std::to_string(bw) + ")");
+  if ((size_t)n * sizeof(double) > 64 * 1024) throw std::runtime_error("exact coarse solve: coarsest level too large");
+  BD_HIP(hipMalloc(&s.cband, sizeof(double) * (size_t)n * (bw + 1)));
+  BD_HIP(hipMalloc(&s.cinv, sizeof(double) * (size_t)n * n));
+  BD_HIP(hipFuncSetAttribute((const void*)k_band_cholesky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  s.cn = n;
+  s.cbw = bw;
+}
+
+void launch_coarse_factor(Solver& s) {
+  Level& L = s.lv[s.nlev - 1];
+  BandArgs B{L.op, L.N, L.Np, L.geo, s.cn, s.cbw, s.cband, s.cinv};
+  const int w = s.cbw + 1;
+  // band load + factor (the window's rows in and out once) + inverse (the band read per column)
+  ProfScope ps(s, PC_COARSE_FACTOR, 8.0 * ((double)s.cn * w * 3 + (double)s.cn * s.cn));
+  k_band_load<<<grid_for(s.cn * w, 256), 256, 0, s.stream>>>(B);
+  k_band_cholesky<<<1, kCholThreads, sizeof(double) * (size_t)w * w, s.stream>>>(B);
+  k_band_inverse<<<s.cn, 64, sizeof(double) * (size_t)s.cn, s.stream>>>(B);
+}
+
+void launch_coarse_apply(Solver& s) {
+  Level& L = s.lv[s.nlev - 1];
+  ProfScope ps(s, PC_COARSE_APPLY, 8.0 * (double)s.cn * s.cn + 24.0 * L.N);
+  k_coarse_apply<<<(s.cn + 3) / 4, 256, 0, s.stream>>>(s.cinv, s.cn, L.Np, L.b, L.x, &s.scal->done);
+}
+
+}  // namespace bd

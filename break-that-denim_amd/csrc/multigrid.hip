@@ -462,6 +462,10 @@ void launch_prolong_add(Solver& s, int k) {
 void vcycle(Solver& s, int k) {
   Level& L = s.lv[k];
   if (k == s.nlev - 1) {
+    if (s.P.coarse_solve == BD_COARSE_EXACT) {
+      launch_coarse_apply(s);
+      return;
+    }
     for (int sw = 0; sw < s.P.coarse_sweeps; sw++) launch_smooth(s, k, sw & 1);
     return;
   }

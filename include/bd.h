@@ -128,8 +128,11 @@ typedef struct bd_params {
   int32_t history;             /* damage field + crossing slip after commit */
   int32_t precision;           /* BD_PRECISION_* */
   int32_t self_contact;        /* node-face self-contact (GpuSolverParams.selfContact, gpu/solver.ts:49-50) */
-  int32_t pad_;
+  int32_t coarse_solve;        /* BD_COARSE_*: how the V-cycle solves its coarsest level */
 } bd_params;
+
+#define BD_COARSE_SWEEPS 0     /* coarse_sweeps colour-ordered line sweeps (the reference's V-cycle) */
+#define BD_COARSE_EXACT 1      /* exact FP64 solve: banded Cholesky per hierarchy build, dense inverse applied */
 
 #define BD_PRECISION_FP32 0    /* FP32 operators and smoothers, FP64 dot products */
 

@@ -303,6 +303,9 @@ typedef struct {
   double* cmeta;
   int floor_on;
   double floor_z;
+  /* exact coarsest solve (BD_COARSE_EXACT): banded Cholesky factor of the coarsest level */
+  int cn, cbw;
+  double* cL;
 } or_solver;
 
 static inline const bd_material* mat_of(const or_solver* S, int n) { return &S->lay[S->layer_of[n]].material; }
@@ -1127,6 +1130,69 @@ static void free_levels(or_solver* S, int from) {
 }
 
 /* buildHierarchy (newtonMultigrid.ts:480-491) on the current level 0 */
+/* Exact solve of the coarsest level (BD_COARSE_EXACT; an extension, the reference smooths it):
+   Cholesky factor L of its DOF matrix (DOF 3 n + c, row-major nodes) within its bandwidth, FP64.
+   L(i, j), i - bw <= j <= i, is stored at L[i * (bw + 1) + i - j]. Returns -1 on a non-positive
+   pivot. */
+static int band_bandwidth(const op_t* A) {
+  int bw = 2;
+  for (int l = 0; l < A->nl; l++) {
+    int b = 3 * (2 * A->lay[l].nx + 2) + 2;
+    if (b > bw) bw = b;
+  }
+  return bw;
+}
+
+static double dof_entry(const op_t* A, int i, int j) {
+  int a = i / 3, r = i % 3, b = j / 3, c = j % 3;
+  if (A->ij[3 * a + 2] != A->ij[3 * b + 2]) return 0;
+  int di = A->ij[3 * b] - A->ij[3 * a], dj = A->ij[3 * b + 1] - A->ij[3 * a + 1];
+  if (abs(di) > 2 || abs(dj) > 2) return 0;
+  return A->A[((size_t)a * STENCIL + offset_index(di, dj)) * 9 + r * 3 + c];
+}
+
+static int band_cholesky(const op_t* A, int* n_out, int* bw_out, double** L_out) {
+  const int n = 3 * A->N, bw = band_bandwidth(A), w = bw + 1;
+  double* L = calloc((size_t)n * w, sizeof(double));
+  for (int i = 0; i < n; i++) {
+    int j0 = i - bw < 0 ? 0 : i - bw;
+    for (int j = j0; j <= i; j++) {
+      double s = dof_entry(A, i, j);
+      int k0 = j0 > j - bw ? j0 : j - bw;
+      for (int k = k0; k < j; k++) s -= L[(size_t)i * w + i - k] * L[(size_t)j * w + j - k];
+      if (i == j) {
+        if (!(s > 0)) {
+          free(L);
+          return -1;
+        }
+        L[(size_t)i * w] = sqrt(s);
+      } else {
+        L[(size_t)i * w + i - j] = s / L[(size_t)j * w];
+      }
+    }
+  }
+  *n_out = n;
+  *bw_out = bw;
+  *L_out = L;
+  return 0;
+}
+
+static void band_solve(int n, int bw, const double* L, const double* b, double* x) {
+  const int w = bw + 1;
+  for (int i = 0; i < n; i++) {
+    double s = b[i];
+    int k0 = i - bw < 0 ? 0 : i - bw;
+    for (int k = k0; k < i; k++) s -= L[(size_t)i * w + i - k] * x[k];
+    x[i] = s / L[(size_t)i * w];
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double s = x[i];
+    int k1 = i + bw > n - 1 ? n - 1 : i + bw;
+    for (int k = i + 1; k <= k1; k++) s -= L[(size_t)k * w + k - i] * x[k];
+    x[i] = s / L[(size_t)i * w];
+  }
+}
+
 static int build_hierarchy(or_solver* S) {
   free_levels(S, 1);
   S->nlev = 1;
@@ -1138,6 +1204,9 @@ static int build_hierarchy(or_solver* S) {
     S->nlev++;
     cur = &S->lv[S->nlev - 1];
   }
+  free(S->cL);
+  S->cL = NULL;
+  if (S->P.coarse_solve == BD_COARSE_EXACT && band_cholesky(cur, &S->cn, &S->cbw, &S->cL) != 0) return -1;
   return 0;
 }
 
@@ -1325,6 +1394,10 @@ static void vcycle(or_solver* S, int k, double* x, const double* b) {
   const op_t* A = &S->lv[k];
   int colors = 3;
   if (k == S->nlev - 1) {
+    if (S->cL) {
+      band_solve(S->cn, S->cbw, S->cL, b, x);
+      return;
+    }
     for (int s = 0; s < S->P.coarse_sweeps; s++) smooth_lines(A, x, b, s & 1, colors);
     return;
   }
@@ -1490,7 +1563,7 @@ void or_destroy(or_solver* S) {
   free(S->x); free(S->v); free(S->dx); free(S->pred); free(S->rest); free(S->softening); free(S->intact);
   free(S->cut_mask); free(S->fail_site); free(S->seg_flags); free(S->tails); free(S->strain);
   free(S->last_update); free(S->nbr); free(S->layer_of); free(S->grip_of); free(S->grip_w); free(S->grip_off);
-  free(S->ext); free(S->cmeta);
+  free(S->ext); free(S->cmeta); free(S->cL);
   free(S);
 }
 

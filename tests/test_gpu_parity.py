@@ -209,3 +209,36 @@ def test_dynamics_60_substeps(coupon_pair):
     err = np.max(np.abs(xg - cpu.x))
     say(f"dynamics: max |x_gpu - x_cpu| = {err * 1e6:.2f} um, max motion {moved * 1e6:.1f} um")
     assert err < 0.25 * moved
+
+
+def test_exact_coarse_solve(coupon_pair):
+    """BD_COARSE_EXACT: the GPU's FP64 banded Cholesky + dense inverse of the coarsest level in the
+    V-cycle and PCG, against the oracle's banded Cholesky solve (bars as above)."""
+    from bd.solver import GpuSolver
+    S, cpu, _ = coupon_pair
+    params = make_params(coarse_solve=1)
+    gpu = GpuSolver(S, params)
+    copy_state(cpu, gpu)
+    ref = O.Oracle(S, params)
+    ref.x[:] = cpu.x
+    ref.v[:] = cpu.v
+    ref.set_grips(None)
+    say("gpu: exact coarse solve: predict + assemble")
+    ref.predict()
+    gpu.encode_predict()
+    gpu.encode_assemble(True)
+    ref.assemble()
+    ref.load_fine_operator(gpu.stencil(0), gpu.fixed(0))
+    b = gpu.read(BD_BUF_RHS)
+    say("gpu: exact coarse solve: V-cycle")
+    gpu.write(BD_BUF_LEVEL_B, b, 0)
+    gpu.encode_vcycle()
+    e = rel_to_scale(gpu.read(BD_BUF_LEVEL_X, 0), ref.vcycle(b))
+    say(f"V-cycle (exact coarsest) maxAbs/scale = {e:.2e}")
+    assert e < 5e-3
+    say("gpu: exact coarse solve: PCG (2 iterations)")
+    gpu.write(BD_BUF_RHS, b)
+    gpu.encode_pcg()
+    e = rel_to_scale(gpu.read(BD_BUF_DELTA), ref.pcg(b, params.pcg_iterations))
+    say(f"PCG step (exact coarsest) maxAbs/scale = {e:.2e}")
+    assert e < 2e-2

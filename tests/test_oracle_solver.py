@@ -316,3 +316,45 @@ def test_pull_ruptures_and_conserves_yarn_length():  # referenceSolver.test.ts:1
     assert o.broken_count > 0
     assert int(np.count_nonzero(o.seg_flags & SEG_BROKEN)) >= o.broken_count
     assert abs(o.yarn_material_length() - L0) / L0 < 1e-9
+
+
+def test_exact_coarse_solve_is_the_inverse():
+    """BD_COARSE_EXACT (an extension): with a single level the V-cycle is the banded Cholesky solve,
+    which must agree with a dense solve of the same operator; with a hierarchy it is the exact
+    two-grid correction at the coarsest level, so the V-cycle beats the swept one."""
+    S = compile_scene([mg_coupon(11, 9)])
+    p = ref_params()
+    p.coarse_solve = 1
+    p.max_coarsest_nodes = 10 ** 6   # the fine level is the coarsest
+    o = O.Oracle(S, p)
+    o.predict()
+    o.assemble()
+    o.build_hierarchy()
+    assert o.level_count() == 1
+    N = S.node_count
+    A = to_dense(o.stencil(0), 11, 9)
+    r = lcg(9)
+    b = np.array([r() for _ in range(3 * N)])
+    b.reshape(N, 3)[o.fixed(0) == 1] = 0
+    x = o.vcycle(b)
+    xd = np.linalg.solve(A, b)
+    assert np.max(np.abs(x - xd)) <= 1e-10 * np.max(np.abs(xd))
+    # on a hierarchy: the exact coarsest solve is the limit of many coarsest sweeps, up to the
+    # sweeps' shifted pivots (PIVOT_SHIFT 2e-6 of each diagonal moves their fixed point by ~1e-4)
+    S2 = compile_scene([mg_coupon(40, 30)])
+    out = []
+    for mode, sweeps in ((0, 400), (1, 2)):
+        q = ref_params()
+        q.coarse_solve = mode
+        q.coarse_sweeps = sweeps
+        o2 = O.Oracle(S2, q)
+        o2.predict()
+        o2.assemble()
+        o2.build_hierarchy()
+        assert o2.level_count() > 2
+        N2 = S2.node_count
+        r2 = lcg(4)
+        b2 = np.array([r2() for _ in range(3 * N2)])
+        b2.reshape(N2, 3)[o2.fixed(0) == 1] = 0
+        out.append(o2.vcycle(b2))
+    assert np.max(np.abs(out[1] - out[0])) <= 5e-4 * np.max(np.abs(out[0]))
