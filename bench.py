@@ -29,6 +29,19 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "break-that-denim_amd"))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_sheet1024_smoother.md"
+
+
+def measured_traffic(kernel_class, size):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3 --pmc summary (FETCH_SIZE
+    and WRITE_SIZE, scaled by a plain copy's counts), for the workload it was measured on."""
+    if size != 1024 or not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel_class:
+            return float(f[2])
+    return None
 
 
 def parse():
@@ -40,7 +53,23 @@ def parse():
     ap.add_argument("--profile-frames", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--method", choices=("both", "mixed", "plain"), default="both")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 8 plain device copies of MIB MiB (calibrates the PMC byte counters)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """A plain device-to-device copy of known size, timed and counted under rocprofv3 --pmc to scale
+    FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(8):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "copies": 8}), flush=True)
 
 
 def cpu_baseline(cfg, gpu_iters_per_newton, newton_per_frame):
@@ -115,6 +144,9 @@ def run_method(args, cfg, coarse_solve, profile_frames):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU strip decomposition is "
               f"a design note, not part of this build", file=sys.stderr)
@@ -141,7 +173,7 @@ def main():
     launches, tot_ms, nbytes = hbm[dom]
     achieved = nbytes / (tot_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, args.size),
             "avg_launch_us": round(1e3 * tot_ms / launches, 2), "bytes_per_launch": round(nbytes / launches)}
     kernels = {k: {"launches": v[0], "ms_per_frame": round(v[1] / args.profile_frames, 3),
                    "GBps": round(v[2] / (v[1] * 1e-3) / 1e9, 1) if v[1] > 0 and v[2] > 0 else None}
