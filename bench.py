@@ -10,8 +10,9 @@ damage field and crossing slip. All state lives in HBM before the timed region; 
 only moves the grip goals (64 bytes).
 
 Two preconditioners are timed on the same frames: "plain", the reference's V-cycle (line sweeps on
-every level), and "mixed", the same V-cycle with the coarsest level solved exactly in FP64 (banded
-Cholesky per hierarchy build, dense inverse applied). `value` is the mixed method's ms/frame; the
+every level, coarsest level <= 400 nodes), and "mixed", the same V-cycle stopped at the first level of
+<= 1100 nodes (33 x 33), which is solved exactly in FP64 (banded Cholesky per hierarchy build, dense
+inverse applied). `value` is the mixed method's ms/frame; the
 plain method's numbers and the iteration ratio are reported beside it.
 
 Prints one JSON line: the metric, the GPU roofline of the dominant kernel class (HIP-event timed in
@@ -29,6 +30,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "break-that-denim_amd"))
 
 HBM_PEAK_GBS = 8000.0
+MIXED_COARSEST_NODES = 1100
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_sheet1024_smoother.md"
 
 
@@ -108,6 +110,8 @@ def run_method(args, cfg, coarse_solve, profile_frames):
     from bd.solver import GpuSolver
     params = cfg["params"]
     params.coarse_solve = coarse_solve
+    # the mixed method stops coarsening at 33 x 33 and solves that level exactly (FP64)
+    params.max_coarsest_nodes = MIXED_COARSEST_NODES if coarse_solve else 400
     gpu = GpuSolver(cfg["scene"], params, cfg["grips"])
     goals = np.array([g["goal"] for g in cfg["grips"]], dtype=np.float64)
     vel = np.array(cfg["velocity"], dtype=np.float64)

@@ -180,6 +180,7 @@ struct Solver {
 
   // exact coarsest solve (BD_COARSE_EXACT): banded Cholesky factor and dense inverse, FP64
   int cn = 0, cbw = 0;
+  bool cwindow = true;  // factor in an LDS window (else in place in global memory)
   double* cband = nullptr;
   double* cinv = nullptr;
 };

@@ -7,6 +7,8 @@
 //   k_band_load      the lower band from the symmetric coarse blocks (FP32 -> FP64);
 //   k_band_cholesky  right-looking banded Cholesky in one workgroup, the active (bw+1)^2 window in
 //                    LDS as a ring of rows (98.6 KB at bw = 110); rows leave the window final;
+//                    wider bands (33 x 33 and up) factor by panels of 16 columns in LDS with
+//                    rank-16 trailing updates in global memory (k_band_cholesky_panel);
 //   k_band_inverse   one wave per column c: L y = e_c, then L^T x = y, written as row c of the
 //                    symmetric inverse (coalesced);
 // and per V-cycle k_coarse_apply: one wave per row of A_c^-1 (FP64 dot with b, 6 MB at 867 DOFs).
@@ -117,6 +119,90 @@ __global__ __launch_bounds__(kCholThreads) void k_band_cholesky(BandArgs B) {
   }
 }
 
+// The same factorization for bands whose window does not fit LDS, blocked by panels of kPanel
+// columns: the panel (its kPanel columns over the kPanel + bw rows they reach) is factored in LDS,
+// written back, and the trailing band triangle (bw x bw) below it takes one rank-kPanel update in
+// global memory (L2-resident), each thread holding a 4 x 4 tile of it in registers.
+constexpr int kPanel = 16;
+
+__global__ __launch_bounds__(kCholThreads) void k_band_cholesky_panel(BandArgs B) {
+  extern __shared__ double pan[];  // (kPanel + bw) x kPanel, row-major
+  const int n = B.n, bw = B.bw, w = bw + 1;
+  const int t = threadIdx.x, T = blockDim.x;
+  double* L = B.L;
+  for (int k0 = 0; k0 < n; k0 += kPanel) {
+    const int kb = min(kPanel, n - k0);
+    const int rows = min(kb + bw, n - k0);  // panel rows k0 .. k0 + rows - 1
+    // load: pan[r][c] = L(k0 + r, k0 + c) for c <= r within the band
+    for (int e = t; e < rows * kPanel; e += T) {
+      const int r = e / kPanel, c = e - r * kPanel;
+      const int i = k0 + r, j = k0 + c;
+      pan[e] = (c < kb && j <= i && i - j <= bw) ? L[(size_t)i * w + (i - j)] : 0.0;
+    }
+    __syncthreads();
+    // factor the panel columns
+    for (int c = 0; c < kb; c++) {
+      if (t == 0) {
+        const double d = pan[c * kPanel + c];
+        pan[c * kPanel + c] = d > 0.0 ? sqrt(d) : 1e-150;
+      }
+      __syncthreads();
+      const double piv = pan[c * kPanel + c];
+      for (int r = c + 1 + t; r < rows; r += T) pan[r * kPanel + c] /= piv;
+      __syncthreads();
+      // columns c+1 .. kb-1 of rows below them
+      const int nc = kb - 1 - c;
+      for (int e = t; e < (rows - c - 1) * nc; e += T) {
+        const int r = c + 1 + e / nc, c2 = c + 1 + e % nc;
+        if (c2 <= r) pan[r * kPanel + c2] -= pan[r * kPanel + c] * pan[c2 * kPanel + c];
+      }
+      __syncthreads();
+    }
+    // write the panel back
+    for (int e = t; e < rows * kPanel; e += T) {
+      const int r = e / kPanel, c = e - r * kPanel;
+      const int i = k0 + r, j = k0 + c;
+      if (c < kb && j <= i && i - j <= bw) L[(size_t)i * w + (i - j)] = pan[e];
+    }
+    // trailing update: L(i, j) -= sum_c pan[i - k0][c] pan[j - k0][c] for k0 + kb <= j <= i < k0 + rows
+    const int m = rows - kb;  // trailing rows/columns
+    const int tiles = (m + 3) / 4;
+    const int ntile = tiles * (tiles + 1) / 2;
+    for (int q = t; q < ntile; q += T) {
+      int a = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+      while ((a + 1) * (a + 2) / 2 <= q) a++;
+      while (a * (a + 1) / 2 > q) a--;
+      const int b = q - a * (a + 1) / 2;  // tile (a, b), b <= a
+      double acc[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[u][v] = 0.0;
+      for (int c = 0; c < kb; c++) {
+        double pi[4], pj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int ri = kb + 4 * a + u, rj = kb + 4 * b + u;
+          pi[u] = ri < rows ? pan[ri * kPanel + c] : 0.0;
+          pj[u] = rj < rows ? pan[rj * kPanel + c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) acc[u][v] += pi[u] * pj[v];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int i = k0 + kb + 4 * a + u, j = k0 + kb + 4 * b + v;
+          if (i < k0 + rows && j <= i && i - j <= bw) L[(size_t)i * w + (i - j)] -= acc[u][v];
+        }
+    }
+    __syncthreads();
+  }
+}
+
 // one wave per column c of the inverse: L y = e_c, L^T x = y; x is row c of the symmetric inverse
 __global__ __launch_bounds__(64) void k_band_inverse(BandArgs B) {
   extern __shared__ double y[];  // n
@@ -180,13 +266,16 @@ void coarse_exact_alloc(Solver& s) {
   Level& L = s.lv[s.nlev - 1];
   const int n = 3 * L.N, bw = band_width(L.geo);
   const size_t lds = sizeof(double) * (size_t)(bw + 1) * (bw + 1);
-  if (lds > 150 * 1024)
-    throw std::runtime_error("exact coarse solve: coarsest level too wide for the LDS window (bandwidth " +
-                             std::to_string(bw) + ")");
   if ((size_t)n * sizeof(double) > 64 * 1024) throw std::runtime_error("exact coarse solve: coarsest level too large");
   BD_HIP(hipMalloc(&s.cband, sizeof(double) * (size_t)n * (bw + 1)));
   BD_HIP(hipMalloc(&s.cinv, sizeof(double) * (size_t)n * n));
-  BD_HIP(hipFuncSetAttribute((const void*)k_band_cholesky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  s.cwindow = lds <= 150 * 1024;
+  if (s.cwindow)
+    BD_HIP(hipFuncSetAttribute((const void*)k_band_cholesky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const size_t plds = sizeof(double) * (size_t)(kPanel + bw) * kPanel;
+  if (!s.cwindow && plds > 150 * 1024) throw std::runtime_error("exact coarse solve: coarsest band too wide");
+  if (!s.cwindow)
+    BD_HIP(hipFuncSetAttribute((const void*)k_band_cholesky_panel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
   s.cn = n;
   s.cbw = bw;
 }
@@ -198,7 +287,10 @@ void launch_coarse_factor(Solver& s) {
   // band load + factor (the window's rows in and out once) + inverse (the band read per column)
   ProfScope ps(s, PC_COARSE_FACTOR, 8.0 * ((double)s.cn * w * 3 + (double)s.cn * s.cn));
   k_band_load<<<grid_for(s.cn * w, 256), 256, 0, s.stream>>>(B);
-  k_band_cholesky<<<1, kCholThreads, sizeof(double) * (size_t)w * w, s.stream>>>(B);
+  if (s.cwindow)
+    k_band_cholesky<<<1, kCholThreads, sizeof(double) * (size_t)w * w, s.stream>>>(B);
+  else
+    k_band_cholesky_panel<<<1, kCholThreads, sizeof(double) * (size_t)(kPanel + s.cbw) * kPanel, s.stream>>>(B);
   k_band_inverse<<<s.cn, 64, sizeof(double) * (size_t)s.cn, s.stream>>>(B);
 }
 

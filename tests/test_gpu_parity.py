@@ -211,12 +211,14 @@ def test_dynamics_60_substeps(coupon_pair):
     assert err < 0.25 * moved
 
 
-def test_exact_coarse_solve(coupon_pair):
+@pytest.mark.parametrize("max_coarsest", [400, 1100])
+def test_exact_coarse_solve(coupon_pair, max_coarsest):
     """BD_COARSE_EXACT: the GPU's FP64 banded Cholesky + dense inverse of the coarsest level in the
-    V-cycle and PCG, against the oracle's banded Cholesky solve (bars as above)."""
+    V-cycle and PCG, against the oracle's banded Cholesky solve (bars as above). Coarsest 13 x 11
+    factors in the LDS window, 25 x 21 (bandwidth 158) in place in global memory."""
     from bd.solver import GpuSolver
     S, cpu, _ = coupon_pair
-    params = make_params(coarse_solve=1)
+    params = make_params(coarse_solve=1, max_coarsest_nodes=max_coarsest)
     gpu = GpuSolver(S, params)
     copy_state(cpu, gpu)
     ref = O.Oracle(S, params)
