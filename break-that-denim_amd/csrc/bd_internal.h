@@ -26,7 +26,9 @@ namespace bd {
 constexpr int kFineComps = 32;
 constexpr int kCoarseBlocks = 13;
 constexpr int kCoarseComps = 6 * kCoarseBlocks;
-constexpr int kDotBlocks = 2048;
+// grid of the deterministic reductions: one 1024-thread block per CU
+constexpr int kDotBlocks = 256;
+constexpr int kDotThreads = 1024;
 constexpr int kMaxLevels = BD_MAX_LEVELS;
 
 // forward offsets of the coarse stencil, block b = 1..12 (block 0 is the centre)

@@ -90,7 +90,7 @@ struct PcgArgs {
   float* zT;  // column-major copy of z (dual-layout fine level) or null
 };
 
-__global__ __launch_bounds__(256) void k_pcg_start(PcgArgs A) {
+__global__ __launch_bounds__(1024) void k_pcg_start(PcgArgs A) {
   double v[1] = {0};
   const int Np = A.Np;
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < A.N; n += gridDim.x * blockDim.x) {
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256) void k_pcg_start(PcgArgs A) {
 }
 
 // rz = r . z ; beta = rz / rz_prev (first, or after a residual replacement: beta = 0)
-__global__ __launch_bounds__(256) void k_dot_rz(PcgArgs A, int first) {
+__global__ __launch_bounds__(1024) void k_dot_rz(PcgArgs A, int first) {
   if (A.sc->done) return;
   double v[1] = {0};
   const int Np = A.Np;
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_update_p(PcgArgs A) {
   for (int q = 0; q < 3; q++) A.p[q * Np + n] = A.z[q * Np + n] + beta * A.p[q * Np + n];
 }
 
-__global__ __launch_bounds__(256) void k_update_xr(PcgArgs A) {
+__global__ __launch_bounds__(1024) void k_update_xr(PcgArgs A) {
   if (A.sc->done) return;
   const int Np = A.Np;
   const double alpha_d = A.sc->alpha;
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void k_update_xr(PcgArgs A) {
 // Residual replacement (tolerance mode): r = b - A d with d in FP64 and FP64 accumulation over the
 // FP32 operator; stop when the true relative residual meets the tolerance, otherwise continue from
 // the true residual with a restarted search direction (iterative refinement of the FP32 inner solve).
-__global__ __launch_bounds__(256) void k_true_residual(PcgArgs A, const float* op, Geo geo) {
+__global__ __launch_bounds__(1024) void k_true_residual(PcgArgs A, const float* op, Geo geo) {
   if (A.sc->done || !A.sc->need_true) return;
   const int Np = A.Np;
   double v[1] = {0};
@@ -247,7 +247,7 @@ __device__ __forceinline__ void blk_load6(const float* op, int Np, int base, int
   for (int q = 0; q < 6; q++) o[q] = op[(base + q) * Np + row];
 }
 
-__global__ __launch_bounds__(256) void k_matvec_dot_fine(PcgArgs A, const float* op, Geo geo) {
+__global__ __launch_bounds__(1024) void k_matvec_dot_fine(PcgArgs A, const float* op, Geo geo) {
   if (A.sc->done) return;
   const int Np = A.Np;
   double v[1] = {0};
@@ -618,7 +618,7 @@ PcgArgs pcg_args(Solver& s) {
 }
 
 int dot_grid(int N) {
-  int g = (N + 255) / 256;
+  int g = (N + kDotThreads - 1) / kDotThreads;
   return g < kDotBlocks ? g : kDotBlocks;
 }
 
@@ -634,13 +634,13 @@ void pcg_solve(Solver& s) {
   k_pcg_setup<<<1, 1, 0, s.stream>>>(s.scal, tol_mode ? s.P.pcg_rel_tol : 0.0, K, K);
   {
     ProfScope ps(s, PC_PCG_VECTOR, N * 85.0);
-    k_pcg_start<<<G, 256, 0, s.stream>>>(A);
+    k_pcg_start<<<G, kDotThreads, 0, s.stream>>>(A);
   }
   if (K == 0) return;
   vcycle(s, 0);
   {
     ProfScope ps(s, PC_PCG_VECTOR, N * 24.0);
-    k_dot_rz<<<G, 256, 0, s.stream>>>(A, 1);
+    k_dot_rz<<<G, kDotThreads, 0, s.stream>>>(A, 1);
   }
   {
     ProfScope ps(s, PC_PCG_VECTOR, N * 36.0);
@@ -650,15 +650,15 @@ void pcg_solve(Solver& s) {
     {
       // fine row (58 floats) + p read + Ap write
       ProfScope ps(s, PC_MATVEC, N * 256.0);
-      k_matvec_dot_fine<<<G, 256, 0, s.stream>>>(A, s.lv[0].op, s.lv[0].geo);
+      k_matvec_dot_fine<<<G, kDotThreads, 0, s.stream>>>(A, s.lv[0].op, s.lv[0].geo);
     }
     {
       ProfScope ps(s, PC_PCG_VECTOR, N * 108.0);
-      k_update_xr<<<G, 256, 0, s.stream>>>(A);
+      k_update_xr<<<G, kDotThreads, 0, s.stream>>>(A);
     }
     if (tol_mode) {
       ProfScope ps(s, PC_TRUE_RESIDUAL, 0.0);
-      k_true_residual<<<G, 256, 0, s.stream>>>(A, s.lv[0].op, s.lv[0].geo);
+      k_true_residual<<<G, kDotThreads, 0, s.stream>>>(A, s.lv[0].op, s.lv[0].geo);
     }
     if (it + 1 == K) break;
     if (tol_mode) {
@@ -670,7 +670,7 @@ void pcg_solve(Solver& s) {
     vcycle(s, 0);
     {
       ProfScope ps(s, PC_PCG_VECTOR, N * 24.0);
-      k_dot_rz<<<G, 256, 0, s.stream>>>(A, 0);
+      k_dot_rz<<<G, kDotThreads, 0, s.stream>>>(A, 0);
     }
     {
       ProfScope ps(s, PC_PCG_VECTOR, N * 36.0);
@@ -692,7 +692,8 @@ void launch_energy(Solver& s) {
                s.cut_mask, s.flags, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k, s.scal, s.partials,
                s.P.self_contact ? s.ext : nullptr, s.floor_on ? 1 : 0, (double)s.floor_z};
   ProfScope ps(s, PC_ENERGY, s.N * 161.0);
-  k_energy<<<dot_grid(s.N), 256, 0, s.stream>>>(E);
+  // 256-thread blocks keep the FP64 energy terms in registers
+  k_energy<<<std::min(kDotBlocks, grid_for(s.N, 256)), 256, 0, s.stream>>>(E);
 }
 
 }  // namespace bd
