@@ -414,6 +414,17 @@ def couple_spans_initial(S):
     return ruptures
 
 
+def default_scene_specs():
+    """defaultSceneSpecs (src/simulation/scene.ts:438-469): the product's 0.62 x 0.4 m worn denim
+    coupon, 9 x 6 bundles, collar rows, seed 1001 (182 x 115 = 20,930 nodes)."""
+    return [layer_spec(
+        DENIM, bundle=[9, 6], sizeU=0.62, sizeV=0.4, z=0, angle=0.06, compressU=0.025, compressV=-0.002,
+        shear=0.06, collarRows=2,
+        wear=dict(center=[0.01, -0.006], axis=[1, -1.15], radii=[0.2, 0.14], amount=1, cellSize=0.075,
+                  featureScale=0.05, defectThreshold=0.89, defectDensity=1, sparseFrom=0.6),
+        wearStrengthLoss=[0.45, 0.9], seed=1001, perturbation=2.5e-3)]
+
+
 def compile_scene(specs, sizes=None):
     """compileScene (scene.ts:233-278). `sizes` optionally fixes (nx, ny) per layer."""
     layers = []

@@ -146,6 +146,10 @@ class GpuSolver:
     def encode_slip(self):
         self._check(self.lib.bd_slip(self.h), "bd_slip")
 
+    def write_velocity(self, v):
+        """Overwrite the velocities (3N), e.g. the kinetic damping of the drape settle."""
+        self.write(BD_BUF_V, np.asarray(v, dtype=np.float64).ravel())
+
     def encode_contact_detect(self):
         """SelfContact.encodeDetect (gpu/selfcontact.ts:139-150)."""
         self._check(self.lib.bd_contact_detect(self.h), "bd_contact_detect")

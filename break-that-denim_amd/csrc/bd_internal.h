@@ -94,13 +94,13 @@ struct Scal {
 enum ProfClass {
   PC_PREDICT_COMMIT, PC_ASSEMBLE, PC_PROLONG_WEIGHTS, PC_RAP, PC_SMOOTH_FINE, PC_SMOOTH_COARSE, PC_RESIDUAL_FINE,
   PC_RESIDUAL_COARSE, PC_RESTRICT, PC_PROLONG, PC_MATVEC, PC_PCG_VECTOR, PC_TRUE_RESIDUAL, PC_ENERGY, PC_DAMAGE, PC_SLIP,
-  PC_CONTACT_HASH, PC_CONTACT_DETECT, PC_CONTACT_TERMS, PC_TRANSPOSE, PC_COARSE_FACTOR, PC_COARSE_APPLY, PC_COUNT
+  PC_CONTACT_HASH, PC_CONTACT_DETECT, PC_CONTACT_TERMS, PC_TRANSPOSE, PC_COARSE_FACTOR, PC_COARSE_INVERSE, PC_COARSE_APPLY, PC_COUNT
 };
 inline const char* prof_name(int c) {
   static const char* n[PC_COUNT] = {"predict_commit", "assemble", "prolong_weights", "rap", "smooth_fine", "smooth_coarse",
                                     "residual_fine", "residual_coarse", "restrict", "prolong", "matvec_dot", "pcg_vector",
                                     "true_residual", "energy", "damage", "slip", "contact_hash", "contact_detect",
-                                    "contact_terms", "transpose", "coarse_factor", "coarse_apply"};
+                                    "contact_terms", "transpose", "coarse_factor", "coarse_inverse", "coarse_apply"};
   return n[c];
 }
 

@@ -284,14 +284,20 @@ void launch_coarse_factor(Solver& s) {
   Level& L = s.lv[s.nlev - 1];
   BandArgs B{L.op, L.N, L.Np, L.geo, s.cn, s.cbw, s.cband, s.cinv};
   const int w = s.cbw + 1;
-  // band load + factor (the window's rows in and out once) + inverse (the band read per column)
-  ProfScope ps(s, PC_COARSE_FACTOR, 8.0 * ((double)s.cn * w * 3 + (double)s.cn * s.cn));
-  k_band_load<<<grid_for(s.cn * w, 256), 256, 0, s.stream>>>(B);
-  if (s.cwindow)
-    k_band_cholesky<<<1, kCholThreads, sizeof(double) * (size_t)w * w, s.stream>>>(B);
-  else
-    k_band_cholesky_panel<<<1, kCholThreads, sizeof(double) * (size_t)(kPanel + s.cbw) * kPanel, s.stream>>>(B);
-  k_band_inverse<<<s.cn, 64, sizeof(double) * (size_t)s.cn, s.stream>>>(B);
+  {
+    // band load + factor (the band in and out once)
+    ProfScope ps(s, PC_COARSE_FACTOR, 8.0 * ((double)s.cn * w * 3));
+    k_band_load<<<grid_for(s.cn * w, 256), 256, 0, s.stream>>>(B);
+    if (s.cwindow)
+      k_band_cholesky<<<1, kCholThreads, sizeof(double) * (size_t)w * w, s.stream>>>(B);
+    else
+      k_band_cholesky_panel<<<1, kCholThreads, sizeof(double) * (size_t)(kPanel + s.cbw) * kPanel, s.stream>>>(B);
+  }
+  {
+    // inverse: the band read per column, the inverse written once
+    ProfScope ps(s, PC_COARSE_INVERSE, 8.0 * (double)s.cn * s.cn);
+    k_band_inverse<<<s.cn, 64, sizeof(double) * (size_t)s.cn, s.stream>>>(B);
+  }
 }
 
 void launch_coarse_apply(Solver& s) {

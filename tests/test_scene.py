@@ -4,17 +4,8 @@ read as data: [0x50415244, 1, 20930, 4202117243])."""
 import numpy as np
 
 from bd.material import DENIM
-from bd.scene import compile_scene, layer_spec, scene_hash, area_code, SEG_BROKEN, SEG_INITIAL_DEFECT
+from bd.scene import compile_scene, default_scene_specs, scene_hash, area_code, SEG_BROKEN, SEG_INITIAL_DEFECT
 from bd import noise
-
-
-def default_scene_specs():  # scene.ts:438-469
-    return [layer_spec(
-        DENIM, bundle=[9, 6], sizeU=0.62, sizeV=0.4, z=0, angle=0.06, compressU=0.025, compressV=-0.002,
-        shear=0.06, collarRows=2,
-        wear=dict(center=[0.01, -0.006], axis=[1, -1.15], radii=[0.2, 0.14], amount=1, cellSize=0.075,
-                  featureScale=0.05, defectThreshold=0.89, defectDensity=1, sparseFrom=0.6),
-        wearStrengthLoss=[0.45, 0.9], seed=1001, perturbation=2.5e-3)]
 
 
 def test_default_scene_hash_matches_drape_header():
