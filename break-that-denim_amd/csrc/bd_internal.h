@@ -185,6 +185,7 @@ struct Solver {
   bool cwindow = true;  // factor in an LDS window (else in place in global memory)
   double* cband = nullptr;
   double* cinv = nullptr;
+  double* crd = nullptr;   // reciprocal pivots of the factor
 };
 
 constexpr int kContactK = 4;

@@ -153,7 +153,7 @@ void destroy_all(Solver& s) {
                   s.layer_of, s.rest, s.intact, s.fail_site, s.tails, s.softening, s.cut_mask, s.flags, s.strain,
                   s.damage, s.area, s.sep, s.pullmark, s.strength, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k,
                   s.partials, s.scal, s.counters, s.ijl, s.cells, s.scan_sums, s.items, s.pairs, s.keys,
-                  s.marks, s.cmeta, s.acc, s.ext, s.cband, s.cinv};
+                  s.marks, s.cmeta, s.acc, s.ext, s.cband, s.cinv, s.crd};
   for (void* p : ptrs) dfree(p);
   for (int k = 0; k < s.nlev; k++) {
     Level& L = s.lv[k];

@@ -140,16 +140,14 @@ __global__ __launch_bounds__(kCholThreads) void k_band_cholesky_panel(BandArgs B
       pan[e] = (c < kb && j <= i && i - j <= bw) ? L[(size_t)i * w + (i - j)] : 0.0;
     }
     __syncthreads();
-    // factor the panel columns
+    // factor the panel columns: every thread takes the pivot itself, the diagonal is stored in the
+    // update phase (which never reads it), so each column costs two barriers
     for (int c = 0; c < kb; c++) {
-      if (t == 0) {
-        const double d = pan[c * kPanel + c];
-        pan[c * kPanel + c] = d > 0.0 ? sqrt(d) : 1e-150;
-      }
-      __syncthreads();
-      const double piv = pan[c * kPanel + c];
+      const double d = pan[c * kPanel + c];
+      const double piv = d > 0.0 ? sqrt(d) : 1e-150;
       for (int r = c + 1 + t; r < rows; r += T) pan[r * kPanel + c] /= piv;
       __syncthreads();
+      if (t == 0) pan[c * kPanel + c] = piv;
       // columns c+1 .. kb-1 of rows below them
       const int nc = kb - 1 - c;
       for (int e = t; e < (rows - c - 1) * nc; e += T) {
@@ -203,31 +201,101 @@ __global__ __launch_bounds__(kCholThreads) void k_band_cholesky_panel(BandArgs B
   }
 }
 
-// one wave per column c of the inverse: L y = e_c, L^T x = y; x is row c of the symmetric inverse
-__global__ __launch_bounds__(64) void k_band_inverse(BandArgs B) {
-  extern __shared__ double y[];  // n
+// one wave per column c of the inverse: L y = e_c, L^T x = y; x is row c of the symmetric inverse.
+// Each lane holds up to kInvPer band entries of the current row (bandwidth < 64 kInvPer) and the next
+// row's are loaded before the current row's reduction; the last bandwidth values of y (forward) and
+// x (backward) sit in a 256-entry LDS ring, y itself goes to the output row (read back, prefetched,
+// in the backward pass). 4 KB of LDS per wave keeps every column's wave resident at once.
+constexpr int kInvPer = 4;
+constexpr int kRing = 64 * kInvPer;
+
+__global__ __launch_bounds__(256) void k_band_recip(const double* L, int n, int w, double* rd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) rd[i] = 1.0 / L[(size_t)i * w];
+}
+
+__global__ __launch_bounds__(64) void k_band_inverse(BandArgs B, const double* rd) {
+  __shared__ double ring[kRing];
   const int n = B.n, bw = B.bw, w = bw + 1;
   const int c = blockIdx.x, lane = threadIdx.x;
   const double* L = B.L;
-  for (int i = lane; i < n; i += 64) y[i] = 0.0;
+  double* out = B.inv + (size_t)c * n;
+  for (int q = lane; q < kRing; q += 64) ring[q] = 0.0;
+  for (int i = lane; i < c; i += 64) out[i] = 0.0;
   __syncthreads();
-  for (int i = c; i < n; i++) {
-    double s = 0.0;
+  double cur[kInvPer], nxt[kInvPer];
+  // forward: row i uses L(i, k), k in [max(c, i - bw), i)
+  auto load_fwd = [&](int i, double (&dst)[kInvPer]) {
     const int k0 = max(c, i - bw);
-    for (int k = k0 + lane; k < i; k += 64) s += L[(size_t)i * w + (i - k)] * y[k];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) y[i] = ((i == c ? 1.0 : 0.0) - s) / L[(size_t)i * w];
-    __syncthreads();
-  }
-  for (int i = n - 1; i >= 0; i--) {
+#pragma unroll
+    for (int m = 0; m < kInvPer; m++) {
+      const int k = k0 + lane + 64 * m;
+      dst[m] = (i < n && k < i) ? L[(size_t)i * w + (i - k)] : 0.0;
+    }
+  };
+  load_fwd(c, cur);
+  double r_cur = rd[c], r_nxt = 0.0;
+  for (int i = c; i < n; i++) {
+    load_fwd(i + 1, nxt);
+    if (i + 1 < n) r_nxt = rd[i + 1];
+    const int k0 = max(c, i - bw);
     double s = 0.0;
-    const int k1 = min(n - 1, i + bw);
-    for (int k = i + 1 + lane; k <= k1; k += 64) s += L[(size_t)k * w + (k - i)] * y[k];
+#pragma unroll
+    for (int m = 0; m < kInvPer; m++) {
+      const int k = k0 + lane + 64 * m;
+      if (k < i) s += cur[m] * ring[k & (kRing - 1)];
+    }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) y[i] = (y[i] - s) / L[(size_t)i * w];
+    if (lane == 0) {
+      const double yi = ((i == c ? 1.0 : 0.0) - s) * r_cur;
+      ring[i & (kRing - 1)] = yi;
+      out[i] = yi;
+    }
     __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kInvPer; m++) cur[m] = nxt[m];
+    r_cur = r_nxt;
   }
-  for (int i = lane; i < n; i += 64) B.inv[(size_t)c * n + i] = y[i];
+  for (int q = lane; q < kRing; q += 64) ring[q] = 0.0;
+  __syncthreads();
+  // backward: row i uses L(k, i), k in (i, min(n - 1, i + bw)]; y_i is read back from the output row
+  // (written by lane 0, which also reads it)
+  auto load_bwd = [&](int i, double (&dst)[kInvPer]) {
+    const int k1 = min(n - 1, i + bw);
+#pragma unroll
+    for (int m = 0; m < kInvPer; m++) {
+      const int k = i + 1 + lane + 64 * m;
+      dst[m] = (i >= 0 && k <= k1) ? L[(size_t)k * w + (k - i)] : 0.0;
+    }
+  };
+  load_bwd(n - 1, cur);
+  r_cur = rd[n - 1];
+  double y_cur = lane == 0 ? out[n - 1] : 0.0, y_nxt = 0.0;
+  for (int i = n - 1; i >= 0; i--) {
+    load_bwd(i - 1, nxt);
+    if (i > 0) {
+      r_nxt = rd[i - 1];
+      if (lane == 0) y_nxt = out[i - 1];
+    }
+    const int k1 = min(n - 1, i + bw);
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < kInvPer; m++) {
+      const int k = i + 1 + lane + 64 * m;
+      if (k <= k1) s += cur[m] * ring[k & (kRing - 1)];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) {
+      const double xi = (y_cur - s) * r_cur;
+      ring[i & (kRing - 1)] = xi;
+      out[i] = xi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kInvPer; m++) cur[m] = nxt[m];
+    r_cur = r_nxt;
+    y_cur = y_nxt;
+  }
 }
 
 // x = A_c^-1 b on the coarsest level: one wave per row, FP64 products and sums
@@ -266,9 +334,10 @@ void coarse_exact_alloc(Solver& s) {
   Level& L = s.lv[s.nlev - 1];
   const int n = 3 * L.N, bw = band_width(L.geo);
   const size_t lds = sizeof(double) * (size_t)(bw + 1) * (bw + 1);
-  if ((size_t)n * sizeof(double) > 64 * 1024) throw std::runtime_error("exact coarse solve: coarsest level too large");
+  if (bw >= kRing) throw std::runtime_error("exact coarse solve: coarsest band wider than 255");
   BD_HIP(hipMalloc(&s.cband, sizeof(double) * (size_t)n * (bw + 1)));
   BD_HIP(hipMalloc(&s.cinv, sizeof(double) * (size_t)n * n));
+  BD_HIP(hipMalloc(&s.crd, sizeof(double) * (size_t)n));
   s.cwindow = lds <= 150 * 1024;
   if (s.cwindow)
     BD_HIP(hipFuncSetAttribute((const void*)k_band_cholesky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -296,7 +365,8 @@ void launch_coarse_factor(Solver& s) {
   {
     // inverse: the band read per column, the inverse written once
     ProfScope ps(s, PC_COARSE_INVERSE, 8.0 * (double)s.cn * s.cn);
-    k_band_inverse<<<s.cn, 64, sizeof(double) * (size_t)s.cn, s.stream>>>(B);
+    k_band_recip<<<grid_for(s.cn, 256), 256, 0, s.stream>>>(s.cband, s.cn, w, s.crd);
+    k_band_inverse<<<s.cn, 64, 0, s.stream>>>(B, s.crd);
   }
 }
 
