@@ -233,3 +233,42 @@ def config_flaps(nx=48, ny=64, pitch=1e-3, gap=2e-3, floor_gap=0.05e-3, newton=3
                          self_contact=True)
     return {"name": f"flaps{nx}x{ny}", "scene": S, "params": params, "grips": [], "floor_z": -(half + floor_gap),
             "steps": 24}
+
+
+def plies_scene(nx=40, ny=32, plies=2, gap=0.75e-3, turn=0.35, perturbation=0.3e-3):
+    """A stack of denim plies compiled by the reference's scene compiler (one LayerSpec per ply,
+    scene.ts:233-278), each mounted by its collar row (pinned): ply 0 at z = 0, each ply above it
+    `gap` higher, 4 nodes narrower and turned by `turn` rad against the one below. The contact
+    distance of two denim plies is 0.7 mm (half the summed 0.7 mm thicknesses)."""
+    from .material import DENIM
+    from .scene import compile_scene, coupon
+    specs, sizes = [], []
+    for l in range(plies):
+        n_u, n_v = nx - 4 * l, ny - 4 * l
+        specs.append(coupon(n_u, n_v, material=DENIM, z=l * gap, angle=l * turn, collarRows=1,
+                            seed=11 + l, perturbation=perturbation if l == 0 else 0.0))
+        sizes.append((n_u, n_v))
+    return compile_scene(specs, sizes)
+
+
+def centre_grip(S, layer, k, size=5):
+    """A size x size patch grip at the centre of a layer's lattice (the ReferenceGrip quadratic
+    spring, solver.ts:25-37): equal weights, goal at the patch centroid, offsets from it."""
+    L = S.layers[layer]
+    i0, j0 = (L.nx - size) // 2, (L.ny - size) // 2
+    nodes = [L.node_offset + (j0 + b) * L.nx + (i0 + a) for b in range(size) for a in range(size)]
+    pos = S.position[nodes]
+    goal = pos.mean(0)
+    return {"nodes": nodes, "weights": [1.0 / len(nodes)] * len(nodes), "offsets": pos - goal, "goal": goal.copy(), "k": k}
+
+
+def config_plies(nx=40, ny=32, plies=2, gap=0.75e-3, press=0.1e-3, newton=3, rel_tol=1e-5):
+    """The top ply is pressed onto the ones below by a 5 x 5 grip at its centre whose goal moves
+    down `press` per substep (12 mm/s at h = 1/120), with cross-layer contact (the design's
+    stacked plies, each mounted by its collar)."""
+    S = plies_scene(nx, ny, plies, gap)
+    params = make_params(h=1 / 120, newton_iterations=newton, pcg_rel_tol=rel_tol, pcg_max_iterations=100,
+                         max_coarsest_nodes=400, coarse_sweeps=2, lag_coarse=True, line_search=True, history=False)
+    grip = centre_grip(S, plies - 1, k=2e4)
+    return {"name": f"plies{plies}x{nx}x{ny}", "scene": S, "params": params, "grips": [grip],
+            "velocity": [np.array([0.0, 0.0, -press])], "steps": 24}

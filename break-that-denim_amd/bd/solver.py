@@ -38,6 +38,9 @@ class GpuSolver:
         if rc != 0:
             raise RuntimeError(f"bd_create failed ({rc}): {self.lib.bd_last_error(None).decode()}")
         self.h = h
+        # a scene of several plies has cross-layer contact; a one-layer scene self-contact when asked
+        # for (gpu/solver.ts:322, 364)
+        self.contact_mode = "plies" if len(scene.layers) > 1 else ("self" if self.params.self_contact else None)
         self._grips = None
         if grips:
             self.set_grips(grips)
@@ -151,11 +154,13 @@ class GpuSolver:
         self.write(BD_BUF_V, np.asarray(v, dtype=np.float64).ravel())
 
     def encode_contact_detect(self):
-        """SelfContact.encodeDetect (gpu/selfcontact.ts:139-150)."""
+        """SelfContact.encodeDetect (gpu/selfcontact.ts:139-150), or Contact.encodeDetect
+        (gpu/contact.ts:129-137) for a scene of several plies."""
         self._check(self.lib.bd_contact_detect(self.h), "bd_contact_detect")
 
     def encode_contact_terms(self):
-        """SelfContact.encodeTerms (gpu/selfcontact.ts:153-157)."""
+        """SelfContact.encodeTerms (gpu/selfcontact.ts:153-157), or Contact.encodeTerms
+        (gpu/contact.ts:139-142) for a scene of several plies."""
         self._check(self.lib.bd_contact_terms(self.h), "bd_contact_terms")
 
     def encode_project_start(self):
@@ -200,15 +205,18 @@ class GpuSolver:
         return self.read(BD_BUF_CMETA).reshape(-1, 2, 4)
 
     def pairs(self):
-        """(N, K + 2): K candidate faces, reciprocal key, row mark."""
-        return self.read(BD_BUF_PAIRS).reshape(-1, BD_CONTACT_K + 2)
+        """Self-contact: (N, K + 2) K candidate faces, reciprocal key, row mark. Plies: (N, K)
+        candidate nodes of the other layers."""
+        return self.read(BD_BUF_PAIRS).reshape(self.N, -1)
 
     def cells(self):
         return self.read(BD_BUF_CELLS)
 
     def items(self):
-        """(2N, 8) face records: centroid now, radius, centroid at start, face id bits."""
-        return self.read(BD_BUF_ITEMS).reshape(-1, 8)
+        """Self-contact: (2N, 8) face records (centroid now, radius, centroid at start, face id
+        bits). Plies: (N,) uint32 node ids in cell order."""
+        a = self.read(BD_BUF_ITEMS)
+        return a.view(np.uint32) if self.contact_mode == "plies" else a.reshape(-1, 8)
 
     @property
     def x(self): return self.vec(BD_BUF_X)

@@ -572,8 +572,8 @@ void launch_assemble(Solver& s) {
   A.grip_w = s.grip_w;
   A.grip_off = s.grip_off;
   A.grip_goal_k = s.grip_goal_k;
-  A.ext = s.P.self_contact ? s.ext : nullptr;
-  A.cmeta = s.P.self_contact ? s.cmeta : nullptr;
+  A.ext = s.contact_mode != kContactNone ? s.ext : nullptr;
+  A.cmeta = s.contact_mode != kContactNone ? s.cmeta : nullptr;
   A.floor_on = s.floor_on ? 1 : 0;
   A.floor_z = s.floor_z;
   {
@@ -594,7 +594,7 @@ void launch_newton_apply(Solver& s, int it) {
   A.last_update = s.last_update;
   A.scal = s.scal;
   A.max_update = &s.scal->max_update[it];
-  A.cmeta = s.P.self_contact ? s.cmeta : nullptr;
+  A.cmeta = s.contact_mode != kContactNone ? s.cmeta : nullptr;
   A.floor_on = s.floor_on ? 1 : 0;
   A.floor_z = s.floor_z;
   float gx = (float)s.P.gravity[0], gy = (float)s.P.gravity[1], gz = (float)s.P.gravity[2];

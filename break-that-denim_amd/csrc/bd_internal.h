@@ -165,7 +165,9 @@ struct Solver {
   bd_stats stats{};
   int coarse_valid_iteration = -1;
 
-  // contact: self-contact (node against intact cell) and the floor
+  // contact: self-contact (node against intact cell) of a one-layer scene, cross-layer contact
+  // (node against node of another ply) of a scene of several layers, and the floor
+  int contact_mode = 0;  // kContactNone, kContactSelf or kContactPlies
   bool floor_on = false;
   float floor_z = 0.f;
   uint32_t* ijl = nullptr;      // per node: i | j << 12 | layer << 24
@@ -179,6 +181,12 @@ struct Solver {
   int* acc = nullptr;           // 9 planes of fixed-point accumulators
   float* ext = nullptr;         // 9 planes: gradient xyz, Hessian xx xy xz yy yz zz
   bool contact_detected = false;
+  // cross-layer contact: a (cell, layer) node hash of 2 x 2^20 u32 (counts scanned into starts,
+  // then fill cursors), node ids in cell order, the K nearest foreign nodes in `pairs`
+  uint32_t* pcells = nullptr;
+  uint32_t* psums = nullptr;     // 1024 block sums of the scan + 1
+  uint32_t* pitems = nullptr;    // N node ids
+  float ply_dist[3] = {0.f, 0.f, 0.f};  // contact distance of layer pairs 0-1, 0-2, 1-2 (m)
 
   // exact coarsest solve (BD_COARSE_EXACT): banded Cholesky factor and dense inverse, FP64
   int cn = 0, cbw = 0;
@@ -188,7 +196,11 @@ struct Solver {
   double* crd = nullptr;   // reciprocal pivots of the factor
 };
 
+constexpr int kContactNone = 0, kContactSelf = 1, kContactPlies = 2;
 constexpr int kContactK = 4;
+constexpr int kPlyTableBits = 20;
+constexpr int kPlyTable = 1 << kPlyTableBits;
+constexpr int kMaxPlies = 3;
 constexpr int kTableBits = 17;
 constexpr int kTable = 1 << kTableBits;
 
@@ -221,6 +233,10 @@ void contact_reset_bounds(Solver& s);
 void launch_contact_detect(Solver& s);
 void launch_contact_terms(Solver& s);
 void launch_project_start(Solver& s);
+// plies.hip
+void ply_contact_alloc(Solver& s);
+void launch_ply_detect(Solver& s);
+void launch_ply_terms(Solver& s);
 // coarse.hip
 void coarse_exact_alloc(Solver& s);
 void launch_coarse_factor(Solver& s);

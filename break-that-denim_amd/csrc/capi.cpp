@@ -153,7 +153,7 @@ void destroy_all(Solver& s) {
                   s.layer_of, s.rest, s.intact, s.fail_site, s.tails, s.softening, s.cut_mask, s.flags, s.strain,
                   s.damage, s.area, s.sep, s.pullmark, s.strength, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k,
                   s.partials, s.scal, s.counters, s.ijl, s.cells, s.scan_sums, s.items, s.pairs, s.keys,
-                  s.marks, s.cmeta, s.acc, s.ext, s.cband, s.cinv, s.crd};
+                  s.marks, s.cmeta, s.acc, s.ext, s.cband, s.cinv, s.crd, s.pcells, s.psums, s.pitems};
   for (void* p : ptrs) dfree(p);
   for (int k = 0; k < s.nlev; k++) {
     Level& L = s.lv[k];
@@ -200,8 +200,18 @@ void build_hierarchy(Solver& s) {
   s.hierarchy = true;
 }
 
+void contact_terms(Solver& s) {
+  if (s.contact_mode == kContactSelf) launch_contact_terms(s);
+  else if (s.contact_mode == kContactPlies) launch_ply_terms(s);
+}
+
+void contact_detect(Solver& s) {
+  if (s.contact_mode == kContactSelf) launch_contact_detect(s);
+  else if (s.contact_mode == kContactPlies) launch_ply_detect(s);
+}
+
 void newton_iteration(Solver& s, int it, bool coarse) {
-  if (s.P.self_contact) launch_contact_terms(s);
+  contact_terms(s);
   launch_assemble(s);
   s.assembled = true;
   if (coarse || !s.hierarchy) build_hierarchy(s);
@@ -257,7 +267,11 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
     g_create_error = "bd_create: unknown coarse_solve";
     return BD_ERR_INVALID;
   }
-  if (prm->self_contact && sc->node_count > BD_CONTACT_MAX_NODES) {
+  if (sc->layer_count > BD_MAX_CONTACT_LAYERS) {
+    g_create_error = "bd_create: cross-layer contact supports at most 3 layers";
+    return BD_ERR_LIMIT;
+  }
+  if (prm->self_contact && sc->layer_count == 1 && sc->node_count > BD_CONTACT_MAX_NODES) {
     g_create_error = "bd_create: self-contact bound keys hold at most 2^20 - 1 nodes";
     return BD_ERR_LIMIT;
   }
@@ -359,7 +373,11 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
       BD_HIP(hipMemsetAsync(s.ext, 0, 9 * (size_t)Np * sizeof(float), s.stream));
       s.cmeta = dalloc<float4>(2 * (size_t)Np);
       contact_reset_bounds(s);
-      if (s.P.self_contact) contact_alloc(s);
+      // a scene of several plies always has cross-layer contact; a one-layer scene self-contact
+      // when asked for (gpu/solver.ts:322, 364)
+      s.contact_mode = s.nl > 1 ? kContactPlies : (s.P.self_contact ? kContactSelf : kContactNone);
+      if (s.contact_mode == kContactSelf) contact_alloc(s);
+      if (s.contact_mode == kContactPlies) ply_contact_alloc(s);
       BD_HIP(hipStreamSynchronize(s.stream));
     }
 
@@ -634,16 +652,18 @@ int bd_slip(bd_solver* h) {
 
 int bd_contact_detect(bd_solver* h) {
   return guarded(h, [&](Solver& s) {
-    if (!s.P.self_contact) return fail(h, BD_ERR_STATE, "bd_contact_detect: created without self_contact");
-    launch_contact_detect(s);
+    if (s.contact_mode == kContactNone)
+      return fail(h, BD_ERR_STATE, "bd_contact_detect: one-layer scene created without self_contact");
+    contact_detect(s);
     return BD_OK;
   });
 }
 
 int bd_contact_terms(bd_solver* h) {
   return guarded(h, [&](Solver& s) {
-    if (!s.P.self_contact) return fail(h, BD_ERR_STATE, "bd_contact_terms: created without self_contact");
-    launch_contact_terms(s);
+    if (s.contact_mode == kContactNone)
+      return fail(h, BD_ERR_STATE, "bd_contact_terms: one-layer scene created without self_contact");
+    contact_terms(s);
     return BD_OK;
   });
 }
@@ -670,8 +690,8 @@ int bd_substep(bd_solver* h, int min_it, int max_it) {
     const int n_max = std::min(BD_MAX_NEWTON, std::max(std::max(s.P.newton_max, n_it), max_it));
     launch_reset_counters(s);
     launch_predict(s);
-    if (s.P.self_contact) launch_contact_detect(s);
-    if (s.P.self_contact || s.floor_on) launch_project_start(s);
+    contact_detect(s);
+    if (s.contact_mode != kContactNone || s.floor_on) launch_project_start(s);
     int done = 0;
     for (int it = 0; it < n_max; it++) {
       if (it >= n_it) {
@@ -758,9 +778,11 @@ size_t bd_buffer_bytes(const bd_solver* h, bd_buffer buf, int level) {
     case BD_BUF_ENERGY: return 8 * sizeof(double);
     case BD_BUF_EXT: return 9 * (size_t)s.N * sizeof(float);
     case BD_BUF_CMETA: return 8 * (size_t)s.N * sizeof(float);
-    case BD_BUF_PAIRS: return s.cells ? (kContactK + 2) * (size_t)s.N * sizeof(uint32_t) : 0;
-    case BD_BUF_CELLS: return s.cells ? 2 * (size_t)kTable * sizeof(uint32_t) : 0;
-    case BD_BUF_ITEMS: return s.cells ? 16 * (size_t)s.N * sizeof(float) : 0;
+    case BD_BUF_PAIRS:
+      return s.cells ? (kContactK + 2) * (size_t)s.N * sizeof(uint32_t) : (s.pcells ? kContactK * (size_t)s.N * sizeof(uint32_t) : 0);
+    case BD_BUF_CELLS:
+      return s.cells ? 2 * (size_t)kTable * sizeof(uint32_t) : (s.pcells ? 2 * (size_t)kPlyTable * sizeof(uint32_t) : 0);
+    case BD_BUF_ITEMS: return s.cells ? 16 * (size_t)s.N * sizeof(float) : (s.pcells ? (size_t)s.N * sizeof(uint32_t) : 0);
     default: return 0;
   }
 }
@@ -900,7 +922,15 @@ int bd_read(bd_solver* h, bd_buffer buf, int level, void* dst, size_t bytes) {
       }
       case BD_BUF_CMETA: download(s, static_cast<float4*>(dst), s.cmeta, 2 * (size_t)s.N); break;
       case BD_BUF_PAIRS: {
-        std::vector<uint32_t> t(kContactK * (size_t)s.Np), k(s.N), m(s.N);
+        std::vector<uint32_t> t(kContactK * (size_t)s.Np);
+        if (s.contact_mode == kContactPlies) {
+          download(s, t.data(), s.pairs, t.size());
+          uint32_t* o = static_cast<uint32_t*>(dst);
+          for (int n = 0; n < s.N; n++)
+            for (int q = 0; q < kContactK; q++) o[kContactK * (size_t)n + q] = t[(size_t)q * s.Np + n];
+          break;
+        }
+        std::vector<uint32_t> k(s.N), m(s.N);
         download(s, t.data(), s.pairs, t.size());
         download(s, k.data(), s.keys, (size_t)s.N);
         download(s, m.data(), s.marks, (size_t)s.N);
@@ -912,8 +942,14 @@ int bd_read(bd_solver* h, bd_buffer buf, int level, void* dst, size_t bytes) {
         }
         break;
       }
-      case BD_BUF_CELLS: download(s, static_cast<uint32_t*>(dst), s.cells, 2 * (size_t)kTable); break;
-      case BD_BUF_ITEMS: download(s, static_cast<float4*>(dst), s.items, 4 * (size_t)s.N); break;
+      case BD_BUF_CELLS:
+        if (s.contact_mode == kContactPlies) download(s, static_cast<uint32_t*>(dst), s.pcells, 2 * (size_t)kPlyTable);
+        else download(s, static_cast<uint32_t*>(dst), s.cells, 2 * (size_t)kTable);
+        break;
+      case BD_BUF_ITEMS:
+        if (s.contact_mode == kContactPlies) download(s, static_cast<uint32_t*>(dst), s.pitems, (size_t)s.N);
+        else download(s, static_cast<float4*>(dst), s.items, 4 * (size_t)s.N);
+        break;
       default: return fail(h, BD_ERR_INVALID, "bd_read: unsupported buffer");
     }
     return BD_OK;

@@ -690,7 +690,7 @@ void launch_energy(Solver& s) {
   }
   EnergyArgs E{s.N, s.Np, s.geo, s.layers, s.P.h, s.x, s.dx, s.pred, s.mass, s.delta, s.cn0, s.pinned, s.rest, s.intact,
                s.cut_mask, s.flags, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k, s.scal, s.partials,
-               s.P.self_contact ? s.ext : nullptr, s.floor_on ? 1 : 0, (double)s.floor_z};
+               s.contact_mode != kContactNone ? s.ext : nullptr, s.floor_on ? 1 : 0, (double)s.floor_z};
   ProfScope ps(s, PC_ENERGY, s.N * 161.0);
   // 256-thread blocks keep the FP64 energy terms in registers
   k_energy<<<std::min(kDotBlocks, grid_for(s.N, 256)), 256, 0, s.stream>>>(E);

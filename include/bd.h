@@ -48,6 +48,15 @@ extern "C" {
 #define BD_CONTACT_NONE 0xFFFFFFFFu
 #define BD_CONTACT_MAX_NODES ((1 << 20) - 1)  /* the reciprocal-bound key holds node + 1 in 20 bits */
 
+/* Cross-layer contact of a scene of 2-3 plies (gpu/contact.ts:8-18 CONTACT, contact.wgsl): always on
+ * for a scene of several layers, in place of self-contact (gpu/solver.ts:322, 364). K nearest nodes
+ * of the other layers per node, a spatial hash of 2^20 (cell, layer) slots of 1 mm cells, quadratic
+ * penalty k = 3000 N/m below the pair's contact distance (half the summed layer thicknesses),
+ * friction 0.45 regularised below 2 mm/s, bound share 0.45. In that mode BD_BUF_PAIRS holds K N
+ * uint32 candidate nodes, BD_BUF_CELLS 2 x 2^20 uint32 and BD_BUF_ITEMS N uint32 node ids. */
+#define BD_PLY_TABLE_BITS 20
+#define BD_MAX_CONTACT_LAYERS 3  /* SimParams holds 3 layer materials (common.wgsl:24-33) */
+
 /* Yarn family law and constants (src/simulation/material.ts:18-54). */
 typedef struct bd_yarn {
   double linear_density;    /* kg/m */

@@ -683,3 +683,172 @@ void oc_project_start(const oc_scene* S, const float* cmeta, float* dx, int floo
     memcpy(&dx[3 * n], d, sizeof d);
   }
 }
+
+/* ------------------------------------------------------------------------------------------------
+ * Cross-layer contact (src/simulation/gpu/wgsl/contact.wgsl:1-164, law gpu/contact.ts:8-18, pair
+ * distances gpu/contact.ts:52-59). A node hash keyed by (1 mm cell, layer) over 2^20 slots, the K
+ * nearest nodes of the other layers within reach, the directional bound to the nearest foreign node,
+ * and per node a quadratic penalty with regularised Coulomb friction. Ties between equally near
+ * nodes go to the lower node index (the reference keeps whichever it met first; its fill order is
+ * that of its atomics). dist3 holds the contact distances of layer pairs 0-1, 0-2 and 1-2. cells is
+ * 2 x 2^20 (starts, then counts), items N node ids, pairs K per node, ext 9 per node.
+ * ------------------------------------------------------------------------------------------------ */
+#define OP_TABLE (1u << 20)
+#define OP_MASK (OP_TABLE - 1u)
+#define OP_CELL 1.0e-3f
+#define OP_STIFF 3000.f
+#define OP_MU 0.45f
+#define OP_MU_SPEED 0.002f
+#define OP_FRACTION 0.45f
+
+static int op_cell_of(float p) {
+  float q = floorf(p / OP_CELL);
+  return (int)fminf(fmaxf(q, -1e9f), 1e9f);
+}
+
+uint32_t oc_ply_hash_cell(int cx, int cy, int cz, uint32_t layer) {
+  return (((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u) ^ (layer * 2654435761u)) &
+         OP_MASK;
+}
+
+static uint32_t op_node_hash(const oc_scene* S, uint32_t n) {
+  float p[3];
+  at_now(S, n, p);
+  return oc_ply_hash_cell(op_cell_of(p[0]), op_cell_of(p[1]), op_cell_of(p[2]), (uint32_t)layer_of(S, n));
+}
+
+static float op_pair_distance(const float* dist3, int la, int lb) {
+  int s = la + lb;
+  return s == 1 ? dist3[0] : (s == 2 ? dist3[1] : dist3[2]);
+}
+
+/* hashCount, the exclusive scan and hashFill (contact.wgsl:62-81): nodes in ascending order */
+static void op_build_hash(const oc_scene* S, uint32_t* cells, uint32_t* items) {
+  memset(cells, 0, 2 * OP_TABLE * sizeof(uint32_t));
+  for (uint32_t n = 0; n < (uint32_t)S->N; n++) cells[op_node_hash(S, n)]++;
+  uint32_t run = 0;
+  for (uint32_t h = 0; h < OP_TABLE; h++) {
+    uint32_t c = cells[h];
+    cells[h] = run;
+    run += c;
+  }
+  for (uint32_t n = 0; n < (uint32_t)S->N; n++) {
+    uint32_t h = op_node_hash(S, n);
+    items[cells[h] + cells[OP_TABLE + h]++] = n;
+  }
+}
+
+/* detect (contact.wgsl:83-137) */
+void oc_ply_detect(const oc_scene* S, const float* dist3, uint32_t* cells, uint32_t* items, uint32_t* pairs, float* cmeta) {
+  op_build_hash(S, cells, items);
+  const float reach = 0.25f * OP_CELL;
+  for (uint32_t n = 0; n < (uint32_t)S->N; n++) {
+    float p[3], q[3], dv[3];
+    at_now(S, n, p);
+    int ln = layer_of(S, n);
+    uint32_t best[OC_K];
+    float bestD[OC_K];
+    for (int k = 0; k < OC_K; k++) {
+      best[k] = OC_NONE;
+      bestD[k] = OC_UNB;
+    }
+    float dmin = OC_UNB, nmin[3] = {0.f, 0.f, 0.f};
+    uint32_t mmin = OC_NONE;
+    int cx = op_cell_of(p[0]), cy = op_cell_of(p[1]), cz = op_cell_of(p[2]);
+    for (int lo = 0; lo < S->nl; lo++) {
+      if (lo == ln) continue;
+      float r = op_pair_distance(dist3, ln, lo);
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+          for (int dxc = -1; dxc <= 1; dxc++) {
+            uint32_t h = oc_ply_hash_cell(cx + dxc, cy + dy, cz + dz, (uint32_t)lo);
+            uint32_t start = cells[h], count = cells[OP_TABLE + h];
+            for (uint32_t t = 0; t < count; t++) {
+              uint32_t m = items[start + t];
+              if (layer_of(S, m) != lo) continue;
+              at_now(S, m, q);
+              sub3f(p, q, dv);
+              float d = len3f(dv);
+              float e = d - r;
+              if ((e < dmin || (e == dmin && m < mmin)) && d > 1e-9f) {
+                dmin = e;
+                mmin = m;
+                for (int a = 0; a < 3; a++) nmin[a] = dv[a] / d;
+              }
+              if (d >= r + reach) continue;
+              uint32_t cand = m;
+              float cd = e;
+              for (int k = 0; k < OC_K; k++) {
+                if (cd < bestD[k] || (cd == bestD[k] && cand < best[k])) {
+                  uint32_t ts = best[k];
+                  float td = bestD[k];
+                  best[k] = cand;
+                  bestD[k] = cd;
+                  cand = ts;
+                  cd = td;
+                }
+              }
+            }
+          }
+    }
+    for (int k = 0; k < OC_K; k++) pairs[OC_K * n + k] = best[k];
+    float* own = &cmeta[8 * n];
+    if (dmin < 1e8f) {
+      own[0] = nmin[0];
+      own[1] = nmin[1];
+      own[2] = nmin[2];
+      own[3] = OP_FRACTION * (fmaxf(0.f, dmin) + 0.5f * dist3[0]);
+    } else {
+      own[0] = own[1] = own[2] = 0.f;
+      own[3] = OC_UNB;
+    }
+    cmeta[8 * n + 4] = cmeta[8 * n + 5] = cmeta[8 * n + 6] = 0.f;
+    cmeta[8 * n + 7] = OC_UNB;
+  }
+}
+
+/* terms (contact.wgsl:139-164): ext = gradient xyz, Hessian xx xy xz yy yz zz per node */
+void oc_ply_terms(const oc_scene* S, const float* dist3, const uint32_t* pairs, float h, float* ext) {
+  const float eps_u = OP_MU_SPEED * h;
+  for (uint32_t n = 0; n < (uint32_t)S->N; n++) {
+    float g[3] = {0.f, 0.f, 0.f}, H[3][3] = {{0.f}};
+    float p[3], q[3], dv[3];
+    at_now(S, n, p);
+    int ln = layer_of(S, n);
+    for (int k = 0; k < OC_K; k++) {
+      uint32_t m = pairs[OC_K * n + k];
+      if (m == OC_NONE) continue;
+      at_now(S, m, q);
+      sub3f(p, q, dv);
+      float d = len3f(dv);
+      float r = op_pair_distance(dist3, ln, layer_of(S, m));
+      if (d >= r || d <= 1e-9f) continue;
+      float nn[3] = {dv[0] / d, dv[1] / d, dv[2] / d};
+      float lam = OP_STIFF * (r - d);
+      for (int a = 0; a < 3; a++) g[a] = g[a] - lam * nn[a];
+      float nnT[3][3];
+      for (int c = 0; c < 3; c++)
+        for (int rr = 0; rr < 3; rr++) nnT[c][rr] = nn[rr] * nn[c];
+      for (int c = 0; c < 3; c++)
+        for (int rr = 0; rr < 3; rr++) H[c][rr] = H[c][rr] + OP_STIFF * nnT[c][rr];
+      float rel[3], ut[3];
+      for (int a = 0; a < 3; a++) rel[a] = S->dx[3 * n + a] - S->dx[3 * m + a];
+      float rn = (rel[0] * nn[0] + rel[1] * nn[1]) + rel[2] * nn[2];
+      for (int a = 0; a < 3; a++) ut[a] = rel[a] - rn * nn[a];
+      float kf = OP_MU * lam / fmaxf(len3f(ut), eps_u);
+      for (int a = 0; a < 3; a++) g[a] = g[a] + kf * ut[a];
+      for (int c = 0; c < 3; c++)
+        for (int rr = 0; rr < 3; rr++) H[c][rr] = H[c][rr] + kf * ((rr == c ? 1.f : 0.f) - nnT[c][rr]);
+    }
+    float* e = &ext[9 * n];
+    e[0] = g[0];
+    e[1] = g[1];
+    e[2] = g[2];
+    e[3] = H[0][0];
+    e[4] = H[0][1];
+    e[5] = H[0][2];
+    e[6] = H[1][1];
+    e[7] = H[1][2];
+    e[8] = H[2][2];
+  }
+}

@@ -88,6 +88,11 @@ def lib():
         L.oc_floor_bound_z.argtypes = [C.c_float] * 7
         L.oc_floor_bound_z.restype = C.c_float
         L.oc_project_start.argtypes = [P(OcScene), fp, fp, C.c_int, C.c_float, fp, C.c_float]
+        # FP32 cross-layer contact restatement (contact_oracle.c)
+        L.oc_ply_detect.argtypes = [P(OcScene), fp, up, up, up, fp]
+        L.oc_ply_terms.argtypes = [P(OcScene), fp, up, C.c_float, fp]
+        L.oc_ply_hash_cell.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32]
+        L.oc_ply_hash_cell.restype = C.c_uint32
         L.or_rhs.argtypes = [vp, dp]
         L.or_node_gradient.argtypes = [vp, C.c_int, dp]
         L.or_potential.argtypes = [vp]
@@ -454,6 +459,35 @@ class ContactTwin:
 
     def face_valid(self, f):
         return bool(lib().oc_face_valid(C.byref(self.S), int(f)))
+
+
+class PlyTwin(ContactTwin):
+    """The FP32 restatement of the reference's cross-layer contact (contact.wgsl, in contact_oracle.c)
+    on one scene state of 2-3 layers: K nearest foreign nodes per node, bounds and terms."""
+
+    TABLE = 1 << 20
+
+    def __init__(self, scene, x, dx, **kw):
+        super().__init__(scene, x, dx, **kw)
+        t = [lay.material.get("thickness", 0.0) for lay in scene.layers] + [0.0, 0.0]
+        # gpu/contact.ts:52-59: half the summed thicknesses, in f64, stored as f32
+        self.dist3 = np.array([0.5 * (t[0] + t[1]), 0.5 * (t[0] + t[2]), 0.5 * (t[1] + t[2])], dtype=np.float32)
+
+    def detect(self):
+        """hashCount .. detect: returns cells (2T,), items (N,), pairs (N, K), cmeta (N, 2, 4)."""
+        cells = np.zeros(2 * self.TABLE, dtype=np.uint32)
+        items = np.zeros(max(self.N, 1), dtype=np.uint32)
+        pairs = np.zeros((self.N, self.K), dtype=np.uint32)
+        cmeta = np.zeros((self.N, 2, 4), dtype=np.float32)
+        lib().oc_ply_detect(C.byref(self.S), _fp(self.dist3), _up(cells), _up(items), _up(pairs), _fp(cmeta))
+        return cells, items[:self.N], pairs, cmeta
+
+    def terms(self, pairs, h):
+        """terms: returns ext (N, 9) float32 (gradient xyz, Hessian xx xy xz yy yz zz)."""
+        ext = np.zeros((self.N, 9), dtype=np.float32)
+        pr = np.ascontiguousarray(pairs, dtype=np.uint32)
+        lib().oc_ply_terms(C.byref(self.S), _fp(self.dist3), _up(pr), C.c_float(h), _fp(ext))
+        return ext
 
 
 def _fp(a):
