@@ -26,7 +26,11 @@ BD_COARSE_EXACT = 1
 (BD_BUF_X, BD_BUF_V, BD_BUF_DX, BD_BUF_PRED, BD_BUF_DELTA, BD_BUF_RHS, BD_BUF_LEVEL_X, BD_BUF_LEVEL_B,
  BD_BUF_LEVEL_R, BD_BUF_STENCIL, BD_BUF_FIXED, BD_BUF_PROLONG, BD_BUF_REST, BD_BUF_INTACT, BD_BUF_CUTMASK,
  BD_BUF_SEGFLAGS, BD_BUF_FAILSITE, BD_BUF_TAILS, BD_BUF_FIELD, BD_BUF_LAST_UPDATE, BD_BUF_ENERGY,
- BD_BUF_EXT, BD_BUF_CMETA, BD_BUF_PAIRS, BD_BUF_CELLS, BD_BUF_ITEMS) = range(26)
+ BD_BUF_EXT, BD_BUF_CMETA, BD_BUF_PAIRS, BD_BUF_CELLS, BD_BUF_ITEMS, BD_BUF_STATE, BD_BUF_RIGIDK, BD_BUF_PIECES,
+ BD_BUF_CURVE) = range(30)
+
+BD_MAX_PIECES = 4096
+BD_NO_PIECE = 0xFFFFFFFF
 
 BD_CONTACT_K = 4
 BD_CONTACT_TABLE_BITS = 17
@@ -245,6 +249,11 @@ def load_library(path: os.PathLike | None = None):
     for name in ("bd_contact_detect", "bd_contact_terms", "bd_project_start"):
         getattr(lib, name).argtypes = [vp]
     lib.bd_set_floor.argtypes = [vp, C.c_int, C.c_double]
+    u32p, i32p, u8p = P(C.c_uint32), P(C.c_int32), P(C.c_uint8)
+    lib.bd_set_pieces.argtypes = [vp, u32p, u32p, C.c_int, u32p]
+    lib.bd_update_topology.argtypes = [vp, i32p, C.c_int, i32p]
+    lib.bd_topology_search.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, u8p, u8p, i32p, C.c_int, u8p, u32p, u32p,
+                                       u32p, i32p]
     if path is None:
         _lib = lib
     return lib
@@ -258,4 +267,5 @@ EXPORTED = (
     "bd_line_search", "bd_newton_apply", "bd_finalize", "bd_damage", "bd_slip", "bd_sync", "bd_read",
     "bd_write", "bd_buffer_bytes", "bd_get_stats", "bd_set_grip_goals", "bd_profile_enable", "bd_profile_read",
     "bd_contact_detect", "bd_contact_terms", "bd_project_start", "bd_set_floor",
+    "bd_set_pieces", "bd_update_topology", "bd_topology_search",
 )

@@ -15,12 +15,13 @@ from .capi import (BD_BUF_CELLS, BD_BUF_CMETA, BD_BUF_EXT, BD_BUF_ITEMS, BD_BUF_
                    BD_BUF_CUTMASK, BD_BUF_DELTA, BD_BUF_DX, BD_BUF_ENERGY, BD_BUF_FAILSITE, BD_BUF_FIELD,
                    BD_BUF_FIXED, BD_BUF_INTACT, BD_BUF_LAST_UPDATE, BD_BUF_LEVEL_B, BD_BUF_LEVEL_R, BD_BUF_LEVEL_X,
                    BD_BUF_PRED, BD_BUF_PROLONG, BD_BUF_REST, BD_BUF_RHS, BD_BUF_SEGFLAGS, BD_BUF_STENCIL,
-                   BD_BUF_TAILS, BD_BUF_V, BD_BUF_X, GripBuffers, SceneBuffers, bd_stats, make_params)
+                   BD_BUF_TAILS, BD_BUF_V, BD_BUF_X, BD_BUF_STATE, BD_BUF_RIGIDK, BD_BUF_PIECES, BD_BUF_CURVE,
+                   GripBuffers, SceneBuffers, bd_stats, make_params)
 
 _DTYPES = {
     BD_BUF_FIXED: np.uint8, BD_BUF_CUTMASK: np.uint32, BD_BUF_SEGFLAGS: np.uint16, BD_BUF_FIELD: np.uint8,
     BD_BUF_EXT: np.float32, BD_BUF_CMETA: np.float32, BD_BUF_PAIRS: np.uint32, BD_BUF_CELLS: np.uint32,
-    BD_BUF_ITEMS: np.float32,
+    BD_BUF_ITEMS: np.float32, BD_BUF_STATE: np.uint8, BD_BUF_RIGIDK: np.float32, BD_BUF_CURVE: np.float32,
 }
 
 
@@ -171,6 +172,45 @@ class GpuSolver:
         """setFloor(z | null) (gpu/solver.ts:796-801)."""
         self._check(self.lib.bd_set_floor(self.h, 0 if z is None else 1, 0.0 if z is None else float(z)), "bd_set_floor")
 
+    # ---- detached pieces (topology.ts, rigid_*.wgsl)
+    def set_pieces(self, piece_of, start, nodes):
+        """GpuSolver.setPieces (gpu/solver.ts:664-676)."""
+        of = np.ascontiguousarray(piece_of, dtype=np.uint32)
+        st = np.ascontiguousarray(start, dtype=np.uint32)
+        nd = np.ascontiguousarray(nodes if len(nodes) else [0], dtype=np.uint32)
+        u32 = C.POINTER(C.c_uint32)
+        self._check(self.lib.bd_set_pieces(self.h, of.ctypes.data_as(u32), st.ctypes.data_as(u32), len(st) - 1,
+                                           nd.ctypes.data_as(u32)), "bd_set_pieces")
+
+    def update_topology(self, anchors=()):
+        """Topology.request (topology.ts:98-160), synchronously: returns the number of nodes freed."""
+        a = np.ascontiguousarray(anchors if len(anchors) else [0], dtype=np.int32)
+        out = C.c_int32()
+        self._check(self.lib.bd_update_topology(self.h, a.ctypes.data_as(C.POINTER(C.c_int32)), len(anchors),
+                                                C.byref(out)), "bd_update_topology")
+        return out.value
+
+    def state(self):
+        """(N,) uint8 node states: 0 cloth, 1 detached, 2 retired."""
+        return self.read(BD_BUF_STATE)
+
+    def set_node_state(self, state):
+        self.write(BD_BUF_STATE, np.asarray(state, dtype=np.uint8))
+
+    def rigidk(self):
+        """(N, 2, 6) per detached node: external block, rotation stiffness (xx xy xz yy yz zz)."""
+        return self.read(BD_BUF_RIGIDK).reshape(-1, 2, 6)
+
+    def pieces(self):
+        """Per piece: centroid (3), valid, E^-1 (6, 6), rigid step t (3), w (3)."""
+        a = self.read(BD_BUF_PIECES).reshape(-1, 48)
+        return {"centroid": a[:, :3], "valid": a[:, 3], "Einv": a[:, 4:40].reshape(-1, 6, 6),
+                "t": a[:, 40:43], "w": a[:, 44:47]}
+
+    def curve(self):
+        """(N, 7): rotation w xyz, flag, arm r xyz."""
+        return self.read(BD_BUF_CURVE).reshape(-1, 7)
+
     def sync(self):
         self._check(self.lib.bd_sync(self.h), "bd_sync")
 
@@ -238,7 +278,11 @@ class GpuSolver:
 
     def energy(self):
         e = self.read(BD_BUF_ENERGY)
-        return e[:4], e[4:]
+        return e[:4], e[4:8]
+
+    def line_alpha(self):
+        """The step length the last line search chose."""
+        return float(self.read(BD_BUF_ENERGY)[8])
 
     def stats(self):
         st = bd_stats()

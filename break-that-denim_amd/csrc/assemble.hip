@@ -176,7 +176,23 @@ struct AsmArgs {
   const float4* cmeta;   // contact bounds or null
   int floor_on;
   float floor_z;
+  // detached pieces: node state and the per-node blocks of the rigid-mode correction (or null)
+  const uint8_t* state;
+  float* rigidk;
 };
+
+// rotationOfSpring (assemble.wgsl:20-27): a rigid rotation w moves the ends of a term's vector d by
+// w x d; a spring block kPar e e^T + kPerp (I - e e^T) along d resists it with kPerp (|d|^2 I - d d^T),
+// a scalar stencil k on a vector r with k (|r|^2 I - r r^T)
+__device__ __forceinline__ void add_rotation(Sym& S, float kperp, const float d[3]) {
+  float l2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  S.a[0] += kperp * (l2 - d[0] * d[0]);
+  S.a[1] -= kperp * (d[0] * d[1]);
+  S.a[2] -= kperp * (d[0] * d[2]);
+  S.a[3] += kperp * (l2 - d[1] * d[1]);
+  S.a[4] -= kperp * (d[1] * d[2]);
+  S.a[5] += kperp * (l2 - d[2] * d[2]);
+}
 
 __device__ __forceinline__ void add_neg_axial(float* blk, float kpar, float kperp, const float e[3]) {
   float d = kpar - kperp;
@@ -215,6 +231,13 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs A) {
   // inertia (newtonMultigrid.ts:184-188, solver.ts:391-392)
   float w = c.mass[n] / (h * h);
   sym_add_diag(C, w);
+  // a detached node's external block and the rotation stiffness of the terms it owns
+  // (assemble.wgsl:174-176, 323-327): the rigid-mode correction's E is summed from them
+  const bool detached = A.state && A.state[n] == kStateDetached;
+  Sym Kx, Kr;
+  sym_zero(Kx);
+  sym_zero(Kr);
+  sym_add_diag(Kx, w);
   float3 dn = ld3(c.dx, Np, n);
   float3 pn = ld3(c.pred, Np, n);
   g[0] += w * (dn.x - pn.x);
@@ -230,6 +253,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs A) {
     double xs[3] = {xn.x, xn.y, xn.z}, ds[3] = {dn.x, dn.y, dn.z};
     for (int q = 0; q < 3; q++) g[q] += (float)(kg * ((xs[q] - G[q]) + ds[q] - A.grip_off[q * Np + n]));
     sym_add_diag(C, (float)kg);
+    sym_add_diag(Kx, (float)kg);
   }
 
   // implicit air drag (solver.ts:132-165), from the accepted positions
@@ -283,7 +307,10 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs A) {
     sym_add_axial(D, 1.f, kpar, kperp, e);
     float dv[3] = {dn.x, dn.y, dn.z}, Dd[3];
     sym_mul(D.a, dv, Dd);
-    for (int q = 0; q < 6; q++) C.a[q] += D.a[q];
+    for (int q = 0; q < 6; q++) {
+      C.a[q] += D.a[q];
+      Kx.a[q] += D.a[q];
+    }
     g[0] += Dd[0];
     g[1] += Dd[1];
     g[2] += Dd[2];
@@ -300,6 +327,11 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs A) {
       g[1] -= fs * e[1];
       g[2] -= fs * e[2];
       if (!c.pinned[nn]) add_neg_axial(blk[f], kpar, kperp, e);
+      if (detached) {
+        float d[3];
+        rel(c, n, nn, d);
+        add_rotation(Kr, kperp, d);
+      }
     }
     int pp = nbr(P, n, f, -1);
     if (pp >= 0 && active(c, 2 * pp + f)) {
@@ -318,6 +350,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs A) {
     float r[3], wa, wc, k, kv;
     if (bend_triple(c, L, P, h, n, f, a, cc, r, wa, wc, k, kv)) {  // n is the middle node
       sym_add_diag(C, 4.f * k);
+      if (detached) add_rotation(Kr, k, r);
       for (int q = 0; q < 3; q++) g[q] += -2.f * kv * r[q];
       if (!c.pinned[cc]) {
         float s = -2.f * wc * k;
@@ -367,6 +400,11 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs A) {
         sym_add_axial(C, 1.f, kpar, kperp, u);
         for (int z = 0; z < 3; z++) g[z] -= fsh * u[z];
         if (!c.pinned[q]) add_neg_axial(ne, kpar, kperp, u);
+        if (detached) {
+          float d[3];
+          rel(c, n, q, d);
+          add_rotation(Kr, kperp, d);
+        }
       }
     }
     // cell (i-1, j): diagonal 1 from r = n to t = (i-1, j+1)
@@ -392,6 +430,12 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs A) {
         shear_diag(c, ks, jam, Ld, p, n, u, fsh, kpar, kperp);
         sym_add_axial(C, 1.f, kpar, kperp, u);
         for (int z = 0; z < 3; z++) g[z] += fsh * u[z];
+        // the diagonals of cell (i, j) and, across it, of cell (i, j - 1) each count at one end
+        if (detached) {
+          float d[3];
+          rel(c, n, p, d);
+          add_rotation(Kr, kperp, d);
+        }
       }
     }
     // cell (i-1, j-1): diagonal 0 from (i-1, j-1) to n
@@ -411,7 +455,10 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs A) {
   // Hessian (assemble.wgsl:183-190)
   if (A.ext) {
     for (int q = 0; q < 3; q++) g[q] += A.ext[q * Np + n];
-    for (int q = 0; q < 6; q++) C.a[q] += A.ext[(3 + q) * Np + n];
+    for (int q = 0; q < 6; q++) {
+      C.a[q] += A.ext[(3 + q) * Np + n];
+      Kx.a[q] += A.ext[(3 + q) * Np + n];
+    }
   }
   // floor contact: cubic penalty along +z and friction on the step (assemble.wgsl:191-209)
   bool onFloorCore = false;
@@ -430,6 +477,9 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs A) {
       C.a[0] += kf;
       C.a[3] += kf;
       C.a[5] += 2.f * kFloorK * s / delta;
+      Kx.a[0] += kf;
+      Kx.a[3] += kf;
+      Kx.a[5] += 2.f * kFloorK * s / delta;
       onFloorCore = zc <= A.floor_z + half * (1.f + 1e-3f);
     }
   }
@@ -445,10 +495,20 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs A) {
         if (approach >= m.w - 1e-3f * fabsf(m.w) && g[0] * m.x + g[1] * m.y + g[2] * m.z > 0.f) {
           float e[3] = {m.x, m.y, m.z};
           sym_add_axial(C, 1.f, kAct, 0.f, e);
+          sym_add_axial(Kx, 1.f, kAct, 0.f, e);
         }
       }
     }
-    if (onFloorCore && g[2] > 0.f) C.a[5] += kAct;
+    if (onFloorCore && g[2] > 0.f) {
+      C.a[5] += kAct;
+      Kx.a[5] += kAct;
+    }
+  }
+  if (detached) {
+    for (int q = 0; q < 6; q++) {
+      A.rigidk[q * Np + n] = Kx.a[q];
+      A.rigidk[(6 + q) * Np + n] = Kr.a[q];
+    }
   }
 
   for (int q = 0; q < 6; q++) {
@@ -537,15 +597,57 @@ __global__ __launch_bounds__(256) void k_newton_apply(ApplyArgs A) {
   atomicMax(A.max_update, __float_as_uint(lu));
 }
 
-// commit (solver.ts:622-628)
-__global__ __launch_bounds__(256) void k_commit(int N, int Np, float inv_h, float* x, float* v, float* dx) {
+// commit (solver.ts:622-628; step.wgsl:128-147). A detached node that left the scene (below
+// kRetireZ or beyond kRetireR from the axis) is retired (step.wgsl:113-126): frozen where it is, its
+// velocity zeroed and its spans removed (no longer existing, not broken), so its operator row becomes
+// the identity with nothing coupled to it. Two nodes retiring together clear the same flag bit.
+constexpr float kRetireZ = -0.25f;
+constexpr float kRetireR = 0.6f;
+
+struct CommitArgs {
+  int N, Np;
+  float inv_h;
+  float *x, *v, *dx;
+  uint8_t* state;  // or null (no detached node)
+  uint8_t* pinned;
+  uint16_t* flags;
+  Geo geo;
+};
+
+__global__ __launch_bounds__(256) void k_commit(CommitArgs A) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+  if (n >= A.N) return;
+  const int Np = A.Np;
+  float d[3];
   for (int q = 0; q < 3; q++) {
-    float d = dx[q * Np + n];
-    v[q * Np + n] = d * inv_h;
-    x[q * Np + n] += d;
-    dx[q * Np + n] = 0.f;
+    d[q] = A.dx[q * Np + n];
+    A.dx[q * Np + n] = 0.f;
+  }
+  if (A.state) {
+    const uint8_t st = A.state[n];
+    if (st == kStateRetired) return;
+    if (st == kStateDetached) {
+      float p[3] = {A.x[n] + d[0], A.x[Np + n] + d[1], A.x[2 * Np + n] + d[2]};
+      if (p[2] < kRetireZ || sqrtf(p[0] * p[0] + p[1] * p[1]) > kRetireR) {
+        for (int q = 0; q < 3; q++) {
+          A.x[q * Np + n] = p[q];
+          A.v[q * Np + n] = 0.f;
+        }
+        A.state[n] = kStateRetired;
+        A.pinned[n] |= kPinRetired;
+        LayerPos P = locate(A.geo, n);
+        for (int f = 0; f < 2; f++) {
+          A.flags[2 * n + f] &= (uint16_t)~SEG_EXISTS;
+          int prev = nbr(P, n, f, -1);
+          if (prev >= 0) A.flags[2 * prev + f] &= (uint16_t)~SEG_EXISTS;
+        }
+        return;
+      }
+    }
+  }
+  for (int q = 0; q < 3; q++) {
+    A.v[q * Np + n] = d[q] * A.inv_h;
+    A.x[q * Np + n] += d[q];
   }
 }
 
@@ -576,6 +678,8 @@ void launch_assemble(Solver& s) {
   A.cmeta = s.contact_mode != kContactNone ? s.cmeta : nullptr;
   A.floor_on = s.floor_on ? 1 : 0;
   A.floor_z = s.floor_z;
+  A.state = s.npieces > 0 ? s.nstate : nullptr;
+  A.rigidk = s.rigidk;
   {
     ProfScope ps(s, PC_ASSEMBLE, s.N * 237.0);
     k_assemble<<<grid_for(s.N, 256), 256, 0, s.stream>>>(A);
@@ -608,7 +712,8 @@ void launch_newton_apply(Solver& s, int it) {
 
 void launch_commit(Solver& s) {
   ProfScope ps(s, PC_PREDICT_COMMIT, s.N * 60.0);
-  k_commit<<<grid_for(s.N, 256), 256, 0, s.stream>>>(s.N, s.Np, (float)(1.0 / s.P.h), s.x, s.v, s.dx);
+  CommitArgs A{s.N, s.Np, (float)(1.0 / s.P.h), s.x, s.v, s.dx, s.any_detached ? s.nstate : nullptr, s.pinned, s.flags, s.geo};
+  k_commit<<<grid_for(s.N, 256), 256, 0, s.stream>>>(A);
 }
 
 }  // namespace bd

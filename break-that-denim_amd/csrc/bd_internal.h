@@ -94,13 +94,13 @@ struct Scal {
 enum ProfClass {
   PC_PREDICT_COMMIT, PC_ASSEMBLE, PC_PROLONG_WEIGHTS, PC_RAP, PC_SMOOTH_FINE, PC_SMOOTH_COARSE, PC_RESIDUAL_FINE,
   PC_RESIDUAL_COARSE, PC_RESTRICT, PC_PROLONG, PC_MATVEC, PC_PCG_VECTOR, PC_TRUE_RESIDUAL, PC_ENERGY, PC_DAMAGE, PC_SLIP,
-  PC_CONTACT_HASH, PC_CONTACT_DETECT, PC_CONTACT_TERMS, PC_TRANSPOSE, PC_COARSE_FACTOR, PC_COARSE_INVERSE, PC_COARSE_APPLY, PC_COUNT
+  PC_CONTACT_HASH, PC_CONTACT_DETECT, PC_CONTACT_TERMS, PC_TRANSPOSE, PC_COARSE_FACTOR, PC_COARSE_INVERSE, PC_COARSE_APPLY, PC_PIECES, PC_COUNT
 };
 inline const char* prof_name(int c) {
   static const char* n[PC_COUNT] = {"predict_commit", "assemble", "prolong_weights", "rap", "smooth_fine", "smooth_coarse",
                                     "residual_fine", "residual_coarse", "restrict", "prolong", "matvec_dot", "pcg_vector",
                                     "true_residual", "energy", "damage", "slip", "contact_hash", "contact_detect",
-                                    "contact_terms", "transpose", "coarse_factor", "coarse_inverse", "coarse_apply"};
+                                    "contact_terms", "transpose", "coarse_factor", "coarse_inverse", "coarse_apply", "pieces"};
   return n[c];
 }
 
@@ -132,6 +132,21 @@ struct Solver {
   float* mass = nullptr;
   double* cn0 = nullptr;
   uint8_t* pinned = nullptr;
+  // node state (state.wgsl): 0 cloth, 1 detached (a piece no mount or grip reaches), 2 retired (left
+  // the scene: frozen, spans removed). `pinned` holds bit kPinMount for mounted nodes and bit
+  // kPinRetired for retired ones, so every kernel that skips pinned rows skips retired ones too.
+  std::vector<uint8_t> mount_h;  // host copy of the mounted flags (topology anchors)
+  uint8_t* nstate = nullptr;
+  bool any_detached = false;  // some node has been detached (commit checks for retirement)
+  // detached pieces and their rigid modes (rigid_frame/apply/spin.wgsl)
+  int npieces = 0;
+  uint32_t* piece_start = nullptr;  // kMaxPieces + 1: first entry of piece p in piece_nodes
+  uint32_t* piece_nodes = nullptr;  // N
+  uint32_t* piece_of = nullptr;     // Np: piece of each node, or kNoPiece
+  double* piece_frame = nullptr;    // kPieceFrame per piece: centroid xyz, valid, E^-1 (6 x 6)
+  double* piece_step = nullptr;     // 8 per piece: rigid step t (or spin centroid), rotation w
+  float* rigidk = nullptr;          // 12 planes: external block, internal rotation stiffness
+  float* curve = nullptr;           // 7 planes: rotation w xyz, flag, arm r xyz
   uint8_t* layer_of = nullptr;
   // spans
   double *rest = nullptr, *intact = nullptr, *fail_site = nullptr, *tails = nullptr, *softening = nullptr;
@@ -197,6 +212,11 @@ struct Solver {
 };
 
 constexpr int kContactNone = 0, kContactSelf = 1, kContactPlies = 2;
+constexpr uint8_t kStateCloth = 0, kStateDetached = 1, kStateRetired = 2;
+constexpr uint8_t kPinMount = 1, kPinRetired = 2;
+constexpr int kMaxPieces = 4096;
+constexpr uint32_t kNoPiece = 0xFFFFFFFFu;
+constexpr int kPieceFrame = 40;
 constexpr int kContactK = 4;
 constexpr int kPlyTableBits = 20;
 constexpr int kPlyTable = 1 << kPlyTableBits;
@@ -237,6 +257,14 @@ void launch_project_start(Solver& s);
 void ply_contact_alloc(Solver& s);
 void launch_ply_detect(Solver& s);
 void launch_ply_terms(Solver& s);
+// pieces.hip
+void pieces_alloc(Solver& s);
+void launch_apply_state(Solver& s);
+void launch_rigid_frame(Solver& s);
+void launch_rigid_start(Solver& s);
+void launch_spin(Solver& s);
+void launch_curve_set(Solver& s);
+void launch_curve_apply(Solver& s);
 // coarse.hip
 void coarse_exact_alloc(Solver& s);
 void launch_coarse_factor(Solver& s);

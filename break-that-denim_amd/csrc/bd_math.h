@@ -218,6 +218,22 @@ __device__ __forceinline__ int geo_layer(const Geo& g, int n) {
   return l;
 }
 
+// rotationCurve (common.wgsl:277-283): a rotation by th (angle |th| about th / |th|) moves the arm
+// r by (R - I) r; its tangent is th x r, and this is what the rotation adds to it:
+// (sin phi - phi) k x r + (1 - cos phi) k x (k x r).
+__device__ __forceinline__ void rotation_curve(const double th[3], const double r[3], double o[3]) {
+  const double phi = sqrt(th[0] * th[0] + th[1] * th[1] + th[2] * th[2]);
+  if (phi < 1e-9) {
+    o[0] = o[1] = o[2] = 0;
+    return;
+  }
+  const double k[3] = {th[0] / phi, th[1] / phi, th[2] / phi};
+  const double kr[3] = {k[1] * r[2] - k[2] * r[1], k[2] * r[0] - k[0] * r[2], k[0] * r[1] - k[1] * r[0]};
+  const double kkr[3] = {k[1] * kr[2] - k[2] * kr[1], k[2] * kr[0] - k[0] * kr[2], k[0] * kr[1] - k[1] * kr[0]};
+  const double a = sin(phi) - phi, c = 1.0 - cos(phi);
+  for (int q = 0; q < 3; q++) o[q] = a * kr[q] + c * kkr[q];
+}
+
 // XCD-aware block index: consecutive logical tiles land on the same XCD (8 XCDs, blocks are
 // dispatched round-robin across them), so neighbouring rows share that XCD's L2.
 __device__ __forceinline__ int xcd_block(int b, int nblocks) {

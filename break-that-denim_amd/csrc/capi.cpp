@@ -153,7 +153,8 @@ void destroy_all(Solver& s) {
                   s.layer_of, s.rest, s.intact, s.fail_site, s.tails, s.softening, s.cut_mask, s.flags, s.strain,
                   s.damage, s.area, s.sep, s.pullmark, s.strength, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k,
                   s.partials, s.scal, s.counters, s.ijl, s.cells, s.scan_sums, s.items, s.pairs, s.keys,
-                  s.marks, s.cmeta, s.acc, s.ext, s.cband, s.cinv, s.crd, s.pcells, s.psums, s.pitems};
+                  s.marks, s.cmeta, s.acc, s.ext, s.cband, s.cinv, s.crd, s.pcells, s.psums, s.pitems,
+                  s.nstate, s.piece_start, s.piece_nodes, s.piece_of, s.piece_frame, s.piece_step, s.rigidk, s.curve};
   for (void* p : ptrs) dfree(p);
   for (int k = 0; k < s.nlev; k++) {
     Level& L = s.lv[k];
@@ -210,15 +211,40 @@ void contact_detect(Solver& s) {
   else if (s.contact_mode == kContactPlies) launch_ply_detect(s);
 }
 
-void newton_iteration(Solver& s, int it, bool coarse) {
-  contact_terms(s);
+// encodeAssemble (gpu/solver.ts:613-622): the Newton system, the detached pieces' rigid modes at
+// this iterate, and the Galerkin hierarchy when asked for
+void assemble_system(Solver& s, bool coarse) {
   launch_assemble(s);
+  launch_rigid_frame(s);
   s.assembled = true;
   if (coarse || !s.hierarchy) build_hierarchy(s);
-  pcg_solve(s);
+}
+
+// encodePredict (gpu/solver.ts:697-705): the inertial prediction, and detached pieces start from
+// their rotation over the substep
+void predict(Solver& s) {
+  launch_predict(s);
+  launch_spin(s);
+}
+
+// encodeLineSearch (gpu/solver.ts:678-684)
+void line_search(Solver& s) {
+  launch_curve_set(s);
   launch_energy(s);
+}
+
+void newton_iteration(Solver& s, int it, bool coarse) {
+  contact_terms(s);
+  assemble_system(s, coarse);
+  pcg_solve(s);
+  line_search(s);
+  launch_curve_apply(s);
   launch_newton_apply(s, it);
   launch_record(s, it);
+}
+
+void ensure_pieces(Solver& s) {
+  if (!s.piece_start) pieces_alloc(s);
 }
 
 }  // namespace
@@ -342,7 +368,14 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
     for (int n = 0; n < N; n++) f[n] = (float)sc->mass[n];
     upload(s, s.mass, f.data(), Np);
     upload(s, s.cn0, sc->crossing_normal0, N);
-    upload(s, s.pinned, sc->pinned, N);
+    {
+      // bit kPinMount: mounted by the scene; bit kPinRetired is set when a node retires
+      s.mount_h.assign(N, 0);
+      for (int n = 0; n < N; n++) s.mount_h[n] = sc->pinned[n] ? kPinMount : 0;
+      upload(s, s.pinned, s.mount_h.data(), N);
+      s.nstate = dalloc<uint8_t>(Np);
+      BD_HIP(hipMemsetAsync(s.nstate, 0, Np, s.stream));
+    }
     std::vector<int> gof(Np, -1);
     upload(s, s.grip_of, gof.data(), Np);
     upload(s, s.rest, sc->rest_length, 2 * (size_t)N);
@@ -554,16 +587,14 @@ int bd_set_pcg(bd_solver* h, int iterations, double rel_tol, int max_iterations)
 
 int bd_predict(bd_solver* h) {
   return guarded(h, [&](Solver& s) {
-    launch_predict(s);
+    predict(s);
     return BD_OK;
   });
 }
 
 int bd_assemble(bd_solver* h, int coarse) {
   return guarded(h, [&](Solver& s) {
-    launch_assemble(s);
-    s.assembled = true;
-    if (coarse || !s.hierarchy) build_hierarchy(s);
+    assemble_system(s, coarse != 0);
     return BD_OK;
   });
 }
@@ -612,7 +643,7 @@ int bd_pcg(bd_solver* h) {
 
 int bd_line_search(bd_solver* h) {
   return guarded(h, [&](Solver& s) {
-    launch_energy(s);
+    line_search(s);
     return BD_OK;
   });
 }
@@ -620,6 +651,7 @@ int bd_line_search(bd_solver* h) {
 int bd_newton_apply(bd_solver* h, int it) {
   return guarded(h, [&](Solver& s) {
     if (it < 0 || it >= BD_MAX_NEWTON) return fail(h, BD_ERR_INVALID, "bd_newton_apply: iteration out of range");
+    launch_curve_apply(s);
     launch_newton_apply(s, it);
     return BD_OK;
   });
@@ -684,12 +716,81 @@ int bd_set_floor(bd_solver* h, int enabled, double z) {
   });
 }
 
+int bd_set_pieces(bd_solver* h, const uint32_t* piece_of, const uint32_t* start, int count, const uint32_t* nodes) {
+  return guarded(h, [&](Solver& s) {
+    if (count < 0 || (count > 0 && (!piece_of || !start || !nodes)))
+      return fail(h, BD_ERR_INVALID, "bd_set_pieces: bad arguments");
+    // pieces beyond BD_MAX_PIECES go without the rigid-mode correction (gpu/solver.ts:664-676)
+    const int c = count < kMaxPieces ? count : kMaxPieces;
+    if (c > 0) {
+      if (start[0] != 0) return fail(h, BD_ERR_INVALID, "bd_set_pieces: start[0] must be 0");
+      for (int p = 0; p < c; p++)
+        if (start[p + 1] < start[p]) return fail(h, BD_ERR_INVALID, "bd_set_pieces: start must not decrease");
+      if (start[c] > (uint32_t)s.N) return fail(h, BD_ERR_INVALID, "bd_set_pieces: more piece entries than nodes");
+      for (uint32_t k = 0; k < start[c]; k++)
+        if (nodes[k] >= (uint32_t)s.N) return fail(h, BD_ERR_INVALID, "bd_set_pieces: node index out of range");
+    }
+    ensure_pieces(s);
+    BD_HIP(hipStreamSynchronize(s.stream));
+    std::vector<uint32_t> of(s.Np, kNoPiece);
+    if (c > 0)
+      for (int n = 0; n < s.N; n++) of[n] = piece_of[n] < (uint32_t)c ? piece_of[n] : kNoPiece;
+    upload(s, s.piece_of, of.data(), (size_t)s.Np);
+    if (c > 0) {
+      upload(s, s.piece_start, start, (size_t)c + 1);
+      if (start[c] > 0) upload(s, s.piece_nodes, nodes, (size_t)start[c]);
+    }
+    // no piece left: no node steps along a rotation
+    if (c == 0 && s.npieces > 0) BD_HIP(hipMemsetAsync(s.curve, 0, 7 * (size_t)s.Np * sizeof(float), s.stream));
+    s.npieces = c;
+    BD_HIP(hipStreamSynchronize(s.stream));
+    return BD_OK;
+  });
+}
+
+int bd_update_topology(bd_solver* h, const int32_t* anchors, int anchor_count, int32_t* freed_out) {
+  return guarded(h, [&](Solver& s) {
+    if (anchor_count < 0 || (anchor_count > 0 && !anchors)) return fail(h, BD_ERR_INVALID, "bd_update_topology: bad anchors");
+    const int N = s.N;
+    std::vector<uint16_t> flags(2 * (size_t)N);
+    std::vector<uint8_t> st(N);
+    download(s, flags.data(), s.flags, 2 * (size_t)N);
+    download(s, st.data(), s.nstate, (size_t)N);
+    // pack (state.wgsl:16-27)
+    std::vector<uint8_t> bits(N), freed(N);
+    auto act = [&](size_t q) { return (flags[q] & (SEG_EXISTS | SEG_BROKEN)) == SEG_EXISTS; };
+    for (int n = 0; n < N; n++)
+      bits[n] = (uint8_t)((act(2 * (size_t)n) ? 1 : 0) | (act(2 * (size_t)n + 1) ? 2 : 0) | (st[n] != kStateCloth ? 4 : 0) |
+                          (st[n] == kStateRetired ? 8 : 0));
+    std::vector<int32_t> nx(s.nl), ny(s.nl), off(s.nl);
+    for (int l = 0; l < s.nl; l++) {
+      nx[l] = s.geo.nx[l];
+      ny[l] = s.geo.ny[l];
+      off[l] = s.geo.off[l];
+    }
+    std::vector<uint32_t> of(N), start((size_t)N + 1), nodes((size_t)N + 1);
+    int32_t count = 0;
+    const int nfree = bd_topology_search(N, s.nl, nx.data(), ny.data(), off.data(), bits.data(), s.mount_h.data(), anchors,
+                                         anchor_count, freed.data(), of.data(), start.data(), nodes.data(), &count);
+    if (nfree < 0) return fail(h, BD_ERR_INVALID, "bd_update_topology: anchors out of range");
+    if (nfree > 0) {
+      // applyFreed (state.wgsl:29-35): freed nodes become detached, irreversibly
+      for (int n = 0; n < N; n++)
+        if (freed[n] && st[n] == kStateCloth) st[n] = kStateDetached;
+      upload(s, s.nstate, st.data(), (size_t)N);
+      s.any_detached = true;
+    }
+    if (freed_out) *freed_out = nfree;
+    return bd_set_pieces(h, of.data(), start.data(), count, nodes.data());
+  });
+}
+
 int bd_substep(bd_solver* h, int min_it, int max_it) {
   return guarded(h, [&](Solver& s) {
     const int n_it = std::min(BD_MAX_NEWTON, std::max(s.P.newton_iterations, min_it));
     const int n_max = std::min(BD_MAX_NEWTON, std::max(std::max(s.P.newton_max, n_it), max_it));
     launch_reset_counters(s);
-    launch_predict(s);
+    predict(s);
     contact_detect(s);
     if (s.contact_mode != kContactNone || s.floor_on) launch_project_start(s);
     int done = 0;
@@ -774,8 +875,12 @@ size_t bd_buffer_bytes(const bd_solver* h, bd_buffer buf, int level) {
     case BD_BUF_SEGFLAGS: return 2 * (size_t)s.N * sizeof(uint16_t);
     case BD_BUF_TAILS: return 4 * (size_t)s.N * sizeof(double);
     case BD_BUF_FIELD: return 2 * (size_t)s.C;
+    case BD_BUF_STATE: return (size_t)s.N;
+    case BD_BUF_RIGIDK: return s.rigidk ? 12 * (size_t)s.N * sizeof(float) : 0;
+    case BD_BUF_PIECES: return s.piece_frame && s.npieces > 0 ? 48 * (size_t)s.npieces * sizeof(double) : 0;
+    case BD_BUF_CURVE: return s.curve ? 7 * (size_t)s.N * sizeof(float) : 0;
     case BD_BUF_LAST_UPDATE: return (size_t)s.N * sizeof(double);
-    case BD_BUF_ENERGY: return 8 * sizeof(double);
+    case BD_BUF_ENERGY: return 9 * sizeof(double);
     case BD_BUF_EXT: return 9 * (size_t)s.N * sizeof(float);
     case BD_BUF_CMETA: return 8 * (size_t)s.N * sizeof(float);
     case BD_BUF_PAIRS:
@@ -898,6 +1003,28 @@ int bd_read(bd_solver* h, bd_buffer buf, int level, void* dst, size_t bytes) {
       case BD_BUF_CUTMASK: download(s, static_cast<uint32_t*>(dst), s.cut_mask, 2 * (size_t)s.N); break;
       case BD_BUF_SEGFLAGS: download(s, static_cast<uint16_t*>(dst), s.flags, 2 * (size_t)s.N); break;
       case BD_BUF_FIELD: download(s, static_cast<uint8_t*>(dst), s.damage, 2 * (size_t)s.C); break;
+      case BD_BUF_STATE: download(s, static_cast<uint8_t*>(dst), s.nstate, (size_t)s.N); break;
+      case BD_BUF_RIGIDK:
+      case BD_BUF_CURVE: {
+        const int planes = buf == BD_BUF_RIGIDK ? 12 : 7;
+        std::vector<float> t((size_t)planes * s.Np);
+        download(s, t.data(), buf == BD_BUF_RIGIDK ? s.rigidk : s.curve, t.size());
+        float* o = static_cast<float*>(dst);
+        for (int n = 0; n < s.N; n++)
+          for (int q = 0; q < planes; q++) o[(size_t)planes * n + q] = t[(size_t)q * s.Np + n];
+        break;
+      }
+      case BD_BUF_PIECES: {
+        std::vector<double> fr((size_t)s.npieces * kPieceFrame), st((size_t)s.npieces * 8);
+        download(s, fr.data(), s.piece_frame, fr.size());
+        download(s, st.data(), s.piece_step, st.size());
+        double* o = static_cast<double*>(dst);
+        for (int p = 0; p < s.npieces; p++) {
+          for (int q = 0; q < kPieceFrame; q++) o[48 * (size_t)p + q] = fr[(size_t)p * kPieceFrame + q];
+          for (int q = 0; q < 8; q++) o[48 * (size_t)p + kPieceFrame + q] = st[(size_t)p * 8 + q];
+        }
+        break;
+      }
       case BD_BUF_LAST_UPDATE: {
         std::vector<float> t(s.N);
         download(s, t.data(), s.last_update, (size_t)s.N);
@@ -910,6 +1037,7 @@ int bd_read(bd_solver* h, bd_buffer buf, int level, void* dst, size_t bytes) {
           d[k] = s.scal_h->e[k];
           d[4 + k] = s.scal_h->m[k];
         }
+        d[8] = s.scal_h->line_alpha;
         break;
       }
       case BD_BUF_EXT: {
@@ -987,6 +1115,15 @@ int bd_write(bd_solver* h, bd_buffer buf, int level, const void* src, size_t byt
       case BD_BUF_CUTMASK: upload(s, s.cut_mask, static_cast<const uint32_t*>(src), 2 * (size_t)s.N); break;
       case BD_BUF_SEGFLAGS: upload(s, s.flags, static_cast<const uint16_t*>(src), 2 * (size_t)s.N); break;
       case BD_BUF_FIELD: upload(s, s.damage, static_cast<const uint8_t*>(src), 2 * (size_t)s.C); break;
+      case BD_BUF_STATE: {
+        const uint8_t* in = static_cast<const uint8_t*>(src);
+        for (int n = 0; n < s.N; n++)
+          if (in[n] > kStateRetired) return fail(h, BD_ERR_INVALID, "bd_write: node state must be 0, 1 or 2");
+        for (int n = 0; n < s.N; n++) s.any_detached = s.any_detached || in[n] != kStateCloth;
+        upload(s, s.nstate, in, (size_t)s.N);
+        launch_apply_state(s);
+        break;
+      }
       case BD_BUF_EXT: {
         const float* in = static_cast<const float*>(src);
         std::vector<float> t(9 * (size_t)s.Np, 0.f);

@@ -75,7 +75,9 @@ __global__ __launch_bounds__(256) void k_strain(HistArgs A) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= 2 * A.N) return;
   int b = seg_other(A, s);
-  A.strain[s] = (b >= 0 && h_active(A, s)) ? seg_length(A, s, b) / A.rest[s] - 1 : __builtin_nan("");
+  // retired nodes are out of the scene (yarnfield.wgsl:50, 60)
+  bool cloth = b >= 0 && !((A.pinned[s >> 1] | A.pinned[b]) & kPinRetired);
+  A.strain[s] = (cloth && h_active(A, s)) ? seg_length(A, s, b) / A.rest[s] - 1 : __builtin_nan("");
 }
 
 __global__ __launch_bounds__(256) void k_sep(HistArgs A) {
@@ -163,6 +165,7 @@ __global__ __launch_bounds__(256) void k_update_field(HistArgs A, int f) {
   if ((f == 0 && i >= L.nx - 1) || (f == 1 && j >= L.ny - 1)) return;
   const int s = 2 * n + f;
   if (!h_active(A, s)) return;
+  if ((A.pinned[n] | A.pinned[f == 0 ? n + 1 : n + L.nx]) & kPinRetired) return;  // yarnfield.wgsl:173
   const DevYarn& y = L.y[f];
   const int B = bundle_of(L, f), P = span_len(L, f), K = L.fk;
   if (q >= B) return;
@@ -305,6 +308,7 @@ __global__ __launch_bounds__(256) void k_couple(HistArgs A) {
     return;
   }
   if (!h_active(A, s)) return;
+  if ((A.pinned[n] | A.pinned[f == 0 ? n + 1 : n + L.nx]) & kPinRetired) return;  // yarnfield.wgsl:397
   int B = bundle_of(L, f), P = span_len(L, f);
   double sum = 0, site = 0;
   uint32_t mask = 0;

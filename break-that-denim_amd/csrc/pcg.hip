@@ -329,6 +329,8 @@ struct EnergyArgs {
   const float* ext;  // external terms (self-contact) or null
   int floor_on;
   double floor_z;
+  // detached pieces' curved steps (7 planes: rotation w xyz, flag, arm r xyz), or null
+  const float* curve;
 };
 
 __device__ __forceinline__ bool act(const EnergyArgs& E, int s) { return (E.flags[s] & (SEG_EXISTS | SEG_BROKEN)) == SEG_EXISTS; }
@@ -352,7 +354,21 @@ __device__ __forceinline__ void len_change(const double* e0, const double* de, d
   la = sqrt(dot3(ea, ea));
   dl = (2 * a * dot3(e0, de) + a * a * dot3(de, de)) / fmax(la + l0, 1e-12);
 }
+// the same for a full displacement D of the term's vector (energy.wgsl:90-94)
+__device__ __forceinline__ void len_change_d(const double* e0, const double* D, double l0, double& dl, double& la) {
+  double ea[3] = {e0[0] + D[0], e0[1] + D[1], e0[2] + D[2]};
+  la = sqrt(dot3(ea, ea));
+  dl = (2 * dot3(e0, D) + dot3(D, D)) / fmax(la + l0, 1e-12);
+}
+__device__ __forceinline__ void symv6(const double* H, const double* v, double* o) {
+  o[0] = H[0] * v[0] + H[1] * v[1] + H[2] * v[2];
+  o[1] = H[1] * v[0] + H[3] * v[1] + H[4] * v[2];
+  o[2] = H[2] * v[0] + H[4] * v[1] + H[5] * v[2];
+}
 
+// CURVE: some nodes belong to detached pieces and step along their piece's rotation
+// (energy.wgsl:65-73): at step length a their step is a d + rotationCurve(a w, r).
+template <bool CURVE>
 __global__ __launch_bounds__(256) void k_energy(EnergyArgs E) {
   const double al[4] = {1.0, 0.5, 0.25, 0.125};
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // e[4], m[4]
@@ -380,22 +396,59 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs E) {
         ld3d(E.d, Np, m, o);
       }
     };
+    // curved steps of detached pieces' nodes
+    auto curved = [&](int m) { return CURVE && !E.pinned[m] && E.curve[3 * Np + m] >= 0.5f; };
+    auto stepk = [&](int m, const double* dm, int k, double* o) {
+      for (int q = 0; q < 3; q++) o[q] = al[k] * dm[q];
+      if (curved(m)) {
+        double th[3], r[3], c[3];
+        for (int q = 0; q < 3; q++) {
+          th[q] = al[k] * (double)E.curve[q * Np + m];
+          r[q] = E.curve[(4 + q) * Np + m];
+        }
+        rotation_curve(th, r, c);
+        for (int q = 0; q < 3; q++) o[q] += c[q];
+      }
+    };
+    // change of a node-local quadratic term g . s + s^T H s / 2 over the node's curved steps
+    auto quad_curved = [&](const double* g, const double* H, const double (*sk)[3]) {
+      for (int k = 0; k < 4; k++) {
+        double Hs[3];
+        symv6(H, sk[k], Hs);
+        double a = dot3(g, sk[k]), b = 0.5 * dot3(sk[k], Hs);
+        acc[k] += a + b;
+        acc[4 + k] += fabs(a) + fabs(b);
+      }
+    };
     double dn[3], d0[3], xn[3];
     dir(n, dn);
     ld3d(E.dx, Np, n, d0);
     ld3d(E.x, Np, n, xn);
+    const bool cn = curved(n);
+    double sn[4][3];
+    if (cn)
+      for (int k = 0; k < 4; k++) stepk(n, dn, k, sn[k]);
     if (!E.pinned[n]) {
       double w = (double)E.mass[n] / (h * h);
       double pr[3];
       ld3d(E.pred, Np, n, pr);
-      quad(w * (dn[0] * (d0[0] - pr[0]) + dn[1] * (d0[1] - pr[1]) + dn[2] * (d0[2] - pr[2])), w * dot3(dn, dn));
+      if (!cn) {
+        quad(w * (dn[0] * (d0[0] - pr[0]) + dn[1] * (d0[1] - pr[1]) + dn[2] * (d0[2] - pr[2])), w * dot3(dn, dn));
+      } else {
+        const double g[3] = {w * (d0[0] - pr[0]), w * (d0[1] - pr[1]), w * (d0[2] - pr[2])}, H[6] = {w, 0, 0, w, 0, w};
+        quad_curved(g, H, sn);
+      }
       int gi = E.grip_of[n];
       if (gi >= 0) {
         const double* G = E.grip_goal_k + 4 * gi;
         double kg = G[3] * (double)E.grip_w[n];
         double g[3];
         for (int q = 0; q < 3; q++) g[q] = kg * ((xn[q] - G[q]) + d0[q] - E.grip_off[q * Np + n]);
-        if (kg > 0) quad(dot3(g, dn), kg * dot3(dn, dn));
+        if (kg > 0 && !cn) quad(dot3(g, dn), kg * dot3(dn, dn));
+        if (kg > 0 && cn) {
+          const double H[6] = {kg, 0, 0, kg, 0, kg};
+          quad_curved(g, H, sn);
+        }
       }
       // external terms (contact): quadratic model from their gradient and Hessian at this iterate
       // (energy.wgsl:106-110)
@@ -406,7 +459,10 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs E) {
         double Hd0 = He[0] * dn[0] + He[1] * dn[1] + He[2] * dn[2];
         double Hd1 = He[1] * dn[0] + He[3] * dn[1] + He[4] * dn[2];
         double Hd2 = He[2] * dn[0] + He[4] * dn[1] + He[5] * dn[2];
-        quad(dot3(ge, dn), dn[0] * Hd0 + dn[1] * Hd1 + dn[2] * Hd2);
+        if (!cn)
+          quad(dot3(ge, dn), dn[0] * Hd0 + dn[1] * Hd1 + dn[2] * Hd2);
+        else
+          quad_curved(ge, He, sn);
       }
       // floor contact: its quadratic model at this iterate (energy.wgsl:112-126)
       if (E.floor_on) {
@@ -416,7 +472,12 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs E) {
           double kc = kFloorK;
           double lam = kc * s0 * s0 / reach;
           double kf = (double)kFloorMu * lam / fmax(sqrt(d0[0] * d0[0] + d0[1] * d0[1]), (double)kFloorSpeed * h);
-          quad(kf * (d0[0] * dn[0] + d0[1] * dn[1]) - lam * dn[2], kf * (dn[0] * dn[0] + dn[1] * dn[1]) + (2.0 * kc * s0 / reach) * dn[2] * dn[2]);
+          if (!cn) {
+            quad(kf * (d0[0] * dn[0] + d0[1] * dn[1]) - lam * dn[2], kf * (dn[0] * dn[0] + dn[1] * dn[1]) + (2.0 * kc * s0 / reach) * dn[2] * dn[2]);
+          } else {
+            const double g[3] = {kf * d0[0], kf * d0[1], -lam}, H[6] = {kf, 0, 0, kf, 0, 2.0 * kc * s0 / reach};
+            quad_curved(g, H, sn);
+          }
         }
       }
       if (L.air_drag > 0) {
@@ -467,7 +528,12 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs E) {
           t1[r] = D[3 * r] * d0[0] + D[3 * r + 1] * d0[1] + D[3 * r + 2] * d0[2];
           t2[r] = D[3 * r] * dn[0] + D[3 * r + 1] * dn[1] + D[3 * r + 2] * dn[2];
         }
-        quad(dot3(t1, dn), dot3(dn, t2));
+        if (!cn) {
+          quad(dot3(t1, dn), dot3(dn, t2));
+        } else {
+          const double H[6] = {D[0], D[1], D[2], D[4], D[5], D[8]};
+          quad_curved(t1, H, sn);
+        }
       }
     }
     for (int f = 0; f < 2; f++) {
@@ -489,9 +555,17 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs E) {
         double l0 = sqrt(dot3(ec, ec)), ln = sqrt(dot3(en, en));
         double T0 = it * axial_tension<double>(y, l0 / Lr - 1);
         double c = y.visc / (Lr * h);
+        const bool cv = cn || curved(next);
         for (int k = 0; k < 4; k++) {
           double dl, la;
-          len_change(ec, de, al[k], l0, dl, la);
+          if (!cv) {
+            len_change(ec, de, al[k], l0, dl, la);
+          } else {
+            double sb[3], D[3];
+            stepk(next, dd, k, sb);
+            for (int q = 0; q < 3; q++) D[q] = sb[q] - (cn ? sn[k][q] : al[k] * dn[q]);
+            len_change_d(ec, D, l0, dl, la);
+          }
           double Ta = it * axial_tension<double>(y, la / Lr - 1);
           add(k, 0.5 * (T0 + Ta) * dl);
           add(k, 0.5 * c * dl * (la + l0 - 2 * ln));
@@ -517,12 +591,23 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs E) {
           dr[q] = wa * dp[q] - 2 * dn[q] + wc * dq[q];
         }
         double q0 = dot3(r0, r0), l0 = sqrt(q0), lrn = sqrt(dot3(rn, rn));
+        const bool cv = cn || curved(prev) || curved(next);
         for (int k = 0; k < 4; k++) {
           double a = al[k];
-          double dqa = 2 * a * dot3(r0, dr) + a * a * dot3(dr, dr);
+          double dqa, dl, la;
+          if (!cv) {
+            dqa = 2 * a * dot3(r0, dr) + a * a * dot3(dr, dr);
+            len_change(r0, dr, a, l0, dl, la);
+          } else {
+            double sa[3], sc2[3], sm[3], D[3];
+            stepk(prev, dp, k, sa);
+            stepk(next, dq, k, sc2);
+            for (int q = 0; q < 3; q++) sm[q] = cn ? sn[k][q] : a * dn[q];
+            for (int q = 0; q < 3; q++) D[q] = wa * sa[q] - 2 * sm[q] + wc * sc2[q];
+            dqa = 2 * dot3(r0, D) + dot3(D, D);
+            len_change_d(r0, D, l0, dl, la);
+          }
           add(k, 0.5 * kb * bend_psi_delta(q0, q0 + dqa, dqa, fold));
-          double dl, la;
-          len_change(r0, dr, a, l0, dl, la);
           add(k, 0.5 * kd * dl * (la + l0 - 2 * lrn));
         }
       }
@@ -552,9 +637,18 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs E) {
         double l0 = sqrt(dot3(ec, ec));
         double f0, k0, e0;
         shear_law<double>(ks, Ld, l0, jam, f0, k0, e0);
+        const bool cv = curved(p) || curved(q2);
         for (int k = 0; k < 4; k++) {
           double dl, la, fa, ka, ea;
-          len_change(ec, de, al[k], l0, dl, la);
+          if (!cv) {
+            len_change(ec, de, al[k], l0, dl, la);
+          } else {
+            double sp[3], sq[3], D[3];
+            stepk(p, vp, k, sp);
+            stepk(q2, vq, k, sq);
+            for (int z = 0; z < 3; z++) D[z] = sq[z] - sp[z];
+            len_change_d(ec, D, l0, dl, la);
+          }
           shear_law<double>(ks, Ld, la, jam, fa, ka, ea);
           add(k, 0.5 * (f0 + fa) * dl);
         }
@@ -588,6 +682,11 @@ __global__ __launch_bounds__(256) void k_energy(EnergyArgs E) {
 }
 
 __global__ void k_set_alpha(Scal* sc, double a) { sc->line_alpha = a; }
+
+// the next k_true_residual runs: r = b - A delta from the rigid-mode start
+__global__ void k_need_true(Scal* sc) {
+  if (!sc->done) sc->need_true = 1;
+}
 
 __global__ void k_record(Scal* sc, int it) {
   sc->rec_iters[it] = sc->iters;
@@ -635,6 +734,14 @@ void pcg_solve(Solver& s) {
   {
     ProfScope ps(s, PC_PCG_VECTOR, N * 85.0);
     k_pcg_start<<<G, kDotThreads, 0, s.stream>>>(A);
+  }
+  if (s.npieces > 0) {
+    // detached pieces (rigid_apply.wgsl): start from each piece's exact rigid response x0 and the
+    // residual b - A x0, so the direction is never worse than that start
+    launch_rigid_start(s);
+    k_need_true<<<1, 1, 0, s.stream>>>(s.scal);
+    ProfScope ps(s, PC_TRUE_RESIDUAL, 0.0);
+    k_true_residual<<<G, kDotThreads, 0, s.stream>>>(A, s.lv[0].op, s.lv[0].geo);
   }
   if (K == 0) return;
   vcycle(s, 0);
@@ -690,10 +797,14 @@ void launch_energy(Solver& s) {
   }
   EnergyArgs E{s.N, s.Np, s.geo, s.layers, s.P.h, s.x, s.dx, s.pred, s.mass, s.delta, s.cn0, s.pinned, s.rest, s.intact,
                s.cut_mask, s.flags, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k, s.scal, s.partials,
-               s.contact_mode != kContactNone ? s.ext : nullptr, s.floor_on ? 1 : 0, (double)s.floor_z};
+               s.contact_mode != kContactNone ? s.ext : nullptr, s.floor_on ? 1 : 0, (double)s.floor_z,
+               s.npieces > 0 ? s.curve : nullptr};
   ProfScope ps(s, PC_ENERGY, s.N * 161.0);
   // 256-thread blocks keep the FP64 energy terms in registers
-  k_energy<<<std::min(kDotBlocks, grid_for(s.N, 256)), 256, 0, s.stream>>>(E);
+  if (E.curve)
+    k_energy<true><<<std::min(kDotBlocks, grid_for(s.N, 256)), 256, 0, s.stream>>>(E);
+  else
+    k_energy<false><<<std::min(kDotBlocks, grid_for(s.N, 256)), 256, 0, s.stream>>>(E);
 }
 
 }  // namespace bd

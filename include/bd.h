@@ -193,12 +193,16 @@ typedef enum bd_buffer {
   BD_BUF_TAILS,        /* 4N doubles */
   BD_BUF_FIELD,        /* 2C uint8 damage codes */
   BD_BUF_LAST_UPDATE,  /* N doubles, norm of each node's last Newton update */
-  BD_BUF_ENERGY,       /* 8 doubles: energy changes [4] and magnitudes [4] of the last line search */
+  BD_BUF_ENERGY,       /* 9 doubles: energy changes [4], magnitudes [4] and the chosen step length of the last line search */
   BD_BUF_EXT,          /* external terms, 9N floats per node: gradient xyz, Hessian xx xy xz yy yz zz */
   BD_BUF_CMETA,        /* contact bounds, 8N floats per node: own (normal xyz, share), reciprocal (dir xyz, share) */
   BD_BUF_PAIRS,        /* (K + 2) N uint32 per node: K candidate faces (NONE = empty), bound key, row mark */
   BD_BUF_CELLS,        /* 2 x 2^17 uint32: cell start offsets, then fill counts */
   BD_BUF_ITEMS,        /* 2N records x 8 floats: centroid now, radius, centroid at start, face (uint32 bits) */
+  BD_BUF_STATE,        /* N uint8 node states: 0 cloth, 1 detached, 2 retired (state.wgsl); writable */
+  BD_BUF_RIGIDK,       /* 12N floats per node (detached nodes): external block, rotation stiffness (xx xy xz yy yz zz) */
+  BD_BUF_PIECES,       /* 48 doubles per piece: centroid xyz, valid, E^-1 (36, row-major), rigid step t xyz, 0, w xyz, 0 */
+  BD_BUF_CURVE,        /* 7N floats per node: its piece's rotation w xyz, flag, arm r xyz (line search) */
   BD_BUF_COUNT_
 } bd_buffer;
 
@@ -262,6 +266,28 @@ int bd_contact_terms(bd_solver* s);
 int bd_project_start(bd_solver* s);
 /* setFloor(z | null) (gpu/solver.ts:796-801): floor plane at height z when `enabled`, else none. */
 int bd_set_floor(bd_solver* s, int enabled, double z);
+
+/* Detached pieces (src/simulation/gpu/topology.ts, rigid_frame/apply/spin.wgsl). A piece is a set of
+ * detached nodes connected through active spans; its six rigid modes are carried by the Newton solve
+ * (PCG start from the piece's exact rigid response, spin start of the substep, line search along the
+ * piece's rotation). At most BD_MAX_PIECES pieces are carried; more go without the correction. */
+#define BD_MAX_PIECES 4096
+#define BD_NO_PIECE 0xFFFFFFFFu
+/* GpuSolver.setPieces (gpu/solver.ts:664-676): piece_of[n] = piece of node n or BD_NO_PIECE; piece p
+ * holds nodes[start[p] .. start[p + 1]); count = 0 removes every piece. */
+int bd_set_pieces(bd_solver* s, const uint32_t* piece_of, const uint32_t* start, int count, const uint32_t* nodes);
+/* Topology.request (topology.ts:98-160), synchronous: the active span graph is flooded from the
+ * mounted nodes and from each anchor (grip patch cores); cloth nodes it does not reach become
+ * detached, and the detached nodes are labelled into pieces and handed to bd_set_pieces. *freed
+ * receives the number of newly detached nodes (may be NULL). */
+int bd_update_topology(bd_solver* s, const int32_t* anchors, int anchor_count, int32_t* freed);
+/* The host search alone (no GPU): bits per node (bit 0 / 1 active span of family u / v owned, bit 2
+ * not cloth, bit 3 retired), mounted flags and anchors in; freed flags and the pieces out
+ * (piece_start holds count + 1 <= N + 1 entries). Returns the number of freed nodes, or < 0. */
+int bd_topology_search(int N, int layer_count, const int32_t* nx, const int32_t* ny, const int32_t* offset,
+                       const uint8_t* bits, const uint8_t* mounted, const int32_t* anchors, int anchor_count,
+                       uint8_t* freed, uint32_t* piece_of, uint32_t* piece_start, uint32_t* piece_nodes,
+                       int32_t* piece_count);
 
 /* Per-kernel timing (the reference's timestamp-query profiling, tools/gpu/scripts/profile.ts):
  * while enabled, every launch on the solver's stream is bracketed by HIP events. bd_profile_read

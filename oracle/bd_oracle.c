@@ -306,6 +306,9 @@ typedef struct {
   /* exact coarsest solve (BD_COARSE_EXACT): banded Cholesky factor of the coarsest level */
   int cn, cbw;
   double* cL;
+  /* detached pieces' curved line-search steps (energy.wgsl:65-73), 7 per node: rotation w xyz,
+     flag, arm r xyz; NULL when no node belongs to a piece */
+  double* curve;
 } or_solver;
 
 static inline const bd_material* mat_of(const or_solver* S, int n) { return &S->lay[S->layer_of[n]].material; }
@@ -402,6 +405,11 @@ static inline void posn(const or_solver* S, int n, double* o) {
 
 static inline double norm3(const double* a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
 static inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void cross3(const double* a, const double* b, double* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
 
 static void span_response(const or_solver* S, int s, double eps, double* T, double* K) {
   const bd_yarn* y = &mat_of(S, s >> 1)->yarns[s & 1];
@@ -641,8 +649,28 @@ static void len_change(const double* e0, const double* de, double a, double l0, 
   *dl = (2 * a * dot3(e0, de) + a * a * dot3(de, de)) / fmax(*la + l0, 1e-12);
 }
 
+/* rotationCurve (common.wgsl:277-283): what a rotation by th adds to its tangent th x r */
+static void rotation_curve(const double* th, const double* r, double* o) {
+  double phi = norm3(th);
+  if (phi < 1e-9) {
+    o[0] = o[1] = o[2] = 0;
+    return;
+  }
+  double k[3] = {th[0] / phi, th[1] / phi, th[2] / phi}, kr[3], kkr[3];
+  cross3(k, r, kr);
+  cross3(k, kr, kkr);
+  double a = sin(phi) - phi, c = 1 - cos(phi);
+  for (int q = 0; q < 3; q++) o[q] = a * kr[q] + c * kkr[q];
+}
+
+static void energy_change_curved(or_solver* S, const double* d, double* out, double* mag);
+
 /* energyChange (solver.ts:430-534) for the 4 line-search alphas */
 void or_energy_change(or_solver* S, const double* d, double* out, double* mag) {
+  if (S->curve) {
+    energy_change_curved(S, d, out, mag);
+    return;
+  }
   const int NA = 4;
   const double* al = LINE_ALPHAS;
   for (int k = 0; k < NA; k++) out[k] = mag[k] = 0;
@@ -755,6 +783,147 @@ void or_energy_change(or_solver* S, const double* d, double* out, double* mag) {
 #undef DIR
 #undef ADD
 #undef QUAD
+}
+
+/* The same with detached pieces' nodes stepping along their piece's rotation (energy.wgsl:65-73):
+   at step length a a node's step is a d + rotationCurve(a w, r) where its curve flag is set. Every
+   term is evaluated from the nodes' steps: node-local quadratic models as g . s + s^T H s / 2, span,
+   bending and shear lengths from the displacement of their vectors. */
+static void len_change_d(const double* e0, const double* D, double l0, double* dl, double* la) {
+  double ea[3] = {e0[0] + D[0], e0[1] + D[1], e0[2] + D[2]};
+  *la = norm3(ea);
+  *dl = (2 * dot3(e0, D) + dot3(D, D)) / fmax(*la + l0, 1e-12);
+}
+
+static void energy_change_curved(or_solver* S, const double* d, double* out, double* mag) {
+  const int NA = 4, N = S->N;
+  const double* al = LINE_ALPHAS;
+  for (int k = 0; k < NA; k++) out[k] = mag[k] = 0;
+  const double h = S->P.h;
+  const double* x = S->x;
+  const double* dx = S->dx;
+  double* st = calloc((size_t)N * NA * 3, sizeof(double));
+  for (int n = 0; n < N; n++) {
+    if (S->pinned[n]) continue;
+    const double* cv = &S->curve[7 * (size_t)n];
+    for (int k = 0; k < NA; k++) {
+      double* o = &st[((size_t)n * NA + k) * 3];
+      for (int q = 0; q < 3; q++) o[q] = al[k] * d[3 * n + q];
+      if (cv[3] >= 0.5) {
+        double th[3] = {al[k] * cv[0], al[k] * cv[1], al[k] * cv[2]}, c[3];
+        rotation_curve(th, &cv[4], c);
+        for (int q = 0; q < 3; q++) o[q] += c[q];
+      }
+    }
+  }
+#define S_(n, k) (&st[((size_t)(n) * NA + (k)) * 3])
+#define ADD(k, v) do { double v_ = (v); out[k] += v_; mag[k] += fabs(v_); } while (0)
+  /* g . s + s^T H s / 2 for a symmetric H given as 9 entries */
+#define QUADS(n, g, H9) do { for (int k = 0; k < NA; k++) { const double* s_ = S_(n, k); double Hs_[3]; \
+    drag_mul(H9, s_, Hs_); ADD(k, dot3(g, s_)); ADD(k, 0.5 * dot3(s_, Hs_)); } } while (0)
+  for (int n = 0; n < N; n++) {
+    const double* d0 = &dx[3 * n];
+    if (!S->pinned[n]) {
+      double w = S->mass[n] / (h * h);
+      double g[3] = {w * (d0[0] - S->pred[3 * n]), w * (d0[1] - S->pred[3 * n + 1]), w * (d0[2] - S->pred[3 * n + 2])};
+      double Hw[9] = {w, 0, 0, 0, w, 0, 0, 0, w};
+      QUADS(n, g, Hw);
+      double gg[3] = {0, 0, 0};
+      double kg = grip_term(S, n, gg);
+      if (kg > 0) {
+        double Hg[9] = {kg, 0, 0, 0, kg, 0, 0, 0, kg};
+        QUADS(n, gg, Hg);
+      }
+      double D[9], t1[3];
+      air_drag(S, n, D);
+      drag_mul(D, d0, t1);
+      QUADS(n, t1, D);
+      if (S->ext || S->floor_on) {
+        double ge[3] = {0, 0, 0}, h6[6] = {0, 0, 0, 0, 0, 0};
+        external_terms(S, n, ge, h6);
+        double He[9] = {h6[0], h6[1], h6[2], h6[1], h6[3], h6[4], h6[2], h6[4], h6[5]};
+        QUADS(n, ge, He);
+      }
+    }
+    const bd_material* mat = mat_of(S, n);
+    for (int f = 0; f < 2; f++) {
+      const bd_yarn* yarn = &mat->yarns[f];
+      int next = S->nbr[4 * n + 2 * f + 1];
+      int s = 2 * n + f;
+      if (next >= 0 && seg_active(S, s)) {
+        double L = S->rest[s];
+        double en[3], ec[3];
+        for (int q = 0; q < 3; q++) {
+          en[q] = x[3 * next + q] - x[3 * n + q];
+          ec[q] = en[q] + dx[3 * next + q] - d0[q];
+        }
+        double l0 = norm3(ec), ln = norm3(en);
+        double T0, K0;
+        span_response(S, s, l0 / L - 1, &T0, &K0);
+        double c = yarn->axial_viscosity / (L * h);
+        for (int k = 0; k < NA; k++) {
+          double Dk[3], dl, la, Ta, Ka;
+          for (int q = 0; q < 3; q++) Dk[q] = S_(next, k)[q] - S_(n, k)[q];
+          len_change_d(ec, Dk, l0, &dl, &la);
+          span_response(S, s, la / L - 1, &Ta, &Ka);
+          ADD(k, 0.5 * (T0 + Ta) * dl);
+          ADD(k, 0.5 * c * dl * (la + l0 - 2 * ln));
+        }
+      }
+      int prev = S->nbr[4 * n + 2 * f];
+      if (prev >= 0 && next >= 0 && seg_active(S, 2 * prev + f) && seg_active(S, s)) {
+        double kb, kd, ab, wa, wc;
+        bend_coeffs(S, n, f, 2 * prev + f, s, &kb, &kd, &ab, &wa, &wc);
+        double rn[3], r0[3];
+        for (int q = 0; q < 3; q++) {
+          rn[q] = wa * x[3 * prev + q] - 2 * x[3 * n + q] + wc * x[3 * next + q];
+          r0[q] = rn[q] + wa * dx[3 * prev + q] - 2 * d0[q] + wc * dx[3 * next + q];
+        }
+        double q0 = dot3(r0, r0), l0 = sqrt(q0), lrn = norm3(rn);
+        for (int k = 0; k < NA; k++) {
+          double DR[3], dl, la;
+          for (int q = 0; q < 3; q++) DR[q] = wa * S_(prev, k)[q] - 2 * S_(n, k)[q] + wc * S_(next, k)[q];
+          double dqa = 2 * dot3(r0, DR) + dot3(DR, DR);
+          ADD(k, 0.5 * kb * or_bend_psi_delta(q0, q0 + dqa, dqa, ab));
+          len_change_d(r0, DR, l0, &dl, &la);
+          ADD(k, 0.5 * kd * dl * (la + l0 - 2 * lrn));
+        }
+      }
+    }
+    if (S->nbr[4 * n + 1] >= 0 && S->nbr[4 * n + 3] >= 0) {
+      int r = S->nbr[4 * n + 1], t = S->nbr[4 * n + 3], rt = S->nbr[4 * r + 3];
+      double Ld = shear_rest(S, n), ks, jam;
+      cell_shear(S, n, &ks, &jam);
+      int P2[2][2] = {{n, rt}, {r, t}};
+      for (int dgi = 0; dgi < 2; dgi++) {
+        if (!diag_active(S, n, dgi)) continue;
+        int p = P2[dgi][0], q2 = P2[dgi][1];
+        double ec[3];
+        for (int z = 0; z < 3; z++) ec[z] = x[3 * q2 + z] + dx[3 * q2 + z] - x[3 * p + z] - dx[3 * p + z];
+        double l0 = norm3(ec), sl[3], sa[3];
+        or_shear_law(ks, Ld, l0, jam, sl);
+        for (int k = 0; k < NA; k++) {
+          double Dk[3], dl, la;
+          for (int z = 0; z < 3; z++) Dk[z] = S_(q2, k)[z] - S_(p, k)[z];
+          len_change_d(ec, Dk, l0, &dl, &la);
+          or_shear_law(ks, Ld, la, jam, sa);
+          ADD(k, 0.5 * (sl[0] + sa[0]) * dl);
+        }
+      }
+    }
+  }
+#undef S_
+#undef ADD
+#undef QUADS
+  free(st);
+}
+
+void or_set_curve(or_solver* S, const double* curve) {
+  free(S->curve);
+  S->curve = NULL;
+  if (!curve) return;
+  S->curve = malloc(sizeof(double) * 7 * (size_t)S->N);
+  memcpy(S->curve, curve, sizeof(double) * 7 * (size_t)S->N);
 }
 
 /* potential (solver.ts:537-595) */
@@ -1563,7 +1732,7 @@ void or_destroy(or_solver* S) {
   free(S->x); free(S->v); free(S->dx); free(S->pred); free(S->rest); free(S->softening); free(S->intact);
   free(S->cut_mask); free(S->fail_site); free(S->seg_flags); free(S->tails); free(S->strain);
   free(S->last_update); free(S->nbr); free(S->layer_of); free(S->grip_of); free(S->grip_w); free(S->grip_off);
-  free(S->ext); free(S->cmeta); free(S->cL);
+  free(S->ext); free(S->cmeta); free(S->cL); free(S->curve);
   free(S);
 }
 
@@ -2142,6 +2311,7 @@ void* or_ptr(or_solver* S, int which) {
     case 12: return S->strain;
     case 13: return S->last_e;
     case 14: return S->last_m;
+    case 15: return S->pinned;
     default: return 0;
   }
 }

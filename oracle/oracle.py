@@ -64,6 +64,7 @@ def lib():
             getattr(L, name).restype = C.c_int
         L.or_set_ext.argtypes = [vp, dp]
         L.or_set_cmeta.argtypes = [vp, dp]
+        L.or_set_curve.argtypes = [vp, dp]
         L.or_set_floor.argtypes = [vp, C.c_int, C.c_double]
         L.or_project_start.argtypes = [vp]
         # FP32 self-contact restatement (contact_oracle.c)
@@ -217,7 +218,7 @@ class Oracle:
     _PTR = {"x": (0, np.float64, 3), "v": (1, np.float64, 3), "dx": (2, np.float64, 3), "pred": (3, np.float64, 3),
             "rest": (4, np.float64, 2), "intact": (5, np.float64, 2), "cut_mask": (6, np.uint32, 2),
             "seg_flags": (7, np.uint16, 2), "fail_site": (8, np.float64, 2), "tails": (9, np.float64, 4),
-            "last_update": (11, np.float64, 1), "strain": (12, np.float64, 2)}
+            "last_update": (11, np.float64, 1), "strain": (12, np.float64, 2), "pinned": (15, np.uint8, 1)}
 
     def __init__(self, scene, params=None, grips=None):
         self.scene = scene
@@ -297,6 +298,14 @@ class Oracle:
         """(N, 9) gradient + Hessian [xx xy xz yy yz zz] per node, or None."""
         a = None if ext is None else np.ascontiguousarray(ext, dtype=np.float64).ravel()
         lib().or_set_ext(self.h, _dp(a) if a is not None else None)
+
+    def set_curve(self, curve):
+        """Detached pieces' curved line-search steps, (N, 7): rotation w xyz, flag, arm r xyz (or None)."""
+        if curve is None:
+            lib().or_set_curve(self.h, None)
+            return
+        c = np.ascontiguousarray(curve, dtype=np.float64).reshape(-1)
+        lib().or_set_curve(self.h, c.ctypes.data_as(C.POINTER(C.c_double)))
 
     def set_cmeta(self, cmeta):
         """(N, 2, 4) own and reciprocal bounds, or None."""
