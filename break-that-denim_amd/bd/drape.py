@@ -36,12 +36,24 @@ def decode_drape(scene, data: bytes):
     return np.frombuffer(data[16:], dtype=np.float32).reshape(scene.node_count, 4).copy()
 
 
+def node_states(w):
+    """The node states a drape's w lane holds: 0 cloth, 1 detached (w > 0.5), 2 retired (w > 1.5)."""
+    w = np.asarray(w)
+    return np.where(w > 1.5, 2, np.where(w > 0.5, 1, 0)).astype(np.uint8)
+
+
 def load_drape(solver, data: bytes) -> bool:
-    """loadDrape (settle.ts:16-22): upload a drape that matches the solver's scene; False otherwise."""
+    """loadDrape (settle.ts:16-22): upload a drape that matches the solver's scene; False otherwise.
+    Detached and retired nodes keep their state (the w lane); detached pieces are then labelled
+    again from the span graph (Topology.request)."""
     p = decode_drape(solver.scene, data)
     if p is None:
         return False
     solver.set_state(p[:, :3].astype(np.float64))
+    st = node_states(p[:, 3])
+    if st.any():
+        solver.set_node_state(st)
+        solver.update_topology()
     return True
 
 

@@ -56,3 +56,9 @@ def test_settled_drape_is_near_equilibrium_of_the_oracle():
     settled, start = residual(p[:, :3]), residual(S.position)
     assert np.median(settled) < 1e-2 * np.median(start)
     assert np.percentile(settled, 90) < 1e-2 * np.percentile(start, 90)
+
+
+def test_node_states_from_the_w_lane():
+    """The drape's w lane holds the node state (state.wgsl): 0 cloth, > 0.5 detached, > 1.5 retired."""
+    from bd.drape import node_states
+    assert node_states([0.0, 0.4, 0.6, 1.0, 1.5, 1.6, 2.0]).tolist() == [0, 0, 1, 1, 1, 2, 2]
