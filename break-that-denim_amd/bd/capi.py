@@ -177,6 +177,16 @@ def make_params(h=1 / 120, gravity=(0.0, 0.0, -9.81), newton_iterations=2, newto
     return p
 
 
+BD_MAX_GRIPS_HELD = 4
+BD_GRIP_SLOT_ENTRIES = 512
+BD_PICK_MAX_WINDOW_RADIUS = 7
+
+
+class bd_pick(C.Structure):
+    _fields_ = [("node", C.c_int32), ("kind", C.c_int32), ("position", C.c_double * 3), ("window_radius", C.c_int32),
+                ("window", C.c_uint32 * ((2 * BD_PICK_MAX_WINDOW_RADIUS + 1) ** 2))]
+
+
 class GripBuffers:
     """A `bd_grips` struct for a list of grips, each a dict with nodes, weights, offsets (E, 3),
     goal (3,) and stiffness k (N/m per unit weight)."""
@@ -250,6 +260,18 @@ def load_library(path: os.PathLike | None = None):
         getattr(lib, name).argtypes = [vp]
     lib.bd_set_floor.argtypes = [vp, C.c_int, C.c_double]
     u32p, i32p, u8p = P(C.c_uint32), P(C.c_int32), P(C.c_uint8)
+    f32p, f64p = P(C.c_float), P(C.c_double)
+    lib.bd_grip_attach.argtypes = [vp, C.c_int, C.c_int32, i32p, f32p, f32p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, f64p]
+    lib.bd_grip_lead.argtypes = [vp]
+    lib.bd_grip_terms.argtypes = [vp]
+    lib.bd_grip_target.argtypes = [vp, C.c_int, f64p]
+    lib.bd_grip_pinch.argtypes = [vp, C.c_int, C.c_double]
+    lib.bd_grip_hold.argtypes = [vp, C.c_int, C.c_double]
+    lib.bd_grip_release.argtypes = [vp, C.c_int]
+    lib.bd_pick_ray.argtypes = [vp, f32p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                f64p, f64p, P(bd_pick)]
+    lib.bd_pick_near.argtypes = [vp, f64p, C.c_double, C.c_int, P(bd_pick)]
     lib.bd_set_pieces.argtypes = [vp, u32p, u32p, C.c_int, u32p]
     lib.bd_update_topology.argtypes = [vp, i32p, C.c_int, i32p]
     lib.bd_topology_search.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, u8p, u8p, i32p, C.c_int, u8p, u32p, u32p,
@@ -268,4 +290,6 @@ EXPORTED = (
     "bd_write", "bd_buffer_bytes", "bd_get_stats", "bd_set_grip_goals", "bd_profile_enable", "bd_profile_read",
     "bd_contact_detect", "bd_contact_terms", "bd_project_start", "bd_set_floor",
     "bd_set_pieces", "bd_update_topology", "bd_topology_search",
+    "bd_grip_attach", "bd_grip_lead", "bd_grip_terms", "bd_grip_target", "bd_grip_pinch", "bd_grip_hold",
+    "bd_grip_release", "bd_pick_ray", "bd_pick_near",
 )

@@ -674,7 +674,7 @@ void launch_assemble(Solver& s) {
   A.grip_w = s.grip_w;
   A.grip_off = s.grip_off;
   A.grip_goal_k = s.grip_goal_k;
-  A.ext = s.contact_mode != kContactNone ? s.ext : nullptr;
+  A.ext = (s.contact_mode != kContactNone || s.grips_active) ? s.ext : nullptr;
   A.cmeta = s.contact_mode != kContactNone ? s.cmeta : nullptr;
   A.floor_on = s.floor_on ? 1 : 0;
   A.floor_z = s.floor_z;

@@ -110,6 +110,38 @@ struct ProfRec {
   double bytes;
 };
 
+// a grip slot (interact.wgsl GripSlot): handle target in the frame of `anchor`, entry count,
+// stiffness per unit weight, largest lead, picked node, pinch scale, pad pitch, slip distance
+struct GripSlot {
+  float goal[3];
+  int count;
+  float stiffness, max_lead;
+  uint32_t picked;
+  float pinch;
+  float anchor[3];
+  float pad_pitch;
+  float slip;
+  float pad_[3];
+};
+// a patch entry (interact.wgsl GripEntry): node, weight, pad side (-1, +1, 0 radial), grab-time offset
+struct GripEntry {
+  uint32_t node;
+  float w, side, pad0;
+  float offset[3];
+  float pad1;
+};
+// a pick query (interact.wgsl PickU)
+struct PickArgs {
+  float view_proj[16];  // column-major
+  float pointer[2];
+  float radius_px, width, height;
+  float strand_radius;  // strand pick radius (m); the search radius of a nearest-material pick
+  int window_radius;
+  float eye[3], ray[3];
+};
+constexpr int kGrips = BD_MAX_GRIPS_HELD;
+constexpr int kSlotEntries = BD_GRIP_SLOT_ENTRIES;
+
 struct Solver {
   bd_params P{};
   // profiling
@@ -136,6 +168,16 @@ struct Solver {
   // the scene: frozen, spans removed). `pinned` holds bit kPinMount for mounted nodes and bit
   // kPinRetired for retired ones, so every kernel that skips pinned rows skips retired ones too.
   std::vector<uint8_t> mount_h;  // host copy of the mounted flags (topology anchors)
+  // grips and picking (interact.wgsl)
+  GripSlot* grip_slots = nullptr;
+  GripSlot grip_slots_h[BD_MAX_GRIPS_HELD] = {};
+  float grip_full_k[BD_MAX_GRIPS_HELD] = {};
+  GripEntry* grip_entries = nullptr;
+  float* grip_handle = nullptr;  // 12 per grip: effective handle, origin, closing axis
+  bool grips_active = false;
+  uint32_t* pick = nullptr;
+  uint32_t* pick_out = nullptr;
+  PickArgs pick_args{};
   uint8_t* nstate = nullptr;
   bool any_detached = false;  // some node has been detached (commit checks for retirement)
   // detached pieces and their rigid modes (rigid_frame/apply/spin.wgsl)
@@ -265,6 +307,12 @@ void launch_rigid_start(Solver& s);
 void launch_spin(Solver& s);
 void launch_curve_set(Solver& s);
 void launch_curve_apply(Solver& s);
+// interact.hip
+void interact_alloc(Solver& s);
+void launch_grip_init(Solver& s, int g);
+void launch_grip_lead(Solver& s);
+void launch_grip_terms(Solver& s);
+void launch_pick(Solver& s, bool ray);
 // coarse.hip
 void coarse_exact_alloc(Solver& s);
 void launch_coarse_factor(Solver& s);

@@ -154,7 +154,8 @@ void destroy_all(Solver& s) {
                   s.damage, s.area, s.sep, s.pullmark, s.strength, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k,
                   s.partials, s.scal, s.counters, s.ijl, s.cells, s.scan_sums, s.items, s.pairs, s.keys,
                   s.marks, s.cmeta, s.acc, s.ext, s.cband, s.cinv, s.crd, s.pcells, s.psums, s.pitems,
-                  s.nstate, s.piece_start, s.piece_nodes, s.piece_of, s.piece_frame, s.piece_step, s.rigidk, s.curve};
+                  s.nstate, s.piece_start, s.piece_nodes, s.piece_of, s.piece_frame, s.piece_step, s.rigidk, s.curve,
+                  s.grip_slots, s.grip_entries, s.grip_handle, s.pick, s.pick_out};
   for (void* p : ptrs) dfree(p);
   for (int k = 0; k < s.nlev; k++) {
     Level& L = s.lv[k];
@@ -227,6 +228,29 @@ void predict(Solver& s) {
   launch_spin(s);
 }
 
+// the grip slot on the device
+void upload_slot(Solver& s, int g) {
+  BD_HIP(hipMemcpyAsync(s.grip_slots + g, &s.grip_slots_h[g], sizeof(GripSlot), hipMemcpyHostToDevice, s.stream));
+  BD_HIP(hipStreamSynchronize(s.stream));
+  bool any = false;
+  for (int q = 0; q < kGrips; q++) any = any || s.grip_slots_h[q].count > 0;
+  s.grips_active = any;
+}
+
+void fill_pick(Solver& s, int window_radius, bd_pick* out) {
+  std::vector<uint32_t> w(8 + 256);
+  download(s, w.data(), s.pick_out, w.size());
+  float p[3];
+  std::memcpy(p, w.data(), sizeof(p));
+  out->kind = (int32_t)w[4];
+  out->node = w[4] == 0 ? -1 : (int32_t)w[3];
+  for (int q = 0; q < 3; q++) out->position[q] = p[q];
+  out->window_radius = window_radius;
+  const int side = 2 * window_radius + 1;
+  std::memset(out->window, 0, sizeof(out->window));
+  for (int k = 0; k < side * side; k++) out->window[k] = w[8 + k];
+}
+
 // encodeLineSearch (gpu/solver.ts:678-684)
 void line_search(Solver& s) {
   launch_curve_set(s);
@@ -235,6 +259,8 @@ void line_search(Solver& s) {
 
 void newton_iteration(Solver& s, int it, bool coarse) {
   contact_terms(s);
+  // grip terms at this iterate, added to contact's (GpuSolver.preAssemble, interaction.ts:341-345)
+  launch_grip_terms(s);
   assemble_system(s, coarse);
   pcg_solve(s);
   line_search(s);
@@ -406,7 +432,8 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
       BD_HIP(hipMemsetAsync(s.ext, 0, 9 * (size_t)Np * sizeof(float), s.stream));
       s.cmeta = dalloc<float4>(2 * (size_t)Np);
       contact_reset_bounds(s);
-      // a scene of several plies always has cross-layer contact; a one-layer scene self-contact
+      // (the grip and pick buffers are allocated on the first attach or pick)
+    // a scene of several plies always has cross-layer contact; a one-layer scene self-contact
       // when asked for (gpu/solver.ts:322, 364)
       s.contact_mode = s.nl > 1 ? kContactPlies : (s.P.self_contact ? kContactSelf : kContactNone);
       if (s.contact_mode == kContactSelf) contact_alloc(s);
@@ -716,6 +743,140 @@ int bd_set_floor(bd_solver* h, int enabled, double z) {
   });
 }
 
+int bd_grip_attach(bd_solver* h, int slot, int32_t picked, const int32_t* nodes, const float* weights, const float* sides,
+                   int count, double stiffness, double max_lead, double pad_pitch, double slip, const double* anchor) {
+  return guarded(h, [&](Solver& s) {
+    if (slot < 0 || slot >= kGrips) return fail(h, BD_ERR_INVALID, "bd_grip_attach: slot out of range");
+    if (picked < 0 || picked >= s.N || count < 0 || (count > 0 && (!nodes || !weights || !sides)) || !anchor)
+      return fail(h, BD_ERR_INVALID, "bd_grip_attach: bad arguments");
+    // entries beyond a slot's capacity are dropped (interaction.ts:228)
+    const int c = count < kSlotEntries ? count : kSlotEntries;
+    std::vector<GripEntry> e(c);
+    for (int k = 0; k < c; k++) {
+      if (nodes[k] < 0 || nodes[k] >= s.N) return fail(h, BD_ERR_INVALID, "bd_grip_attach: node out of range");
+      e[k] = GripEntry{(uint32_t)nodes[k], weights[k], sides[k], 0.f, {0.f, 0.f, 0.f}, 0.f};
+    }
+    interact_alloc(s);
+    if (c > 0) upload(s, s.grip_entries + (size_t)slot * kSlotEntries, e.data(), (size_t)c);
+    GripSlot& S = s.grip_slots_h[slot];
+    S = GripSlot{};
+    for (int q = 0; q < 3; q++) {
+      S.goal[q] = (float)anchor[q];
+      S.anchor[q] = (float)anchor[q];
+    }
+    S.count = c;
+    S.stiffness = (float)stiffness;
+    S.max_lead = (float)max_lead;
+    S.picked = (uint32_t)picked;
+    S.pinch = 1.f;
+    S.pad_pitch = (float)pad_pitch;
+    S.slip = (float)slip;
+    s.grip_full_k[slot] = (float)stiffness;
+    upload_slot(s, slot);
+    launch_grip_init(s, slot);
+    return BD_OK;
+  });
+}
+
+int bd_grip_lead(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    launch_grip_lead(s);
+    return BD_OK;
+  });
+}
+
+int bd_grip_terms(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    launch_grip_terms(s);
+    return BD_OK;
+  });
+}
+
+int bd_grip_target(bd_solver* h, int slot, const double* target) {
+  return guarded(h, [&](Solver& s) {
+    if (slot < 0 || slot >= kGrips || !target) return fail(h, BD_ERR_INVALID, "bd_grip_target: bad arguments");
+    if (!s.grip_slots || s.grip_slots_h[slot].count == 0) return BD_OK;
+    for (int q = 0; q < 3; q++) s.grip_slots_h[slot].goal[q] = (float)target[q];
+    upload_slot(s, slot);
+    return BD_OK;
+  });
+}
+
+int bd_grip_pinch(bd_solver* h, int slot, double scale) {
+  return guarded(h, [&](Solver& s) {
+    if (slot < 0 || slot >= kGrips) return fail(h, BD_ERR_INVALID, "bd_grip_pinch: slot out of range");
+    if (!s.grip_slots || s.grip_slots_h[slot].count == 0) return BD_OK;
+    s.grip_slots_h[slot].pinch = (float)scale;
+    upload_slot(s, slot);
+    return BD_OK;
+  });
+}
+
+int bd_grip_hold(bd_solver* h, int slot, double fraction) {
+  return guarded(h, [&](Solver& s) {
+    if (slot < 0 || slot >= kGrips) return fail(h, BD_ERR_INVALID, "bd_grip_hold: slot out of range");
+    if (!s.grip_slots || s.grip_slots_h[slot].count == 0) return BD_OK;
+    s.grip_slots_h[slot].stiffness = (float)(s.grip_full_k[slot] * fraction);
+    upload_slot(s, slot);
+    return BD_OK;
+  });
+}
+
+int bd_grip_release(bd_solver* h, int slot) {
+  return guarded(h, [&](Solver& s) {
+    if (slot < 0 || slot >= kGrips) return fail(h, BD_ERR_INVALID, "bd_grip_release: slot out of range");
+    if (!s.grip_slots || s.grip_slots_h[slot].count == 0) return BD_OK;
+    s.grip_slots_h[slot].count = 0;
+    upload_slot(s, slot);
+    // without contact the grips own ext: clear it (the remaining grips rewrite their entries)
+    if (s.contact_mode == kContactNone) BD_HIP(hipMemsetAsync(s.ext, 0, 9 * (size_t)s.Np * sizeof(float), s.stream));
+    return BD_OK;
+  });
+}
+
+int bd_pick_ray(bd_solver* h, const float* view_proj, double px, double py, double width, double height,
+                double radius_px, double strand_radius, int window_radius, const double* eye, const double* dir,
+                bd_pick* out) {
+  return guarded(h, [&](Solver& s) {
+    if (!view_proj || !eye || !dir || !out || window_radius < 0 || window_radius > BD_PICK_MAX_WINDOW_RADIUS)
+      return fail(h, BD_ERR_INVALID, "bd_pick_ray: bad arguments");
+    interact_alloc(s);
+    PickArgs& P = s.pick_args;
+    for (int q = 0; q < 16; q++) P.view_proj[q] = view_proj[q];
+    P.pointer[0] = (float)px;
+    P.pointer[1] = (float)py;
+    P.radius_px = (float)radius_px;
+    P.width = (float)width;
+    P.height = (float)height;
+    P.strand_radius = (float)strand_radius;
+    P.window_radius = window_radius;
+    for (int q = 0; q < 3; q++) {
+      P.eye[q] = (float)eye[q];
+      P.ray[q] = (float)dir[q];
+    }
+    launch_pick(s, true);
+    fill_pick(s, window_radius, out);
+    return BD_OK;
+  });
+}
+
+int bd_pick_near(bd_solver* h, const double* p, double radius, int window_radius, bd_pick* out) {
+  return guarded(h, [&](Solver& s) {
+    if (!p || !out || !(radius > 0) || window_radius < 0 || window_radius > BD_PICK_MAX_WINDOW_RADIUS)
+      return fail(h, BD_ERR_INVALID, "bd_pick_near: bad arguments");
+    interact_alloc(s);
+    PickArgs& P = s.pick_args;
+    P = PickArgs{};
+    P.width = P.height = 1.f;
+    P.strand_radius = (float)radius;
+    P.window_radius = window_radius;
+    for (int q = 0; q < 3; q++) P.eye[q] = (float)p[q];
+    launch_pick(s, false);
+    fill_pick(s, window_radius, out);
+    return BD_OK;
+  });
+}
+
 int bd_set_pieces(bd_solver* h, const uint32_t* piece_of, const uint32_t* start, int count, const uint32_t* nodes) {
   return guarded(h, [&](Solver& s) {
     if (count < 0 || (count > 0 && (!piece_of || !start || !nodes)))
@@ -791,6 +952,8 @@ int bd_substep(bd_solver* h, int min_it, int max_it) {
     const int n_max = std::min(BD_MAX_NEWTON, std::max(std::max(s.P.newton_max, n_it), max_it));
     launch_reset_counters(s);
     predict(s);
+    // once per substep: each grip's handle lead over its patch (interaction.ts:335-338)
+    launch_grip_lead(s);
     contact_detect(s);
     if (s.contact_mode != kContactNone || s.floor_on) launch_project_start(s);
     int done = 0;

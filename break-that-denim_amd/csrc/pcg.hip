@@ -797,7 +797,7 @@ void launch_energy(Solver& s) {
   }
   EnergyArgs E{s.N, s.Np, s.geo, s.layers, s.P.h, s.x, s.dx, s.pred, s.mass, s.delta, s.cn0, s.pinned, s.rest, s.intact,
                s.cut_mask, s.flags, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k, s.scal, s.partials,
-               s.contact_mode != kContactNone ? s.ext : nullptr, s.floor_on ? 1 : 0, (double)s.floor_z,
+               (s.contact_mode != kContactNone || s.grips_active) ? s.ext : nullptr, s.floor_on ? 1 : 0, (double)s.floor_z,
                s.npieces > 0 ? s.curve : nullptr};
   ProfScope ps(s, PC_ENERGY, s.N * 161.0);
   // 256-thread blocks keep the FP64 energy terms in registers

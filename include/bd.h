@@ -267,6 +267,47 @@ int bd_project_start(bd_solver* s);
 /* setFloor(z | null) (gpu/solver.ts:796-801): floor plane at height z when `enabled`, else none. */
 int bd_set_floor(bd_solver* s, int enabled, double z);
 
+/* Grips and picking (src/simulation/gpu/interaction.ts, interact.wgsl): up to BD_MAX_GRIPS_HELD grips
+ * hold disjoint material patches of at most BD_GRIP_SLOT_ENTRIES entries, each a quadratic spring from
+ * the node to its target (the handle plus its grab-time offset, gathered by the pinch) whose handle is
+ * clamped once per substep to within max_lead of the held patch's centroid (bounded total force). The
+ * terms enter the Newton system through the external terms (added to contact's). */
+#define BD_MAX_GRIPS_HELD 4
+#define BD_GRIP_SLOT_ENTRIES 512
+#define BD_PICK_MAX_CANDIDATES 128
+#define BD_PICK_MAX_WINDOW_RADIUS 7
+/* Interaction.attachPatch (interaction.ts:225-262): grip `slot` holds nodes[0..count) (weights, pad
+ * sides -1 / +1 or 0 for a radial pinch) with stiffness per unit weight, largest handle lead, pad
+ * pitch and slip distance (0: the hold does not slip); offsets and the handle's origin are taken on
+ * the GPU from node `picked` now; targets are in the frame of `anchor`. */
+int bd_grip_attach(bd_solver* s, int slot, int32_t picked, const int32_t* nodes, const float* weights, const float* sides,
+                   int count, double stiffness, double max_lead, double pad_pitch, double slip, const double* anchor);
+int bd_grip_lead(bd_solver* s);  /* Interaction.encodeSubstep: the handles' lead clamp (bd_substep runs it) */
+int bd_grip_terms(bd_solver* s); /* Interaction.encode: the grip terms into ext (each Newton iteration) */
+int bd_grip_target(bd_solver* s, int slot, const double* target); /* setTarget */
+int bd_grip_pinch(bd_solver* s, int slot, double scale);          /* setPinch: spread kept by the pinch */
+int bd_grip_hold(bd_solver* s, int slot, double fraction);        /* setHold: fraction of the stiffness */
+int bd_grip_release(bd_solver* s, int slot);                      /* release */
+/* The result of a pick (interaction.ts PickResult): node (-1: none), kind (1 surface hit, 2 proximity),
+ * the picked point, and the connectivity window around the node: per lattice offset in [-R, R]^2,
+ * row-major from (-R, -R), bit 0 / 1 = its own span along u / v survives, bit 2 = not cloth, bit 3 =
+ * outside the lattice. */
+typedef struct bd_pick {
+  int32_t node;
+  int32_t kind;
+  double position[3];
+  int32_t window_radius;
+  uint32_t window[(2 * BD_PICK_MAX_WINDOW_RADIUS + 1) * (2 * BD_PICK_MAX_WINDOW_RADIUS + 1)];
+} bd_pick;
+/* Interaction.pickAt (interaction.ts:114-144): the nearest surface (intact face, or strand over an
+ * opening within strand_radius) along the ray eye + t dir; over none, the nearest crossing within
+ * radius_px of the pointer that no intact face hides (view_proj column-major, drawing-buffer pixels). */
+int bd_pick_ray(bd_solver* s, const float* view_proj, double px, double py, double width, double height,
+                double radius_px, double strand_radius, int window_radius, const double* eye, const double* dir,
+                bd_pick* out);
+/* Interaction.nearest (interaction.ts:150-166): the crossing with material nearest to p within radius. */
+int bd_pick_near(bd_solver* s, const double* p, double radius, int window_radius, bd_pick* out);
+
 /* Detached pieces (src/simulation/gpu/topology.ts, rigid_frame/apply/spin.wgsl). A piece is a set of
  * detached nodes connected through active spans; its six rigid modes are carried by the Newton solve
  * (PCG start from the piece's exact rigid response, spin start of the substep, line search along the
