@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--profile-frames", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default smoother kernel per level")
     ap.add_argument("--method", choices=("both", "mixed", "plain"), default="both")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 8 plain device copies of MIB MiB (calibrates the PMC byte counters)")
@@ -125,6 +126,8 @@ def run_method(args, cfg, coarse_solve, profile_frames):
     for _ in range(args.warmup):
         frame()
     gpu.sync()
+    # the fastest smoother kernel per level and family for this hierarchy (bd_autotune)
+    tuned = gpu.autotune(5) if not args.no_autotune else {}
     iters, newton = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -135,7 +138,7 @@ def run_method(args, cfg, coarse_solve, profile_frames):
     gpu.sync()
     t1 = time.perf_counter()
     out = {"ms": 1e3 * (t1 - t0) / args.steps, "iters": float(np.mean(iters)), "newton": float(np.mean(newton)),
-           "levels": gpu.stats()["level_nodes"]}
+           "levels": gpu.stats()["level_nodes"], "tuned": tuned}
     gpu.profile_enable(True)
     for _ in range(profile_frames):
         frame()
@@ -209,7 +212,8 @@ def main():
                    "method": main_name + (" (FP64 exact coarsest solve)" if main_name == "mixed" else " (line sweeps)")},
         "pcg_iterations_per_frame": iters_per_frame,
         "pcg_iterations_per_newton": round(iters_per_frame / max(newton_per_frame, 1), 2),
-        "methods": {k: {"ms_per_frame": round(v["ms"], 3), "pcg_iterations_per_frame": v["iters"]}
+        "methods": {k: {"ms_per_frame": round(v["ms"], 3), "pcg_iterations_per_frame": v["iters"],
+                        "smoother_kernels": {str(lv): list(c) for lv, c in v["tuned"].items()}}
                     for k, v in runs.items()},
         "iteration_ratio_plain_over_mixed": round(runs["plain"]["iters"] / runs["mixed"]["iters"], 3) if both else None,
         "speedup_plain_over_mixed": round(runs["plain"]["ms"] / runs["mixed"]["ms"], 3) if both else None,

@@ -272,6 +272,7 @@ def load_library(path: os.PathLike | None = None):
     lib.bd_pick_ray.argtypes = [vp, f32p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                 f64p, f64p, P(bd_pick)]
     lib.bd_pick_near.argtypes = [vp, f64p, C.c_double, C.c_int, P(bd_pick)]
+    lib.bd_autotune.argtypes = [vp, C.c_int, i32p]
     lib.bd_set_pieces.argtypes = [vp, u32p, u32p, C.c_int, u32p]
     lib.bd_update_topology.argtypes = [vp, i32p, C.c_int, i32p]
     lib.bd_topology_search.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, u8p, u8p, i32p, C.c_int, u8p, u32p, u32p,
@@ -291,5 +292,5 @@ EXPORTED = (
     "bd_contact_detect", "bd_contact_terms", "bd_project_start", "bd_set_floor",
     "bd_set_pieces", "bd_update_topology", "bd_topology_search",
     "bd_grip_attach", "bd_grip_lead", "bd_grip_terms", "bd_grip_target", "bd_grip_pinch", "bd_grip_hold",
-    "bd_grip_release", "bd_pick_ray", "bd_pick_near",
+    "bd_grip_release", "bd_pick_ray", "bd_pick_near", "bd_autotune",
 )

@@ -211,6 +211,13 @@ class GpuSolver:
         """(N, 7): rotation w xyz, flag, arm r xyz."""
         return self.read(BD_BUF_CURVE).reshape(-1, 7)
 
+    def autotune(self, reps=3):
+        """Pick the fastest smoother kernel per level and family (bd_autotune): {level: (u, v)}."""
+        nlev = len(self.stats()["level_nodes"])
+        out = np.zeros(2 * nlev, dtype=np.int32)
+        self._check(self.lib.bd_autotune(self.h, reps, out.ctypes.data_as(C.POINTER(C.c_int32))), "bd_autotune")
+        return {k: (int(out[2 * k]), int(out[2 * k + 1])) for k in range(nlev)}
+
     def sync(self):
         self._check(self.lib.bd_sync(self.h), "bd_sync")
 

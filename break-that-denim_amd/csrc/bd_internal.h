@@ -65,6 +65,8 @@ struct Level {
   // dual layout (lines longer than 128 nodes): column-major copies (node (i, j) of a layer at
   // off + i * ny + j) of the operator, fixed flags and iterate for the v-line passes; the v-line
   // entries of `lines` then start in that layout with stride 1
+  // smoother kernel per family (u, v): -1 the default rule, 0 direct, 4 / 8 staged (bd_autotune)
+  int variant[2] = {-1, -1};
   bool dual = false;
   float* opT = nullptr;
   uint8_t* fixedT = nullptr;
@@ -278,6 +280,7 @@ void launch_residual(Solver& s, int k, const float* x, const float* b, float* r)
 void launch_restrict(Solver& s, int k);
 void launch_prolong_add(Solver& s, int k);
 void launch_smooth(Solver& s, int k, int reverse);
+std::vector<int> smooth_variants(Solver& s, int k, int fam);
 void launch_zero(Solver& s, float* p, size_t n);
 // column-major copies of a dual-layout level: operator and fixed flags, or the iterate
 void launch_colmajor_op(Solver& s, int k);

@@ -877,6 +877,64 @@ int bd_pick_near(bd_solver* h, const double* p, double radius, int window_radius
   });
 }
 
+int bd_autotune(bd_solver* h, int reps, int32_t* chosen) {
+  return guarded(h, [&](Solver& s) {
+    if (!s.hierarchy) return fail(h, BD_ERR_STATE, "bd_autotune: assemble first");
+    if (reps < 1) return fail(h, BD_ERR_INVALID, "bd_autotune: reps must be positive");
+    // Greedy over levels and families, fine first: each variant is timed inside whole V-cycles
+    // (the others at their current choice), so that the level's data is as warm or cold as in the
+    // solve; the fastest of `reps` V-cycles counts. b = x = 0: every kernel still does all its work.
+    hipEvent_t e0, e1;
+    BD_HIP(hipEventCreate(&e0));
+    BD_HIP(hipEventCreate(&e1));
+    Level& F = s.lv[0];
+    auto time_vcycles = [&]() {
+      float best = 0.f;
+      for (int r = 0; r < reps + 1; r++) {
+        launch_zero(s, F.b, 3 * (size_t)F.Np);
+        launch_zero(s, F.x, 3 * (size_t)F.Np);
+        if (F.dual) launch_zero(s, F.xT, 3 * (size_t)F.Np);
+        BD_HIP(hipEventRecord(e0, s.stream));
+        vcycle(s, 0);
+        BD_HIP(hipEventRecord(e1, s.stream));
+        BD_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        BD_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (r > 0 && (r == 1 || ms < best)) best = ms;  // the first run warms up
+      }
+      return best;
+    };
+    BD_HIP(hipMemsetAsync(&s.scal->done, 0, sizeof(int), s.stream));
+    for (int k = 0; k < s.nlev; k++)
+      for (int fam = 1; fam >= 0; fam--) {
+        std::vector<int> vs = smooth_variants(s, k, fam);
+        if (vs.size() < 2) {
+          s.lv[k].variant[fam] = vs.empty() ? -1 : vs[0];
+          continue;
+        }
+        float best = 0.f;
+        int pick = -1;
+        for (int v : vs) {
+          s.lv[k].variant[fam] = v;
+          const float ms = time_vcycles();
+          if (pick < 0 || ms < best) {
+            best = ms;
+            pick = v;
+          }
+        }
+        s.lv[k].variant[fam] = pick;
+      }
+    BD_HIP(hipEventDestroy(e0));
+    BD_HIP(hipEventDestroy(e1));
+    if (chosen)
+      for (int k = 0; k < s.nlev; k++) {
+        chosen[2 * k] = s.lv[k].variant[0];
+        chosen[2 * k + 1] = s.lv[k].variant[1];
+      }
+    return BD_OK;
+  });
+}
+
 int bd_set_pieces(bd_solver* h, const uint32_t* piece_of, const uint32_t* start, int count, const uint32_t* nodes) {
   return guarded(h, [&](Solver& s) {
     if (count < 0 || (count > 0 && (!piece_of || !start || !nodes)))

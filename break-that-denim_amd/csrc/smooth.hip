@@ -658,9 +658,24 @@ size_t staged_shm(int len) {
   return sizeof(float) * ((size_t)27 * Lr + (size_t)(6 + 9 + 3 + 3 + 4 * G) * len) + (size_t)len + 16;
 }
 
+// LDS a workgroup may use (the staged kernels' attribute is raised to it for the autotuner)
+constexpr int kMaxLds = 160 * 1024;
+
+// variant: -1 the default rule below, 0 the direct kernel, 4 or 8 the staged kernel with that gather
+// width (bd_autotune picks per level and family)
 template <bool FINE, int FAM, bool DUAL>
-void launch_one(const SmoothArgs& S, int count, int len, hipStream_t st) {
-  if (len <= kStagedMaxLen) {
+void launch_one(const SmoothArgs& S, int count, int len, hipStream_t st, int variant) {
+  if (variant == 4 || variant == 8) {
+    const int G = variant;
+    const int R2 = (len + 1) / 2;
+    const int threads = std::max(std::min(1024, ((len * G + 63) / 64) * 64), ((R2 + 63) / 64) * 64);
+    if (G == 8)
+      k_smooth_staged<FINE, FAM, 8, DUAL><<<count, threads, staged_shm<FINE, FAM, 8>(len), st>>>(S);
+    else
+      k_smooth_staged<FINE, FAM, 4, DUAL><<<count, threads, staged_shm<FINE, FAM, 4>(len), st>>>(S);
+    return;
+  }
+  if (variant < 0 && len <= kStagedMaxLen) {
     // widest gather that fits 1024 threads: couplings per thread ~ (FINE ? 12 : 24) / G
     const int G = len <= 128 ? 8 : 4;
     int threads = std::min(1024, ((len * G + 63) / 64) * 64);
@@ -706,10 +721,62 @@ void set_attrs(int shm, int shm_staged) {
 
 void smooth_set_attributes() {
   int shm = 27 * ((BD_MAX_LINE / 2 + 63) / 64 * 64) * (int)sizeof(float);
-  int shm_staged = (int)staged_shm<false, 0, 8>(128);
-  shm_staged = std::max(shm_staged, (int)staged_shm<false, 0, 4>(kStagedMaxLen));
-  set_attrs<false>(shm, shm_staged);
-  set_attrs<true>(shm, shm_staged);
+  set_attrs<false>(shm, kMaxLds);
+  set_attrs<true>(shm, kMaxLds);
+}
+
+namespace {
+
+// one colour pass of level k with the given variant
+void launch_colour(Solver& s, int k, int c, int variant) {
+  Level& L = s.lv[k];
+  int count = L.color_start[c + 1] - L.color_start[c];
+  if (count == 0) return;
+  const int fam = c < 3 ? 1 : 0;
+  int maxlen = 0;
+  for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
+  const bool colmajor = L.dual && fam == 1;
+  SmoothArgs S{colmajor ? L.opT : L.op, colmajor ? L.fixedT : L.fixed, L.Np, L.lines + L.color_start[c],
+               colmajor ? L.xT : L.x, L.b, L.dual ? (colmajor ? L.x : L.xT) : nullptr, &s.scal->done, L.geo};
+  // each node of the colour's lines: its full row (fine 58 floats, coarse 150) + b, x read, x write
+  // (dual-layout levels write x twice)
+  double nodes = (double)count * maxlen;
+  ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * ((L.fine ? 268.0 : 636.0) + (L.dual ? 12.0 : 0.0)));
+  if (L.dual) {
+    if (L.fine) {
+      if (fam == 0) launch_one<true, 0, true>(S, count, maxlen, s.stream, variant);
+      else launch_one<true, 1, true>(S, count, maxlen, s.stream, variant);
+    } else {
+      if (fam == 0) launch_one<false, 0, true>(S, count, maxlen, s.stream, variant);
+      else launch_one<false, 1, true>(S, count, maxlen, s.stream, variant);
+    }
+  } else {
+    if (L.fine) {
+      if (fam == 0) launch_one<true, 0, false>(S, count, maxlen, s.stream, variant);
+      else launch_one<true, 1, false>(S, count, maxlen, s.stream, variant);
+    } else {
+      if (fam == 0) launch_one<false, 0, false>(S, count, maxlen, s.stream, variant);
+      else launch_one<false, 1, false>(S, count, maxlen, s.stream, variant);
+    }
+  }
+}
+
+// the variants a line of `len` rows can run: the direct kernel, and the staged ones whose LDS fits
+std::vector<int> variants_for(bool fine, int len) {
+  std::vector<int> v{0};
+  if ((fine ? staged_shm<true, 0, 4>(len) : staged_shm<false, 0, 4>(len)) <= (size_t)kMaxLds) v.push_back(4);
+  if ((fine ? staged_shm<true, 0, 8>(len) : staged_shm<false, 0, 8>(len)) <= (size_t)kMaxLds) v.push_back(8);
+  return v;
+}
+
+}  // namespace
+
+// The kernel variants a level's passes of one family can run.
+std::vector<int> smooth_variants(Solver& s, int k, int fam) {
+  Level& L = s.lv[k];
+  int maxlen = 0;
+  for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
+  return variants_for(L.fine, maxlen);
 }
 
 // Smooth all six colours of level k (v-lines colours 0..2 then u-lines 0..2; reversed for the
@@ -718,35 +785,7 @@ void launch_smooth(Solver& s, int k, int reverse) {
   Level& L = s.lv[k];
   for (int q = 0; q < 6; q++) {
     int c = reverse ? 5 - q : q;
-    int count = L.color_start[c + 1] - L.color_start[c];
-    if (count == 0) continue;
-    const int fam = c < 3 ? 1 : 0;
-    int maxlen = 0;
-    for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
-    const bool colmajor = L.dual && fam == 1;
-    SmoothArgs S{colmajor ? L.opT : L.op, colmajor ? L.fixedT : L.fixed, L.Np, L.lines + L.color_start[c],
-                 colmajor ? L.xT : L.x, L.b, L.dual ? (colmajor ? L.x : L.xT) : nullptr, &s.scal->done, L.geo};
-    // each node of the colour's lines: its full row (fine 58 floats, coarse 150) + b, x read, x write
-    // (dual-layout levels write x twice)
-    double nodes = (double)count * maxlen;
-    ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * ((L.fine ? 268.0 : 636.0) + (L.dual ? 12.0 : 0.0)));
-    if (L.dual) {
-      if (L.fine) {
-        if (fam == 0) launch_one<true, 0, true>(S, count, maxlen, s.stream);
-        else launch_one<true, 1, true>(S, count, maxlen, s.stream);
-      } else {
-        if (fam == 0) launch_one<false, 0, true>(S, count, maxlen, s.stream);
-        else launch_one<false, 1, true>(S, count, maxlen, s.stream);
-      }
-    } else {
-      if (L.fine) {
-        if (fam == 0) launch_one<true, 0, false>(S, count, maxlen, s.stream);
-        else launch_one<true, 1, false>(S, count, maxlen, s.stream);
-      } else {
-        if (fam == 0) launch_one<false, 0, false>(S, count, maxlen, s.stream);
-        else launch_one<false, 1, false>(S, count, maxlen, s.stream);
-      }
-    }
+    launch_colour(s, k, c, L.variant[c < 3 ? 1 : 0]);
   }
 }
 

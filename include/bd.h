@@ -267,6 +267,14 @@ int bd_project_start(bd_solver* s);
 /* setFloor(z | null) (gpu/solver.ts:796-801): floor plane at height z when `enabled`, else none. */
 int bd_set_floor(bd_solver* s, int enabled, double z);
 
+/* Autotuner (no counterpart in the reference): for every level's v-line and u-line passes of the
+ * current hierarchy, the smoother kernel variant (the direct kernel, the staged kernel with 4 or 8
+ * threads per row) whose V-cycles run fastest, chosen greedily level by level with each variant
+ * timed inside whole V-cycles (zero b and x; the fastest of `reps`). chosen receives 2 entries per
+ * level (u, v): 0 direct, 4 / 8 staged; may be NULL. Results of a pass differ between variants only
+ * by FP32 summation order. */
+int bd_autotune(bd_solver* s, int reps, int32_t* chosen);
+
 /* Grips and picking (src/simulation/gpu/interaction.ts, interact.wgsl): up to BD_MAX_GRIPS_HELD grips
  * hold disjoint material patches of at most BD_GRIP_SLOT_ENTRIES entries, each a quadratic spring from
  * the node to its target (the handle plus its grab-time offset, gathered by the pinch) whose handle is
