@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default smoother kernel per level")
     ap.add_argument("--method", choices=("both", "mixed", "plain"), default="both")
+    ap.add_argument("--sweeps", type=int, default=1, help="pre- and post-smoothing sweeps per level")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 8 plain device copies of MIB MiB (calibrates the PMC byte counters)")
     return ap.parse_args()
@@ -113,6 +114,7 @@ def run_method(args, cfg, coarse_solve, profile_frames):
     params.coarse_solve = coarse_solve
     # the mixed method stops coarsening at 33 x 33 and solves that level exactly (FP64)
     params.max_coarsest_nodes = MIXED_COARSEST_NODES if coarse_solve else 400
+    params.smooth_sweeps = args.sweeps
     gpu = GpuSolver(cfg["scene"], params, cfg["grips"])
     goals = np.array([g["goal"] for g in cfg["grips"]], dtype=np.float64)
     vel = np.array(cfg["velocity"], dtype=np.float64)
@@ -138,7 +140,8 @@ def run_method(args, cfg, coarse_solve, profile_frames):
     gpu.sync()
     t1 = time.perf_counter()
     out = {"ms": 1e3 * (t1 - t0) / args.steps, "iters": float(np.mean(iters)), "newton": float(np.mean(newton)),
-           "levels": gpu.stats()["level_nodes"], "tuned": tuned}
+           "levels": gpu.stats()["level_nodes"], "tuned": tuned,
+           "sweeps": params.smooth_sweeps}
     gpu.profile_enable(True)
     for _ in range(profile_frames):
         frame()
@@ -213,6 +216,7 @@ def main():
         "pcg_iterations_per_frame": iters_per_frame,
         "pcg_iterations_per_newton": round(iters_per_frame / max(newton_per_frame, 1), 2),
         "methods": {k: {"ms_per_frame": round(v["ms"], 3), "pcg_iterations_per_frame": v["iters"],
+                        "smooth_sweeps": v["sweeps"],
                         "smoother_kernels": {str(lv): list(c) for lv, c in v["tuned"].items()}}
                     for k, v in runs.items()},
         "iteration_ratio_plain_over_mixed": round(runs["plain"]["iters"] / runs["mixed"]["iters"], 3) if both else None,

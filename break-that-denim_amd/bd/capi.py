@@ -76,7 +76,8 @@ class bd_params(C.Structure):
                 ("newton_max", C.c_int32), ("newton_tolerance", C.c_double), ("pcg_iterations", C.c_int32),
                 ("pcg_max_iterations", C.c_int32), ("pcg_rel_tol", C.c_double), ("coarse_sweeps", C.c_int32),
                 ("max_coarsest_nodes", C.c_int32), ("lag_coarse", C.c_int32), ("line_search", C.c_int32),
-                ("history", C.c_int32), ("precision", C.c_int32), ("self_contact", C.c_int32), ("coarse_solve", C.c_int32)]
+                ("history", C.c_int32), ("precision", C.c_int32), ("self_contact", C.c_int32), ("coarse_solve", C.c_int32),
+                ("smooth_sweeps", C.c_int32)]
 
 
 class bd_grips(C.Structure):
@@ -154,7 +155,8 @@ class SceneBuffers:
 def make_params(h=1 / 120, gravity=(0.0, 0.0, -9.81), newton_iterations=2, newton_max=None,
                 newton_tolerance=1e-5, pcg_iterations=2, pcg_rel_tol=0.0, pcg_max_iterations=200,
                 coarse_sweeps=2, max_coarsest_nodes=400, lag_coarse=True, line_search=True, history=True,
-                precision=BD_PRECISION_FP32, self_contact=False, coarse_solve=BD_COARSE_SWEEPS) -> bd_params:
+                precision=BD_PRECISION_FP32, self_contact=False, coarse_solve=BD_COARSE_SWEEPS,
+                smooth_sweeps=1) -> bd_params:
     """Solver parameters; the defaults are the reference's production schedule (GPU_DEFAULTS,
     gpu/solver.ts:61-72: h 1/120, 2 Newton x 2 PCG, 2 coarse sweeps, coarsest <= 400 nodes)."""
     p = bd_params()
@@ -174,6 +176,7 @@ def make_params(h=1 / 120, gravity=(0.0, 0.0, -9.81), newton_iterations=2, newto
     p.precision = precision
     p.self_contact = int(bool(self_contact))
     p.coarse_solve = int(coarse_solve)
+    p.smooth_sweeps = int(smooth_sweeps)
     return p
 
 

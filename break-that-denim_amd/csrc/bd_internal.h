@@ -26,6 +26,7 @@ namespace bd {
 constexpr int kFineComps = 32;
 constexpr int kCoarseBlocks = 13;
 constexpr int kCoarseComps = 6 * kCoarseBlocks;
+constexpr int kLineColors = 3;  // line colours per family: lines of one colour share no coupling
 // grid of the deterministic reductions: one 1024-thread block per CU
 constexpr int kDotBlocks = 256;
 constexpr int kDotThreads = 1024;
@@ -61,6 +62,7 @@ struct Level {
   // line tables: per colour (6: v-lines 0..2, u-lines 0..2) a list of int4 (start, stride, len, family)
   int4* lines = nullptr;
   int color_start[7] = {0};
+  int ncol = 3;  // colours per family (v-lines 0 .. ncol-1, then u-lines): kLineColors
   int max_len = 0;
   // dual layout (lines longer than 128 nodes): column-major copies (node (i, j) of a layer at
   // off + i * ny + j) of the operator, fixed flags and iterate for the v-line passes; the v-line

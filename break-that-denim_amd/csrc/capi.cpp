@@ -127,19 +127,22 @@ std::vector<Geo> level_layouts(const Geo& fine, int max_coarsest) {
 }
 
 // lineTable (lattice.ts:86-106): v-lines (family 1) colours 0..2, then u-lines colours 0..2
-void build_lines(Solver& s, Level& L) {
-  std::vector<std::vector<int4>> by(6);
+// line tables (lattice.ts:86-106): per colour the lines of one family, v-lines first; `ncol`
+// colours per family (3, or 2 on the fine level where only neighbouring lines couple)
+void build_lines(Solver& s, Level& L, int ncol) {
+  L.ncol = ncol;
+  std::vector<std::vector<int4>> by(2 * ncol);
   int maxlen = 0;
   for (int l = 0; l < L.geo.nl; l++) {
     int nx = L.geo.nx[l], ny = L.geo.ny[l], off = L.geo.off[l];
     for (int i = 0; i < nx; i++)
-      by[i % 3].push_back(L.dual ? make_int4(off + i * ny, 1, ny, 1) : make_int4(off + i, nx, ny, 1));
-    for (int j = 0; j < ny; j++) by[3 + j % 3].push_back(make_int4(off + j * nx, 1, nx, 0));
+      by[i % ncol].push_back(L.dual ? make_int4(off + i * ny, 1, ny, 1) : make_int4(off + i, nx, ny, 1));
+    for (int j = 0; j < ny; j++) by[ncol + j % ncol].push_back(make_int4(off + j * nx, 1, nx, 0));
     maxlen = std::max(maxlen, std::max(nx, ny));
   }
   std::vector<int4> flat;
   L.color_start[0] = 0;
-  for (int c = 0; c < 6; c++) {
+  for (int c = 0; c < 2 * ncol; c++) {
     flat.insert(flat.end(), by[c].begin(), by[c].end());
     L.color_start[c + 1] = (int)flat.size();
   }
@@ -319,6 +322,10 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
     g_create_error = "bd_create: unknown coarse_solve";
     return BD_ERR_INVALID;
   }
+  if (prm->smooth_sweeps < 1 || prm->smooth_sweeps > 8) {
+    g_create_error = "bd_create: smooth_sweeps must be 1..8";
+    return BD_ERR_INVALID;
+  }
   if (sc->layer_count > BD_MAX_CONTACT_LAYERS) {
     g_create_error = "bd_create: cross-layer contact supports at most 3 layers";
     return BD_ERR_LIMIT;
@@ -471,7 +478,7 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
         L.xT = dalloc<float>(3 * (size_t)L.Np);
         BD_HIP(hipMemsetAsync(L.xT, 0, 3 * (size_t)L.Np * sizeof(float), s.stream));
       }
-      build_lines(s, L);
+      build_lines(s, L, kLineColors);
     }
     smooth_set_attributes();
     if (s.P.coarse_solve == BD_COARSE_EXACT) coarse_exact_alloc(s);

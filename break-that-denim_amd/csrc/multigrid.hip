@@ -469,12 +469,12 @@ void vcycle(Solver& s, int k) {
     for (int sw = 0; sw < s.P.coarse_sweeps; sw++) launch_smooth(s, k, sw & 1);
     return;
   }
-  launch_smooth(s, k, 0);
+  for (int sw = 0; sw < s.P.smooth_sweeps; sw++) launch_smooth(s, k, 0);
   launch_residual(s, k, L.x, L.b, L.r);
   launch_restrict(s, k);
   vcycle(s, k + 1);
   launch_prolong_add(s, k);
-  launch_smooth(s, k, 1);
+  for (int sw = 0; sw < s.P.smooth_sweeps; sw++) launch_smooth(s, k, 1);
 }
 
 }  // namespace bd

@@ -11,9 +11,11 @@
 // ~110 loads of a row issue at once, clamped indices with masks) because a colour pass has only a
 // few hundred lines in flight and is latency-bound. The first odd-even reduction level runs in
 // registers (the thread owns both rows of a pair; neighbours exchange one 9-float term via LDS), the
-// remaining log2(len) - 1 levels by block cyclic reduction over the kept rows in LDS (27 floats per
-// kept row: 55 KB for a 1024-node line, two workgroups per CU). Every pivot is shifted by
-// PIVOT_SHIFT of its largest diagonal entry, as in the reference.
+// remaining log2(len) - 1 levels by block parallel cyclic reduction, all reduced rows active at every
+// level (one barrier per level and no back-substitution: half the serial steps of odd-even cyclic
+// reduction, which is what bounds a pass). LDS: 36 floats per reduced row, 72 KB for a 1024-node
+// line, two workgroups per CU. Every pivot is shifted by PIVOT_SHIFT of its largest diagonal entry,
+// as in the reference.
 //
 // Levels with lines longer than 128 nodes also keep their operator, fixed flags and iterate in a
 // column-major copy (node (i, j) of a layer at off + i * ny + j): v-line passes read that copy, so
@@ -252,24 +254,30 @@ __device__ __forceinline__ void gather_task(const SmoothArgs& S, int n, int i, i
   pm[g * len + row] = maj;
 }
 
-// Block tridiagonal solve of one line by two-rows-per-thread odd-even cyclic reduction.
+// LDS floats per reduced row the line solve uses: two buffers of D (6), U (9), r (3)
+constexpr int kSolveFloats = 36;
+
+// Block tridiagonal solve of one line, two rows per thread.
 // Thread t holds rows a = 2t and b = 2t+1 (D sym, U full coupling to the next row, r) and
-// receives their solutions. cr: 27 * Lr floats of LDS. One barrier per reduction level: a kept
-// row recomputes its eliminated neighbours' shifted pivots itself.
+// receives their solutions. The first odd-even level eliminates row b in registers (neighbours
+// exchange one 9-float term through LDS), leaving R2 = ceil(len/2) reduced rows, one per thread.
+// Those are solved by block parallel cyclic reduction: at level d every row eliminates both its
+// neighbours at distance d at once, so after ceil(log2 R2) levels (one barrier each, the row kept
+// in registers, LDS ping-ponged) each row stands alone; there is no back-substitution sweep. The
+// reduced system stays symmetric (row k's coupling to k - d is row k - d's coupling to k,
+// transposed), so a row carries only its forward coupling. cr: kSolveFloats * Lr floats of LDS.
 __device__ __forceinline__ void line_solve(float* cr, int Lr, int t, int len, const float* Da_in, const float* Ua,
                                            const float* ra_in, const float* Db, const float* Ub, const float* rb,
                                            float xa[3], float xb[3]) {
-  float* sD = cr;           // 6 planes
-  float* sU = sD + 6 * Lr;  // 9 planes
-  float* sL = sU + 9 * Lr;  // 9 planes
-  float* sR = sL + 9 * Lr;  // 3 planes
+  float* B0 = cr;            // 18 planes: D 6, U 9, r 3
+  float* B1 = cr + 18 * Lr;  // 18 planes; the register level's exchange uses its first 9
   const int R2 = (len + 1) >> 1;
   const bool has_a = 2 * t < len, has_b = 2 * t + 1 < len;
-  float Da[6], ra[3];
+  float D[6], r[3], U[9];
 #pragma unroll
-  for (int q = 0; q < 6; q++) Da[q] = Da_in[q];
+  for (int q = 0; q < 6; q++) D[q] = Da_in[q];
 #pragma unroll
-  for (int q = 0; q < 3; q++) ra[q] = ra_in[q];
+  for (int q = 0; q < 3; q++) r[q] = ra_in[q];
 
   // register level: eliminate row b (U_a couples a -> b, U_b couples b -> 2t+2)
   float Dib[6], Dibf[9], yb[3];
@@ -282,182 +290,132 @@ __device__ __forceinline__ void line_solve(float* cr, int Lr, int t, int len, co
     mtm(Ub, T, P);
     mtv(Ub, yb, p);
     if (has_b && t + 1 < R2) {
-      sL[0 * Lr + t] = P[0];
-      sL[1 * Lr + t] = 0.5f * (P[1] + P[3]);
-      sL[2 * Lr + t] = 0.5f * (P[2] + P[6]);
-      sL[3 * Lr + t] = P[4];
-      sL[4 * Lr + t] = 0.5f * (P[5] + P[7]);
-      sL[5 * Lr + t] = P[8];
-      sL[6 * Lr + t] = p[0];
-      sL[7 * Lr + t] = p[1];
-      sL[8 * Lr + t] = p[2];
+      B1[0 * Lr + t] = P[0];
+      B1[1 * Lr + t] = 0.5f * (P[1] + P[3]);
+      B1[2 * Lr + t] = 0.5f * (P[2] + P[6]);
+      B1[3 * Lr + t] = P[4];
+      B1[4 * Lr + t] = 0.5f * (P[5] + P[7]);
+      B1[5 * Lr + t] = P[8];
+      B1[6 * Lr + t] = p[0];
+      B1[7 * Lr + t] = p[1];
+      B1[8 * Lr + t] = p[2];
     }
   }
-  float Unew[9];
   {
     float T2[9], W[9], t3[3];
     mm(Ua, Dibf, T2);
     mmt(T2, Ua, W);
-    Da[0] -= W[0];
-    Da[1] -= 0.5f * (W[1] + W[3]);
-    Da[2] -= 0.5f * (W[2] + W[6]);
-    Da[3] -= W[4];
-    Da[4] -= 0.5f * (W[5] + W[7]);
-    Da[5] -= W[8];
+    D[0] -= W[0];
+    D[1] -= 0.5f * (W[1] + W[3]);
+    D[2] -= 0.5f * (W[2] + W[6]);
+    D[3] -= W[4];
+    D[4] -= 0.5f * (W[5] + W[7]);
+    D[5] -= W[8];
     mv(T2, rb, t3);
-    ra[0] -= t3[0];
-    ra[1] -= t3[1];
-    ra[2] -= t3[2];
+    r[0] -= t3[0];
+    r[1] -= t3[1];
+    r[2] -= t3[2];
     mm(T2, Ub, W);
 #pragma unroll
-    for (int q = 0; q < 9; q++) Unew[q] = -W[q];
+    for (int q = 0; q < 9; q++) U[q] = -W[q];
   }
   __syncthreads();
   if (has_a && t >= 1) {
 #pragma unroll
-    for (int q = 0; q < 6; q++) Da[q] -= sL[q * Lr + t - 1];
+    for (int q = 0; q < 6; q++) D[q] -= B1[q * Lr + t - 1];
 #pragma unroll
-    for (int q = 0; q < 3; q++) ra[q] -= sL[(6 + q) * Lr + t - 1];
+    for (int q = 0; q < 3; q++) r[q] -= B1[(6 + q) * Lr + t - 1];
   }
-  __syncthreads();
-  if (t < R2) {
+  // B1's exchange terms are read before the first level's barrier; that level writes B0 and the
+  // next one B1, so one barrier per level orders every read before the overwrite.
+  const bool row = t < R2;
+  int lvl = 0;
+  for (int d = 1; d < R2; d <<= 1, lvl ^= 1) {
+    float* B = lvl ? B1 : B0;
+    if (row) {
 #pragma unroll
-    for (int q = 0; q < 6; q++) sD[q * Lr + t] = Da[q];
+      for (int q = 0; q < 6; q++) B[q * Lr + t] = D[q];
 #pragma unroll
-    for (int q = 0; q < 9; q++) sU[q * Lr + t] = Unew[q];
+      for (int q = 0; q < 9; q++) B[(6 + q) * Lr + t] = U[q];
 #pragma unroll
-    for (int q = 0; q < 3; q++) sR[q * Lr + t] = ra[q];
-  }
-  __syncthreads();
-
-  const int k = t;
-  const bool row = k < R2;
-  int top = 1;
-  while (2 * top < R2) top <<= 1;
-  for (int d = 1; d < R2; d <<= 1) {
-    if (row && (k & (2 * d - 1)) == 0) {
-      float D[9], Ds[6], R[3], Un[9];
-#pragma unroll
-      for (int q = 0; q < 6; q++) Ds[q] = sD[q * Lr + k];
-      sym_full(Ds, D);
-#pragma unroll
-      for (int q = 0; q < 3; q++) R[q] = sR[q * Lr + k];
+      for (int q = 0; q < 3; q++) B[(15 + q) * Lr + t] = r[q];
+    }
+    __syncthreads();
+    if (row) {
+      float Dn[9], Un[9];
+      sym_full(D, Dn);
 #pragma unroll
       for (int q = 0; q < 9; q++) Un[q] = 0.f;
-      const int j1 = k - d, j2 = k + d;
+      const int j1 = t - d, j2 = t + d;
       if (j1 >= 0) {
-        float Ds1[6], Di[6], Dif[9], U1[9], T[9], W[9], r1[3], y1[3], tt[3];
+        // row j1 couples to this row by U1: eliminate it
+        float D1[6], Di[6], Dif[9], U1[9], T[9], W[9], r1[3], y1[3], tt[3];
 #pragma unroll
-        for (int q = 0; q < 6; q++) Ds1[q] = sD[q * Lr + j1];
-        pivot_inv(Ds1, Di);
+        for (int q = 0; q < 6; q++) D1[q] = B[q * Lr + j1];
+#pragma unroll
+        for (int q = 0; q < 9; q++) U1[q] = B[(6 + q) * Lr + j1];
+#pragma unroll
+        for (int q = 0; q < 3; q++) r1[q] = B[(15 + q) * Lr + j1];
+        pivot_inv(D1, Di);
         sym_full(Di, Dif);
-#pragma unroll
-        for (int q = 0; q < 9; q++) U1[q] = sU[q * Lr + j1];
-#pragma unroll
-        for (int q = 0; q < 3; q++) r1[q] = sR[q * Lr + j1];
         mm(Dif, U1, T);
         mtm(U1, T, W);
 #pragma unroll
-        for (int q = 0; q < 9; q++) D[q] -= W[q];
+        for (int q = 0; q < 9; q++) Dn[q] -= W[q];
         mv(Dif, r1, y1);
         mtv(U1, y1, tt);
 #pragma unroll
-        for (int q = 0; q < 3; q++) R[q] -= tt[q];
+        for (int q = 0; q < 3; q++) r[q] -= tt[q];
       }
       if (j2 < R2) {
-        float Ds2[6], Di[6], Dif[9], Uk[9], U2[9], T2[9], W[9], r2[3], tt[3];
+        // this row couples to row j2 by U; row j2 couples on to j2 + d by U2
+        float D2[6], Di[6], Dif[9], U2[9], T2[9], W[9], r2[3], tt[3];
 #pragma unroll
-        for (int q = 0; q < 6; q++) Ds2[q] = sD[q * Lr + j2];
-        pivot_inv(Ds2, Di);
+        for (int q = 0; q < 6; q++) D2[q] = B[q * Lr + j2];
+#pragma unroll
+        for (int q = 0; q < 9; q++) U2[q] = B[(6 + q) * Lr + j2];
+#pragma unroll
+        for (int q = 0; q < 3; q++) r2[q] = B[(15 + q) * Lr + j2];
+        pivot_inv(D2, Di);
         sym_full(Di, Dif);
+        mm(U, Dif, T2);
+        mmt(T2, U, W);
 #pragma unroll
-        for (int q = 0; q < 9; q++) Uk[q] = sU[q * Lr + k];
-#pragma unroll
-        for (int q = 0; q < 9; q++) U2[q] = sU[q * Lr + j2];
-#pragma unroll
-        for (int q = 0; q < 3; q++) r2[q] = sR[q * Lr + j2];
-        // the eliminated row's coupling to this row, kept for its back-substitution
-        sL[0 * Lr + j2] = Uk[0]; sL[1 * Lr + j2] = Uk[3]; sL[2 * Lr + j2] = Uk[6];
-        sL[3 * Lr + j2] = Uk[1]; sL[4 * Lr + j2] = Uk[4]; sL[5 * Lr + j2] = Uk[7];
-        sL[6 * Lr + j2] = Uk[2]; sL[7 * Lr + j2] = Uk[5]; sL[8 * Lr + j2] = Uk[8];
-        mm(Uk, Dif, T2);
-        mmt(T2, Uk, W);
-#pragma unroll
-        for (int q = 0; q < 9; q++) D[q] -= W[q];
+        for (int q = 0; q < 9; q++) Dn[q] -= W[q];
         mv(T2, r2, tt);
 #pragma unroll
-        for (int q = 0; q < 3; q++) R[q] -= tt[q];
+        for (int q = 0; q < 3; q++) r[q] -= tt[q];
         mm(T2, U2, W);
 #pragma unroll
         for (int q = 0; q < 9; q++) Un[q] = -W[q];
       }
-      sD[0 * Lr + k] = D[0];
-      sD[1 * Lr + k] = 0.5f * (D[1] + D[3]);
-      sD[2 * Lr + k] = 0.5f * (D[2] + D[6]);
-      sD[3 * Lr + k] = D[4];
-      sD[4 * Lr + k] = 0.5f * (D[5] + D[7]);
-      sD[5 * Lr + k] = D[8];
+      D[0] = Dn[0];
+      D[1] = 0.5f * (Dn[1] + Dn[3]);
+      D[2] = 0.5f * (Dn[2] + Dn[6]);
+      D[3] = Dn[4];
+      D[4] = 0.5f * (Dn[5] + Dn[7]);
+      D[5] = Dn[8];
 #pragma unroll
-      for (int q = 0; q < 3; q++) sR[q * Lr + k] = R[q];
-#pragma unroll
-      for (int q = 0; q < 9; q++) sU[q * Lr + k] = Un[q];
+      for (int q = 0; q < 9; q++) U[q] = Un[q];
     }
-    __syncthreads();
   }
-  if (k == 0) {
-    float s[6], inv[6], R[3], x0[3];
-#pragma unroll
-    for (int q = 0; q < 6; q++) s[q] = sD[q * Lr];
-    pivot_inv(s, inv);
-#pragma unroll
-    for (int q = 0; q < 3; q++) R[q] = sR[q * Lr];
-    smv(inv, R, x0);
-#pragma unroll
-    for (int q = 0; q < 3; q++) sR[q * Lr] = x0[q];
+  // every reduced row now stands alone
+  float* B = lvl ? B1 : B0;
+  xa[0] = xa[1] = xa[2] = 0.f;
+  if (row) {
+    float inv[6];
+    pivot_inv(D, inv);
+    smv(inv, r, xa);
+    B[t] = xa[0];
+    B[Lr + t] = xa[1];
+    B[2 * Lr + t] = xa[2];
   }
   __syncthreads();
-  for (int d = top; d >= 1; d >>= 1) {
-    if (row && (k & (2 * d - 1)) == d) {
-      float rr[3], xl[3], tt[3], Lk[9], Uk[9], Ds[6], Di[6], z[3];
-#pragma unroll
-      for (int q = 0; q < 3; q++) rr[q] = sR[q * Lr + k];
-#pragma unroll
-      for (int q = 0; q < 3; q++) xl[q] = sR[q * Lr + k - d];
-#pragma unroll
-      for (int q = 0; q < 9; q++) Lk[q] = sL[q * Lr + k];
-      mv(Lk, xl, tt);
-#pragma unroll
-      for (int q = 0; q < 3; q++) rr[q] -= tt[q];
-      if (k + d < R2) {
-        float xr[3];
-#pragma unroll
-        for (int q = 0; q < 3; q++) xr[q] = sR[q * Lr + k + d];
-#pragma unroll
-        for (int q = 0; q < 9; q++) Uk[q] = sU[q * Lr + k];
-        mv(Uk, xr, tt);
-#pragma unroll
-        for (int q = 0; q < 3; q++) rr[q] -= tt[q];
-      }
-#pragma unroll
-      for (int q = 0; q < 6; q++) Ds[q] = sD[q * Lr + k];
-      pivot_inv(Ds, Di);
-      smv(Di, rr, z);
-#pragma unroll
-      for (int q = 0; q < 3; q++) sR[q * Lr + k] = z[q];
-    }
-    __syncthreads();
-  }
   float xn[3] = {0.f, 0.f, 0.f};
-  xa[0] = xa[1] = xa[2] = 0.f;
-  if (t < R2) {
-    xa[0] = sR[t];
-    xa[1] = sR[Lr + t];
-    xa[2] = sR[2 * Lr + t];
-  }
   if (t + 1 < R2) {
-    xn[0] = sR[t + 1];
-    xn[1] = sR[Lr + t + 1];
-    xn[2] = sR[2 * Lr + t + 1];
+    xn[0] = B[t + 1];
+    xn[1] = B[Lr + t + 1];
+    xn[2] = B[2 * Lr + t + 1];
   }
   float tt[3], q1[3];
   mtv(Ua, xa, tt);
@@ -559,8 +517,8 @@ __global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S) {
   const int len = L.ln.z;
   const int R2 = (len + 1) >> 1;
   const int Lr = ((R2 + 63) / 64) * 64;
-  float* cr = sm;              // 27 * Lr: reduction workspace
-  float* gD = cr + 27 * Lr;    // 6 * len: centre blocks, then D
+  float* cr = sm;              // kSolveFloats * Lr: line-solve workspace
+  float* gD = cr + kSolveFloats * Lr;  // 6 * len: centre blocks, then D
   float* gU = gD + 6 * len;    // 9 * len
   float* gR = gU + 9 * len;    // 3 * len: b, then r
   float* gX = gR + 3 * len;    // 3 * len: own x
@@ -655,7 +613,7 @@ template <bool FINE, int FAM, int G>
 size_t staged_shm(int len) {
   int R2 = (len + 1) / 2;
   int Lr = ((R2 + 63) / 64) * 64;
-  return sizeof(float) * ((size_t)27 * Lr + (size_t)(6 + 9 + 3 + 3 + 4 * G) * len) + (size_t)len + 16;
+  return sizeof(float) * ((size_t)kSolveFloats * Lr + (size_t)(6 + 9 + 3 + 3 + 4 * G) * len) + (size_t)len + 16;
 }
 
 // LDS a workgroup may use (the staged kernels' attribute is raised to it for the autotuner)
@@ -689,7 +647,7 @@ void launch_one(const SmoothArgs& S, int count, int len, hipStream_t st, int var
   }
   int R2 = (len + 1) / 2;
   int block = ((R2 + 63) / 64) * 64;
-  size_t shm = (size_t)27 * block * sizeof(float);
+  size_t shm = (size_t)kSolveFloats * block * sizeof(float);
   k_smooth_direct<FINE, FAM, DUAL><<<count, block, shm, st>>>(S);
 }
 
@@ -720,7 +678,7 @@ void set_attrs(int shm, int shm_staged) {
 }  // namespace
 
 void smooth_set_attributes() {
-  int shm = 27 * ((BD_MAX_LINE / 2 + 63) / 64 * 64) * (int)sizeof(float);
+  int shm = kSolveFloats * ((BD_MAX_LINE / 2 + 63) / 64 * 64) * (int)sizeof(float);
   set_attrs<false>(shm, kMaxLds);
   set_attrs<true>(shm, kMaxLds);
 }
@@ -732,7 +690,7 @@ void launch_colour(Solver& s, int k, int c, int variant) {
   Level& L = s.lv[k];
   int count = L.color_start[c + 1] - L.color_start[c];
   if (count == 0) return;
-  const int fam = c < 3 ? 1 : 0;
+  const int fam = c < L.ncol ? 1 : 0;
   int maxlen = 0;
   for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
   const bool colmajor = L.dual && fam == 1;
@@ -779,13 +737,14 @@ std::vector<int> smooth_variants(Solver& s, int k, int fam) {
   return variants_for(L.fine, maxlen);
 }
 
-// Smooth all six colours of level k (v-lines colours 0..2 then u-lines 0..2; reversed for the
+// Smooth all colours of level k (v-line colours first, then u-line colours; reversed for the
 // post-smoothing sweep so the V-cycle stays symmetric).
 void launch_smooth(Solver& s, int k, int reverse) {
   Level& L = s.lv[k];
-  for (int q = 0; q < 6; q++) {
-    int c = reverse ? 5 - q : q;
-    launch_colour(s, k, c, L.variant[c < 3 ? 1 : 0]);
+  const int nc = 2 * L.ncol;
+  for (int q = 0; q < nc; q++) {
+    int c = reverse ? nc - 1 - q : q;
+    launch_colour(s, k, c, L.variant[c < L.ncol ? 1 : 0]);
   }
 }
 

@@ -138,6 +138,7 @@ typedef struct bd_params {
   int32_t precision;           /* BD_PRECISION_* */
   int32_t self_contact;        /* node-face self-contact (GpuSolverParams.selfContact, gpu/solver.ts:49-50) */
   int32_t coarse_solve;        /* BD_COARSE_*: how the V-cycle solves its coarsest level */
+  int32_t smooth_sweeps;       /* pre- and post-smoothing sweeps per level of the V-cycle (the reference: 1) */
 } bd_params;
 
 #define BD_COARSE_SWEEPS 0     /* coarse_sweeps colour-ordered line sweeps (the reference's V-cycle) */

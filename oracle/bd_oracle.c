@@ -1571,7 +1571,7 @@ static void vcycle(or_solver* S, int k, double* x, const double* b) {
     return;
   }
   const prol_t* P = &S->pr[k];
-  smooth_lines(A, x, b, 0, colors);
+  for (int sw = 0; sw < S->P.smooth_sweeps; sw++) smooth_lines(A, x, b, 0, colors);
   double* Ax = malloc(sizeof(double) * 3 * A->N);
   op_apply(A, x, Ax);
   const op_t* C = &S->lv[k + 1];
@@ -1598,7 +1598,7 @@ static void vcycle(or_solver* S, int k, double* x, const double* b) {
       for (int q = 0; q < 3; q++) x[3 * f + q] += w * xc[3 * c + q];
     }
   }
-  smooth_lines(A, x, b, 1, colors);
+  for (int sw = 0; sw < S->P.smooth_sweeps; sw++) smooth_lines(A, x, b, 1, colors);
   free(Ax);
   free(rc);
   free(xc);
@@ -1833,7 +1833,9 @@ int* or_level_pidx(or_solver* S, int k) { return S->pr[k].idx; }
 double* or_level_pw(or_solver* S, int k) { return S->pr[k].w; }
 
 void or_apply(or_solver* S, int k, const double* x, double* y) { op_apply(&S->lv[k], x, y); }
-void or_smooth(or_solver* S, int k, double* x, const double* b, int reverse) { smooth_lines(&S->lv[k], x, b, reverse, 3); }
+void or_smooth(or_solver* S, int k, double* x, const double* b, int reverse) {
+  smooth_lines(&S->lv[k], x, b, reverse, 3);
+}
 void or_vcycle(or_solver* S, double* x, const double* b) { vcycle(S, 0, x, b); }
 void or_pcg(or_solver* S, const double* b, double* x, int iterations, double rel_tol, int max_iter) { pcg(S, b, x, iterations, rel_tol, max_iter); }
 int or_pcg_iterations(or_solver* S) { return S->pcg_iters; }

@@ -117,3 +117,41 @@ def test_autotuned_smoothers_keep_parity():
     e = rel_to_scale(gpu.read(BD_BUF_DELTA), cpu.pcg(b, params.pcg_iterations))
     say(f"PCG (autotuned) maxAbs/scale = {e:.2e}")
     assert e < 2e-2
+
+
+@pytest.mark.parametrize("sweeps", [2, 3])
+def test_repeated_sweeps_keep_parity(sweeps):
+    """Repeated pre- and post-smoothing sweeps per level: the V-cycle and PCG meet the same bars
+    against the oracle run with the same schedule."""
+    from bd.solver import GpuSolver
+    n = 136
+    pitch = 1e-3
+    mat = linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / 2.4)
+    S = sheet_scene(n, pitch, mat)
+    params = make_params(history=False, smooth_sweeps=sweeps)
+    cpu = O.Oracle(S, params)
+    for _ in range(4):
+        cpu.substep()
+    xs = cpu.x.astype(np.float32).astype(np.float64)
+    vs = cpu.v.astype(np.float32).astype(np.float64)
+    cpu.x[:] = xs
+    cpu.v[:] = vs
+    say(f"gpu: 136x136 sheet, {sweeps} sweeps")
+    gpu = GpuSolver(S, params)
+    gpu.set_state(xs, vs)
+    cpu.predict()
+    cpu.assemble()
+    gpu.encode_predict()
+    gpu.encode_assemble(True)
+    cpu.load_fine_operator(gpu.stencil(0), gpu.fixed(0))
+    b = gpu.read(BD_BUF_RHS)
+    gpu.write(BD_BUF_LEVEL_B, b, 0)
+    gpu.encode_vcycle()
+    e = rel_to_scale(gpu.read(BD_BUF_LEVEL_X, 0), cpu.vcycle(b))
+    say(f"  V-cycle maxAbs/scale = {e:.2e}")
+    assert e < 5e-3
+    gpu.write(BD_BUF_RHS, b)
+    gpu.encode_pcg()
+    e = rel_to_scale(gpu.read(BD_BUF_DELTA), cpu.pcg(b, params.pcg_iterations))
+    say(f"  PCG maxAbs/scale = {e:.2e}")
+    assert e < 2e-2
