@@ -147,15 +147,20 @@ __device__ __forceinline__ void gather_row(const SmoothArgs& S, int n, int nb, i
       for (int q = 0; q < 6; q++) blk[q] = op[(base + q) * Np + src];
     }
     const float xm[3] = {x[m], x[Np + m], x[2 * Np + m]};
-    const bool use = in && S.fixed[m] == 0;
+    // the fixed-neighbour mask is arithmetic: a branch on the loaded flag would split the gather
+    // into one memory round trip per coupling
+    const uint8_t fm = S.fixed[m];  // m is clamped in bounds: the load is unconditional
+    const float w = (in & (fm == 0)) ? 1.f : 0.f;
     const int along = FAM == 0 ? di : dj;
     const int across = FAM == 0 ? dj : di;
     if (across == 0 && along == 1) {
-      if (use) sym_full(blk, U);
+      float full[9];
+      sym_full(blk, full);
+#pragma unroll
+      for (int q = 0; q < 9; q++) U[q] = w * full[q];
     } else if (!(across == 0 && along == -1)) {
       float t[3];
       smv(blk, xm, t);
-      const float w = use ? 1.f : 0.f;
       r[0] -= w * t[0];
       r[1] -= w * t[1];
       r[2] -= w * t[2];
@@ -192,16 +197,22 @@ __device__ __forceinline__ void gather_row(const SmoothArgs& S, int n, int nb, i
   r[2] += maj * xn2;
 }
 
-// Partial row gather of one (row, g) task: the couplings c with c % G == g. Writes the partial
-// rhs correction and majorisation to LDS; the task owning the +1 along-line coupling writes U.
-template <bool FINE, int FAM, int G, bool DUAL>
-__device__ __forceinline__ void gather_task(const SmoothArgs& S, int n, int i, int j, int nx, int ny, int g, float* pr, float* pm,
-                                            float* gU, int row, int len) {
+// Partial row gather of rows on one wave's 64 lanes for the couplings c with c % G == GI (GI is a
+// compile-time constant, so the part is straight-line code whose loads all issue together). GI 0
+// also loads the row's centre block, b, x and fixed flag. Writes the partial rhs correction and
+// majorisation to LDS; the part owning the +1 along-line coupling writes U. Lanes past the line's
+// end (valid false) gather a clamped row and write nothing.
+template <bool FINE, int FAM, int G, int GI, bool DUAL>
+__device__ __forceinline__ void gather_part(const SmoothArgs& S, int n, int nb, int i, int j, int nx, int ny, bool valid,
+                                            int row, int len, float* pr, float* pm, float* gU, float* gD, float* gR,
+                                            float* gX, uint8_t* gF) {
   const int Np = S.Np;
   const float* op = S.op;
   const float* x = S.x;
   const int si = stride_i<FAM, DUAL>(nx, ny), sj = stride_j<FAM, DUAL>(nx, ny);
   float r0 = 0.f, r1 = 0.f, r2 = 0.f, maj = 0.f;
+  float U[9];
+  bool ownsU = false;
   auto one = [&](int di, int dj, int sdi, int sdj, int base, bool scalar) {
     const bool in = i + di >= 0 && i + di < nx && j + dj >= 0 && j + dj < ny;
     const int m = in ? n + dj * sj + di * si : n;
@@ -215,18 +226,19 @@ __device__ __forceinline__ void gather_task(const SmoothArgs& S, int n, int i, i
       for (int q = 0; q < 6; q++) blk[q] = op[(base + q) * Np + src];
     }
     const float xm[3] = {x[m], x[Np + m], x[2 * Np + m]};
-    const bool use = in && S.fixed[m] == 0;
+    const uint8_t fm = S.fixed[m];  // m is clamped in bounds: the load is unconditional
+    const float w = (in & (fm == 0)) ? 1.f : 0.f;
     const int along = FAM == 0 ? di : dj;
     const int across = FAM == 0 ? dj : di;
     if (across == 0 && along == 1) {
       float full[9];
       sym_full(blk, full);
 #pragma unroll
-      for (int q = 0; q < 9; q++) gU[q * len + row] = use ? full[q] : 0.f;
+      for (int q = 0; q < 9; q++) U[q] = w * full[q];
+      ownsU = true;
     } else if (!(across == 0 && along == -1)) {
       float t[3];
       smv(blk, xm, t);
-      const float w = use ? 1.f : 0.f;
       r0 += w * t[0];
       r1 += w * t[1];
       r2 += w * t[2];
@@ -236,11 +248,11 @@ __device__ __forceinline__ void gather_task(const SmoothArgs& S, int n, int i, i
   if (FINE) {
 #pragma unroll
     for (int k = 0; k < 12; k++)
-      if (k % G == g) one(kFine[k].di, kFine[k].dj, kFine[k].sdi, kFine[k].sdj, kFine[k].base, kFine[k].scalar);
+      if (k % G == GI) one(kFine[k].di, kFine[k].dj, kFine[k].sdi, kFine[k].sdj, kFine[k].base, kFine[k].scalar);
   } else {
 #pragma unroll
     for (int k = 0; k < 24; k++) {
-      if (k % G != g) continue;
+      if (k % G != GI) continue;
       const int b = 1 + (k >> 1);
       if (k & 1)
         one(-coarse_di(b), -coarse_dj(b), -coarse_di(b), -coarse_dj(b), 6 * b, false);
@@ -248,10 +260,50 @@ __device__ __forceinline__ void gather_task(const SmoothArgs& S, int n, int i, i
         one(coarse_di(b), coarse_dj(b), 0, 0, 6 * b, false);
     }
   }
-  pr[(0 * G + g) * len + row] = r0;
-  pr[(1 * G + g) * len + row] = r1;
-  pr[(2 * G + g) * len + row] = r2;
-  pm[g * len + row] = maj;
+  float c[6], bb[3], xx[3];
+  uint8_t f = 0;
+  if (GI == 0) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) c[q] = op[q * Np + n];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      bb[q] = S.b[q * Np + nb];
+      xx[q] = x[q * Np + n];
+    }
+    f = S.fixed[n];
+  }
+  if (!valid) return;
+  pr[(0 * G + GI) * len + row] = r0;
+  pr[(1 * G + GI) * len + row] = r1;
+  pr[(2 * G + GI) * len + row] = r2;
+  pm[GI * len + row] = maj;
+  if (ownsU) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) gU[q * len + row] = U[q];
+  }
+  if (GI == 0) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) gD[q * len + row] = c[q];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      gR[q * len + row] = bb[q];
+      gX[q * len + row] = xx[q];
+    }
+    gF[row] = f;
+  }
+}
+
+// Runs part g (wave-uniform) of the gather.
+template <bool FINE, int FAM, int G, bool DUAL, int GI = 0>
+__device__ __forceinline__ void gather_dispatch(int g, const SmoothArgs& S, int n, int nb, int i, int j, int nx, int ny,
+                                                bool valid, int row, int len, float* pr, float* pm, float* gU, float* gD,
+                                                float* gR, float* gX, uint8_t* gF) {
+  if constexpr (GI < G) {
+    if (g == GI)
+      gather_part<FINE, FAM, G, GI, DUAL>(S, n, nb, i, j, nx, ny, valid, row, len, pr, pm, gU, gD, gR, gX, gF);
+    else
+      gather_dispatch<FINE, FAM, G, DUAL, GI + 1>(g, S, n, nb, i, j, nx, ny, valid, row, len, pr, pm, gU, gD, gR, gX, gF);
+  }
 }
 
 // LDS floats per reduced row the line solve uses: two buffers of D (6), U (9), r (3)
@@ -526,22 +578,17 @@ __global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S) {
   float* pm = pr + 3 * G * len;  // G * len
   uint8_t* gF = reinterpret_cast<uint8_t*>(pm + G * len);
   const int T = blockDim.x;
-  const int Np = S.Np;
-  for (int task = threadIdx.x; task < len * G; task += T) {
-    const int row = task / G, g = task - row * G;
+  // work items: (64 consecutive rows, coupling part g), one wave each, so that g is wave-uniform
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int items = ((len + 63) >> 6) * G;
+  for (int item = wave; item < items; item += T >> 6) {
+    const int g = item % G, row0 = (item / G) * 64 + lane;
+    const bool valid = row0 < len;
+    const int row = valid ? row0 : len - 1;
     int n, nb, no, i, j;
     row_index<FAM, DUAL>(L, row, n, nb, no, i, j);
-    gather_task<FINE, FAM, G, DUAL>(S, n, i, j, L.nx, L.ny, g, pr, pm, gU, row, len);
-    if (g == 0) {
-#pragma unroll
-      for (int q = 0; q < 6; q++) gD[q * len + row] = S.op[q * Np + n];
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        gR[q * len + row] = S.b[q * Np + nb];
-        gX[q * len + row] = S.x[q * Np + n];
-      }
-      gF[row] = S.fixed[n];
-    }
+    gather_dispatch<FINE, FAM, G, DUAL>(g, S, n, nb, i, j, L.nx, L.ny, valid, row, len, pr, pm, gU, gD, gR, gX, gF);
   }
   __syncthreads();
   for (int row = threadIdx.x; row < len; row += T) {
