@@ -22,6 +22,11 @@ BD_MAX_GRIPS = 16
 BD_PRECISION_FP32 = 0
 BD_COARSE_SWEEPS = 0
 BD_COARSE_EXACT = 1
+BD_SMOOTHER_DEFAULT = -1
+BD_SMOOTHER_DIRECT = 0
+BD_SMOOTHER_STAGED4 = 4
+BD_SMOOTHER_STAGED8 = 8
+BD_SMOOTHER_SPLIT = 16
 
 (BD_BUF_X, BD_BUF_V, BD_BUF_DX, BD_BUF_PRED, BD_BUF_DELTA, BD_BUF_RHS, BD_BUF_LEVEL_X, BD_BUF_LEVEL_B,
  BD_BUF_LEVEL_R, BD_BUF_STENCIL, BD_BUF_FIXED, BD_BUF_PROLONG, BD_BUF_REST, BD_BUF_INTACT, BD_BUF_CUTMASK,
@@ -276,6 +281,7 @@ def load_library(path: os.PathLike | None = None):
                                 f64p, f64p, P(bd_pick)]
     lib.bd_pick_near.argtypes = [vp, f64p, C.c_double, C.c_int, P(bd_pick)]
     lib.bd_autotune.argtypes = [vp, C.c_int, i32p]
+    lib.bd_set_smoother.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     lib.bd_set_pieces.argtypes = [vp, u32p, u32p, C.c_int, u32p]
     lib.bd_update_topology.argtypes = [vp, i32p, C.c_int, i32p]
     lib.bd_topology_search.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, u8p, u8p, i32p, C.c_int, u8p, u32p, u32p,
@@ -295,5 +301,5 @@ EXPORTED = (
     "bd_contact_detect", "bd_contact_terms", "bd_project_start", "bd_set_floor",
     "bd_set_pieces", "bd_update_topology", "bd_topology_search",
     "bd_grip_attach", "bd_grip_lead", "bd_grip_terms", "bd_grip_target", "bd_grip_pinch", "bd_grip_hold",
-    "bd_grip_release", "bd_pick_ray", "bd_pick_near", "bd_autotune",
+    "bd_grip_release", "bd_pick_ray", "bd_pick_near", "bd_autotune", "bd_set_smoother",
 )

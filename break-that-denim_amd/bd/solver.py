@@ -218,6 +218,10 @@ class GpuSolver:
         self._check(self.lib.bd_autotune(self.h, reps, out.ctypes.data_as(C.POINTER(C.c_int32))), "bd_autotune")
         return {k: (int(out[2 * k]), int(out[2 * k + 1])) for k in range(nlev)}
 
+    def set_smoother(self, level, family, variant):
+        """Pin the smoother kernel variant of one level's u-line (0) or v-line (1) passes."""
+        self._check(self.lib.bd_set_smoother(self.h, level, family, variant), "bd_set_smoother")
+
     def sync(self):
         self._check(self.lib.bd_sync(self.h), "bd_sync")
 

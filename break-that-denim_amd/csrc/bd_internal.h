@@ -67,13 +67,21 @@ struct Level {
   // dual layout (lines longer than 128 nodes): column-major copies (node (i, j) of a layer at
   // off + i * ny + j) of the operator, fixed flags and iterate for the v-line passes; the v-line
   // entries of `lines` then start in that layout with stride 1
-  // smoother kernel per family (u, v): -1 the default rule, 0 direct, 4 / 8 staged (bd_autotune)
-  int variant[2] = {-1, -1};
   bool dual = false;
   float* opT = nullptr;
   uint8_t* fixedT = nullptr;
   float* xT = nullptr;
+  // smoother kernel per family (u, v): -1 the default rule, 0 direct, 4 / 8 staged, 16 split
+  // (bd_autotune)
+  int variant[2] = {-1, -1};
+  // split variant (levels with lines longer than kStagedMaxLen): the assembled rows of one colour
+  // pass, 18 planes (D 6, U 9, r 3) of rows_cap = lines per colour x longest line
+  float* rows = nullptr;
+  size_t rows_cap = 0;
 };
+
+// longest line the staged smoother kernel takes by default; longer lines use the direct kernel
+constexpr int kStagedMaxLen = 257;
 
 constexpr int kDualMinLine = 129;
 
@@ -283,6 +291,7 @@ void launch_restrict(Solver& s, int k);
 void launch_prolong_add(Solver& s, int k);
 void launch_smooth(Solver& s, int k, int reverse);
 std::vector<int> smooth_variants(Solver& s, int k, int fam);
+void smooth_alloc_rows(Solver& s, Level& L);
 void launch_zero(Solver& s, float* p, size_t n);
 // column-major copies of a dual-layout level: operator and fixed flags, or the iterate
 void launch_colmajor_op(Solver& s, int k);

@@ -172,6 +172,7 @@ void destroy_all(Solver& s) {
     dfree(L.opT);
     dfree(L.fixedT);
     dfree(L.xT);
+    dfree(L.rows);
   }
   if (s.scal_h) (void)hipHostFree(s.scal_h);
   if (s.grip_goal_h) (void)hipHostFree(s.grip_goal_h);
@@ -479,6 +480,7 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
         BD_HIP(hipMemsetAsync(L.xT, 0, 3 * (size_t)L.Np * sizeof(float), s.stream));
       }
       build_lines(s, L, kLineColors);
+      if (L.max_len > kStagedMaxLen) smooth_alloc_rows(s, L);
     }
     smooth_set_attributes();
     if (s.P.coarse_solve == BD_COARSE_EXACT) coarse_exact_alloc(s);
@@ -880,6 +882,21 @@ int bd_pick_near(bd_solver* h, const double* p, double radius, int window_radius
     for (int q = 0; q < 3; q++) P.eye[q] = (float)p[q];
     launch_pick(s, false);
     fill_pick(s, window_radius, out);
+    return BD_OK;
+  });
+}
+
+int bd_set_smoother(bd_solver* h, int level, int family, int variant) {
+  return guarded(h, [&](Solver& s) {
+    if (!s.hierarchy) return fail(h, BD_ERR_STATE, "bd_set_smoother: assemble first");
+    if (level < 0 || level >= s.nlev) return fail(h, BD_ERR_INVALID, "bd_set_smoother: no such level");
+    if (family != 0 && family != 1) return fail(h, BD_ERR_INVALID, "bd_set_smoother: family must be 0 (u) or 1 (v)");
+    if (variant != BD_SMOOTHER_DEFAULT) {
+      std::vector<int> vs = smooth_variants(s, level, family);
+      if (std::find(vs.begin(), vs.end(), variant) == vs.end())
+        return fail(h, BD_ERR_INVALID, "bd_set_smoother: the level cannot run that variant");
+    }
+    s.lv[level].variant[family] = variant;
     return BD_OK;
   });
 }

@@ -13,8 +13,8 @@
 // registers (the thread owns both rows of a pair; neighbours exchange one 9-float term via LDS), the
 // remaining log2(len) - 1 levels by block parallel cyclic reduction, all reduced rows active at every
 // level (one barrier per level and no back-substitution: half the serial steps of odd-even cyclic
-// reduction, which is what bounds a pass). LDS: 36 floats per reduced row, 72 KB for a 1024-node
-// line, two workgroups per CU. Every pivot is shifted by PIVOT_SHIFT of its largest diagonal entry,
+// reduction, which is what bounds a pass). LDS: two 20-float records per reduced row (80 KB for a
+// 1024-node line, two workgroups per CU). Every pivot is shifted by PIVOT_SHIFT of its largest diagonal entry,
 // as in the reference.
 //
 // Levels with lines longer than 128 nodes also keep their operator, fixed flags and iterate in a
@@ -57,7 +57,8 @@ __device__ __forceinline__ void pivot_inv(const float* s, float* o) {
   float e = (float)kPivotShift * fmaxf(fabsf(s[0]), fmaxf(fabsf(s[3]), fabsf(s[5])));
   float a00 = s[0] + e, a01 = s[1], a02 = s[2], a11 = s[3] + e, a12 = s[4], a22 = s[5] + e;
   float c00 = a11 * a22 - a12 * a12, c01 = a12 * a02 - a01 * a22, c02 = a01 * a12 - a11 * a02;
-  float id = 1.f / (a00 * c00 + a01 * c01 + a02 * c02);
+  // hardware reciprocal (1 ulp): the pivots of a preconditioner need no correctly rounded division
+  float id = __builtin_amdgcn_rcpf(a00 * c00 + a01 * c01 + a02 * c02);
   o[0] = c00 * id;
   o[1] = c01 * id;
   o[2] = c02 * id;
@@ -306,8 +307,30 @@ __device__ __forceinline__ void gather_dispatch(int g, const SmoothArgs& S, int 
   }
 }
 
-// LDS floats per reduced row the line solve uses: two buffers of D (6), U (9), r (3)
-constexpr int kSolveFloats = 36;
+// LDS record of one reduced row in the line solve: D^-1 (6, symmetric), U (9), r (3) and 2 floats of
+// padding, so that a record is five 16-byte vector accesses with immediate offsets
+constexpr int kRowF = 20;
+// LDS floats per reduced row: two record buffers
+constexpr int kSolveFloats = 2 * kRowF;
+// variant code of the split smoother (row kernel + solve kernel)
+constexpr int kSplitVariant = 16;
+
+__device__ __forceinline__ void put_row(float* B, int t, const float Di[6], const float U[9], const float r[3]) {
+  float4* p = reinterpret_cast<float4*>(B + kRowF * t);
+  p[0] = make_float4(Di[0], Di[1], Di[2], Di[3]);
+  p[1] = make_float4(Di[4], Di[5], U[0], U[1]);
+  p[2] = make_float4(U[2], U[3], U[4], U[5]);
+  p[3] = make_float4(U[6], U[7], U[8], r[0]);
+  p[4] = make_float4(r[1], r[2], 0.f, 0.f);
+}
+
+__device__ __forceinline__ void get_row(const float* B, int j, float Di[6], float U[9], float r[3]) {
+  const float4* p = reinterpret_cast<const float4*>(B + kRowF * j);
+  const float4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+  Di[0] = a.x; Di[1] = a.y; Di[2] = a.z; Di[3] = a.w; Di[4] = b.x; Di[5] = b.y;
+  U[0] = b.z; U[1] = b.w; U[2] = c.x; U[3] = c.y; U[4] = c.z; U[5] = c.w; U[6] = d.x; U[7] = d.y; U[8] = d.z;
+  r[0] = d.w; r[1] = e.x; r[2] = e.y;
+}
 
 // Block tridiagonal solve of one line, two rows per thread.
 // Thread t holds rows a = 2t and b = 2t+1 (D sym, U full coupling to the next row, r) and
@@ -317,12 +340,13 @@ constexpr int kSolveFloats = 36;
 // neighbours at distance d at once, so after ceil(log2 R2) levels (one barrier each, the row kept
 // in registers, LDS ping-ponged) each row stands alone; there is no back-substitution sweep. The
 // reduced system stays symmetric (row k's coupling to k - d is row k - d's coupling to k,
-// transposed), so a row carries only its forward coupling. cr: kSolveFloats * Lr floats of LDS.
+// transposed), so a row carries only its forward coupling, and each row publishes its own shifted
+// pivot inverse with it (one inversion per row and level). cr: kSolveFloats * Lr floats of LDS.
 __device__ __forceinline__ void line_solve(float* cr, int Lr, int t, int len, const float* Da_in, const float* Ua,
                                            const float* ra_in, const float* Db, const float* Ub, const float* rb,
                                            float xa[3], float xb[3]) {
-  float* B0 = cr;            // 18 planes: D 6, U 9, r 3
-  float* B1 = cr + 18 * Lr;  // 18 planes; the register level's exchange uses its first 9
+  float* B0 = cr;
+  float* B1 = cr + kRowF * Lr;  // also carries the register level's exchange (its first 9 floats)
   const int R2 = (len + 1) >> 1;
   const bool has_a = 2 * t < len, has_b = 2 * t + 1 < len;
   float D[6], r[3], U[9];
@@ -342,15 +366,10 @@ __device__ __forceinline__ void line_solve(float* cr, int Lr, int t, int len, co
     mtm(Ub, T, P);
     mtv(Ub, yb, p);
     if (has_b && t + 1 < R2) {
-      B1[0 * Lr + t] = P[0];
-      B1[1 * Lr + t] = 0.5f * (P[1] + P[3]);
-      B1[2 * Lr + t] = 0.5f * (P[2] + P[6]);
-      B1[3 * Lr + t] = P[4];
-      B1[4 * Lr + t] = 0.5f * (P[5] + P[7]);
-      B1[5 * Lr + t] = P[8];
-      B1[6 * Lr + t] = p[0];
-      B1[7 * Lr + t] = p[1];
-      B1[8 * Lr + t] = p[2];
+      float4* e = reinterpret_cast<float4*>(B1 + kRowF * t);
+      e[0] = make_float4(P[0], 0.5f * (P[1] + P[3]), 0.5f * (P[2] + P[6]), P[4]);
+      e[1] = make_float4(0.5f * (P[5] + P[7]), P[8], p[0], p[1]);
+      e[2] = make_float4(p[2], 0.f, 0.f, 0.f);
     }
   }
   {
@@ -373,10 +392,10 @@ __device__ __forceinline__ void line_solve(float* cr, int Lr, int t, int len, co
   }
   __syncthreads();
   if (has_a && t >= 1) {
-#pragma unroll
-    for (int q = 0; q < 6; q++) D[q] -= B1[q * Lr + t - 1];
-#pragma unroll
-    for (int q = 0; q < 3; q++) r[q] -= B1[(6 + q) * Lr + t - 1];
+    const float4* e = reinterpret_cast<const float4*>(B1 + kRowF * (t - 1));
+    const float4 e0 = e[0], e1 = e[1], e2 = e[2];
+    D[0] -= e0.x; D[1] -= e0.y; D[2] -= e0.z; D[3] -= e0.w; D[4] -= e1.x; D[5] -= e1.y;
+    r[0] -= e1.z; r[1] -= e1.w; r[2] -= e2.x;
   }
   // B1's exchange terms are read before the first level's barrier; that level writes B0 and the
   // next one B1, so one barrier per level orders every read before the overwrite.
@@ -385,68 +404,59 @@ __device__ __forceinline__ void line_solve(float* cr, int Lr, int t, int len, co
   for (int d = 1; d < R2; d <<= 1, lvl ^= 1) {
     float* B = lvl ? B1 : B0;
     if (row) {
-#pragma unroll
-      for (int q = 0; q < 6; q++) B[q * Lr + t] = D[q];
-#pragma unroll
-      for (int q = 0; q < 9; q++) B[(6 + q) * Lr + t] = U[q];
-#pragma unroll
-      for (int q = 0; q < 3; q++) B[(15 + q) * Lr + t] = r[q];
+      float Di[6];
+      pivot_inv(D, Di);
+      put_row(B, t, Di, U, r);
     }
     __syncthreads();
     if (row) {
-      float Dn[9], Un[9];
-      sym_full(D, Dn);
+      float Un[9];
 #pragma unroll
       for (int q = 0; q < 9; q++) Un[q] = 0.f;
       const int j1 = t - d, j2 = t + d;
       if (j1 >= 0) {
-        // row j1 couples to this row by U1: eliminate it
-        float D1[6], Di[6], Dif[9], U1[9], T[9], W[9], r1[3], y1[3], tt[3];
-#pragma unroll
-        for (int q = 0; q < 6; q++) D1[q] = B[q * Lr + j1];
-#pragma unroll
-        for (int q = 0; q < 9; q++) U1[q] = B[(6 + q) * Lr + j1];
-#pragma unroll
-        for (int q = 0; q < 3; q++) r1[q] = B[(15 + q) * Lr + j1];
-        pivot_inv(D1, Di);
+        // row j1 couples to this row by U1: D -= U1^T D1^-1 U1, r -= U1^T D1^-1 r1
+        float Di[6], U1[9], r1[3], T[9];
+        get_row(B, j1, Di, U1, r1);
+        float Dif[9];
         sym_full(Di, Dif);
-        mm(Dif, U1, T);
-        mtm(U1, T, W);
-#pragma unroll
-        for (int q = 0; q < 9; q++) Dn[q] -= W[q];
-        mv(Dif, r1, y1);
-        mtv(U1, y1, tt);
-#pragma unroll
-        for (int q = 0; q < 3; q++) r[q] -= tt[q];
+        mm(Dif, U1, T);  // T = D1^-1 U1
+        D[0] -= U1[0] * T[0] + U1[3] * T[3] + U1[6] * T[6];
+        D[1] -= U1[0] * T[1] + U1[3] * T[4] + U1[6] * T[7];
+        D[2] -= U1[0] * T[2] + U1[3] * T[5] + U1[6] * T[8];
+        D[3] -= U1[1] * T[1] + U1[4] * T[4] + U1[7] * T[7];
+        D[4] -= U1[1] * T[2] + U1[4] * T[5] + U1[7] * T[8];
+        D[5] -= U1[2] * T[2] + U1[5] * T[5] + U1[8] * T[8];
+        float tt[3];
+        mtv(T, r1, tt);  // U1^T D1^-1 r1
+        r[0] -= tt[0];
+        r[1] -= tt[1];
+        r[2] -= tt[2];
       }
       if (j2 < R2) {
-        // this row couples to row j2 by U; row j2 couples on to j2 + d by U2
-        float D2[6], Di[6], Dif[9], U2[9], T2[9], W[9], r2[3], tt[3];
-#pragma unroll
-        for (int q = 0; q < 6; q++) D2[q] = B[q * Lr + j2];
-#pragma unroll
-        for (int q = 0; q < 9; q++) U2[q] = B[(6 + q) * Lr + j2];
-#pragma unroll
-        for (int q = 0; q < 3; q++) r2[q] = B[(15 + q) * Lr + j2];
-        pivot_inv(D2, Di);
+        // this row couples to row j2 by U, row j2 on to j2 + d by U2: D -= U D2^-1 U^T,
+        // r -= U D2^-1 r2, and the new coupling to j2 + d is -U D2^-1 U2
+        float Di[6], U2[9], r2[3], T2[9];
+        get_row(B, j2, Di, U2, r2);
+        float Dif[9];
         sym_full(Di, Dif);
-        mm(U, Dif, T2);
-        mmt(T2, U, W);
-#pragma unroll
-        for (int q = 0; q < 9; q++) Dn[q] -= W[q];
+        mm(U, Dif, T2);  // T2 = U D2^-1
+        D[0] -= T2[0] * U[0] + T2[1] * U[1] + T2[2] * U[2];
+        D[1] -= T2[0] * U[3] + T2[1] * U[4] + T2[2] * U[5];
+        D[2] -= T2[0] * U[6] + T2[1] * U[7] + T2[2] * U[8];
+        D[3] -= T2[3] * U[3] + T2[4] * U[4] + T2[5] * U[5];
+        D[4] -= T2[3] * U[6] + T2[4] * U[7] + T2[5] * U[8];
+        D[5] -= T2[6] * U[6] + T2[7] * U[7] + T2[8] * U[8];
+        float tt[3];
         mv(T2, r2, tt);
-#pragma unroll
-        for (int q = 0; q < 3; q++) r[q] -= tt[q];
+        r[0] -= tt[0];
+        r[1] -= tt[1];
+        r[2] -= tt[2];
+        float W[9];
         mm(T2, U2, W);
 #pragma unroll
         for (int q = 0; q < 9; q++) Un[q] = -W[q];
       }
-      D[0] = Dn[0];
-      D[1] = 0.5f * (Dn[1] + Dn[3]);
-      D[2] = 0.5f * (Dn[2] + Dn[6]);
-      D[3] = Dn[4];
-      D[4] = 0.5f * (Dn[5] + Dn[7]);
-      D[5] = Dn[8];
 #pragma unroll
       for (int q = 0; q < 9; q++) U[q] = Un[q];
     }
@@ -458,16 +468,16 @@ __device__ __forceinline__ void line_solve(float* cr, int Lr, int t, int len, co
     float inv[6];
     pivot_inv(D, inv);
     smv(inv, r, xa);
-    B[t] = xa[0];
-    B[Lr + t] = xa[1];
-    B[2 * Lr + t] = xa[2];
+    B[kRowF * t] = xa[0];
+    B[kRowF * t + 1] = xa[1];
+    B[kRowF * t + 2] = xa[2];
   }
   __syncthreads();
   float xn[3] = {0.f, 0.f, 0.f};
   if (t + 1 < R2) {
-    xn[0] = B[t + 1];
-    xn[1] = B[Lr + t + 1];
-    xn[2] = B[2 * Lr + t + 1];
+    xn[0] = B[kRowF * (t + 1)];
+    xn[1] = B[kRowF * (t + 1) + 1];
+    xn[2] = B[kRowF * (t + 1) + 2];
   }
   float tt[3], q1[3];
   mtv(Ua, xa, tt);
@@ -489,9 +499,9 @@ struct LineGeo {
 };
 
 template <int FAM, bool DUAL>
-__device__ __forceinline__ LineGeo line_geo(const SmoothArgs& S) {
+__device__ __forceinline__ LineGeo line_geo_at(const SmoothArgs& S, int idx) {
   LineGeo g;
-  g.ln = S.lines[xcd_block(blockIdx.x, gridDim.x)];
+  g.ln = S.lines[idx];
   int l = geo_layer(S.geo, g.ln.x);
   g.nx = S.geo.nx[l];
   g.ny = S.geo.ny[l];
@@ -505,6 +515,12 @@ __device__ __forceinline__ LineGeo line_geo(const SmoothArgs& S) {
     g.i0 = r0 - g.j0 * g.nx;
   }
   return g;
+}
+
+// the line of this workgroup (blocks spread over the XCDs)
+template <int FAM, bool DUAL>
+__device__ __forceinline__ LineGeo line_geo(const SmoothArgs& S) {
+  return line_geo_at<FAM, DUAL>(S, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // row k of the line: its index in the pass's layout, in the row-major layout (b), and in the other
@@ -535,7 +551,7 @@ __device__ __forceinline__ void store_x(const SmoothArgs& S, int n, int no, cons
 template <bool FINE, int FAM, bool DUAL>
 __global__ __launch_bounds__(512, 4) void k_smooth_direct(SmoothArgs S) {
   if (S.gate && *S.gate) return;
-  extern __shared__ float sm[];
+  extern __shared__ __align__(16) float sm[];
   const LineGeo L = line_geo<FAM, DUAL>(S);
   const int len = L.ln.z;
   const int t = threadIdx.x;
@@ -564,7 +580,7 @@ __global__ __launch_bounds__(512, 4) void k_smooth_direct(SmoothArgs S) {
 template <bool FINE, int FAM, int G, bool DUAL>
 __global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S) {
   if (S.gate && *S.gate) return;
-  extern __shared__ float sm[];
+  extern __shared__ __align__(16) float sm[];
   const LineGeo L = line_geo<FAM, DUAL>(S);
   const int len = L.ln.z;
   const int R2 = (len + 1) >> 1;
@@ -653,8 +669,65 @@ __global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S) {
   }
 }
 
-// rows per staged line up to which the staged kernel is used, and its gather width
-constexpr int kStagedMaxLen = 257;
+// Split variant, first kernel: one thread per row of the pass's lines (blockIdx.y the line, x the
+// 256-row chunk), so that the whole chip gathers with full occupancy and no line's workgroup waits
+// on another's CU; writes the row's D, U and r to the rows planes (line slot * stride + row).
+template <bool FINE, int FAM, bool DUAL>
+__global__ __launch_bounds__(256) void k_line_rows(SmoothArgs S, float* rows, size_t cap, int stride) {
+  if (S.gate && *S.gate) return;
+  const int slot = blockIdx.y;
+  const LineGeo L = line_geo_at<FAM, DUAL>(S, slot);
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= L.ln.z) return;
+  int n, nb, no, i, j;
+  row_index<FAM, DUAL>(L, k, n, nb, no, i, j);
+  float D[6], U[9], r[3];
+  bool f;
+  gather_row<FINE, FAM, DUAL>(S, n, nb, i, j, L.nx, L.ny, D, U, r, f);
+  float* o = rows + (size_t)slot * stride + k;
+#pragma unroll
+  for (int q = 0; q < 6; q++) o[q * cap] = D[q];
+#pragma unroll
+  for (int q = 0; q < 9; q++) o[(6 + q) * cap] = U[q];
+#pragma unroll
+  for (int q = 0; q < 3; q++) o[(15 + q) * cap] = r[q];
+}
+
+// Split variant, second kernel: the line solve of k_smooth_direct over the rows k_line_rows wrote.
+template <bool FINE, int FAM, bool DUAL>
+__global__ __launch_bounds__(512, 4) void k_smooth_rows(SmoothArgs S, const float* rows, size_t cap, int stride) {
+  if (S.gate && *S.gate) return;
+  extern __shared__ __align__(16) float sm[];
+  const int slot = xcd_block(blockIdx.x, gridDim.x);
+  const LineGeo L = line_geo_at<FAM, DUAL>(S, slot);
+  const int len = L.ln.z;
+  const int t = threadIdx.x;
+  const int ka = 2 * t, kb = 2 * t + 1;
+  int na, nba, noa, ia, ja, nb, nbb, nob, ib, jb;
+  row_index<FAM, DUAL>(L, ka, na, nba, noa, ia, ja);
+  row_index<FAM, DUAL>(L, kb, nb, nbb, nob, ib, jb);
+  float Da[6], Ua[9], ra[3], Db[6], Ub[9], rb[3];
+  const float* o = rows + (size_t)slot * stride;
+  auto load = [&](int k, float D[6], float U[9], float r[3]) {
+    if (k < len) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) D[q] = o[q * cap + k];
+#pragma unroll
+      for (int q = 0; q < 9; q++) U[q] = o[(6 + q) * cap + k];
+#pragma unroll
+      for (int q = 0; q < 3; q++) r[q] = o[(15 + q) * cap + k];
+    } else {
+      set_identity(D, U, r);
+    }
+  };
+  load(ka, Da, Ua, ra);
+  load(kb, Db, Ub, rb);
+  const bool fa = ka >= len || S.fixed[na] != 0, fb = kb >= len || S.fixed[nb] != 0;
+  float xa[3], xb[3];
+  line_solve(sm, blockDim.x, t, len, Da, Ua, ra, Db, Ub, rb, xa, xb);
+  if (!fa) store_x(S, na, noa, xa, DUAL);
+  if (!fb) store_x(S, nb, nob, xb, DUAL);
+}
 
 template <bool FINE, int FAM, int G>
 size_t staged_shm(int len) {
@@ -669,7 +742,14 @@ constexpr int kMaxLds = 160 * 1024;
 // variant: -1 the default rule below, 0 the direct kernel, 4 or 8 the staged kernel with that gather
 // width (bd_autotune picks per level and family)
 template <bool FINE, int FAM, bool DUAL>
-void launch_one(const SmoothArgs& S, int count, int len, hipStream_t st, int variant) {
+void launch_one(const SmoothArgs& S, int count, int len, hipStream_t st, int variant, float* rows, size_t cap) {
+  if (variant == kSplitVariant) {
+    if (!rows || (size_t)count * len > cap) throw std::runtime_error("smoother: split variant without room for its rows");
+    k_line_rows<FINE, FAM, DUAL><<<dim3((len + 255) / 256, count), 256, 0, st>>>(S, rows, cap, len);
+    const int block = ((len + 1) / 2 + 63) / 64 * 64;
+    k_smooth_rows<FINE, FAM, DUAL><<<count, block, (size_t)kSolveFloats * block * sizeof(float), st>>>(S, rows, cap, len);
+    return;
+  }
   if (variant == 4 || variant == 8) {
     const int G = variant;
     const int R2 = (len + 1) / 2;
@@ -701,6 +781,7 @@ void launch_one(const SmoothArgs& S, int count, int len, hipStream_t st, int var
 template <bool F, int A, bool D>
 void set_direct_attr(int shm) {
   BD_HIP(hipFuncSetAttribute((const void*)k_smooth_direct<F, A, D>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_rows<F, A, D>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
 }
 template <bool F, int A, int G, bool D>
 void set_staged_attr(int shm) {
@@ -749,26 +830,28 @@ void launch_colour(Solver& s, int k, int c, int variant) {
   ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * ((L.fine ? 268.0 : 636.0) + (L.dual ? 12.0 : 0.0)));
   if (L.dual) {
     if (L.fine) {
-      if (fam == 0) launch_one<true, 0, true>(S, count, maxlen, s.stream, variant);
-      else launch_one<true, 1, true>(S, count, maxlen, s.stream, variant);
+      if (fam == 0) launch_one<true, 0, true>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+      else launch_one<true, 1, true>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
     } else {
-      if (fam == 0) launch_one<false, 0, true>(S, count, maxlen, s.stream, variant);
-      else launch_one<false, 1, true>(S, count, maxlen, s.stream, variant);
+      if (fam == 0) launch_one<false, 0, true>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+      else launch_one<false, 1, true>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
     }
   } else {
     if (L.fine) {
-      if (fam == 0) launch_one<true, 0, false>(S, count, maxlen, s.stream, variant);
-      else launch_one<true, 1, false>(S, count, maxlen, s.stream, variant);
+      if (fam == 0) launch_one<true, 0, false>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+      else launch_one<true, 1, false>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
     } else {
-      if (fam == 0) launch_one<false, 0, false>(S, count, maxlen, s.stream, variant);
-      else launch_one<false, 1, false>(S, count, maxlen, s.stream, variant);
+      if (fam == 0) launch_one<false, 0, false>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+      else launch_one<false, 1, false>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
     }
   }
 }
 
-// the variants a line of `len` rows can run: the direct kernel, and the staged ones whose LDS fits
-std::vector<int> variants_for(bool fine, int len) {
+// the variants a line of `len` rows can run: the direct kernel, the split one where the level keeps
+// a rows buffer, and the staged ones whose LDS fits
+std::vector<int> variants_for(bool fine, int len, bool has_rows) {
   std::vector<int> v{0};
+  if (has_rows) v.push_back(kSplitVariant);
   if ((fine ? staged_shm<true, 0, 4>(len) : staged_shm<false, 0, 4>(len)) <= (size_t)kMaxLds) v.push_back(4);
   if ((fine ? staged_shm<true, 0, 8>(len) : staged_shm<false, 0, 8>(len)) <= (size_t)kMaxLds) v.push_back(8);
   return v;
@@ -781,7 +864,16 @@ std::vector<int> smooth_variants(Solver& s, int k, int fam) {
   Level& L = s.lv[k];
   int maxlen = 0;
   for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
-  return variants_for(L.fine, maxlen);
+  return variants_for(L.fine, maxlen, L.rows != nullptr);
+}
+
+// The rows buffer of the split variant: 18 planes of (most lines of one colour) x (longest line).
+void smooth_alloc_rows(Solver& s, Level& L) {
+  (void)s;
+  int most = 0;
+  for (int c = 0; c < 2 * L.ncol; c++) most = std::max(most, L.color_start[c + 1] - L.color_start[c]);
+  L.rows_cap = (size_t)most * L.max_len;
+  BD_HIP(hipMalloc(&L.rows, 18 * L.rows_cap * sizeof(float)));
 }
 
 // Smooth all colours of level k (v-line colours first, then u-line colours; reversed for the

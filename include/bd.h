@@ -268,13 +268,26 @@ int bd_project_start(bd_solver* s);
 /* setFloor(z | null) (gpu/solver.ts:796-801): floor plane at height z when `enabled`, else none. */
 int bd_set_floor(bd_solver* s, int enabled, double z);
 
+/* Smoother kernel variants of a level's line passes (bd_autotune, bd_set_smoother). */
+#define BD_SMOOTHER_DEFAULT -1 /* direct for lines longer than 257 nodes, else staged */
+#define BD_SMOOTHER_DIRECT 0   /* one workgroup per line gathers its rows into registers and solves */
+#define BD_SMOOTHER_STAGED4 4  /* rows gathered into LDS, 4 threads per row */
+#define BD_SMOOTHER_STAGED8 8  /* rows gathered into LDS, 8 threads per row */
+#define BD_SMOOTHER_SPLIT 16   /* a chip-wide row kernel, then one workgroup per line solves (levels
+                                  with lines longer than 257 nodes) */
+
 /* Autotuner (no counterpart in the reference): for every level's v-line and u-line passes of the
- * current hierarchy, the smoother kernel variant (the direct kernel, the staged kernel with 4 or 8
- * threads per row) whose V-cycles run fastest, chosen greedily level by level with each variant
- * timed inside whole V-cycles (zero b and x; the fastest of `reps`). chosen receives 2 entries per
- * level (u, v): 0 direct, 4 / 8 staged; may be NULL. Results of a pass differ between variants only
- * by FP32 summation order. */
+ * current hierarchy, the smoother kernel variant (BD_SMOOTHER_*) whose V-cycles run fastest, chosen
+ * greedily level by level with each variant timed inside whole V-cycles (zero b and x; the fastest
+ * of `reps`). chosen receives 2 entries per level (u, v); may be NULL. Results of a pass differ
+ * between variants only by FP32 summation order. */
 int bd_autotune(bd_solver* s, int reps, int32_t* chosen);
+
+/* Pins the smoother kernel variant of one level's u-line (family 0) or v-line (family 1) passes
+ * (no counterpart in the reference: for tests and callers that tune offline). BD_ERR_INVALID for
+ * a variant the level cannot run (its LDS does not fit, or no rows buffer for BD_SMOOTHER_SPLIT);
+ * BD_ERR_STATE before the first assembly. A new hierarchy keeps the pinned variants. */
+int bd_set_smoother(bd_solver* s, int level, int family, int variant);
 
 /* Grips and picking (src/simulation/gpu/interaction.ts, interact.wgsl): up to BD_MAX_GRIPS_HELD grips
  * hold disjoint material patches of at most BD_GRIP_SLOT_ENTRIES entries, each a quadratic spring from

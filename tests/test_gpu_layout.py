@@ -155,3 +155,68 @@ def test_repeated_sweeps_keep_parity(sweeps):
     e = rel_to_scale(gpu.read(BD_BUF_DELTA), cpu.pcg(b, params.pcg_iterations))
     say(f"  PCG maxAbs/scale = {e:.2e}")
     assert e < 2e-2
+
+
+def test_every_smoother_variant_keeps_parity():
+    """Each smoother kernel variant a level can run (bd_set_smoother), pinned on every level at once:
+    one sweep, the V-cycle and PCG meet the same bars against the oracle on a 300 x 300 sheet, whose
+    fine lines (300 nodes) take the direct, split and staged kernels."""
+    from bd.capi import (BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT, BD_SMOOTHER_STAGED4, BD_SMOOTHER_STAGED8,
+                         BD_SMOOTHER_DEFAULT)
+    from bd.solver import GpuSolver
+    n = 300
+    pitch = 1e-3
+    mat = linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / 2.4)
+    S = sheet_scene(n, pitch, mat)
+    params = make_params(history=False)
+    cpu = O.Oracle(S, params)
+    for _ in range(2):
+        cpu.substep()
+    xs = cpu.x.astype(np.float32).astype(np.float64)
+    vs = cpu.v.astype(np.float32).astype(np.float64)
+    cpu.x[:] = xs
+    cpu.v[:] = vs
+    say("gpu: 300x300 sheet")
+    gpu = GpuSolver(S, params)
+    gpu.set_state(xs, vs)
+    cpu.predict()
+    cpu.assemble()
+    gpu.encode_predict()
+    gpu.encode_assemble(True)
+    cpu.load_fine_operator(gpu.stencil(0), gpu.fixed(0))
+    b = gpu.read(BD_BUF_RHS)
+    rng = np.random.default_rng(6)
+    x0 = rng.standard_normal(3 * S.node_count) * 1e-4
+    x0[np.repeat(cpu.fixed(0) == 1, 3)] = 0
+    xs_ref = cpu.smooth(0, x0.copy(), b, False)
+    xv_ref = cpu.vcycle(b)
+    xp_ref = cpu.pcg(b, params.pcg_iterations)
+    nlev = len(gpu.stats()["level_nodes"])
+    for variant in (BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT, BD_SMOOTHER_STAGED4, BD_SMOOTHER_STAGED8):
+        pinned = []
+        for k in range(nlev):
+            for fam in (0, 1):
+                try:
+                    gpu.set_smoother(k, fam, variant)
+                    pinned.append((k, fam))
+                except RuntimeError:
+                    gpu.set_smoother(k, fam, BD_SMOOTHER_DEFAULT)
+        if variant == BD_SMOOTHER_SPLIT:
+            assert (0, 0) in pinned and (0, 1) in pinned, "the fine level keeps a rows buffer"
+        say(f"variant {variant}: pinned on {pinned}")
+        gpu.write(BD_BUF_LEVEL_B, b, 0)
+        gpu.write(BD_BUF_LEVEL_X, x0, 0)
+        gpu.encode_smooth(0, False)
+        e = rel_to_scale(gpu.read(BD_BUF_LEVEL_X, 0), xs_ref)
+        say(f"  sweep maxAbs/scale = {e:.2e}")
+        assert e < 1e-3
+        gpu.write(BD_BUF_LEVEL_B, b, 0)
+        gpu.encode_vcycle()
+        e = rel_to_scale(gpu.read(BD_BUF_LEVEL_X, 0), xv_ref)
+        say(f"  V-cycle maxAbs/scale = {e:.2e}")
+        assert e < 5e-3
+        gpu.write(BD_BUF_RHS, b)
+        gpu.encode_pcg()
+        e = rel_to_scale(gpu.read(BD_BUF_DELTA), xp_ref)
+        say(f"  PCG maxAbs/scale = {e:.2e}")
+        assert e < 2e-2
