@@ -9,8 +9,9 @@
 //                    LDS as a ring of rows (98.6 KB at bw = 110); rows leave the window final;
 //                    wider bands (33 x 33 and up) factor by panels of 16 columns in LDS with
 //                    rank-16 trailing updates in global memory (k_band_cholesky_panel);
+//   k_band_inverse_blk  16 columns per workgroup, 16 rows per step (bands up to 240);
 //   k_band_inverse   one wave per column c: L y = e_c, then L^T x = y, written as row c of the
-//                    symmetric inverse (coalesced);
+//                    symmetric inverse (coalesced), for wider bands;
 // and per V-cycle k_coarse_apply: one wave per row of A_c^-1 (FP64 dot with b, 6 MB at 867 DOFs).
 #include <stdexcept>
 
@@ -298,6 +299,112 @@ __global__ __launch_bounds__(64) void k_band_inverse(BandArgs B, const double* r
   }
 }
 
+// Blocked inverse: a workgroup of 256 threads solves kInvCols = 16 columns c of the inverse at once,
+// kInvRows = 16 rows per step, so that each band row is read once per 16 columns and the serial
+// chain is n / 16 steps per pass instead of n. Thread (cc, r) = (t / 16, t % 16) owns row i0 + r of
+// column c0 + cc. A step: the band rows of the block (prefetched into registers during the previous
+// step) go to LDS; each thread sums its row's band against the solution rows already known (an LDS
+// ring of ring_mask + 1 >= bw + 16 rows x 16 columns); the 16 x 16 diagonal block is then solved by
+// broadcasts within the column's 16 lanes. Forward L y = e_c from row c0 (y is zero above c), y to
+// the output row; backward L^T x = y over all rows, y read back by the thread that wrote it.
+constexpr int kInvCols = 16;
+constexpr int kInvRows = 16;
+
+__global__ __launch_bounds__(256) void k_band_inverse_blk(BandArgs B, const double* rd, int ring_mask) {
+  extern __shared__ double sh[];
+  const int n = B.n, bw = B.bw, w = bw + 1, lw = bw + kInvRows;
+  const int ring_rows = ring_mask + 1;
+  double* ring = sh;                        // ring_rows x kInvCols
+  double* Lb = sh + ring_rows * kInvCols;   // kInvRows x lw (forward) or lw x kInvRows (backward)
+  const double* L = B.L;
+  const int c0 = blockIdx.x * kInvCols;
+  const int t = threadIdx.x, cc = t >> 4, r = t & 15;
+  const int c = c0 + cc;
+  double* out = B.inv + (size_t)(c < n ? c : 0) * n;
+  const bool col = c < n;
+  constexpr int kPer = 16;  // band values a thread stages per step (>= kInvRows * lw / 256)
+  double pre[kPer];
+
+  for (int q = t; q < ring_rows * kInvCols; q += 256) ring[q] = 0.0;
+  if (col)
+    for (int i = r; i < c0; i += kInvRows) out[i] = 0.0;
+
+  // forward: Lb[rr * lw + j] = L(i0 + rr, i0 - bw + j)
+  auto fetch_fwd = [&](int i0) {
+#pragma unroll
+    for (int m = 0; m < kPer; m++) {
+      const int q = t + 256 * m;
+      const int rr = q / lw, j = q - rr * lw;
+      const int i = i0 + rr, k = i0 - bw + j, d = i - k;
+      pre[m] = (q < kInvRows * lw && i < n && k >= 0 && d >= 0 && d <= bw) ? L[(size_t)i * w + d] : 0.0;
+    }
+  };
+  fetch_fwd(c0);
+  for (int i0 = c0; i0 < n; i0 += kInvRows) {
+    __syncthreads();  // the previous step's Lb and ring reads are done
+#pragma unroll
+    for (int m = 0; m < kPer; m++) {
+      const int q = t + 256 * m;
+      if (q < kInvRows * lw) Lb[q] = pre[m];
+    }
+    __syncthreads();
+    if (i0 + kInvRows < n) fetch_fwd(i0 + kInvRows);
+    const int i = i0 + r;
+    const double rdi = i < n ? rd[i] : 0.0;
+    double s = 0.0;
+    const double* Lr = Lb + r * lw;
+    for (int j = r; j < bw; j++) s += Lr[j] * ring[((i0 - bw + j) & ring_mask) * kInvCols + cc];
+    double v = (i == c ? 1.0 : 0.0) - s, y = 0.0;
+#pragma unroll
+    for (int q = 0; q < kInvRows; q++) {
+      const double yq = __shfl(v * rdi, q, 16);
+      if (r == q) y = yq;
+      if (r > q) v -= Lr[bw + q] * yq;
+    }
+    ring[(i & ring_mask) * kInvCols + cc] = y;
+    if (col && i < n) out[i] = y;
+  }
+  __syncthreads();
+  for (int q = t; q < ring_rows * kInvCols; q += 256) ring[q] = 0.0;
+
+  // backward: Lb[kk * kInvRows + q] = L(i0 + kk, i0 + q) for 0 < kk - q <= bw
+  auto fetch_bwd = [&](int i0) {
+#pragma unroll
+    for (int m = 0; m < kPer; m++) {
+      const int e = t + 256 * m;
+      const int kk = e / kInvRows, q = e - kk * kInvRows;
+      const int k = i0 + kk, d = kk - q;
+      pre[m] = (e < lw * kInvRows && k < n && d > 0 && d <= bw) ? L[(size_t)k * w + d] : 0.0;
+    }
+  };
+  const int last = ((n - 1) / kInvRows) * kInvRows;
+  fetch_bwd(last);
+  for (int i0 = last; i0 >= 0; i0 -= kInvRows) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kPer; m++) {
+      const int e = t + 256 * m;
+      if (e < lw * kInvRows) Lb[e] = pre[m];
+    }
+    __syncthreads();
+    if (i0 > 0) fetch_bwd(i0 - kInvRows);
+    const int i = i0 + r;
+    const double rdi = i < n ? rd[i] : 0.0;
+    const double yi = (col && i < n) ? out[i] : 0.0;
+    double s = 0.0;
+    for (int kk = kInvRows; kk < lw; kk++) s += Lb[kk * kInvRows + r] * ring[((i0 + kk) & ring_mask) * kInvCols + cc];
+    double v = yi - s, x = 0.0;
+#pragma unroll
+    for (int q = kInvRows - 1; q >= 0; q--) {
+      const double xq = __shfl(v * rdi, q, 16);
+      if (r == q) x = xq;
+      if (r < q) v -= Lb[q * kInvRows + r] * xq;
+    }
+    ring[(i & ring_mask) * kInvCols + cc] = x;
+    if (col && i < n) out[i] = x;
+  }
+}
+
 // x = A_c^-1 b on the coarsest level: one wave per row, FP64 products and sums
 __global__ __launch_bounds__(256) void k_coarse_apply(const double* inv, int n, int Np, const float* b, float* x,
                                                       const int* gate) {
@@ -366,7 +473,13 @@ void launch_coarse_factor(Solver& s) {
     // inverse: the band read per column, the inverse written once
     ProfScope ps(s, PC_COARSE_INVERSE, 8.0 * (double)s.cn * s.cn);
     k_band_recip<<<grid_for(s.cn, 256), 256, 0, s.stream>>>(s.cband, s.cn, w, s.crd);
-    k_band_inverse<<<s.cn, 64, 0, s.stream>>>(B, s.crd);
+    if (s.cbw + kInvRows <= 256) {
+      // the blocked kernel's ring (256 rows) and staging (16 values per thread) cover the band
+      const size_t shm = sizeof(double) * (256 * kInvCols + (size_t)kInvRows * (s.cbw + kInvRows));
+      k_band_inverse_blk<<<(s.cn + kInvCols - 1) / kInvCols, 256, shm, s.stream>>>(B, s.crd, 255);
+    } else {
+      k_band_inverse<<<s.cn, 64, 0, s.stream>>>(B, s.crd);
+    }
   }
 }
 
