@@ -7,8 +7,9 @@
 //   k_band_load      the lower band from the symmetric coarse blocks (FP32 -> FP64);
 //   k_band_cholesky  right-looking banded Cholesky in one workgroup, the active (bw+1)^2 window in
 //                    LDS as a ring of rows (98.6 KB at bw = 110); rows leave the window final;
-//                    wider bands (33 x 33 and up) factor by panels of 16 columns in LDS with
-//                    rank-16 trailing updates in global memory (k_band_cholesky_panel);
+//                    wider bands (33 x 33 and up) factor by panels of 32 columns in LDS (the
+//                    diagonal block by one wave, the rows below one thread each) with rank-32
+//                    trailing updates in global memory (k_band_cholesky_panel);
 //   k_band_inverse_blk  16 columns per workgroup, 16 rows per step (bands up to 240);
 //   k_band_inverse   one wave per column c: L y = e_c, then L^T x = y, written as row c of the
 //                    symmetric inverse (coalesced), for wider bands;
@@ -121,13 +122,27 @@ __global__ __launch_bounds__(kCholThreads) void k_band_cholesky(BandArgs B) {
 }
 
 // The same factorization for bands whose window does not fit LDS, blocked by panels of kPanel
-// columns: the panel (its kPanel columns over the kPanel + bw rows they reach) is factored in LDS,
-// written back, and the trailing band triangle (bw x bw) below it takes one rank-kPanel update in
-// global memory (L2-resident), each thread holding a 4 x 4 tile of it in registers.
-constexpr int kPanel = 16;
+// columns: the panel (its kPanel columns over the kPanel + bw rows they reach) is loaded to LDS;
+// wave 0 alone factors its kPanel x kPanel diagonal block (a row per lane in registers, columns
+// passed by scalar lane reads, no workgroup barriers); every row below then solves against that factor in registers, one thread per row; the
+// panel is written back and the trailing band triangle (bw x bw) below it takes one rank-kPanel
+// update in global memory (L2-resident), each thread holding a 4 x 4 tile of it in registers. Four
+// workgroup barriers per panel.
+constexpr int kPanel = 32;
+// LDS row stride of the panel: odd in doubles, so that a column's rows fall in different banks
+constexpr int kPanelStride = kPanel + 1;
+// threads of the panel factorization (register room for a 32-wide row and a 4 x 4 tile pair)
+constexpr int kPanelThreads = 512;
 
-__global__ __launch_bounds__(kCholThreads) void k_band_cholesky_panel(BandArgs B) {
-  extern __shared__ double pan[];  // (kPanel + bw) x kPanel, row-major
+// lane `src`'s value of v (src uniform), for every lane
+__device__ __forceinline__ double lane_read(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+
+__global__ __launch_bounds__(kPanelThreads) void k_band_cholesky_panel(BandArgs B) {
+  extern __shared__ double pan[];  // (kPanel + bw) x kPanelStride, row-major
   const int n = B.n, bw = B.bw, w = bw + 1;
   const int t = threadIdx.x, T = blockDim.x;
   double* L = B.L;
@@ -138,30 +153,59 @@ __global__ __launch_bounds__(kCholThreads) void k_band_cholesky_panel(BandArgs B
     for (int e = t; e < rows * kPanel; e += T) {
       const int r = e / kPanel, c = e - r * kPanel;
       const int i = k0 + r, j = k0 + c;
-      pan[e] = (c < kb && j <= i && i - j <= bw) ? L[(size_t)i * w + (i - j)] : 0.0;
+      pan[r * kPanelStride + c] = (c < kb && j <= i && i - j <= bw) ? L[(size_t)i * w + (i - j)] : 0.0;
     }
     __syncthreads();
-    // factor the panel columns: every thread takes the pivot itself, the diagonal is stored in the
-    // update phase (which never reads it), so each column costs two barriers
-    for (int c = 0; c < kb; c++) {
-      const double d = pan[c * kPanel + c];
-      const double piv = d > 0.0 ? sqrt(d) : 1e-150;
-      for (int r = c + 1 + t; r < rows; r += T) pan[r * kPanel + c] /= piv;
-      __syncthreads();
-      if (t == 0) pan[c * kPanel + c] = piv;
-      // columns c+1 .. kb-1 of rows below them
-      const int nc = kb - 1 - c;
-      for (int e = t; e < (rows - c - 1) * nc; e += T) {
-        const int r = c + 1 + e / nc, c2 = c + 1 + e % nc;
-        if (c2 <= r) pan[r * kPanel + c2] -= pan[r * kPanel + c] * pan[c2 * kPanel + c];
+    // the diagonal block, right-looking, by wave 0: lane r holds row r in registers and column c's
+    // values come from the lanes that own them (scalar lane reads)
+    if (t < 64) {
+      double a[kPanel];
+#pragma unroll
+      for (int c = 0; c < kPanel; c++) a[c] = (t < kb && c <= t) ? pan[t * kPanelStride + c] : 0.0;
+#pragma unroll
+      for (int c = 0; c < kPanel; c++) {
+        if (c < kb) {
+          const double d = lane_read(a[c], c);
+          const double piv = d > 0.0 ? sqrt(d) : 1e-150;
+          if (t == c) a[c] = piv;
+          else if (t > c) a[c] /= piv;
+#pragma unroll
+          for (int c2 = c + 1; c2 < kPanel; c2++) {
+            const double l = lane_read(a[c], c2);  // l(c2, c)
+            if (t >= c2) a[c2] -= a[c] * l;
+          }
+        }
       }
-      __syncthreads();
+      if (t < kb) {
+#pragma unroll
+        for (int c = 0; c < kPanel; c++)
+          if (c <= t) pan[t * kPanelStride + c] = a[c];
+      }
     }
+    __syncthreads();
+    // rows below the block: l_rc = (a_rc - sum_{c' < c} l_rc' l_cc') / l_cc, one thread per row
+    for (int r = kb + t; r < rows; r += T) {
+      double a[kPanel];
+#pragma unroll
+      for (int c = 0; c < kPanel; c++) a[c] = pan[r * kPanelStride + c];
+#pragma unroll
+      for (int c = 0; c < kPanel; c++) {
+        if (c < kb) {
+          double v = a[c];
+#pragma unroll
+          for (int c2 = 0; c2 < c; c2++) v -= a[c2] * pan[c * kPanelStride + c2];
+          a[c] = v / pan[c * kPanelStride + c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < kPanel; c++) pan[r * kPanelStride + c] = a[c];
+    }
+    __syncthreads();
     // write the panel back
     for (int e = t; e < rows * kPanel; e += T) {
       const int r = e / kPanel, c = e - r * kPanel;
       const int i = k0 + r, j = k0 + c;
-      if (c < kb && j <= i && i - j <= bw) L[(size_t)i * w + (i - j)] = pan[e];
+      if (c < kb && j <= i && i - j <= bw) L[(size_t)i * w + (i - j)] = pan[r * kPanelStride + c];
     }
     // trailing update: L(i, j) -= sum_c pan[i - k0][c] pan[j - k0][c] for k0 + kb <= j <= i < k0 + rows
     const int m = rows - kb;  // trailing rows/columns
@@ -172,18 +216,23 @@ __global__ __launch_bounds__(kCholThreads) void k_band_cholesky_panel(BandArgs B
       while ((a + 1) * (a + 2) / 2 <= q) a++;
       while (a * (a + 1) / 2 > q) a--;
       const int b = q - a * (a + 1) / 2;  // tile (a, b), b <= a
-      double acc[4][4];
+      // the tile's current values load while the rank-kPanel product is formed
+      double acc[4][4], old[4][4];
 #pragma unroll
       for (int u = 0; u < 4; u++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) acc[u][v] = 0.0;
+        for (int v = 0; v < 4; v++) {
+          const int i = k0 + kb + 4 * a + u, j = k0 + kb + 4 * b + v;
+          old[u][v] = (i < k0 + rows && j <= i && i - j <= bw) ? L[(size_t)i * w + (i - j)] : 0.0;
+          acc[u][v] = 0.0;
+        }
       for (int c = 0; c < kb; c++) {
         double pi[4], pj[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           const int ri = kb + 4 * a + u, rj = kb + 4 * b + u;
-          pi[u] = ri < rows ? pan[ri * kPanel + c] : 0.0;
-          pj[u] = rj < rows ? pan[rj * kPanel + c] : 0.0;
+          pi[u] = ri < rows ? pan[ri * kPanelStride + c] : 0.0;
+          pj[u] = rj < rows ? pan[rj * kPanelStride + c] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < 4; u++)
@@ -195,7 +244,7 @@ __global__ __launch_bounds__(kCholThreads) void k_band_cholesky_panel(BandArgs B
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const int i = k0 + kb + 4 * a + u, j = k0 + kb + 4 * b + v;
-          if (i < k0 + rows && j <= i && i - j <= bw) L[(size_t)i * w + (i - j)] -= acc[u][v];
+          if (i < k0 + rows && j <= i && i - j <= bw) L[(size_t)i * w + (i - j)] = old[u][v] - acc[u][v];
         }
     }
     __syncthreads();
@@ -310,6 +359,20 @@ __global__ __launch_bounds__(64) void k_band_inverse(BandArgs B, const double* r
 constexpr int kInvCols = 16;
 constexpr int kInvRows = 16;
 
+// value of lane 16 g + q for every lane of 16-lane group g (q uniform): four scalar lane reads
+// instead of an LDS permute round trip
+__device__ __forceinline__ double group16_bcast(double v, int q) {
+  const int g = (threadIdx.x & 63) >> 4;
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int l0 = __builtin_amdgcn_readlane(lo, q), l1 = __builtin_amdgcn_readlane(lo, 16 + q);
+  const int l2 = __builtin_amdgcn_readlane(lo, 32 + q), l3 = __builtin_amdgcn_readlane(lo, 48 + q);
+  const int h0 = __builtin_amdgcn_readlane(hi, q), h1 = __builtin_amdgcn_readlane(hi, 16 + q);
+  const int h2 = __builtin_amdgcn_readlane(hi, 32 + q), h3 = __builtin_amdgcn_readlane(hi, 48 + q);
+  const int rl = g == 0 ? l0 : g == 1 ? l1 : g == 2 ? l2 : l3;
+  const int rh = g == 0 ? h0 : g == 1 ? h1 : g == 2 ? h2 : h3;
+  return __hiloint2double(rh, rl);
+}
+
 __global__ __launch_bounds__(256) void k_band_inverse_blk(BandArgs B, const double* rd, int ring_mask) {
   extern __shared__ double sh[];
   const int n = B.n, bw = B.bw, w = bw + 1, lw = bw + kInvRows;
@@ -340,6 +403,7 @@ __global__ __launch_bounds__(256) void k_band_inverse_blk(BandArgs B, const doub
     }
   };
   fetch_fwd(c0);
+  double rd_next = c0 + r < n ? rd[c0 + r] : 0.0;
   for (int i0 = c0; i0 < n; i0 += kInvRows) {
     __syncthreads();  // the previous step's Lb and ring reads are done
 #pragma unroll
@@ -350,14 +414,24 @@ __global__ __launch_bounds__(256) void k_band_inverse_blk(BandArgs B, const doub
     __syncthreads();
     if (i0 + kInvRows < n) fetch_fwd(i0 + kInvRows);
     const int i = i0 + r;
-    const double rdi = i < n ? rd[i] : 0.0;
-    double s = 0.0;
+    const double rdi = rd_next;
+    rd_next = i + kInvRows < n ? rd[i + kInvRows] : 0.0;
     const double* Lr = Lb + r * lw;
-    for (int j = r; j < bw; j++) s += Lr[j] * ring[((i0 - bw + j) & ring_mask) * kInvCols + cc];
+    // four independent sums keep the LDS reads of the 200-odd terms in flight
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int j = r;
+    for (; j + 3 < bw; j += 4) {
+      s0 += Lr[j] * ring[((i0 - bw + j) & ring_mask) * kInvCols + cc];
+      s1 += Lr[j + 1] * ring[((i0 - bw + j + 1) & ring_mask) * kInvCols + cc];
+      s2 += Lr[j + 2] * ring[((i0 - bw + j + 2) & ring_mask) * kInvCols + cc];
+      s3 += Lr[j + 3] * ring[((i0 - bw + j + 3) & ring_mask) * kInvCols + cc];
+    }
+    for (; j < bw; j++) s0 += Lr[j] * ring[((i0 - bw + j) & ring_mask) * kInvCols + cc];
+    const double s = (s0 + s1) + (s2 + s3);
     double v = (i == c ? 1.0 : 0.0) - s, y = 0.0;
 #pragma unroll
     for (int q = 0; q < kInvRows; q++) {
-      const double yq = __shfl(v * rdi, q, 16);
+      const double yq = group16_bcast(v * rdi, q);
       if (r == q) y = yq;
       if (r > q) v -= Lr[bw + q] * yq;
     }
@@ -379,6 +453,8 @@ __global__ __launch_bounds__(256) void k_band_inverse_blk(BandArgs B, const doub
   };
   const int last = ((n - 1) / kInvRows) * kInvRows;
   fetch_bwd(last);
+  rd_next = last + r < n ? rd[last + r] : 0.0;
+  double y_next = (col && last + r < n) ? out[last + r] : 0.0;
   for (int i0 = last; i0 >= 0; i0 -= kInvRows) {
     __syncthreads();
 #pragma unroll
@@ -389,14 +465,25 @@ __global__ __launch_bounds__(256) void k_band_inverse_blk(BandArgs B, const doub
     __syncthreads();
     if (i0 > 0) fetch_bwd(i0 - kInvRows);
     const int i = i0 + r;
-    const double rdi = i < n ? rd[i] : 0.0;
-    const double yi = (col && i < n) ? out[i] : 0.0;
-    double s = 0.0;
-    for (int kk = kInvRows; kk < lw; kk++) s += Lb[kk * kInvRows + r] * ring[((i0 + kk) & ring_mask) * kInvCols + cc];
+    const double rdi = rd_next, yi = y_next;
+    if (i0 > 0) {
+      rd_next = rd[i - kInvRows];
+      y_next = col ? out[i - kInvRows] : 0.0;
+    }
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int kk = kInvRows;
+    for (; kk + 3 < lw; kk += 4) {
+      s0 += Lb[kk * kInvRows + r] * ring[((i0 + kk) & ring_mask) * kInvCols + cc];
+      s1 += Lb[(kk + 1) * kInvRows + r] * ring[((i0 + kk + 1) & ring_mask) * kInvCols + cc];
+      s2 += Lb[(kk + 2) * kInvRows + r] * ring[((i0 + kk + 2) & ring_mask) * kInvCols + cc];
+      s3 += Lb[(kk + 3) * kInvRows + r] * ring[((i0 + kk + 3) & ring_mask) * kInvCols + cc];
+    }
+    for (; kk < lw; kk++) s0 += Lb[kk * kInvRows + r] * ring[((i0 + kk) & ring_mask) * kInvCols + cc];
+    const double s = (s0 + s1) + (s2 + s3);
     double v = yi - s, x = 0.0;
 #pragma unroll
     for (int q = kInvRows - 1; q >= 0; q--) {
-      const double xq = __shfl(v * rdi, q, 16);
+      const double xq = group16_bcast(v * rdi, q);
       if (r == q) x = xq;
       if (r < q) v -= Lb[q * kInvRows + r] * xq;
     }
@@ -448,7 +535,7 @@ void coarse_exact_alloc(Solver& s) {
   s.cwindow = lds <= 150 * 1024;
   if (s.cwindow)
     BD_HIP(hipFuncSetAttribute((const void*)k_band_cholesky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const size_t plds = sizeof(double) * (size_t)(kPanel + bw) * kPanel;
+  const size_t plds = sizeof(double) * (size_t)(kPanel + bw) * kPanelStride;
   if (!s.cwindow && plds > 150 * 1024) throw std::runtime_error("exact coarse solve: coarsest band too wide");
   if (!s.cwindow)
     BD_HIP(hipFuncSetAttribute((const void*)k_band_cholesky_panel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
@@ -467,7 +554,7 @@ void launch_coarse_factor(Solver& s) {
     if (s.cwindow)
       k_band_cholesky<<<1, kCholThreads, sizeof(double) * (size_t)w * w, s.stream>>>(B);
     else
-      k_band_cholesky_panel<<<1, kCholThreads, sizeof(double) * (size_t)(kPanel + s.cbw) * kPanel, s.stream>>>(B);
+      k_band_cholesky_panel<<<1, kPanelThreads, sizeof(double) * (size_t)(kPanel + s.cbw) * kPanelStride, s.stream>>>(B);
   }
   {
     // inverse: the band read per column, the inverse written once
