@@ -64,7 +64,7 @@ struct Level {
   int color_start[7] = {0};
   int ncol = 3;  // colours per family (v-lines 0 .. ncol-1, then u-lines): kLineColors
   int max_len = 0;
-  // dual layout (lines longer than 128 nodes): column-major copies (node (i, j) of a layer at
+  // dual layout (lines of kDualMinLine nodes or more): column-major copies (node (i, j) of a layer at
   // off + i * ny + j) of the operator, fixed flags and iterate for the v-line passes; the v-line
   // entries of `lines` then start in that layout with stride 1
   bool dual = false;
@@ -83,7 +83,9 @@ struct Level {
 // longest line the staged smoother kernel takes by default; longer lines use the direct kernel
 constexpr int kStagedMaxLen = 257;
 
-constexpr int kDualMinLine = 129;
+// levels whose longest line has at least this many nodes keep the column-major copy: below it a
+// v-line pass's strided gather costs less than the copy (about twice a u-line pass at 65 nodes)
+constexpr int kDualMinLine = 33;
 
 struct Scal {
   // PCG and line-search scalars kept on the device

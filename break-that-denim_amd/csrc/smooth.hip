@@ -17,7 +17,7 @@
 // 1024-node line, two workgroups per CU). Every pivot is shifted by PIVOT_SHIFT of its largest diagonal entry,
 // as in the reference.
 //
-// Levels with lines longer than 128 nodes also keep their operator, fixed flags and iterate in a
+// Levels with lines of kDualMinLine (33) or more nodes also keep their operator, fixed flags and iterate in a
 // column-major copy (node (i, j) of a layer at off + i * ny + j): v-line passes read that copy, so
 // consecutive rows of a v-line are consecutive in memory and the gathers coalesce as they do for
 // u-lines. Each pass writes its solution to both copies (the other one with a 12-byte strided

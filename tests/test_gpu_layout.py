@@ -1,7 +1,7 @@
-"""The column-major copy used by the v-line passes of long-line levels (lines of 129+ nodes),
+"""The column-major copy used by the v-line passes of long-line levels (lines of 33+ nodes),
 against the FP64 oracle with the reference's bars (tests/gpu/correctness.ts): one forward and one
 reverse smoother sweep on the fine level (1e-3), the V-cycle (5e-3) and a 2-iteration PCG (2e-2),
-on a 136 x 136 hanging sheet whose fine level keeps both layouts and whose coarser levels do not."""
+on a 136 x 136 hanging sheet whose finer levels keep both layouts and whose coarsest does not."""
 import sys
 
 import numpy as np
