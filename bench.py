@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default smoother kernel per level")
     ap.add_argument("--method", choices=("both", "mixed", "plain"), default="both")
     ap.add_argument("--sweeps", type=int, default=1, help="pre- and post-smoothing sweeps per level")
+    ap.add_argument("--smoother-probe", type=int, default=0, metavar="SWEEPS",
+                    help="only run SWEEPS forward sweeps of every smoothed level of the mixed method's hierarchy "
+                         "(after one frame and the autotuner), outside PCG: the smoother passes for PMC counters")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 8 plain device copies of MIB MiB (calibrates the PMC byte counters)")
     return ap.parse_args()
@@ -74,6 +77,33 @@ def copy_probe(mib):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "copies": 8}), flush=True)
+
+
+def smoother_probe(args, sweeps):
+    """The mixed method's smoother passes on their own: one frame of the 1024^2 workload builds the hierarchy, the
+    autotuner picks the kernels, then `sweeps` forward sweeps of each smoothed level run through bd_smooth (every
+    pass does its full work; inside PCG a converged solve gates the remaining V-cycle kernels)."""
+    import numpy as np
+    from bd.capi import BD_COARSE_EXACT
+    from bd.configs import config_sheet
+    from bd.solver import GpuSolver
+    cfg = config_sheet(args.size)
+    params = cfg["params"]
+    params.coarse_solve = BD_COARSE_EXACT
+    params.max_coarsest_nodes = MIXED_COARSEST_NODES
+    gpu = GpuSolver(cfg["scene"], params, cfg["grips"])
+    goals = np.array([g["goal"] for g in cfg["grips"]], dtype=np.float64) + np.array(cfg["velocity"])
+    gpu.set_grip_goals(goals)
+    gpu.encode_substep()
+    tuned = gpu.autotune(3)
+    levels = gpu.stats()["level_nodes"]
+    for k in range(len(levels) - 1):
+        for _ in range(sweeps):
+            gpu.encode_smooth(k, False)
+    gpu.sync()
+    print(json.dumps({"smoother_probe_levels": levels[:-1], "sweeps": sweeps,
+                      "smoother_kernels": {str(k): list(v) for k, v in tuned.items()}}), flush=True)
+    gpu.close()
 
 
 def cpu_baseline(cfg, gpu_iters_per_newton, newton_per_frame):
@@ -156,6 +186,9 @@ def main():
     args = parse()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.smoother_probe:
+        smoother_probe(args, args.smoother_probe)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU strip decomposition is "

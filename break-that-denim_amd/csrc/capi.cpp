@@ -18,6 +18,7 @@ struct bd_solver {
 namespace bd {
 void smooth_set_attributes();
 void launch_record(Solver& s, int it);
+void launch_open_gate(Solver& s);
 void launch_reset_counters(Solver& s);
 }  // namespace bd
 
@@ -638,7 +639,7 @@ int bd_assemble(bd_solver* h, int coarse) {
 int bd_vcycle(bd_solver* h) {
   return guarded(h, [&](Solver& s) {
     if (!s.hierarchy) return fail(h, BD_ERR_STATE, "bd_vcycle: assemble first");
-    BD_HIP(hipMemsetAsync(&s.scal->done, 0, sizeof(int), s.stream));
+    launch_open_gate(s);
     launch_zero(s, s.lv[0].x, 3 * (size_t)s.lv[0].Np);
     if (s.lv[0].dual) launch_zero(s, s.lv[0].xT, 3 * (size_t)s.lv[0].Np);
     vcycle(s, 0);
@@ -650,7 +651,7 @@ int bd_smooth(bd_solver* h, int level, int reverse) {
   return guarded(h, [&](Solver& s) {
     if (!s.hierarchy) return fail(h, BD_ERR_STATE, "bd_smooth: assemble first");
     if (level < 0 || level >= s.nlev) return fail(h, BD_ERR_INVALID, "bd_smooth: level out of range");
-    BD_HIP(hipMemsetAsync(&s.scal->done, 0, sizeof(int), s.stream));
+    launch_open_gate(s);
     launch_colmajor_x(s, level);
     launch_smooth(s, level, reverse);
     return BD_OK;
@@ -661,7 +662,7 @@ int bd_residual(bd_solver* h, int level) {
   return guarded(h, [&](Solver& s) {
     if (!s.hierarchy) return fail(h, BD_ERR_STATE, "bd_residual: assemble first");
     if (level < 0 || level >= s.nlev) return fail(h, BD_ERR_INVALID, "bd_residual: level out of range");
-    BD_HIP(hipMemsetAsync(&s.scal->done, 0, sizeof(int), s.stream));
+    launch_open_gate(s);
     Level& L = s.lv[level];
     launch_residual(s, level, L.x, L.b, L.r);
     return BD_OK;
@@ -928,7 +929,7 @@ int bd_autotune(bd_solver* h, int reps, int32_t* chosen) {
       }
       return best;
     };
-    BD_HIP(hipMemsetAsync(&s.scal->done, 0, sizeof(int), s.stream));
+    launch_open_gate(s);
     for (int k = 0; k < s.nlev; k++)
       for (int fam = 1; fam >= 0; fam--) {
         std::vector<int> vs = smooth_variants(s, k, fam);

@@ -433,7 +433,8 @@ void launch_rap(Solver& s, int k) {
 
 void launch_residual(Solver& s, int k, const float* x, const float* b, float* r) {
   Level& L = s.lv[k];
-  ProfScope ps(s, L.fine ? PC_RESIDUAL_FINE : PC_RESIDUAL_COARSE, L.N * (L.fine ? 268.0 : 636.0));
+  // unique bytes: the node's forward planes (32 fine, 78 coarse) + fixed flag + x, b read and r written
+  ProfScope ps(s, L.fine ? PC_RESIDUAL_FINE : PC_RESIDUAL_COARSE, L.N * (L.fine ? 165.0 : 349.0));
   if (L.fine)
     k_residual<true><<<grid_for(L.N, 256), 256, 0, s.stream>>>(lv_args(L), x, b, r, pcg_gate(s));
   else

@@ -247,51 +247,57 @@ __device__ __forceinline__ void blk_load6(const float* op, int Np, int base, int
   for (int q = 0; q < 6; q++) o[q] = op[(base + q) * Np + row];
 }
 
+// Each block takes a contiguous chunk of nodes (a few lattice rows), so that the mirrored couplings a
+// node reads from the rows below were fetched by the same block moments before (L2 hits); the loads
+// of a node's 13 couplings are branch-free (clamped indices, masked products) and issue together.
 __global__ __launch_bounds__(1024) void k_matvec_dot_fine(PcgArgs A, const float* op, Geo geo) {
   if (A.sc->done) return;
   const int Np = A.Np;
   double v[1] = {0};
-  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < A.N; n += gridDim.x * blockDim.x) {
+  const int chunk = (A.N + gridDim.x - 1) / gridDim.x;
+  const int n0 = blockIdx.x * chunk, n1 = min(A.N, n0 + chunk);
+  const float* p = A.p;
+  for (int n = n0 + threadIdx.x; n < n1; n += blockDim.x) {
     float y[3] = {0.f, 0.f, 0.f};
-    if (!A.fixed[n]) {
-      int l = geo_layer(geo, n);
-      int r0 = n - geo.off[l], nx = geo.nx[l], ny = geo.ny[l];
-      int j = r0 / nx, i = r0 - j * nx;
-      const float* p = A.p;
-      float b[6], xm[3], t[3];
-      auto acc = [&](int m) {
-        xm[0] = p[m];
-        xm[1] = p[Np + m];
-        xm[2] = p[2 * Np + m];
-        sym_mul(b, xm, t);
-        y[0] += t[0];
-        y[1] += t[1];
-        y[2] += t[2];
-      };
-      auto accs = [&](float s, int m) {
-        y[0] += s * p[m];
-        y[1] += s * p[Np + m];
-        y[2] += s * p[2 * Np + m];
-      };
-      blk_load6(op, Np, F_C, n, b);
-      acc(n);
-      const bool e = i + 1 < nx, w = i > 0, no = j + 1 < ny, so = j > 0;
-      if (e) { blk_load6(op, Np, F_E, n, b); acc(n + 1); }
-      if (w) { blk_load6(op, Np, F_E, n - 1, b); acc(n - 1); }
-      if (no) { blk_load6(op, Np, F_N, n, b); acc(n + nx); }
-      if (so) { blk_load6(op, Np, F_N, n - nx, b); acc(n - nx); }
-      if (e && no) { blk_load6(op, Np, F_NE, n, b); acc(n + nx + 1); }
-      if (w && so) { blk_load6(op, Np, F_NE, n - nx - 1, b); acc(n - nx - 1); }
-      if (w && no) { blk_load6(op, Np, F_NW, n, b); acc(n + nx - 1); }
-      if (e && so) { blk_load6(op, Np, F_NW, n - nx + 1, b); acc(n - nx + 1); }
-      if (i + 2 < nx) accs(op[F_EE * Np + n], n + 2);
-      if (i >= 2) accs(op[F_EE * Np + n - 2], n - 2);
-      if (j + 2 < ny) accs(op[F_NN * Np + n], n + 2 * nx);
-      if (j >= 2) accs(op[F_NN * Np + n - 2 * nx], n - 2 * nx);
-    }
+    const int l = geo_layer(geo, n);
+    const int r0 = n - geo.off[l], nx = geo.nx[l], ny = geo.ny[l];
+    const int j = r0 / nx, i = r0 - j * nx;
+    auto acc = [&](bool in, int base, int src, int m) {
+      float b[6], t[3];
+      blk_load6(op, Np, base, in ? src : n, b);
+      const int mm = in ? m : n;
+      const float xm[3] = {p[mm], p[Np + mm], p[2 * Np + mm]};
+      sym_mul(b, xm, t);
+      const float w = in ? 1.f : 0.f;
+      y[0] += w * t[0];
+      y[1] += w * t[1];
+      y[2] += w * t[2];
+    };
+    auto accs = [&](bool in, int plane, int src, int m) {
+      const float sc = in ? op[plane * Np + src] : 0.f;
+      const int mm = in ? m : n;
+      y[0] += sc * p[mm];
+      y[1] += sc * p[Np + mm];
+      y[2] += sc * p[2 * Np + mm];
+    };
+    const bool e = i + 1 < nx, w = i > 0, no = j + 1 < ny, so = j > 0;
+    acc(true, F_C, n, n);
+    acc(e, F_E, n, n + 1);
+    acc(w, F_E, n - 1, n - 1);
+    acc(no, F_N, n, n + nx);
+    acc(so, F_N, n - nx, n - nx);
+    acc(e && no, F_NE, n, n + nx + 1);
+    acc(w && so, F_NE, n - nx - 1, n - nx - 1);
+    acc(w && no, F_NW, n, n + nx - 1);
+    acc(e && so, F_NW, n - nx + 1, n - nx + 1);
+    accs(i + 2 < nx, F_EE, n, n + 2);
+    accs(i >= 2, F_EE, n - 2, n - 2);
+    accs(j + 2 < ny, F_NN, n, n + 2 * nx);
+    accs(j >= 2, F_NN, n - 2 * nx, n - 2 * nx);
+    if (A.fixed[n]) y[0] = y[1] = y[2] = 0.f;
     for (int q = 0; q < 3; q++) {
       A.ap[q * Np + n] = y[q];
-      v[0] += (double)A.p[q * Np + n] * (double)y[q];
+      v[0] += (double)p[q * Np + n] * (double)y[q];
     }
   }
   double tot[1];
@@ -787,6 +793,12 @@ void pcg_solve(Solver& s) {
 }
 
 void launch_record(Solver& s, int it) { k_record<<<1, 1, 0, s.stream>>>(s.scal, it); }
+
+__global__ void k_open_gate(Scal* s) { s->done = 0; }
+
+// Clears the gate of the V-cycle kernels with a kernel on the solver's stream (ordered like every
+// other launch, and capturable), so that a V-cycle, sweep or residual run outside PCG does its work.
+void launch_open_gate(Solver& s) { k_open_gate<<<1, 1, 0, s.stream>>>(s.scal); }
 
 void launch_reset_counters(Solver& s) { k_reset_counters<<<1, 1, 0, s.stream>>>(s.scal); }
 
