@@ -31,7 +31,7 @@ sys.path.insert(0, str(ROOT / "break-that-denim_amd"))
 
 HBM_PEAK_GBS = 8000.0
 MIXED_COARSEST_NODES = 1100
-TRAFFIC_FILE = ROOT / "profiles" / "traffic_sheet1024_smoother.md"
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_sheet1024_solver_kernels.md"
 
 
 def measured_traffic(kernel_class, size):
