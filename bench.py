@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default smoother kernel per level")
     ap.add_argument("--method", choices=("both", "mixed", "plain"), default="both")
     ap.add_argument("--sweeps", type=int, default=1, help="pre- and post-smoothing sweeps per level")
+    ap.add_argument("--mixed-precision", choices=("fp16", "fp32"), default="fp16",
+                    help="operator storage of the mixed method's coarse levels (fine level and plain method: fp32)")
     ap.add_argument("--smoother-probe", type=int, default=0, metavar="SWEEPS",
                     help="only run SWEEPS forward sweeps of every smoothed level of the mixed method's hierarchy "
                          "(after one frame and the autotuner), outside PCG: the smoother passes for PMC counters")
@@ -145,6 +147,8 @@ def run_method(args, cfg, coarse_solve, profile_frames):
     # the mixed method stops coarsening at 33 x 33 and solves that level exactly (FP64)
     params.max_coarsest_nodes = MIXED_COARSEST_NODES if coarse_solve else 400
     params.smooth_sweeps = args.sweeps
+    from bd.capi import BD_PRECISION_FP16, BD_PRECISION_FP32
+    params.precision = BD_PRECISION_FP16 if coarse_solve and args.mixed_precision == "fp16" else BD_PRECISION_FP32
     gpu = GpuSolver(cfg["scene"], params, cfg["grips"])
     goals = np.array([g["goal"] for g in cfg["grips"]], dtype=np.float64)
     vel = np.array(cfg["velocity"], dtype=np.float64)
@@ -170,7 +174,7 @@ def run_method(args, cfg, coarse_solve, profile_frames):
     gpu.sync()
     t1 = time.perf_counter()
     out = {"ms": 1e3 * (t1 - t0) / args.steps, "iters": float(np.mean(iters)), "newton": float(np.mean(newton)),
-           "levels": gpu.stats()["level_nodes"], "tuned": tuned,
+           "levels": gpu.stats()["level_nodes"], "tuned": tuned, "precision": params.precision,
            "sweeps": params.smooth_sweeps}
     gpu.profile_enable(True)
     for _ in range(profile_frames):
@@ -240,7 +244,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 (FP64 dots, residual replacement, line search" +
-                 (", coarsest solve)" if main_name == "mixed" else ")"),
+                 (", coarsest solve; fp16 coarse-level operators in the V-cycle)" if main_name == "mixed" and R["precision"]
+                  else ", coarsest solve)" if main_name == "mixed" else ")"),
         "data": "synthetic",
         "config": {"workload": f"sheet{args.size}x{args.size} FP32 MG-PCG to rel-res 1e-6, "
                                f"central {args.size // 4}x{args.size // 4} breakable yarn region, 2 grips at 5 mm/step",
@@ -249,7 +254,7 @@ def main():
         "pcg_iterations_per_frame": iters_per_frame,
         "pcg_iterations_per_newton": round(iters_per_frame / max(newton_per_frame, 1), 2),
         "methods": {k: {"ms_per_frame": round(v["ms"], 3), "pcg_iterations_per_frame": v["iters"],
-                        "smooth_sweeps": v["sweeps"],
+                        "smooth_sweeps": v["sweeps"], "vcycle_operators": "fp16" if v["precision"] else "fp32",
                         "smoother_kernels": {str(lv): list(c) for lv, c in v["tuned"].items()}}
                     for k, v in runs.items()},
         "iteration_ratio_plain_over_mixed": round(runs["plain"]["iters"] / runs["mixed"]["iters"], 3) if both else None,

@@ -20,6 +20,7 @@ BD_MAX_LEVELS = 12
 BD_MAX_NEWTON = 8
 BD_MAX_GRIPS = 16
 BD_PRECISION_FP32 = 0
+BD_PRECISION_FP16 = 1
 BD_COARSE_SWEEPS = 0
 BD_COARSE_EXACT = 1
 BD_SMOOTHER_DEFAULT = -1

@@ -78,6 +78,11 @@ struct Level {
   // pass, 18 planes (D 6, U 9, r 3) of rows_cap = lines per colour x longest line
   float* rows = nullptr;
   size_t rows_cap = 0;
+  // BD_PRECISION_FP16: op (and opT) as FP16 pairs, comps / 2 u32 planes (oprd.h); hs[0] the scale,
+  // hs[1] its inverse, hs[2] the bits of the largest free diagonal entry
+  uint32_t* oph = nullptr;
+  uint32_t* ophT = nullptr;
+  float* hs = nullptr;
 };
 
 // longest line the staged smoother kernel takes by default; longer lines use the direct kernel

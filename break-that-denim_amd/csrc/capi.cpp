@@ -174,6 +174,9 @@ void destroy_all(Solver& s) {
     dfree(L.fixedT);
     dfree(L.xT);
     dfree(L.rows);
+    dfree(L.oph);
+    dfree(L.ophT);
+    dfree(L.hs);
   }
   if (s.scal_h) (void)hipHostFree(s.scal_h);
   if (s.grip_goal_h) (void)hipHostFree(s.grip_goal_h);
@@ -322,6 +325,10 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
   }
   if (prm->coarse_solve != BD_COARSE_SWEEPS && prm->coarse_solve != BD_COARSE_EXACT) {
     g_create_error = "bd_create: unknown coarse_solve";
+    return BD_ERR_INVALID;
+  }
+  if (prm->precision != BD_PRECISION_FP32 && prm->precision != BD_PRECISION_FP16) {
+    g_create_error = "bd_create: unknown precision";
     return BD_ERR_INVALID;
   }
   if (prm->smooth_sweeps < 1 || prm->smooth_sweeps > 8) {
@@ -482,6 +489,14 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
       }
       build_lines(s, L, kLineColors);
       if (L.max_len > kStagedMaxLen) smooth_alloc_rows(s, L);
+      // FP16 on the Galerkin levels only: FP16 fine operators cost 15% more PCG iterations at 1024^2, the
+      // coarse ones none (and save 4% of the frame)
+      if (s.P.precision == BD_PRECISION_FP16 && !L.fine) {
+        const size_t words = (size_t)(L.fine ? kFineComps : kCoarseComps) / 2 * L.Np;
+        L.oph = dalloc<uint32_t>(words);
+        if (L.dual) L.ophT = dalloc<uint32_t>(words);
+        L.hs = dalloc<float>(4);
+      }
     }
     smooth_set_attributes();
     if (s.P.coarse_solve == BD_COARSE_EXACT) coarse_exact_alloc(s);

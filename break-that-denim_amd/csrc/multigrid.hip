@@ -7,6 +7,7 @@
 #include <stdexcept>
 
 #include "bd_internal.h"
+#include "oprd.h"
 
 namespace bd {
 
@@ -94,8 +95,62 @@ __device__ __forceinline__ void apply_row(const LvArgs& a, const float* x, int n
   });
 }
 
-template <bool FINE>
-__global__ __launch_bounds__(256) void k_residual(LvArgs a, const float* x, const float* b, float* r, const int* gate) {
+// y = A x for row n in the V-cycle, through an operator reader (FP32 or FP16 planes). Branch-free:
+// absent neighbours read row n itself and are masked, so every load of the row issues at once. The
+// couplings are summed in the order of for_couplings.
+template <bool FINE, class R>
+__device__ __forceinline__ void vc_apply_row(const R& rd, const Geo& g, const float* x, int Np, int n, float y[3]) {
+  int l, i, j;
+  node_ij(g, n, l, i, j);
+  const int nx = g.nx[l], ny = g.ny[l];
+  float c[6], xv[3] = {x[n], x[Np + n], x[2 * Np + n]};
+  rd.blk(F_C, n, c);
+  sym_mul(c, xv, y);
+  auto one = [&](bool in, int base, int src, int m) {
+    float b[6], t[3];
+    rd.blk(base, in ? src : n, b);
+    const int mm = in ? m : n;
+    const float xm[3] = {x[mm], x[Np + mm], x[2 * Np + mm]};
+    sym_mul(b, xm, t);
+    const float w = in ? 1.f : 0.f;
+    y[0] += w * t[0];
+    y[1] += w * t[1];
+    y[2] += w * t[2];
+  };
+  auto ones = [&](bool in, int plane, int src, int m) {
+    const float s = in ? rd.scalar(plane, src) : 0.f;
+    const int mm = in ? m : n;
+    y[0] += s * x[mm];
+    y[1] += s * x[Np + mm];
+    y[2] += s * x[2 * Np + mm];
+  };
+  if (FINE) {
+    const bool e = i + 1 < nx, w = i > 0, no = j + 1 < ny, so = j > 0;
+    one(e, F_E, n, n + 1);
+    one(w, F_E, n - 1, n - 1);
+    one(no, F_N, n, n + nx);
+    one(so, F_N, n - nx, n - nx);
+    one(e && no, F_NE, n, n + nx + 1);
+    one(w && so, F_NE, n - nx - 1, n - nx - 1);
+    one(w && no, F_NW, n, n + nx - 1);
+    one(e && so, F_NW, n - nx + 1, n - nx + 1);
+    ones(i + 2 < nx, F_EE, n, n + 2);
+    ones(i >= 2, F_EE, n - 2, n - 2);
+    ones(j + 2 < ny, F_NN, n, n + 2 * nx);
+    ones(j >= 2, F_NN, n - 2 * nx, n - 2 * nx);
+  } else {
+#pragma unroll
+    for (int k = 1; k < kCoarseBlocks; k++) {
+      const int di = coarse_di(k), dj = coarse_dj(k);
+      const int mf = n + dj * nx + di, mb = n - dj * nx - di;
+      one(i + di >= 0 && i + di < nx && j + dj < ny, 6 * k, n, mf);
+      one(i - di >= 0 && i - di < nx && j - dj >= 0, 6 * k, mb, mb);
+    }
+  }
+}
+
+template <bool FINE, class R>
+__global__ __launch_bounds__(256) void k_residual(LvArgs a, R rd, const float* x, const float* b, float* r, const int* gate) {
   if (gate && *gate) return;
   int n = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= a.N) return;
@@ -105,10 +160,56 @@ __global__ __launch_bounds__(256) void k_residual(LvArgs a, const float* x, cons
     return;
   }
   float y[3];
-  apply_row<FINE>(a, x, n, y);
+  vc_apply_row<FINE>(rd.live(), a.geo, x, Np, n, y);
   r[n] = b[n] - y[0];
   r[Np + n] = b[Np + n] - y[1];
   r[2 * Np + n] = b[2 * Np + n] - y[2];
+}
+
+// The FP16 scale of a level: the largest diagonal entry of a free row (it bounds every entry of the
+// symmetric positive definite operator), as float bits through atomicMax (non-negative floats order
+// like their bits).
+__global__ __launch_bounds__(256) void k_half_max(const float* op, const uint8_t* fixed, int N, int Np, float* hs) {
+  __shared__ float red[256];
+  float m = 0.f;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x)
+    if (!fixed[n]) m = fmaxf(m, fmaxf(fabsf(op[n]), fmaxf(fabsf(op[3 * Np + n]), fabsf(op[5 * Np + n]))));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(reinterpret_cast<unsigned*>(hs + 2), __float_as_uint(red[0]));
+}
+
+// scale 2^e with the largest free diagonal entry in (2^13, 2^14] (1 when the level has none)
+__device__ __forceinline__ float half_scale(const float* hs) {
+  const float mx = __uint_as_float(*reinterpret_cast<const unsigned*>(hs + 2));
+  if (!(mx > 0.f)) return 1.f;
+  int ex;
+  const float fr = frexpf(mx, &ex);  // mx = fr 2^ex, fr in [0.5, 1)
+  const int c = fr == 0.5f ? ex - 1 : ex;  // ceil(log2 mx)
+  return ldexpf(1.f, 14 - c);
+}
+
+// planes 2w and 2w + 1 of a level operator -> word w (FP16, round to nearest even), times the scale
+__global__ __launch_bounds__(256) void k_pack_half(const float* op, int words, int Np, float* hs, uint32_t* out) {
+  const float sc = half_scale(hs);
+  const size_t total = (size_t)words * Np;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    hs[0] = sc;
+    hs[1] = 1.f / sc;
+  }
+  if (t >= total) return;
+  const size_t w = t / Np, n = t - w * Np;
+  // saturated at the largest finite half: a fixed row (outside the scale's maximum) never becomes inf,
+  // so a masked product with it stays zero
+  const float a = fminf(fmaxf(op[(2 * w) * Np + n] * sc, -65504.f), 65504.f);
+  const float b = fminf(fmaxf(op[(2 * w + 1) * Np + n] * sc, -65504.f), 65504.f);
+  const _Float16 lo = (_Float16)a, hi = (_Float16)b;
+  out[t] = (uint32_t)__builtin_bit_cast(uint16_t, lo) | ((uint32_t)__builtin_bit_cast(uint16_t, hi) << 16);
 }
 
 // ---------------------------------------------------------------- prolongation (lattice.ts:13-41)
@@ -398,13 +499,25 @@ void launch_zero(Solver& s, float* p, size_t n) {
   k_zero<<<(unsigned)((n + 255) / 256), 256, 0, s.stream>>>(p, n);
 }
 
+// Refreshes a level operator's derived copies once it is final (after assembly or RAP): the
+// column-major copy of dual-layout levels and, under BD_PRECISION_FP16, the FP16 pairs of both.
 void launch_colmajor_op(Solver& s, int k) {
   Level& L = s.lv[k];
-  if (!L.dual) return;
   const int comps = L.fine ? kFineComps : kCoarseComps;
-  ProfScope ps(s, PC_TRANSPOSE, 2.0 * L.N * (4.0 * comps + 1.0));
-  to_colmajor<float>(s, L, L.op, L.opT, comps);
-  to_colmajor<uint8_t>(s, L, L.fixed, L.fixedT, 1);
+  if (L.dual) {
+    ProfScope ps(s, PC_TRANSPOSE, 2.0 * L.N * (4.0 * comps + 1.0));
+    to_colmajor<float>(s, L, L.op, L.opT, comps);
+    to_colmajor<uint8_t>(s, L, L.fixed, L.fixedT, 1);
+  }
+  if (L.oph) {
+    const int words = comps / 2;
+    const size_t total = (size_t)words * L.Np;
+    ProfScope ps(s, PC_TRANSPOSE, (double)total * (L.dual ? 24.0 : 12.0));
+    launch_zero(s, L.hs, 4);
+    k_half_max<<<std::min(1024, grid_for(L.N, 256)), 256, 0, s.stream>>>(L.op, L.fixed, L.N, L.Np, L.hs);
+    k_pack_half<<<(unsigned)((total + 255) / 256), 256, 0, s.stream>>>(L.op, words, L.Np, L.hs, L.oph);
+    if (L.dual) k_pack_half<<<(unsigned)((total + 255) / 256), 256, 0, s.stream>>>(L.opT, words, L.Np, L.hs, L.ophT);
+  }
 }
 
 void launch_colmajor_x(Solver& s, int k) {
@@ -435,10 +548,21 @@ void launch_residual(Solver& s, int k, const float* x, const float* b, float* r)
   Level& L = s.lv[k];
   // unique bytes: the node's forward planes (32 fine, 78 coarse) + fixed flag + x, b read and r written
   ProfScope ps(s, L.fine ? PC_RESIDUAL_FINE : PC_RESIDUAL_COARSE, L.N * (L.fine ? 165.0 : 349.0));
-  if (L.fine)
-    k_residual<true><<<grid_for(L.N, 256), 256, 0, s.stream>>>(lv_args(L), x, b, r, pcg_gate(s));
-  else
-    k_residual<false><<<grid_for(L.N, 256), 256, 0, s.stream>>>(lv_args(L), x, b, r, pcg_gate(s));
+  const unsigned g = grid_for(L.N, 256);
+  if (L.oph) {
+    // the FP16 pairs; the level's inverse scale is read from device memory by the kernel
+    const OpF16Dev rd{L.oph, L.Np, L.hs};
+    if (L.fine)
+      k_residual<true><<<g, 256, 0, s.stream>>>(lv_args(L), rd, x, b, r, pcg_gate(s));
+    else
+      k_residual<false><<<g, 256, 0, s.stream>>>(lv_args(L), rd, x, b, r, pcg_gate(s));
+  } else {
+    const OpF32 rd{L.op, L.Np};
+    if (L.fine)
+      k_residual<true><<<g, 256, 0, s.stream>>>(lv_args(L), rd, x, b, r, pcg_gate(s));
+    else
+      k_residual<false><<<g, 256, 0, s.stream>>>(lv_args(L), rd, x, b, r, pcg_gate(s));
+  }
 }
 
 void launch_restrict(Solver& s, int k) {

@@ -23,6 +23,7 @@
 // u-lines. Each pass writes its solution to both copies (the other one with a 12-byte strided
 // store per row) and reads b from the row-major vector.
 #include "bd_internal.h"
+#include "oprd.h"
 
 namespace bd {
 
@@ -117,17 +118,15 @@ __device__ constexpr Cpl kFine[12] = {
 
 // Gathers D (6), U (9: +1 along coupling, full), r (3) of row n on a line of family FAM.
 // All loads use clamped in-bounds indices and are masked afterwards, so they issue together.
-template <bool FINE, int FAM, bool DUAL>
-__device__ __forceinline__ void gather_row(const SmoothArgs& S, int n, int nb, int i, int j, int nx, int ny, float D[6],
-                                           float U[9], float r[3], bool& fixed_n) {
+template <bool FINE, int FAM, bool DUAL, class RD>
+__device__ __forceinline__ void gather_row(const SmoothArgs& S, const RD& rd, int n, int nb, int i, int j, int nx, int ny,
+                                           float D[6], float U[9], float r[3], bool& fixed_n) {
   const int Np = S.Np;
-  const float* op = S.op;
   const float* x = S.x;
   const int si = stride_i<FAM, DUAL>(nx, ny), sj = stride_j<FAM, DUAL>(nx, ny);
   fixed_n = S.fixed[n] != 0;
   float c[6];
-#pragma unroll
-  for (int q = 0; q < 6; q++) c[q] = op[q * Np + n];
+  rd.blk(F_C, n, c);
   const float xn0 = x[n], xn1 = x[Np + n], xn2 = x[2 * Np + n];
   r[0] = S.b[nb];
   r[1] = S.b[Np + nb];
@@ -141,11 +140,10 @@ __device__ __forceinline__ void gather_row(const SmoothArgs& S, int n, int nb, i
     const int src = in ? n + sdj * sj + sdi * si : n;
     float blk[6];
     if (scalar) {
-      float s = op[base * Np + src];
+      float s = rd.scalar(base, src);
       blk[0] = s; blk[1] = 0.f; blk[2] = 0.f; blk[3] = s; blk[4] = 0.f; blk[5] = s;
     } else {
-#pragma unroll
-      for (int q = 0; q < 6; q++) blk[q] = op[(base + q) * Np + src];
+      rd.blk(base, src, blk);
     }
     const float xm[3] = {x[m], x[Np + m], x[2 * Np + m]};
     // the fixed-neighbour mask is arithmetic: a branch on the loaded flag would split the gather
@@ -203,12 +201,11 @@ __device__ __forceinline__ void gather_row(const SmoothArgs& S, int n, int nb, i
 // also loads the row's centre block, b, x and fixed flag. Writes the partial rhs correction and
 // majorisation to LDS; the part owning the +1 along-line coupling writes U. Lanes past the line's
 // end (valid false) gather a clamped row and write nothing.
-template <bool FINE, int FAM, int G, int GI, bool DUAL>
-__device__ __forceinline__ void gather_part(const SmoothArgs& S, int n, int nb, int i, int j, int nx, int ny, bool valid,
-                                            int row, int len, float* pr, float* pm, float* gU, float* gD, float* gR,
-                                            float* gX, uint8_t* gF) {
+template <bool FINE, int FAM, int G, int GI, bool DUAL, class RD>
+__device__ __forceinline__ void gather_part(const SmoothArgs& S, const RD& rd, int n, int nb, int i, int j, int nx, int ny,
+                                            bool valid, int row, int len, float* pr, float* pm, float* gU, float* gD,
+                                            float* gR, float* gX, uint8_t* gF) {
   const int Np = S.Np;
-  const float* op = S.op;
   const float* x = S.x;
   const int si = stride_i<FAM, DUAL>(nx, ny), sj = stride_j<FAM, DUAL>(nx, ny);
   float r0 = 0.f, r1 = 0.f, r2 = 0.f, maj = 0.f;
@@ -220,11 +217,10 @@ __device__ __forceinline__ void gather_part(const SmoothArgs& S, int n, int nb, 
     const int src = in ? n + sdj * sj + sdi * si : n;
     float blk[6];
     if (scalar) {
-      float s = op[base * Np + src];
+      float s = rd.scalar(base, src);
       blk[0] = s; blk[1] = 0.f; blk[2] = 0.f; blk[3] = s; blk[4] = 0.f; blk[5] = s;
     } else {
-#pragma unroll
-      for (int q = 0; q < 6; q++) blk[q] = op[(base + q) * Np + src];
+      rd.blk(base, src, blk);
     }
     const float xm[3] = {x[m], x[Np + m], x[2 * Np + m]};
     const uint8_t fm = S.fixed[m];  // m is clamped in bounds: the load is unconditional
@@ -264,8 +260,7 @@ __device__ __forceinline__ void gather_part(const SmoothArgs& S, int n, int nb, 
   float c[6], bb[3], xx[3];
   uint8_t f = 0;
   if (GI == 0) {
-#pragma unroll
-    for (int q = 0; q < 6; q++) c[q] = op[q * Np + n];
+    rd.blk(F_C, n, c);
 #pragma unroll
     for (int q = 0; q < 3; q++) {
       bb[q] = S.b[q * Np + nb];
@@ -295,15 +290,16 @@ __device__ __forceinline__ void gather_part(const SmoothArgs& S, int n, int nb, 
 }
 
 // Runs part g (wave-uniform) of the gather.
-template <bool FINE, int FAM, int G, bool DUAL, int GI = 0>
-__device__ __forceinline__ void gather_dispatch(int g, const SmoothArgs& S, int n, int nb, int i, int j, int nx, int ny,
-                                                bool valid, int row, int len, float* pr, float* pm, float* gU, float* gD,
-                                                float* gR, float* gX, uint8_t* gF) {
+template <bool FINE, int FAM, int G, bool DUAL, class RD, int GI = 0>
+__device__ __forceinline__ void gather_dispatch(int g, const SmoothArgs& S, const RD& rd, int n, int nb, int i, int j, int nx,
+                                                int ny, bool valid, int row, int len, float* pr, float* pm, float* gU,
+                                                float* gD, float* gR, float* gX, uint8_t* gF) {
   if constexpr (GI < G) {
     if (g == GI)
-      gather_part<FINE, FAM, G, GI, DUAL>(S, n, nb, i, j, nx, ny, valid, row, len, pr, pm, gU, gD, gR, gX, gF);
+      gather_part<FINE, FAM, G, GI, DUAL>(S, rd, n, nb, i, j, nx, ny, valid, row, len, pr, pm, gU, gD, gR, gX, gF);
     else
-      gather_dispatch<FINE, FAM, G, DUAL, GI + 1>(g, S, n, nb, i, j, nx, ny, valid, row, len, pr, pm, gU, gD, gR, gX, gF);
+      gather_dispatch<FINE, FAM, G, DUAL, RD, GI + 1>(g, S, rd, n, nb, i, j, nx, ny, valid, row, len, pr, pm, gU, gD, gR,
+                                                      gX, gF);
   }
 }
 
@@ -548,9 +544,10 @@ __device__ __forceinline__ void store_x(const SmoothArgs& S, int n, int no, cons
 }
 
 // Long lines: each thread gathers its two rows straight into registers.
-template <bool FINE, int FAM, bool DUAL>
-__global__ __launch_bounds__(512, 4) void k_smooth_direct(SmoothArgs S) {
+template <bool FINE, int FAM, bool DUAL, class RDD>
+__global__ __launch_bounds__(512, 4) void k_smooth_direct(SmoothArgs S, RDD rdd) {
   if (S.gate && *S.gate) return;
+  const auto rd = rdd.live();
   extern __shared__ __align__(16) float sm[];
   const LineGeo L = line_geo<FAM, DUAL>(S);
   const int len = L.ln.z;
@@ -562,11 +559,11 @@ __global__ __launch_bounds__(512, 4) void k_smooth_direct(SmoothArgs S) {
   float Da[6], Ua[9], ra[3], Db[6], Ub[9], rb[3];
   bool fa = true, fb = true;
   if (ka < len)
-    gather_row<FINE, FAM, DUAL>(S, na, nba, ia, ja, L.nx, L.ny, Da, Ua, ra, fa);
+    gather_row<FINE, FAM, DUAL>(S, rd, na, nba, ia, ja, L.nx, L.ny, Da, Ua, ra, fa);
   else
     set_identity(Da, Ua, ra);
   if (kb < len)
-    gather_row<FINE, FAM, DUAL>(S, nb, nbb, ib, jb, L.nx, L.ny, Db, Ub, rb, fb);
+    gather_row<FINE, FAM, DUAL>(S, rd, nb, nbb, ib, jb, L.nx, L.ny, Db, Ub, rb, fb);
   else
     set_identity(Db, Ub, rb);
   float xa[3], xb[3];
@@ -577,9 +574,10 @@ __global__ __launch_bounds__(512, 4) void k_smooth_direct(SmoothArgs S) {
 
 // Short lines: every row's couplings are split over G threads so that each thread issues about
 // one round trip of loads; rows are assembled in LDS, then solved by the first ceil(len/2) threads.
-template <bool FINE, int FAM, int G, bool DUAL>
-__global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S) {
+template <bool FINE, int FAM, int G, bool DUAL, class RDD>
+__global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S, RDD rdd) {
   if (S.gate && *S.gate) return;
+  const auto rd = rdd.live();
   extern __shared__ __align__(16) float sm[];
   const LineGeo L = line_geo<FAM, DUAL>(S);
   const int len = L.ln.z;
@@ -604,7 +602,7 @@ __global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S) {
     const int row = valid ? row0 : len - 1;
     int n, nb, no, i, j;
     row_index<FAM, DUAL>(L, row, n, nb, no, i, j);
-    gather_dispatch<FINE, FAM, G, DUAL>(g, S, n, nb, i, j, L.nx, L.ny, valid, row, len, pr, pm, gU, gD, gR, gX, gF);
+    gather_dispatch<FINE, FAM, G, DUAL>(g, S, rd, n, nb, i, j, L.nx, L.ny, valid, row, len, pr, pm, gU, gD, gR, gX, gF);
   }
   __syncthreads();
   for (int row = threadIdx.x; row < len; row += T) {
@@ -672,9 +670,10 @@ __global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S) {
 // Split variant, first kernel: one thread per row of the pass's lines (blockIdx.y the line, x the
 // 256-row chunk), so that the whole chip gathers with full occupancy and no line's workgroup waits
 // on another's CU; writes the row's D, U and r to the rows planes (line slot * stride + row).
-template <bool FINE, int FAM, bool DUAL>
-__global__ __launch_bounds__(256) void k_line_rows(SmoothArgs S, float* rows, size_t cap, int stride) {
+template <bool FINE, int FAM, bool DUAL, class RDD>
+__global__ __launch_bounds__(256) void k_line_rows(SmoothArgs S, RDD rdd, float* rows, size_t cap, int stride) {
   if (S.gate && *S.gate) return;
+  const auto rd = rdd.live();
   const int slot = blockIdx.y;
   const LineGeo L = line_geo_at<FAM, DUAL>(S, slot);
   const int k = blockIdx.x * 256 + threadIdx.x;
@@ -683,7 +682,7 @@ __global__ __launch_bounds__(256) void k_line_rows(SmoothArgs S, float* rows, si
   row_index<FAM, DUAL>(L, k, n, nb, no, i, j);
   float D[6], U[9], r[3];
   bool f;
-  gather_row<FINE, FAM, DUAL>(S, n, nb, i, j, L.nx, L.ny, D, U, r, f);
+  gather_row<FINE, FAM, DUAL>(S, rd, n, nb, i, j, L.nx, L.ny, D, U, r, f);
   float* o = rows + (size_t)slot * stride + k;
 #pragma unroll
   for (int q = 0; q < 6; q++) o[q * cap] = D[q];
@@ -741,11 +740,11 @@ constexpr int kMaxLds = 160 * 1024;
 
 // variant: -1 the default rule below, 0 the direct kernel, 4 or 8 the staged kernel with that gather
 // width (bd_autotune picks per level and family)
-template <bool FINE, int FAM, bool DUAL>
-void launch_one(const SmoothArgs& S, int count, int len, hipStream_t st, int variant, float* rows, size_t cap) {
+template <bool FINE, int FAM, bool DUAL, class RDD>
+void launch_one(const SmoothArgs& S, RDD rdd, int count, int len, hipStream_t st, int variant, float* rows, size_t cap) {
   if (variant == kSplitVariant) {
     if (!rows || (size_t)count * len > cap) throw std::runtime_error("smoother: split variant without room for its rows");
-    k_line_rows<FINE, FAM, DUAL><<<dim3((len + 255) / 256, count), 256, 0, st>>>(S, rows, cap, len);
+    k_line_rows<FINE, FAM, DUAL, RDD><<<dim3((len + 255) / 256, count), 256, 0, st>>>(S, rdd, rows, cap, len);
     const int block = ((len + 1) / 2 + 63) / 64 * 64;
     k_smooth_rows<FINE, FAM, DUAL><<<count, block, (size_t)kSolveFloats * block * sizeof(float), st>>>(S, rows, cap, len);
     return;
@@ -755,9 +754,9 @@ void launch_one(const SmoothArgs& S, int count, int len, hipStream_t st, int var
     const int R2 = (len + 1) / 2;
     const int threads = std::max(std::min(1024, ((len * G + 63) / 64) * 64), ((R2 + 63) / 64) * 64);
     if (G == 8)
-      k_smooth_staged<FINE, FAM, 8, DUAL><<<count, threads, staged_shm<FINE, FAM, 8>(len), st>>>(S);
+      k_smooth_staged<FINE, FAM, 8, DUAL, RDD><<<count, threads, staged_shm<FINE, FAM, 8>(len), st>>>(S, rdd);
     else
-      k_smooth_staged<FINE, FAM, 4, DUAL><<<count, threads, staged_shm<FINE, FAM, 4>(len), st>>>(S);
+      k_smooth_staged<FINE, FAM, 4, DUAL, RDD><<<count, threads, staged_shm<FINE, FAM, 4>(len), st>>>(S, rdd);
     return;
   }
   if (variant < 0 && len <= kStagedMaxLen) {
@@ -767,48 +766,61 @@ void launch_one(const SmoothArgs& S, int count, int len, hipStream_t st, int var
     const int R2 = (len + 1) / 2;
     threads = std::max(threads, ((R2 + 63) / 64) * 64);
     if (G == 8)
-      k_smooth_staged<FINE, FAM, 8, DUAL><<<count, threads, staged_shm<FINE, FAM, 8>(len), st>>>(S);
+      k_smooth_staged<FINE, FAM, 8, DUAL, RDD><<<count, threads, staged_shm<FINE, FAM, 8>(len), st>>>(S, rdd);
     else
-      k_smooth_staged<FINE, FAM, 4, DUAL><<<count, threads, staged_shm<FINE, FAM, 4>(len), st>>>(S);
+      k_smooth_staged<FINE, FAM, 4, DUAL, RDD><<<count, threads, staged_shm<FINE, FAM, 4>(len), st>>>(S, rdd);
     return;
   }
   int R2 = (len + 1) / 2;
   int block = ((R2 + 63) / 64) * 64;
   size_t shm = (size_t)kSolveFloats * block * sizeof(float);
-  k_smooth_direct<FINE, FAM, DUAL><<<count, block, shm, st>>>(S);
+  k_smooth_direct<FINE, FAM, DUAL, RDD><<<count, block, shm, st>>>(S, rdd);
 }
 
-template <bool F, int A, bool D>
+template <bool F, int A, bool D, class R>
 void set_direct_attr(int shm) {
-  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_direct<F, A, D>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_direct<F, A, D, R>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+}
+template <bool F, int A, int G, bool D, class R>
+void set_staged_attr(int shm) {
+  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_staged<F, A, G, D, R>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
+}
+template <bool F, int A, bool D>
+void set_rows_attr(int shm) {
   BD_HIP(hipFuncSetAttribute((const void*)k_smooth_rows<F, A, D>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
 }
-template <bool F, int A, int G, bool D>
-void set_staged_attr(int shm) {
-  BD_HIP(hipFuncSetAttribute((const void*)k_smooth_staged<F, A, G, D>, hipFuncAttributeMaxDynamicSharedMemorySize, shm));
-}
-template <bool D>
+template <bool D, class R>
 void set_attrs(int shm, int shm_staged) {
-  set_direct_attr<true, 0, D>(shm);
-  set_direct_attr<true, 1, D>(shm);
-  set_direct_attr<false, 0, D>(shm);
-  set_direct_attr<false, 1, D>(shm);
-  set_staged_attr<true, 0, 4, D>(shm_staged);
-  set_staged_attr<true, 1, 4, D>(shm_staged);
-  set_staged_attr<true, 0, 8, D>(shm_staged);
-  set_staged_attr<true, 1, 8, D>(shm_staged);
-  set_staged_attr<false, 0, 4, D>(shm_staged);
-  set_staged_attr<false, 1, 4, D>(shm_staged);
-  set_staged_attr<false, 0, 8, D>(shm_staged);
-  set_staged_attr<false, 1, 8, D>(shm_staged);
+  set_direct_attr<true, 0, D, R>(shm);
+  set_direct_attr<true, 1, D, R>(shm);
+  set_direct_attr<false, 0, D, R>(shm);
+  set_direct_attr<false, 1, D, R>(shm);
+  set_staged_attr<true, 0, 4, D, R>(shm_staged);
+  set_staged_attr<true, 1, 4, D, R>(shm_staged);
+  set_staged_attr<true, 0, 8, D, R>(shm_staged);
+  set_staged_attr<true, 1, 8, D, R>(shm_staged);
+  set_staged_attr<false, 0, 4, D, R>(shm_staged);
+  set_staged_attr<false, 1, 4, D, R>(shm_staged);
+  set_staged_attr<false, 0, 8, D, R>(shm_staged);
+  set_staged_attr<false, 1, 8, D, R>(shm_staged);
 }
 
 }  // namespace
 
 void smooth_set_attributes() {
   int shm = kSolveFloats * ((BD_MAX_LINE / 2 + 63) / 64 * 64) * (int)sizeof(float);
-  set_attrs<false>(shm, kMaxLds);
-  set_attrs<true>(shm, kMaxLds);
+  set_attrs<false, OpF32>(shm, kMaxLds);
+  set_attrs<true, OpF32>(shm, kMaxLds);
+  set_attrs<false, OpF16Dev>(shm, kMaxLds);
+  set_attrs<true, OpF16Dev>(shm, kMaxLds);
+  set_rows_attr<true, 0, false>(shm);
+  set_rows_attr<true, 1, false>(shm);
+  set_rows_attr<false, 0, false>(shm);
+  set_rows_attr<false, 1, false>(shm);
+  set_rows_attr<true, 0, true>(shm);
+  set_rows_attr<true, 1, true>(shm);
+  set_rows_attr<false, 0, true>(shm);
+  set_rows_attr<false, 1, true>(shm);
 }
 
 namespace {
@@ -828,23 +840,31 @@ void launch_colour(Solver& s, int k, int c, int variant) {
   // (dual-layout levels write x twice)
   double nodes = (double)count * maxlen;
   ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * ((L.fine ? 268.0 : 636.0) + (L.dual ? 12.0 : 0.0)));
-  if (L.dual) {
-    if (L.fine) {
-      if (fam == 0) launch_one<true, 0, true>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
-      else launch_one<true, 1, true>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+  auto go = [&](auto rdd) {
+    using R = decltype(rdd);
+    if (L.dual) {
+      if (L.fine) {
+        if (fam == 0) launch_one<true, 0, true, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+        else launch_one<true, 1, true, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+      } else {
+        if (fam == 0) launch_one<false, 0, true, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+        else launch_one<false, 1, true, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+      }
     } else {
-      if (fam == 0) launch_one<false, 0, true>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
-      else launch_one<false, 1, true>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+      if (L.fine) {
+        if (fam == 0) launch_one<true, 0, false, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+        else launch_one<true, 1, false, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+      } else {
+        if (fam == 0) launch_one<false, 0, false, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+        else launch_one<false, 1, false, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+      }
     }
-  } else {
-    if (L.fine) {
-      if (fam == 0) launch_one<true, 0, false>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
-      else launch_one<true, 1, false>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
-    } else {
-      if (fam == 0) launch_one<false, 0, false>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
-      else launch_one<false, 1, false>(S, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
-    }
-  }
+  };
+  // the operator in the pass's layout, FP32 or FP16 pairs
+  if (L.oph)
+    go(OpF16Dev{colmajor ? L.ophT : L.oph, L.Np, L.hs});
+  else
+    go(OpF32{colmajor ? L.opT : L.op, L.Np});
 }
 
 // the variants a line of `len` rows can run: the direct kernel, the split one where the level keeps

@@ -145,6 +145,10 @@ typedef struct bd_params {
 #define BD_COARSE_EXACT 1      /* exact FP64 solve: banded Cholesky per hierarchy build, dense inverse applied */
 
 #define BD_PRECISION_FP32 0    /* FP32 operators and smoothers, FP64 dot products */
+#define BD_PRECISION_FP16 1    /* the V-cycle (smoother gathers, its residuals) reads each Galerkin coarse level's
+                                  operator as FP16 scaled by a power of two per level; the fine operator, PCG
+                                  and its matvec, the FP64 residual replacement, line search and the exact
+                                  coarsest solve keep FP32 / FP64 */
 
 /* Quadratic grip springs (solver.ts:25-37 ReferenceGrip, interact.wgsl:159-181 gripTerms). */
 typedef struct bd_grips {

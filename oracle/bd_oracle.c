@@ -1559,7 +1559,46 @@ static void smooth_lines(const op_t* A, double* x, const double* b, int reverse,
 }
 
 /* vcycle (newtonMultigrid.ts:495-528), pre = post = 1 */
+/* FP16 with round to nearest even (10 stored mantissa bits; subnormals in steps of 2^-24), saturated
+ * at the largest finite half, as the GPU's v_cvt_f16_f32 of the saturated value */
+static double fp16_round(double a) {
+  float f = (float)a;
+  if (f > 65504.f) f = 65504.f;
+  if (f < -65504.f) f = -65504.f;
+  if (fabsf(f) < 6.103515625e-05f) return nearbyint((double)f * 16777216.0) / 16777216.0;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0xFFFu + ((u >> 13) & 1u);
+  u &= ~0x1FFFu;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+
+/* BD_PRECISION_FP16 (csrc/oprd.h, k_pack_half): the V-cycle's smoother and residual read a coarse
+ * level's operator rounded to FP16 at the level's scale 2^e, e such that the largest diagonal entry of a free
+ * row lands in (2^13, 2^14]. Returns a rounded copy (caller frees), or NULL in FP32 mode. */
+static double* half_operator(const or_solver* S, const op_t* A) {
+  if (S->P.precision != BD_PRECISION_FP16 || A == &S->lv[0]) return NULL;  /* coarse levels only */
+  const size_t m = (size_t)A->N * 225;
+  double mx = 0;
+  for (int n = 0; n < A->N; n++) {
+    if (A->fixed[n]) continue;
+    const double* c = A->A + (size_t)n * 225 + 12 * 9;
+    mx = fmax(mx, fmax(fabs((double)(float)c[0]), fmax(fabs((double)(float)c[4]), fabs((double)(float)c[8]))));
+  }
+  double sc = 1.0;
+  if (mx > 0) {
+    int ex;
+    const double fr = frexp((double)(float)mx, &ex);
+    sc = ldexp(1.0, 14 - (fr == 0.5 ? ex - 1 : ex));
+  }
+  double* r = malloc(sizeof(double) * m);
+  for (size_t q = 0; q < m; q++) r[q] = fp16_round(A->A[q] * sc) / sc;
+  return r;
+}
+
 static void vcycle(or_solver* S, int k, double* x, const double* b) {
+  op_t Ah = S->lv[k];
   const op_t* A = &S->lv[k];
   int colors = 3;
   if (k == S->nlev - 1) {
@@ -1567,8 +1606,19 @@ static void vcycle(or_solver* S, int k, double* x, const double* b) {
       band_solve(S->cn, S->cbw, S->cL, b, x);
       return;
     }
+    double* h = half_operator(S, A);
+    if (h) {
+      Ah.A = h;
+      A = &Ah;
+    }
     for (int s = 0; s < S->P.coarse_sweeps; s++) smooth_lines(A, x, b, s & 1, colors);
+    free(h);
     return;
+  }
+  double* h = half_operator(S, A);
+  if (h) {
+    Ah.A = h;
+    A = &Ah;
   }
   const prol_t* P = &S->pr[k];
   for (int sw = 0; sw < S->P.smooth_sweeps; sw++) smooth_lines(A, x, b, 0, colors);
@@ -1602,6 +1652,7 @@ static void vcycle(or_solver* S, int k, double* x, const double* b) {
   free(Ax);
   free(rc);
   free(xc);
+  free(h);
 }
 
 static double dotn(const double* a, const double* b, int n) {
@@ -1833,8 +1884,13 @@ int* or_level_pidx(or_solver* S, int k) { return S->pr[k].idx; }
 double* or_level_pw(or_solver* S, int k) { return S->pr[k].w; }
 
 void or_apply(or_solver* S, int k, const double* x, double* y) { op_apply(&S->lv[k], x, y); }
+double or_fp16_round(double a) { return fp16_round(a); }
 void or_smooth(or_solver* S, int k, double* x, const double* b, int reverse) {
-  smooth_lines(&S->lv[k], x, b, reverse, 3);
+  op_t Ah = S->lv[k];
+  double* h = half_operator(S, &S->lv[k]);
+  if (h) Ah.A = h;
+  smooth_lines(&Ah, x, b, reverse, 3);
+  free(h);
 }
 void or_vcycle(or_solver* S, double* x, const double* b) { vcycle(S, 0, x, b); }
 void or_pcg(or_solver* S, const double* b, double* x, int iterations, double rel_tol, int max_iter) { pcg(S, b, x, iterations, rel_tol, max_iter); }

@@ -114,6 +114,8 @@ def lib():
         L.or_level_pw.argtypes = [vp, C.c_int]
         L.or_level_pw.restype = dp
         L.or_apply.argtypes = [vp, C.c_int, dp, dp]
+        L.or_fp16_round.argtypes = [C.c_double]
+        L.or_fp16_round.restype = C.c_double
         L.or_smooth.argtypes = [vp, C.c_int, dp, dp, C.c_int]
         L.or_vcycle.argtypes = [vp, dp, dp]
         L.or_pcg.argtypes = [vp, dp, dp, C.c_int, C.c_double, C.c_int]
@@ -544,3 +546,8 @@ def feasible(cmeta_n, d):
 
 def floor_bound_z(z0, linz, dz, half, mass, gnorm, fz):
     return lib().oc_floor_bound_z(z0, linz, dz, half, mass, gnorm, fz)
+
+
+def fp16_round(a):
+    """The oracle's FP16 rounding of one value (round to nearest even, saturated at 65504)."""
+    return lib().or_fp16_round(float(a))

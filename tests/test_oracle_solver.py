@@ -358,3 +358,18 @@ def test_exact_coarse_solve_is_the_inverse():
         b2.reshape(N2, 3)[o2.fixed(0) == 1] = 0
         out.append(o2.vcycle(b2))
     assert np.max(np.abs(out[1] - out[0])) <= 5e-4 * np.max(np.abs(out[0]))
+
+
+def test_fp16_rounding_matches_ieee_half():
+    """The oracle's FP16 rounding of the V-cycle operators (BD_PRECISION_FP16) is IEEE binary16 round
+    to nearest even (numpy's float32 -> float16 conversion), subnormals included, saturated at the
+    largest finite half instead of overflowing to inf."""
+    rng = np.random.default_rng(11)
+    vals = np.concatenate([
+        rng.standard_normal(4000) * 10.0 ** rng.uniform(-9, 4.8, 4000),
+        np.array([0.0, -0.0, 6.103515625e-05, 6.0e-05, 5.96e-08, 2.98e-08, 2.99e-08, 1.0, 1.0 + 2 ** -11,
+                  1.0 + 3 * 2 ** -11, 2048.0 + 1.0, 2048.0 + 3.0, 65504.0, 65519.0, 70000.0, -1e6]),
+    ])
+    got = np.array([O.fp16_round(v) for v in vals])
+    want = np.clip(vals.astype(np.float32), -65504, 65504).astype(np.float16).astype(np.float64)
+    assert np.array_equal(got, want), vals[got != want][:5]
