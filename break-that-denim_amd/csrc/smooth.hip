@@ -706,21 +706,35 @@ __global__ __launch_bounds__(512, 4) void k_smooth_rows(SmoothArgs S, const floa
   row_index<FAM, DUAL>(L, ka, na, nba, noa, ia, ja);
   row_index<FAM, DUAL>(L, kb, nb, nbb, nob, ib, jb);
   float Da[6], Ua[9], ra[3], Db[6], Ub[9], rb[3];
+  // rows ka and kb are adjacent: one 8-byte load per plane (stride and cap are even)
   const float* o = rows + (size_t)slot * stride;
-  auto load = [&](int k, float D[6], float U[9], float r[3]) {
-    if (k < len) {
+  float v[18][2];
 #pragma unroll
-      for (int q = 0; q < 6; q++) D[q] = o[q * cap + k];
+  for (int q = 0; q < 18; q++) {
+    const float2 p = ka < len ? *reinterpret_cast<const float2*>(o + q * cap + ka) : make_float2(0.f, 0.f);
+    v[q][0] = p.x;
+    v[q][1] = p.y;
+  }
+  if (ka < len) {
 #pragma unroll
-      for (int q = 0; q < 9; q++) U[q] = o[(6 + q) * cap + k];
+    for (int q = 0; q < 6; q++) Da[q] = v[q][0];
 #pragma unroll
-      for (int q = 0; q < 3; q++) r[q] = o[(15 + q) * cap + k];
-    } else {
-      set_identity(D, U, r);
-    }
-  };
-  load(ka, Da, Ua, ra);
-  load(kb, Db, Ub, rb);
+    for (int q = 0; q < 9; q++) Ua[q] = v[6 + q][0];
+#pragma unroll
+    for (int q = 0; q < 3; q++) ra[q] = v[15 + q][0];
+  } else {
+    set_identity(Da, Ua, ra);
+  }
+  if (kb < len) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) Db[q] = v[q][1];
+#pragma unroll
+    for (int q = 0; q < 9; q++) Ub[q] = v[6 + q][1];
+#pragma unroll
+    for (int q = 0; q < 3; q++) rb[q] = v[15 + q][1];
+  } else {
+    set_identity(Db, Ub, rb);
+  }
   const bool fa = ka >= len || S.fixed[na] != 0, fb = kb >= len || S.fixed[nb] != 0;
   float xa[3], xb[3];
   line_solve(sm, blockDim.x, t, len, Da, Ua, ra, Db, Ub, rb, xa, xb);
@@ -743,10 +757,11 @@ constexpr int kMaxLds = 160 * 1024;
 template <bool FINE, int FAM, bool DUAL, class RDD>
 void launch_one(const SmoothArgs& S, RDD rdd, int count, int len, hipStream_t st, int variant, float* rows, size_t cap) {
   if (variant == kSplitVariant) {
-    if (!rows || (size_t)count * len > cap) throw std::runtime_error("smoother: split variant without room for its rows");
-    k_line_rows<FINE, FAM, DUAL, RDD><<<dim3((len + 255) / 256, count), 256, 0, st>>>(S, rdd, rows, cap, len);
+    const int stride = (len + 1) & ~1;  // even, so that a row pair is one aligned float2
+    if (!rows || (size_t)count * stride > cap) throw std::runtime_error("smoother: split variant without room for its rows");
+    k_line_rows<FINE, FAM, DUAL, RDD><<<dim3((len + 255) / 256, count), 256, 0, st>>>(S, rdd, rows, cap, stride);
     const int block = ((len + 1) / 2 + 63) / 64 * 64;
-    k_smooth_rows<FINE, FAM, DUAL><<<count, block, (size_t)kSolveFloats * block * sizeof(float), st>>>(S, rows, cap, len);
+    k_smooth_rows<FINE, FAM, DUAL><<<count, block, (size_t)kSolveFloats * block * sizeof(float), st>>>(S, rows, cap, stride);
     return;
   }
   if (variant == 4 || variant == 8) {
@@ -892,7 +907,7 @@ void smooth_alloc_rows(Solver& s, Level& L) {
   (void)s;
   int most = 0;
   for (int c = 0; c < 2 * L.ncol; c++) most = std::max(most, L.color_start[c + 1] - L.color_start[c]);
-  L.rows_cap = (size_t)most * L.max_len;
+  L.rows_cap = (size_t)most * ((L.max_len + 1) & ~1);
   BD_HIP(hipMalloc(&L.rows, 18 * L.rows_cap * sizeof(float)));
 }
 
