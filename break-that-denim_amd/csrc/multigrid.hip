@@ -139,7 +139,8 @@ __device__ __forceinline__ void vc_apply_row(const R& rd, const Geo& g, const fl
     ones(j + 2 < ny, F_NN, n, n + 2 * nx);
     ones(j >= 2, F_NN, n - 2 * nx, n - 2 * nx);
   } else {
-#pragma unroll
+    // fully unrolled so that the 24 couplings' loads issue together
+#pragma clang loop unroll(full)
     for (int k = 1; k < kCoarseBlocks; k++) {
       const int di = coarse_di(k), dj = coarse_dj(k);
       const int mf = n + dj * nx + di, mb = n - dj * nx - di;
@@ -149,8 +150,12 @@ __device__ __forceinline__ void vc_apply_row(const R& rd, const Geo& g, const fl
   }
 }
 
+// at most 4 waves per SIMD: the register room lets a row's couplings load together (at the default
+// occupancy target the scheduler waits for each coupling before loading the next)
 template <bool FINE, class R>
-__global__ __launch_bounds__(256) void k_residual(LvArgs a, R rd, const float* x, const float* b, float* r, const int* gate) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_residual(LvArgs a, R rd, const float* x,
+                                                                                              const float* b, float* r,
+                                                                                              const int* gate) {
   if (gate && *gate) return;
   int n = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= a.N) return;
@@ -311,6 +316,14 @@ struct RapArgs {
 };
 
 // weight of coarse (I, J) in fine node a's bracket (a at fine (i, j) of extent nx, ny), or 0
+// bracket's first coarse index and count without the weights
+__device__ __forceinline__ int bracket_lo(int i, int n, int& lo) {
+  const int nc = ccount(n);
+  lo = (i >> 1) < nc - 1 ? (i >> 1) : nc - 1;
+  if (c2f(lo, n) > i) lo--;
+  return c2f(lo, n) == i ? 1 : 2;
+}
+
 __device__ __forceinline__ float pw_of(const float* pw, int Np, int a, int i, int j, int nx, int ny, int I, int J) {
   int loj, loi;
   double wj[2], wi[2];
@@ -391,7 +404,7 @@ struct XferArgs {
   float* xfT;      // column-major fine solution (dual-layout fine level) or null
 };
 
-__global__ __launch_bounds__(256) void k_restrict(XferArgs X) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_restrict(XferArgs X) {
   if (X.gate && *X.gate) return;
   int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= X.Nc) return;
@@ -400,21 +413,30 @@ __global__ __launch_bounds__(256) void k_restrict(XferArgs X) {
   const int nx = X.a.geo.nx[l], ny = X.a.geo.ny[l], foff = X.a.geo.off[l];
   const int Np = X.a.Np, Npc = X.Npc;
   float s[3] = {0.f, 0.f, 0.f};
-  if (!X.cfixed[I]) {
+  {
+    // the 3 x 3 fine nodes around the coarse node's fine position, branch-free: an absent node, a
+    // fixed one or a weight plane that does not reach this coarse node reads row a0 and adds 0
     const int cfi = c2f(Ii, nx), cfj = c2f(Ij, ny);
-    for (int aj = cfj - 1; aj <= cfj + 1; aj++) {
-      if (aj < 0 || aj >= ny) continue;
-      for (int ai = cfi - 1; ai <= cfi + 1; ai++) {
-        if (ai < 0 || ai >= nx) continue;
-        int a = foff + aj * nx + ai;
-        if (X.a.fixed[a]) continue;
-        float w = pw_of(X.pw, Np, a, ai, aj, nx, ny, Ii, Ij);
-        if (w == 0.f) continue;
+    const bool cfree = !X.cfixed[I];
+    const int a0 = foff + cfj * nx + cfi;
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+      for (int di = -1; di <= 1; di++) {
+        const int ai = cfi + di, aj = cfj + dj;
+        const bool in = ai >= 0 && ai < nx && aj >= 0 && aj < ny;
+        const int a = in ? foff + aj * nx + ai : a0;
+        int loj, loi;
+        const int nj = bracket_lo(in ? aj : cfj, ny, loj), ni = bracket_lo(in ? ai : cfi, nx, loi);
+        const int ej = Ij - loj, ei = Ii - loi;
+        const bool reach = ej >= 0 && ej < nj && ei >= 0 && ei < ni;
+        const float wv = X.pw[(reach ? ej * ni + ei : 0) * Np + a];
+        const uint8_t fx = X.a.fixed[a];
+        const float w = (cfree & in & reach & (fx == 0)) ? wv : 0.f;
         s[0] += w * X.r[a];
         s[1] += w * X.r[Np + a];
         s[2] += w * X.r[2 * Np + a];
       }
-    }
   }
   for (int q = 0; q < 3; q++) {
     X.bc[q * Npc + I] = s[q];
@@ -434,14 +456,17 @@ __global__ __launch_bounds__(256) void k_prolong_add(XferArgs X) {
   const int cnx = X.cgeo.nx[l], coff = X.cgeo.off[l];
   const int Np = X.a.Np, Npc = X.Npc;
   int loj, loi;
-  double wj[2], wi[2];
-  int nj = bracket(j, ny, loj, wj);
-  int ni = bracket(i, nx, loi, wi);
+  const int nj = bracket_lo(j, ny, loj), ni = bracket_lo(i, nx, loi);
   float s[3] = {0.f, 0.f, 0.f};
-  for (int ej = 0; ej < nj; ej++)
-    for (int ei = 0; ei < ni; ei++) {
-      float w = X.pw[(ej * ni + ei) * Np + f];
-      int c = coff + (loj + ej) * cnx + loi + ei;
+  // the up to 2 x 2 coarse nodes, branch-free (absent terms read the first one with weight 0)
+#pragma unroll
+  for (int ej = 0; ej < 2; ej++)
+#pragma unroll
+    for (int ei = 0; ei < 2; ei++) {
+      const bool use = ej < nj && ei < ni;
+      const float wv = X.pw[(use ? ej * ni + ei : 0) * Np + f];
+      const float w = use ? wv : 0.f;
+      const int c = coff + (loj + (use ? ej : 0)) * cnx + loi + (use ? ei : 0);
       s[0] += w * X.xc[c];
       s[1] += w * X.xc[Npc + c];
       s[2] += w * X.xc[2 * Npc + c];
