@@ -306,3 +306,27 @@ def test_fp16_pcg_reaches_the_fp32_tolerance():
         assert st["rel_residual"][0] <= 1e-6
         gpu.close()
     assert its[BD_PRECISION_FP16] <= 1.1 * its[0] + 2
+
+
+def test_set_smoother_rejects_what_a_level_cannot_run():
+    """bd_set_smoother: unknown levels, families and variants, and the split variant on a level without
+    a rows buffer, are rejected with BD_ERR_INVALID; a valid choice pins and the default unpins."""
+    from bd.capi import BD_SMOOTHER_DEFAULT, BD_SMOOTHER_SPLIT, BD_SMOOTHER_STAGED8
+    from bd.solver import GpuSolver
+    pitch = 1e-3
+    mat = linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / 2.4)
+    S = sheet_scene(136, pitch, mat)  # lines of 136 nodes: no level keeps a rows buffer
+    gpu = GpuSolver(S, make_params(history=False))
+    with pytest.raises(RuntimeError):
+        gpu.set_smoother(0, 0, BD_SMOOTHER_STAGED8)  # before the first assembly
+    gpu.encode_predict()
+    gpu.encode_assemble(True)
+    nlev = len(gpu.stats()["level_nodes"])
+    for level, fam, variant in ((nlev, 0, 0), (-1, 0, 0), (0, 2, 0), (0, 0, 5), (0, 0, BD_SMOOTHER_SPLIT)):
+        with pytest.raises(RuntimeError):
+            gpu.set_smoother(level, fam, variant)
+    gpu.set_smoother(0, 1, BD_SMOOTHER_STAGED8)
+    gpu.set_smoother(0, 1, BD_SMOOTHER_DEFAULT)
+    gpu.encode_vcycle()
+    assert np.isfinite(gpu.read(BD_BUF_LEVEL_X, 0)).all()
+    gpu.close()
