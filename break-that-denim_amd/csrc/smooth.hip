@@ -851,10 +851,11 @@ void launch_colour(Solver& s, int k, int c, int variant) {
   const bool colmajor = L.dual && fam == 1;
   SmoothArgs S{colmajor ? L.opT : L.op, colmajor ? L.fixedT : L.fixed, L.Np, L.lines + L.color_start[c],
                colmajor ? L.xT : L.x, L.b, L.dual ? (colmajor ? L.x : L.xT) : nullptr, &s.scal->done, L.geo};
-  // each node of the colour's lines: its full row (fine 58 floats, coarse 150) + b, x read, x write
-  // (dual-layout levels write x twice)
+  // each node of the colour's lines: its full row (fine 58 floats; coarse 150 values, 4 B each or 2 B
+  // as FP16 pairs) + b, x read, x write (dual-layout levels write x twice)
   double nodes = (double)count * maxlen;
-  ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * ((L.fine ? 268.0 : 636.0) + (L.dual ? 12.0 : 0.0)));
+  const double row = L.fine ? 268.0 : (L.oph ? 336.0 : 636.0);
+  ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * (row + (L.dual ? 12.0 : 0.0)));
   auto go = [&](auto rdd) {
     using R = decltype(rdd);
     if (L.dual) {
