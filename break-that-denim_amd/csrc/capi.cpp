@@ -34,7 +34,11 @@ T* dalloc(size_t count) {
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, count * sizeof(T));
   if (e != hipSuccess) throw std::runtime_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+  // hipMemset runs on the null stream and may return before it completes; the solver's stream is
+  // non-blocking and is not ordered after it, so an upload or kernel issued there next could land
+  // first and then be cleared. Wait for the clear here.
   BD_HIP(hipMemset(p, 0, count * sizeof(T)));
+  BD_HIP(hipDeviceSynchronize());
   return static_cast<T*>(p);
 }
 

@@ -296,3 +296,30 @@ def test_flap_passes_through_without_self_contact():
     gaps = layer_gap(S, gpu.x)
     say(f"layer gap without self-contact: min {gaps.min() * 1e3:.3f} mm")
     assert gaps.min() < 0.3e-3
+
+
+def test_fresh_solvers_detect_what_the_scene_holds():
+    """Three fresh solvers of the default worn coupon (self-contact on): contact detection on the
+    initial state matches the FP32 restatement bit for bit each time: the row marks of the
+    coupon's broken spans, the candidate lists and the bounds. The buffers a solver uploads at
+    creation (lattice coordinates, span flags, positions) must hold the scene before its first
+    kernel runs: their allocations are cleared on the null stream, which the solver's non-blocking
+    stream does not wait for, so each clear has to finish before the upload is queued."""
+    from bd.scene import compile_scene, default_scene_specs
+    from bd.solver import GpuSolver
+    S = compile_scene(default_scene_specs())
+    x = S.position.astype(np.float32)
+    T = O.ContactTwin(S, x, np.zeros_like(x))
+    _, _, pairs, cmeta = T.detect()
+    marks = int(pairs[:, K + 1].sum())
+    say(f"restatement: {marks} row marks, {(cmeta[:, 0, 3] < 1e8).sum()} bounded, "
+        f"{(cmeta[:, 1, 3] < 1e8).sum()} reciprocal")
+    assert marks > 0
+    for rep in range(3):
+        gpu = GpuSolver(S, make_params(self_contact=True))
+        gpu.encode_contact_detect()
+        gp, gc = gpu.pairs(), gpu.cmeta()
+        say(f"solver {rep}: {int(gp[:, K + 1].sum())} row marks")
+        assert np.array_equal(gp, pairs), f"solver {rep}: candidate faces, keys, row marks"
+        assert np.array_equal(gc.view(np.uint32), cmeta.view(np.uint32)), f"solver {rep}: bounds"
+        del gpu
