@@ -445,20 +445,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
   }
 }
 
-__global__ __launch_bounds__(256) void k_prolong_add(XferArgs X) {
-  if (X.gate && *X.gate) return;
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= X.a.N) return;
-  if (X.a.fixed[f]) return;
-  int l, i, j;
-  node_ij(X.a.geo, f, l, i, j);
+// coarse correction at fine node f = (l, i, j): the up to 2 x 2 bracketing coarse nodes,
+// branch-free (absent terms read the first one with weight 0)
+__device__ __forceinline__ void prolong_sum(const XferArgs& X, int f, int l, int i, int j, float s[3]) {
   const int nx = X.a.geo.nx[l], ny = X.a.geo.ny[l];
   const int cnx = X.cgeo.nx[l], coff = X.cgeo.off[l];
   const int Np = X.a.Np, Npc = X.Npc;
   int loj, loi;
   const int nj = bracket_lo(j, ny, loj), ni = bracket_lo(i, nx, loi);
-  float s[3] = {0.f, 0.f, 0.f};
-  // the up to 2 x 2 coarse nodes, branch-free (absent terms read the first one with weight 0)
+  s[0] = s[1] = s[2] = 0.f;
 #pragma unroll
   for (int ej = 0; ej < 2; ej++)
 #pragma unroll
@@ -471,11 +466,75 @@ __global__ __launch_bounds__(256) void k_prolong_add(XferArgs X) {
       s[1] += w * X.xc[Npc + c];
       s[2] += w * X.xc[2 * Npc + c];
     }
-  const int fT = X.a.geo.off[l] + i * ny + j;
-  for (int q = 0; q < 3; q++) {
-    const float v = X.xf[q * Np + f] + s[q];
-    X.xf[q * Np + f] = v;
-    if (X.xfT) X.xfT[q * Np + fT] = v;
+}
+
+// levels without a column-major copy: one node per thread
+__global__ __launch_bounds__(256) void k_prolong_add(XferArgs X) {
+  if (X.gate && *X.gate) return;
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= X.a.N) return;
+  if (X.a.fixed[f]) return;
+  int l, i, j;
+  node_ij(X.a.geo, f, l, i, j);
+  float s[3];
+  prolong_sum(X, f, l, i, j, s);
+  for (int q = 0; q < 3; q++) X.xf[q * X.a.Np + f] += s[q];
+}
+
+// number of 32 x 32 node tiles of a level (every layer tiled on its own)
+int prolong_tiles(const Geo& g) {
+  int n = 0;
+  for (int l = 0; l < g.nl; l++) n += ((g.nx[l] + 31) / 32) * ((g.ny[l] + 31) / 32);
+  return n;
+}
+
+// levels with a column-major copy: one 32 x 32 node tile of one layer per block (thread (tx, ty)
+// takes rows ty, ty + 8, ...). x is updated row by row; the copy is written from LDS along
+// columns, 32 consecutive nodes (128 bytes) per column, where a node-per-thread store puts every
+// lane of a wave in its own cache line. Fixed nodes write nothing to either copy.
+__global__ __launch_bounds__(256) void k_prolong_add_tiled(XferArgs X) {
+  if (X.gate && *X.gate) return;
+  __shared__ float tile[3][32][33];
+  __shared__ int live[32][33];
+  int b = blockIdx.x, l = 0, tnx = 1;
+  for (; l < X.a.geo.nl; l++) {
+    tnx = (X.a.geo.nx[l] + 31) >> 5;
+    const int nt = tnx * ((X.a.geo.ny[l] + 31) >> 5);
+    if (b < nt) break;
+    b -= nt;
+  }
+  if (l >= X.a.geo.nl) return;  // the grid holds exactly the level's tiles: not reached
+  const int nx = X.a.geo.nx[l], ny = X.a.geo.ny[l], off = X.a.geo.off[l], Np = X.a.Np;
+  const int i0 = (b % tnx) * 32, j0 = (b / tnx) * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, j = j0 + r;
+    int on = 0;
+    float v[3] = {0.f, 0.f, 0.f};
+    if (i < nx && j < ny) {
+      const int f = off + j * nx + i;
+      if (!X.a.fixed[f]) {
+        float s[3];
+        prolong_sum(X, f, l, i, j, s);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          v[q] = X.xf[q * Np + f] + s[q];
+          X.xf[q * Np + f] = v[q];
+        }
+        on = 1;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) tile[q][r][tx] = v[q];
+    live[r][tx] = on;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int j = j0 + tx, i = i0 + r;  // column i, row j: tile entry [j - j0][i - i0]
+    if (i < nx && j < ny && live[tx][r]) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) X.xfT[q * Np + off + i * ny + j] = tile[q][tx][r];
+    }
   }
 }
 
@@ -605,7 +664,10 @@ void launch_prolong_add(Solver& s, int k) {
   XferArgs X{lv_args(L), Cl.geo, Cl.N, Cl.Np, L.pw, Cl.fixed, L.r, Cl.b, Cl.x, L.x, pcg_gate(s),
              nullptr, L.dual ? L.xT : nullptr};
   ProfScope ps(s, PC_PROLONG, L.N * (L.dual ? 53.0 : 41.0) + Cl.N * 12.0);
-  k_prolong_add<<<grid_for(L.N, 256), 256, 0, s.stream>>>(X);
+  if (L.dual)
+    k_prolong_add_tiled<<<prolong_tiles(L.geo), 256, 0, s.stream>>>(X);
+  else
+    k_prolong_add<<<grid_for(L.N, 256), 256, 0, s.stream>>>(X);
 }
 
 // vcycle (newtonMultigrid.ts:495-528) on level k's x / b buffers, x zero on entry
