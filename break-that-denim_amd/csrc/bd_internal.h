@@ -71,6 +71,7 @@ struct Level {
   float* opT = nullptr;
   uint8_t* fixedT = nullptr;
   float* xT = nullptr;
+  float* bT = nullptr;  // column-major copy of b, refreshed before the level is smoothed
   // smoother kernel per family (u, v): -1 the default rule, 0 direct, 4 / 8 staged, 16 split
   // (bd_autotune)
   int variant[2] = {-1, -1};
@@ -303,6 +304,7 @@ void launch_zero(Solver& s, float* p, size_t n);
 // column-major copies of a dual-layout level: operator and fixed flags, or the iterate
 void launch_colmajor_op(Solver& s, int k);
 void launch_colmajor_x(Solver& s, int k);
+void launch_colmajor_b(Solver& s, int k);
 void vcycle(Solver& s, int k);
 // pcg.hip
 void pcg_solve(Solver& s);

@@ -177,6 +177,7 @@ void destroy_all(Solver& s) {
     dfree(L.opT);
     dfree(L.fixedT);
     dfree(L.xT);
+    dfree(L.bT);
     dfree(L.rows);
     dfree(L.oph);
     dfree(L.ophT);
@@ -490,6 +491,7 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
         L.fixedT = dalloc<uint8_t>(L.Np);
         L.xT = dalloc<float>(3 * (size_t)L.Np);
         BD_HIP(hipMemsetAsync(L.xT, 0, 3 * (size_t)L.Np * sizeof(float), s.stream));
+        L.bT = dalloc<float>(3 * (size_t)L.Np);
       }
       build_lines(s, L, kLineColors);
       if (L.max_len > kStagedMaxLen) smooth_alloc_rows(s, L);
@@ -672,6 +674,7 @@ int bd_smooth(bd_solver* h, int level, int reverse) {
     if (level < 0 || level >= s.nlev) return fail(h, BD_ERR_INVALID, "bd_smooth: level out of range");
     launch_open_gate(s);
     launch_colmajor_x(s, level);
+    launch_colmajor_b(s, level);
     launch_smooth(s, level, reverse);
     return BD_OK;
   });

@@ -610,6 +610,15 @@ void launch_colmajor_x(Solver& s, int k) {
   to_colmajor<float>(s, L, L.x, L.xT, 3);
 }
 
+// b in the column-major layout, for the v-line passes of a dual-layout level (b is fixed while the
+// level is smoothed: the V-cycle refreshes it once per visit, before the pre-smoothing)
+void launch_colmajor_b(Solver& s, int k) {
+  Level& L = s.lv[k];
+  if (!L.dual) return;
+  ProfScope ps(s, PC_TRANSPOSE, 2.0 * L.N * 12.0);
+  to_colmajor<float>(s, L, L.b, L.bT, 3);
+}
+
 void launch_prolong_weights(Solver& s, int k) {
   Level& L = s.lv[k];
   PwArgs P{lv_args(L), s.lv[k + 1].geo, L.pw, L.fine, s.flags, s.intact};
@@ -678,9 +687,11 @@ void vcycle(Solver& s, int k) {
       launch_coarse_apply(s);
       return;
     }
+    launch_colmajor_b(s, k);
     for (int sw = 0; sw < s.P.coarse_sweeps; sw++) launch_smooth(s, k, sw & 1);
     return;
   }
+  launch_colmajor_b(s, k);
   for (int sw = 0; sw < s.P.smooth_sweeps; sw++) launch_smooth(s, k, 0);
   launch_residual(s, k, L.x, L.b, L.r);
   launch_restrict(s, k);

@@ -35,7 +35,7 @@ struct SmoothArgs {
   int Np;
   const int4* lines;     // (start, stride, length, family), start in the pass's layout
   float* x;              // same layout
-  const float* b;        // row-major
+  const float* b;        // same layout
   float* xo;             // the other layout's copy of x (dual-layout levels), or null
   const int* gate;
   Geo geo;
@@ -128,9 +128,9 @@ __device__ __forceinline__ void gather_row(const SmoothArgs& S, const RD& rd, in
   float c[6];
   rd.blk(F_C, n, c);
   const float xn0 = x[n], xn1 = x[Np + n], xn2 = x[2 * Np + n];
-  r[0] = S.b[nb];
-  r[1] = S.b[Np + nb];
-  r[2] = S.b[2 * Np + nb];
+  r[0] = S.b[n];
+  r[1] = S.b[Np + n];
+  r[2] = S.b[2 * Np + n];
 #pragma unroll
   for (int q = 0; q < 9; q++) U[q] = 0.f;
   float maj = 0.f;
@@ -181,9 +181,9 @@ __device__ __forceinline__ void gather_row(const SmoothArgs& S, const RD& rd, in
     D[0] = 1.f; D[1] = 0.f; D[2] = 0.f; D[3] = 1.f; D[4] = 0.f; D[5] = 1.f;
 #pragma unroll
     for (int q = 0; q < 9; q++) U[q] = 0.f;
-    r[0] = S.b[nb];
-    r[1] = S.b[Np + nb];
-    r[2] = S.b[2 * Np + nb];
+    r[0] = S.b[n];
+    r[1] = S.b[Np + n];
+    r[2] = S.b[2 * Np + n];
     return;
   }
 #pragma unroll
@@ -263,7 +263,7 @@ __device__ __forceinline__ void gather_part(const SmoothArgs& S, const RD& rd, i
     rd.blk(F_C, n, c);
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-      bb[q] = S.b[q * Np + nb];
+      bb[q] = S.b[q * Np + n];
       xx[q] = x[q * Np + n];
     }
     f = S.fixed[n];
@@ -850,7 +850,8 @@ void launch_colour(Solver& s, int k, int c, int variant) {
   for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
   const bool colmajor = L.dual && fam == 1;
   SmoothArgs S{colmajor ? L.opT : L.op, colmajor ? L.fixedT : L.fixed, L.Np, L.lines + L.color_start[c],
-               colmajor ? L.xT : L.x, L.b, L.dual ? (colmajor ? L.x : L.xT) : nullptr, &s.scal->done, L.geo};
+               colmajor ? L.xT : L.x, colmajor ? L.bT : L.b, L.dual ? (colmajor ? L.x : L.xT) : nullptr, &s.scal->done,
+               L.geo};
   // each node of the colour's lines: its full row (fine 58 floats; coarse 150 values, 4 B each or 2 B
   // as FP16 pairs) + b, x read, x write (dual-layout levels write x twice)
   double nodes = (double)count * maxlen;
