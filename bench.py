@@ -55,6 +55,10 @@ def parse():
     ap.add_argument("--profile-frames", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default smoother kernel per level")
+    ap.add_argument("--pin", action="append", default=[], metavar="LEVEL:U:V",
+                    help="after the autotuner, run level LEVEL's u- and v-line passes with kernel variants U and V "
+                         "(0 direct, 4 / 8 staged, 16 split); e.g. under a kernel tracer, whose per-dispatch cost "
+                         "counts against the two-kernel split variant in the autotuner's timing")
     ap.add_argument("--method", choices=("both", "mixed", "plain"), default="both")
     ap.add_argument("--sweeps", type=int, default=1, help="pre- and post-smoothing sweeps per level")
     ap.add_argument("--mixed-precision", choices=("fp16", "fp32"), default="fp16",
@@ -164,6 +168,12 @@ def run_method(args, cfg, coarse_solve, profile_frames):
     gpu.sync()
     # the fastest smoother kernel per level and family for this hierarchy (bd_autotune)
     tuned = gpu.autotune(5) if not args.no_autotune else {}
+    for pin in args.pin:
+        lvl, u, v = (int(q) for q in pin.split(":"))
+        gpu.set_smoother(lvl, 0, u)
+        gpu.set_smoother(lvl, 1, v)
+        tuned = dict(tuned)
+        tuned[lvl] = (u, v)
     iters, newton = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
