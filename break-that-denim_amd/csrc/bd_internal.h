@@ -234,6 +234,9 @@ struct Solver {
   double* partials = nullptr;  // kDotBlocks * 8
   Scal* scal = nullptr;
   Scal* scal_h = nullptr;      // pinned host mirror
+  int* flag_h = nullptr;       // coherent pinned: PCG's convergence flag [0] and its sequence number [1]
+  int* flag_d = nullptr;       // the same memory as the device addresses it
+  int flag_seq = 0;
   int* counters = nullptr;     // [0] ruptures, [1] pulled out
   // multigrid
   int nlev = 0;

@@ -184,6 +184,7 @@ void destroy_all(Solver& s) {
     dfree(L.hs);
   }
   if (s.scal_h) (void)hipHostFree(s.scal_h);
+  if (s.flag_h) (void)hipHostFree(s.flag_h);
   if (s.grip_goal_h) (void)hipHostFree(s.grip_goal_h);
   for (int q = 0; q < Solver::kGoalSlots; q++)
     if (s.goal_ev[q]) (void)hipEventDestroy(s.goal_ev[q]);
@@ -406,6 +407,9 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
     s.counters = dalloc<int>(8);
     BD_HIP(hipHostMalloc((void**)&s.scal_h, sizeof(Scal), hipHostMallocDefault));
     std::memset(s.scal_h, 0, sizeof(Scal));
+    BD_HIP(hipHostMalloc((void**)&s.flag_h, 2 * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+    s.flag_h[0] = s.flag_h[1] = 0;
+    BD_HIP(hipHostGetDevicePointer((void**)&s.flag_d, s.flag_h, 0));
     const size_t goal_bytes = sizeof(double) * 4 * BD_MAX_GRIPS * (Solver::kGoalSlots + 1);
     BD_HIP(hipHostMalloc((void**)&s.grip_goal_h, goal_bytes, hipHostMallocDefault));
     std::memset(s.grip_goal_h, 0, goal_bytes);

@@ -10,6 +10,7 @@
 // k_energy restates energyChange / chooseStep (solver.ts:430-534, newtonMultigrid.ts:582-603) in
 // FP64: per node-owned term, the change of the incremental potential at alpha in {1, 1/2, 1/4, 1/8}
 // by per-term differences and their magnitude sums, reduced deterministically, then the step rule.
+#include <chrono>
 #include <stdexcept>
 
 #include "bd_internal.h"
@@ -729,6 +730,29 @@ int dot_grid(int N) {
 
 }  // namespace
 
+// PCG's convergence flag for the host: the flag, then (after a system-scope fence) the sequence
+// number the host waits for, both in coherent pinned host memory
+__global__ void k_post_flag(const Scal* sc, int* host, int seq) {
+  volatile int* h = host;
+  h[0] = sc->done;
+  __threadfence_system();
+  h[1] = seq;
+}
+
+// Waits for k_post_flag's sequence number by polling the pinned word (a stream synchronize would
+// wake the host tens of microseconds after the copy ends); after 2 s without it, falls back to a
+// stream synchronize and the device flag.
+bool wait_flag(Solver& s, int seq) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned spin = 0;; spin++) {
+    if (__atomic_load_n(&s.flag_h[1], __ATOMIC_ACQUIRE) == seq) return __atomic_load_n(&s.flag_h[0], __ATOMIC_ACQUIRE) != 0;
+    if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) break;
+  }
+  BD_HIP(hipStreamSynchronize(s.stream));
+  BD_HIP(hipMemcpy(&s.scal_h->done, &s.scal->done, sizeof(int), hipMemcpyDeviceToHost));
+  return s.scal_h->done != 0;
+}
+
 // One PCG solve on level 0 (newtonMultigrid.ts:537-576), leaving the direction in delta.
 void pcg_solve(Solver& s) {
   PcgArgs A = pcg_args(s);
@@ -776,9 +800,9 @@ void pcg_solve(Solver& s) {
     if (it + 1 == K) break;
     if (tol_mode) {
       // the device decides convergence; check it before queueing the next V-cycle
-      BD_HIP(hipMemcpyAsync(&s.scal_h->done, &s.scal->done, sizeof(int), hipMemcpyDeviceToHost, s.stream));
-      BD_HIP(hipStreamSynchronize(s.stream));
-      if (s.scal_h->done) break;
+      const int seq = ++s.flag_seq;
+      k_post_flag<<<1, 1, 0, s.stream>>>(s.scal, s.flag_d, seq);
+      if (wait_flag(s, seq)) break;
     }
     vcycle(s, 0);
     {
