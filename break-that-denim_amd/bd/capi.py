@@ -41,7 +41,7 @@ BD_NO_PIECE = 0xFFFFFFFF
 BD_CONTACT_K = 4
 BD_CONTACT_TABLE_BITS = 17
 BD_CONTACT_NONE = 0xFFFFFFFF
-BD_CONTACT_MAX_NODES = (1 << 20) - 1
+BD_CONTACT_MAX_NODES = 1 << 24
 
 
 class bd_yarn(C.Structure):
@@ -98,7 +98,7 @@ class bd_stats(C.Structure):
                 ("rel_residual", C.c_double * BD_MAX_NEWTON), ("alpha", C.c_double * BD_MAX_NEWTON),
                 ("max_update", C.c_double * BD_MAX_NEWTON), ("rejections", C.c_int32),
                 ("ruptures", C.c_int32), ("pulled_out", C.c_int32), ("level_count", C.c_int32),
-                ("level_nodes", C.c_int32 * BD_MAX_LEVELS)]
+                ("level_nodes", C.c_int32 * BD_MAX_LEVELS), ("poll_fallbacks", C.c_int32), ("pad_", C.c_int32)]
 
 
 class bd_kernel_profile(C.Structure):

@@ -309,4 +309,5 @@ class GpuSolver:
             "ruptures": st.ruptures,
             "pulled_out": st.pulled_out,
             "level_nodes": list(st.level_nodes[:st.level_count]),
+            "poll_fallbacks": st.poll_fallbacks,
         }

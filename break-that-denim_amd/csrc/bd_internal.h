@@ -237,6 +237,8 @@ struct Solver {
   int* flag_h = nullptr;       // coherent pinned: PCG's convergence flag [0] and its sequence number [1]
   int* flag_d = nullptr;       // the same memory as the device addresses it
   int flag_seq = 0;
+  int poll_timeout_ms = 2000;  // BD_POLL_TIMEOUT_MS at creation; 0 takes the synchronizing path at once
+  int poll_fallbacks = 0;      // convergence checks that took the synchronizing path
   int* counters = nullptr;     // [0] ruptures, [1] pulled out
   // multigrid
   int nlev = 0;
@@ -255,7 +257,7 @@ struct Solver {
   uint32_t* scan_sums = nullptr;
   float4* items = nullptr;      // 2 per record, at most 2 records per face
   uint32_t* pairs = nullptr;    // kContactK planes of Np: candidate faces
-  uint32_t* keys = nullptr;     // reciprocal-bound keys (cleared by resolve)
+  unsigned long long* keys = nullptr;  // reciprocal-bound keys, share code << 32 | node + 1 (cleared by resolve)
   uint32_t* marks = nullptr;    // damaged-row marks
   float4* cmeta = nullptr;      // 2 per node: own bound, reciprocal bound
   int* acc = nullptr;           // 9 planes of fixed-point accumulators

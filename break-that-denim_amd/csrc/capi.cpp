@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -168,9 +169,9 @@ void destroy_all(Solver& s) {
   for (int k = 0; k < s.nlev; k++) {
     Level& L = s.lv[k];
     dfree(L.op);
-    if (k > 0) dfree(L.fixed);
-    if (k > 0) dfree(L.x);
-    if (k > 0) dfree(L.b);
+    if (k > 0) dfree(L.fixed);  // level 0 shares the solver's pinned flags
+    dfree(L.x);
+    dfree(L.b);
     dfree(L.r);
     dfree(L.pw);
     dfree(L.lines);
@@ -188,6 +189,8 @@ void destroy_all(Solver& s) {
   if (s.grip_goal_h) (void)hipHostFree(s.grip_goal_h);
   for (int q = 0; q < Solver::kGoalSlots; q++)
     if (s.goal_ev[q]) (void)hipEventDestroy(s.goal_ev[q]);
+  for (hipEvent_t e : s.prof_pool) (void)hipEventDestroy(e);
+  s.prof_pool.clear();
   if (s.stream) (void)hipStreamDestroy(s.stream);
 }
 
@@ -337,6 +340,10 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
     g_create_error = "bd_create: unknown precision";
     return BD_ERR_INVALID;
   }
+  if (prm->newton_iterations < 1 || prm->newton_iterations > BD_MAX_NEWTON) {
+    g_create_error = "bd_create: newton_iterations must be 1.." + std::to_string(BD_MAX_NEWTON);
+    return BD_ERR_INVALID;
+  }
   if (prm->smooth_sweeps < 1 || prm->smooth_sweeps > 8) {
     g_create_error = "bd_create: smooth_sweeps must be 1..8";
     return BD_ERR_INVALID;
@@ -346,7 +353,7 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
     return BD_ERR_LIMIT;
   }
   if (prm->self_contact && sc->layer_count == 1 && sc->node_count > BD_CONTACT_MAX_NODES) {
-    g_create_error = "bd_create: self-contact bound keys hold at most 2^20 - 1 nodes";
+    g_create_error = "bd_create: self-contact supports at most BD_CONTACT_MAX_NODES nodes";
     return BD_ERR_LIMIT;
   }
   int ndev = 0;
@@ -410,6 +417,7 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
     BD_HIP(hipHostMalloc((void**)&s.flag_h, 2 * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
     s.flag_h[0] = s.flag_h[1] = 0;
     BD_HIP(hipHostGetDevicePointer((void**)&s.flag_d, s.flag_h, 0));
+    if (const char* t = std::getenv("BD_POLL_TIMEOUT_MS")) s.poll_timeout_ms = std::max(0, std::atoi(t));
     const size_t goal_bytes = sizeof(double) * 4 * BD_MAX_GRIPS * (Solver::kGoalSlots + 1);
     BD_HIP(hipHostMalloc((void**)&s.grip_goal_h, goal_bytes, hipHostMallocDefault));
     std::memset(s.grip_goal_h, 0, goal_bytes);
@@ -1330,14 +1338,15 @@ int bd_read(bd_solver* h, bd_buffer buf, int level, void* dst, size_t bytes) {
             for (int q = 0; q < kContactK; q++) o[kContactK * (size_t)n + q] = t[(size_t)q * s.Np + n];
           break;
         }
-        std::vector<uint32_t> k(s.N), m(s.N);
+        std::vector<unsigned long long> k(s.N);
+        std::vector<uint32_t> m(s.N);
         download(s, t.data(), s.pairs, t.size());
         download(s, k.data(), s.keys, (size_t)s.N);
         download(s, m.data(), s.marks, (size_t)s.N);
         uint32_t* o = static_cast<uint32_t*>(dst);
         for (int n = 0; n < s.N; n++) {
           for (int q = 0; q < kContactK; q++) o[(kContactK + 2) * (size_t)n + q] = t[(size_t)q * s.Np + n];
-          o[(kContactK + 2) * (size_t)n + kContactK] = k[n];
+          o[(kContactK + 2) * (size_t)n + kContactK] = (uint32_t)(k[n] & 0xFFFFFFFFull);  // node + 1 of a pending key
           o[(kContactK + 2) * (size_t)n + kContactK + 1] = m[n];
         }
         break;
@@ -1427,6 +1436,7 @@ int bd_get_stats(bd_solver* h, bd_stats* out) {
       st.max_update[k] = mu;
     }
     st.rejections = s.scal_h->rejections;
+    st.poll_fallbacks = s.poll_fallbacks;
     st.ruptures = counters[0];
     st.pulled_out = counters[1];
     *out = st;

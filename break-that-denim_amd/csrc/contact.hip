@@ -70,7 +70,7 @@ struct CtArgs {
   uint32_t* cells;
   float4* items;
   uint32_t* pairs;
-  uint32_t* keys;
+  unsigned long long* keys;
   uint32_t* marks;
   float4* cmeta;
   int* acc;
@@ -466,7 +466,10 @@ __global__ __launch_bounds__(256) void k_detect(CtArgs A) {
   }
   float share = kBoundFraction * fmaxf(db - kHardCore, 0.f);
   A.cmeta[2 * n] = make_float4(nb.x, nb.y, nb.z, share);
-  uint32_t key = ((4095u - min(4095u, (uint32_t)(share / kRecipUnit))) << 20) | (n + 1u);
+  // the tightest share wins, then the larger node index: the share code above the node, as the reference's
+  // 32-bit key (selfcontact.wgsl:462-474), but with the node in a full 32-bit word so any node count fits
+  const unsigned long long key =
+      ((unsigned long long)(4095u - min(4095u, (uint32_t)(share / kRecipUnit))) << 32) | (unsigned long long)(n + 1u);
   Corners c = corners(A, fb);
   for (int k = 0; k < 4; k++) atomicMax(&A.keys[c.c[k]], key);
 }
@@ -475,14 +478,14 @@ __global__ __launch_bounds__(256) void k_detect(CtArgs A) {
 __global__ __launch_bounds__(256) void k_resolve(CtArgs A) {
   uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (uint32_t)A.N) return;
-  uint32_t key = A.keys[n];
-  A.keys[n] = 0u;
-  if (key == 0u) {
+  const unsigned long long key = A.keys[n];
+  A.keys[n] = 0ull;
+  if (key == 0ull) {
     A.cmeta[2 * n + 1] = make_float4(0.f, 0.f, 0.f, kUnbounded);
     return;
   }
-  uint32_t j = (key & 0xFFFFFu) - 1u;
-  float share = (float)(4095u - (key >> 20)) * kRecipUnit;
+  uint32_t j = (uint32_t)(key & 0xFFFFFFFFull) - 1u;
+  float share = (float)(4095u - (uint32_t)(key >> 32)) * kRecipUnit;
   float4 o = A.cmeta[2 * j];
   A.cmeta[2 * n + 1] = make_float4(-o.x, -o.y, -o.z, share);
 }
@@ -621,12 +624,12 @@ void contact_alloc(Solver& s) {
   BD_HIP(hipMalloc(&s.scan_sums, 128 * sizeof(uint32_t)));
   BD_HIP(hipMalloc(&s.items, 4 * N * sizeof(float4)));
   BD_HIP(hipMalloc(&s.pairs, kContactK * Np * sizeof(uint32_t)));
-  BD_HIP(hipMalloc(&s.keys, Np * sizeof(uint32_t)));
+  BD_HIP(hipMalloc(&s.keys, Np * sizeof(unsigned long long)));
   BD_HIP(hipMalloc(&s.marks, Np * sizeof(uint32_t)));
   BD_HIP(hipMalloc(&s.acc, 9 * Np * sizeof(int)));
   BD_HIP(hipMemsetAsync(s.cells, 0, 2 * (size_t)kTable * sizeof(uint32_t), s.stream));
   BD_HIP(hipMemsetAsync(s.pairs, 0xFF, kContactK * Np * sizeof(uint32_t), s.stream));
-  BD_HIP(hipMemsetAsync(s.keys, 0, Np * sizeof(uint32_t), s.stream));
+  BD_HIP(hipMemsetAsync(s.keys, 0, Np * sizeof(unsigned long long), s.stream));
   BD_HIP(hipMemsetAsync(s.marks, 0, Np * sizeof(uint32_t), s.stream));
   BD_HIP(hipMemsetAsync(s.acc, 0, 9 * Np * sizeof(int), s.stream));
 }
@@ -652,7 +655,7 @@ void launch_contact_detect(Solver& s) {
   }
   {
     // per node: row marks (7 x 4 B flags), x, dx, ijl, K pairs, cmeta, key (minimal unique bytes)
-    ProfScope ps(s, PC_CONTACT_DETECT, s.N * (28.0 + 4.0 + 24.0 + 4.0 + 4.0 * kContactK + 32.0 + 8.0));
+    ProfScope ps(s, PC_CONTACT_DETECT, s.N * (28.0 + 4.0 + 24.0 + 4.0 + 4.0 * kContactK + 32.0 + 16.0));
     k_damage_row<<<g, 256, 0, s.stream>>>(A);
     k_detect<<<g, 256, 0, s.stream>>>(A);
     k_resolve<<<g, 256, 0, s.stream>>>(A);

@@ -740,14 +740,18 @@ __global__ void k_post_flag(const Scal* sc, int* host, int seq) {
 }
 
 // Waits for k_post_flag's sequence number by polling the pinned word (a stream synchronize would
-// wake the host tens of microseconds after the copy ends); after 2 s without it, falls back to a
-// stream synchronize and the device flag.
+// wake the host tens of microseconds after the copy ends), with a pause hint in the spin; after
+// s.poll_timeout_ms without it, falls back to a stream synchronize and the device flag (the path a
+// poll timeout of 0 takes at once).
 bool wait_flag(Solver& s, int seq) {
   const auto t0 = std::chrono::steady_clock::now();
-  for (unsigned spin = 0;; spin++) {
+  const auto limit = std::chrono::milliseconds(s.poll_timeout_ms);
+  for (unsigned spin = 0; s.poll_timeout_ms > 0; spin++) {
     if (__atomic_load_n(&s.flag_h[1], __ATOMIC_ACQUIRE) == seq) return __atomic_load_n(&s.flag_h[0], __ATOMIC_ACQUIRE) != 0;
-    if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) break;
+    __builtin_ia32_pause();
+    if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > limit) break;
   }
+  s.poll_fallbacks += 1;
   BD_HIP(hipStreamSynchronize(s.stream));
   BD_HIP(hipMemcpy(&s.scal_h->done, &s.scal->done, sizeof(int), hipMemcpyDeviceToHost));
   return s.scal_h->done != 0;
@@ -802,6 +806,7 @@ void pcg_solve(Solver& s) {
       // the device decides convergence; check it before queueing the next V-cycle
       const int seq = ++s.flag_seq;
       k_post_flag<<<1, 1, 0, s.stream>>>(s.scal, s.flag_d, seq);
+      BD_HIP(hipGetLastError());
       if (wait_flag(s, seq)) break;
     }
     vcycle(s, 0);

@@ -46,7 +46,9 @@ extern "C" {
 #define BD_CONTACT_K 4
 #define BD_CONTACT_TABLE_BITS 17
 #define BD_CONTACT_NONE 0xFFFFFFFFu
-#define BD_CONTACT_MAX_NODES ((1 << 20) - 1)  /* the reciprocal-bound key holds node + 1 in 20 bits */
+/* reciprocal-bound keys are 64-bit (share code << 32 | node + 1); the reference's 32-bit key holds node + 1
+ * in 20 bits (gpu/selfcontact.ts:84). The limit here is the lattice index packing (12 bits per i, j). */
+#define BD_CONTACT_MAX_NODES (1 << 24)
 
 /* Cross-layer contact of a scene of 2-3 plies (gpu/contact.ts:8-18 CONTACT, contact.wgsl): always on
  * for a scene of several layers, in place of self-contact (gpu/solver.ts:322, 364). K nearest nodes
@@ -174,6 +176,8 @@ typedef struct bd_stats {
   int32_t pulled_out;                     /* spans released by crossing slip (total) */
   int32_t level_count;
   int32_t level_nodes[BD_MAX_LEVELS];
+  int32_t poll_fallbacks;                 /* PCG convergence checks that took the synchronizing path (total) */
+  int32_t pad_;
 } bd_stats;
 
 /* Buffers for bd_read / bd_write. `level` selects the multigrid level where it applies. */

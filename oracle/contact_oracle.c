@@ -386,7 +386,7 @@ static void face_record(const oc_scene* S, uint32_t f, float* rec) {
 }
 
 /* the bound of a node from its query, and its reciprocal key (0: none) */
-static uint32_t finish_bound(const oc_scene* S, uint32_t n, const oc_query* Q, const float* p0, float* own, uint32_t* fb_out) {
+static uint64_t finish_bound(const oc_scene* S, uint32_t n, const oc_query* Q, const float* p0, float* own, uint32_t* fb_out) {
   float nb[3] = {Q->n0[0], Q->n0[1], Q->n0[2]};
   float db = Q->d0min;
   uint32_t fb = Q->f0;
@@ -424,13 +424,15 @@ static uint32_t finish_bound(const oc_scene* S, uint32_t n, const oc_query* Q, c
   own[3] = share;
   uint32_t u = (uint32_t)(share / OC_RECIP);
   *fb_out = fb;
-  return ((4095u - (u < 4095u ? u : 4095u)) << 20) | (n + 1u);
+  /* 64-bit key: the share code above a full 32-bit node + 1 (the reference packs node + 1 into 20 bits,
+     selfcontact.wgsl:462-474); the same order: tightest share, then the larger node */
+  return ((uint64_t)(4095u - (u < 4095u ? u : 4095u)) << 32) | (uint64_t)(n + 1u);
 }
 
 static void detect_common(const oc_scene* S, int brute, uint32_t* cells, float* items, uint32_t* pairs, float* cmeta) {
   const int N = S->N;
   uint32_t* mark = calloc((size_t)N, sizeof(uint32_t));
-  uint32_t* keys = calloc((size_t)N, sizeof(uint32_t));
+  uint64_t* keys = calloc((size_t)N, sizeof(uint64_t));
   uint32_t* valid = NULL;
   int nvalid = 0;
   float* recs = NULL;
@@ -535,7 +537,7 @@ static void detect_common(const oc_scene* S, int brute, uint32_t* cells, float* 
     }
     for (int k = 0; k < OC_K; k++) pr[k] = Q.best[k];
     uint32_t fb;
-    uint32_t key = finish_bound(S, (uint32_t)n, &Q, p0, own, &fb);
+    uint64_t key = finish_bound(S, (uint32_t)n, &Q, p0, own, &fb);
     if (key) {
       uint32_t c[4];
       face_corners(S, fb, c);
@@ -546,7 +548,7 @@ static void detect_common(const oc_scene* S, int brute, uint32_t* cells, float* 
   /* resolveBounds */
   for (int n = 0; n < N; n++) {
     float* rec = &cmeta[8 * (size_t)n + 4];
-    uint32_t key = keys[n];
+    uint64_t key = keys[n];
     pairs[(OC_K + 2) * (size_t)n + OC_K] = 0;
     pairs[(OC_K + 2) * (size_t)n + OC_K + 1] = mark[n];
     if (!key) {
@@ -554,8 +556,8 @@ static void detect_common(const oc_scene* S, int brute, uint32_t* cells, float* 
       rec[3] = OC_UNB;
       continue;
     }
-    uint32_t j = (key & 0xFFFFFu) - 1;
-    float share = (float)(4095u - (key >> 20)) * OC_RECIP;
+    uint32_t j = (uint32_t)(key & 0xFFFFFFFFu) - 1;
+    float share = (float)(4095u - (uint32_t)(key >> 32)) * OC_RECIP;
     for (int q = 0; q < 3; q++) rec[q] = -cmeta[8 * (size_t)j + q];
     rec[3] = share;
   }

@@ -53,13 +53,28 @@ def test_create_rejects_lines_beyond_the_limit():
     assert rc == -3 and "BD_MAX_LINE" in msg
 
 
-def test_create_rejects_self_contact_beyond_the_key_range():
-    # the reciprocal-bound key holds node + 1 in 20 bits (gpu/selfcontact.ts:84)
+def test_create_accepts_self_contact_at_a_million_nodes():
+    # the reference's reciprocal-bound key holds node + 1 in 20 bits (gpu/selfcontact.ts:84), which
+    # refuses a 1024^2 sheet; the 64-bit keys here take it: validation passes and only the missing
+    # device stops creation on a CPU host
     mat = linear_sheet_material(1e-3, 2e4, 1.6, 1e-5, 0.35, 8e3)
     S = sheet_scene(1024, 1e-3, mat, perturbation=0.0)
-    assert S.node_count == capi.BD_CONTACT_MAX_NODES + 1
-    rc, msg, _ = _create(S, capi.make_params(self_contact=True))
-    assert rc == -3 and "2^20" in msg
+    assert S.node_count == 1 << 20
+    rc, msg, h = _create(S, capi.make_params(self_contact=True))
+    if rc == 0:
+        capi.load_library().bd_destroy(h)
+    assert rc != -3, msg
+    assert S.node_count <= capi.BD_CONTACT_MAX_NODES
+
+
+def test_create_rejects_newton_iterations_out_of_range():
+    mat = linear_sheet_material(1e-3, 2e4, 1.6, 1e-5, 0.35, 8e3)
+    S = sheet_scene(8, 1e-3, mat)
+    for n in (0, -1, capi.BD_MAX_NEWTON + 1):
+        rc, msg, h = _create(S, capi.make_params(newton_iterations=n))
+        if rc == 0:
+            capi.load_library().bd_destroy(h)
+        assert rc == -1 and "newton_iterations" in msg
 
 
 def test_create_rejects_inconsistent_scene():

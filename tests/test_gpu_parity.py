@@ -195,6 +195,35 @@ def test_pcg_relative_tolerance(coupon_pair):
     assert 1 <= st["pcg_iterations"][0] < 200
 
 
+def test_pcg_convergence_check_fallback_path(coupon_pair, monkeypatch):
+    """The host normally polls a pinned flag for PCG's convergence; with a poll timeout of 0 every
+    check takes the synchronizing fallback (stream synchronize, then the device flag). Both paths must
+    stop at the same iteration with the same bits (the kernels and their reductions are identical)."""
+    from bd.solver import GpuSolver
+    S, cpu, gpu = coupon_pair
+    copy_state(cpu, gpu)
+    monkeypatch.setenv("BD_POLL_TIMEOUT_MS", "0")
+    slow = GpuSolver(S, PARAMS)
+    monkeypatch.delenv("BD_POLL_TIMEOUT_MS")
+    copy_state(cpu, slow)
+    res = []
+    for g in (gpu, slow):
+        g.encode_predict()
+        g.encode_assemble(True)
+        g.set_pcg(2, 1e-6, 200)
+        before = g.stats()["poll_fallbacks"]
+        g.encode_pcg()
+        st = g.stats()
+        g.set_pcg(PARAMS.pcg_iterations, 0.0, 200)
+        res.append((st["pcg_iterations"][0], st["rel_residual"][0], g.read(BD_BUF_DELTA), st["poll_fallbacks"] - before))
+    say(f"polled: {res[0][0]} iterations, rel {res[0][1]:.3e}; fallback: {res[1][0]} iterations, rel {res[1][1]:.3e}, "
+        f"{res[1][3]} synchronizing checks")
+    assert res[0][3] == 0 and res[1][3] == res[1][0]
+    assert res[0][0] == res[1][0] and res[0][1] == res[1][1]
+    assert np.array_equal(res[0][2], res[1][2])
+    del slow
+
+
 def test_dynamics_60_substeps(coupon_pair):
     S, cpu, gpu = coupon_pair
     xs, _ = copy_state(cpu, gpu)

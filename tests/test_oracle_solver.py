@@ -373,3 +373,93 @@ def test_fp16_rounding_matches_ieee_half():
     got = np.array([O.fp16_round(v) for v in vals])
     want = np.clip(vals.astype(np.float32), -65504, 65504).astype(np.float16).astype(np.float64)
     assert np.array_equal(got, want), vals[got != want][:5]
+
+
+def _linear_sheet(n, pitch=1e-3, bend_k=1e-5, shear=2e4 / 2.4, drag=None):
+    from bd.configs import sheet_scene
+    from bd.material import linear_sheet_material
+    mat = linear_sheet_material(pitch, 2e4, 1.6, bend_k, 0.35, shear)
+    if drag is not None:
+        mat["airDrag"] = drag
+    return sheet_scene(n, pitch, mat, perturbation=0.3e-3)
+
+
+def test_energy_change_equals_potential_differences():
+    # energyChange (solver.ts:430-534) against potential (solver.ts:387-422 integrand) on a linear-law
+    # sheet: the axial and shear trapezoids are exact for linear laws, bendPsiDelta is exact by
+    # construction and every node term is quadratic, so each of the four alphas must reproduce
+    # Phi(dx + alpha d) - Phi(dx) to rounding. A dropped term, a wrong alpha power or a sign error
+    # shows up at O(1) relative.
+    S = _linear_sheet(12)
+    grip = {"nodes": [0, 1, 12, 13], "weights": [0.25] * 4, "offsets": np.zeros((4, 3)),
+            "goal": S.position[[0, 1, 12, 13]].mean(0) + np.array([2e-3, -1e-3, 0]), "k": 2e4}
+    o = O.Oracle(S, make_params(h=1 / 120, newton_iterations=1, line_search=True, history=False), [grip])
+    o.predict()
+    r = np.random.default_rng(5)
+    free = np.repeat(o.pinned == 0, 3)
+    # steps of a few percent of the 1 mm pitch keep the shear diagonals out of their lock ramp
+    # (shearLaw is linear while compression stays under the jam strain, laws.ts:96-115)
+    o.dx[:] = np.where(free, 2e-5 * r.standard_normal(3 * S.node_count), 0.0)
+    d = np.where(free, 4e-5 * r.standard_normal(3 * S.node_count), 0.0)
+    dx0 = o.dx.copy()
+    e0 = o.potential()
+    e, mag = o.energy_change(d)
+    for k, a in enumerate((1.0, 0.5, 0.25, 0.125)):
+        o.dx[:] = dx0 + a * d
+        diff = o.potential() - e0
+        o.dx[:] = dx0
+        assert abs(e[k] - diff) <= 1e-9 * mag[k], (k, e[k], diff, mag[k])
+        assert abs(diff) > 1e-3 * mag[k]  # the check is not vacuous
+
+
+def test_assembled_operator_is_the_gradient_jacobian_in_tension():
+    # assembleFineOperator (newtonMultigrid.ts:169-274) against a central-difference Jacobian of
+    # nodeGradient (solver.ts:387-422). Axial springs and shear diagonals in tension with linear
+    # laws, no bending and no air drag: there the clamp projections (max(K, 0), max(f, 0) / l) are
+    # inactive and the projected Hessian is the exact one, so a factor error in any block (mass,
+    # grip, axial, shear, viscosity) fails. Bending is left out because its block is a majoriser
+    # (kb (Psi' + 2 Psi'' q) w w^T), not the Hessian.
+    n = 8
+    S = _linear_sheet(n, bend_k=0.0, drag=0.0)
+    S.position[:, 0] *= 1.03  # stretch both families (x: u, z: v) by 3 %
+    S.position[:, 2] *= 1.02
+    S.mass[:] = 0.5  # inertia m / h^2 = 7200 N/m, of the order of the span stiffness
+    grip = {"nodes": [0, 1], "weights": [0.5, 0.5], "offsets": np.zeros((2, 3)),
+            "goal": S.position[[0, 1]].mean(0) + np.array([0, 1e-3, -1e-3]), "k": 2e4}
+    o = O.Oracle(S, make_params(h=1 / 120, newton_iterations=1, history=False), [grip])
+    o.predict()
+    r = np.random.default_rng(11)
+    free = np.repeat(o.pinned == 0, 3)
+    # an iterate a fraction of a percent of the pitch away: every span and diagonal stays stretched
+    o.dx[:] = np.where(free, 1e-6 * r.standard_normal(3 * S.node_count), 0.0)
+    o.assemble()
+    A = o.stencil(0)
+    N = S.node_count
+    eps = 1e-8
+    worst = 0.0
+    scale = np.max(np.abs(A))
+    for m in range(N):
+        if o.pinned[m]:
+            continue
+        im, jm = m % n, m // n
+        nbrs = [q for q in range(N) if abs(q % n - im) <= 2 and abs(q // n - jm) <= 2 and not o.pinned[q]]
+        for c in range(3):
+            base = o.dx[3 * m + c]
+            o.dx[3 * m + c] = base + eps
+            gp = {q: o.node_gradient(q) for q in nbrs}
+            o.dx[3 * m + c] = base - eps
+            gm = {q: o.node_gradient(q) for q in nbrs}
+            o.dx[3 * m + c] = base
+            for q in nbrs:
+                col = (gp[q] - gm[q]) / (2 * eps)
+                off = ((m // n - q // n) + 2) * 5 + (m % n - q % n) + 2
+                worst = max(worst, np.max(np.abs(A[q, off][:, c] - col)))
+    tol = 1e-6 * scale
+    assert worst <= tol, (worst, scale)
+    # every term is far above the tolerance: inertia, the grip (k w = 1e4), the axial viscosity
+    # c = visc / (L h) and the transverse tension stiffness (the smallest, f / l on y)
+    visc = S.layers[0].material["yarns"][0]["axialViscosity"] / (1e-3 / 120)
+    assert min(0.5 * 120 ** 2, 1e4, visc) > 100 * tol
+    assert np.min(A[~o.pinned.astype(bool), 12][:, 1, 1] - 0.5 * 120 ** 2) > 100 * tol
+    # the tension really holds: every span's tangent block is the unclamped one
+    assert np.all(np.abs(A[~o.pinned.astype(bool), 12]).sum(axis=(1, 2)) > 0)
