@@ -1,19 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark: ms/frame and PCG iterations to relative residual 1e-6 of the implicit solve.
 
-Workload (default): 1024 x 1024 hanging sheet (1,048,576 nodes, 1 mm pitch) with a central
-256 x 256 breakable woven region (per-yarn lognormal break strains, seed 7), top row pinned, two
-bottom-corner grips pulled apart at 5 mm/step; per frame (one time step h = 1/120 s): prediction,
-3 projected-Newton iterations, each assembled and solved by FP32 Galerkin MG-PCG (7 levels,
-line smoothers) to rel-res 1e-6 with FP64 residual replacement, FP64 line search, commit, then the
-damage field and crossing slip. All state lives in HBM before the timed region; per frame the host
-only moves the grip goals (64 bytes).
+Workload (default): 1024 x 1024 hanging sheet (1,048,576 nodes, 1 mm pitch) with node-face
+self-contact, a central 256 x 256 breakable woven region (per-yarn lognormal break strains, seed 7),
+top row pinned, two bottom-corner grips pulled apart at 5 mm/step; per frame (one time step
+h = 1/120 s): prediction, contact detection, 3 projected-Newton iterations (contact terms, assembly,
+Galerkin hierarchy, MG-PCG to rel-res 1e-6 with FP64 residual replacement, FP64 line search), commit,
+then the damage field and crossing slip. All state lives in HBM before the timed region; per frame
+the host only moves the grip goals (64 bytes).
 
-Two preconditioners are timed on the same frames: "plain", the reference's V-cycle (line sweeps on
-every level, coarsest level <= 400 nodes), and "mixed", the same V-cycle stopped at the first level of
-<= 1100 nodes (33 x 33), which is solved exactly in FP64 (banded Cholesky per hierarchy build, dense
-inverse applied). `value` is the mixed method's ms/frame; the
-plain method's numbers and the iteration ratio are reported beside it.
+Three methods run the same frames from the same scene:
+  plain    the reference's schedule: FP32 V-cycle with line sweeps on every level (coarsest <= 400
+           nodes), the Galerkin hierarchy of the first Newton iteration reused by the later ones;
+  mixed    the hierarchy rebuilt at every Newton iteration, the V-cycle stopped at the first level of
+           <= 1100 nodes (33 x 33) and that level solved exactly in FP64 (banded Cholesky, dense
+           inverse), FP32 operators (FP16 coarse storage loses the contact terms' range);
+  learned  mixed plus the learned coarse-space correction (BF16 message-passing net on the 257 x 257
+           level): warm start and additive correction in flexible PCG, plain fallback; the net is
+           trained on the GPU from samples the mixed method's warm-up frames record (not the timed
+           frames).
+`value` is the learned method's ms/frame; every method's ms/frame and PCG iterations, and the ratios
+against plain, are reported beside it.
 
 Prints one JSON line: the metric, the GPU roofline of the dominant kernel class (HIP-event timed in
 an instrumented pass over the same frames) and a CPU baseline (the FP64 C oracle, single thread, on
@@ -50,7 +57,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--profile-frames", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -59,9 +66,11 @@ def parse():
                     help="after the autotuner, run level LEVEL's u- and v-line passes with kernel variants U and V "
                          "(0 direct, 4 / 8 staged, 16 split); e.g. under a kernel tracer, whose per-dispatch cost "
                          "counts against the two-kernel split variant in the autotuner's timing")
-    ap.add_argument("--method", choices=("both", "mixed", "plain"), default="both")
+    ap.add_argument("--method", choices=("all", "plain", "mixed", "learned"), default="all")
+    ap.add_argument("--epochs", type=int, default=40, help="training epochs of the learned method's net")
+    ap.add_argument("--no-contact", action="store_true", help="leave node-face self-contact off")
     ap.add_argument("--sweeps", type=int, default=1, help="pre- and post-smoothing sweeps per level")
-    ap.add_argument("--mixed-precision", choices=("fp16", "fp32"), default="fp16",
+    ap.add_argument("--mixed-precision", choices=("fp16", "fp32"), default="fp32",
                     help="operator storage of the mixed method's coarse levels (fine level and plain method: fp32)")
     ap.add_argument("--smoother-probe", type=int, default=0, metavar="SWEEPS",
                     help="only run SWEEPS forward sweeps of every smoothed level of the mixed method's hierarchy "
@@ -141,19 +150,35 @@ def cpu_baseline(cfg, gpu_iters_per_newton, newton_per_frame):
                       f"iterations per frame"}
 
 
-def run_method(args, cfg, coarse_solve, profile_frames):
-    """Warm-up, then exactly `args.steps` timed frames of one preconditioner, then an instrumented
-    pass of `profile_frames` frames (per-kernel-class HIP event times)."""
+def method_params(cfg, method, args):
+    import copy
+    from bd.capi import BD_COARSE_EXACT, BD_COARSE_SWEEPS, BD_PRECISION_FP16, BD_PRECISION_FP32
+    params = copy.copy(cfg["params"])
+    params.smooth_sweeps = args.sweeps
+    if method == "plain":
+        params.coarse_solve = BD_COARSE_SWEEPS
+        params.max_coarsest_nodes = 400
+        params.lag_coarse = 1
+        params.precision = BD_PRECISION_FP32
+    else:
+        params.coarse_solve = BD_COARSE_EXACT
+        params.max_coarsest_nodes = MIXED_COARSEST_NODES
+        params.lag_coarse = 0
+        params.precision = BD_PRECISION_FP16 if args.mixed_precision == "fp16" else BD_PRECISION_FP32
+    return params
+
+
+def run_method(args, cfg, method, profile_frames):
+    """Warm-up, then exactly `args.steps` timed frames of one method, then an instrumented pass of
+    `profile_frames` frames (per-kernel-class HIP event times). The learned method records training
+    samples during its warm-up (plain solves, the net off), trains, and then switches the net on."""
     import numpy as np
     from bd.solver import GpuSolver
-    params = cfg["params"]
-    params.coarse_solve = coarse_solve
-    # the mixed method stops coarsening at 33 x 33 and solves that level exactly (FP64)
-    params.max_coarsest_nodes = MIXED_COARSEST_NODES if coarse_solve else 400
-    params.smooth_sweeps = args.sweeps
-    from bd.capi import BD_PRECISION_FP16, BD_PRECISION_FP32
-    params.precision = BD_PRECISION_FP16 if coarse_solve and args.mixed_precision == "fp16" else BD_PRECISION_FP32
+    params = method_params(cfg, method, args)
     gpu = GpuSolver(cfg["scene"], params, cfg["grips"])
+    train = None
+    if method == "learned":
+        gpu.learn_record(3 * args.warmup)
     goals = np.array([g["goal"] for g in cfg["grips"]], dtype=np.float64)
     vel = np.array(cfg["velocity"], dtype=np.float64)
 
@@ -166,6 +191,14 @@ def run_method(args, cfg, coarse_solve, profile_frames):
     for _ in range(args.warmup):
         frame()
     gpu.sync()
+    if method == "learned":
+        t0 = time.perf_counter()
+        losses = gpu.learn_train(args.epochs, lr=1e-3, batch=3)
+        info = gpu.learn_info()
+        gpu.set_learned(True)
+        train = {"samples": info["samples"], "net_level_nodes": info["nodes"], "epochs": args.epochs,
+                 "loss_first_epoch": round(float(losses[0]), 5), "loss_last_epoch": round(float(losses[-1]), 5),
+                 "seconds": round(time.perf_counter() - t0, 2)}
     # the fastest smoother kernel per level and family for this hierarchy (bd_autotune)
     tuned = gpu.autotune(5) if not args.no_autotune else {}
     for pin in args.pin:
@@ -174,16 +207,19 @@ def run_method(args, cfg, coarse_solve, profile_frames):
         gpu.set_smoother(lvl, 1, v)
         tuned = dict(tuned)
         tuned[lvl] = (u, v)
-    iters, newton = [], []
+    iters, newton, relres = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         frame()
         st = gpu.stats()
         iters.append(sum(st["pcg_iterations"]))
         newton.append(st["newton_iterations"])
+        relres.append(max(st["rel_residual"][:st["newton_iterations"]]))
     gpu.sync()
     t1 = time.perf_counter()
+    st = gpu.stats()
     out = {"ms": 1e3 * (t1 - t0) / args.steps, "iters": float(np.mean(iters)), "newton": float(np.mean(newton)),
+           "max_rel_residual": float(max(relres)), "train": train, "learned_fallbacks": st["learned_fallbacks"],
            "levels": gpu.stats()["level_nodes"], "tuned": tuned, "precision": params.precision,
            "sweeps": params.smooth_sweeps}
     gpu.profile_enable(True)
@@ -194,6 +230,10 @@ def run_method(args, cfg, coarse_solve, profile_frames):
     out["finite"] = bool(np.isfinite(gpu.read(0)).all())
     gpu.close()
     return out
+
+
+def ratio(runs, a, b, key):
+    return round(runs[a][key] / runs[b][key], 3) if a in runs and b in runs and runs[b][key] > 0 else None
 
 
 def main():
@@ -208,17 +248,15 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU strip decomposition is "
               f"a design note, not part of this build", file=sys.stderr)
         sys.exit(2)
-    from bd.capi import BD_COARSE_EXACT, BD_COARSE_SWEEPS
     from bd.configs import config_sheet
 
-    cfg = config_sheet(args.size)
+    cfg = config_sheet(args.size, self_contact=not args.no_contact)
     S = cfg["scene"]
     runs = {}
-    if args.method in ("both", "plain"):
-        runs["plain"] = run_method(args, cfg, BD_COARSE_SWEEPS, args.profile_frames if args.method == "plain" else 0)
-    if args.method in ("both", "mixed"):
-        runs["mixed"] = run_method(args, cfg, BD_COARSE_EXACT, args.profile_frames)
-    main_name = "mixed" if "mixed" in runs else "plain"
+    order = ("plain", "mixed", "learned") if args.method == "all" else (args.method,)
+    main_name = order[-1]
+    for m in order:
+        runs[m] = run_method(args, cfg, m, args.profile_frames if m == main_name else 0)
     R = runs[main_name]
     ms = R["ms"]
     prof = R["prof"]
@@ -241,7 +279,6 @@ def main():
     if not args.no_cpu_baseline and os.environ.get("BD_BENCH_NO_CPU") is None:
         P = runs.get("plain", R)
         cpu = cpu_baseline(cfg, P["iters"] / max(P["newton"], 1), P["newton"])
-    both = len(runs) == 2
     out = {
         "metric": "ms/frame & PCG iters to rel-res 1e-6: learned+mixed vs plain MG-PCG",
         "value": round(ms, 3),
@@ -254,27 +291,42 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 (FP64 dots, residual replacement, line search" +
-                 (", coarsest solve; fp16 coarse-level operators in the V-cycle)" if main_name == "mixed" and R["precision"]
-                  else ", coarsest solve)" if main_name == "mixed" else ")"),
+                 (", coarsest solve; fp16 coarse-level operators in the V-cycle" if main_name != "plain" and R["precision"]
+                  else ", coarsest solve" if main_name != "plain" else "") +
+                 ("; bf16 net on the matrix cores, fp32 accumulation)" if main_name == "learned" else ")"),
         "data": "synthetic",
         "config": {"workload": f"sheet{args.size}x{args.size} FP32 MG-PCG to rel-res 1e-6, "
-                               f"central {args.size // 4}x{args.size // 4} breakable yarn region, 2 grips at 5 mm/step",
+                               f"central {args.size // 4}x{args.size // 4} breakable yarn region, 2 grips at 5 mm/step"
+                               + (", self-contact" if not args.no_contact else ""),
                    "nodes": S.node_count, "levels": R["levels"], "newton_per_frame": newton_per_frame,
-                   "method": main_name + (" (FP64 exact coarsest solve)" if main_name == "mixed" else " (line sweeps)")},
+                   "method": {"plain": "plain (reference V-cycle schedule)", "mixed": "mixed (FP64 exact coarsest solve)",
+                              "learned": "learned+mixed (learned coarse-space correction, FP64 exact coarsest solve)"}[main_name]},
         "pcg_iterations_per_frame": iters_per_frame,
         "pcg_iterations_per_newton": round(iters_per_frame / max(newton_per_frame, 1), 2),
         "methods": {k: {"ms_per_frame": round(v["ms"], 3), "pcg_iterations_per_frame": v["iters"],
                         "smooth_sweeps": v["sweeps"], "vcycle_operators": "fp16" if v["precision"] else "fp32",
+                        "max_rel_residual": v["max_rel_residual"], "learned_fallbacks": v["learned_fallbacks"],
+                        "training": v["train"],
                         "smoother_kernels": {str(lv): list(c) for lv, c in v["tuned"].items()}}
                     for k, v in runs.items()},
-        "iteration_ratio_plain_over_mixed": round(runs["plain"]["iters"] / runs["mixed"]["iters"], 3) if both else None,
-        "speedup_plain_over_mixed": round(runs["plain"]["ms"] / runs["mixed"]["ms"], 3) if both else None,
+        "iteration_ratio_plain_over_learned": ratio(runs, "plain", "learned", "iters"),
+        "speedup_plain_over_learned": ratio(runs, "plain", "learned", "ms"),
+        "iteration_ratio_mixed_over_learned": ratio(runs, "mixed", "learned", "iters"),
+        "speedup_mixed_over_learned": ratio(runs, "mixed", "learned", "ms"),
+        "iteration_ratio_plain_over_mixed": ratio(runs, "plain", "mixed", "iters"),
+        "speedup_plain_over_mixed": ratio(runs, "plain", "mixed", "ms"),
         "finite": all(r["finite"] for r in runs.values()),
+        # every timed Newton solve reached the tolerance (a solve stopped by the iteration cap would not)
+        "max_rel_residual": max(r["max_rel_residual"] for r in runs.values()),
+        "converged": all(r["max_rel_residual"] <= 1.05e-6 for r in runs.values()),
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if not out["converged"]:
+        print(f"bench.py: a timed solve stopped above rel-res 1.05e-6 ({out['max_rel_residual']:.3e})", file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

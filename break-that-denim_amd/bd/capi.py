@@ -43,6 +43,10 @@ BD_CONTACT_TABLE_BITS = 17
 BD_CONTACT_NONE = 0xFFFFFFFF
 BD_CONTACT_MAX_NODES = 1 << 24
 
+BD_NET_FIELDS = 8
+BD_NET_PARAMS = 37216
+BD_NET_SLOW_ITER = 4
+
 
 class bd_yarn(C.Structure):
     _fields_ = [("linear_density", C.c_double), ("ea_low", C.c_double), ("ea_high", C.c_double),
@@ -83,7 +87,7 @@ class bd_params(C.Structure):
                 ("pcg_max_iterations", C.c_int32), ("pcg_rel_tol", C.c_double), ("coarse_sweeps", C.c_int32),
                 ("max_coarsest_nodes", C.c_int32), ("lag_coarse", C.c_int32), ("line_search", C.c_int32),
                 ("history", C.c_int32), ("precision", C.c_int32), ("self_contact", C.c_int32), ("coarse_solve", C.c_int32),
-                ("smooth_sweeps", C.c_int32)]
+                ("smooth_sweeps", C.c_int32), ("net_level", C.c_int32)]
 
 
 class bd_grips(C.Structure):
@@ -98,7 +102,8 @@ class bd_stats(C.Structure):
                 ("rel_residual", C.c_double * BD_MAX_NEWTON), ("alpha", C.c_double * BD_MAX_NEWTON),
                 ("max_update", C.c_double * BD_MAX_NEWTON), ("rejections", C.c_int32),
                 ("ruptures", C.c_int32), ("pulled_out", C.c_int32), ("level_count", C.c_int32),
-                ("level_nodes", C.c_int32 * BD_MAX_LEVELS), ("poll_fallbacks", C.c_int32), ("pad_", C.c_int32)]
+                ("level_nodes", C.c_int32 * BD_MAX_LEVELS), ("poll_fallbacks", C.c_int32),
+                ("learned_fallbacks", C.c_int32), ("warm_relres", C.c_double * BD_MAX_NEWTON)]
 
 
 class bd_kernel_profile(C.Structure):
@@ -162,7 +167,7 @@ def make_params(h=1 / 120, gravity=(0.0, 0.0, -9.81), newton_iterations=2, newto
                 newton_tolerance=1e-5, pcg_iterations=2, pcg_rel_tol=0.0, pcg_max_iterations=200,
                 coarse_sweeps=2, max_coarsest_nodes=400, lag_coarse=True, line_search=True, history=True,
                 precision=BD_PRECISION_FP32, self_contact=False, coarse_solve=BD_COARSE_SWEEPS,
-                smooth_sweeps=1) -> bd_params:
+                smooth_sweeps=1, net_level=0) -> bd_params:
     """Solver parameters; the defaults are the reference's production schedule (GPU_DEFAULTS,
     gpu/solver.ts:61-72: h 1/120, 2 Newton x 2 PCG, 2 coarse sweeps, coarsest <= 400 nodes)."""
     p = bd_params()
@@ -183,6 +188,7 @@ def make_params(h=1 / 120, gravity=(0.0, 0.0, -9.81), newton_iterations=2, newto
     p.self_contact = int(bool(self_contact))
     p.coarse_solve = int(coarse_solve)
     p.smooth_sweeps = int(smooth_sweeps)
+    p.net_level = int(net_level)
     return p
 
 
@@ -287,6 +293,16 @@ def load_library(path: os.PathLike | None = None):
     lib.bd_update_topology.argtypes = [vp, i32p, C.c_int, i32p]
     lib.bd_topology_search.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, u8p, u8p, i32p, C.c_int, u8p, u32p, u32p,
                                        u32p, i32p]
+    lib.bd_learn_record.argtypes = [vp, C.c_int]
+    lib.bd_learn_train.argtypes = [vp, C.c_int, C.c_double, C.c_int, f64p]
+    lib.bd_set_learned.argtypes = [vp, C.c_int]
+    lib.bd_learn_get_weights.argtypes = [vp, f32p, C.c_int]
+    lib.bd_learn_set_weights.argtypes = [vp, f32p, C.c_int]
+    lib.bd_learn_loss_grad.argtypes = [vp, C.c_int, f64p, f32p, C.c_int]
+    lib.bd_learn_read_sample.argtypes = [vp, C.c_int, f32p, f64p, f32p, u8p]
+    lib.bd_learn_info.argtypes = [vp, i32p, i32p, i32p, i32p]
+    lib.bd_learn_read_fields.argtypes = [vp, f32p]
+    lib.bd_apply.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     if path is None:
         _lib = lib
     return lib
@@ -303,4 +319,6 @@ EXPORTED = (
     "bd_set_pieces", "bd_update_topology", "bd_topology_search",
     "bd_grip_attach", "bd_grip_lead", "bd_grip_terms", "bd_grip_target", "bd_grip_pinch", "bd_grip_hold",
     "bd_grip_release", "bd_pick_ray", "bd_pick_near", "bd_autotune", "bd_set_smoother",
+    "bd_learn_record", "bd_learn_train", "bd_set_learned", "bd_learn_get_weights", "bd_learn_set_weights",
+    "bd_learn_loss_grad", "bd_learn_read_sample", "bd_learn_info", "bd_learn_read_fields", "bd_apply",
 )

@@ -108,17 +108,18 @@ def config1():
             "steps": 120}
 
 
-def config_sheet(n=1024, pitch=1e-3, region=256, rel_tol=1e-6, history=True, newton=3):
+def config_sheet(n=1024, pitch=1e-3, region=256, rel_tol=1e-6, history=True, newton=3, self_contact=False):
     """n x n sheet at 1 mm pitch with a central breakable woven region (per-yarn lognormal break
     strains, median 0.12, sigma 0.3, seed 7) and two bottom-corner grips pulled apart at 5 mm/step;
-    plain FP32 MG-PCG to rel-res `rel_tol`."""
+    plain FP32 MG-PCG to rel-res `rel_tol`, with node-face self-contact when asked for."""
     mat = linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / (2 * 1.2))
     S = sheet_scene(n, pitch, mat, yarn_region=region)
     g0 = corner_grip(S, 0, 0, k=2e4)
     g1 = corner_grip(S, 1, 0, k=2e4)
     params = make_params(h=1 / 120, newton_iterations=newton, pcg_rel_tol=rel_tol, pcg_max_iterations=200,
-                         max_coarsest_nodes=400, coarse_sweeps=2, lag_coarse=True, line_search=True, history=history)
-    return {"name": f"sheet{n}-yarns{region}", "scene": S, "params": params, "grips": [g0, g1],
+                         max_coarsest_nodes=400, coarse_sweeps=2, lag_coarse=True, line_search=True, history=history,
+                         self_contact=self_contact)
+    return {"name": f"sheet{n}-yarns{region}" + ("-contact" if self_contact else ""), "scene": S, "params": params, "grips": [g0, g1],
             "velocity": [np.array([-5e-3, 0, 0]), np.array([5e-3, 0, 0])], "steps": 240}
 
 
@@ -168,6 +169,64 @@ def fold_scene(nx=48, ny=64, pitch=1e-3, radius=1e-3, material=None):
     S.extra["fold"] = {"L1": L1, "radius": radius, "j_lower": int(np.floor(L1 / pitch)),
                        "j_upper": int(np.ceil((L1 + arc) / pitch))}
     return S
+
+
+def yarn_region_strengths(S, region, material, break_median=0.12, break_sigma=0.3, break_seed=7, i0=None, j0=None):
+    """Per-yarn lognormal break strains in a region x region block of crossings (lower-left corner
+    (i0, j0), centred by default): one strain per warp (column) and per weft (row), turned into the
+    peak tension of the yarn law at that strain; everything else stays unbreakable."""
+    L = S.layers[0]
+    nx, ny = L.nx, L.ny
+    i0 = (nx - region) // 2 if i0 is None else i0
+    j0 = (ny - region) // 2 if j0 is None else j0
+    rng = np.random.default_rng(break_seed)
+    eb_warp = np.exp(math.log(break_median) + break_sigma * rng.standard_normal(region))
+    eb_weft = np.exp(math.log(break_median) + break_sigma * rng.standard_normal(region))
+    kk, mm = np.meshgrid(np.arange(i0, i0 + region), np.arange(j0, j0 + region), indexing="xy")
+    c = (mm * nx + kk).ravel()
+    S.field_strength[2 * c + 1] = _axial_tension(material["yarns"][1], eb_warp)[(kk - i0).ravel()].astype(np.float32)
+    S.field_strength[2 * c + 0] = _axial_tension(material["yarns"][0], eb_weft)[(mm - j0).ravel()].astype(np.float32)
+    S.extra["yarn_region"] = (i0, j0, region)
+    return S
+
+
+def config2(n=256, region=64, pitch=1e-3, steps=240, rel_tol=1e-6, mixed=False):
+    """The plain method at scale: an n x n sheet at 1 mm pitch folded over itself along v (the upper
+    half turned over 2 mm above the lower one) and dropped under gravity with node-face self-contact
+    (1 mm contact distance); a central region x region block of crossings, across the fold, breaks at
+    per-yarn lognormal strains (median 0.12, sigma 0.3, seed 7), the rest is unbreakable; two 3 x 3
+    grips at the lower layer's two front corners pulled apart at 5 mm/step (their goals led at most
+    6 mm ahead of the held patch, the reference's 120 N grip force bound at k = 2e4 N/m);
+    FP32 MG-PCG to rel-res 1e-6, history on. The Galerkin hierarchy is rebuilt at every Newton
+    iteration: the contact terms change between iterations, and with the hierarchy of the first one
+    the later solves need about 3x the iterations (and do not all reach 1e-6 within 400).
+    mixed=True gives the mixed-precision method on the same scene: exact FP64 solve of a coarsest
+    level of <= 1100 nodes, FP32 operators (FP16 coarse-level storage loses the contact terms' range:
+    5x the iterations), FP64 reductions and residual replacement."""
+    from .capi import BD_COARSE_EXACT
+    mat = linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / (2 * 1.2))
+    S = fold_scene(n, n, pitch, radius=1e-3, material=mat)
+    yarn_region_strengths(S, region, mat)
+    g0 = corner_grip(S, 0, 0, k=2e4)
+    g1 = corner_grip(S, 1, 0, k=2e4)
+    params = make_params(h=1 / 120, newton_iterations=3, pcg_rel_tol=rel_tol, pcg_max_iterations=400,
+                         max_coarsest_nodes=1100 if mixed else 400, coarse_sweeps=2, lag_coarse=False, line_search=True,
+                         history=True, self_contact=True, coarse_solve=BD_COARSE_EXACT if mixed else 0)
+    return {"name": f"fold{n}-yarns{region}" + ("-mixed" if mixed else ""), "scene": S, "params": params,
+            "grips": [g0, g1], "velocity": [np.array([-5e-3, 0, 0]), np.array([5e-3, 0, 0])], "steps": steps,
+            "max_lead": 6e-3}
+
+
+def led_goals(goals, velocity, centroids, max_lead):
+    """Grip goals moved by their per-step velocity, each held within max_lead of its patch centroid
+    (the reference's handle lead clamp, a bound on the grip force)."""
+    out = []
+    for g, v, c in zip(goals, velocity, centroids):
+        t = np.asarray(g, dtype=np.float64) + v
+        d = t - c
+        n = np.linalg.norm(d)
+        out.append(c + d * (max_lead / n) if n > max_lead else t)
+    return np.array(out)
 
 
 def config_fold(nx=48, ny=64, pitch=1e-3, radius=1e-3, floor_gap=0.05e-3, newton=3, rel_tol=1e-5):

@@ -310,4 +310,62 @@ class GpuSolver:
             "pulled_out": st.pulled_out,
             "level_nodes": list(st.level_nodes[:st.level_count]),
             "poll_fallbacks": st.poll_fallbacks,
+            "learned_fallbacks": st.learned_fallbacks,
+            "warm_relres": list(st.warm_relres[:max(n, 1)]),
         }
+
+    # ---- learned coarse-space correction (bd_learn_*)
+    def learn_info(self):
+        lv, n, npad, cnt = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.bd_learn_info(self.h, C.byref(lv), C.byref(n), C.byref(npad), C.byref(cnt)), "bd_learn_info")
+        return {"level": lv.value, "nodes": n.value, "padded": npad.value, "samples": cnt.value}
+
+    def learn_record(self, capacity):
+        self._check(self.lib.bd_learn_record(self.h, int(capacity)), "bd_learn_record")
+
+    def learn_train(self, epochs, lr=1e-3, batch=1):
+        losses = np.zeros(max(int(epochs), 1))
+        self._check(self.lib.bd_learn_train(self.h, int(epochs), float(lr), int(batch),
+                                            losses.ctypes.data_as(C.POINTER(C.c_double))), "bd_learn_train")
+        return losses[:int(epochs)]
+
+    def set_learned(self, on):
+        self._check(self.lib.bd_set_learned(self.h, int(bool(on))), "bd_set_learned")
+
+    def learn_weights(self):
+        w = np.zeros(capi.BD_NET_PARAMS, np.float32)
+        self._check(self.lib.bd_learn_get_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_float)), w.size),
+                    "bd_learn_get_weights")
+        return w
+
+    def set_learn_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(self.lib.bd_learn_set_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_float)), w.size),
+                    "bd_learn_set_weights")
+
+    def learn_loss_grad(self, sample):
+        loss = C.c_double()
+        g = np.zeros(capi.BD_NET_PARAMS, np.float32)
+        self._check(self.lib.bd_learn_loss_grad(self.h, int(sample), C.byref(loss),
+                                                g.ctypes.data_as(C.POINTER(C.c_float)), g.size), "bd_learn_loss_grad")
+        return loss.value, g
+
+    def learn_sample(self, sample):
+        """A recorded sample: targets (2, 3, Np_l), energies (2,), operator (78, Np_l), fixed flags (Np_l,)."""
+        info = self.learn_info()
+        npad = info["padded"]
+        b = np.zeros((2, 3, npad), np.float32)
+        e = np.zeros(2)
+        op = np.zeros((78, npad), np.float32)
+        fx = np.zeros(npad, np.uint8)
+        self._check(self.lib.bd_learn_read_sample(self.h, int(sample), b.ctypes.data_as(C.POINTER(C.c_float)),
+                                                  e.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  op.ctypes.data_as(C.POINTER(C.c_float)),
+                                                  fx.ctypes.data_as(C.POINTER(C.c_uint8))), "bd_learn_read_sample")
+        return b, e, op, fx
+
+    def learn_fields(self):
+        npad = self.learn_info()["padded"]
+        z = np.zeros((capi.BD_NET_FIELDS, 3, npad), np.float32)
+        self._check(self.lib.bd_learn_read_fields(self.h, z.ctypes.data_as(C.POINTER(C.c_float))), "bd_learn_read_fields")
+        return z

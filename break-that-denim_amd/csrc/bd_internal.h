@@ -102,6 +102,12 @@ struct Scal {
   int done, iters, max_iters, fixed_iters;
   int rejections, pad;
   int need_true, restart, last, replacements;
+  // learned correction: on for this solve, switched off by the fallback, its gate for the
+  // transfer kernels (done or off), fallbacks taken, the residual history of the fallback test
+  int learn_on, learn_off, lgate, learn_fallbacks;
+  double hist[8];
+  double warm;               // relative residual of the start (1: zero start)
+  double rec_warm[BD_MAX_NEWTON];
   unsigned int ticket;       // last-block-done counter of the current reduction
   unsigned int max_update[BD_MAX_NEWTON];  // float bits
   // per Newton iteration record of the substep
@@ -114,13 +120,15 @@ struct Scal {
 enum ProfClass {
   PC_PREDICT_COMMIT, PC_ASSEMBLE, PC_PROLONG_WEIGHTS, PC_RAP, PC_SMOOTH_FINE, PC_SMOOTH_COARSE, PC_RESIDUAL_FINE,
   PC_RESIDUAL_COARSE, PC_RESTRICT, PC_PROLONG, PC_MATVEC, PC_PCG_VECTOR, PC_TRUE_RESIDUAL, PC_ENERGY, PC_DAMAGE, PC_SLIP,
-  PC_CONTACT_HASH, PC_CONTACT_DETECT, PC_CONTACT_TERMS, PC_TRANSPOSE, PC_COARSE_FACTOR, PC_COARSE_INVERSE, PC_COARSE_APPLY, PC_PIECES, PC_COUNT
+  PC_CONTACT_HASH, PC_CONTACT_DETECT, PC_CONTACT_TERMS, PC_TRANSPOSE, PC_COARSE_FACTOR, PC_COARSE_INVERSE, PC_COARSE_APPLY, PC_PIECES,
+  PC_LEARNED, PC_COUNT
 };
 inline const char* prof_name(int c) {
   static const char* n[PC_COUNT] = {"predict_commit", "assemble", "prolong_weights", "rap", "smooth_fine", "smooth_coarse",
                                     "residual_fine", "residual_coarse", "restrict", "prolong", "matvec_dot", "pcg_vector",
                                     "true_residual", "energy", "damage", "slip", "contact_hash", "contact_detect",
-                                    "contact_terms", "transpose", "coarse_factor", "coarse_inverse", "coarse_apply", "pieces"};
+                                    "contact_terms", "transpose", "coarse_factor", "coarse_inverse", "coarse_apply", "pieces",
+                                    "learned"};
   return n[c];
 }
 
@@ -239,6 +247,9 @@ struct Solver {
   int flag_seq = 0;
   int poll_timeout_ms = 2000;  // BD_POLL_TIMEOUT_MS at creation; 0 takes the synchronizing path at once
   int poll_fallbacks = 0;      // convergence checks that took the synchronizing path
+  // learned coarse-space correction (learned.hip): network, recorded samples, chain vectors
+  struct Learn* learn = nullptr;
+  bool learned = false;        // Newton solves use the learned warm start and correction
   int* counters = nullptr;     // [0] ruptures, [1] pulled out
   // multigrid
   int nlev = 0;
@@ -310,9 +321,40 @@ void launch_zero(Solver& s, float* p, size_t n);
 void launch_colmajor_op(Solver& s, int k);
 void launch_colmajor_x(Solver& s, int k);
 void launch_colmajor_b(Solver& s, int k);
+// a vector's restriction to level k + 1 (xc of the transfer untouched) and prolongation from it,
+// both gated by `gate` (skipped while it is nonzero); `set`: the fine vector is overwritten, else added to
+void launch_restrict_vec(Solver& s, int k, const float* r, float* bc, const int* gate);
+// y = A_k x and x^T A_k x (FP64) into device memory; scratch: kApplyScratch bytes of device memory
+constexpr size_t kApplyScratch = (kDotBlocks + 2) * sizeof(double);
+void launch_apply_dot(Solver& s, int k, const float* x, float* y, double* scratch, double* xax, hipStream_t st);
+void launch_prolong_vec(Solver& s, int k, const float* xc, float* xf, bool set, const int* gate);
 void vcycle(Solver& s, int k);
 // pcg.hip
 void pcg_solve(Solver& s);
+inline const int* pcg_done(Solver& s) { return &s.scal->done; }
+inline const int* learn_off(Solver& s) { return &s.scal->learn_off; }
+inline const int* learn_gate(Solver& s) { return &s.scal->lgate; }
+// learned.hip
+void learn_alloc(Solver& s);
+void learn_free(Solver& s);
+int learn_level(const Solver& s);
+void learn_predict(Solver& s);
+void learn_correct(Solver& s, const float* r, float* z);
+bool learn_ready(const Solver& s);
+void learn_record_start(Solver& s, int capacity);
+bool learn_recording(const Solver& s);
+void learn_record_begin(Solver& s, const float* r);
+void learn_record_mid(Solver& s, const float* r, const double* delta);
+void learn_record_end(Solver& s, const double* delta, const float* rhs);
+int learn_sample_count(const Solver& s);
+void learn_train(Solver& s, int epochs, double lr, int batch, double* losses);
+void learn_loss_grad(Solver& s, int sample, double* loss, float* grad);
+void learn_get_weights(Solver& s, float* out);
+void learn_set_weights(Solver& s, const float* w);
+void learn_read_sample(Solver& s, int sample, float* b, double* st, float* op, uint8_t* fixed);
+void learn_read_z(Solver& s, float* z);
+int learn_padded(const Solver& s);
+int learn_nodes(const Solver& s);
 void launch_energy(Solver& s);
 // history.hip
 void launch_damage(Solver& s);

@@ -190,6 +190,7 @@ void destroy_all(Solver& s) {
   for (int q = 0; q < Solver::kGoalSlots; q++)
     if (s.goal_ev[q]) (void)hipEventDestroy(s.goal_ev[q]);
   for (hipEvent_t e : s.prof_pool) (void)hipEventDestroy(e);
+  learn_free(s);
   s.prof_pool.clear();
   if (s.stream) (void)hipStreamDestroy(s.stream);
 }
@@ -217,6 +218,7 @@ void build_hierarchy(Solver& s) {
   }
   if (s.P.coarse_solve == BD_COARSE_EXACT) launch_coarse_factor(s);
   s.hierarchy = true;
+  if (s.learned) learn_predict(s);
 }
 
 void contact_terms(Solver& s) {
@@ -1131,6 +1133,116 @@ int bd_profile_read(bd_solver* h, bd_kernel_profile* out, int max_entries, int* 
   });
 }
 
+int bd_learn_record(bd_solver* h, int capacity) {
+  return guarded(h, [&](Solver& s) {
+    if (capacity < 0) return fail(h, BD_ERR_INVALID, "bd_learn_record: negative capacity");
+    BD_HIP(hipStreamSynchronize(s.stream));
+    learn_record_start(s, capacity);
+    return BD_OK;
+  });
+}
+
+int bd_learn_train(bd_solver* h, int epochs, double lr, int batch, double* losses) {
+  return guarded(h, [&](Solver& s) {
+    if (epochs < 0 || !(lr > 0) || batch < 1) return fail(h, BD_ERR_INVALID, "bd_learn_train: epochs >= 0, lr > 0, batch >= 1");
+    if (learn_sample_count(s) == 0) return fail(h, BD_ERR_STATE, "bd_learn_train: no recorded samples");
+    learn_train(s, epochs, lr, batch, losses);
+    return BD_OK;
+  });
+}
+
+int bd_set_learned(bd_solver* h, int on) {
+  return guarded(h, [&](Solver& s) {
+    if (on) {
+      if (s.nlev < 2) return fail(h, BD_ERR_STATE, "bd_set_learned: the hierarchy has a single level");
+      learn_alloc(s);
+    }
+    s.learned = on != 0;
+    // the next Newton iteration predicts from its hierarchy
+    if (s.learned && s.hierarchy) learn_predict(s);
+    return BD_OK;
+  });
+}
+
+int bd_learn_get_weights(bd_solver* h, float* out, int count) {
+  return guarded(h, [&](Solver& s) {
+    if (!out || count != BD_NET_PARAMS) return fail(h, BD_ERR_INVALID, "bd_learn_get_weights: need BD_NET_PARAMS floats");
+    if (s.nlev < 2) return fail(h, BD_ERR_STATE, "bd_learn_get_weights: the hierarchy has a single level");
+    learn_alloc(s);
+    learn_get_weights(s, out);
+    return BD_OK;
+  });
+}
+
+int bd_learn_set_weights(bd_solver* h, const float* w, int count) {
+  return guarded(h, [&](Solver& s) {
+    if (!w || count != BD_NET_PARAMS) return fail(h, BD_ERR_INVALID, "bd_learn_set_weights: need BD_NET_PARAMS floats");
+    if (s.nlev < 2) return fail(h, BD_ERR_STATE, "bd_learn_set_weights: the hierarchy has a single level");
+    learn_alloc(s);
+    learn_set_weights(s, w);
+    if (s.learned && s.hierarchy) learn_predict(s);
+    return BD_OK;
+  });
+}
+
+int bd_learn_loss_grad(bd_solver* h, int sample, double* loss, float* grad, int count) {
+  return guarded(h, [&](Solver& s) {
+    if (!loss || (grad && count != BD_NET_PARAMS)) return fail(h, BD_ERR_INVALID, "bd_learn_loss_grad: bad output");
+    if (sample < 0 || sample >= learn_sample_count(s)) return fail(h, BD_ERR_INVALID, "bd_learn_loss_grad: no such sample");
+    learn_loss_grad(s, sample, loss, grad);
+    if (s.learned && s.hierarchy) learn_predict(s);
+    return BD_OK;
+  });
+}
+
+int bd_learn_read_sample(bd_solver* h, int sample, float* b, double* energy, float* op, uint8_t* fixed) {
+  return guarded(h, [&](Solver& s) {
+    if (sample < 0 || sample >= learn_sample_count(s)) return fail(h, BD_ERR_INVALID, "bd_learn_read_sample: no such sample");
+    learn_read_sample(s, sample, b, energy, op, fixed);
+    return BD_OK;
+  });
+}
+
+int bd_learn_info(bd_solver* h, int* level, int* nodes, int* padded, int* samples) {
+  return guarded(h, [&](Solver& s) {
+    if (s.nlev < 2) return fail(h, BD_ERR_STATE, "bd_learn_info: the hierarchy has a single level");
+    learn_alloc(s);
+    if (level) *level = learn_level(s);
+    if (nodes) *nodes = learn_nodes(s);
+    if (padded) *padded = learn_padded(s);
+    if (samples) *samples = learn_sample_count(s);
+    return BD_OK;
+  });
+}
+
+int bd_learn_read_fields(bd_solver* h, float* z) {
+  return guarded(h, [&](Solver& s) {
+    if (!z) return fail(h, BD_ERR_INVALID, "bd_learn_read_fields: null output");
+    if (!learn_ready(s)) return fail(h, BD_ERR_STATE, "bd_learn_read_fields: no prediction yet");
+    learn_read_z(s, z);
+    return BD_OK;
+  });
+}
+
+int bd_apply(bd_solver* h, int level, const float* x, float* y, double* xax, void* stream) {
+  return guarded(h, [&](Solver& s) {
+    if (level < 0 || level >= s.nlev) return fail(h, BD_ERR_INVALID, "bd_apply: no such level");
+    if (!x || !y || !xax) return fail(h, BD_ERR_INVALID, "bd_apply: null pointer");
+    if (!s.assembled) return fail(h, BD_ERR_STATE, "bd_apply: no assembled operator");
+    if (level > 0 && !s.hierarchy) return fail(h, BD_ERR_STATE, "bd_apply: no Galerkin hierarchy");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s.stream;
+    // a workspace per call: the caller's stream may run beside the solver's own work
+    void* ws = nullptr;
+    BD_HIP(hipMalloc(&ws, kApplyScratch));
+    if (st != s.stream) BD_HIP(hipStreamSynchronize(s.stream));
+    launch_apply_dot(s, level, x, y, static_cast<double*>(ws), xax, st);
+    BD_HIP(hipGetLastError());
+    BD_HIP(hipStreamSynchronize(st));
+    BD_HIP(hipFree(ws));
+    return BD_OK;
+  });
+}
+
 int bd_sync(bd_solver* h) {
   return guarded(h, [&](Solver& s) {
     BD_HIP(hipStreamSynchronize(s.stream));
@@ -1437,6 +1549,8 @@ int bd_get_stats(bd_solver* h, bd_stats* out) {
     }
     st.rejections = s.scal_h->rejections;
     st.poll_fallbacks = s.poll_fallbacks;
+    st.learned_fallbacks = s.scal_h->learn_fallbacks;
+    for (int k = 0; k < BD_MAX_NEWTON; k++) st.warm_relres[k] = s.scal_h->rec_warm[k];
     st.ruptures = counters[0];
     st.pulled_out = counters[1];
     *out = st;

@@ -8,6 +8,7 @@
 
 #include "bd_internal.h"
 #include "oprd.h"
+#include "reduce.h"
 
 namespace bd {
 
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
   }
   for (int q = 0; q < 3; q++) {
     X.bc[q * Npc + I] = s[q];
-    X.xc[q * Npc + I] = 0.f;
+    if (X.xc) X.xc[q * Npc + I] = 0.f;    // null for a plain restriction of a vector
     if (X.xcT) X.xcT[q * Npc + I] = 0.f;  // every entry of the permuted copy is cleared as well
   }
 }
@@ -677,6 +678,56 @@ void launch_prolong_add(Solver& s, int k) {
     k_prolong_add_tiled<<<prolong_tiles(L.geo), 256, 0, s.stream>>>(X);
   else
     k_prolong_add<<<grid_for(L.N, 256), 256, 0, s.stream>>>(X);
+}
+
+namespace {
+
+// y = A x on a level (fixed rows 0) and x^T A x in FP64; the reduction's block partials, its ticket
+// and the result live in the caller's scratch (kDotBlocks doubles, then the ticket, then the result)
+template <bool FINE>
+__global__ __launch_bounds__(1024) void k_apply_dot(LvArgs a, const float* x, float* y, double* scratch, double* out) {
+  double v[1] = {0};
+  const int Np = a.Np;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < a.N; n += gridDim.x * blockDim.x) {
+    float r[3] = {0.f, 0.f, 0.f};
+    if (!a.fixed[n]) apply_row<FINE>(a, x, n, r);
+    for (int q = 0; q < 3; q++) {
+      y[q * Np + n] = r[q];
+      v[0] += (double)x[q * Np + n] * (double)r[q];
+    }
+  }
+  double tot[1];
+  if (grid_sum<1>(v, scratch, reinterpret_cast<unsigned int*>(scratch + kDotBlocks), tot)) *out = tot[0];
+}
+
+__global__ void k_clear_ticket(double* scratch) { *reinterpret_cast<unsigned int*>(scratch + kDotBlocks) = 0u; }
+
+}  // namespace
+
+void launch_apply_dot(Solver& s, int k, const float* x, float* y, double* scratch, double* xax, hipStream_t st) {
+  Level& L = s.lv[k];
+  const int g = std::min(kDotBlocks, grid_for(L.N, 1024));
+  k_clear_ticket<<<1, 1, 0, st>>>(scratch);
+  if (L.fine)
+    k_apply_dot<true><<<g, 1024, 0, st>>>(lv_args(L), x, y, scratch, xax);
+  else
+    k_apply_dot<false><<<g, 1024, 0, st>>>(lv_args(L), x, y, scratch, xax);
+}
+
+void launch_restrict_vec(Solver& s, int k, const float* r, float* bc, const int* gate) {
+  Level& L = s.lv[k];
+  Level& Cl = s.lv[k + 1];
+  XferArgs X{lv_args(L), Cl.geo, Cl.N, Cl.Np, L.pw, Cl.fixed, r, bc, nullptr, nullptr, gate, nullptr, nullptr};
+  k_restrict<<<grid_for(Cl.N, 256), 256, 0, s.stream>>>(X);
+}
+
+void launch_prolong_vec(Solver& s, int k, const float* xc, float* xf, bool set, const int* gate) {
+  Level& L = s.lv[k];
+  Level& Cl = s.lv[k + 1];
+  if (set) launch_zero(s, xf, 3 * (size_t)L.Np);
+  XferArgs X{lv_args(L), Cl.geo, Cl.N, Cl.Np, L.pw, Cl.fixed, nullptr, nullptr, const_cast<float*>(xc), xf, gate, nullptr,
+             nullptr};
+  k_prolong_add<<<grid_for(L.N, 256), 256, 0, s.stream>>>(X);
 }
 
 // vcycle (newtonMultigrid.ts:495-528) on level k's x / b buffers, x zero on entry

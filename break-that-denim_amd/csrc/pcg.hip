@@ -15,70 +15,11 @@
 
 #include "bd_internal.h"
 #include "bounds.h"
+#include "reduce.h"
 
 namespace bd {
 
 namespace {
-
-__device__ __forceinline__ void store_agent(double* p, double v) {
-  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __double_as_longlong(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_agent(const double* p) {
-  return __longlong_as_double(__hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-
-// Grid-wide deterministic sum of NV doubles per thread. Returns true in thread 0 of the last block
-// to finish, with the totals in tot. Every sum runs in a fixed order (per block: wave shuffles, then
-// waves in order; the last block: thread t sums partials t, t + blockDim, ..., then the same block
-// tree), so the result does not depend on which block finishes last.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double (&sh)[NV][16]) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < NV; q++) {
-    double x = v[q];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-    if (lane == 0) sh[q][wid] = x;
-  }
-}
-
-template <int NV>
-__device__ bool grid_sum(double (&v)[NV], double* partials, unsigned int* ticket, double (&tot)[NV]) {
-  __shared__ double sh[NV][16];
-  __shared__ int last;
-  const int nw = (blockDim.x + 63) >> 6;
-  block_sum<NV>(v, sh);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < NV; q++) {
-      double s = 0;
-      for (int w = 0; w < nw; w++) s += sh[q][w];
-      store_agent(&partials[blockIdx.x * NV + q], s);
-    }
-    __threadfence();
-    unsigned int t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double acc[NV];
-#pragma unroll
-  for (int q = 0; q < NV; q++) acc[q] = 0;
-  for (unsigned int b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-#pragma unroll
-    for (int q = 0; q < NV; q++) acc[q] += load_agent(&partials[b * NV + q]);
-  block_sum<NV>(acc, sh);
-  __syncthreads();
-  if (threadIdx.x != 0) return false;
-  for (int q = 0; q < NV; q++) {
-    double s = 0;
-    for (int w = 0; w < nw; w++) s += sh[q][w];
-    tot[q] = s;
-  }
-  *ticket = 0u;
-  return true;
-}
 
 struct PcgArgs {
   int N, Np;
@@ -120,20 +61,34 @@ __global__ __launch_bounds__(1024) void k_pcg_start(PcgArgs A) {
     s->last = 0;
     s->replacements = 0;
     s->done = (s->tol > 0 && !(tot[0] > 0)) ? 1 : 0;
+    s->learn_off = 0;
+    s->warm = 1.0;
+    for (int q = 0; q < 8; q++) s->hist[q] = 0;
   }
 }
 
-// rz = r . z ; beta = rz / rz_prev (first, or after a residual replacement: beta = 0)
+// rz = r . z ; beta = rz / rz_prev (first, or after a residual replacement: beta = 0). While the
+// learned correction is on the preconditioner may change between iterations (the fallback switches
+// it off), so beta is the flexible one of FCG(1): beta = -(z . A p_prev) / (p_prev . A p_prev), with
+// A p_prev still in ap (Notay, Flexible conjugate gradients, SIAM J. Sci. Comput. 22, 2000).
 __global__ __launch_bounds__(1024) void k_dot_rz(PcgArgs A, int first) {
   if (A.sc->done) return;
-  double v[1] = {0};
+  double v[2] = {0, 0};
   const int Np = A.Np;
+  const bool flex = A.sc->learn_on;
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < A.N; n += gridDim.x * blockDim.x)
-    for (int q = 0; q < 3; q++) v[0] += (double)A.r[q * Np + n] * (double)A.z[q * Np + n];
-  double tot[1];
-  if (grid_sum<1>(v, A.partials, &A.sc->ticket, tot)) {
+    for (int q = 0; q < 3; q++) {
+      const double z = A.z[q * Np + n];
+      v[0] += (double)A.r[q * Np + n] * z;
+      if (flex) v[1] += z * (double)A.ap[q * Np + n];
+    }
+  double tot[2];
+  if (grid_sum<2>(v, A.partials, &A.sc->ticket, tot)) {
     Scal* s = A.sc;
-    s->beta = (first || s->restart) ? 0.0 : tot[0] / s->rz;
+    if (first || s->restart)
+      s->beta = 0.0;
+    else
+      s->beta = flex ? (s->pap > 0 ? -tot[1] / s->pap : 0.0) : tot[0] / s->rz;
     s->restart = 0;
     s->rz = tot[0];
   }
@@ -169,6 +124,16 @@ __global__ __launch_bounds__(1024) void k_update_xr(PcgArgs A) {
     s->rr = tot[0];
     s->relres = s->bb > 0 ? sqrt(tot[0] / s->bb) : 0.0;
     s->iters += 1;
+    if (s->learn_on && !s->learn_off) {
+      // fallback to plain MG-PCG when the residual has not dropped over the last 8 iterations
+      const int it = s->iters;
+      if (it > 8 && !(s->relres < s->hist[it % 8])) {
+        s->learn_off = 1;
+        s->restart = 1;
+        s->learn_fallbacks += 1;
+      }
+      s->hist[it % 8] = s->relres;
+    }
     if (s->tol > 0) {
       // the recursive FP32 residual drifts from the true one: confirm with an FP64 recomputation
       s->last = s->iters >= s->max_iters;
@@ -699,6 +664,7 @@ __global__ void k_record(Scal* sc, int it) {
   sc->rec_iters[it] = sc->iters;
   sc->rec_relres[it] = sc->relres;
   sc->rec_alpha[it] = sc->line_alpha;
+  sc->rec_warm[it] = sc->warm;
 }
 
 __global__ void k_reset_counters(Scal* sc) {
@@ -707,10 +673,53 @@ __global__ void k_reset_counters(Scal* sc) {
     sc->rec_iters[k] = 0;
     sc->rec_relres[k] = 0;
     sc->rec_alpha[k] = 0;
+    sc->rec_warm[k] = 1.0;
   }
 }
 
-__global__ void k_pcg_setup(Scal* sc, double tol, int max_iters, int fixed_iters) {
+// the learned warm start: delta = z (the correction applied to b), z cleared; the true residual follows
+__global__ __launch_bounds__(256) void k_warm_apply(PcgArgs A) {
+  if (A.sc->done) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n == 0) A.sc->need_true = 1;
+  if (n >= A.N) return;
+  const int Np = A.Np;
+  for (int q = 0; q < 3; q++) {
+    A.delta[q * Np + n] = A.z[q * Np + n];
+    A.z[q * Np + n] = 0.f;
+  }
+}
+
+// A warm start d0 = P Z y with y = (Z^T A_l Z)^-1 Z^T R b lowers the energy 1/2 d^T A d - b^T d by
+// 1/2 c^T y >= 0 (the A-norm error never grows); its 2-norm residual may still exceed ||b||, so the
+// start is only dropped when it is not finite or absurd (zero start, r = b, correction off).
+__global__ __launch_bounds__(256) void k_warm_guard(PcgArgs A) {
+  Scal* sc = A.sc;
+  const bool bad = !(sc->relres < 1e3);
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!bad) {
+    if (n == 0) sc->warm = sc->relres;
+    return;
+  }
+  if (n == 0) {
+    // the fields gave no usable start (no decrease, or not finite): plain MG-PCG for this solve
+    sc->warm = 1.0;
+    sc->done = 0;
+    sc->relres = 1.0;
+    sc->restart = 1;
+    sc->learn_off = 1;
+    sc->learn_fallbacks += 1;
+  }
+  if (n >= A.N) return;
+  const int Np = A.Np;
+  for (int q = 0; q < 3; q++) {
+    A.delta[q * Np + n] = 0.0;
+    A.r[q * Np + n] = A.fixed[n] ? 0.f : A.rhs[q * Np + n];
+  }
+}
+
+__global__ void k_pcg_setup(Scal* sc, double tol, int max_iters, int fixed_iters, int learn) {
+  sc->learn_on = learn;
   sc->tol = tol;
   sc->max_iters = max_iters;
   sc->fixed_iters = fixed_iters;
@@ -764,10 +773,23 @@ void pcg_solve(Solver& s) {
   const int K = tol_mode ? std::max(1, s.P.pcg_max_iterations) : std::max(0, s.P.pcg_iterations);
   const int G = dot_grid(s.N), B = grid_for(s.N, 256);
   const double N = s.N;
-  k_pcg_setup<<<1, 1, 0, s.stream>>>(s.scal, tol_mode ? s.P.pcg_rel_tol : 0.0, K, K);
+  const bool learned = s.learned && learn_ready(s) && s.npieces == 0;
+  const bool record = !learned && tol_mode && learn_recording(s);
+  k_pcg_setup<<<1, 1, 0, s.stream>>>(s.scal, tol_mode ? s.P.pcg_rel_tol : 0.0, K, K, learned ? 1 : 0);
   {
     ProfScope ps(s, PC_PCG_VECTOR, N * 85.0);
     k_pcg_start<<<G, kDotThreads, 0, s.stream>>>(A);
+  }
+  if (record) learn_record_begin(s, A.r);
+  if (learned && K > 0) {
+    // warm start d0 = P Z Ginv Z^T R b, kept only if it reduces the residual
+    learn_correct(s, A.r, A.z);
+    k_warm_apply<<<B, 256, 0, s.stream>>>(A);
+    {
+      ProfScope ps(s, PC_TRUE_RESIDUAL, 0.0);
+      k_true_residual<<<G, kDotThreads, 0, s.stream>>>(A, s.lv[0].op, s.lv[0].geo);
+    }
+    k_warm_guard<<<B, 256, 0, s.stream>>>(A);
   }
   if (s.npieces > 0) {
     // detached pieces (rigid_apply.wgsl): start from each piece's exact rigid response x0 and the
@@ -779,6 +801,7 @@ void pcg_solve(Solver& s) {
   }
   if (K == 0) return;
   vcycle(s, 0);
+  if (learned) learn_correct(s, A.r, A.z);
   {
     ProfScope ps(s, PC_PCG_VECTOR, N * 24.0);
     k_dot_rz<<<G, kDotThreads, 0, s.stream>>>(A, 1);
@@ -801,6 +824,7 @@ void pcg_solve(Solver& s) {
       ProfScope ps(s, PC_TRUE_RESIDUAL, 0.0);
       k_true_residual<<<G, kDotThreads, 0, s.stream>>>(A, s.lv[0].op, s.lv[0].geo);
     }
+    if (record && it + 1 == BD_NET_SLOW_ITER) learn_record_mid(s, A.r, A.delta);
     if (it + 1 == K) break;
     if (tol_mode) {
       // the device decides convergence; check it before queueing the next V-cycle
@@ -810,6 +834,7 @@ void pcg_solve(Solver& s) {
       if (wait_flag(s, seq)) break;
     }
     vcycle(s, 0);
+    if (learned) learn_correct(s, A.r, A.z);
     {
       ProfScope ps(s, PC_PCG_VECTOR, N * 24.0);
       k_dot_rz<<<G, kDotThreads, 0, s.stream>>>(A, 0);
@@ -819,6 +844,7 @@ void pcg_solve(Solver& s) {
       k_update_p<<<B, 256, 0, s.stream>>>(A);
     }
   }
+  if (record) learn_record_end(s, A.delta, A.rhs);
 }
 
 void launch_record(Solver& s, int it) { k_record<<<1, 1, 0, s.stream>>>(s.scal, it); }

@@ -141,6 +141,7 @@ typedef struct bd_params {
   int32_t self_contact;        /* node-face self-contact (GpuSolverParams.selfContact, gpu/solver.ts:49-50) */
   int32_t coarse_solve;        /* BD_COARSE_*: how the V-cycle solves its coarsest level */
   int32_t smooth_sweeps;       /* pre- and post-smoothing sweeps per level of the V-cycle (the reference: 1) */
+  int32_t net_level;           /* level of the learned coarse-space network (0: the first with <= 70,000 nodes) */
 } bd_params;
 
 #define BD_COARSE_SWEEPS 0     /* coarse_sweeps colour-ordered line sweeps (the reference's V-cycle) */
@@ -177,7 +178,8 @@ typedef struct bd_stats {
   int32_t level_count;
   int32_t level_nodes[BD_MAX_LEVELS];
   int32_t poll_fallbacks;                 /* PCG convergence checks that took the synchronizing path (total) */
-  int32_t pad_;
+  int32_t learned_fallbacks;              /* learned solves that fell back to plain MG-PCG (total) */
+  double warm_relres[BD_MAX_NEWTON];      /* ||b - A d0|| / ||b|| of each Newton solve's start (1: zero start) */
 } bd_stats;
 
 /* Buffers for bd_read / bd_write. `level` selects the multigrid level where it applies. */
@@ -373,6 +375,48 @@ typedef struct bd_kernel_profile {
 } bd_kernel_profile;
 int bd_profile_enable(bd_solver* s, int on);
 int bd_profile_read(bd_solver* s, bd_kernel_profile* out, int max_entries, int* count);
+
+/* Learned coarse-space correction (no counterpart in the reference). A message-passing network on
+ * one coarse level of the Galerkin hierarchy (bd_params.net_level) maps that level's operator and
+ * the restricted Newton right-hand side to BD_NET_FIELDS displacement fields Z; prolonged to the fine
+ * level they give the Newton solve an A-optimal warm start in span(P Z) and an additive coarse
+ * correction P Z (Z^T A_l Z)^-1 Z^T P^T r inside flexible PCG. Weights are BF16 on the matrix cores
+ * with FP32 accumulation and FP32 master copies. The solve falls back to plain MG-PCG (correction
+ * off, zero start, direction restarted) when the fields give a start that is not finite, and when
+ * its residual fails to drop over 8 iterations. Training samples come from plain solves: the net
+ * level's operator and, as targets,
+ * the restricted right-hand side with the Newton direction's energy and the restricted residual left
+ * after BD_NET_SLOW_ITER iterations with the remaining error's energy. */
+#define BD_NET_FIELDS 8
+#define BD_NET_PARAMS 37216
+#define BD_NET_SLOW_ITER 4
+/* Record up to `capacity` training samples from the following plain Newton solves (0: stop and free). */
+int bd_learn_record(bd_solver* s, int capacity);
+/* Adam on the GPU over the recorded samples (`batch` samples per step); losses[e] receives epoch e's
+ * mean loss (the fraction of the targets' energy that span(P Z) misses), may be NULL. */
+int bd_learn_train(bd_solver* s, int epochs, double lr, int batch, double* losses);
+/* 1: Newton solves use the learned warm start and correction (flexible PCG); 0: plain MG-PCG. */
+int bd_set_learned(bd_solver* s, int on);
+/* FP32 master weights (count = BD_NET_PARAMS). */
+int bd_learn_get_weights(bd_solver* s, float* out, int count);
+int bd_learn_set_weights(bd_solver* s, const float* w, int count);
+/* Loss and weight gradient of recorded sample `sample` at the current weights (gradient checks). */
+int bd_learn_loss_grad(bd_solver* s, int sample, double* loss, float* grad, int count);
+/* A recorded sample: targets (2 x 3 planes of the net level's padded size), energies (2), operator
+ * (78 planes) and fixed flags; any pointer may be NULL. */
+int bd_learn_read_sample(bd_solver* s, int sample, float* b, double* energy, float* op, uint8_t* fixed);
+/* The net level, its node count and padded size, and the number of recorded samples. */
+int bd_learn_info(bd_solver* s, int* level, int* nodes, int* padded, int* samples);
+/* The last prediction's fields, BD_NET_FIELDS x 3 planes of the net level's padded size. */
+int bd_learn_read_fields(bd_solver* s, float* z);
+
+/* y = A x with the assembled operator of `level` (0 fine, k > 0 Galerkin level k; fixed rows 0) and
+ * x^T A x accumulated in FP64, for callers that run their own Krylov loops on the solver's operators
+ * (the reference's callers encode passes over GpuSolver.levels[k].op, gpu/solver.ts:574-728).
+ * x and y: device pointers to 3 planes of the level's padded node count (BD_BUF_LEVEL_X layout);
+ * xax: a device pointer to one double. Enqueued on `stream` (a hipStream_t; NULL: the solver's
+ * stream) and waited for. Allocates a small reduction workspace for the call. */
+int bd_apply(bd_solver* s, int level, const float* x, float* y, double* xax, void* stream);
 
 int bd_sync(bd_solver* s);
 int bd_read(bd_solver* s, bd_buffer buf, int level, void* dst, size_t bytes);
