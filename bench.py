@@ -67,7 +67,7 @@ def parse():
                          "(0 direct, 4 / 8 staged, 16 split); e.g. under a kernel tracer, whose per-dispatch cost "
                          "counts against the two-kernel split variant in the autotuner's timing")
     ap.add_argument("--method", choices=("all", "plain", "mixed", "learned"), default="all")
-    ap.add_argument("--epochs", type=int, default=40, help="training epochs of the learned method's net")
+    ap.add_argument("--epochs", type=int, default=200, help="training epochs of the learned method's net")
     ap.add_argument("--no-contact", action="store_true", help="leave node-face self-contact off")
     ap.add_argument("--sweeps", type=int, default=1, help="pre- and post-smoothing sweeps per level")
     ap.add_argument("--mixed-precision", choices=("fp16", "fp32"), default="fp32",
@@ -193,7 +193,7 @@ def run_method(args, cfg, method, profile_frames):
     gpu.sync()
     if method == "learned":
         t0 = time.perf_counter()
-        losses = gpu.learn_train(args.epochs, lr=1e-3, batch=3)
+        losses = gpu.learn_train(args.epochs, lr=3e-3, batch=3, targets=3)
         info = gpu.learn_info()
         gpu.set_learned(True)
         train = {"samples": info["samples"], "net_level_nodes": info["nodes"], "epochs": args.epochs,

@@ -294,7 +294,7 @@ def load_library(path: os.PathLike | None = None):
     lib.bd_topology_search.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, u8p, u8p, i32p, C.c_int, u8p, u32p, u32p,
                                        u32p, i32p]
     lib.bd_learn_record.argtypes = [vp, C.c_int]
-    lib.bd_learn_train.argtypes = [vp, C.c_int, C.c_double, C.c_int, f64p]
+    lib.bd_learn_train.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_int, f64p]
     lib.bd_set_learned.argtypes = [vp, C.c_int]
     lib.bd_learn_get_weights.argtypes = [vp, f32p, C.c_int]
     lib.bd_learn_set_weights.argtypes = [vp, f32p, C.c_int]

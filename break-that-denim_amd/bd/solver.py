@@ -323,9 +323,9 @@ class GpuSolver:
     def learn_record(self, capacity):
         self._check(self.lib.bd_learn_record(self.h, int(capacity)), "bd_learn_record")
 
-    def learn_train(self, epochs, lr=1e-3, batch=1):
+    def learn_train(self, epochs, lr=1e-3, batch=1, targets=3):
         losses = np.zeros(max(int(epochs), 1))
-        self._check(self.lib.bd_learn_train(self.h, int(epochs), float(lr), int(batch),
+        self._check(self.lib.bd_learn_train(self.h, int(epochs), float(lr), int(batch), int(targets),
                                             losses.ctypes.data_as(C.POINTER(C.c_double))), "bd_learn_train")
         return losses[:int(epochs)]
 

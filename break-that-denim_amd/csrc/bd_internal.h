@@ -327,6 +327,8 @@ void launch_restrict_vec(Solver& s, int k, const float* r, float* bc, const int*
 // y = A_k x and x^T A_k x (FP64) into device memory; scratch: kApplyScratch bytes of device memory
 constexpr size_t kApplyScratch = (kDotBlocks + 2) * sizeof(double);
 void launch_apply_dot(Solver& s, int k, const float* x, float* y, double* scratch, double* xax, hipStream_t st);
+// y = A_k x with FP64 products and sums, rounded once to FP32
+void launch_apply_f64(Solver& s, int k, const float* x, float* y);
 void launch_prolong_vec(Solver& s, int k, const float* xc, float* xf, bool set, const int* gate);
 void vcycle(Solver& s, int k);
 // pcg.hip
@@ -340,6 +342,7 @@ void learn_free(Solver& s);
 int learn_level(const Solver& s);
 void learn_predict(Solver& s);
 void learn_correct(Solver& s, const float* r, float* z);
+void learn_deflate(Solver& s, float* z);
 bool learn_ready(const Solver& s);
 void learn_record_start(Solver& s, int capacity);
 bool learn_recording(const Solver& s);
@@ -347,7 +350,7 @@ void learn_record_begin(Solver& s, const float* r);
 void learn_record_mid(Solver& s, const float* r, const double* delta);
 void learn_record_end(Solver& s, const double* delta, const float* rhs);
 int learn_sample_count(const Solver& s);
-void learn_train(Solver& s, int epochs, double lr, int batch, double* losses);
+void learn_train(Solver& s, int epochs, double lr, int batch, int targets, double* losses);
 void learn_loss_grad(Solver& s, int sample, double* loss, float* grad);
 void learn_get_weights(Solver& s, float* out);
 void learn_set_weights(Solver& s, const float* w);

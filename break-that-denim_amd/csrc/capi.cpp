@@ -27,6 +27,7 @@ using namespace bd;
 
 static thread_local std::string g_create_error;
 
+
 namespace {
 
 template <class T>
@@ -1142,11 +1143,12 @@ int bd_learn_record(bd_solver* h, int capacity) {
   });
 }
 
-int bd_learn_train(bd_solver* h, int epochs, double lr, int batch, double* losses) {
+int bd_learn_train(bd_solver* h, int epochs, double lr, int batch, int targets, double* losses) {
   return guarded(h, [&](Solver& s) {
     if (epochs < 0 || !(lr > 0) || batch < 1) return fail(h, BD_ERR_INVALID, "bd_learn_train: epochs >= 0, lr > 0, batch >= 1");
+    if (targets < 1 || targets > 3) return fail(h, BD_ERR_INVALID, "bd_learn_train: targets is 1 (warm start), 2 (slow modes) or 3");
     if (learn_sample_count(s) == 0) return fail(h, BD_ERR_STATE, "bd_learn_train: no recorded samples");
-    learn_train(s, epochs, lr, batch, losses);
+    learn_train(s, epochs, lr, batch, targets, losses);
     return BD_OK;
   });
 }

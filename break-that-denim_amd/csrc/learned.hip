@@ -4,9 +4,13 @@
 //   warm start          d0 = P Z (Z^T A_l Z)^-1 Z^T R b        (A-optimal start in span(P Z))
 //   coarse correction   z = V(r) + P Z (Z^T A_l Z)^-1 Z^T R r   (additive, inside flexible PCG)
 // where R = P^T restricts the fine residual down to the net's level l and P prolongs back, and
-// A_l = P^T A P is the level's Galerkin operator, so Z^T A_l Z is the exact Gram matrix of the
-// prolonged fields. The network and its training are this build's own (the reference has no
-// learned part).
+// A_l = P^T A P is the level's Galerkin operator, so Z^T A_l Z is the Gram matrix of the prolonged
+// fields. Inside PCG the fields deflate: with W = A P Z stored per Newton system and
+// G = (P Z)^T W, every preconditioned residual is projected, z <- z - P Z G^-1 W^T z, which with
+// the warm start keeps the residual orthogonal to span(P Z) (deflated CG: Saad, Yeung, Erhel and
+// Guyomarc'h, SIAM J. Sci. Comput. 21, 2000); fields that capture little cannot make the iteration
+// worse than without them, the ones that capture slow modes remove them. The network and its
+// training are this build's own (the reference has no learned part).
 //
 // Network (BF16 weights and activations, FP32 accumulation on the matrix cores, MFMA
 // 16x16x32 bf16): per level node, 32 input features (lattice position encodings, fixed flag, the
@@ -19,7 +23,7 @@
 // (the warm-start target) and the error the plain solve still had after kSlowIter iterations (the
 // slow modes the coarse correction must capture). The loss is the fraction of each target's
 // energy that span(P Z) misses,
-//   L = sum_t w_t (1 - c_t^T y_t / s_t),  c_t = Z^T b_t,  y_t = (Z^T A_l Z + mu I)^-1 c_t,
+//   L = sum_t w_t (1 - c_t^T y_t / s_t),  c_t = Z^T b_t,  y_t = (Z^T A_l Z)^+ c_t,
 // whose gradient with respect to Z is dL/dZ = -sum_t (2 w_t / s_t) (b_t - A_l Z y_t) y_t^T.
 // Backward through the layers runs on the same matrix cores; Adam updates FP32 master weights.
 #include <hip/hip_runtime.h>
@@ -283,14 +287,16 @@ __device__ __forceinline__ void ld_blk(const float* op, int Np, int base, int ro
   for (int q = 0; q < 6; q++) o[q] = op[(base + q) * Np + row];
 }
 
-// A_l Z for all fields at once: each coupling block is loaded once and applied to every field
+// A_l Z for all fields at once: each coupling block is loaded once and applied to every field. The
+// products and sums are FP64: for smooth fields a row's terms cancel to a small fraction of their
+// size, which FP32 accumulation would not resolve (the Gram matrix and the loss depend on it).
 __global__ __launch_bounds__(256) void k_block_apply(SubArgs A) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= A.N) return;
   const int Np = A.Np;
-  float y[BD_NET_FIELDS][3];
+  double y[BD_NET_FIELDS][3];
 #pragma unroll
-  for (int k = 0; k < BD_NET_FIELDS; k++) y[k][0] = y[k][1] = y[k][2] = 0.f;
+  for (int k = 0; k < BD_NET_FIELDS; k++) y[k][0] = y[k][1] = y[k][2] = 0.0;
   if (!A.fixed[n]) {
     int l, i, j;
     lv_ij(A.geo, n, l, i, j);
@@ -299,12 +305,10 @@ __global__ __launch_bounds__(256) void k_block_apply(SubArgs A) {
 #pragma unroll
       for (int k = 0; k < BD_NET_FIELDS; k++) {
         const float* zk = A.z + (size_t)3 * k * Np;
-        const float x[3] = {zk[m], zk[Np + m], zk[2 * Np + m]};
-        float t[3];
-        sym_mul(b, x, t);
-        y[k][0] += t[0];
-        y[k][1] += t[1];
-        y[k][2] += t[2];
+        const double x0 = zk[m], x1 = zk[Np + m], x2 = zk[2 * Np + m];
+        y[k][0] += (double)b[0] * x0 + (double)b[1] * x1 + (double)b[2] * x2;
+        y[k][1] += (double)b[1] * x0 + (double)b[3] * x1 + (double)b[4] * x2;
+        y[k][2] += (double)b[2] * x0 + (double)b[4] * x1 + (double)b[5] * x2;
       }
     };
     float b[6];
@@ -326,7 +330,7 @@ __global__ __launch_bounds__(256) void k_block_apply(SubArgs A) {
 #pragma unroll
   for (int k = 0; k < BD_NET_FIELDS; k++)
 #pragma unroll
-    for (int q = 0; q < 3; q++) A.az[(size_t)(3 * k + q) * Np + n] = y[k][q];
+    for (int q = 0; q < 3; q++) A.az[(size_t)(3 * k + q) * Np + n] = (float)y[k][q];
 }
 
 struct GramArgs {
@@ -369,36 +373,59 @@ __global__ __launch_bounds__(1024) void k_gram_row(GramArgs A) {
   }
 }
 
-// (G + mu I)^-1 by Cholesky in FP64 (mu = kRidge times the largest diagonal entry); a column of Z
-// that is zero or dependent on the others gets the ridge alone
-constexpr double kRidge = 1e-6;
+// The pseudo-inverse of the symmetric Gram matrix G in FP64: Jacobi eigen-decomposition, then the
+// eigenvalues below kPinvTol times the largest dropped. Fields that are zero or (nearly) dependent
+// on the others then add nothing, and P Z G^+ (P Z)^T A stays an A-orthogonal projection.
+constexpr double kPinvTol = 1e-7;
 __device__ void inverse_spd(const double* G, double* inv) {
   constexpr int K = BD_NET_FIELDS;
-  double L[K][K], dmax = 0;
-  for (int a = 0; a < K; a++) dmax = fmax(dmax, G[a * K + a]);
-  const double mu = kRidge * dmax + 1e-300;
-  for (int a = 0; a < K; a++)
-    for (int b = 0; b <= a; b++) {
-      double s = G[a * K + b] + (a == b ? mu : 0.0);
-      for (int q = 0; q < b; q++) s -= L[a][q] * L[b][q];
-      if (a == b)
-        L[a][a] = sqrt(fmax(s, mu));
-      else
-        L[a][b] = s / L[b][b];
+  double a[K][K], v[K][K];
+  for (int i = 0; i < K; i++)
+    for (int j = 0; j < K; j++) {
+      a[i][j] = 0.5 * (G[i * K + j] + G[j * K + i]);
+      v[i][j] = i == j ? 1.0 : 0.0;
     }
-  for (int col = 0; col < K; col++) {
-    double y[K];
-    for (int a = 0; a < K; a++) {
-      double s = a == col ? 1.0 : 0.0;
-      for (int q = 0; q < a; q++) s -= L[a][q] * y[q];
-      y[a] = s / L[a][a];
-    }
-    for (int a = K - 1; a >= 0; a--) {
-      double s = y[a];
-      for (int q = a + 1; q < K; q++) s -= L[q][a] * inv[q * K + col];
-      inv[a * K + col] = s / L[a][a];
-    }
+  for (int sweep = 0; sweep < 50; sweep++) {
+    double off = 0, tot = 0;
+    for (int i = 0; i < K; i++)
+      for (int j = 0; j < K; j++) {
+        tot += a[i][j] * a[i][j];
+        if (i != j) off += a[i][j] * a[i][j];
+      }
+    if (!(off > 1e-30 * tot)) break;
+    for (int p = 0; p < K - 1; p++)
+      for (int q = p + 1; q < K; q++) {
+        if (a[p][q] == 0.0) continue;
+        const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < K; k++) {
+          const double akp = a[k][p], akq = a[k][q];
+          a[k][p] = c * akp - sn * akq;
+          a[k][q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < K; k++) {
+          const double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = c * apk - sn * aqk;
+          a[q][k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < K; k++) {
+          const double vkp = v[k][p], vkq = v[k][q];
+          v[k][p] = c * vkp - sn * vkq;
+          v[k][q] = sn * vkp + c * vkq;
+        }
+      }
   }
+  double lmax = 0;
+  for (int i = 0; i < K; i++) lmax = fmax(lmax, a[i][i]);
+  double d[K];
+  for (int i = 0; i < K; i++) d[i] = (a[i][i] > kPinvTol * lmax && a[i][i] > 0) ? 1.0 / a[i][i] : 0.0;
+  for (int i = 0; i < K; i++)
+    for (int j = 0; j < K; j++) {
+      double s = 0;
+      for (int k = 0; k < K; k++) s += v[i][k] * d[k] * v[j][k];
+      inv[i * K + j] = s;
+    }
 }
 
 struct LossScal {
@@ -415,11 +442,11 @@ struct LossScal {
 };
 
 // the inverse, the coefficients y_t and the sample's loss (one thread)
-__global__ void k_solve_small(LossScal* L, const double* s_t, int ntargets) {
+__global__ void k_solve_small(LossScal* L, const double* s_t, int ntargets, int mask) {
   inverse_spd(L->G, L->Ginv);
   double loss = 0, wsum = 0;
   for (int t = 0; t < kTargets; t++) {
-    const bool on = t < ntargets && s_t && s_t[t] > 0;
+    const bool on = t < ntargets && s_t && s_t[t] > 0 && ((mask >> t) & 1);
     double cy = 0;
     for (int a = 0; a < BD_NET_FIELDS; a++) {
       double y = 0;
@@ -433,7 +460,7 @@ __global__ void k_solve_small(LossScal* L, const double* s_t, int ntargets) {
     }
   }
   for (int t = 0; t < kTargets; t++) {
-    const bool on = t < ntargets && s_t && s_t[t] > 0;
+    const bool on = t < ntargets && s_t && s_t[t] > 0 && ((mask >> t) & 1);
     L->coef[t] = on ? -2.0 / (wsum * s_t[t]) : 0.0;
   }
   L->loss = wsum > 0 ? loss / wsum : 0.0;
@@ -713,6 +740,81 @@ __global__ __launch_bounds__(1024) void k_corr_dots(CorrArgs A) {
   }
 }
 
+// W^T z over the fine level (the deflation's dots), y = -Ginv W^T z into L->y
+struct DeflArgs {
+  int N, Np;
+  const uint8_t* fixed;
+  const float* w;  // kNetK fields x 3 planes of the fine level
+  const float* z;
+  double* partials;
+  unsigned int* ticket;
+  LossScal* L;
+  const int* done;
+  const int* off;
+};
+
+__global__ __launch_bounds__(1024) void k_defl_dots(DeflArgs A) {
+  if (*A.done || *A.off) return;
+  double v[BD_NET_FIELDS];
+#pragma unroll
+  for (int k = 0; k < BD_NET_FIELDS; k++) v[k] = 0;
+  const int Np = A.Np;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < A.N; n += gridDim.x * blockDim.x) {
+    if (A.fixed[n]) continue;
+    const double z0 = A.z[n], z1 = A.z[Np + n], z2 = A.z[2 * Np + n];
+#pragma unroll
+    for (int k = 0; k < BD_NET_FIELDS; k++) {
+      const float* wk = A.w + (size_t)3 * k * Np;
+      v[k] += z0 * wk[n] + z1 * wk[Np + n] + z2 * wk[2 * Np + n];
+    }
+  }
+  double tot[BD_NET_FIELDS];
+  if (grid_sum<BD_NET_FIELDS>(v, A.partials, A.ticket, tot)) {
+    for (int a = 0; a < BD_NET_FIELDS; a++) {
+      double y = 0;
+      for (int b = 0; b < BD_NET_FIELDS; b++) y += A.L->Ginv[a * BD_NET_FIELDS + b] * tot[b];
+      A.L->y[a] = -y;
+    }
+  }
+}
+
+// column `col` of G = (P Z)^T (A P Z) from the restricted column R A P Z_col at the net level
+__global__ __launch_bounds__(1024) void k_gram_col(CorrArgs A, int col) {
+  double v[BD_NET_FIELDS];
+#pragma unroll
+  for (int k = 0; k < BD_NET_FIELDS; k++) v[k] = 0;
+  const int Np = A.Np;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < A.N; n += gridDim.x * blockDim.x) {
+    if (A.fixed[n]) continue;
+    const double t0 = A.t[n], t1 = A.t[Np + n], t2 = A.t[2 * Np + n];
+#pragma unroll
+    for (int k = 0; k < BD_NET_FIELDS; k++) {
+      const float* zk = A.z + (size_t)3 * k * Np;
+      v[k] += t0 * zk[n] + t1 * zk[Np + n] + t2 * zk[2 * Np + n];
+    }
+  }
+  double tot[BD_NET_FIELDS];
+  if (grid_sum<BD_NET_FIELDS>(v, A.partials, A.ticket, tot))
+    for (int a = 0; a < BD_NET_FIELDS; a++) A.L->G[a * BD_NET_FIELDS + col] = tot[a];
+}
+
+// the symmetric part of G (FP32 rounding leaves it slightly asymmetric) and its inverse
+__global__ void k_gram_inverse(LossScal* L) {
+  for (int a = 0; a < BD_NET_FIELDS; a++)
+    for (int b = 0; b < a; b++) {
+      const double m = 0.5 * (L->G[a * BD_NET_FIELDS + b] + L->G[b * BD_NET_FIELDS + a]);
+      L->G[a * BD_NET_FIELDS + b] = L->G[b * BD_NET_FIELDS + a] = m;
+    }
+  inverse_spd(L->G, L->Ginv);
+}
+
+// one field of Z into the level vector t
+__global__ __launch_bounds__(256) void k_copy_field(const float* z, int Np, int k, float* t) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= Np) return;
+  for (int q = 0; q < 3; q++) t[q * Np + n] = z[(size_t)(3 * k + q) * Np + n];
+}
+
 // t = Z y
 __global__ __launch_bounds__(256) void k_corr_combine(CorrArgs A) {
   if (*A.done || *A.off) return;
@@ -752,6 +854,9 @@ struct Learn {
   uint16_t* mean[kMp] = {};
   float* z = nullptr;
   float* az = nullptr;
+  float* wf = nullptr;    // W = A P Z on the fine level (kNetK fields x 3 planes)
+  float* ftmp = nullptr;  // a fine vector
+  double* dummy = nullptr;
   uint16_t* dz = nullptr;
   float* dh[2] = {};
   float* du = nullptr;
@@ -788,7 +893,7 @@ T* lalloc(size_t n) {
 int pick_level(const Solver& s) {
   if (s.P.net_level > 0) return std::min(s.P.net_level, s.nlev - 1);
   for (int k = 1; k < s.nlev; k++)
-    if (s.lv[k].N <= 70000) return k;
+    if (s.lv[k].N <= 5000) return k;
   return s.nlev - 1;
 }
 
@@ -837,7 +942,8 @@ void forward(Learn& G, Solver& s, const float* op, const uint8_t* fixed, const f
 }
 
 // A Z, the Gram matrix and c_t for targets b, its inverse, y_t and the loss
-void subspace(Learn& G, Solver& s, const float* op, const uint8_t* fixed, const float* const* b, const double* s_t) {
+void subspace(Learn& G, Solver& s, const float* op, const uint8_t* fixed, const float* const* b, const double* s_t,
+              int mask = 3) {
   const Geo g = level_geo(s, G.level);
   SubArgs S{op, fixed, G.Nl, G.Nlp, g, G.z, G.az};
   k_block_apply<<<grid_for(G.Nl, 256), 256, 0, s.stream>>>(S);
@@ -846,7 +952,7 @@ void subspace(Learn& G, Solver& s, const float* op, const uint8_t* fixed, const 
                G.L->c, row};
     k_gram_row<<<std::min(kDotBlocks, grid_for(G.Nl, 1024)), 1024, 0, s.stream>>>(A);
   }
-  k_solve_small<<<1, 1, 0, s.stream>>>(G.L, s_t, b ? kTargets : 0);
+  k_solve_small<<<1, 1, 0, s.stream>>>(G.L, s_t, b ? kTargets : 0, mask);
 }
 
 void backward(Learn& G, Solver& s, const uint8_t* fixed, const float* const* b) {
@@ -923,6 +1029,9 @@ void learn_free(Solver& s) {
   for (int l = 0; l < kMp; l++) f(G->mean[l]);
   f(G->z);
   f(G->az);
+  f(G->wf);
+  f(G->ftmp);
+  f(G->dummy);
   f(G->dz);
   f(G->dh[0]);
   f(G->dh[1]);
@@ -969,6 +1078,9 @@ void learn_alloc(Solver& s) {
   }
   for (int l = 0; l < kMp; l++) G->mean[l] = lalloc<uint16_t>(Np * kHid);
   G->z = lalloc<float>(Np * 3 * BD_NET_FIELDS);
+  G->wf = lalloc<float>((size_t)s.Np * 3 * BD_NET_FIELDS);
+  G->ftmp = lalloc<float>((size_t)s.Np * 3);
+  G->dummy = lalloc<double>(1);
   G->az = lalloc<float>(Np * 3 * BD_NET_FIELDS);
   G->dz = lalloc<uint16_t>(Np * kOut);
   G->dh[0] = lalloc<float>(Np * kHid);
@@ -989,19 +1101,43 @@ int learn_level(const Solver& s) { return s.learn ? s.learn->level : -1; }
 int learn_padded(const Solver& s) { return s.learn ? s.learn->Nlp : 0; }
 int learn_nodes(const Solver& s) { return s.learn ? s.learn->Nl : 0; }
 
-// after every hierarchy build while the learned correction is on: Z from the net level's operator,
-// A Z and the inverse Gram matrix
+__global__ void k_learn_gate(Scal* sc) { sc->lgate = (sc->done || sc->learn_off) ? 1 : 0; }
+
+// after every hierarchy build while the learned correction is on: Z from the net level's operator
+// and the restricted rhs; then per field W = A P Z on the fine level and G = (P Z)^T W (exact for the
+// current fine operator), and G's inverse
 void learn_predict(Solver& s) {
   Learn& G = *s.learn;
   const Level& L = s.lv[G.level];
   ProfScope ps(s, PC_LEARNED, 0.0);
   restrict_to(G, s, s.rhs, nullptr);
   forward(G, s, L.op, L.fixed, G.chain[G.level]);
-  subspace(G, s, L.op, L.fixed, nullptr, nullptr);
+  CorrArgs C{G.Nl, G.Nlp, L.fixed, G.z, G.chain[G.level], G.rpart, &G.L->ticket, G.L, nullptr, nullptr};
+  for (int k = 0; k < BD_NET_FIELDS; k++) {
+    k_copy_field<<<grid_for(G.Nlp, 256), 256, 0, s.stream>>>(G.z, G.Nlp, k, G.chain[G.level]);
+    launch_zero(s, G.ftmp, 3 * (size_t)s.Np);
+    prolong_from(G, s, G.ftmp, nullptr);
+    float* wk = G.wf + (size_t)3 * k * s.Np;
+    launch_apply_f64(s, 0, G.ftmp, wk);
+    restrict_to(G, s, wk, nullptr);
+    k_gram_col<<<std::min(kDotBlocks, grid_for(G.Nl, 1024)), 1024, 0, s.stream>>>(C, k);
+  }
+  k_gram_inverse<<<1, 1, 0, s.stream>>>(G.L);
   G.have_z = true;
 }
 
-__global__ void k_learn_gate(Scal* sc) { sc->lgate = (sc->done || sc->learn_off) ? 1 : 0; }
+// the deflation of a preconditioned residual: z -= P Z Ginv W^T z (gated like the correction)
+void learn_deflate(Solver& s, float* z) {
+  Learn& G = *s.learn;
+  const int* gate = learn_gate(s);
+  ProfScope ps(s, PC_LEARNED, 0.0);
+  k_learn_gate<<<1, 1, 0, s.stream>>>(s.scal);
+  DeflArgs D{s.N, s.Np, s.pinned, G.wf, z, G.rpart, &G.L->ticket, G.L, pcg_done(s), learn_off(s)};
+  k_defl_dots<<<std::min(kDotBlocks, grid_for(s.N, 1024)), 1024, 0, s.stream>>>(D);
+  CorrArgs C{G.Nl, G.Nlp, s.lv[G.level].fixed, G.z, G.chain[G.level], G.rpart, &G.L->ticket, G.L, pcg_done(s), learn_off(s)};
+  k_corr_combine<<<grid_for(G.Nl, 256), 256, 0, s.stream>>>(C);
+  prolong_from(G, s, z, gate);
+}
 
 // z += P Z Ginv Z^T R r (gated by PCG's done flag and the correction's off switch)
 void learn_correct(Solver& s, const float* r, float* z) {
@@ -1125,7 +1261,7 @@ void learn_record_end(Solver& s, const double* delta, const float* rhs) {
 int learn_sample_count(const Solver& s) { return s.learn ? s.learn->count : 0; }
 
 // Adam over the recorded samples; losses[e] = mean loss of epoch e (before its updates)
-void learn_train(Solver& s, int epochs, double lr, int batch, double* losses) {
+void learn_train(Solver& s, int epochs, double lr, int batch, int targets, double* losses) {
   Learn& G = *s.learn;
   if (G.count == 0) throw std::invalid_argument("no recorded training samples");
   batch = std::max(1, batch);
@@ -1142,7 +1278,7 @@ void learn_train(Solver& s, int epochs, double lr, int batch, double* losses) {
         const int c = order[q];
         const float* b[kTargets] = {G.s_b[c], G.s_b[c] + 3 * (size_t)G.Nlp};
         forward(G, s, G.s_op[c], G.s_fixed[c], b[0]);
-        subspace(G, s, G.s_op[c], G.s_fixed[c], b, G.s_s + (size_t)c * kTargets);
+        subspace(G, s, G.s_op[c], G.s_fixed[c], b, G.s_s + (size_t)c * kTargets, targets);
         backward(G, s, G.s_fixed[c], b);
       }
       G.adam_t += 1;

@@ -702,6 +702,31 @@ __global__ __launch_bounds__(1024) void k_apply_dot(LvArgs a, const float* x, fl
 
 __global__ void k_clear_ticket(double* scratch) { *reinterpret_cast<unsigned int*>(scratch + kDotBlocks) = 0u; }
 
+// y = A x on a level with every product and sum in FP64 (fixed rows 0): for smooth x the row sums
+// cancel to a small fraction of their terms, which FP32 accumulation would not resolve
+template <bool FINE>
+__global__ __launch_bounds__(256) void k_apply_f64(LvArgs a, const float* x, float* y) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.N) return;
+  const int Np = a.Np;
+  double r[3] = {0, 0, 0};
+  if (!a.fixed[n]) {
+    int l, i, j;
+    node_ij(a.geo, n, l, i, j);
+    auto add = [&](const float* b, int m) {
+      const double x0 = x[m], x1 = x[Np + m], x2 = x[2 * Np + m];
+      r[0] += (double)b[0] * x0 + (double)b[1] * x1 + (double)b[2] * x2;
+      r[1] += (double)b[1] * x0 + (double)b[3] * x1 + (double)b[4] * x2;
+      r[2] += (double)b[2] * x0 + (double)b[4] * x1 + (double)b[5] * x2;
+    };
+    float c[6];
+    blk_load(a.op, Np, 0, n, c);
+    add(c, n);
+    for_couplings<FINE>(a.op, Np, n, i, j, a.geo.nx[l], a.geo.ny[l], [&](int, int, int m, const float* b) { add(b, m); });
+  }
+  for (int q = 0; q < 3; q++) y[q * Np + n] = (float)r[q];
+}
+
 }  // namespace
 
 void launch_apply_dot(Solver& s, int k, const float* x, float* y, double* scratch, double* xax, hipStream_t st) {
@@ -712,6 +737,14 @@ void launch_apply_dot(Solver& s, int k, const float* x, float* y, double* scratc
     k_apply_dot<true><<<g, 1024, 0, st>>>(lv_args(L), x, y, scratch, xax);
   else
     k_apply_dot<false><<<g, 1024, 0, st>>>(lv_args(L), x, y, scratch, xax);
+}
+
+void launch_apply_f64(Solver& s, int k, const float* x, float* y) {
+  Level& L = s.lv[k];
+  if (L.fine)
+    k_apply_f64<true><<<grid_for(L.N, 256), 256, 0, s.stream>>>(lv_args(L), x, y);
+  else
+    k_apply_f64<false><<<grid_for(L.N, 256), 256, 0, s.stream>>>(lv_args(L), x, y);
 }
 
 void launch_restrict_vec(Solver& s, int k, const float* r, float* bc, const int* gate) {

@@ -125,14 +125,17 @@ __global__ __launch_bounds__(1024) void k_update_xr(PcgArgs A) {
     s->relres = s->bb > 0 ? sqrt(tot[0] / s->bb) : 0.0;
     s->iters += 1;
     if (s->learn_on && !s->learn_off) {
-      // fallback to plain MG-PCG when the residual has not dropped over the last 8 iterations
+      // fallback to plain MG-PCG when the residual has not dropped below its best for 16 iterations
+      // (hist[0]: the best relative residual, hist[1]: its iteration)
       const int it = s->iters;
-      if (it > 8 && !(s->relres < s->hist[it % 8])) {
+      if (it == 1 || s->relres < s->hist[0]) {
+        s->hist[0] = s->relres;
+        s->hist[1] = it;
+      } else if (it - (int)s->hist[1] >= 16) {
         s->learn_off = 1;
         s->restart = 1;
         s->learn_fallbacks += 1;
       }
-      s->hist[it % 8] = s->relres;
     }
     if (s->tol > 0) {
       // the recursive FP32 residual drifts from the true one: confirm with an FP64 recomputation
@@ -801,7 +804,7 @@ void pcg_solve(Solver& s) {
   }
   if (K == 0) return;
   vcycle(s, 0);
-  if (learned) learn_correct(s, A.r, A.z);
+  if (learned) learn_deflate(s, A.z);
   {
     ProfScope ps(s, PC_PCG_VECTOR, N * 24.0);
     k_dot_rz<<<G, kDotThreads, 0, s.stream>>>(A, 1);
@@ -834,7 +837,7 @@ void pcg_solve(Solver& s) {
       if (wait_flag(s, seq)) break;
     }
     vcycle(s, 0);
-    if (learned) learn_correct(s, A.r, A.z);
+    if (learned) learn_deflate(s, A.z);
     {
       ProfScope ps(s, PC_PCG_VECTOR, N * 24.0);
       k_dot_rz<<<G, kDotThreads, 0, s.stream>>>(A, 0);

@@ -383,7 +383,7 @@ int bd_profile_read(bd_solver* s, bd_kernel_profile* out, int max_entries, int* 
  * correction P Z (Z^T A_l Z)^-1 Z^T P^T r inside flexible PCG. Weights are BF16 on the matrix cores
  * with FP32 accumulation and FP32 master copies. The solve falls back to plain MG-PCG (correction
  * off, zero start, direction restarted) when the fields give a start that is not finite, and when
- * its residual fails to drop over 8 iterations. Training samples come from plain solves: the net
+ * its residual fails to drop below its best for 16 iterations. Training samples come from plain solves: the net
  * level's operator and, as targets,
  * the restricted right-hand side with the Newton direction's energy and the restricted residual left
  * after BD_NET_SLOW_ITER iterations with the remaining error's energy. */
@@ -392,9 +392,11 @@ int bd_profile_read(bd_solver* s, bd_kernel_profile* out, int max_entries, int* 
 #define BD_NET_SLOW_ITER 4
 /* Record up to `capacity` training samples from the following plain Newton solves (0: stop and free). */
 int bd_learn_record(bd_solver* s, int capacity);
-/* Adam on the GPU over the recorded samples (`batch` samples per step); losses[e] receives epoch e's
- * mean loss (the fraction of the targets' energy that span(P Z) misses), may be NULL. */
-int bd_learn_train(bd_solver* s, int epochs, double lr, int batch, double* losses);
+/* Adam on the GPU over the recorded samples (`batch` samples per step) for the targets in the mask
+ * `targets` (1: the Newton directions, the warm start's job; 2: the slow-mode errors, the deflation's
+ * job; 3: both); losses[e] receives epoch e's mean loss (the fraction of the targets' energy that
+ * span(P Z) misses), may be NULL. */
+int bd_learn_train(bd_solver* s, int epochs, double lr, int batch, int targets, double* losses);
 /* 1: Newton solves use the learned warm start and correction (flexible PCG); 0: plain MG-PCG. */
 int bd_set_learned(bd_solver* s, int on);
 /* FP32 master weights (count = BD_NET_PARAMS). */

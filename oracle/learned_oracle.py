@@ -9,7 +9,7 @@ build's own network from its definition:
   encoder    h0 = relu(x We + be)                                      (32 -> 64)
   4 layers   h  = h + relu([h, mean of the lattice neighbours' h] W + b)  (128 -> 64)
   decoder    z  = h Wd + bd                                            (64 -> 32, 24 used: 8 fields x 3)
-  loss       L = sum_t w_t (1 - c_t^T y_t / s_t),  c_t = Z^T b_t,  y_t = (Z^T A Z + mu I)^-1 c_t
+  loss       L = sum_t w_t (1 - c_t^T y_t / s_t),  c_t = Z^T b_t,  y_t = (Z^T A Z)^+ c_t
 
 `emulate_bf16` rounds at the points where the GPU stores bf16 (inputs, layer outputs, neighbour
 means, the weights; in the backward the output gradient and every ReLU-masked gradient), so the
@@ -27,7 +27,7 @@ import math
 import numpy as np
 
 N_IN, N_HID, N_MP, N_OUT, K_FIELDS = 32, 64, 4, 32, 8
-RIDGE = 1e-6
+PINV_TOL = 1e-7
 
 # (in, out, weight offset, bias offset) of the flat parameter vector, weights in-major W[i][o]
 LAYERS = [(N_IN, N_HID, 0, N_IN * N_HID)]
@@ -146,6 +146,14 @@ def coarse_apply(op, x, nx, ny, fixed):
     return y
 
 
+def pinv_gram(G):
+    """Pseudo-inverse of the symmetric Gram matrix: eigenvalues below PINV_TOL times the largest are
+    dropped (the GPU's Jacobi decomposition does the same)."""
+    w, V = np.linalg.eigh(0.5 * (G + G.T))
+    d = np.where((w > PINV_TOL * w.max()) & (w > 0), 1.0 / np.where(w > 0, w, 1.0), 0.0)
+    return (V * d) @ V.T
+
+
 def fields(z):
     """(N, 24) decoder output -> list of 8 (N, 3) fields (column 3 k + q is field k, component q)."""
     return [z[:, 3 * k:3 * k + 3] for k in range(K_FIELDS)]
@@ -158,8 +166,7 @@ def loss_and_dz(z, op, nx, ny, fixed, b, s):
     AZ = [coarse_apply(op, f, nx, ny, fixed) for f in Z]
     free = (fixed[:len(z)] == 0)[:, None]
     G = np.array([[np.sum(Z[a] * AZ[c] * free) for c in range(K_FIELDS)] for a in range(K_FIELDS)])
-    mu = RIDGE * max(np.max(np.diag(G)), 0.0) + 1e-300
-    Ginv = np.linalg.inv(G + mu * np.eye(K_FIELDS))
+    Ginv = pinv_gram(G)
     on = [t for t in range(len(b)) if s[t] > 0]
     loss = 0.0
     dz = np.zeros_like(z)
