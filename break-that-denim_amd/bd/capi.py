@@ -303,6 +303,7 @@ def load_library(path: os.PathLike | None = None):
     lib.bd_learn_info.argtypes = [vp, i32p, i32p, i32p, i32p]
     lib.bd_learn_read_fields.argtypes = [vp, f32p]
     lib.bd_apply.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+    lib.bd_apply_noalloc.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     if path is None:
         _lib = lib
     return lib
@@ -321,4 +322,5 @@ EXPORTED = (
     "bd_grip_release", "bd_pick_ray", "bd_pick_near", "bd_autotune", "bd_set_smoother",
     "bd_learn_record", "bd_learn_train", "bd_set_learned", "bd_learn_get_weights", "bd_learn_set_weights",
     "bd_learn_loss_grad", "bd_learn_read_sample", "bd_learn_info", "bd_learn_read_fields", "bd_apply",
+    "bd_apply_noalloc",
 )

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -27,6 +28,32 @@ using namespace bd;
 
 static thread_local std::string g_create_error;
 
+// Scratch of bd_apply_noalloc: the FP64 block partials of its reduction (kDotBlocks doubles), its
+// ticket and its result slot. It does not grow with the level (the reduction grid is capped at
+// kDotBlocks blocks). Every device the library runs on holds its own copy until the process ends,
+// and every call on that device shares it: calls that overlap on one device race.
+static __device__ __align__(256) double scratch[kDotBlocks + 2];
+static_assert(sizeof(scratch) >= kApplyScratch, "bd_apply_noalloc scratch");
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch_ptr;  // per device: the array's device address
+
+// the scratch array's address on the current device, looked up once per device (the array's host
+// name is only a stand-in for it)
+static hipError_t resolve_scratch(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
 
 namespace {
 
@@ -363,6 +390,15 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     g_create_error = "bd_create: no HIP device";
     return BD_ERR_HIP;
+  }
+  {
+    // set-up of bd_apply_noalloc's scratch on this device
+    double* sp = nullptr;
+    const hipError_t e = resolve_scratch(&sp);
+    if (e != hipSuccess) {
+      g_create_error = std::string("bd_create: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return BD_ERR_HIP;
+    }
   }
   bd_solver* h = new bd_solver();
   Solver& s = h->s;
@@ -1241,6 +1277,24 @@ int bd_apply(bd_solver* h, int level, const float* x, float* y, double* xax, voi
     BD_HIP(hipGetLastError());
     BD_HIP(hipStreamSynchronize(st));
     BD_HIP(hipFree(ws));
+    return BD_OK;
+  });
+}
+
+int bd_apply_noalloc(bd_solver* h, int level, const float* x, float* y, double* xax, void* stream) {
+  return guarded(h, [&](Solver& s) {
+    if (level < 0 || level >= s.nlev) return fail(h, BD_ERR_INVALID, "bd_apply_noalloc: no such level");
+    if (!x || !y || !xax) return fail(h, BD_ERR_INVALID, "bd_apply_noalloc: null pointer");
+    if (!s.assembled) return fail(h, BD_ERR_STATE, "bd_apply_noalloc: no assembled operator");
+    if (level > 0 && !s.hierarchy) return fail(h, BD_ERR_STATE, "bd_apply_noalloc: no Galerkin hierarchy");
+    double* sp = nullptr;
+    const hipError_t e = resolve_scratch(&sp);
+    if (e != hipSuccess || !sp) return fail(h, BD_ERR_HIP, std::string("bd_apply_noalloc: scratch: ") + hipGetErrorString(e));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s.stream;
+    if (st != s.stream) BD_HIP(hipStreamSynchronize(s.stream));
+    launch_apply_dot(s, level, x, y, sp, xax, st);
+    BD_HIP(hipGetLastError());
+    BD_HIP(hipStreamSynchronize(st));
     return BD_OK;
   });
 }

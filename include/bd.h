@@ -419,6 +419,12 @@ int bd_learn_read_fields(bd_solver* s, float* z);
  * xax: a device pointer to one double. Enqueued on `stream` (a hipStream_t; NULL: the solver's
  * stream) and waited for. Allocates a small reduction workspace for the call. */
 int bd_apply(bd_solver* s, int level, const float* x, float* y, double* xax, void* stream);
+/* bd_apply without any allocation (no counterpart in the reference): for callers that cannot
+ * allocate, such as code under HIP graph capture or a tight serving loop. Its reduction scratch is a
+ * fixed device array of (256 + 2) doubles per device, set up by bd_create and shared by every call on
+ * that device, so calls that overlap on one device race. The scratch does not grow with the level
+ * (the reduction grid is capped), so every level fits; results equal bd_apply's bit for bit. */
+int bd_apply_noalloc(bd_solver* s, int level, const float* x, float* y, double* xax, void* stream);
 
 int bd_sync(bd_solver* s);
 int bd_read(bd_solver* s, bd_buffer buf, int level, void* dst, size_t bytes);
