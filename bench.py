@@ -68,6 +68,8 @@ def parse():
                          "counts against the two-kernel split variant in the autotuner's timing")
     ap.add_argument("--method", choices=("all", "plain", "mixed", "learned"), default="all")
     ap.add_argument("--epochs", type=int, default=200, help="training epochs of the learned method's net")
+    ap.add_argument("--learned-mode", type=int, default=1, choices=(1, 2),
+                    help="1: the learned warm start; 2: the warm start and the deflated correction")
     ap.add_argument("--no-contact", action="store_true", help="leave node-face self-contact off")
     ap.add_argument("--sweeps", type=int, default=1, help="pre- and post-smoothing sweeps per level")
     ap.add_argument("--mixed-precision", choices=("fp16", "fp32"), default="fp32",
@@ -195,7 +197,7 @@ def run_method(args, cfg, method, profile_frames):
         t0 = time.perf_counter()
         losses = gpu.learn_train(args.epochs, lr=3e-3, batch=3, targets=3)
         info = gpu.learn_info()
-        gpu.set_learned(True)
+        gpu.set_learned(args.learned_mode)
         train = {"samples": info["samples"], "net_level_nodes": info["nodes"], "epochs": args.epochs,
                  "loss_first_epoch": round(float(losses[0]), 5), "loss_last_epoch": round(float(losses[-1]), 5),
                  "seconds": round(time.perf_counter() - t0, 2)}

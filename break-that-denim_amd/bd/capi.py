@@ -46,6 +46,7 @@ BD_CONTACT_MAX_NODES = 1 << 24
 BD_NET_FIELDS = 8
 BD_NET_PARAMS = 37216
 BD_NET_SLOW_ITER = 4
+BD_LEARNED_OFF, BD_LEARNED_WARM, BD_LEARNED_DEFLATE = 0, 1, 2
 
 
 class bd_yarn(C.Structure):
@@ -103,7 +104,8 @@ class bd_stats(C.Structure):
                 ("max_update", C.c_double * BD_MAX_NEWTON), ("rejections", C.c_int32),
                 ("ruptures", C.c_int32), ("pulled_out", C.c_int32), ("level_count", C.c_int32),
                 ("level_nodes", C.c_int32 * BD_MAX_LEVELS), ("poll_fallbacks", C.c_int32),
-                ("learned_fallbacks", C.c_int32), ("warm_relres", C.c_double * BD_MAX_NEWTON)]
+                ("learned_fallbacks", C.c_int32), ("warm_relres", C.c_double * BD_MAX_NEWTON),
+                ("max_speed", C.c_double)]
 
 
 class bd_kernel_profile(C.Structure):

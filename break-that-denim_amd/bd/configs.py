@@ -116,7 +116,7 @@ def config_sheet(n=1024, pitch=1e-3, region=256, rel_tol=1e-6, history=True, new
     S = sheet_scene(n, pitch, mat, yarn_region=region)
     g0 = corner_grip(S, 0, 0, k=2e4)
     g1 = corner_grip(S, 1, 0, k=2e4)
-    params = make_params(h=1 / 120, newton_iterations=newton, pcg_rel_tol=rel_tol, pcg_max_iterations=200,
+    params = make_params(h=1 / 120, newton_iterations=newton, pcg_rel_tol=rel_tol, pcg_max_iterations=600,
                          max_coarsest_nodes=400, coarse_sweeps=2, lag_coarse=True, line_search=True, history=history,
                          self_contact=self_contact)
     return {"name": f"sheet{n}-yarns{region}" + ("-contact" if self_contact else ""), "scene": S, "params": params, "grips": [g0, g1],

@@ -1,66 +1,79 @@
-"""Grip patches on the host (src/simulation/grip.ts): the smoothly weighted material disc a fingertip
-pinch holds, split into two pads either side of the grab line, claws hooked into the weave, the
-pinch's closing schedule and the connectivity search the patch grows through.
+"""Grip patches on the host: which crossings a grab holds, with what weight, and how hard.
 
-A grip is a quadratic spring from each patch node to the handle (exact for Newton); its force is
-bounded by limiting how far the handle may lead the gripped material (the GPU's lead clamp), so the
-total force stays below GRIP["maxForce"].
+Interface after the reference's grip module (src/simulation/grip.ts: pinchScale, patchDisc,
+gripPatch, hookPatch, reachable, patchWindowRadius, the GRIP constants); the computation here is
+this build's own: every patch is evaluated on the lattice window around the grabbed crossing as
+numpy arrays (rest-frame distances, a smoothstep falloff, pad membership by the sign of the offset
+along the closing axis), and the reach through surviving spans is a heap-ordered shortest-path
+search over the window's span graph.
+
+A grip is a quadratic spring from each patch node to the handle (exact for Newton). Its force is
+bounded by how far the handle may lead the held material (the GPU's lead clamp): the stiffness per
+unit weight is chosen so that the whole patch pulls GRIP["maxForce"] at the largest lead.
 """
 from __future__ import annotations
 
+import heapq
 import math
 from dataclasses import dataclass, field
 
 import numpy as np
 
-# grip.ts:8-35
+# The reference's grip constants (grip.ts:8-35), kept as the interface's parameters.
 GRIP = {
-    "radius": 0.008,       # material radius of the gripped patch (m): a fingertip pinch
-    "maxForce": 120.0,     # total grip force bound (N)
-    "maxLead": 0.006,      # largest lead of the handle over the patch centroid (m)
-    "pinch": 0.3,          # spread kept by a closed pinch
-    "pinchMode": "pads",   # 'pads' | 'radial'
-    "pinchAxis": 0,        # material axis the fingertips close along when the cloth allows (0: u)
-    "engageTime": 0.15,    # time over which the fingers close and lift the cloth (s)
+    "radius": 0.008,
+    "maxForce": 120.0,
+    "maxLead": 0.006,
+    "pinch": 0.3,
+    "pinchMode": "pads",
+    "pinchAxis": 0,
+    "engageTime": 0.15,
 }
-# longest path through surviving spans, in patch radii, by which a patch member may be reached
+# a patch member must be reachable through surviving spans within this many patch radii
 PATCH_PATH = 1.5
+
+# window bits (the GPU's pick window layout)
+BIT_SPAN_U, BIT_SPAN_V, BIT_NOT_CLOTH, BIT_OUTSIDE = 1, 2, 4, 8
 
 
 def pinch_scale(t):
-    """pinchScale (grip.ts:37-40): the fraction of the grab-time spread kept t seconds after the grab."""
-    e = min(1.0, max(0.0, t / GRIP["engageTime"]))
-    return 1 - (1 - GRIP["pinch"]) * e * e * (3 - 2 * e)
+    """Share of the grab-time spread still kept t seconds after the grab: a smoothstep from 1 down to
+    GRIP["pinch"] over GRIP["engageTime"]."""
+    e = float(np.clip(t / GRIP["engageTime"], 0.0, 1.0))
+    s = e * e * (3.0 - 2.0 * e)
+    return 1.0 + (GRIP["pinch"] - 1.0) * s
 
 
 @dataclass
 class GripPatch:
-    """grip.ts:42-61."""
+    """A held patch: entries (nodes, weights, pad sides -1 / +1 or 0 for a radial pinch), the pitch
+    along the closing axis, the crossings between the pads (inside the pinch, not held), the spring
+    stiffness per unit weight, and the displacement beyond which an entry slips out (0: never)."""
     nodes: list
     weights: list
-    sides: list              # pad of each entry (-1, +1), 0 in a radial pinch
-    pad_pitch: float         # lattice pitch along the closing axis (m)
-    between: list            # crossings between the pads: inside the pinch, not held
-    stiffness: float         # per unit weight (N/m): the summed force reaches maxForce at the largest lead
-    slip: float              # an entry displaced farther than this from the held patch slips out (0: never)
+    sides: list
+    pad_pitch: float
+    between: list
+    stiffness: float
+    slip: float
 
 
 @dataclass
 class PatchWindow:
-    """Connectivity around a picked node (grip.ts:63-71): per offset in [-radius, radius]^2, row-major
-    from (-radius, -radius): bit 0 / 1 its own span along u / v survives, bit 2 not cloth, bit 3
-    outside the lattice."""
+    """Lattice connectivity around a grabbed node: per offset in [-radius, radius]^2 (row-major from
+    (-radius, -radius)) the window bits BIT_SPAN_U / BIT_SPAN_V (its own span survives),
+    BIT_NOT_CLOTH and BIT_OUTSIDE."""
     radius: int
     bits: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
 
 
 def patch_window_radius(scene):
-    """patchWindowRadius (grip.ts:76-80): lattice cells either side of a picked node a window covers."""
+    """Lattice cells either side of the grabbed node that a patch of GRIP["radius"] can reach."""
     pitch = min(min(L.material["pitchU"], L.material["pitchV"]) for L in scene.layers)
     return int(math.ceil(GRIP["radius"] / pitch))
 
 
-def _layer_of(scene, node):
+def _layer(scene, node):
     for L in scene.layers:
         if L.node_offset <= node < L.node_offset + L.node_count:
             return L
@@ -68,130 +81,120 @@ def _layer_of(scene, node):
 
 
 def reachable(win: PatchWindow, pitch_u, pitch_v, max_path):
-    """reachable (grip.ts:196-222): window cells reached from its centre through surviving spans
-    within path length max_path (m), over cloth crossings only (Dijkstra)."""
+    """Window cells whose shortest path from the centre through surviving spans, over cloth crossings
+    only, is at most max_path (m); uint8 flags per cell."""
     side = 2 * win.radius + 1
-    count = side * side
-    dist = np.full(count, np.inf)
-    out = np.zeros(count, np.uint8)
-    bits = np.asarray(win.bits)
-    blocked = lambda q: (int(bits[q]) & 12) != 0
-    centre = win.radius * side + win.radius
-    if blocked(centre):
+    bits = np.asarray(win.bits, dtype=np.uint32)
+    out = np.zeros(side * side, np.uint8)
+    start = win.radius * (side + 1)
+    if bits[start] & (BIT_NOT_CLOTH | BIT_OUTSIDE):
         return out
-    dist[centre] = 0.0
-    while True:
-        k = -1
-        for q in range(count):
-            if not out[q] and dist[q] <= max_path and (k < 0 or dist[q] < dist[k]):
-                k = q
-        if k < 0:
-            return out
+    best = {start: 0.0}
+    heap = [(0.0, start)]
+    while heap:
+        d, k = heapq.heappop(heap)
+        if out[k] or d > best.get(k, math.inf):
+            continue
         out[k] = 1
         i, j = k % side, k // side
-
-        def relax(q, length):
-            if not blocked(q) and not out[q]:
-                dist[q] = min(dist[q], dist[k] + length)
-
-        if i + 1 < side and bits[k] & 1:
-            relax(k + 1, pitch_u)
-        if i > 0 and bits[k - 1] & 1:
-            relax(k - 1, pitch_u)
-        if j + 1 < side and bits[k] & 2:
-            relax(k + side, pitch_v)
-        if j > 0 and bits[k - side] & 2:
-            relax(k - side, pitch_v)
-
-
-def patch_disc(scene, node, available=lambda n: True, window: PatchWindow | None = None, R=None):
-    """patchDisc (grip.ts:157-193): the smoothly weighted disc of material radius R around `node`
-    (same layer, free nodes only), reached through surviving spans when a window is given."""
-    R = GRIP["radius"] if R is None else R
-    L = _layer_of(scene, node)
-    if L is None:
-        return None
-    u0, v0 = scene.rest_uv[node]
-    m = L.material
-    reach = reachable(window, m["pitchU"], m["pitchV"], PATCH_PATH * R) if window is not None else None
-    ri = window.radius if window is not None else int(math.ceil(R / m["pitchU"]))
-    rj = window.radius if window is not None else int(math.ceil(R / m["pitchV"]))
-    side = 2 * ri + 1
-    local = node - L.node_offset
-    i0, j0 = local % L.nx, local // L.nx
-    nodes, weights = [], []
-    W = 0.0
-    for j in range(max(0, j0 - rj), min(L.ny - 1, j0 + rj) + 1):
-        for i in range(max(0, i0 - ri), min(L.nx - 1, i0 + ri) + 1):
-            n = L.node_offset + j * L.nx + i
-            if scene.pinned[n] or not available(n):
+        # the four lattice moves and the span each crosses (owned by the lower / left end)
+        for q, owner, bit, step, ok in ((k + 1, k, BIT_SPAN_U, pitch_u, i + 1 < side),
+                                        (k - 1, k - 1, BIT_SPAN_U, pitch_u, i > 0),
+                                        (k + side, k, BIT_SPAN_V, pitch_v, j + 1 < side),
+                                        (k - side, k - side, BIT_SPAN_V, pitch_v, j > 0)):
+            if not ok or out[q] or not (bits[owner] & bit) or bits[q] & (BIT_NOT_CLOTH | BIT_OUTSIDE):
                 continue
-            if reach is not None and not reach[(j - j0 + rj) * side + (i - i0 + ri)]:
-                continue
-            d = math.hypot(scene.rest_uv[n, 0] - u0, scene.rest_uv[n, 1] - v0) / R
-            if d >= 1:
-                continue
-            # smooth falloff: full weight in the core, tapering to zero at the rim
-            w = 1.0 if d < 0.5 else 1 - (d - 0.5) / 0.5
-            ws = w * w * (3 - 2 * w)
-            if ws <= 1e-3:
-                continue
-            nodes.append(n)
-            weights.append(ws)
-            W += ws
-    if not nodes:
-        return None
-    return GripPatch(nodes, weights, [0] * len(nodes), 0.0, [], GRIP["maxForce"] / (GRIP["maxLead"] * W), R)
-
-
-def _split_pads(scene, node, disc: GripPatch, axis):
-    """splitPads (grip.ts:106-128): the disc split into two pads either side of the grab line across
-    `axis`, or None when lopsided (neither pad under a quarter of the other)."""
-    L = _layer_of(scene, node)
-    pitch = L.material["pitchU"] if axis == 0 else L.material["pitchV"]
-    c = scene.rest_uv[node, axis]
-    out = GripPatch([], [], [], pitch, [], 0.0, disc.slip)
-    side = [0.0, 0.0]
-    for n, w in zip(disc.nodes, disc.weights):
-        u = scene.rest_uv[n, axis] - c
-        if abs(u) < 0.5 * pitch:
-            out.between.append(n)
-            continue
-        out.nodes.append(n)
-        out.weights.append(w)
-        out.sides.append(1 if u > 0 else -1)
-        side[1 if u > 0 else 0] += w
-    if min(side) < 0.25 * max(side) or side[0] == 0:
-        return None
-    out.stiffness = GRIP["maxForce"] / (GRIP["maxLead"] * (side[0] + side[1]))
+            nd = d + step
+            if nd <= max_path and nd < best.get(q, math.inf):
+                best[q] = nd
+                heapq.heappush(heap, (nd, q))
     return out
 
 
+def _disc_arrays(scene, node, available, window, R):
+    """The disc's candidate nodes and smoothstep weights as arrays (same layer, free, available,
+    reachable when a window is given); empty arrays when nothing qualifies."""
+    L = _layer(scene, node)
+    if L is None:
+        return None, None, None
+    m = L.material
+    if window is not None:
+        ri = rj = window.radius
+        reach = reachable(window, m["pitchU"], m["pitchV"], PATCH_PATH * R).astype(bool)
+    else:
+        ri, rj = int(math.ceil(R / m["pitchU"])), int(math.ceil(R / m["pitchV"]))
+        reach = None
+    local = node - L.node_offset
+    i0, j0 = local % L.nx, local // L.nx
+    di, dj = np.meshgrid(np.arange(-ri, ri + 1), np.arange(-rj, rj + 1), indexing="xy")
+    di, dj = di.ravel(), dj.ravel()
+    inside = (i0 + di >= 0) & (i0 + di < L.nx) & (j0 + dj >= 0) & (j0 + dj < L.ny)
+    if reach is not None:
+        inside &= reach
+    n = L.node_offset + (j0 + dj) * L.nx + (i0 + di)
+    n = n[inside]
+    keep = (scene.pinned[n] == 0) & np.array([bool(available(int(q))) for q in n], dtype=bool)
+    n = n[keep]
+    rel = np.hypot(*(scene.rest_uv[n] - scene.rest_uv[node]).T) / R
+    # full weight inside half the radius, a smoothstep falloff to the rim
+    ramp = np.clip(2.0 * (1.0 - rel), 0.0, 1.0)
+    w = ramp * ramp * (3.0 - 2.0 * ramp)
+    sel = (rel < 1.0) & (w > 1e-3)
+    return L, n[sel], w[sel]
+
+
+def patch_disc(scene, node, available=lambda n: True, window: PatchWindow | None = None, R=None):
+    """The smoothly weighted material disc of radius R (default GRIP["radius"]) around `node`, as a
+    radial patch; None when it holds nothing."""
+    R = GRIP["radius"] if R is None else R
+    L, n, w = _disc_arrays(scene, node, available, window, R)
+    if L is None or len(n) == 0:
+        return None
+    return GripPatch(n.tolist(), w.tolist(), [0] * len(n), 0.0, [], GRIP["maxForce"] / (GRIP["maxLead"] * float(w.sum())), R)
+
+
+def _pads(scene, node, disc: GripPatch, axis):
+    """The disc as two pads either side of the grab line across material axis `axis` (the crossings
+    within half a pitch of the line lie between them); None when one pad carries less than a quarter
+    of the other's weight."""
+    L = _layer(scene, node)
+    pitch = L.material["pitchU"] if axis == 0 else L.material["pitchV"]
+    n = np.asarray(disc.nodes)
+    w = np.asarray(disc.weights)
+    off = scene.rest_uv[n, axis] - scene.rest_uv[node, axis]
+    mid = np.abs(off) < 0.5 * pitch
+    side = np.where(off > 0, 1, -1)
+    wl, wr = float(w[~mid & (side < 0)].sum()), float(w[~mid & (side > 0)].sum())
+    if wl == 0.0 or min(wl, wr) < 0.25 * max(wl, wr):
+        return None
+    held = ~mid
+    return GripPatch(n[held].tolist(), w[held].tolist(), side[held].tolist(), pitch, n[mid].tolist(),
+                     GRIP["maxForce"] / (GRIP["maxLead"] * (wl + wr)), disc.slip)
+
+
 def grip_patch(scene, node, available=lambda n: True, window: PatchWindow | None = None):
-    """gripPatch (grip.ts:93-104): the pinch a fingertip grab holds, as two pads along the preferred
-    axis when the cloth reaches both sides, else along the other axis, else the radial disc."""
+    """A fingertip grab: two pads along the preferred axis when the cloth reaches both sides of the
+    grab line, else along the other axis, else the radial disc."""
     disc = patch_disc(scene, node, available, window)
     if disc is None or GRIP["pinchMode"] == "radial":
         return disc
-    for axis in (GRIP["pinchAxis"], 1 - GRIP["pinchAxis"]):
-        pads = _split_pads(scene, node, disc, axis)
-        if pads is not None:
-            return pads
-    return disc
+    first = GRIP["pinchAxis"]
+    return _pads(scene, node, disc, first) or _pads(scene, node, disc, 1 - first) or disc
 
 
 def hook_patch(scene, claws, radius, max_force, max_lead, available=lambda n: True, window=None):
-    """hookPatch (grip.ts:139-156): claws hooked into the cloth at several crossings, the union of a
-    disc of `radius` around each (a node in two discs takes the larger weight); no pinch, no slip."""
+    """Claws hooked into the cloth at several crossings: the union of a disc of `radius` around each
+    (a node in several discs keeps its largest weight); no pinch and no slip."""
     weight = {}
     for c in claws:
         disc = patch_disc(scene, c, available, window, radius)
         if disc is None:
             continue
         for n, w in zip(disc.nodes, disc.weights):
-            weight[n] = max(weight.get(n, 0.0), w)
+            if w > weight.get(n, 0.0):
+                weight[n] = w
     if not weight:
         return None
-    nodes = list(weight)
+    nodes = sorted(weight)
     weights = [weight[n] for n in nodes]
     return GripPatch(nodes, weights, [0] * len(nodes), 0.0, [], max_force / (max_lead * sum(weights)), 0.0)

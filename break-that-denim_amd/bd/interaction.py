@@ -116,7 +116,7 @@ class Interaction:
                                                    a.ctypes.data_as(C.POINTER(C.c_double))), "bd_grip_attach")
         held_nodes = list(patch.nodes[:count])
         core = [n for n, wt in zip(patch.nodes[:count], patch.weights[:count]) if wt >= 0.5]
-        # the strip between the pads is inside the pinch: no other finger takes it, but it is not held
+        # crossings on the grab line belong to this pinch (another grab may not take them) without being held
         held_nodes += list(patch.between)
         for n in held_nodes:
             self.owner[n] = slot

@@ -312,6 +312,7 @@ class GpuSolver:
             "poll_fallbacks": st.poll_fallbacks,
             "learned_fallbacks": st.learned_fallbacks,
             "warm_relres": list(st.warm_relres[:max(n, 1)]),
+            "max_speed": st.max_speed,
         }
 
     # ---- learned coarse-space correction (bd_learn_*)
@@ -329,8 +330,10 @@ class GpuSolver:
                                             losses.ctypes.data_as(C.POINTER(C.c_double))), "bd_learn_train")
         return losses[:int(epochs)]
 
-    def set_learned(self, on):
-        self._check(self.lib.bd_set_learned(self.h, int(bool(on))), "bd_set_learned")
+    def set_learned(self, mode):
+        """0 / False: plain MG-PCG; 1 / True: the learned warm start; 2: the warm start and the deflated
+        correction (BD_LEARNED_*)."""
+        self._check(self.lib.bd_set_learned(self.h, int(mode)), "bd_set_learned")
 
     def learn_weights(self):
         w = np.zeros(capi.BD_NET_PARAMS, np.float32)

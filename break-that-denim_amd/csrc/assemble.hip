@@ -612,11 +612,12 @@ struct CommitArgs {
   uint8_t* pinned;
   uint16_t* flags;
   Geo geo;
+  unsigned int* max_speed;  // float bits: the largest committed speed since the host last read it
 };
 
-__global__ __launch_bounds__(256) void k_commit(CommitArgs A) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= A.N) return;
+// commit (step.wgsl:128-147): v = dx / h, x += dx, detached debris below kRetireZ or beyond kRetireR
+// retired; returns the committed speed (0 for retired nodes)
+__device__ float commit_node(const CommitArgs& A, int n) {
   const int Np = A.Np;
   float d[3];
   for (int q = 0; q < 3; q++) {
@@ -625,7 +626,7 @@ __global__ __launch_bounds__(256) void k_commit(CommitArgs A) {
   }
   if (A.state) {
     const uint8_t st = A.state[n];
-    if (st == kStateRetired) return;
+    if (st == kStateRetired) return 0.f;
     if (st == kStateDetached) {
       float p[3] = {A.x[n] + d[0], A.x[Np + n] + d[1], A.x[2 * Np + n] + d[2]};
       if (p[2] < kRetireZ || sqrtf(p[0] * p[0] + p[1] * p[1]) > kRetireR) {
@@ -641,7 +642,7 @@ __global__ __launch_bounds__(256) void k_commit(CommitArgs A) {
           int prev = nbr(P, n, f, -1);
           if (prev >= 0) A.flags[2 * prev + f] &= (uint16_t)~SEG_EXISTS;
         }
-        return;
+        return 0.f;
       }
     }
   }
@@ -649,6 +650,20 @@ __global__ __launch_bounds__(256) void k_commit(CommitArgs A) {
     A.v[q * Np + n] = d[q] * A.inv_h;
     A.x[q * Np + n] += d[q];
   }
+  return sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * A.inv_h;
+}
+
+// the largest speed of a wave's committed nodes into *max_speed (one atomic per wave)
+__device__ __forceinline__ void wave_max_speed(unsigned int* out, float v) {
+  for (int m = 32; m > 0; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
+  if ((threadIdx.x & 63) == 0 && v > 0.f) atomicMax(out, __float_as_uint(v));
+}
+
+__global__ __launch_bounds__(256) void k_commit(CommitArgs A) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  float speed = 0.f;
+  if (n < A.N) speed = commit_node(A, n);
+  wave_max_speed(A.max_speed, speed);
 }
 
 }  // namespace
@@ -712,7 +727,8 @@ void launch_newton_apply(Solver& s, int it) {
 
 void launch_commit(Solver& s) {
   ProfScope ps(s, PC_PREDICT_COMMIT, s.N * 60.0);
-  CommitArgs A{s.N, s.Np, (float)(1.0 / s.P.h), s.x, s.v, s.dx, s.any_detached ? s.nstate : nullptr, s.pinned, s.flags, s.geo};
+  CommitArgs A{s.N, s.Np, (float)(1.0 / s.P.h), s.x, s.v, s.dx, s.any_detached ? s.nstate : nullptr, s.pinned, s.flags, s.geo,
+               reinterpret_cast<unsigned int*>(s.counters + 2)};
   k_commit<<<grid_for(s.N, 256), 256, 0, s.stream>>>(A);
 }
 

@@ -249,7 +249,7 @@ struct Solver {
   int poll_fallbacks = 0;      // convergence checks that took the synchronizing path
   // learned coarse-space correction (learned.hip): network, recorded samples, chain vectors
   struct Learn* learn = nullptr;
-  bool learned = false;        // Newton solves use the learned warm start and correction
+  int learned = 0;             // BD_LEARNED_*: the learned warm start, and the deflated correction
   int* counters = nullptr;     // [0] ruptures, [1] pulled out
   // multigrid
   int nlev = 0;

@@ -160,9 +160,9 @@ std::vector<Geo> level_layouts(const Geo& fine, int max_coarsest) {
   return out;
 }
 
-// lineTable (lattice.ts:86-106): v-lines (family 1) colours 0..2, then u-lines colours 0..2
-// line tables (lattice.ts:86-106): per colour the lines of one family, v-lines first; `ncol`
-// colours per family (3, or 2 on the fine level where only neighbouring lines couple)
+// line tables (lattice.ts:86-106 lineTable): per colour the lines of one family, v-lines (family 1)
+// first; `ncol` colours per family (kLineColors = 3 on every level: the fine level's along-line
+// bending couples lines two apart as well, so same-colour lines must be three apart)
 void build_lines(Solver& s, Level& L, int ncol) {
   L.ncol = ncol;
   std::vector<std::vector<int4>> by(2 * ncol);
@@ -933,9 +933,11 @@ int bd_pick_ray(bd_solver* h, const float* view_proj, double px, double py, doub
     P.height = (float)height;
     P.strand_radius = (float)strand_radius;
     P.window_radius = window_radius;
+    const double dl = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    if (!(dl > 0)) return fail(h, BD_ERR_INVALID, "bd_pick_ray: zero ray direction");
     for (int q = 0; q < 3; q++) {
       P.eye[q] = (float)eye[q];
-      P.ray[q] = (float)dir[q];
+      P.ray[q] = (float)(dir[q] / dl);  // unit: hit distances in metres
     }
     launch_pick(s, true);
     fill_pick(s, window_radius, out);
@@ -1191,11 +1193,12 @@ int bd_learn_train(bd_solver* h, int epochs, double lr, int batch, int targets, 
 
 int bd_set_learned(bd_solver* h, int on) {
   return guarded(h, [&](Solver& s) {
+    if (on < BD_LEARNED_OFF || on > BD_LEARNED_DEFLATE) return fail(h, BD_ERR_INVALID, "bd_set_learned: unknown mode");
     if (on) {
       if (s.nlev < 2) return fail(h, BD_ERR_STATE, "bd_set_learned: the hierarchy has a single level");
       learn_alloc(s);
     }
-    s.learned = on != 0;
+    s.learned = on;
     // the next Newton iteration predicts from its hierarchy
     if (s.learned && s.hierarchy) learn_predict(s);
     return BD_OK;
@@ -1609,6 +1612,11 @@ int bd_get_stats(bd_solver* h, bd_stats* out) {
     for (int k = 0; k < BD_MAX_NEWTON; k++) st.warm_relres[k] = s.scal_h->rec_warm[k];
     st.ruptures = counters[0];
     st.pulled_out = counters[1];
+    float vmax;
+    std::memcpy(&vmax, &counters[2], sizeof(float));
+    st.max_speed = vmax;
+    // the largest speed counts from this read on (rest detection reads and clears it)
+    BD_HIP(hipMemsetAsync(s.counters + 2, 0, sizeof(int), s.stream));
     *out = st;
     return BD_OK;
   });

@@ -1,20 +1,26 @@
-// Grips and picking: the product's input path into the solve. Restates
-// src/simulation/gpu/wgsl/interact.wgsl:
-//   k_grip_init      gripInit (interact.wgsl:183-196)   offsets of a grip's patch from its picked node
-//                    at grab time, and the handle's origin (the picked node's position then)
-//   k_grip_lead      gripLead (:80-156)  once per substep from committed positions: the pads' closing
-//                    axis, the held patch's weighted centroid, entries torn out from under the pads
-//                    slip out (weight 0), and the effective handle clamped to within the largest lead
-//                    of the centroid (bounded total force)
-//   k_grip_terms     gripTerms (:158-179) per Newton iteration: each entry's quadratic spring to its
-//                    target (handle + gathered offset) into ext, added to the contact terms
-//   k_pick_*         pickRay, pickRayNode, pickCandidates, pickOcclude, pickVisible, pickWrite,
-//                    pickWindow, pickNear (:197-456): the nearest surface along a pointer ray (intact
-//                    faces and strands over openings), else the nearest unoccluded crossing within a
-//                    cursor disc, else (staged grabs) the nearest crossing with material to a point;
-//                    and the connectivity window around the picked node.
-// Up to kGrips grips hold disjoint patches of at most kSlotEntries entries each, so every ext entry
-// has a single writer.
+// Grips and picking: the product's input path into the solve (the reference's Interaction,
+// src/simulation/gpu/interaction.ts with interact.wgsl). Same behaviour, this build's own kernels:
+//
+// Grips. A slot holds up to kSlotEntries patch entries (node, weight, pad side, offset from the
+// grabbed node at grab time). Once per substep one 64-lane wave per slot (k_grip_wave) reduces
+// with cross-lane shuffles, from committed positions: the closing axis (the weighted offsets of the
+// + pad minus those of the - pad), the weighted centroid of the held material less each entry's
+// pinch-gathered offset, the entries torn out from under the pads (farther than `slip` from where
+// the rest of the patch holds them: weight 0), and the handle: the requested position clamped to
+// within max_lead of that centroid, so the summed spring force never exceeds stiffness x weight x
+// max_lead. Every Newton iteration each entry adds its spring to its target (handle + gathered
+// offset) into the external terms (k_grip_springs); slots hold disjoint nodes, so each ext row has
+// one writer.
+//
+// Picking. k_pick_surface: every node tests the primitives it owns (its cell's two triangles when
+// the cell is intact, and its strands over openings as capsules) against the pointer ray and folds
+// (hit distance bits, picked corner) into one 64-bit atomicMin, so the nearest hit and the lowest node
+// at an exact tie come out of one pass. Without a surface hit: the crossings with material inside
+// the cursor disc are compacted per wave (ballot, one atomic per wave), every intact cell whose screen
+// box overlaps the disc marks the candidates it hides, and the visible candidate nearest the pointer
+// on screen wins a (screen distance bits, node) atomicMin. A nearest-material pick (staged grabs) is
+// the same atomicMin over (distance, node) in a sphere. Then the lattice connectivity window around
+// the picked node.
 #include <stdexcept>
 
 #include "bd_internal.h"
@@ -23,449 +29,424 @@ namespace bd {
 
 namespace {
 
-constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr int kMaxCand = BD_PICK_MAX_CANDIDATES;
-constexpr float kOccludeMargin = 0.5e-3f;
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
+constexpr unsigned long long kNoKey = ~0ull;
+constexpr int kCand = BD_PICK_MAX_CANDIDATES;
+constexpr float kBehind = 0.5e-3f;  // an occluder must lie this far in front of the candidate
 
-struct V3 {
+// pick scratch layout (u32 words): [0,1] surface key, [2,3] choice key, [4] candidate count,
+// [5 .. 5 + kCand) candidates, [5 + kCand .. 5 + 2 kCand) hidden flags
+constexpr int kPickWords = 5 + 2 * kCand;
+
+struct F3 {
   float x, y, z;
 };
-__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
-__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float len(V3 a) { return sqrtf(dot(a, a)); }
-__device__ __forceinline__ V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ F3 f3(const float* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ F3 add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 scale(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float norm(F3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ F3 cross3(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 
-struct IaArgs {
+struct Ctx {
   int N, Np;
   Geo geo;
   const float* x;
   const float* dx;
   const uint8_t* state;
   const uint16_t* flags;
-  GripSlot* slots;       // kGrips
-  GripEntry* entries;    // kGrips x kSlotEntries
-  float* handle;         // 12 floats per grip: effective handle, origin, closing axis (xyz + pad)
-  float* ext;            // 9 planes
-  int adds;              // add to ext (contact wrote it this iteration) or overwrite
-  PickArgs pk;
-  uint32_t* pick;        // 4 + 2 kMaxCand words
-  uint32_t* pick_out;    // 8 + 256 words
+  GripSlot* slots;
+  GripEntry* entries;
+  float* handle;  // per slot 12 floats: handle xyz, -, origin xyz, -, closing axis xyz, -
+  float* ext;
+  int accumulate;  // contact wrote ext this iteration: add to it
+  PickArgs q;
+  uint32_t* work;  // kPickWords
+  uint32_t* out;   // 8 + 256: point xyz, node, kind, -, -, -, window
 };
 
-__device__ __forceinline__ V3 pos(const IaArgs& A, uint32_t n) { return {A.x[n], A.x[A.Np + n], A.x[2 * A.Np + n]}; }
-__device__ __forceinline__ V3 cur(const IaArgs& A, uint32_t n) {
-  return {A.x[n] + A.dx[n], A.x[A.Np + n] + A.dx[A.Np + n], A.x[2 * A.Np + n] + A.dx[2 * A.Np + n]};
+__device__ __forceinline__ F3 at(const Ctx& c, uint32_t n) { return {c.x[n], c.x[c.Np + n], c.x[2 * c.Np + n]}; }
+__device__ __forceinline__ F3 at_iterate(const Ctx& c, uint32_t n) {
+  return add(at(c, n), F3{c.dx[n], c.dx[c.Np + n], c.dx[2 * c.Np + n]});
 }
-// a node of the cloth solve (detached and retired nodes no longer respond to a grip)
-__device__ __forceinline__ bool live(const IaArgs& A, uint32_t n) { return A.state[n] == kStateCloth; }
-__device__ __forceinline__ V3 ldv(const float* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ bool cloth(const Ctx& c, uint32_t n) { return c.state[n] == kStateCloth; }
 
-// target offset of an entry at pinch scale s (interact.wgsl:74-78)
-__device__ __forceinline__ V3 gathered(const GripEntry& e, const GripSlot& S, V3 axis) {
-  V3 o = {e.offset[0], e.offset[1], e.offset[2]};
-  if (e.side == 0.f) return S.pinch * o;
-  return o - (e.side * (1.f - S.pinch) * S.pad_pitch) * axis;
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
 }
 
-__global__ __launch_bounds__(64) void k_grip_init(IaArgs A, int g) {
-  const GripSlot& S = A.slots[g];
-  const uint32_t picked = S.picked;
-  const V3 pp = pos(A, picked);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    float* H = A.handle + 12 * g;
-    H[0] = pp.x;
-    H[1] = pp.y;
-    H[2] = pp.z;
-    H[4] = pp.x;
-    H[5] = pp.y;
-    H[6] = pp.z;
-  }
+// where an entry is held relative to the handle: its grab-time offset, pulled toward the grab line
+// by the pinch (pads along the closing axis, a radial pinch by scaling)
+__device__ __forceinline__ F3 held_offset(const GripEntry& e, const GripSlot& S, F3 axis) {
+  const F3 o = f3(e.offset);
+  if (e.side == 0.f) return scale(S.pinch, o);
+  return sub(o, scale(e.side * (1.f - S.pinch) * S.pad_pitch, axis));
+}
+
+__global__ __launch_bounds__(64) void k_grip_offsets(Ctx c, int g) {
+  const GripSlot& S = c.slots[g];
+  const F3 p = at(c, S.picked);
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) {
+    float* H = c.handle + 12 * g;
+    H[0] = H[4] = p.x;
+    H[1] = H[5] = p.y;
+    H[2] = H[6] = p.z;
+  }
   if (k >= S.count) return;
-  GripEntry& e = A.entries[g * kSlotEntries + k];
-  const V3 o = pos(A, e.node) - pp;
+  GripEntry& e = c.entries[g * kSlotEntries + k];
+  const F3 o = sub(at(c, e.node), p);
   e.offset[0] = o.x;
   e.offset[1] = o.y;
   e.offset[2] = o.z;
 }
 
-// one workgroup of 256 per grip
-__global__ __launch_bounds__(256) void k_grip_lead(IaArgs A) {
-  __shared__ float red[4][256];
-  const int g = blockIdx.x, t = threadIdx.x;
-  const GripSlot& S = A.slots[g];
+// one wave per slot
+__global__ __launch_bounds__(64) void k_grip_wave(Ctx c) {
+  const int g = blockIdx.x, lane = threadIdx.x;
+  const GripSlot& S = c.slots[g];
   const int count = S.count;
   if (count == 0) return;
-  GripEntry* E = A.entries + g * kSlotEntries;
-  // the closing axis: from the - pad to the + pad, as they lay at grab time
-  float d[3] = {0.f, 0.f, 0.f};
-  for (int k = t; k < count; k += 256)
-    for (int q = 0; q < 3; q++) d[q] += E[k].side * E[k].w * E[k].offset[q];
-  for (int q = 0; q < 3; q++) red[q][t] = d[q];
-  __syncthreads();
-  for (int r = 128; r > 0; r >>= 1) {
-    if (t < r)
-      for (int q = 0; q < 3; q++) red[q][t] += red[q][t + r];
-    __syncthreads();
+  GripEntry* E = c.entries + g * kSlotEntries;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  for (int k = lane; k < count; k += 64) {
+    const float sw = E[k].side * E[k].w;
+    ax += sw * E[k].offset[0];
+    ay += sw * E[k].offset[1];
+    az += sw * E[k].offset[2];
   }
-  const V3 ds = {red[0][0], red[1][0], red[2][0]};
-  const float ld = len(ds);
-  const V3 axis = ld > 1e-12f ? (1.f / fmaxf(ld, 1e-20f)) * ds : V3{0.f, 0.f, 0.f};
-  __syncthreads();
-  float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  for (int k = t; k < count; k += 256) {
+  F3 axis = {wave_sum(ax), wave_sum(ay), wave_sum(az)};
+  const float la = norm(axis);
+  axis = la > 1e-12f ? scale(1.f / la, axis) : F3{0.f, 0.f, 0.f};
+  float sx = 0.f, sy = 0.f, sz = 0.f, sw = 0.f;
+  for (int k = lane; k < count; k += 64) {
     const GripEntry& e = E[k];
-    if (!live(A, e.node)) continue;
-    const V3 p = pos(A, e.node) - gathered(e, S, axis);
-    acc[0] += e.w * p.x;
-    acc[1] += e.w * p.y;
-    acc[2] += e.w * p.z;
-    acc[3] += e.w;
+    if (!cloth(c, e.node)) continue;
+    const F3 h = sub(at(c, e.node), held_offset(e, S, axis));
+    sx += e.w * h.x;
+    sy += e.w * h.y;
+    sz += e.w * h.z;
+    sw += e.w;
   }
-  for (int q = 0; q < 4; q++) red[q][t] = acc[q];
-  __syncthreads();
-  for (int r = 128; r > 0; r >>= 1) {
-    if (t < r)
-      for (int q = 0; q < 4; q++) red[q][t] += red[q][t + r];
-    __syncthreads();
-  }
-  const float hw = red[3][0];
-  const V3 c = hw > 0.f ? (1.f / hw) * V3{red[0][0], red[1][0], red[2][0]} : V3{0.f, 0.f, 0.f};
-  // an entry farther than a pad's radius from where the rest of the patch holds it slips out
-  if (hw > 0.f && S.slip > 0.f)
-    for (int k = t; k < count; k += 256) {
-      GripEntry& e = E[k];
-      if (e.w > 0.f && live(A, e.node) && len(pos(A, e.node) - gathered(e, S, axis) - c) > S.slip) e.w = 0.f;
-    }
-  if (t == 0) {
-    float* H = A.handle + 12 * g;
-    H[8] = axis.x;
-    H[9] = axis.y;
-    H[10] = axis.z;
-    if (!(hw > 0.f)) {
-      // nothing held is still cloth: the handle exerts nothing
+  sx = wave_sum(sx);
+  sy = wave_sum(sy);
+  sz = wave_sum(sz);
+  sw = wave_sum(sw);
+  float* H = c.handle + 12 * g;
+  if (!(sw > 0.f)) {
+    // no held entry is cloth any more: the handle sits at its origin and pulls nothing
+    if (lane == 0) {
       H[0] = H[4];
       H[1] = H[5];
       H[2] = H[6];
-      return;
+      H[8] = axis.x;
+      H[9] = axis.y;
+      H[10] = axis.z;
     }
-    const V3 requested = V3{H[4], H[5], H[6]} + (ldv(S.goal) - ldv(S.anchor));
-    V3 lead = requested - c;
-    const float l = len(lead);
-    if (l > S.max_lead) lead = (S.max_lead / l) * lead;
-    const V3 h = c + lead;
+    return;
+  }
+  const F3 centre = scale(1.f / sw, F3{sx, sy, sz});
+  if (S.slip > 0.f)
+    for (int k = lane; k < count; k += 64) {
+      GripEntry& e = E[k];
+      if (e.w > 0.f && cloth(c, e.node) && norm(sub(sub(at(c, e.node), held_offset(e, S, axis)), centre)) > S.slip) e.w = 0.f;
+    }
+  if (lane == 0) {
+    const F3 want = add(F3{H[4], H[5], H[6]}, sub(f3(S.goal), f3(S.anchor)));
+    F3 lead = sub(want, centre);
+    const float l = norm(lead);
+    if (l > S.max_lead) lead = scale(S.max_lead / l, lead);
+    const F3 h = add(centre, lead);
     H[0] = h.x;
     H[1] = h.y;
     H[2] = h.z;
+    H[8] = axis.x;
+    H[9] = axis.y;
+    H[10] = axis.z;
   }
 }
 
-__global__ __launch_bounds__(64) void k_grip_terms(IaArgs A) {
+// each entry's spring at the current iterate into ext (gradient, then the isotropic Hessian k w I)
+__global__ __launch_bounds__(64) void k_grip_springs(Ctx c) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = id / kSlotEntries, k = id % kSlotEntries;
-  if (g >= kGrips) return;
-  const GripSlot& S = A.slots[g];
-  if (k >= S.count) return;
-  const GripEntry& e = A.entries[id];
-  if (!live(A, e.node)) return;
-  const float* H = A.handle + 12 * g;
-  const V3 target = V3{H[0], H[1], H[2]} + gathered(e, S, V3{H[8], H[9], H[10]});
-  const V3 dd = cur(A, e.node) - target;
-  const float kg = S.stiffness * e.w;
-  const float keep = A.adds ? 1.f : 0.f;
-  const int Np = A.Np;
-  const uint32_t n = e.node;
-  A.ext[n] = keep * A.ext[n] + kg * dd.x;
-  A.ext[Np + n] = keep * A.ext[Np + n] + kg * dd.y;
-  A.ext[2 * Np + n] = keep * A.ext[2 * Np + n] + kg * dd.z;
-  A.ext[3 * Np + n] = keep * A.ext[3 * Np + n] + kg;
-  A.ext[4 * Np + n] = keep * A.ext[4 * Np + n];
-  A.ext[5 * Np + n] = keep * A.ext[5 * Np + n];
-  A.ext[6 * Np + n] = keep * A.ext[6 * Np + n] + kg;
-  A.ext[7 * Np + n] = keep * A.ext[7 * Np + n];
-  A.ext[8 * Np + n] = keep * A.ext[8 * Np + n] + kg;
+  const int g = id / kSlotEntries;
+  if (g >= kGrips || id % kSlotEntries >= c.slots[g].count) return;
+  const GripSlot& S = c.slots[g];
+  const GripEntry& e = c.entries[id];
+  if (!cloth(c, e.node)) return;
+  const float* H = c.handle + 12 * g;
+  const F3 target = add(f3(H), held_offset(e, S, f3(H + 8)));
+  const F3 d = sub(at_iterate(c, e.node), target);
+  const float k = S.stiffness * e.w;
+  const float terms[9] = {k * d.x, k * d.y, k * d.z, k, 0.f, 0.f, k, 0.f, k};
+  for (int q = 0; q < 9; q++) {
+    float* p = c.ext + (size_t)q * c.Np + e.node;
+    *p = c.accumulate ? *p + terms[q] : terms[q];
+  }
 }
 
 // ---------------------------------------------------------------- picking
 
-struct Lat {
+struct Cell {
   int off, nx, ny, i, j;
 };
-__device__ __forceinline__ Lat lat(const IaArgs& A, uint32_t n) {
-  int l = geo_layer(A.geo, (int)n);
-  int r = (int)n - A.geo.off[l];
-  int j = r / A.geo.nx[l];
-  return {A.geo.off[l], A.geo.nx[l], A.geo.ny[l], r - j * A.geo.nx[l], j};
+__device__ __forceinline__ Cell cell_of(const Ctx& c, uint32_t n) {
+  const int l = geo_layer(c.geo, (int)n);
+  const int r = (int)n - c.geo.off[l];
+  const int j = r / c.geo.nx[l];
+  return {c.geo.off[l], c.geo.nx[l], c.geo.ny[l], r - j * c.geo.nx[l], j};
 }
-__device__ __forceinline__ bool span_on(const IaArgs& A, uint32_t owner, int f) {
-  return (A.flags[2 * owner + f] & (SEG_EXISTS | SEG_BROKEN)) == SEG_EXISTS;
+__device__ __forceinline__ bool span_alive(const Ctx& c, uint32_t owner, int fam) {
+  return (c.flags[2 * owner + fam] & (SEG_EXISTS | SEG_BROKEN)) == SEG_EXISTS;
 }
-// a cloth crossing with at least one surviving incident span (interact.wgsl:206-212)
-__device__ bool has_material(const IaArgs& A, uint32_t n) {
-  if (!live(A, n)) return false;
-  if (span_on(A, n, 0) || span_on(A, n, 1)) return true;
-  Lat L = lat(A, n);
-  if (L.i > 0 && span_on(A, n - 1, 0)) return true;
-  return L.j > 0 && span_on(A, n - L.nx, 1);
+// a cloth crossing with at least one surviving span at it
+__device__ bool has_material(const Ctx& c, uint32_t n) {
+  if (!cloth(c, n)) return false;
+  const Cell L = cell_of(c, n);
+  return span_alive(c, n, 0) || span_alive(c, n, 1) || (L.i > 0 && span_alive(c, n - 1, 0)) ||
+         (L.j > 0 && span_alive(c, n - L.nx, 1));
 }
-// an intact cell with lower-left node f, all corners cloth (interact.wgsl:214-223)
-__device__ bool face_valid(const IaArgs& A, uint32_t f) {
-  Lat L = lat(A, f);
+// the cell whose lower-left corner is f is intact: its four spans survive and its corners are cloth
+__device__ bool cell_intact(const Ctx& c, uint32_t f) {
+  const Cell L = cell_of(c, f);
   if (L.i + 1 >= L.nx || L.j + 1 >= L.ny) return false;
-  const uint32_t c10 = f + 1, c01 = f + L.nx;
-  if (!live(A, f) || !live(A, c10) || !live(A, c01) || !live(A, c01 + 1)) return false;
-  return span_on(A, f, 0) && span_on(A, f, 1) && span_on(A, c10, 1) && span_on(A, c01, 0);
+  const uint32_t e = f + 1, nn = f + L.nx;
+  return cloth(c, f) && cloth(c, e) && cloth(c, nn) && cloth(c, nn + 1) && span_alive(c, f, 0) && span_alive(c, f, 1) &&
+         span_alive(c, e, 1) && span_alive(c, nn, 0);
 }
 
-// ray (o, d) against triangle (a, b, c): (t, u, v), t < 0 when missed (Moller-Trumbore)
-__device__ void ray_tri(V3 o, V3 d, V3 a, V3 b, V3 c, float out[3]) {
-  out[0] = -1.f;
-  out[1] = out[2] = 0.f;
-  const V3 e1 = b - a, e2 = c - a, p = cross(d, e2);
-  const float det = dot(e1, p);
-  if (fabsf(det) < 1e-14f) return;
-  const float inv = 1.f / det;
-  const V3 s = o - a;
-  const float u = dot(s, p) * inv;
-  if (u < 0.f || u > 1.f) return;
-  const V3 q = cross(s, e1);
-  const float v = dot(d, q) * inv;
-  if (v < 0.f || u + v > 1.f) return;
-  out[0] = dot(e2, q) * inv;
-  out[1] = u;
-  out[2] = v;
+// distance along the unit ray (o, d) to triangle (a, b, c), with the barycentric weights of b and c
+// (a negative distance: no hit). Solves [-d, b - a, c - a] (t, u, v) = o - a by Cramer's rule.
+__device__ float ray_triangle(F3 o, F3 d, F3 a, F3 b, F3 cc, float& u, float& v) {
+  const F3 e1 = sub(b, a), e2 = sub(cc, a), r = sub(o, a);
+  const F3 n = cross3(e1, e2);
+  const float den = -dot3(d, n);
+  u = v = 0.f;
+  if (fabsf(den) < 1e-14f) return -1.f;
+  const float inv = 1.f / den;
+  const F3 rd = cross3(r, d);
+  u = dot3(e2, rd) * inv;
+  v = -dot3(e1, rd) * inv;
+  if (u < 0.f || v < 0.f || u + v > 1.f) return -1.f;
+  return dot3(r, n) * inv;
 }
 
-// ray (o, unit d) against the capsule of radius r around (a, b): (t, s), t < 0 when missed
-__device__ void ray_capsule(V3 o, V3 d, V3 a, V3 b, float r, float out[2]) {
-  const V3 e = b - a, w = o - a;
-  const float ee = dot(e, e), de = dot(d, e), den = ee - de * de;
-  float s = 0.f;
-  if (den > 1e-14f) s = fminf(fmaxf((dot(e, w) * dot(d, d) - de * dot(d, w)) / den, 0.f), 1.f);
-  const V3 q = a + s * e;
-  const float t = fmaxf(dot(q - o, d), 0.f);
-  const float gap = len(o + t * d - q);
-  if (gap > r) {
-    out[0] = -1.f;
-    out[1] = 0.f;
-    return;
-  }
-  out[0] = t - sqrtf(r * r - gap * gap);
-  out[1] = s;
+// distance along the unit ray (o, d) to the capsule of radius rad around segment (a, b), with the
+// segment parameter of the nearest point (a negative distance: no hit)
+__device__ float ray_strand(F3 o, F3 d, F3 a, F3 b, float rad, float& s) {
+  const F3 e = sub(b, a), w = sub(o, a);
+  const float ee = dot3(e, e), de = dot3(d, e), den = ee - de * de;
+  s = den > 1e-14f ? fminf(fmaxf((dot3(e, w) - de * dot3(d, w)) / den, 0.f), 1.f) : 0.f;
+  const F3 p = add(a, scale(s, e));
+  const float t = fmaxf(dot3(sub(p, o), d), 0.f);
+  const float miss = norm(sub(add(o, scale(t, d)), p));
+  if (miss > rad) return -1.f;
+  return t - sqrtf(rad * rad - miss * miss);
 }
 
-// nearest pointer-ray hit among the primitives node n owns (interact.wgsl:263-300)
-__device__ void ray_prims(const IaArgs& A, uint32_t n, float& bt, uint32_t& bn) {
-  bt = 3.4e38f;
-  bn = kNone;
-  const V3 o = ldv(A.pk.eye), d = ldv(A.pk.ray);
-  const Lat L = lat(A, n);
-  const bool fv = face_valid(A, n);
-  if (fv) {
-    const uint32_t c0 = n, c1 = n + 1, c2 = n + L.nx, c3 = n + L.nx + 1;
-    const V3 p00 = pos(A, c0), p10 = pos(A, c1), p01 = pos(A, c2), p11 = pos(A, c3);
-    float h[3];
-    ray_tri(o, d, p00, p10, p11, h);
-    if (h[0] > 0.f && h[0] < bt) {
-      const float w0 = 1.f - h[1] - h[2], w1 = h[1], w2 = h[2];
-      bt = h[0];
-      bn = w0 >= fmaxf(w1, w2) ? c0 : (w1 >= w2 ? c1 : c3);
+__device__ __forceinline__ unsigned long long key64(float d, uint32_t n) {
+  return ((unsigned long long)__float_as_uint(d) << 32) | n;  // d >= 0: its bits order like its value
+}
+
+// the nearest hit among node n's primitives, as a key (kNoKey: none)
+__device__ unsigned long long surface_key(const Ctx& c, uint32_t n) {
+  const F3 o = f3(c.q.eye), d = f3(c.q.ray);
+  const Cell L = cell_of(c, n);
+  unsigned long long best = kNoKey;
+  const bool intact = cell_intact(c, n);
+  if (intact) {
+    const uint32_t v00 = n, v10 = n + 1, v01 = n + L.nx, v11 = n + L.nx + 1;
+    const F3 p00 = at(c, v00), p10 = at(c, v10), p01 = at(c, v01), p11 = at(c, v11);
+    // the cell as triangles (00, 10, 11) and (00, 11, 01); the hit picks its heaviest corner
+    float u, v;
+    const float t1 = ray_triangle(o, d, p00, p10, p11, u, v);
+    if (t1 > 0.f) {
+      const float w0 = 1.f - u - v;
+      const uint32_t m = (w0 >= u && w0 >= v) ? v00 : (u >= v ? v10 : v11);
+      best = min(best, key64(t1, m));
     }
-    ray_tri(o, d, p00, p11, p01, h);
-    if (h[0] > 0.f && h[0] < bt) {
-      const float w0 = 1.f - h[1] - h[2], w1 = h[1], w2 = h[2];
-      bt = h[0];
-      bn = w0 >= fmaxf(w1, w2) ? c0 : (w1 >= w2 ? c3 : c2);
+    const float t2 = ray_triangle(o, d, p00, p11, p01, u, v);
+    if (t2 > 0.f) {
+      const float w0 = 1.f - u - v;
+      const uint32_t m = (w0 >= u && w0 >= v) ? v00 : (u >= v ? v11 : v01);
+      best = min(best, key64(t2, m));
     }
   }
-  if (!live(A, n)) return;
-  for (int f = 0; f < 2; f++) {
-    if (!span_on(A, n, f)) continue;
-    const uint32_t other = f == 0 ? n + 1 : n + L.nx;
-    if (!live(A, other)) continue;
-    // the two cells beside the span: a strand covered by an intact face is picked as that face
-    const bool has_side = f == 0 ? L.j > 0 : L.i > 0;
-    const uint32_t side = f == 0 ? n - L.nx : n - 1;
-    if (fv || (has_side && face_valid(A, side))) continue;
-    float h[2];
-    ray_capsule(o, d, pos(A, n), pos(A, other), A.pk.strand_radius, h);
-    if (h[0] > 0.f && h[0] < bt) {
-      bt = h[0];
-      bn = h[1] > 0.5f ? other : n;
-    }
+  if (!cloth(c, n)) return best;
+  for (int fam = 0; fam < 2; fam++) {
+    const uint32_t m = fam == 0 ? n + 1 : n + L.nx;
+    if (!span_alive(c, n, fam) || !cloth(c, m)) continue;
+    // a strand bordering an intact cell is picked through that cell
+    const bool beside = fam == 0 ? (L.j > 0 && cell_intact(c, n - L.nx)) : (L.i > 0 && cell_intact(c, n - 1));
+    if (intact || beside) continue;
+    float sp;
+    const float t = ray_strand(o, d, at(c, n), at(c, m), c.q.strand_radius, sp);
+    if (t > 0.f) best = min(best, key64(t, sp > 0.5f ? m : n));
+  }
+  return best;
+}
+
+__global__ __launch_bounds__(256) void k_pick_surface(Ctx c) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (uint32_t)c.N) return;
+  const unsigned long long k = surface_key(c, n);
+  if (k != kNoKey) atomicMin(reinterpret_cast<unsigned long long*>(c.work), k);
+}
+
+// screen position (pixels x, y), NDC depth and clip w of a point
+__device__ void to_screen(const Ctx& c, F3 p, float s[4]) {
+  const float* M = c.q.view_proj;  // column-major
+  float h[4];
+  for (int r = 0; r < 4; r++) h[r] = M[r] * p.x + M[4 + r] * p.y + M[8 + r] * p.z + M[12 + r];
+  s[0] = (0.5f + 0.5f * h[0] / h[3]) * c.q.width;
+  s[1] = (0.5f - 0.5f * h[1] / h[3]) * c.q.height;
+  s[2] = h[2] / h[3];
+  s[3] = h[3];
+}
+
+__device__ __forceinline__ bool surface_hit(const Ctx& c) {
+  return *reinterpret_cast<const unsigned long long*>(c.work) != kNoKey;
+}
+
+// crossings with material inside the cursor disc, compacted per wave
+__global__ __launch_bounds__(256) void k_pick_disc(Ctx c) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (surface_hit(c)) return;
+  bool in = false;
+  if (n < (uint32_t)c.N && has_material(c, n)) {
+    float s[4];
+    to_screen(c, at(c, n), s);
+    const float px = s[0] - c.q.pointer[0], py = s[1] - c.q.pointer[1];
+    in = s[3] > 0.f && s[2] > 0.f && s[2] < 1.f && px * px + py * py <= c.q.radius_px * c.q.radius_px;
+  }
+  const unsigned long long mask = __ballot(in);
+  if (mask == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(&c.work[4], (uint32_t)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (in) {
+    const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (slot < (uint32_t)kCand) c.work[5 + slot] = n;
   }
 }
 
-// projection to drawing-buffer pixels (x, y), NDC depth, clip w
-__device__ void project(const IaArgs& A, V3 p, float o[4]) {
-  const float* M = A.pk.view_proj;  // column-major 4 x 4
-  float c[4];
-  for (int r = 0; r < 4; r++) c[r] = M[r] * p.x + M[4 + r] * p.y + M[8 + r] * p.z + M[12 + r];
-  const float nx = c[0] / c[3], ny = c[1] / c[3], nz = c[2] / c[3];
-  o[0] = (nx * 0.5f + 0.5f) * A.pk.width;
-  o[1] = (0.5f - ny * 0.5f) * A.pk.height;
-  o[2] = nz;
-  o[3] = c[3];
-}
-
-__global__ __launch_bounds__(256) void k_pick_ray(IaArgs A) {
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= (uint32_t)A.N) return;
-  float t;
-  uint32_t m;
-  ray_prims(A, n, t, m);
-  if (m != kNone) atomicMin(&A.pick[0], __float_as_uint(t));
-}
-
-// the material node of that hit (the lowest index on an exact tie)
-__global__ __launch_bounds__(256) void k_pick_ray_node(IaArgs A) {
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= (uint32_t)A.N) return;
-  const uint32_t front = __hip_atomic_load(&A.pick[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (front == kNone) return;
-  float t;
-  uint32_t m;
-  ray_prims(A, n, t, m);
-  if (m != kNone && __float_as_uint(t) == front) atomicMin(&A.pick[1], m);
-}
-
-// over no surface: crossings within the cursor disc are candidates
-__global__ __launch_bounds__(256) void k_pick_candidates(IaArgs A) {
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= (uint32_t)A.N || A.pick[0] != kNone || !has_material(A, n)) return;
-  float p[4];
-  project(A, pos(A, n), p);
-  if (p[3] <= 0.f || p[2] <= 0.f || p[2] >= 1.f) return;
-  const float dxp = p[0] - A.pk.pointer[0], dyp = p[1] - A.pk.pointer[1];
-  if (sqrtf(dxp * dxp + dyp * dyp) > A.pk.radius_px) return;
-  const uint32_t k = atomicAdd(&A.pick[2], 1u);
-  if (k < (uint32_t)kMaxCand) A.pick[4 + k] = n;
-}
-
-// a candidate is hidden when an intact face lies on the line of sight in front of it
-__global__ __launch_bounds__(256) void k_pick_occlude(IaArgs A) {
+// every intact cell whose screen box (grown by the disc radius) holds the pointer marks the
+// candidates it stands in front of
+__global__ __launch_bounds__(256) void k_pick_hide(Ctx c) {
   const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= (uint32_t)A.N) return;
-  const uint32_t count = min(A.pick[2], (uint32_t)kMaxCand);
-  if (count == 0 || !face_valid(A, f)) return;
-  const Lat L = lat(A, f);
-  const uint32_t c[4] = {f, f + 1, f + L.nx, f + L.nx + 1};
-  const V3 p00 = pos(A, c[0]), p10 = pos(A, c[1]), p01 = pos(A, c[2]), p11 = pos(A, c[3]);
-  float s[4][4];
-  project(A, p00, s[0]);
-  project(A, p10, s[1]);
-  project(A, p01, s[2]);
-  project(A, p11, s[3]);
-  if (fminf(fminf(s[0][3], s[1][3]), fminf(s[2][3], s[3][3])) <= 0.f) return;
-  const float R = A.pk.radius_px;
-  const float lox = fminf(fminf(s[0][0], s[1][0]), fminf(s[2][0], s[3][0])) - R;
-  const float loy = fminf(fminf(s[0][1], s[1][1]), fminf(s[2][1], s[3][1])) - R;
-  const float hix = fmaxf(fmaxf(s[0][0], s[1][0]), fmaxf(s[2][0], s[3][0])) + R;
-  const float hiy = fmaxf(fmaxf(s[0][1], s[1][1]), fmaxf(s[2][1], s[3][1])) + R;
-  const float px = A.pk.pointer[0], py = A.pk.pointer[1];
-  if (px < lox || py < loy || px > hix || py > hiy) return;
-  const V3 o = ldv(A.pk.eye);
+  if (f >= (uint32_t)c.N || surface_hit(c)) return;
+  const uint32_t count = min(c.work[4], (uint32_t)kCand);
+  if (count == 0 || !cell_intact(c, f)) return;
+  const Cell L = cell_of(c, f);
+  const uint32_t v[4] = {f, f + 1, f + L.nx, f + L.nx + 1};
+  F3 p[4];
+  float lo[2] = {3.4e38f, 3.4e38f}, hi[2] = {-3.4e38f, -3.4e38f};
+  for (int q = 0; q < 4; q++) {
+    p[q] = at(c, v[q]);
+    float s[4];
+    to_screen(c, p[q], s);
+    if (s[3] <= 0.f) return;
+    lo[0] = fminf(lo[0], s[0]);
+    lo[1] = fminf(lo[1], s[1]);
+    hi[0] = fmaxf(hi[0], s[0]);
+    hi[1] = fmaxf(hi[1], s[1]);
+  }
+  const float R = c.q.radius_px, px = c.q.pointer[0], py = c.q.pointer[1];
+  if (px < lo[0] - R || px > hi[0] + R || py < lo[1] - R || py > hi[1] + R) return;
+  const F3 eye = f3(c.q.eye);
   for (uint32_t k = 0; k < count; k++) {
-    const uint32_t m = A.pick[4 + k];
-    if (m == c[0] || m == c[1] || m == c[2] || m == c[3]) continue;
-    const V3 to = pos(A, m) - o;
-    const float l = len(to);
-    const V3 d = (1.f / l) * to;
-    const float tmax = l - kOccludeMargin;
-    float h1[3], h2[3];
-    ray_tri(o, d, p00, p10, p11, h1);
-    ray_tri(o, d, p00, p11, p01, h2);
-    if ((h1[0] > 0.f && h1[0] < tmax) || (h2[0] > 0.f && h2[0] < tmax)) atomicOr(&A.pick[4 + kMaxCand + k], 1u);
+    const uint32_t m = c.work[5 + k];
+    if (m == v[0] || m == v[1] || m == v[2] || m == v[3]) continue;
+    const F3 to = sub(at(c, m), eye);
+    const float l = norm(to);
+    const F3 d = scale(1.f / l, to);
+    float u, w;
+    const float t1 = ray_triangle(eye, d, p[0], p[1], p[3], u, w);
+    const float t2 = ray_triangle(eye, d, p[0], p[3], p[2], u, w);
+    if ((t1 > 0.f && t1 < l - kBehind) || (t2 > 0.f && t2 < l - kBehind)) atomicOr(&c.work[5 + kCand + k], 1u);
   }
 }
 
-// the visible candidate nearest to the pointer on screen
-__global__ __launch_bounds__(128) void k_pick_visible(IaArgs A) {
+// the visible candidate nearest the pointer on screen
+__global__ __launch_bounds__(128) void k_pick_visible(Ctx c) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= min(A.pick[2], (uint32_t)kMaxCand) || A.pick[4 + kMaxCand + k] != 0) return;
-  const uint32_t n = A.pick[4 + k];
-  float p[4];
-  project(A, pos(A, n), p);
-  const float dxp = p[0] - A.pk.pointer[0], dyp = p[1] - A.pk.pointer[1];
-  const uint32_t dk = min(4095u, (uint32_t)(sqrtf(dxp * dxp + dyp * dyp) * 64.f));
-  atomicMin(&A.pick[3], (dk << 20) | (n & 0xFFFFFu));
+  if (surface_hit(c) || k >= min(c.work[4], (uint32_t)kCand) || c.work[5 + kCand + k]) return;
+  const uint32_t n = c.work[5 + k];
+  float s[4];
+  to_screen(c, at(c, n), s);
+  const float px = s[0] - c.q.pointer[0], py = s[1] - c.q.pointer[1];
+  atomicMin(reinterpret_cast<unsigned long long*>(c.work + 2), key64(sqrtf(px * px + py * py), n));
 }
 
-// the crossing with material nearest to a point within a radius (staged grabs, interact.wgsl:444-456)
-__global__ __launch_bounds__(256) void k_pick_near(IaArgs A) {
+// the crossing with material nearest to a point within a radius (staged grabs)
+__global__ __launch_bounds__(256) void k_pick_nearest(Ctx c) {
   const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= (uint32_t)A.N || !has_material(A, n)) return;
-  const float d = len(pos(A, n) - ldv(A.pk.eye));
-  if (d > A.pk.strand_radius) return;
-  const uint32_t dk = min(4095u, (uint32_t)(d / A.pk.strand_radius * 4095.f));
-  atomicMin(&A.pick[3], (dk << 20) | (n & 0xFFFFFu));
+  if (n >= (uint32_t)c.N || !has_material(c, n)) return;
+  const float d = norm(sub(at(c, n), f3(c.q.eye)));
+  if (d <= c.q.strand_radius) atomicMin(reinterpret_cast<unsigned long long*>(c.work + 2), key64(d, n));
 }
 
-__global__ void k_pick_write(IaArgs A) {
-  const uint32_t front = A.pick[0];
-  V3 p = {0.f, 0.f, 0.f};
-  uint32_t n = kNone, kind = 0;
-  if (front != kNone) {
-    n = A.pick[1];
-    p = ldv(A.pk.eye) + __uint_as_float(front) * ldv(A.pk.ray);
+// the result (point, node, kind 1 surface / 2 proximity / 0 none) and the connectivity window around
+// the node: per lattice offset in [-R, R]^2, bit 0 / 1 its own u / v span survives, bit 2 not cloth,
+// bit 3 outside the lattice
+__global__ __launch_bounds__(256) void k_pick_result(Ctx c) {
+  const unsigned long long surf = *reinterpret_cast<const unsigned long long*>(c.work);
+  const unsigned long long near = *reinterpret_cast<const unsigned long long*>(c.work + 2);
+  uint32_t node = kNoNode, kind = 0;
+  F3 p = {0.f, 0.f, 0.f};
+  if (surf != kNoKey) {
+    node = (uint32_t)(surf & 0xFFFFFFFFull);
+    p = add(f3(c.q.eye), scale(__uint_as_float((uint32_t)(surf >> 32)), f3(c.q.ray)));
     kind = 1;
-  } else if (A.pick[3] != kNone) {
-    n = A.pick[3] & 0xFFFFFu;
-    p = pos(A, n);
+  } else if (near != kNoKey) {
+    node = (uint32_t)(near & 0xFFFFFFFFull);
+    p = at(c, node);
     kind = 2;
   }
-  A.pick_out[0] = __float_as_uint(p.x);
-  A.pick_out[1] = __float_as_uint(p.y);
-  A.pick_out[2] = __float_as_uint(p.z);
-  A.pick_out[3] = n;
-  A.pick_out[4] = kind;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    c.out[0] = __float_as_uint(p.x);
+    c.out[1] = __float_as_uint(p.y);
+    c.out[2] = __float_as_uint(p.z);
+    c.out[3] = node;
+    c.out[4] = kind;
+  }
+  const int R = c.q.window_radius, side = 2 * R + 1;
+  if (t >= side * side) return;
+  uint32_t b = 8u;
+  if (kind != 0) {
+    const Cell L = cell_of(c, node);
+    const int i = L.i + t % side - R, j = L.j + t / side - R;
+    if (i >= 0 && j >= 0 && i < L.nx && j < L.ny) {
+      const uint32_t m = (uint32_t)(L.off + j * L.nx + i);
+      b = (span_alive(c, m, 0) ? 1u : 0u) | (span_alive(c, m, 1) ? 2u : 0u) | (cloth(c, m) ? 0u : 4u);
+    }
+  }
+  c.out[8 + t] = b;
 }
 
-// connectivity around the picked node: per lattice offset in [-R, R]^2, bit 0 / 1 its own span
-// along u / v survives, bit 2 not cloth, bit 3 outside the lattice (interact.wgsl:418-440)
-__global__ __launch_bounds__(256) void k_pick_window(IaArgs A) {
-  const int k = threadIdx.x;
-  const int R = A.pk.window_radius, side = 2 * R + 1;
-  if (k >= side * side) return;
-  if (A.pick_out[4] == 0) {
-    A.pick_out[8 + k] = 8u;
-    return;
-  }
-  const uint32_t n = A.pick_out[3];
-  const Lat L = lat(A, n);
-  const int i = L.i + k % side - R, j = L.j + k / side - R;
-  if (i < 0 || j < 0 || i >= L.nx || j >= L.ny) {
-    A.pick_out[8 + k] = 8u;
-    return;
-  }
-  const uint32_t m = (uint32_t)(L.off + j * L.nx + i);
-  uint32_t b = 0;
-  if (span_on(A, m, 0)) b |= 1u;
-  if (span_on(A, m, 1)) b |= 2u;
-  if (!live(A, m)) b |= 4u;
-  A.pick_out[8 + k] = b;
-}
-
-IaArgs make_args(Solver& s) {
-  IaArgs A;
-  A.N = s.N;
-  A.Np = s.Np;
-  A.geo = s.geo;
-  A.x = s.x;
-  A.dx = s.dx;
-  A.state = s.nstate;
-  A.flags = s.flags;
-  A.slots = s.grip_slots;
-  A.entries = s.grip_entries;
-  A.handle = s.grip_handle;
-  A.ext = s.ext;
-  A.adds = s.contact_mode != kContactNone ? 1 : 0;
-  A.pk = s.pick_args;
-  A.pick = s.pick;
-  A.pick_out = s.pick_out;
-  return A;
+Ctx context(Solver& s) {
+  Ctx c;
+  c.N = s.N;
+  c.Np = s.Np;
+  c.geo = s.geo;
+  c.x = s.x;
+  c.dx = s.dx;
+  c.state = s.nstate;
+  c.flags = s.flags;
+  c.slots = s.grip_slots;
+  c.entries = s.grip_entries;
+  c.handle = s.grip_handle;
+  c.ext = s.ext;
+  c.accumulate = s.contact_mode != kContactNone ? 1 : 0;
+  c.q = s.pick_args;
+  c.work = s.pick;
+  c.out = s.pick_out;
+  return c;
 }
 
 }  // namespace
@@ -475,7 +456,7 @@ void interact_alloc(Solver& s) {
   BD_HIP(hipMalloc(&s.grip_slots, kGrips * sizeof(GripSlot)));
   BD_HIP(hipMalloc(&s.grip_entries, (size_t)kGrips * kSlotEntries * sizeof(GripEntry)));
   BD_HIP(hipMalloc(&s.grip_handle, 12 * kGrips * sizeof(float)));
-  BD_HIP(hipMalloc(&s.pick, (4 + 2 * kMaxCand) * sizeof(uint32_t)));
+  BD_HIP(hipMalloc(&s.pick, kPickWords * sizeof(uint32_t)));
   BD_HIP(hipMalloc(&s.pick_out, (8 + 256) * sizeof(uint32_t)));
   BD_HIP(hipMemsetAsync(s.grip_slots, 0, kGrips * sizeof(GripSlot), s.stream));
   BD_HIP(hipMemsetAsync(s.grip_entries, 0, (size_t)kGrips * kSlotEntries * sizeof(GripEntry), s.stream));
@@ -483,42 +464,38 @@ void interact_alloc(Solver& s) {
 }
 
 void launch_grip_init(Solver& s, int g) {
-  IaArgs A = make_args(s);
   const int count = s.grip_slots_h[g].count;
-  k_grip_init<<<grid_for(count > 0 ? count : 1, 64), 64, 0, s.stream>>>(A, g);
+  k_grip_offsets<<<grid_for(count > 0 ? count : 1, 64), 64, 0, s.stream>>>(context(s), g);
 }
 
 void launch_grip_lead(Solver& s) {
   if (!s.grips_active) return;
-  IaArgs A = make_args(s);
   ProfScope ps(s, PC_PREDICT_COMMIT, 0.0);
-  k_grip_lead<<<kGrips, 256, 0, s.stream>>>(A);
+  k_grip_wave<<<kGrips, 64, 0, s.stream>>>(context(s));
 }
 
 void launch_grip_terms(Solver& s) {
   if (!s.grips_active) return;
-  IaArgs A = make_args(s);
   ProfScope ps(s, PC_CONTACT_TERMS, 0.0);
-  k_grip_terms<<<kGrips * kSlotEntries / 64, 64, 0, s.stream>>>(A);
+  k_grip_springs<<<kGrips * kSlotEntries / 64, 64, 0, s.stream>>>(context(s));
 }
 
 void launch_pick(Solver& s, bool ray) {
-  IaArgs A = make_args(s);
-  std::vector<uint32_t> init(4 + 2 * kMaxCand, 0u);
-  init[0] = init[1] = init[3] = kNone;
+  const Ctx c = context(s);
+  // the keys start at "none", the candidate list empty
+  std::vector<uint32_t> init(kPickWords, 0u);
+  init[0] = init[1] = init[2] = init[3] = kNoNode;
   BD_HIP(hipMemcpyAsync(s.pick, init.data(), init.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s.stream));
   const int g = grid_for(s.N, 256);
   if (ray) {
-    k_pick_ray<<<g, 256, 0, s.stream>>>(A);
-    k_pick_ray_node<<<g, 256, 0, s.stream>>>(A);
-    k_pick_candidates<<<g, 256, 0, s.stream>>>(A);
-    k_pick_occlude<<<g, 256, 0, s.stream>>>(A);
-    k_pick_visible<<<grid_for(kMaxCand, 128), 128, 0, s.stream>>>(A);
+    k_pick_surface<<<g, 256, 0, s.stream>>>(c);
+    k_pick_disc<<<g, 256, 0, s.stream>>>(c);
+    k_pick_hide<<<g, 256, 0, s.stream>>>(c);
+    k_pick_visible<<<grid_for(kCand, 128), 128, 0, s.stream>>>(c);
   } else {
-    k_pick_near<<<g, 256, 0, s.stream>>>(A);
+    k_pick_nearest<<<g, 256, 0, s.stream>>>(c);
   }
-  k_pick_write<<<1, 1, 0, s.stream>>>(A);
-  k_pick_window<<<1, 256, 0, s.stream>>>(A);
+  k_pick_result<<<1, 256, 0, s.stream>>>(c);
 }
 
 }  // namespace bd

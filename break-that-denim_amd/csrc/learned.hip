@@ -1112,6 +1112,12 @@ void learn_predict(Solver& s) {
   ProfScope ps(s, PC_LEARNED, 0.0);
   restrict_to(G, s, s.rhs, nullptr);
   forward(G, s, L.op, L.fixed, G.chain[G.level]);
+  if (s.learned != BD_LEARNED_DEFLATE) {
+    // the warm start alone: G = Z^T A_l Z on the net level (the Galerkin operator, FP64 sums)
+    subspace(G, s, L.op, L.fixed, nullptr, nullptr);
+    G.have_z = true;
+    return;
+  }
   CorrArgs C{G.Nl, G.Nlp, L.fixed, G.z, G.chain[G.level], G.rpart, &G.L->ticket, G.L, nullptr, nullptr};
   for (int k = 0; k < BD_NET_FIELDS; k++) {
     k_copy_field<<<grid_for(G.Nlp, 256), 256, 0, s.stream>>>(G.z, G.Nlp, k, G.chain[G.level]);

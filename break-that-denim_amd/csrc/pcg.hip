@@ -776,7 +776,8 @@ void pcg_solve(Solver& s) {
   const int K = tol_mode ? std::max(1, s.P.pcg_max_iterations) : std::max(0, s.P.pcg_iterations);
   const int G = dot_grid(s.N), B = grid_for(s.N, 256);
   const double N = s.N;
-  const bool learned = s.learned && learn_ready(s) && s.npieces == 0;
+  const bool learned = s.learned != BD_LEARNED_OFF && learn_ready(s) && s.npieces == 0;
+  const bool deflate = learned && s.learned == BD_LEARNED_DEFLATE;
   const bool record = !learned && tol_mode && learn_recording(s);
   k_pcg_setup<<<1, 1, 0, s.stream>>>(s.scal, tol_mode ? s.P.pcg_rel_tol : 0.0, K, K, learned ? 1 : 0);
   {
@@ -804,7 +805,7 @@ void pcg_solve(Solver& s) {
   }
   if (K == 0) return;
   vcycle(s, 0);
-  if (learned) learn_deflate(s, A.z);
+  if (deflate) learn_deflate(s, A.z);
   {
     ProfScope ps(s, PC_PCG_VECTOR, N * 24.0);
     k_dot_rz<<<G, kDotThreads, 0, s.stream>>>(A, 1);
@@ -837,7 +838,7 @@ void pcg_solve(Solver& s) {
       if (wait_flag(s, seq)) break;
     }
     vcycle(s, 0);
-    if (learned) learn_deflate(s, A.z);
+    if (deflate) learn_deflate(s, A.z);
     {
       ProfScope ps(s, PC_PCG_VECTOR, N * 24.0);
       k_dot_rz<<<G, kDotThreads, 0, s.stream>>>(A, 0);

@@ -386,6 +386,7 @@ static napi_value stats(napi_env env, napi_callback_info info) {
   set_num(env, o, "ruptures", st.ruptures);
   set_num(env, o, "pulledOut", st.pulled_out);
   set_num(env, o, "learnedFallbacks", st.learned_fallbacks);
+  set_num(env, o, "maxSpeed", st.max_speed);
   double lv[BD_MAX_LEVELS];
   for (int k = 0; k < BD_MAX_LEVELS; k++) lv[k] = st.level_nodes[k];
   set_arr(env, o, "levelNodes", lv, st.level_count);

@@ -50,8 +50,9 @@ class GpuSolver {
   /** The grips' goals for the following substeps (3 per grip). */
   setGripGoals(goals) { addon.setGripGoals(this.handle, Float64Array.from(goals)); }
 
-  /** Newton solves with the learned coarse-space correction (flexible PCG) or plain MG-PCG. */
-  setLearned(on) { addon.setLearned(this.handle, on ? 1 : 0); }
+  /** Newton solves with the learned coarse-space field: 0 (or false) plain MG-PCG, 1 (or true) a
+   *  warm start from the predicted fields, 2 deflated flexible PCG over them. */
+  setLearned(mode) { addon.setLearned(this.handle, mode === true ? 1 : mode === false ? 0 : mode | 0); }
   learnRecord(capacity) { addon.learnRecord(this.handle, capacity); }
   learnTrain(epochs, lr = 1e-3, batch = 1, targets = 3) { return addon.learnTrain(this.handle, epochs, lr, batch, targets); }
 

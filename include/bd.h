@@ -180,6 +180,8 @@ typedef struct bd_stats {
   int32_t poll_fallbacks;                 /* PCG convergence checks that took the synchronizing path (total) */
   int32_t learned_fallbacks;              /* learned solves that fell back to plain MG-PCG (total) */
   double warm_relres[BD_MAX_NEWTON];      /* ||b - A d0|| / ||b|| of each Newton solve's start (1: zero start) */
+  double max_speed;                       /* largest committed node speed since the last bd_get_stats, m/s
+                                             (step.wgsl:145 counters[2]); reading clears it */
 } bd_stats;
 
 /* Buffers for bd_read / bd_write. `level` selects the multigrid level where it applies. */
@@ -397,8 +399,11 @@ int bd_learn_record(bd_solver* s, int capacity);
  * job; 3: both); losses[e] receives epoch e's mean loss (the fraction of the targets' energy that
  * span(P Z) misses), may be NULL. */
 int bd_learn_train(bd_solver* s, int epochs, double lr, int batch, int targets, double* losses);
-/* 1: Newton solves use the learned warm start and correction (flexible PCG); 0: plain MG-PCG. */
-int bd_set_learned(bd_solver* s, int on);
+/* How Newton solves use the learned fields (BD_LEARNED_*). */
+#define BD_LEARNED_OFF 0      /* plain MG-PCG */
+#define BD_LEARNED_WARM 1     /* the A-optimal warm start in span(P Z) (G from the net level's Galerkin operator) */
+#define BD_LEARNED_DEFLATE 2  /* the warm start, and every preconditioned residual deflated (W = A P Z per system) */
+int bd_set_learned(bd_solver* s, int mode);
 /* FP32 master weights (count = BD_NET_PARAMS). */
 int bd_learn_get_weights(bd_solver* s, float* out, int count);
 int bd_learn_set_weights(bd_solver* s, const float* w, int count);

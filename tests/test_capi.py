@@ -34,7 +34,7 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(capi.bd_yarn) == 88
     assert C.sizeof(capi.bd_material) == 2 * 8 + 2 * 88 + 6 * 8
     assert C.sizeof(capi.bd_params) == 8 * 4 + 4 * 2 + 8 + 4 * 2 + 8 + 4 * 10
-    assert C.sizeof(capi.bd_stats) == 304 + 8 * capi.BD_MAX_NEWTON
+    assert C.sizeof(capi.bd_stats) == 304 + 8 * capi.BD_MAX_NEWTON + 8
     assert C.sizeof(capi.bd_pick) == 4 + 4 + 3 * 8 + 4 + 4 * 225
 
 
