@@ -12,13 +12,14 @@ the host only moves the grip goals (64 bytes).
 Three methods run the same frames from the same scene:
   plain    the reference's schedule: FP32 V-cycle with line sweeps on every level (coarsest <= 400
            nodes), the Galerkin hierarchy of the first Newton iteration reused by the later ones;
-  mixed    the hierarchy rebuilt at every Newton iteration, the V-cycle stopped at the first level of
-           <= 1100 nodes (33 x 33) and that level solved exactly in FP64 (banded Cholesky, dense
-           inverse), FP32 operators (FP16 coarse storage loses the contact terms' range);
-  learned  mixed plus the learned coarse-space correction (BF16 message-passing net on the 257 x 257
-           level): warm start and additive correction in flexible PCG, plain fallback; the net is
-           trained on the GPU from samples the mixed method's warm-up frames record (not the timed
-           frames).
+  mixed    the hierarchy rebuilt at every Newton iteration, the coarsest level (17 x 17, the same
+           level count as plain) solved exactly in FP64 (banded Cholesky, dense inverse) instead of
+           swept, FP32 operators (FP16 coarse storage loses the contact terms' range);
+  learned  mixed plus the learned coarse-space fields (BF16 message-passing net on the 65 x 65
+           level): an A-optimal warm start from them (or, with --learned-mode 2, deflated flexible
+           PCG over them too), kept only when it lowers the residual, plain MG-PCG otherwise; the
+           net is trained on the GPU from samples that the warm-up frames' plain solves record (not
+           the timed frames).
 `value` is the learned method's ms/frame; every method's ms/frame and PCG iterations, and the ratios
 against plain, are reported beside it.
 
@@ -37,7 +38,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "break-that-denim_amd"))
 
 HBM_PEAK_GBS = 8000.0
-MIXED_COARSEST_NODES = 1100
+MIXED_COARSEST_NODES = 400
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_sheet1024_solver_kernels.md"
 
 
@@ -72,6 +73,9 @@ def parse():
                     help="1: the learned warm start; 2: the warm start and the deflated correction")
     ap.add_argument("--no-contact", action="store_true", help="leave node-face self-contact off")
     ap.add_argument("--sweeps", type=int, default=1, help="pre- and post-smoothing sweeps per level")
+    ap.add_argument("--mixed-coarsest", type=int, default=MIXED_COARSEST_NODES, metavar="NODES",
+                    help="the mixed and learned methods stop the V-cycle at the first level of at most NODES nodes "
+                         "and solve it exactly")
     ap.add_argument("--mixed-precision", choices=("fp16", "fp32"), default="fp32",
                     help="operator storage of the mixed method's coarse levels (fine level and plain method: fp32)")
     ap.add_argument("--smoother-probe", type=int, default=0, metavar="SWEEPS",
@@ -164,7 +168,7 @@ def method_params(cfg, method, args):
         params.precision = BD_PRECISION_FP32
     else:
         params.coarse_solve = BD_COARSE_EXACT
-        params.max_coarsest_nodes = MIXED_COARSEST_NODES
+        params.max_coarsest_nodes = args.mixed_coarsest
         params.lag_coarse = 0
         params.precision = BD_PRECISION_FP16 if args.mixed_precision == "fp16" else BD_PRECISION_FP32
     return params

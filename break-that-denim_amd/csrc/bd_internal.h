@@ -283,7 +283,6 @@ struct Solver {
 
   // exact coarsest solve (BD_COARSE_EXACT): banded Cholesky factor and dense inverse, FP64
   int cn = 0, cbw = 0;
-  bool cwindow = true;  // factor in an LDS window (else in place in global memory)
   double* cband = nullptr;
   double* cinv = nullptr;
   double* crd = nullptr;   // reciprocal pivots of the factor

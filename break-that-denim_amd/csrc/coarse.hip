@@ -5,11 +5,9 @@
 // The coarsest level's DOF matrix (DOF 3 n + c, nodes row-major per layer) is banded with half
 // bandwidth bw = 3 (2 nx + 2) + 2 (867 DOFs and bw = 110 for 17 x 17). Per hierarchy build:
 //   k_band_load      the lower band from the symmetric coarse blocks (FP32 -> FP64);
-//   k_band_cholesky  right-looking banded Cholesky in one workgroup, the active (bw+1)^2 window in
-//                    LDS as a ring of rows (98.6 KB at bw = 110); rows leave the window final;
-//                    wider bands (33 x 33 and up) factor by panels of 32 columns in LDS (the
-//                    diagonal block by one wave, the rows below one thread each) with rank-32
-//                    trailing updates in global memory (k_band_cholesky_panel);
+//   k_band_cholesky_panel  right-looking banded Cholesky in one workgroup by panels of 32 columns
+//                    in LDS (the diagonal block by one wave, the rows below one thread each) with
+//                    rank-32 trailing updates in global memory;
 //   k_band_inverse_blk  16 columns per workgroup, 16 rows per step (bands up to 240);
 //   k_band_inverse   one wave per column c: L y = e_c, then L^T x = y, written as row c of the
 //                    symmetric inverse (coalesced), for wider bands;
@@ -21,8 +19,6 @@
 namespace bd {
 
 namespace {
-
-constexpr int kCholThreads = 1024;
 
 __device__ __forceinline__ int sym_index(int r, int c) {
   // [xx, xy, xz, yy, yz, zz]
@@ -68,61 +64,9 @@ __global__ __launch_bounds__(256) void k_band_load(BandArgs B) {
   }
 }
 
-// Right-looking banded Cholesky in one workgroup. The window holds rows k .. k + bw of the band
-// (row i in slot i mod (bw + 1)); after step k row k is final and is replaced by row k + bw + 1.
-__global__ __launch_bounds__(kCholThreads) void k_band_cholesky(BandArgs B) {
-  extern __shared__ double win[];  // (bw + 1) x (bw + 1)
-  __shared__ double piv;
-  const int n = B.n, bw = B.bw, w = bw + 1;
-  const int t = threadIdx.x, T = blockDim.x;
-  for (int e = t; e < w * w; e += T) {
-    const int row = e / w;
-    win[e] = row < n ? B.L[(size_t)row * w + (e - row * w)] : 0.0;
-  }
-  __syncthreads();
-  for (int k = 0; k < n; k++) {
-    double* rk = win + (k % w) * w;
-    if (t == 0) {
-      double d = rk[0];
-      // an SPD operator has positive pivots; a non-positive one (rounding in a near-singular
-      // level) is replaced by a tiny positive value so the factor stays finite
-      d = d > 0.0 ? sqrt(d) : 1e-150;
-      rk[0] = d;
-      piv = d;
-    }
-    __syncthreads();
-    const int m = min(bw, n - 1 - k);
-    // column k below the diagonal: L(i, k) at row i, offset i - k
-    for (int q = t; q < m; q += T) {
-      const int i = k + 1 + q;
-      win[(i % w) * w + (i - k)] /= piv;
-    }
-    __syncthreads();
-    // trailing update of rows k+1 .. k+m, columns k+1 .. i
-    const int pairs = m * (m + 1) / 2;
-    for (int p = t; p < pairs; p += T) {
-      int a = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
-      while ((a + 1) * (a + 2) / 2 <= p) a++;
-      while (a * (a + 1) / 2 > p) a--;
-      const int b = p - a * (a + 1) / 2;  // 0 <= b <= a < m
-      const int i = k + 1 + a, j = k + 1 + b;
-      const double lik = win[(i % w) * w + (i - k)];
-      const double ljk = win[(j % w) * w + (j - k)];
-      win[(i % w) * w + (i - j)] -= lik * ljk;
-    }
-    __syncthreads();
-    // row k is final: store it, load row k + w into its slot
-    const int nk = k + w;
-    for (int o = t; o < w; o += T) {
-      B.L[(size_t)k * w + o] = rk[o];
-      rk[o] = nk < n ? B.L[(size_t)nk * w + o] : 0.0;
-    }
-    __syncthreads();
-  }
-}
-
-// The same factorization for bands whose window does not fit LDS, blocked by panels of kPanel
-// columns: the panel (its kPanel columns over the kPanel + bw rows they reach) is loaded to LDS;
+// Right-looking banded Cholesky in one workgroup, blocked by panels of kPanel columns (a
+// column-at-a-time form with the (bw + 1)^2 window in LDS measured 4.2 us per column at bw = 110,
+// against 1.7 us for this one at bw = 207: it waits on a barrier and a global row load per column): the panel (its kPanel columns over the kPanel + bw rows they reach) is loaded to LDS;
 // wave 0 alone factors its kPanel x kPanel diagonal block (a row per lane in registers, columns
 // passed by scalar lane reads, no workgroup barriers); every row below then solves against that factor in registers, one thread per row; the
 // panel is written back and the trailing band triangle (bw x bw) below it takes one rank-kPanel
@@ -527,18 +471,13 @@ size_t coarse_exact_bytes(const Level& L) {
 void coarse_exact_alloc(Solver& s) {
   Level& L = s.lv[s.nlev - 1];
   const int n = 3 * L.N, bw = band_width(L.geo);
-  const size_t lds = sizeof(double) * (size_t)(bw + 1) * (bw + 1);
   if (bw >= kRing) throw std::runtime_error("exact coarse solve: coarsest band wider than 255");
   BD_HIP(hipMalloc(&s.cband, sizeof(double) * (size_t)n * (bw + 1)));
   BD_HIP(hipMalloc(&s.cinv, sizeof(double) * (size_t)n * n));
   BD_HIP(hipMalloc(&s.crd, sizeof(double) * (size_t)n));
-  s.cwindow = lds <= 150 * 1024;
-  if (s.cwindow)
-    BD_HIP(hipFuncSetAttribute((const void*)k_band_cholesky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const size_t plds = sizeof(double) * (size_t)(kPanel + bw) * kPanelStride;
-  if (!s.cwindow && plds > 150 * 1024) throw std::runtime_error("exact coarse solve: coarsest band too wide");
-  if (!s.cwindow)
-    BD_HIP(hipFuncSetAttribute((const void*)k_band_cholesky_panel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
+  if (plds > 150 * 1024) throw std::runtime_error("exact coarse solve: coarsest band too wide");
+  BD_HIP(hipFuncSetAttribute((const void*)k_band_cholesky_panel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds));
   s.cn = n;
   s.cbw = bw;
 }
@@ -551,10 +490,7 @@ void launch_coarse_factor(Solver& s) {
     // band load + factor (the band in and out once)
     ProfScope ps(s, PC_COARSE_FACTOR, 8.0 * ((double)s.cn * w * 3));
     k_band_load<<<grid_for(s.cn * w, 256), 256, 0, s.stream>>>(B);
-    if (s.cwindow)
-      k_band_cholesky<<<1, kCholThreads, sizeof(double) * (size_t)w * w, s.stream>>>(B);
-    else
-      k_band_cholesky_panel<<<1, kPanelThreads, sizeof(double) * (size_t)(kPanel + s.cbw) * kPanelStride, s.stream>>>(B);
+    k_band_cholesky_panel<<<1, kPanelThreads, sizeof(double) * (size_t)(kPanel + s.cbw) * kPanelStride, s.stream>>>(B);
   }
   {
     // inverse: the band read per column, the inverse written once

@@ -240,6 +240,32 @@ def test_dynamics_60_substeps(coupon_pair):
     assert err < 0.25 * moved
 
 
+def test_max_speed_counter(coupon_pair):
+    """stats()["max_speed"] is the largest committed node speed since the previous read (the value a
+    rest detector polls instead of reading back every velocity), and reading it clears it."""
+    S, cpu, gpu = coupon_pair
+
+    def speeds():  # the velocity buffer is read back node-major, xyz per node
+        return np.sqrt((gpu.read(1).astype(np.float64).reshape(-1, 3) ** 2).sum(axis=1))
+
+    gpu.stats()
+    for _ in range(3):
+        gpu.encode_substep()
+    last3 = speeds().max()
+    first = gpu.stats()["max_speed"]
+    again = gpu.stats()["max_speed"]
+    gpu.encode_substep()
+    last1 = speeds().max()
+    one = gpu.stats()["max_speed"]
+    say(f"max speed: counter over 3 substeps {first:.6e} m/s (last substep's velocities {last3:.6e}), "
+        f"after a read {again}, over 1 substep {one:.6e} (velocities {last1:.6e})")
+    assert last3 > 0.0
+    # over three substeps the counter is at least the last one's largest speed
+    assert first >= last3 * (1 - 1e-6)
+    assert again == 0.0
+    assert abs(one - last1) <= 1e-5 * last1
+
+
 @pytest.mark.parametrize("max_coarsest", [400, 1100])
 def test_exact_coarse_solve(coupon_pair, max_coarsest):
     """BD_COARSE_EXACT: the GPU's FP64 banded Cholesky + dense inverse of the coarsest level in the
