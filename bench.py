@@ -14,7 +14,7 @@ Three methods run the same frames from the same scene:
            nodes), the Galerkin hierarchy of the first Newton iteration reused by the later ones;
   mixed    the hierarchy rebuilt at every Newton iteration, the coarsest level (17 x 17, the same
            level count as plain) solved exactly in FP64 (banded Cholesky, dense inverse) instead of
-           swept, FP32 operators (FP16 coarse storage loses the contact terms' range);
+           swept, FP32 operators (FP16 coarse storage needs 4x the iterations with contact);
   learned  mixed plus the learned coarse-space fields (BF16 message-passing net on the 65 x 65
            level): an A-optimal warm start from them (or, with --learned-mode 2, deflated flexible
            PCG over them too), kept only when it lowers the residual, plain MG-PCG otherwise; the
@@ -72,6 +72,10 @@ def parse():
     ap.add_argument("--learned-mode", type=int, default=1, choices=(1, 2),
                     help="1: the learned warm start; 2: the warm start and the deflated correction")
     ap.add_argument("--no-contact", action="store_true", help="leave node-face self-contact off")
+    ap.add_argument("--workload", choices=("sheet1024", "sheet64", "patches48"), default="sheet1024",
+                    help="sheet1024: the headline workload (default); sheet64: the 64 x 64 sheet, eager against "
+                         "graph-captured substeps; patches48: 48 uneven patches, one eager substep per patch "
+                         "against one graph launch for the batch")
     ap.add_argument("--sweeps", type=int, default=1, help="pre- and post-smoothing sweeps per level")
     ap.add_argument("--mixed-coarsest", type=int, default=MIXED_COARSEST_NODES, metavar="NODES",
                     help="the mixed and learned methods stop the V-cycle at the first level of at most NODES nodes "
@@ -238,6 +242,100 @@ def run_method(args, cfg, method, profile_frames):
     return out
 
 
+def graph_budget(iters_per_newton, limit):
+    """The PCG budget of a captured Newton solve: 1.5x the most iterations an eager warm-up solve
+    took, plus 4, within the solver's own limit."""
+    return int(min(limit, max(8, int(1.5 * iters_per_newton) + 4)))
+
+
+def graph_bench(args):
+    """Launch-bound workloads, eager against graph-captured substeps on the same frames: cfg 1's 64 x 64
+    sheet (one solver each way) or 48 uneven patches (each its own Newton / MG-PCG system; eager: one
+    substep per patch in turn, each with its host convergence checks; batched: one graph launch per
+    frame for all of them). The graph's PCG budget per Newton solve comes from the eager warm-up."""
+    import numpy as np
+    from bd.configs import advance_grips, config1, config_patches
+    from bd.solver import Batch, GpuSolver
+    if args.workload == "sheet64":
+        cfgs = [config1()]
+        name = "sheet64 (cfg 1: 64x64 sheet, 3 Newton, 3-level MG-PCG to rel-res 1e-5, one corner grip at 2 mm/step)"
+    else:
+        cp = config_patches()
+        cfgs = cp["patches"]
+        name = (f"patches48 ({len(cfgs)} uneven patches, sides {min(p['side'] for p in cfgs)}-{max(p['side'] for p in cfgs)},"
+                f" {sum(p['side'] ** 2 for p in cfgs)} nodes, each 3 Newton x MG-PCG to rel-res 1e-6, random grip paths)")
+    def make():
+        out = []
+        for c in cfgs:
+            grips = [dict(g) for g in c["grips"]]
+            out.append((GpuSolver(c["scene"], c["params"], grips), grips, c["velocity"]))
+        return out
+
+    def frame(group, step):
+        for g, grips, vel in group:
+            advance_grips(grips, vel)
+            g.set_grip_goals(np.array([q["goal"] for q in grips]))
+            step(g)
+
+    def collect(group):
+        its, rel = 0, 0.0
+        for g, _, _ in group:
+            st = g.stats()
+            n = st["newton_iterations"]
+            its += sum(st["pcg_iterations"][:n])
+            rel = max(rel, max(st["rel_residual"][:n]))
+        return its, rel
+
+    results = {}
+    eager, graph = make(), make()
+    budget = []
+    for _ in range(args.warmup):
+        frame(eager, lambda g: g.encode_substep())
+        frame(graph, lambda g: g.encode_substep())
+        for g, _, _ in graph:
+            st = g.stats()
+            budget.append(max(st["pcg_iterations"][:st["newton_iterations"]]))
+    per = [0] * len(graph)
+    for r in range(args.warmup):
+        for i in range(len(graph)):
+            per[i] = max(per[i], budget[r * len(graph) + i])
+    for (g, _, _), c, m in zip(graph, cfgs, per):
+        g.graph_capture(graph_budget(m, c["params"].pcg_max_iterations))
+    batch = Batch([g for g, _, _ in graph]) if len(graph) > 1 else None
+    for label, group, step in (("eager", eager, lambda g: g.encode_substep()),
+                               ("graph", graph, (lambda g: None) if batch else (lambda g: g.graph_substep()))):
+        its, rel = 0, 0.0
+        for g, _, _ in group:
+            g.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            frame(group, step)
+            if label == "graph" and batch:
+                batch.substep()
+            i, r = collect(group)
+            its += i
+            rel = max(rel, r)
+        for g, _, _ in group:
+            g.sync()
+        results[label] = {"ms_per_frame": round(1e3 * (time.perf_counter() - t0) / args.steps, 3),
+                          "pcg_iterations_per_frame": round(its / args.steps, 1), "max_rel_residual": rel}
+    tol = cfgs[0]["params"].pcg_rel_tol
+    if batch:
+        batch.close()
+    for g, _, _ in eager + graph:
+        g.close()
+    ms = results["graph"]["ms_per_frame"]
+    print(json.dumps({
+        "metric": "ms/frame, graph-captured substeps against eager ones", "value": ms, "unit": "ms/frame",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (FP64 dots, residual replacement, line search)",
+        "data": "synthetic", "config": {"workload": name, "pcg_budget_per_newton": [graph_budget(m, c["params"].pcg_max_iterations)
+                                                                                 for m, c in zip(per, cfgs)]},
+        "methods": results, "speedup_eager_over_graph": round(results["eager"]["ms_per_frame"] / ms, 3),
+        "converged": all(v["max_rel_residual"] <= 1.05 * tol for v in results.values()),
+        "roofline": None, "cpu_baseline": None}), flush=True)
+
+
 def ratio(runs, a, b, key):
     return round(runs[a][key] / runs[b][key], 3) if a in runs and b in runs and runs[b][key] > 0 else None
 
@@ -254,6 +352,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU strip decomposition is "
               f"a design note, not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "sheet1024":
+        graph_bench(args)
+        return
     from bd.configs import config_sheet
 
     cfg = config_sheet(args.size, self_contact=not args.no_contact)

@@ -123,6 +123,31 @@ def config_sheet(n=1024, pitch=1e-3, region=256, rel_tol=1e-6, history=True, new
             "velocity": [np.array([-5e-3, 0, 0]), np.array([5e-3, 0, 0])], "steps": 240}
 
 
+def config_patches(count=48, side_range=(32, 192), speed_range=(1e-3, 8e-3), seed=4, pitch=1e-3, rel_tol=1e-6,
+                   newton=3, steps=120):
+    """`count` independent hanging patches (batched uneven solves): grid side uniform in `side_range`
+    from `seed`, 1 mm pitch, the sheet material of the 1024 x 1024 workload, top row pinned, one bottom
+    corner gripped and moved along a random in-plane direction at a speed uniform in `speed_range`
+    (m per step). Each patch is its own Newton / MG-PCG system with the mixed method (hierarchy rebuilt
+    every Newton iteration, coarsest level of at most 400 nodes solved exactly in FP64), rel-res
+    `rel_tol`, h = 1/120."""
+    rng = np.random.default_rng(seed)
+    mat = linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / (2 * 1.2))
+    patches = []
+    for p in range(count):
+        n = int(rng.integers(side_range[0], side_range[1] + 1))
+        S = sheet_scene(n, pitch, mat, seed=101 + p)
+        grip = corner_grip(S, int(rng.integers(0, 2)), 0, k=2e4)
+        ang = rng.uniform(0.0, 2.0 * math.pi)
+        speed = rng.uniform(*speed_range)
+        params = make_params(h=1 / 120, newton_iterations=newton, pcg_rel_tol=rel_tol, pcg_max_iterations=200,
+                             max_coarsest_nodes=400, coarse_sweeps=2, lag_coarse=False, line_search=True, history=False,
+                             coarse_solve=1)
+        patches.append({"side": n, "scene": S, "params": params, "grips": [grip],
+                        "velocity": [np.array([speed * math.cos(ang), 0.0, speed * math.sin(ang)])]})
+    return {"name": f"patches{count}", "patches": patches, "steps": steps}
+
+
 def advance_grips(grips, velocity):
     """Move every grip goal by its per-step velocity (the caller's handle motion)."""
     for g, v in zip(grips, velocity):

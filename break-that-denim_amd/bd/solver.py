@@ -117,6 +117,18 @@ class GpuSolver:
 
     substep = encode_substep
 
+    # ---- graph-captured substep (encodeSubstep recorded once, replayed per frame)
+    def graph_capture(self, pcg_iterations):
+        """Record one substep into a HIP graph; each Newton solve gets `pcg_iterations` PCG iterations,
+        those past convergence gated off on the device."""
+        self._check(self.lib.bd_graph_capture(self.h, int(pcg_iterations)), "bd_graph_capture")
+
+    def graph_substep(self):
+        self._check(self.lib.bd_graph_substep(self.h), "bd_graph_substep")
+
+    def graph_release(self):
+        self._check(self.lib.bd_graph_release(self.h), "bd_graph_release")
+
     def encode_predict(self):
         self._check(self.lib.bd_predict(self.h), "bd_predict")
 
@@ -372,3 +384,36 @@ class GpuSolver:
         z = np.zeros((capi.BD_NET_FIELDS, 3, npad), np.float32)
         self._check(self.lib.bd_learn_read_fields(self.h, z.ctypes.data_as(C.POINTER(C.c_float))), "bd_learn_read_fields")
         return z
+
+
+class Batch:
+    """Independent solvers (uneven patches, each its own Newton / MG-PCG system) advanced by one graph
+    launch per frame: every solver's captured substep is a branch of the batch's graph."""
+
+    def __init__(self, solvers):
+        self.lib = solvers[0].lib
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[g.h for g in self.solvers])
+        out = C.c_void_p()
+        rc = self.lib.bd_batch_create(arr, len(self.solvers), C.byref(out))
+        if rc != 0:
+            raise RuntimeError(f"bd_batch_create failed ({rc}): {self.lib.bd_batch_last_error(None).decode()}")
+        self.b = out
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {self.lib.bd_batch_last_error(self.b).decode()}")
+
+    def substep(self):
+        self._check(self.lib.bd_batch_substep(self.b), "bd_batch_substep")
+
+    def sync(self):
+        self._check(self.lib.bd_batch_sync(self.b), "bd_batch_sync")
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.bd_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        self.close()

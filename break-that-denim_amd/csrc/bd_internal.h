@@ -250,6 +250,15 @@ struct Solver {
   // learned coarse-space correction (learned.hip): network, recorded samples, chain vectors
   struct Learn* learn = nullptr;
   int learned = 0;             // BD_LEARNED_*: the learned warm start, and the deflated correction
+  // graph-captured substep (bd_graph_capture): while capturing, graph_pcg caps each Newton solve's PCG
+  // iterations and replaces the host's convergence poll by the device gate (iterations queued past
+  // convergence return at once); the graph, its executable and its Newton count; config_gen counts
+  // the changes of what a substep launches, graph_gen is its value at the capture
+  int graph_pcg = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_newton = 0;
+  unsigned config_gen = 0, graph_gen = 0;
   int* counters = nullptr;     // [0] ruptures, [1] pulled out
   // multigrid
   int nlev = 0;

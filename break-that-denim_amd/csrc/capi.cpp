@@ -185,6 +185,14 @@ void build_lines(Solver& s, Level& L, int ncol) {
   upload(s, L.lines, flat.data(), flat.size());
 }
 
+// the captured substep, if any
+void release_graph(Solver& s) {
+  if (s.graph_exec) (void)hipGraphExecDestroy(s.graph_exec);
+  if (s.graph) (void)hipGraphDestroy(s.graph);
+  s.graph_exec = nullptr;
+  s.graph = nullptr;
+}
+
 void destroy_all(Solver& s) {
   void* ptrs[] = {s.layers, s.x, s.v, s.dx, s.pred, s.delta, s.rhs, s.p, s.ap, s.last_update, s.mass, s.cn0, s.pinned,
                   s.layer_of, s.rest, s.intact, s.fail_site, s.tails, s.softening, s.cut_mask, s.flags, s.strain,
@@ -220,6 +228,7 @@ void destroy_all(Solver& s) {
   for (hipEvent_t e : s.prof_pool) (void)hipEventDestroy(e);
   learn_free(s);
   s.prof_pool.clear();
+  release_graph(s);
   if (s.stream) (void)hipStreamDestroy(s.stream);
 }
 
@@ -281,6 +290,7 @@ void upload_slot(Solver& s, int g) {
   BD_HIP(hipStreamSynchronize(s.stream));
   bool any = false;
   for (int q = 0; q < kGrips; q++) any = any || s.grip_slots_h[q].count > 0;
+  if (any != s.grips_active) s.config_gen++;
   s.grips_active = any;
 }
 
@@ -314,6 +324,38 @@ void newton_iteration(Solver& s, int it, bool coarse) {
   launch_curve_apply(s);
   launch_newton_apply(s, it);
   launch_record(s, it);
+}
+
+// one substep (encodeSubstep, gpu/solver.ts:737-760): prediction, grip lead, contact detection and
+// projectStart, n_it Newton iterations and the optional ones up to n_max while the last update moved a
+// node by more than the Newton tolerance (gate.wgsl), commit, damage field and slip; returns the Newton
+// iterations run
+int substep_body(Solver& s, int n_it, int n_max) {
+  launch_reset_counters(s);
+  predict(s);
+  // once per substep: each grip's handle lead over its patch (interaction.ts:335-338)
+  launch_grip_lead(s);
+  contact_detect(s);
+  if (s.contact_mode != kContactNone || s.floor_on) launch_project_start(s);
+  int done = 0;
+  for (int it = 0; it < n_max; it++) {
+    if (it >= n_it) {
+      // gated optional iteration (gate.wgsl): only while the last update moved a node > tolerance
+      BD_HIP(hipMemcpyAsync(s.scal_h->max_update, s.scal->max_update, sizeof(s.scal_h->max_update), hipMemcpyDeviceToHost, s.stream));
+      BD_HIP(hipStreamSynchronize(s.stream));
+      float moved;
+      std::memcpy(&moved, &s.scal_h->max_update[it - 1], sizeof(float));
+      if (!(moved > (float)s.P.newton_tolerance)) break;
+    }
+    newton_iteration(s, it, it == 0 || !s.P.lag_coarse);
+    done = it + 1;
+  }
+  launch_commit(s);
+  if (s.P.history) {
+    launch_damage(s);
+    launch_slip(s);
+  }
+  return done;
 }
 
 void ensure_pieces(Solver& s) {
@@ -635,6 +677,7 @@ int bd_set_grips(bd_solver* h, const bd_grips* g) {
       }
     }
     s.grip_count = count;
+    s.config_gen++;
     std::memcpy(s.grip_goal_h + (size_t)Solver::kGoalSlots * 4 * BD_MAX_GRIPS, gk.data(), sizeof(double) * 4 * BD_MAX_GRIPS);
     upload(s, s.grip_of, gof.data(), gof.size());
     upload(s, s.grip_w, w.data(), w.size());
@@ -666,6 +709,7 @@ int bd_set_timestep(bd_solver* h, double dt) {
   return guarded(h, [&](Solver& s) {
     if (!(dt > 0)) return fail(h, BD_ERR_INVALID, "bd_set_timestep: h must be positive");
     s.P.h = dt;
+    s.config_gen++;
     return BD_OK;
   });
 }
@@ -673,6 +717,7 @@ int bd_set_timestep(bd_solver* h, double dt) {
 int bd_set_history(bd_solver* h, int on) {
   return guarded(h, [&](Solver& s) {
     s.P.history = on ? 1 : 0;
+    s.config_gen++;
     return BD_OK;
   });
 }
@@ -680,6 +725,7 @@ int bd_set_history(bd_solver* h, int on) {
 int bd_set_newton_tolerance(bd_solver* h, double tol) {
   return guarded(h, [&](Solver& s) {
     s.P.newton_tolerance = tol;
+    s.config_gen++;
     return BD_OK;
   });
 }
@@ -690,6 +736,7 @@ int bd_set_pcg(bd_solver* h, int iterations, double rel_tol, int max_iterations)
     s.P.pcg_iterations = iterations;
     s.P.pcg_rel_tol = rel_tol;
     s.P.pcg_max_iterations = max_iterations;
+    s.config_gen++;
     return BD_OK;
   });
 }
@@ -822,6 +869,7 @@ int bd_set_floor(bd_solver* h, int enabled, double z) {
     BD_HIP(hipStreamSynchronize(s.stream));
     s.floor_on = enabled != 0;
     s.floor_z = enabled ? (float)z : 0.f;
+    s.config_gen++;
     return BD_OK;
   });
 }
@@ -973,6 +1021,7 @@ int bd_set_smoother(bd_solver* h, int level, int family, int variant) {
         return fail(h, BD_ERR_INVALID, "bd_set_smoother: the level cannot run that variant");
     }
     s.lv[level].variant[family] = variant;
+    s.config_gen++;
     return BD_OK;
   });
 }
@@ -1026,6 +1075,7 @@ int bd_autotune(bd_solver* h, int reps, int32_t* chosen) {
       }
     BD_HIP(hipEventDestroy(e0));
     BD_HIP(hipEventDestroy(e1));
+    s.config_gen++;
     if (chosen)
       for (int k = 0; k < s.nlev; k++) {
         chosen[2 * k] = s.lv[k].variant[0];
@@ -1062,6 +1112,7 @@ int bd_set_pieces(bd_solver* h, const uint32_t* piece_of, const uint32_t* start,
     // no piece left: no node steps along a rotation
     if (c == 0 && s.npieces > 0) BD_HIP(hipMemsetAsync(s.curve, 0, 7 * (size_t)s.Np * sizeof(float), s.stream));
     s.npieces = c;
+    s.config_gen++;
     BD_HIP(hipStreamSynchronize(s.stream));
     return BD_OK;
   });
@@ -1098,6 +1149,7 @@ int bd_update_topology(bd_solver* h, const int32_t* anchors, int anchor_count, i
         if (freed[n] && st[n] == kStateCloth) st[n] = kStateDetached;
       upload(s, s.nstate, st.data(), (size_t)N);
       s.any_detached = true;
+      s.config_gen++;
     }
     if (freed_out) *freed_out = nfree;
     return bd_set_pieces(h, of.data(), start.data(), count, nodes.data());
@@ -1108,31 +1160,58 @@ int bd_substep(bd_solver* h, int min_it, int max_it) {
   return guarded(h, [&](Solver& s) {
     const int n_it = std::min(BD_MAX_NEWTON, std::max(s.P.newton_iterations, min_it));
     const int n_max = std::min(BD_MAX_NEWTON, std::max(std::max(s.P.newton_max, n_it), max_it));
-    launch_reset_counters(s);
-    predict(s);
-    // once per substep: each grip's handle lead over its patch (interaction.ts:335-338)
-    launch_grip_lead(s);
-    contact_detect(s);
-    if (s.contact_mode != kContactNone || s.floor_on) launch_project_start(s);
-    int done = 0;
-    for (int it = 0; it < n_max; it++) {
-      if (it >= n_it) {
-        // gated optional iteration (gate.wgsl): only while the last update moved a node > tolerance
-        BD_HIP(hipMemcpyAsync(s.scal_h->max_update, s.scal->max_update, sizeof(s.scal_h->max_update), hipMemcpyDeviceToHost, s.stream));
-        BD_HIP(hipStreamSynchronize(s.stream));
-        float moved;
-        std::memcpy(&moved, &s.scal_h->max_update[it - 1], sizeof(float));
-        if (!(moved > (float)s.P.newton_tolerance)) break;
-      }
-      newton_iteration(s, it, it == 0 || !s.P.lag_coarse);
-      done = it + 1;
+    s.stats.newton_iterations = substep_body(s, n_it, n_max);
+    return BD_OK;
+  });
+}
+
+int bd_graph_capture(bd_solver* h, int pcg_iterations) {
+  return guarded(h, [&](Solver& s) {
+    if (pcg_iterations < 1) return fail(h, BD_ERR_INVALID, "bd_graph_capture: pcg_iterations must be positive");
+    if (s.prof_on) return fail(h, BD_ERR_STATE, "bd_graph_capture: profiling is on (its events are host work)");
+    if (s.npieces > 0) return fail(h, BD_ERR_STATE, "bd_graph_capture: detached pieces are updated by the host");
+    if (s.learned != BD_LEARNED_OFF || learn_recording(s))
+      return fail(h, BD_ERR_STATE, "bd_graph_capture: the learned path and sample recording run eagerly");
+    const int n_it = std::min(BD_MAX_NEWTON, s.P.newton_iterations);
+    if (std::max(s.P.newton_max, n_it) > n_it)
+      return fail(h, BD_ERR_STATE, "bd_graph_capture: optional Newton iterations (newton_max > newton_iterations) are gated by the host");
+    BD_HIP(hipStreamSynchronize(s.stream));
+    release_graph(s);
+    s.graph_pcg = pcg_iterations;
+    BD_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+    try {
+      substep_body(s, n_it, n_it);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s.stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      s.graph_pcg = 0;
+      throw;
     }
-    s.stats.newton_iterations = done;
-    launch_commit(s);
-    if (s.P.history) {
-      launch_damage(s);
-      launch_slip(s);
-    }
+    s.graph_pcg = 0;
+    BD_HIP(hipStreamEndCapture(s.stream, &s.graph));
+    BD_HIP(hipGraphInstantiate(&s.graph_exec, s.graph, nullptr, nullptr, 0));
+    s.graph_newton = n_it;
+    s.graph_gen = s.config_gen;
+    return BD_OK;
+  });
+}
+
+int bd_graph_substep(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    if (!s.graph_exec) return fail(h, BD_ERR_STATE, "bd_graph_substep: no captured substep (bd_graph_capture)");
+    if (s.graph_gen != s.config_gen)
+      return fail(h, BD_ERR_STATE, "bd_graph_substep: the solver's configuration changed since bd_graph_capture; capture again");
+    BD_HIP(hipGraphLaunch(s.graph_exec, s.stream));
+    s.stats.newton_iterations = s.graph_newton;
+    return BD_OK;
+  });
+}
+
+int bd_graph_release(bd_solver* h) {
+  return guarded(h, [&](Solver& s) {
+    BD_HIP(hipStreamSynchronize(s.stream));
+    release_graph(s);
     return BD_OK;
   });
 }
@@ -1141,6 +1220,7 @@ int bd_profile_enable(bd_solver* h, int on) {
   return guarded(h, [&](Solver& s) {
     BD_HIP(hipStreamSynchronize(s.stream));
     s.prof_on = on != 0;
+    s.config_gen++;
     s.prof_recs.clear();
     return BD_OK;
   });
@@ -1177,6 +1257,7 @@ int bd_learn_record(bd_solver* h, int capacity) {
     if (capacity < 0) return fail(h, BD_ERR_INVALID, "bd_learn_record: negative capacity");
     BD_HIP(hipStreamSynchronize(s.stream));
     learn_record_start(s, capacity);
+    s.config_gen++;
     return BD_OK;
   });
 }
@@ -1199,6 +1280,7 @@ int bd_set_learned(bd_solver* h, int on) {
       learn_alloc(s);
     }
     s.learned = on;
+    s.config_gen++;
     // the next Newton iteration predicts from its hierarchy
     if (s.learned && s.hierarchy) learn_predict(s);
     return BD_OK;
@@ -1572,6 +1654,7 @@ int bd_write(bd_solver* h, bd_buffer buf, int level, const void* src, size_t byt
         for (int n = 0; n < s.N; n++)
           if (in[n] > kStateRetired) return fail(h, BD_ERR_INVALID, "bd_write: node state must be 0, 1 or 2");
         for (int n = 0; n < s.N; n++) s.any_detached = s.any_detached || in[n] != kStateCloth;
+        s.config_gen++;
         upload(s, s.nstate, in, (size_t)s.N);
         launch_apply_state(s);
         break;
@@ -1620,6 +1703,132 @@ int bd_get_stats(bd_solver* h, bd_stats* out) {
     *out = st;
     return BD_OK;
   });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches of independent solvers
+
+struct bd_batch {
+  std::vector<bd_solver*> solvers;
+  std::vector<hipGraph_t> captured;  // each solver's graph when the batch was made
+  std::vector<hipEvent_t> ready;     // per solver: its stream's work before a launch
+  hipEvent_t done = nullptr;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string err;
+};
+
+namespace {
+
+void batch_free(bd_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (hipEvent_t e : b->ready) (void)hipEventDestroy(e);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+template <class F>
+int batch_guarded(bd_batch* b, F&& f) {
+  if (!b) return BD_ERR_INVALID;
+  try {
+    return f(*b);
+  } catch (const std::exception& e) {
+    b->err = e.what();
+    return BD_ERR_HIP;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bd_batch_last_error(const bd_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int bd_batch_create(bd_solver* const* solvers, int count, bd_batch** out) {
+  g_create_error.clear();
+  if (!out || !solvers || count < 1) {
+    g_create_error = "bd_batch_create: need at least one solver and an output";
+    return BD_ERR_INVALID;
+  }
+  *out = nullptr;
+  for (int i = 0; i < count; i++) {
+    if (!solvers[i] || !solvers[i]->s.graph_exec) {
+      g_create_error = "bd_batch_create: every solver needs a captured substep (bd_graph_capture)";
+      return BD_ERR_STATE;
+    }
+    for (int j = 0; j < i; j++)
+      if (solvers[j] == solvers[i]) {
+        g_create_error = "bd_batch_create: a solver appears twice";
+        return BD_ERR_INVALID;
+      }
+  }
+  bd_batch* b = new bd_batch;
+  try {
+    BD_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BD_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+    BD_HIP(hipGraphCreate(&b->graph, 0));
+    for (int i = 0; i < count; i++) {
+      Solver& s = solvers[i]->s;
+      // each patch its own branch: no dependencies between them
+      hipGraphNode_t node;
+      BD_HIP(hipGraphAddChildGraphNode(&node, b->graph, nullptr, 0, s.graph));
+      hipEvent_t e;
+      BD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      b->ready.push_back(e);
+      b->solvers.push_back(solvers[i]);
+      b->captured.push_back(s.graph);
+    }
+    BD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    batch_free(b);
+    return BD_ERR_HIP;
+  }
+  *out = b;
+  return BD_OK;
+}
+
+int bd_batch_substep(bd_batch* b) {
+  return batch_guarded(b, [&](bd_batch& B) {
+    for (size_t i = 0; i < B.solvers.size(); i++) {
+      const Solver& s = B.solvers[i]->s;
+      if (s.graph != B.captured[i] || s.graph_gen != s.config_gen) {
+        B.err = "bd_batch_substep: solver " + std::to_string(i) + " was captured again or changed since bd_batch_create";
+        return BD_ERR_STATE;
+      }
+    }
+    // the launch follows each solver's queued work (grip goals, uploads), and each solver's later
+    // work follows the launch
+    for (size_t i = 0; i < B.solvers.size(); i++) {
+      BD_HIP(hipEventRecord(B.ready[i], B.solvers[i]->s.stream));
+      BD_HIP(hipStreamWaitEvent(B.stream, B.ready[i], 0));
+    }
+    BD_HIP(hipGraphLaunch(B.exec, B.stream));
+    BD_HIP(hipEventRecord(B.done, B.stream));
+    for (bd_solver* h : B.solvers) {
+      BD_HIP(hipStreamWaitEvent(h->s.stream, B.done, 0));
+      h->s.stats.newton_iterations = h->s.graph_newton;
+    }
+    return BD_OK;
+  });
+}
+
+int bd_batch_sync(bd_batch* b) {
+  return batch_guarded(b, [&](bd_batch& B) {
+    BD_HIP(hipStreamSynchronize(B.stream));
+    return BD_OK;
+  });
+}
+
+int bd_batch_destroy(bd_batch* b) {
+  if (!b) return BD_OK;
+  (void)hipStreamSynchronize(b->stream);
+  batch_free(b);
+  return BD_OK;
 }
 
 }  // extern "C"

@@ -773,7 +773,10 @@ bool wait_flag(Solver& s, int seq) {
 void pcg_solve(Solver& s) {
   PcgArgs A = pcg_args(s);
   const bool tol_mode = s.P.pcg_rel_tol > 0;
-  const int K = tol_mode ? std::max(1, s.P.pcg_max_iterations) : std::max(0, s.P.pcg_iterations);
+  // captured into a graph: a fixed budget of iterations, the device's done flag gating those past
+  // convergence, no host poll
+  const bool graph = s.graph_pcg > 0;
+  const int K = tol_mode ? (graph ? s.graph_pcg : std::max(1, s.P.pcg_max_iterations)) : std::max(0, s.P.pcg_iterations);
   const int G = dot_grid(s.N), B = grid_for(s.N, 256);
   const double N = s.N;
   const bool learned = s.learned != BD_LEARNED_OFF && learn_ready(s) && s.npieces == 0;
@@ -830,7 +833,7 @@ void pcg_solve(Solver& s) {
     }
     if (record && it + 1 == BD_NET_SLOW_ITER) learn_record_mid(s, A.r, A.delta);
     if (it + 1 == K) break;
-    if (tol_mode) {
+    if (tol_mode && !graph) {
       // the device decides convergence; check it before queueing the next V-cycle
       const int seq = ++s.flag_seq;
       k_post_flag<<<1, 1, 0, s.stream>>>(s.scal, s.flag_d, seq);

@@ -251,6 +251,35 @@ int bd_set_pcg(bd_solver* s, int iterations, double rel_tol, int max_iterations)
 /* encodeSubstep(encoder, minIterations, maxIterations) (gpu/solver.ts:737-760). */
 int bd_substep(bd_solver* s, int min_iterations, int max_iterations);
 
+/* A substep recorded once and replayed: the reference records a whole substep into one compute pass
+ * with its optional Newton iterations gated on the device (gpu/solver.ts:737-760, gate.wgsl:12-23).
+ * bd_graph_capture records one bd_substep (the solver's Newton iterations, no optional ones) into a
+ * HIP graph; each Newton solve in tolerance mode then runs a fixed budget of pcg_iterations, those
+ * past convergence returning at once on the device's done flag instead of the host's per-iteration
+ * check (in fixed-count mode the count is the solver's). A solve that needs more than the budget ends
+ * unconverged, visible in bd_get_stats (rel_residual above the tolerance). bd_graph_substep replays it
+ * on the solver's stream, reading the grip goals and state that the calls before it left on the
+ * device. Refused (BD_ERR_STATE) with profiling on, detached pieces, the learned path or sample
+ * recording, or newton_max above newton_iterations. A setter that changes what a substep launches
+ * (timestep, history, PCG schedule, floor, grips, smoother variants, pieces, learned mode) makes
+ * bd_graph_substep return BD_ERR_STATE until the next capture. */
+int bd_graph_capture(bd_solver* s, int pcg_iterations);
+int bd_graph_substep(bd_solver* s);
+int bd_graph_release(bd_solver* s);
+
+/* Batched independent solves (uneven patches, each its own Newton / MG-PCG system with its own
+ * scalars and convergence): the captured substeps of `count` solvers as the branches of one graph,
+ * one launch per frame on the batch's stream. The launch waits for each solver's queued work (grip
+ * goals, uploads) and each solver's later calls wait for it. Every solver needs bd_graph_capture
+ * first, and capturing one again or changing it (as above) makes bd_batch_substep return
+ * BD_ERR_STATE. No counterpart in the reference, which runs one scene per solver. */
+typedef struct bd_batch bd_batch;
+int bd_batch_create(bd_solver* const* solvers, int count, bd_batch** out);
+int bd_batch_substep(bd_batch* b);
+int bd_batch_sync(bd_batch* b);
+int bd_batch_destroy(bd_batch* b);
+const char* bd_batch_last_error(const bd_batch* b);
+
 /* Stage entry points (gpu/solver.ts:574-728 encodePredict, encodeAssemble, encodeVCycle,
  * encodePCG, encodeLineSearch, encodeSmooth, encodeFinalize, encodeDamage). */
 int bd_predict(bd_solver* s);

@@ -90,3 +90,20 @@ def test_null_handle_is_an_error():
     lib = capi.load_library()
     assert lib.bd_substep(None, 0, 0) == -1
     assert lib.bd_read(None, 0, 0, None, 0) == -1
+
+
+def test_graph_and_batch_argument_errors():
+    """The capture and batch entry points reject missing handles and empty batches before any device
+    work, with the library's message."""
+    import ctypes as C
+    lib = capi.load_library()
+    assert lib.bd_graph_capture(None, 10) == -1
+    assert lib.bd_graph_substep(None) == -1
+    assert lib.bd_batch_substep(None) == -1
+    out = C.c_void_p()
+    assert lib.bd_batch_create(None, 0, C.byref(out)) == -1
+    assert b"at least one solver" in lib.bd_batch_last_error(None)
+    arr = (C.c_void_p * 1)(None)
+    assert lib.bd_batch_create(arr, 1, C.byref(out)) == capi.BD_ERR_STATE
+    assert b"bd_graph_capture" in lib.bd_batch_last_error(None)
+    assert lib.bd_batch_destroy(None) == 0
