@@ -16,6 +16,7 @@ LIB_PATH = PKG_DIR / "libbd.so"
 
 BD_OK = 0
 BD_ERR_INVALID, BD_ERR_HIP, BD_ERR_LIMIT, BD_ERR_NOMEM, BD_ERR_STATE = -1, -2, -3, -4, -5
+BD_PCG_GRID, BD_PCG_WORKGROUP = 0, 1
 BD_MAX_LINE = 1024
 BD_MAX_LEVELS = 12
 BD_MAX_NEWTON = 8
@@ -307,6 +308,7 @@ def load_library(path: os.PathLike | None = None):
     lib.bd_learn_read_fields.argtypes = [vp, f32p]
     lib.bd_apply.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     lib.bd_apply_noalloc.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+    lib.bd_set_pcg_mode.argtypes = [vp, C.c_int]
     lib.bd_graph_capture.argtypes = [vp, C.c_int]
     lib.bd_graph_substep.argtypes = [vp]
     lib.bd_graph_release.argtypes = [vp]
@@ -333,6 +335,6 @@ EXPORTED = (
     "bd_grip_release", "bd_pick_ray", "bd_pick_near", "bd_autotune", "bd_set_smoother",
     "bd_learn_record", "bd_learn_train", "bd_set_learned", "bd_learn_get_weights", "bd_learn_set_weights",
     "bd_learn_loss_grad", "bd_learn_read_sample", "bd_learn_info", "bd_learn_read_fields", "bd_apply",
-    "bd_apply_noalloc", "bd_graph_capture", "bd_graph_substep", "bd_graph_release", "bd_batch_create",
+    "bd_apply_noalloc", "bd_set_pcg_mode", "bd_graph_capture", "bd_graph_substep", "bd_graph_release", "bd_batch_create",
     "bd_batch_substep", "bd_batch_sync", "bd_batch_destroy", "bd_batch_last_error",
 )

@@ -117,6 +117,11 @@ class GpuSolver:
 
     substep = encode_substep
 
+    def set_pcg_mode(self, mode):
+        """BD_PCG_GRID (0): PCG kernels over the chip; BD_PCG_WORKGROUP (1): the whole solve in one
+        workgroup (small systems, batches of patches)."""
+        self._check(self.lib.bd_set_pcg_mode(self.h, int(mode)), "bd_set_pcg_mode")
+
     # ---- graph-captured substep (encodeSubstep recorded once, replayed per frame)
     def graph_capture(self, pcg_iterations):
         """Record one substep into a HIP graph; each Newton solve gets `pcg_iterations` PCG iterations,

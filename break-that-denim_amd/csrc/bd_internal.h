@@ -92,6 +92,8 @@ constexpr int kStagedMaxLen = 257;
 // levels whose longest line has at least this many nodes keep the column-major copy: below it a
 // v-line pass's strided gather costs less than the copy (about twice a u-line pass at 65 nodes)
 constexpr int kDualMinLine = 33;
+// longest line the one-workgroup PCG solves (its LDS holds the line records of a colour round)
+constexpr int kWgMaxLine = 512;
 
 struct Scal {
   // PCG and line-search scalars kept on the device
@@ -259,6 +261,12 @@ struct Solver {
   hipGraphExec_t graph_exec = nullptr;
   int graph_newton = 0;
   unsigned config_gen = 0, graph_gen = 0;
+  unsigned graph_serial = 0;  // captures so far (a batch keeps the serial of the graph it copied)
+  // PCG path (bd_set_pcg_mode): BD_PCG_GRID kernels over the chip, or BD_PCG_WORKGROUP, the whole
+  // solve in one workgroup (wgsolve.hip) with its device descriptor and dynamic LDS
+  int pcg_mode = BD_PCG_GRID;
+  void* wg_sys = nullptr;
+  size_t wg_lds = 0;
   int* counters = nullptr;     // [0] ruptures, [1] pulled out
   // multigrid
   int nlev = 0;
@@ -341,6 +349,13 @@ void launch_prolong_vec(Solver& s, int k, const float* xc, float* xf, bool set, 
 void vcycle(Solver& s, int k);
 // pcg.hip
 void pcg_solve(Solver& s);
+void launch_pcg_setup(Solver& s);
+std::string wg_unsupported(const Solver& s);
+void wg_prepare(Solver& s);
+void wg_free(Solver& s);
+void launch_pcg_wg(Solver& s);
+size_t wg_sys_bytes();
+void launch_pcg_wg_many(const void* systems, int count, size_t lds, hipStream_t st);
 inline const int* pcg_done(Solver& s) { return &s.scal->done; }
 inline const int* learn_off(Solver& s) { return &s.scal->learn_off; }
 inline const int* learn_gate(Solver& s) { return &s.scal->lgate; }

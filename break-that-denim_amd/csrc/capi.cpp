@@ -229,6 +229,7 @@ void destroy_all(Solver& s) {
   learn_free(s);
   s.prof_pool.clear();
   release_graph(s);
+  wg_free(s);
   if (s.stream) (void)hipStreamDestroy(s.stream);
 }
 
@@ -314,29 +315,51 @@ void line_search(Solver& s) {
   launch_energy(s);
 }
 
-void newton_iteration(Solver& s, int it, bool coarse) {
+// a Newton iteration before its linear solve: contact and grip terms at this iterate (grip terms
+// added to contact's, GpuSolver.preAssemble, interaction.ts:341-345), the system and its hierarchy
+void newton_pre(Solver& s, bool coarse) {
   contact_terms(s);
-  // grip terms at this iterate, added to contact's (GpuSolver.preAssemble, interaction.ts:341-345)
   launch_grip_terms(s);
   assemble_system(s, coarse);
-  pcg_solve(s);
+}
+
+// and after it: line search, the update, the iteration's record
+void newton_post(Solver& s, int it) {
   line_search(s);
   launch_curve_apply(s);
   launch_newton_apply(s, it);
   launch_record(s, it);
 }
 
+void newton_iteration(Solver& s, int it, bool coarse) {
+  newton_pre(s, coarse);
+  pcg_solve(s);
+  newton_post(s, it);
+}
+
 // one substep (encodeSubstep, gpu/solver.ts:737-760): prediction, grip lead, contact detection and
 // projectStart, n_it Newton iterations and the optional ones up to n_max while the last update moved a
 // node by more than the Newton tolerance (gate.wgsl), commit, damage field and slip; returns the Newton
 // iterations run
-int substep_body(Solver& s, int n_it, int n_max) {
+void substep_begin(Solver& s) {
   launch_reset_counters(s);
   predict(s);
   // once per substep: each grip's handle lead over its patch (interaction.ts:335-338)
   launch_grip_lead(s);
   contact_detect(s);
   if (s.contact_mode != kContactNone || s.floor_on) launch_project_start(s);
+}
+
+void substep_end(Solver& s) {
+  launch_commit(s);
+  if (s.P.history) {
+    launch_damage(s);
+    launch_slip(s);
+  }
+}
+
+int substep_body(Solver& s, int n_it, int n_max) {
+  substep_begin(s);
   int done = 0;
   for (int it = 0; it < n_max; it++) {
     if (it >= n_it) {
@@ -350,11 +373,7 @@ int substep_body(Solver& s, int n_it, int n_max) {
     newton_iteration(s, it, it == 0 || !s.P.lag_coarse);
     done = it + 1;
   }
-  launch_commit(s);
-  if (s.P.history) {
-    launch_damage(s);
-    launch_slip(s);
-  }
+  substep_end(s);
   return done;
 }
 
@@ -736,6 +755,21 @@ int bd_set_pcg(bd_solver* h, int iterations, double rel_tol, int max_iterations)
     s.P.pcg_iterations = iterations;
     s.P.pcg_rel_tol = rel_tol;
     s.P.pcg_max_iterations = max_iterations;
+    s.config_gen++;
+    return BD_OK;
+  });
+}
+
+int bd_set_pcg_mode(bd_solver* h, int mode) {
+  return guarded(h, [&](Solver& s) {
+    if (mode != BD_PCG_GRID && mode != BD_PCG_WORKGROUP) return fail(h, BD_ERR_INVALID, "bd_set_pcg_mode: unknown mode");
+    if (mode == BD_PCG_WORKGROUP) {
+      const std::string why = wg_unsupported(s);
+      if (!why.empty()) return fail(h, BD_ERR_STATE, "bd_set_pcg_mode: no one-workgroup solve: " + why);
+      BD_HIP(hipStreamSynchronize(s.stream));
+      wg_prepare(s);
+    }
+    s.pcg_mode = mode;
     s.config_gen++;
     return BD_OK;
   });
@@ -1193,6 +1227,7 @@ int bd_graph_capture(bd_solver* h, int pcg_iterations) {
     BD_HIP(hipGraphInstantiate(&s.graph_exec, s.graph, nullptr, nullptr, 0));
     s.graph_newton = n_it;
     s.graph_gen = s.config_gen;
+    s.graph_serial++;
     return BD_OK;
   });
 }
@@ -1711,7 +1746,14 @@ int bd_get_stats(bd_solver* h, bd_stats* out) {
 
 struct bd_batch {
   std::vector<bd_solver*> solvers;
-  std::vector<hipGraph_t> captured;  // each solver's graph when the batch was made
+  // fused form: every solver in BD_PCG_WORKGROUP mode, the PCG solves of a Newton iteration in one
+  // launch over the systems' descriptors (copied back to back)
+  bool fused = false;
+  void* systems = nullptr;
+  size_t lds = 0;
+  int newton = 0;
+  std::vector<unsigned> gens;
+  std::vector<unsigned> captured;    // each solver's capture serial when the batch was made
   std::vector<hipEvent_t> ready;     // per solver: its stream's work before a launch
   hipEvent_t done = nullptr;
   hipStream_t stream = nullptr;
@@ -1723,6 +1765,7 @@ struct bd_batch {
 namespace {
 
 void batch_free(bd_batch* b) {
+  if (b->systems) (void)hipFree(b->systems);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (hipEvent_t e : b->ready) (void)hipEventDestroy(e);
@@ -1755,9 +1798,10 @@ int bd_batch_create(bd_solver* const* solvers, int count, bd_batch** out) {
     return BD_ERR_INVALID;
   }
   *out = nullptr;
+  bool fused = true;
   for (int i = 0; i < count; i++) {
-    if (!solvers[i] || !solvers[i]->s.graph_exec) {
-      g_create_error = "bd_batch_create: every solver needs a captured substep (bd_graph_capture)";
+    if (!solvers[i]) {
+      g_create_error = "bd_batch_create: every solver needs a captured substep (bd_graph_capture) or BD_PCG_WORKGROUP";
       return BD_ERR_STATE;
     }
     for (int j = 0; j < i; j++)
@@ -1765,24 +1809,53 @@ int bd_batch_create(bd_solver* const* solvers, int count, bd_batch** out) {
         g_create_error = "bd_batch_create: a solver appears twice";
         return BD_ERR_INVALID;
       }
+    fused = fused && solvers[i]->s.pcg_mode == BD_PCG_WORKGROUP;
+  }
+  const int newton = fused ? std::min(BD_MAX_NEWTON, solvers[0]->s.P.newton_iterations) : 0;
+  for (int i = 0; i < count; i++) {
+    const Solver& s = solvers[i]->s;
+    if (fused) {
+      if (std::min(BD_MAX_NEWTON, s.P.newton_iterations) != newton || std::max(s.P.newton_max, s.P.newton_iterations) > s.P.newton_iterations) {
+        g_create_error = "bd_batch_create: solvers in lockstep need the same Newton iterations and no optional ones";
+        return BD_ERR_STATE;
+      }
+    } else if (!s.graph_exec) {
+      g_create_error = "bd_batch_create: every solver needs a captured substep (bd_graph_capture) or BD_PCG_WORKGROUP";
+      return BD_ERR_STATE;
+    }
   }
   bd_batch* b = new bd_batch;
   try {
     BD_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     BD_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
-    BD_HIP(hipGraphCreate(&b->graph, 0));
+    b->fused = fused;
+    b->newton = newton;
+    if (fused) {
+      const size_t one = wg_sys_bytes();
+      BD_HIP(hipMalloc(&b->systems, one * count));
+      for (int i = 0; i < count; i++) {
+        Solver& s = solvers[i]->s;
+        BD_HIP(hipMemcpy(static_cast<char*>(b->systems) + one * i, s.wg_sys, one, hipMemcpyDeviceToDevice));
+        b->lds = std::max(b->lds, s.wg_lds);
+      }
+    } else {
+      BD_HIP(hipGraphCreate(&b->graph, 0));
+    }
     for (int i = 0; i < count; i++) {
       Solver& s = solvers[i]->s;
-      // each patch its own branch: no dependencies between them
-      hipGraphNode_t node;
-      BD_HIP(hipGraphAddChildGraphNode(&node, b->graph, nullptr, 0, s.graph));
+      if (!fused) {
+        // each patch its own branch: no dependencies between them
+        hipGraphNode_t node;
+        BD_HIP(hipGraphAddChildGraphNode(&node, b->graph, nullptr, 0, s.graph));
+      }
       hipEvent_t e;
       BD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       b->ready.push_back(e);
       b->solvers.push_back(solvers[i]);
-      b->captured.push_back(s.graph);
+      b->captured.push_back(s.graph_serial);
+      b->gens.push_back(s.config_gen);
     }
-    BD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    if (!fused) BD_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   } catch (const std::exception& e) {
     g_create_error = e.what();
     batch_free(b);
@@ -1796,10 +1869,37 @@ int bd_batch_substep(bd_batch* b) {
   return batch_guarded(b, [&](bd_batch& B) {
     for (size_t i = 0; i < B.solvers.size(); i++) {
       const Solver& s = B.solvers[i]->s;
-      if (s.graph != B.captured[i] || s.graph_gen != s.config_gen) {
+      const bool stale = B.fused ? (s.config_gen != B.gens[i] || s.pcg_mode != BD_PCG_WORKGROUP)
+                                 : (!s.graph_exec || s.graph_serial != B.captured[i] || s.graph_gen != s.config_gen);
+      if (stale) {
         B.err = "bd_batch_substep: solver " + std::to_string(i) + " was captured again or changed since bd_batch_create";
         return BD_ERR_STATE;
       }
+    }
+    if (B.fused) {
+      // the substeps in lockstep on their own streams; each Newton iteration's solves in one launch
+      for (bd_solver* h : B.solvers) substep_begin(h->s);
+      for (int it = 0; it < B.newton; it++) {
+        for (size_t i = 0; i < B.solvers.size(); i++) {
+          Solver& s = B.solvers[i]->s;
+          newton_pre(s, it == 0 || !s.P.lag_coarse);
+          launch_pcg_setup(s);
+          BD_HIP(hipEventRecord(B.ready[i], s.stream));
+          BD_HIP(hipStreamWaitEvent(B.stream, B.ready[i], 0));
+        }
+        launch_pcg_wg_many(B.systems, (int)B.solvers.size(), B.lds, B.stream);
+        BD_HIP(hipGetLastError());
+        BD_HIP(hipEventRecord(B.done, B.stream));
+        for (bd_solver* h : B.solvers) {
+          BD_HIP(hipStreamWaitEvent(h->s.stream, B.done, 0));
+          newton_post(h->s, it);
+        }
+      }
+      for (bd_solver* h : B.solvers) {
+        substep_end(h->s);
+        h->s.stats.newton_iterations = B.newton;
+      }
+      return BD_OK;
     }
     // the launch follows each solver's queued work (grip goals, uploads), and each solver's later
     // work follows the launch
@@ -1820,6 +1920,7 @@ int bd_batch_substep(bd_batch* b) {
 int bd_batch_sync(bd_batch* b) {
   return batch_guarded(b, [&](bd_batch& B) {
     BD_HIP(hipStreamSynchronize(B.stream));
+    for (bd_solver* h : B.solvers) BD_HIP(hipStreamSynchronize(h->s.stream));
     return BD_OK;
   });
 }

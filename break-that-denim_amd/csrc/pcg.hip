@@ -15,6 +15,7 @@
 
 #include "bd_internal.h"
 #include "bounds.h"
+#include "mg_dev.h"
 #include "reduce.h"
 
 namespace bd {
@@ -156,39 +157,7 @@ __global__ __launch_bounds__(1024) void k_true_residual(PcgArgs A, const float* 
   double v[1] = {0};
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < A.N; n += gridDim.x * blockDim.x) {
     double y[3] = {0, 0, 0};
-    if (!A.fixed[n]) {
-      int l = geo_layer(geo, n);
-      int r0 = n - geo.off[l], nx = geo.nx[l], ny = geo.ny[l];
-      int j = r0 / nx, i = r0 - j * nx;
-      const double* d = A.delta;
-      auto acc = [&](int base, int row, int m) {
-        double s0 = op[(base + 0) * Np + row], s1 = op[(base + 1) * Np + row], s2 = op[(base + 2) * Np + row];
-        double s3 = op[(base + 3) * Np + row], s4 = op[(base + 4) * Np + row], s5 = op[(base + 5) * Np + row];
-        double x0 = d[m], x1 = d[Np + m], x2 = d[2 * Np + m];
-        y[0] += s0 * x0 + s1 * x1 + s2 * x2;
-        y[1] += s1 * x0 + s3 * x1 + s4 * x2;
-        y[2] += s2 * x0 + s4 * x1 + s5 * x2;
-      };
-      auto accs = [&](double s, int m) {
-        y[0] += s * d[m];
-        y[1] += s * d[Np + m];
-        y[2] += s * d[2 * Np + m];
-      };
-      acc(F_C, n, n);
-      const bool e = i + 1 < nx, w = i > 0, no = j + 1 < ny, so = j > 0;
-      if (e) acc(F_E, n, n + 1);
-      if (w) acc(F_E, n - 1, n - 1);
-      if (no) acc(F_N, n, n + nx);
-      if (so) acc(F_N, n - nx, n - nx);
-      if (e && no) acc(F_NE, n, n + nx + 1);
-      if (w && so) acc(F_NE, n - nx - 1, n - nx - 1);
-      if (w && no) acc(F_NW, n, n + nx - 1);
-      if (e && so) acc(F_NW, n - nx + 1, n - nx + 1);
-      if (i + 2 < nx) accs(op[F_EE * Np + n], n + 2);
-      if (i >= 2) accs(op[F_EE * Np + n - 2], n - 2);
-      if (j + 2 < ny) accs(op[F_NN * Np + n], n + 2 * nx);
-      if (j >= 2) accs(op[F_NN * Np + n - 2 * nx], n - 2 * nx);
-    }
+    if (!A.fixed[n]) fine_apply_f64(op, Np, geo, A.delta, n, y);
     for (int q = 0; q < 3; q++) {
       double r = A.fixed[n] ? 0.0 : (double)A.rhs[q * Np + n] - y[q];
       A.r[q * Np + n] = (float)r;
@@ -769,8 +738,22 @@ bool wait_flag(Solver& s, int seq) {
   return s.scal_h->done != 0;
 }
 
+// The solve's schedule into Scal for the one-workgroup path (the grid path sets its own below).
+void launch_pcg_setup(Solver& s) {
+  const bool tol_mode = s.P.pcg_rel_tol > 0;
+  const int K = tol_mode ? std::max(1, s.P.pcg_max_iterations) : std::max(0, s.P.pcg_iterations);
+  k_pcg_setup<<<1, 1, 0, s.stream>>>(s.scal, tol_mode ? s.P.pcg_rel_tol : 0.0, K, K, 0);
+}
+
 // One PCG solve on level 0 (newtonMultigrid.ts:537-576), leaving the direction in delta.
 void pcg_solve(Solver& s) {
+  if (s.pcg_mode == BD_PCG_WORKGROUP) {
+    const std::string why = wg_unsupported(s);
+    if (!why.empty()) throw std::runtime_error("no one-workgroup solve: " + why);
+    launch_pcg_setup(s);
+    launch_pcg_wg(s);
+    return;
+  }
   PcgArgs A = pcg_args(s);
   const bool tol_mode = s.P.pcg_rel_tol > 0;
   // captured into a graph: a fixed budget of iterations, the device's done flag gating those past

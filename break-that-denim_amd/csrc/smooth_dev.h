@@ -1,0 +1,528 @@
+// Device code of the line smoother shared by its kernels (smooth.hip) and the one-workgroup solver
+// (wgsolve.hip): the pass arguments, the branch-free row gather, the block parallel cyclic
+// reduction of one line and the line geometry. The algorithm is described in smooth.hip.
+#pragma once
+#include "bd_internal.h"
+#include "oprd.h"
+
+namespace bd {
+
+namespace {
+
+struct SmoothArgs {
+  const float* op;       // in the pass's layout (column-major for v-lines of a dual-layout level)
+  const uint8_t* fixed;  // same layout
+  int Np;
+  const int4* lines;     // (start, stride, length, family), start in the pass's layout
+  float* x;              // same layout
+  const float* b;        // same layout
+  float* xo;             // the other layout's copy of x (dual-layout levels), or null
+  const int* gate;
+  Geo geo;
+};
+
+// index strides of lattice offsets (di, dj) in the pass's layout
+template <int FAM, bool DUAL>
+__device__ __forceinline__ int stride_i(int nx, int ny) { return (FAM == 1 && DUAL) ? ny : 1; }
+template <int FAM, bool DUAL>
+__device__ __forceinline__ int stride_j(int nx, int ny) { return (FAM == 1 && DUAL) ? 1 : nx; }
+
+__device__ __forceinline__ void sym_full(const float* s, float* o) {
+  o[0] = s[0]; o[1] = s[1]; o[2] = s[2];
+  o[3] = s[1]; o[4] = s[3]; o[5] = s[4];
+  o[6] = s[2]; o[7] = s[4]; o[8] = s[5];
+}
+
+// inverse of the symmetric block shifted by PIVOT_SHIFT * max|diag| (newtonMultigrid.ts:414-423)
+__device__ __forceinline__ void pivot_inv(const float* s, float* o) {
+  float e = (float)kPivotShift * fmaxf(fabsf(s[0]), fmaxf(fabsf(s[3]), fabsf(s[5])));
+  float a00 = s[0] + e, a01 = s[1], a02 = s[2], a11 = s[3] + e, a12 = s[4], a22 = s[5] + e;
+  float c00 = a11 * a22 - a12 * a12, c01 = a12 * a02 - a01 * a22, c02 = a01 * a12 - a11 * a02;
+  // hardware reciprocal (1 ulp): the pivots of a preconditioner need no correctly rounded division
+  float id = __builtin_amdgcn_rcpf(a00 * c00 + a01 * c01 + a02 * c02);
+  o[0] = c00 * id;
+  o[1] = c01 * id;
+  o[2] = c02 * id;
+  o[3] = (a00 * a22 - a02 * a02) * id;
+  o[4] = (a02 * a01 - a00 * a12) * id;
+  o[5] = (a00 * a11 - a01 * a01) * id;
+}
+
+__device__ __forceinline__ void mm(const float* a, const float* b, float* o) {
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) o[r * 3 + c] = a[r * 3] * b[c] + a[r * 3 + 1] * b[3 + c] + a[r * 3 + 2] * b[6 + c];
+}
+__device__ __forceinline__ void mtm(const float* a, const float* b, float* o) {  // a^T b
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) o[r * 3 + c] = a[r] * b[c] + a[3 + r] * b[3 + c] + a[6 + r] * b[6 + c];
+}
+__device__ __forceinline__ void mmt(const float* a, const float* b, float* o) {  // a b^T
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) o[r * 3 + c] = a[r * 3] * b[c * 3] + a[r * 3 + 1] * b[c * 3 + 1] + a[r * 3 + 2] * b[c * 3 + 2];
+}
+__device__ __forceinline__ void mv(const float* a, const float* v, float* o) {
+  o[0] = a[0] * v[0] + a[1] * v[1] + a[2] * v[2];
+  o[1] = a[3] * v[0] + a[4] * v[1] + a[5] * v[2];
+  o[2] = a[6] * v[0] + a[7] * v[1] + a[8] * v[2];
+}
+__device__ __forceinline__ void mtv(const float* a, const float* v, float* o) {
+  o[0] = a[0] * v[0] + a[3] * v[1] + a[6] * v[2];
+  o[1] = a[1] * v[0] + a[4] * v[1] + a[7] * v[2];
+  o[2] = a[2] * v[0] + a[5] * v[1] + a[8] * v[2];
+}
+__device__ __forceinline__ void smv(const float* s, const float* v, float* o) {
+  o[0] = s[0] * v[0] + s[1] * v[1] + s[2] * v[2];
+  o[1] = s[1] * v[0] + s[3] * v[1] + s[4] * v[2];
+  o[2] = s[2] * v[0] + s[4] * v[1] + s[5] * v[2];
+}
+
+// one coupling of the row gather: source row and plane base of its symmetric block (or scalar)
+struct Cpl {
+  int di, dj;   // offset of the neighbour
+  int sdi, sdj; // offset of the row that stores the block (0,0: own forward planes)
+  int base;     // first plane
+  bool scalar;
+};
+
+// fine stencil: 4 forward blocks, their 4 mirrors, 2 forward scalars and their mirrors
+__device__ constexpr Cpl kFine[12] = {
+    {1, 0, 0, 0, F_E, false},   {-1, 0, -1, 0, F_E, false}, {0, 1, 0, 0, F_N, false},    {0, -1, 0, -1, F_N, false},
+    {1, 1, 0, 0, F_NE, false},  {-1, -1, -1, -1, F_NE, false}, {-1, 1, 0, 0, F_NW, false}, {1, -1, 1, -1, F_NW, false},
+    {2, 0, 0, 0, F_EE, true},   {-2, 0, -2, 0, F_EE, true}, {0, 2, 0, 0, F_NN, true},    {0, -2, 0, -2, F_NN, true}};
+
+// Gathers D (6), U (9: +1 along coupling, full), r (3) of row n on a line of family FAM.
+// All loads use clamped in-bounds indices and are masked afterwards, so they issue together.
+template <bool FINE, int FAM, bool DUAL, class RD>
+__device__ __forceinline__ void gather_row(const SmoothArgs& S, const RD& rd, int n, int nb, int i, int j, int nx, int ny,
+                                           float D[6], float U[9], float r[3], bool& fixed_n) {
+  const int Np = S.Np;
+  const float* x = S.x;
+  const int si = stride_i<FAM, DUAL>(nx, ny), sj = stride_j<FAM, DUAL>(nx, ny);
+  fixed_n = S.fixed[n] != 0;
+  float c[6];
+  rd.blk(F_C, n, c);
+  const float xn0 = x[n], xn1 = x[Np + n], xn2 = x[2 * Np + n];
+  r[0] = S.b[n];
+  r[1] = S.b[Np + n];
+  r[2] = S.b[2 * Np + n];
+#pragma unroll
+  for (int q = 0; q < 9; q++) U[q] = 0.f;
+  float maj = 0.f;
+  auto one = [&](int di, int dj, int sdi, int sdj, int base, bool scalar, bool neg_src) {
+    const bool in = i + di >= 0 && i + di < nx && j + dj >= 0 && j + dj < ny;
+    const int m = in ? n + dj * sj + di * si : n;
+    const int src = in ? n + sdj * sj + sdi * si : n;
+    float blk[6];
+    if (scalar) {
+      float s = rd.scalar(base, src);
+      blk[0] = s; blk[1] = 0.f; blk[2] = 0.f; blk[3] = s; blk[4] = 0.f; blk[5] = s;
+    } else {
+      rd.blk(base, src, blk);
+    }
+    const float xm[3] = {x[m], x[Np + m], x[2 * Np + m]};
+    // the fixed-neighbour mask is arithmetic: a branch on the loaded flag would split the gather
+    // into one memory round trip per coupling
+    const uint8_t fm = S.fixed[m];  // m is clamped in bounds: the load is unconditional
+    const float w = (in & (fm == 0)) ? 1.f : 0.f;
+    const int along = FAM == 0 ? di : dj;
+    const int across = FAM == 0 ? dj : di;
+    if (across == 0 && along == 1) {
+      float full[9];
+      sym_full(blk, full);
+#pragma unroll
+      for (int q = 0; q < 9; q++) U[q] = w * full[q];
+    } else if (!(across == 0 && along == -1)) {
+      float t[3];
+      smv(blk, xm, t);
+      r[0] -= w * t[0];
+      r[1] -= w * t[1];
+      r[2] -= w * t[2];
+      if (across == 0) maj += w * sym_frob(blk);  // +-2 along the line
+    }
+  };
+  if (FINE) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) one(kFine[k].di, kFine[k].dj, kFine[k].sdi, kFine[k].sdj, kFine[k].base, kFine[k].scalar, false);
+  } else {
+    // 24 couplings of 6-plane blocks: batches of 6 keep the loads in flight within the VGPR budget
+#pragma unroll 3
+    for (int k = 1; k < kCoarseBlocks; k++) {
+      one(coarse_di(k), coarse_dj(k), 0, 0, 6 * k, false, false);
+      one(-coarse_di(k), -coarse_dj(k), -coarse_di(k), -coarse_dj(k), 6 * k, false, true);
+    }
+  }
+  if (fixed_n) {
+    D[0] = 1.f; D[1] = 0.f; D[2] = 0.f; D[3] = 1.f; D[4] = 0.f; D[5] = 1.f;
+#pragma unroll
+    for (int q = 0; q < 9; q++) U[q] = 0.f;
+    r[0] = S.b[n];
+    r[1] = S.b[Np + n];
+    r[2] = S.b[2 * Np + n];
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < 6; q++) D[q] = c[q];
+  D[0] += maj;
+  D[3] += maj;
+  D[5] += maj;
+  r[0] += maj * xn0;
+  r[1] += maj * xn1;
+  r[2] += maj * xn2;
+}
+
+// Partial row gather of rows on one wave's 64 lanes for the couplings c with c % G == GI (GI is a
+// compile-time constant, so the part is straight-line code whose loads all issue together). GI 0
+// also loads the row's centre block, b, x and fixed flag. Writes the partial rhs correction and
+// majorisation to LDS; the part owning the +1 along-line coupling writes U. Lanes past the line's
+// end (valid false) gather a clamped row and write nothing.
+template <bool FINE, int FAM, int G, int GI, bool DUAL, class RD>
+__device__ __forceinline__ void gather_part(const SmoothArgs& S, const RD& rd, int n, int nb, int i, int j, int nx, int ny,
+                                            bool valid, int row, int len, float* pr, float* pm, float* gU, float* gD,
+                                            float* gR, float* gX, uint8_t* gF) {
+  const int Np = S.Np;
+  const float* x = S.x;
+  const int si = stride_i<FAM, DUAL>(nx, ny), sj = stride_j<FAM, DUAL>(nx, ny);
+  float r0 = 0.f, r1 = 0.f, r2 = 0.f, maj = 0.f;
+  float U[9];
+  bool ownsU = false;
+  auto one = [&](int di, int dj, int sdi, int sdj, int base, bool scalar) {
+    const bool in = i + di >= 0 && i + di < nx && j + dj >= 0 && j + dj < ny;
+    const int m = in ? n + dj * sj + di * si : n;
+    const int src = in ? n + sdj * sj + sdi * si : n;
+    float blk[6];
+    if (scalar) {
+      float s = rd.scalar(base, src);
+      blk[0] = s; blk[1] = 0.f; blk[2] = 0.f; blk[3] = s; blk[4] = 0.f; blk[5] = s;
+    } else {
+      rd.blk(base, src, blk);
+    }
+    const float xm[3] = {x[m], x[Np + m], x[2 * Np + m]};
+    const uint8_t fm = S.fixed[m];  // m is clamped in bounds: the load is unconditional
+    const float w = (in & (fm == 0)) ? 1.f : 0.f;
+    const int along = FAM == 0 ? di : dj;
+    const int across = FAM == 0 ? dj : di;
+    if (across == 0 && along == 1) {
+      float full[9];
+      sym_full(blk, full);
+#pragma unroll
+      for (int q = 0; q < 9; q++) U[q] = w * full[q];
+      ownsU = true;
+    } else if (!(across == 0 && along == -1)) {
+      float t[3];
+      smv(blk, xm, t);
+      r0 += w * t[0];
+      r1 += w * t[1];
+      r2 += w * t[2];
+      if (across == 0) maj += w * sym_frob(blk);
+    }
+  };
+  if (FINE) {
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+      if (k % G == GI) one(kFine[k].di, kFine[k].dj, kFine[k].sdi, kFine[k].sdj, kFine[k].base, kFine[k].scalar);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 24; k++) {
+      if (k % G != GI) continue;
+      const int b = 1 + (k >> 1);
+      if (k & 1)
+        one(-coarse_di(b), -coarse_dj(b), -coarse_di(b), -coarse_dj(b), 6 * b, false);
+      else
+        one(coarse_di(b), coarse_dj(b), 0, 0, 6 * b, false);
+    }
+  }
+  float c[6], bb[3], xx[3];
+  uint8_t f = 0;
+  if (GI == 0) {
+    rd.blk(F_C, n, c);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      bb[q] = S.b[q * Np + n];
+      xx[q] = x[q * Np + n];
+    }
+    f = S.fixed[n];
+  }
+  if (!valid) return;
+  pr[(0 * G + GI) * len + row] = r0;
+  pr[(1 * G + GI) * len + row] = r1;
+  pr[(2 * G + GI) * len + row] = r2;
+  pm[GI * len + row] = maj;
+  if (ownsU) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) gU[q * len + row] = U[q];
+  }
+  if (GI == 0) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) gD[q * len + row] = c[q];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      gR[q * len + row] = bb[q];
+      gX[q * len + row] = xx[q];
+    }
+    gF[row] = f;
+  }
+}
+
+// Runs part g (wave-uniform) of the gather.
+template <bool FINE, int FAM, int G, bool DUAL, class RD, int GI = 0>
+__device__ __forceinline__ void gather_dispatch(int g, const SmoothArgs& S, const RD& rd, int n, int nb, int i, int j, int nx,
+                                                int ny, bool valid, int row, int len, float* pr, float* pm, float* gU,
+                                                float* gD, float* gR, float* gX, uint8_t* gF) {
+  if constexpr (GI < G) {
+    if (g == GI)
+      gather_part<FINE, FAM, G, GI, DUAL>(S, rd, n, nb, i, j, nx, ny, valid, row, len, pr, pm, gU, gD, gR, gX, gF);
+    else
+      gather_dispatch<FINE, FAM, G, DUAL, RD, GI + 1>(g, S, rd, n, nb, i, j, nx, ny, valid, row, len, pr, pm, gU, gD, gR,
+                                                      gX, gF);
+  }
+}
+
+// LDS record of one reduced row in the line solve: D^-1 (6, symmetric), U (9), r (3) and 2 floats of
+// padding, so that a record is five 16-byte vector accesses with immediate offsets
+constexpr int kRowF = 20;
+// LDS floats per reduced row: two record buffers
+constexpr int kSolveFloats = 2 * kRowF;
+// variant code of the split smoother (row kernel + solve kernel)
+constexpr int kSplitVariant = 16;
+
+__device__ __forceinline__ void put_row(float* B, int t, const float Di[6], const float U[9], const float r[3]) {
+  float4* p = reinterpret_cast<float4*>(B + kRowF * t);
+  p[0] = make_float4(Di[0], Di[1], Di[2], Di[3]);
+  p[1] = make_float4(Di[4], Di[5], U[0], U[1]);
+  p[2] = make_float4(U[2], U[3], U[4], U[5]);
+  p[3] = make_float4(U[6], U[7], U[8], r[0]);
+  p[4] = make_float4(r[1], r[2], 0.f, 0.f);
+}
+
+__device__ __forceinline__ void get_row(const float* B, int j, float Di[6], float U[9], float r[3]) {
+  const float4* p = reinterpret_cast<const float4*>(B + kRowF * j);
+  const float4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+  Di[0] = a.x; Di[1] = a.y; Di[2] = a.z; Di[3] = a.w; Di[4] = b.x; Di[5] = b.y;
+  U[0] = b.z; U[1] = b.w; U[2] = c.x; U[3] = c.y; U[4] = c.z; U[5] = c.w; U[6] = d.x; U[7] = d.y; U[8] = d.z;
+  r[0] = d.w; r[1] = e.x; r[2] = e.y;
+}
+
+// Block tridiagonal solve of one line, two rows per thread.
+// Thread t holds rows a = 2t and b = 2t+1 (D sym, U full coupling to the next row, r) and
+// receives their solutions. The first odd-even level eliminates row b in registers (neighbours
+// exchange one 9-float term through LDS), leaving R2 = ceil(len/2) reduced rows, one per thread.
+// Those are solved by block parallel cyclic reduction: at level d every row eliminates both its
+// neighbours at distance d at once, so after ceil(log2 R2) levels (one barrier each, the row kept
+// in registers, LDS ping-ponged) each row stands alone; there is no back-substitution sweep. The
+// reduced system stays symmetric (row k's coupling to k - d is row k - d's coupling to k,
+// transposed), so a row carries only its forward coupling, and each row publishes its own shifted
+// pivot inverse with it (one inversion per row and level). cr: kSolveFloats * Lr floats of LDS.
+__device__ __forceinline__ void line_solve(float* cr, int Lr, int t, int len, const float* Da_in, const float* Ua,
+                                           const float* ra_in, const float* Db, const float* Ub, const float* rb,
+                                           float xa[3], float xb[3]) {
+  float* B0 = cr;
+  float* B1 = cr + kRowF * Lr;  // also carries the register level's exchange (its first 9 floats)
+  const int R2 = (len + 1) >> 1;
+  const bool has_a = 2 * t < len, has_b = 2 * t + 1 < len;
+  float D[6], r[3], U[9];
+#pragma unroll
+  for (int q = 0; q < 6; q++) D[q] = Da_in[q];
+#pragma unroll
+  for (int q = 0; q < 3; q++) r[q] = ra_in[q];
+
+  // register level: eliminate row b (U_a couples a -> b, U_b couples b -> 2t+2)
+  float Dib[6], Dibf[9], yb[3];
+  pivot_inv(Db, Dib);
+  sym_full(Dib, Dibf);
+  smv(Dib, rb, yb);
+  {
+    float T[9], P[9], p[3];
+    mm(Dibf, Ub, T);
+    mtm(Ub, T, P);
+    mtv(Ub, yb, p);
+    if (has_b && t + 1 < R2) {
+      float4* e = reinterpret_cast<float4*>(B1 + kRowF * t);
+      e[0] = make_float4(P[0], 0.5f * (P[1] + P[3]), 0.5f * (P[2] + P[6]), P[4]);
+      e[1] = make_float4(0.5f * (P[5] + P[7]), P[8], p[0], p[1]);
+      e[2] = make_float4(p[2], 0.f, 0.f, 0.f);
+    }
+  }
+  {
+    float T2[9], W[9], t3[3];
+    mm(Ua, Dibf, T2);
+    mmt(T2, Ua, W);
+    D[0] -= W[0];
+    D[1] -= 0.5f * (W[1] + W[3]);
+    D[2] -= 0.5f * (W[2] + W[6]);
+    D[3] -= W[4];
+    D[4] -= 0.5f * (W[5] + W[7]);
+    D[5] -= W[8];
+    mv(T2, rb, t3);
+    r[0] -= t3[0];
+    r[1] -= t3[1];
+    r[2] -= t3[2];
+    mm(T2, Ub, W);
+#pragma unroll
+    for (int q = 0; q < 9; q++) U[q] = -W[q];
+  }
+  __syncthreads();
+  if (has_a && t >= 1) {
+    const float4* e = reinterpret_cast<const float4*>(B1 + kRowF * (t - 1));
+    const float4 e0 = e[0], e1 = e[1], e2 = e[2];
+    D[0] -= e0.x; D[1] -= e0.y; D[2] -= e0.z; D[3] -= e0.w; D[4] -= e1.x; D[5] -= e1.y;
+    r[0] -= e1.z; r[1] -= e1.w; r[2] -= e2.x;
+  }
+  // B1's exchange terms are read before the first level's barrier; that level writes B0 and the
+  // next one B1, so one barrier per level orders every read before the overwrite.
+  const bool row = t < R2;
+  int lvl = 0;
+  for (int d = 1; d < R2; d <<= 1, lvl ^= 1) {
+    float* B = lvl ? B1 : B0;
+    if (row) {
+      float Di[6];
+      pivot_inv(D, Di);
+      put_row(B, t, Di, U, r);
+    }
+    __syncthreads();
+    if (row) {
+      float Un[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) Un[q] = 0.f;
+      const int j1 = t - d, j2 = t + d;
+      if (j1 >= 0) {
+        // row j1 couples to this row by U1: D -= U1^T D1^-1 U1, r -= U1^T D1^-1 r1
+        float Di[6], U1[9], r1[3], T[9];
+        get_row(B, j1, Di, U1, r1);
+        float Dif[9];
+        sym_full(Di, Dif);
+        mm(Dif, U1, T);  // T = D1^-1 U1
+        D[0] -= U1[0] * T[0] + U1[3] * T[3] + U1[6] * T[6];
+        D[1] -= U1[0] * T[1] + U1[3] * T[4] + U1[6] * T[7];
+        D[2] -= U1[0] * T[2] + U1[3] * T[5] + U1[6] * T[8];
+        D[3] -= U1[1] * T[1] + U1[4] * T[4] + U1[7] * T[7];
+        D[4] -= U1[1] * T[2] + U1[4] * T[5] + U1[7] * T[8];
+        D[5] -= U1[2] * T[2] + U1[5] * T[5] + U1[8] * T[8];
+        float tt[3];
+        mtv(T, r1, tt);  // U1^T D1^-1 r1
+        r[0] -= tt[0];
+        r[1] -= tt[1];
+        r[2] -= tt[2];
+      }
+      if (j2 < R2) {
+        // this row couples to row j2 by U, row j2 on to j2 + d by U2: D -= U D2^-1 U^T,
+        // r -= U D2^-1 r2, and the new coupling to j2 + d is -U D2^-1 U2
+        float Di[6], U2[9], r2[3], T2[9];
+        get_row(B, j2, Di, U2, r2);
+        float Dif[9];
+        sym_full(Di, Dif);
+        mm(U, Dif, T2);  // T2 = U D2^-1
+        D[0] -= T2[0] * U[0] + T2[1] * U[1] + T2[2] * U[2];
+        D[1] -= T2[0] * U[3] + T2[1] * U[4] + T2[2] * U[5];
+        D[2] -= T2[0] * U[6] + T2[1] * U[7] + T2[2] * U[8];
+        D[3] -= T2[3] * U[3] + T2[4] * U[4] + T2[5] * U[5];
+        D[4] -= T2[3] * U[6] + T2[4] * U[7] + T2[5] * U[8];
+        D[5] -= T2[6] * U[6] + T2[7] * U[7] + T2[8] * U[8];
+        float tt[3];
+        mv(T2, r2, tt);
+        r[0] -= tt[0];
+        r[1] -= tt[1];
+        r[2] -= tt[2];
+        float W[9];
+        mm(T2, U2, W);
+#pragma unroll
+        for (int q = 0; q < 9; q++) Un[q] = -W[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 9; q++) U[q] = Un[q];
+    }
+  }
+  // every reduced row now stands alone
+  float* B = lvl ? B1 : B0;
+  xa[0] = xa[1] = xa[2] = 0.f;
+  if (row) {
+    float inv[6];
+    pivot_inv(D, inv);
+    smv(inv, r, xa);
+    B[kRowF * t] = xa[0];
+    B[kRowF * t + 1] = xa[1];
+    B[kRowF * t + 2] = xa[2];
+  }
+  __syncthreads();
+  float xn[3] = {0.f, 0.f, 0.f};
+  if (t + 1 < R2) {
+    xn[0] = B[kRowF * (t + 1)];
+    xn[1] = B[kRowF * (t + 1) + 1];
+    xn[2] = B[kRowF * (t + 1) + 2];
+  }
+  float tt[3], q1[3];
+  mtv(Ua, xa, tt);
+  mv(Ub, xn, q1);
+  float rr[3] = {rb[0] - tt[0] - q1[0], rb[1] - tt[1] - q1[1], rb[2] - tt[2] - q1[2]};
+  smv(Dib, rr, xb);
+}
+
+__device__ __forceinline__ void set_identity(float D[6], float U[9], float r[3]) {
+  D[0] = 1.f; D[1] = 0.f; D[2] = 0.f; D[3] = 1.f; D[4] = 0.f; D[5] = 1.f;
+#pragma unroll
+  for (int q = 0; q < 9; q++) U[q] = 0.f;
+  r[0] = r[1] = r[2] = 0.f;
+}
+
+struct LineGeo {
+  int4 ln;
+  int nx, ny, off, i0, j0;
+};
+
+template <int FAM, bool DUAL>
+__device__ __forceinline__ LineGeo line_geo_at(const SmoothArgs& S, int idx) {
+  LineGeo g;
+  g.ln = S.lines[idx];
+  int l = geo_layer(S.geo, g.ln.x);
+  g.nx = S.geo.nx[l];
+  g.ny = S.geo.ny[l];
+  g.off = S.geo.off[l];
+  int r0 = g.ln.x - g.off;
+  if (FAM == 1 && DUAL) {
+    g.i0 = r0 / g.ny;
+    g.j0 = r0 - g.i0 * g.ny;
+  } else {
+    g.j0 = r0 / g.nx;
+    g.i0 = r0 - g.j0 * g.nx;
+  }
+  return g;
+}
+
+// the line of this workgroup (blocks spread over the XCDs)
+template <int FAM, bool DUAL>
+__device__ __forceinline__ LineGeo line_geo(const SmoothArgs& S) {
+  return line_geo_at<FAM, DUAL>(S, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// row k of the line: its index in the pass's layout, in the row-major layout (b), and in the other
+// layout (the second copy of x on dual-layout levels)
+template <int FAM, bool DUAL>
+__device__ __forceinline__ void row_index(const LineGeo& L, int k, int& n, int& nb, int& no, int& i, int& j) {
+  n = L.ln.x + k * L.ln.y;
+  i = FAM == 0 ? L.i0 + k : L.i0;
+  j = FAM == 0 ? L.j0 : L.j0 + k;
+  const int rm = L.off + j * L.nx + i, cm = L.off + i * L.ny + j;
+  nb = (FAM == 1 && DUAL) ? rm : n;
+  no = DUAL ? (FAM == 1 ? rm : cm) : n;
+}
+
+__device__ __forceinline__ void store_x(const SmoothArgs& S, int n, int no, const float v[3], bool dual) {
+  const int Np = S.Np;
+  S.x[n] = v[0];
+  S.x[Np + n] = v[1];
+  S.x[2 * Np + n] = v[2];
+  if (dual) {
+    S.xo[no] = v[0];
+    S.xo[Np + no] = v[1];
+    S.xo[2 * Np + no] = v[2];
+  }
+}
+
+}  // namespace
+
+}  // namespace bd

@@ -251,6 +251,17 @@ int bd_set_pcg(bd_solver* s, int iterations, double rel_tol, int max_iterations)
 /* encodeSubstep(encoder, minIterations, maxIterations) (gpu/solver.ts:737-760). */
 int bd_substep(bd_solver* s, int min_iterations, int max_iterations);
 
+/* Where a Newton system's PCG runs. BD_PCG_GRID (the default): kernels across the whole chip, the
+ * host checking convergence per iteration. BD_PCG_WORKGROUP: the whole solve (every iteration's
+ * matvec, updates, FP64 true residuals and V-cycle) in one workgroup of one launch, for systems small
+ * enough that the grid path is bound by launch latency (a 64 x 64 sheet, the patches of a batch):
+ * the same preconditioned CG, with its FP64 sums in another order. It needs one layer, FP32
+ * operators, lines of at most 512 nodes, no detached pieces and the learned path off
+ * (BD_ERR_STATE with the reason otherwise). No counterpart in the reference. */
+#define BD_PCG_GRID 0
+#define BD_PCG_WORKGROUP 1
+int bd_set_pcg_mode(bd_solver* s, int mode);
+
 /* A substep recorded once and replayed: the reference records a whole substep into one compute pass
  * with its optional Newton iterations gated on the device (gpu/solver.ts:737-760, gate.wgsl:12-23).
  * bd_graph_capture records one bd_substep (the solver's Newton iterations, no optional ones) into a
@@ -268,10 +279,15 @@ int bd_graph_substep(bd_solver* s);
 int bd_graph_release(bd_solver* s);
 
 /* Batched independent solves (uneven patches, each its own Newton / MG-PCG system with its own
- * scalars and convergence): the captured substeps of `count` solvers as the branches of one graph,
- * one launch per frame on the batch's stream. The launch waits for each solver's queued work (grip
- * goals, uploads) and each solver's later calls wait for it. Every solver needs bd_graph_capture
- * first, and capturing one again or changing it (as above) makes bd_batch_substep return
+ * scalars and convergence), advanced together by bd_batch_substep on the batch's stream; the batch
+ * waits for each solver's queued work (grip goals, uploads) and each solver's later calls wait for
+ * it. Two forms, chosen at creation:
+ *  - every solver in BD_PCG_WORKGROUP mode (and without optional Newton iterations): the substeps run
+ *    in lockstep and each Newton iteration's PCG solves of all solvers are ONE launch, a workgroup
+ *    per system, each iterating to its own tolerance;
+ *  - otherwise every solver needs bd_graph_capture first: the captured substeps are the branches of
+ *    one graph, one launch per frame.
+ * Changing a solver's configuration (as above) or capturing it again makes bd_batch_substep return
  * BD_ERR_STATE. No counterpart in the reference, which runs one scene per solver. */
 typedef struct bd_batch bd_batch;
 int bd_batch_create(bd_solver* const* solvers, int count, bd_batch** out);
