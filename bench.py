@@ -249,33 +249,37 @@ def graph_budget(iters_per_newton, limit):
 
 
 def graph_bench(args):
-    """Launch-bound workloads, eager against graph-captured substeps on the same frames: cfg 1's 64 x 64
-    sheet (one solver each way) or 48 uneven patches (each its own Newton / MG-PCG system; eager: one
-    substep per patch in turn, each with its host convergence checks; batched: one graph launch per
-    frame for all of them). The graph's PCG budget per Newton solve comes from the eager warm-up."""
+    """Launch-bound workloads, several ways on the same frames: cfg 1's 64 x 64 sheet (one solver per
+    method) or 48 uneven patches (each its own Newton / MG-PCG system). Methods:
+      grid            PCG kernels over the chip, the host checking convergence each iteration; the
+                      patches one substep after another;
+      grid+graph      the substep captured in a graph (the PCG budget per Newton solve from the warm-up:
+                      iterations past convergence are launched and return on the device's flag); the
+                      patches as the branches of one graph launch;
+      workgroup       the whole PCG in one workgroup (BD_PCG_WORKGROUP); the patches in lockstep with
+                      each Newton iteration's 48 solves in one launch (fused batch);
+      workgroup+graph (the sheet) the one-workgroup substep captured in a graph."""
     import numpy as np
+    from bd.capi import BD_PCG_WORKGROUP
     from bd.configs import advance_grips, config1, config_patches
     from bd.solver import Batch, GpuSolver
-    if args.workload == "sheet64":
+    single = args.workload == "sheet64"
+    if single:
         cfgs = [config1()]
         name = "sheet64 (cfg 1: 64x64 sheet, 3 Newton, 3-level MG-PCG to rel-res 1e-5, one corner grip at 2 mm/step)"
+        methods = ("grid", "grid+graph", "workgroup", "workgroup+graph")
     else:
-        cp = config_patches()
-        cfgs = cp["patches"]
+        cfgs = config_patches()["patches"]
         name = (f"patches48 ({len(cfgs)} uneven patches, sides {min(p['side'] for p in cfgs)}-{max(p['side'] for p in cfgs)},"
                 f" {sum(p['side'] ** 2 for p in cfgs)} nodes, each 3 Newton x MG-PCG to rel-res 1e-6, random grip paths)")
-    def make():
-        out = []
-        for c in cfgs:
-            grips = [dict(g) for g in c["grips"]]
-            out.append((GpuSolver(c["scene"], c["params"], grips), grips, c["velocity"]))
-        return out
+        methods = ("grid", "grid+graph", "workgroup")
 
     def frame(group, step):
         for g, grips, vel in group:
             advance_grips(grips, vel)
             g.set_grip_goals(np.array([q["goal"] for q in grips]))
-            step(g)
+            if step:
+                step(g)
 
     def collect(group):
         its, rel = 0, 0.0
@@ -286,53 +290,62 @@ def graph_bench(args):
             rel = max(rel, max(st["rel_residual"][:n]))
         return its, rel
 
-    results = {}
-    eager, graph = make(), make()
-    budget = []
-    for _ in range(args.warmup):
-        frame(eager, lambda g: g.encode_substep())
-        frame(graph, lambda g: g.encode_substep())
-        for g, _, _ in graph:
-            st = g.stats()
-            budget.append(max(st["pcg_iterations"][:st["newton_iterations"]]))
-    per = [0] * len(graph)
-    for r in range(args.warmup):
-        for i in range(len(graph)):
-            per[i] = max(per[i], budget[r * len(graph) + i])
-    for (g, _, _), c, m in zip(graph, cfgs, per):
-        g.graph_capture(graph_budget(m, c["params"].pcg_max_iterations))
-    batch = Batch([g for g, _, _ in graph]) if len(graph) > 1 else None
-    for label, group, step in (("eager", eager, lambda g: g.encode_substep()),
-                               ("graph", graph, (lambda g: None) if batch else (lambda g: g.graph_substep()))):
-        its, rel = 0, 0.0
+    results, budgets = {}, None
+    for m in methods:
+        group = []
+        for c in cfgs:
+            grips = [dict(g) for g in c["grips"]]
+            g = GpuSolver(c["scene"], c["params"], grips)
+            if m.startswith("workgroup"):
+                g.set_pcg_mode(BD_PCG_WORKGROUP)
+            group.append((g, grips, c["velocity"]))
+        most = [0] * len(group)
+        for _ in range(args.warmup):
+            frame(group, lambda g: g.encode_substep())
+            for i, (g, _, _) in enumerate(group):
+                st = g.stats()
+                most[i] = max(most[i], max(st["pcg_iterations"][:st["newton_iterations"]]))
+        batch, step = None, (lambda g: g.encode_substep())
+        if m.endswith("graph"):
+            caps = [graph_budget(k, c["params"].pcg_max_iterations) for k, c in zip(most, cfgs)]
+            if m == "grid+graph":
+                budgets = caps
+            for (g, _, _), cap in zip(group, caps):
+                g.graph_capture(cap)
+            step = (lambda g: g.graph_substep()) if single else None
+        if not single and m != "grid":
+            batch = Batch([g for g, _, _ in group])
+            step = None
         for g, _, _ in group:
             g.sync()
+        its, rel = 0, 0.0
         t0 = time.perf_counter()
         for _ in range(args.steps):
             frame(group, step)
-            if label == "graph" and batch:
+            if batch:
                 batch.substep()
             i, r = collect(group)
             its += i
             rel = max(rel, r)
         for g, _, _ in group:
             g.sync()
-        results[label] = {"ms_per_frame": round(1e3 * (time.perf_counter() - t0) / args.steps, 3),
-                          "pcg_iterations_per_frame": round(its / args.steps, 1), "max_rel_residual": rel}
+        results[m] = {"ms_per_frame": round(1e3 * (time.perf_counter() - t0) / args.steps, 3),
+                      "pcg_iterations_per_frame": round(its / args.steps, 1), "max_rel_residual": rel}
+        if batch:
+            batch.close()
+        for g, _, _ in group:
+            g.close()
     tol = cfgs[0]["params"].pcg_rel_tol
-    if batch:
-        batch.close()
-    for g, _, _ in eager + graph:
-        g.close()
-    ms = results["graph"]["ms_per_frame"]
+    ms = results["workgroup"]["ms_per_frame"]
     print(json.dumps({
-        "metric": "ms/frame, graph-captured substeps against eager ones", "value": ms, "unit": "ms/frame",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (FP64 dots, residual replacement, line search)",
-        "data": "synthetic", "config": {"workload": name, "pcg_budget_per_newton": [graph_budget(m, c["params"].pcg_max_iterations)
-                                                                                 for m, c in zip(per, cfgs)]},
-        "methods": results, "speedup_eager_over_graph": round(results["eager"]["ms_per_frame"] / ms, 3),
-        "converged": all(v["max_rel_residual"] <= 1.05 * tol for v in results.values()),
+        "metric": "ms/frame of launch-bound solves: one-workgroup PCG and graphs against the grid path",
+        "value": ms, "unit": "ms/frame", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (FP64 dots, residual replacement, line search)", "data": "synthetic",
+        "config": {"workload": name, "method": "workgroup", "graph_pcg_budget_per_newton": budgets},
+        "methods": results,
+        "speedup_grid_over_workgroup": round(results["grid"]["ms_per_frame"] / ms, 3),
+        "converged": {k: v["max_rel_residual"] <= 1.05 * tol for k, v in results.items()},
         "roofline": None, "cpu_baseline": None}), flush=True)
 
 

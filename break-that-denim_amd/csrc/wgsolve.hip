@@ -83,7 +83,18 @@ __device__ void wg_colour(const WgLevel& L, const SmoothArgs& S, int c, int ncol
   const int R2 = (len + 1) >> 1;
   const int per = kWgSlots / R2;
   const OpF32 rd{L.op, L.Np};
-  const int li = threadIdx.x / R2, t = threadIdx.x - li * R2;
+  // u-lines: a line's rows on consecutive threads (rows 2t, 2t + 1 sit next to each other in memory);
+  // v-lines: consecutive threads on consecutive lines of the colour (row t of neighbouring lines is
+  // ncol nodes apart, where a thread walking down one v-line would stride nx nodes)
+  const int tid = threadIdx.x;
+  int li, t;
+  if (FAM == 0 || tid >= per * R2) {
+    li = tid / R2;
+    t = tid - li * R2;
+  } else {
+    t = tid / per;
+    li = tid - t * per;
+  }
   for (int base = 0; base < count; base += per) {
     const int line = base + li;
     const bool active = li < per && line < count;
