@@ -17,6 +17,8 @@
 #pragma clang fp contract(off)
 #include <stdexcept>
 
+#include <climits>
+
 #include "bd_internal.h"
 #include "bounds.h"
 
@@ -350,128 +352,310 @@ __global__ __launch_bounds__(256) void k_damage_row(CtArgs A) {
   A.marks[n] = mark;
 }
 
-__global__ __launch_bounds__(256) void k_detect(CtArgs A) {
-  uint32_t n = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (n >= (uint32_t)A.N) return;
-  const int Np = A.Np;
-  for (int k = 0; k < kContactK; k++) A.pairs[k * Np + n] = kNone;
-  if (A.pinned[n]) {
-    A.cmeta[2 * n] = make_float4(0.f, 0.f, 0.f, kUnbounded);
-    return;
-  }
-  V3 p = cur(A, n), p0 = x0(A, n);
-  V3 path = p - p0;
-  bool moving = dot(path, path) > kHardCore * kHardCore;
+// One node's detection (selfcontact.wgsl:317-457): the start, every record of the scanned cells in
+// order, and the end (K candidates, the bound, the reciprocal keys of the bounding face's corners).
+struct Detect {
+  uint32_t n;
+  bool live;  // not pinned: scans and takes a bound
+  V3 p, p0, path, mid;
+  float half;  // half the path's length
+  bool moving, damaged;
+  bool one_layer;  // every face is in the node's layer: its lattice position follows from its id
+  float inv_nx;
+  int lox, loy, loz, ncx, ncy, ncz;
+  uint32_t nl;
+  int ni, nj, off, nx;
   uint32_t best[kContactK];
   float bestD[kContactK];
-#pragma unroll
-  for (int k = 0; k < kContactK; k++) {
-    best[k] = kNone;
-    bestD[k] = 1e9f;
-  }
-  float d0min = 1e9f;
-  V3 n0 = {0.f, 0.f, 0.f};
-  uint32_t f0 = kNone;
-  float tc = 2.f;
-  uint32_t fc = kNone;
-  V3 lo3 = vmin(p, p0), hi3 = vmax(p, p0);
-  // the scanned box: lo .. min(hi, lo + 3) per axis, as counts so no cell index can overflow
-  int lox = (int)floorf(lo3.x / kCell - 0.5f), loy = (int)floorf(lo3.y / kCell - 0.5f), loz = (int)floorf(lo3.z / kCell - 0.5f);
-  int ncx = (int)min((long long)(int)floorf(hi3.x / kCell + 0.5f) - lox, 3LL);
-  int ncy = (int)min((long long)(int)floorf(hi3.y / kCell + 0.5f) - loy, 3LL);
-  int ncz = (int)min((long long)(int)floorf(hi3.z / kCell + 0.5f) - loz, 3LL);
-  uint32_t q = A.ijl[n], nl = q >> 24;
-  int ni = (int)(q & 0xFFFu), nj = (int)((q >> 12) & 0xFFFu);
-  int off = A.geo.off[nl], nx = A.geo.nx[nl], ny = A.geo.ny[nl];
-  bool damaged = false;
-  for (int b = max(nj - kWindow, 0); b <= min(nj + kWindow, ny - 1); b++)
-    if (A.marks[lattice_node(off, nx, ni, b)] != 0u) damaged = true;
-  const float startCap = 0.5f * kCell;
-  for (int dz = 0; dz <= ncz; dz++)
-    for (int dy = 0; dy <= ncy; dy++)
-      for (int dx_ = 0; dx_ <= ncx; dx_++) {
-        uint32_t h = hash_cell((int)((uint32_t)lox + (uint32_t)dx_), (int)((uint32_t)loy + (uint32_t)dy),
-                               (int)((uint32_t)loz + (uint32_t)dz));
-        uint32_t start = A.cells[h], count = A.cells[kTable + h];
-        for (uint32_t r = 0; r < count; r++) {
-          float4 i1 = A.items[2 * (start + r)];
-          float4 i0 = A.items[2 * (start + r) + 1];
-          V3 c1 = {i1.x, i1.y, i1.z}, c0 = {i0.x, i0.y, i0.z};
-          bool nearNow = len(p - c1) - i1.w < kReach;
-          bool nearStart = len(p0 - c0) - i1.w < fminf(d0min, startCap);
-          bool onPath = moving && fminf(seg_dist(c0, p0, p), seg_dist(c1, p0, p)) < i1.w;
-          if (!nearNow && !nearStart && !onPath) continue;
-          uint32_t f = __float_as_uint(i0.w);
-          uint32_t fq = A.ijl[f];
-          int di = (int)(fq & 0xFFFu) - ni, dj = (int)((fq >> 12) & 0xFFFu) - nj;
-          if ((fq >> 24) == nl && di >= -3 && di <= 2 && dj >= -3 && dj <= 2 &&
-              (!damaged || lattice_linked(A, off, nx, ni, nj, ni + di, nj + dj)))
-            continue;
-          Corners cf = corners(A, f);
-          if (nearStart) {
-            FaceHit s0 = face_hit_at(A, p0, cf);
-            if ((s0.d < d0min || (s0.d == d0min && f < f0)) && s0.d > 1e-9f) {
-              d0min = s0.d;
-              n0 = divs(p0 - s0.q, s0.d);
-              f0 = f;
-            }
-          }
-          if (onPath) {
-            float t = path_crossing(A, p0, path, cf);
-            if (t < tc || (t == tc && t <= 1.f && f < fc)) {
-              tc = t;
-              fc = f;
-            }
-          }
-          if (!nearNow) continue;
-          FaceHit hit = face_hit(A, p, cf);
-          if (hit.d >= kReach) continue;
-          uint32_t cand = f;
-          float cd = hit.d;
-#pragma unroll
-          for (int k = 0; k < kContactK; k++) {
-            if (cand == best[k]) break;
-            if (cd < bestD[k] || (cd == bestD[k] && cand < best[k])) {
-              uint32_t ts = best[k];
-              float td = bestD[k];
-              best[k] = cand;
-              bestD[k] = cd;
-              cand = ts;
-              cd = td;
-            }
-          }
-        }
-      }
-  for (int k = 0; k < kContactK; k++) A.pairs[k * Np + n] = best[k];
-  // the bound: the plane of the first face the path crosses, else the nearest face at the start
-  V3 nb = n0;
-  float db = d0min;
-  uint32_t fb = f0;
-  if (fc != kNone && fc != f0) {
-    Corners c = corners(A, fc);
-    V3 a = x0(A, c.c[0]);
-    V3 nrm = cross(x0(A, c.c[1]) - a, x0(A, c.c[2]) - a);
-    float ln = len(nrm);
-    if (ln > 1e-12f) {
-      nrm = divs(nrm, ln);
-      float side = dot(p0 - a, nrm);
-      nb = side >= 0.f ? nrm : V3{-nrm.x, -nrm.y, -nrm.z};
-      db = fabsf(side);
-      fb = fc;
-    }
-  }
-  if (fb == kNone || (fb == f0 && db >= startCap)) {
+  float d0min;
+  V3 n0;
+  uint32_t f0;
+  float tc;
+  uint32_t fc;
+};
+
+constexpr float kStartCap = 0.5f * kCell;
+
+__device__ void detect_begin(const CtArgs& A, uint32_t n, Detect& D) {
+  D.n = n;
+  const int Np = A.Np;
+  for (int k = 0; k < kContactK; k++) A.pairs[k * Np + n] = kNone;
+  D.live = !A.pinned[n];
+  if (!D.live) {
     A.cmeta[2 * n] = make_float4(0.f, 0.f, 0.f, kUnbounded);
     return;
   }
-  float share = kBoundFraction * fmaxf(db - kHardCore, 0.f);
+  D.p = cur(A, n);
+  D.p0 = x0(A, n);
+  D.path = D.p - D.p0;
+  D.moving = dot(D.path, D.path) > kHardCore * kHardCore;
+  D.one_layer = A.geo.nl == 1;
+  D.mid = 0.5f * (D.p0 + D.p);
+  D.half = 0.5f * len(D.path);
+#pragma unroll
+  for (int k = 0; k < kContactK; k++) {
+    D.best[k] = kNone;
+    D.bestD[k] = 1e9f;
+  }
+  D.d0min = 1e9f;
+  D.n0 = {0.f, 0.f, 0.f};
+  D.f0 = kNone;
+  D.tc = 2.f;
+  D.fc = kNone;
+  const V3 lo3 = vmin(D.p, D.p0), hi3 = vmax(D.p, D.p0);
+  // the scanned box: lo .. min(hi, lo + 3) per axis, as counts so no cell index can overflow
+  D.lox = (int)floorf(lo3.x / kCell - 0.5f);
+  D.loy = (int)floorf(lo3.y / kCell - 0.5f);
+  D.loz = (int)floorf(lo3.z / kCell - 0.5f);
+  D.ncx = (int)min((long long)(int)floorf(hi3.x / kCell + 0.5f) - D.lox, 3LL);
+  D.ncy = (int)min((long long)(int)floorf(hi3.y / kCell + 0.5f) - D.loy, 3LL);
+  D.ncz = (int)min((long long)(int)floorf(hi3.z / kCell + 0.5f) - D.loz, 3LL);
+  const uint32_t q = A.ijl[n];
+  D.nl = q >> 24;
+  D.ni = (int)(q & 0xFFFu);
+  D.nj = (int)((q >> 12) & 0xFFFu);
+  D.off = A.geo.off[D.nl];
+  D.nx = A.geo.nx[D.nl];
+  D.inv_nx = 1.f / (float)D.nx;
+  const int ny = A.geo.ny[D.nl];
+  D.damaged = false;
+  for (int b = max(D.nj - kWindow, 0); b <= min(D.nj + kWindow, ny - 1); b++)
+    if (A.marks[lattice_node(D.off, D.nx, D.ni, b)] != 0u) D.damaged = true;
+}
+
+// one face record (i1: centroid now and bounding radius; i0: centroid at start and the face id)
+// Pre-filter: most records of the scanned cells lie far from the node, and the tests of
+// detect_record take square roots and divisions. False only where each of those tests must fail:
+//  - near now / near at start: the squared distance is the very dot product that len() roots, and
+//    it exceeds the squared threshold by a relative 1e-5, far beyond the rounding of the root, the
+//    subtraction and the comparison (d0min only decreases, so a filter taken earlier in the node's
+//    sequence is at least as permissive);
+//  - on the path: the distance to the segment is at least the distance to its midpoint minus half
+//    its length, here beyond the radius by 10 um, far beyond the rounding of either form;
+//  - in a one-layer scene, a face of the node's exclusion window with no broken span nearby (the
+//    face's lattice position is its id's, which is what the ijl record holds).
+__device__ __forceinline__ bool detect_maybe(const Detect& D, float4 i1, float4 i0) {
+  if (D.one_layer && !D.damaged) {
+    // a face in the node's exclusion window is skipped by detect_record whatever its distance
+    // row and column of the face by a float reciprocal, corrected by one either way (exact for the
+    // 2^24 nodes self-contact allows)
+    const int r = (int)__float_as_uint(i0.w) - D.off;
+    int fj = (int)((float)r * D.inv_nx);
+    fj -= fj * D.nx > r ? 1 : 0;
+    fj += (fj + 1) * D.nx <= r ? 1 : 0;
+    const int fi = r - fj * D.nx;
+    const int di = fi - D.ni, dj = fj - D.nj;
+    if (di >= -3 && di <= 2 && dj >= -3 && dj <= 2) return false;
+  }
+  const V3 c1 = {i1.x, i1.y, i1.z}, c0 = {i0.x, i0.y, i0.z};
+  const V3 u1 = D.p - c1, u0 = D.p0 - c0;
+  const float a1 = kReach + i1.w, a0 = fminf(D.d0min, kStartCap) + i1.w;
+  if (dot(u1, u1) <= a1 * a1 * (1.f + 1e-5f) || dot(u0, u0) <= a0 * a0 * (1.f + 1e-5f)) return true;
+  if (!D.moving) return false;
+  const float a2 = i1.w + D.half + 1e-5f;
+  const V3 v1 = c1 - D.mid, v0 = c0 - D.mid;
+  return dot(v1, v1) <= a2 * a2 * (1.f + 1e-5f) || dot(v0, v0) <= a2 * a2 * (1.f + 1e-5f);
+}
+
+__device__ __forceinline__ void detect_record(const CtArgs& A, Detect& D, float4 i1, float4 i0) {
+  if (!detect_maybe(D, i1, i0)) return;
+  const V3 c1 = {i1.x, i1.y, i1.z}, c0 = {i0.x, i0.y, i0.z};
+  const bool nearNow = len(D.p - c1) - i1.w < kReach;
+  const bool nearStart = len(D.p0 - c0) - i1.w < fminf(D.d0min, kStartCap);
+  const bool onPath = D.moving && fminf(seg_dist(c0, D.p0, D.p), seg_dist(c1, D.p0, D.p)) < i1.w;
+  if (!nearNow && !nearStart && !onPath) return;
+  const uint32_t f = __float_as_uint(i0.w);
+  const uint32_t fq = A.ijl[f];
+  const int di = (int)(fq & 0xFFFu) - D.ni, dj = (int)((fq >> 12) & 0xFFFu) - D.nj;
+  if ((fq >> 24) == D.nl && di >= -3 && di <= 2 && dj >= -3 && dj <= 2 &&
+      (!D.damaged || lattice_linked(A, D.off, D.nx, D.ni, D.nj, D.ni + di, D.nj + dj)))
+    return;
+  const Corners cf = corners(A, f);
+  if (nearStart) {
+    const FaceHit s0 = face_hit_at(A, D.p0, cf);
+    if ((s0.d < D.d0min || (s0.d == D.d0min && f < D.f0)) && s0.d > 1e-9f) {
+      D.d0min = s0.d;
+      D.n0 = divs(D.p0 - s0.q, s0.d);
+      D.f0 = f;
+    }
+  }
+  if (onPath) {
+    const float t = path_crossing(A, D.p0, D.path, cf);
+    if (t < D.tc || (t == D.tc && t <= 1.f && f < D.fc)) {
+      D.tc = t;
+      D.fc = f;
+    }
+  }
+  if (!nearNow) return;
+  const FaceHit hit = face_hit(A, D.p, cf);
+  if (hit.d >= kReach) return;
+  uint32_t cand = f;
+  float cd = hit.d;
+#pragma unroll
+  for (int k = 0; k < kContactK; k++) {
+    if (cand == D.best[k]) break;
+    if (cd < D.bestD[k] || (cd == D.bestD[k] && cand < D.best[k])) {
+      const uint32_t ts = D.best[k];
+      const float td = D.bestD[k];
+      D.best[k] = cand;
+      D.bestD[k] = cd;
+      cand = ts;
+      cd = td;
+    }
+  }
+}
+
+__device__ void detect_end(const CtArgs& A, const Detect& D) {
+  const int Np = A.Np;
+  const uint32_t n = D.n;
+  for (int k = 0; k < kContactK; k++) A.pairs[k * Np + n] = D.best[k];
+  // the bound: the plane of the first face the path crosses, else the nearest face at the start
+  V3 nb = D.n0;
+  float db = D.d0min;
+  uint32_t fb = D.f0;
+  if (D.fc != kNone && D.fc != D.f0) {
+    const Corners c = corners(A, D.fc);
+    const V3 a = x0(A, c.c[0]);
+    V3 nrm = cross(x0(A, c.c[1]) - a, x0(A, c.c[2]) - a);
+    const float ln = len(nrm);
+    if (ln > 1e-12f) {
+      nrm = divs(nrm, ln);
+      const float side = dot(D.p0 - a, nrm);
+      nb = side >= 0.f ? nrm : V3{-nrm.x, -nrm.y, -nrm.z};
+      db = fabsf(side);
+      fb = D.fc;
+    }
+  }
+  if (fb == kNone || (fb == D.f0 && db >= kStartCap)) {
+    A.cmeta[2 * n] = make_float4(0.f, 0.f, 0.f, kUnbounded);
+    return;
+  }
+  const float share = kBoundFraction * fmaxf(db - kHardCore, 0.f);
   A.cmeta[2 * n] = make_float4(nb.x, nb.y, nb.z, share);
   // the tightest share wins, then the larger node index: the share code above the node, as the reference's
   // 32-bit key (selfcontact.wgsl:462-474), but with the node in a full 32-bit word so any node count fits
   const unsigned long long key =
       ((unsigned long long)(4095u - min(4095u, (uint32_t)(share / kRecipUnit))) << 32) | (unsigned long long)(n + 1u);
-  Corners c = corners(A, fb);
+  const Corners c = corners(A, fb);
   for (int k = 0; k < 4; k++) atomicMax(&A.keys[c.c[k]], key);
+}
+
+// every record of the node's scanned cells, cell by cell in (z, y, x) order, each cell's records in
+// storage order, read from the hash in global memory
+__device__ void detect_scan_global(const CtArgs& A, Detect& D) {
+  for (int dz = 0; dz <= D.ncz; dz++)
+    for (int dy = 0; dy <= D.ncy; dy++)
+      for (int dx_ = 0; dx_ <= D.ncx; dx_++) {
+        const uint32_t h = hash_cell((int)((uint32_t)D.lox + (uint32_t)dx_), (int)((uint32_t)D.loy + (uint32_t)dy),
+                                     (int)((uint32_t)D.loz + (uint32_t)dz));
+        const uint32_t start = A.cells[h], count = A.cells[kTable + h];
+        for (uint32_t r = 0; r < count; r++) detect_record(A, D, A.items[2 * (start + r)], A.items[2 * (start + r) + 1]);
+      }
+}
+
+__global__ __launch_bounds__(256) void k_detect(CtArgs A) {
+  const uint32_t n = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (n >= (uint32_t)A.N) return;
+  Detect D;
+  detect_begin(A, n, D);
+  if (!D.live) return;
+  detect_scan_global(A, D);
+  detect_end(A, D);
+}
+
+// The same detection for a one-layer scene by 16 x 16 node tiles: the tile's nodes scan boxes of a
+// few shared cells, so each cell's records are staged in LDS once per tile (kStage at a time) and
+// every node whose box holds the cell reads them from there. The cells of the union of the tile's
+// boxes are visited in (z, y, x) order, so every node still sees its own cells in its own order and
+// each cell's records in storage order: the same sequence of detect_record calls as k_detect, hence
+// the same results bit for bit. A tile whose union box exceeds kUnionCells (fast nodes) scans from
+// global memory as k_detect does.
+constexpr int kTile = 16;
+constexpr int kStage = 256;
+constexpr int kUnionCells = 64;
+
+__global__ __launch_bounds__(256) void k_detect_tiled(CtArgs A, int tiles_x) {
+  __shared__ float4 rec[2 * kStage];
+  __shared__ int box[6];
+  __shared__ uint32_t span[2];
+  const int tile = blockIdx.x;
+  const int ti = tile % tiles_x, tj = tile / tiles_x;
+  const int nx = A.geo.nx[0], ny = A.geo.ny[0];
+  const int i = ti * kTile + (threadIdx.x % kTile), j = tj * kTile + (threadIdx.x / kTile);
+  const bool inside = i < nx && j < ny;
+  const uint32_t n = inside ? (uint32_t)(A.geo.off[0] + j * nx + i) : 0u;
+  Detect D;
+  D.live = false;
+  if (inside) detect_begin(A, n, D);
+  // the union of the live nodes' scanned boxes
+  if (threadIdx.x < 6) box[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : INT_MIN;
+  __syncthreads();
+  // a box end past INT_MAX (positions far out of range) leaves the union to the global scan
+  const bool wraps = D.live && ((long long)D.lox + D.ncx > INT_MAX || (long long)D.loy + D.ncy > INT_MAX ||
+                                (long long)D.loz + D.ncz > INT_MAX);
+  if (D.live && !wraps) {
+    atomicMin(&box[0], D.lox);
+    atomicMin(&box[1], D.loy);
+    atomicMin(&box[2], D.loz);
+    atomicMax(&box[3], D.lox + D.ncx);
+    atomicMax(&box[4], D.loy + D.ncy);
+    atomicMax(&box[5], D.loz + D.ncz);
+  }
+  const bool any_live = __syncthreads_or(D.live);
+  const bool any_wrap = __syncthreads_or(wraps);
+  if (!any_live) return;  // uniform
+  const int bx0 = box[0], by0 = box[1], bz0 = box[2], bx1 = box[3], by1 = box[4], bz1 = box[5];
+  const long long cells = bx0 > bx1 ? 0 : ((long long)bx1 - bx0 + 1) * ((long long)by1 - by0 + 1) * ((long long)bz1 - bz0 + 1);
+  if (any_wrap || cells > kUnionCells) {
+    if (D.live) {
+      detect_scan_global(A, D);
+      detect_end(A, D);
+    }
+    return;
+  }
+  for (int cz = bz0; cz <= bz1; cz++)
+    for (int cy = by0; cy <= by1; cy++)
+      for (int cx = bx0; cx <= bx1; cx++) {
+        const bool mine = D.live && cx >= D.lox && cx <= D.lox + D.ncx && cy >= D.loy && cy <= D.loy + D.ncy &&
+                          cz >= D.loz && cz <= D.loz + D.ncz;
+        if (!__syncthreads_or(mine)) continue;
+        if (threadIdx.x == 0) {
+          const uint32_t h = hash_cell(cx, cy, cz);
+          span[0] = A.cells[h];
+          span[1] = A.cells[kTable + h];
+        }
+        __syncthreads();
+        const uint32_t start = span[0], count = span[1];
+        for (uint32_t r0 = 0; r0 < count; r0 += kStage) {
+          const uint32_t m = min((uint32_t)kStage, count - r0);
+          for (uint32_t q = threadIdx.x; q < 2 * m; q += blockDim.x) rec[q] = A.items[2 * (start + r0) + q];
+          __syncthreads();
+          if (mine) {
+            // the chunk's records this node may take, as bits, then those in order: the lanes of
+            // a wave diverge over their own few records instead of all lanes taking every record
+            // that any lane takes
+            uint32_t take[kStage / 32];
+#pragma unroll
+            for (int w = 0; w < kStage / 32; w++) {
+              uint32_t bits = 0u;
+              const uint32_t r0 = 32u * w, nb = r0 < m ? min(32u, m - r0) : 0u;
+#pragma unroll 4
+              for (uint32_t b = 0; b < nb; b++)
+                bits |= detect_maybe(D, rec[2 * (r0 + b)], rec[2 * (r0 + b) + 1]) ? 1u << b : 0u;
+              take[w] = bits;
+            }
+#pragma unroll
+            for (int w = 0; w < kStage / 32; w++)
+              while (take[w]) {
+                const uint32_t r = 32u * w + (uint32_t)__builtin_ctz(take[w]);
+                take[w] &= take[w] - 1u;
+                detect_record(A, D, rec[2 * r], rec[2 * r + 1]);
+              }
+          }
+          __syncthreads();
+        }
+      }
+  if (D.live) detect_end(A, D);
 }
 
 // reciprocal bounds of face corners; clears the keys (selfcontact.wgsl:470-484)
@@ -657,7 +841,12 @@ void launch_contact_detect(Solver& s) {
     // per node: row marks (7 x 4 B flags), x, dx, ijl, K pairs, cmeta, key (minimal unique bytes)
     ProfScope ps(s, PC_CONTACT_DETECT, s.N * (28.0 + 4.0 + 24.0 + 4.0 + 4.0 * kContactK + 32.0 + 16.0));
     k_damage_row<<<g, 256, 0, s.stream>>>(A);
-    k_detect<<<g, 256, 0, s.stream>>>(A);
+    if (s.nl == 1) {
+      const int tx = (s.geo.nx[0] + kTile - 1) / kTile, ty = (s.geo.ny[0] + kTile - 1) / kTile;
+      k_detect_tiled<<<tx * ty, kTile * kTile, 0, s.stream>>>(A, tx);
+    } else {
+      k_detect<<<g, 256, 0, s.stream>>>(A);
+    }
     k_resolve<<<g, 256, 0, s.stream>>>(A);
   }
   s.contact_detected = true;
