@@ -19,7 +19,7 @@ extern "C" int bd_topology_search(int N, int layer_count, const int32_t* nx, con
       !piece_nodes || !piece_count || (anchor_count > 0 && !anchors))
     return BD_ERR_INVALID;
   // lattice row length, column and row of every node
-  std::vector<int32_t> stride(N, 0), column(N, 0), row(N, 0);
+  std::vector<int32_t> stride(N, 0), column(N, 0), row(N, 0), rows(N, 0);
   long total = 0;
   for (int l = 0; l < layer_count; l++) {
     if (nx[l] <= 0 || ny[l] <= 0 || offset[l] < 0 || (long)offset[l] + (long)nx[l] * ny[l] > N) return BD_ERR_INVALID;
@@ -28,6 +28,7 @@ extern "C" int bd_topology_search(int N, int layer_count, const int32_t* nx, con
       stride[n] = nx[l];
       column[n] = k % nx[l];
       row[n] = k / nx[l];
+      rows[n] = ny[l];
     }
     total += (long)nx[l] * ny[l];
   }
@@ -35,12 +36,13 @@ extern "C" int bd_topology_search(int N, int layer_count, const int32_t* nx, con
   for (int a = 0; a < anchor_count; a++)
     if (anchors[a] < 0 || anchors[a] >= N) return BD_ERR_INVALID;
 
-  // neighbours of n through active spans (it owns the spans toward +u and +v)
+  // neighbours of n through active spans (it owns the spans toward +u and +v; a span bit at the
+  // lattice's edge names no neighbour and is ignored)
   auto neighbours = [&](int n, int out[4]) {
     int c = 0;
     const int s = stride[n];
-    if (bits[n] & 1) out[c++] = n + 1;
-    if (bits[n] & 2) out[c++] = n + s;
+    if ((bits[n] & 1) && column[n] + 1 < s) out[c++] = n + 1;
+    if ((bits[n] & 2) && row[n] + 1 < rows[n]) out[c++] = n + s;
     if (column[n] > 0 && (bits[n - 1] & 1)) out[c++] = n - 1;
     if (row[n] > 0 && (bits[n - s] & 2)) out[c++] = n - s;
     return c;

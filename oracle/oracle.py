@@ -12,6 +12,7 @@ yarnField .test.ts), closed forms, brute force on tiny inputs and the sceneHash 
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 import sys
 from pathlib import Path
@@ -24,7 +25,8 @@ sys.path.insert(0, str(ROOT / "break-that-denim_amd"))
 
 from bd.capi import SceneBuffers, GripBuffers, bd_params, bd_scene, bd_grips, bd_yarn, make_params  # noqa: E402
 
-LIB = HERE / "liboracle.so"
+# BD_ORACLE_LIB names another build of the same sources (the sanitizer test's instrumented one)
+LIB = Path(os.environ["BD_ORACLE_LIB"]) if os.environ.get("BD_ORACLE_LIB") else HERE / "liboracle.so"
 _lib = None
 
 STENCIL = 25

@@ -189,3 +189,27 @@ def test_curved_line_search_along_a_rigid_rotation_changes_only_inertia():
     assert np.allclose(e, want, rtol=1e-7), (e, want)
     # the straight tangent step stretches every span: more energy than the curved one
     assert (es > e + 1e-3 * np.abs(e)).all()
+
+
+def test_topology_search_ignores_span_bits_at_the_lattice_edge():
+    """A +u bit on a layer's last column or a +v bit on its last row names no neighbour: the search
+    gives the same result as without those bits (and reads nothing past the layer)."""
+    layers = [(7, 5, 0), (4, 3, 35)]
+    N = 35 + 12
+    rng = np.random.default_rng(9)
+    bits = (rng.random(N) > 0.3).astype(np.uint8) | ((rng.random(N) > 0.3).astype(np.uint8) << 1)
+    clean = bits.copy()
+    for (nx, ny, off) in layers:
+        for j in range(ny):
+            for i in range(nx):
+                n = off + j * nx + i
+                if i == nx - 1:
+                    clean[n] &= ~np.uint8(1)
+                if j == ny - 1:
+                    clean[n] &= ~np.uint8(2)
+    mounted = np.zeros(N, np.uint8)
+    mounted[:7] = 1
+    a = search(layers, bits, mounted, [40])
+    b = search(layers, clean, mounted, [40])
+    for x, y in zip(a, b):
+        assert np.array_equal(np.asarray(x), np.asarray(y))
