@@ -131,33 +131,53 @@ def smoother_probe(args, sweeps):
     gpu.close()
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(cfg, gpu_iters_per_newton, newton_per_frame):
-    """Time the FP64 oracle (1 thread) on one Newton iteration of the same frame: assemble + rhs +
-    Galerkin hierarchy + 2 PCG iterations + line search; extrapolate to the GPU's iteration counts."""
+    """Time the FP64 oracle on one Newton iteration of the same frame (assemble + rhs + Galerkin
+    hierarchy + 2 PCG iterations + line search), on one thread and on the threads OpenMP is given
+    (the matvec and line-smoother loops are threaded; the results are the same); extrapolate both to
+    the GPU's iteration counts. `value` is the threaded one."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     from bd.configs import advance_grips
-    grips = [dict(g) for g in cfg["grips"]]
-    advance_grips(grips, cfg["velocity"])
-    o = O.Oracle(cfg["scene"], cfg["params"], grips)
-    o.predict()
-    t0 = time.perf_counter()
-    o.assemble()
-    b = o.rhs()
-    o.build_hierarchy()
-    t1 = time.perf_counter()
-    d = o.pcg(b, 2)
-    t2 = time.perf_counter()
-    o.energy_change(d)
-    t3 = time.perf_counter()
-    setup = (t1 - t0) + (t3 - t2)
-    per_iter = (t2 - t1) / 2.0
-    frame_ms = 1e3 * newton_per_frame * (setup + gpu_iters_per_newton * per_iter)
-    return {"value": round(frame_ms, 1), "unit": "ms/frame", "cores": 1, "kind": "port",
+
+    def sample(threads):
+        used = O.set_threads(threads)
+        grips = [dict(g) for g in cfg["grips"]]
+        advance_grips(grips, cfg["velocity"])
+        o = O.Oracle(cfg["scene"], cfg["params"], grips)
+        o.predict()
+        t0 = time.perf_counter()
+        o.assemble()
+        b = o.rhs()
+        o.build_hierarchy()
+        t1 = time.perf_counter()
+        d = o.pcg(b, 2)
+        t2 = time.perf_counter()
+        o.energy_change(d)
+        t3 = time.perf_counter()
+        setup = (t1 - t0) + (t3 - t2)
+        per_iter = (t2 - t1) / 2.0
+        return used, 1e3 * newton_per_frame * (setup + gpu_iters_per_newton * per_iter), (t1 - t0, t2 - t1, t3 - t2)
+
+    most = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    _, one_ms, one = sample(1)
+    cores, many_ms, many = sample(most)
+    return {"value": round(many_ms, 1), "unit": "ms/frame", "cores": cores, "kind": "port",
+            "single_core_value": round(one_ms, 1), "cpu": cpu_model(),
             "sample": f"FP64 C oracle on one Newton iteration of the same {cfg['scene'].node_count}-node frame "
-                      f"(assemble+rhs+hierarchy {t1 - t0:.2f} s, 2 PCG iterations {t2 - t1:.2f} s, line search "
-                      f"{t3 - t2:.2f} s), extrapolated to {newton_per_frame} Newton x {gpu_iters_per_newton:.1f} PCG "
-                      f"iterations per frame"}
+                      f"(assemble+rhs+hierarchy {many[0]:.2f} s, 2 PCG iterations {many[1]:.2f} s, line search "
+                      f"{many[2]:.2f} s on {cores} threads; {one[0]:.2f} / {one[1]:.2f} / {one[2]:.2f} s on one), "
+                      f"extrapolated to {newton_per_frame} Newton x {gpu_iters_per_newton:.1f} PCG iterations per frame"}
 
 
 def method_params(cfg, method, args):

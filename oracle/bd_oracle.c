@@ -15,6 +15,9 @@
  * (gate.wgsl semantics). Compile with -ffp-contract=off so products are rounded as written.
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -235,6 +238,8 @@ static inline void add_block(op_t* A, int row, int o, double s, const double* B)
 
 /* StencilOperator.apply (newtonMultigrid.ts:95-124) */
 static void op_apply(const op_t* A, const double* x, double* y) {
+  /* rows are independent: threads change no result */
+#pragma omp parallel for schedule(static)
   for (int n = 0; n < A->N; n++) {
     if (A->fixed[n]) {
       y[3 * n] = x[3 * n];
@@ -1475,11 +1480,6 @@ static void smooth_lines(const op_t* A, double* x, const double* b, int reverse,
     if (A->lay[l].nx > maxLen) maxLen = A->lay[l].nx;
     if (A->lay[l].ny > maxLen) maxLen = A->lay[l].ny;
   }
-  double* D = malloc(sizeof(double) * 9 * maxLen);
-  double* E = malloc(sizeof(double) * 9 * maxLen);
-  double* F = calloc(9 * maxLen, sizeof(double));
-  double* r = malloc(sizeof(double) * 3 * maxLen);
-  double* sol = calloc(3 * maxLen, sizeof(double));
   double* xOld = malloc(sizeof(double) * 3 * A->N);
   int order[12][2], no = 0;
   int fams[2] = {1, 0};
@@ -1497,65 +1497,88 @@ static void smooth_lines(const op_t* A, double* x, const double* b, int reverse,
       const lay_t* L = &A->lay[l];
       int lines = family == 0 ? L->ny : L->nx;
       int len = family == 0 ? L->nx : L->ny;
-      for (int line = color; line < lines; line += colors) {
-        memset(D, 0, sizeof(double) * 9 * len);
-        memset(E, 0, sizeof(double) * 9 * len);
-        for (int k = 0; k < len; k++) {
-          int n = family == 0 ? L->offset + line * L->nx + k : L->offset + k * L->nx + line;
-          r[3 * k] = b[3 * n];
-          r[3 * k + 1] = b[3 * n + 1];
-          r[3 * k + 2] = b[3 * n + 2];
-          if (A->fixed[n]) {
-            D[9 * k] = D[9 * k + 4] = D[9 * k + 8] = 1;
-            continue;
-          }
-          double maj = 0;
-          for (int dj = -2; dj <= 2; dj++)
-            for (int di = -2; di <= 2; di++) {
-              int m = op_nbr(A, n, di, dj);
-              if (m < 0 || (A->fixed[m] && !(di == 0 && dj == 0))) continue;
-              const double* base = A->A + ((size_t)n * STENCIL + offset_index(di, dj)) * 9;
-              int along = family == 0 ? di : dj;
-              int across = family == 0 ? dj : di;
-              if (across == 0 && along == 0) {
-                for (int q = 0; q < 9; q++) D[9 * k + q] += base[q];
-                continue;
-              }
-              if (across == 0 && along == 1) {
-                for (int q = 0; q < 9; q++) E[9 * k + q] = base[q];
-                continue;
-              }
-              if (across == 0 && along == -1) continue;
-              if (across == 0 || abs(across) == colors) maj += frob(base);
-              double x0 = xOld[3 * m], x1 = xOld[3 * m + 1], x2 = xOld[3 * m + 2];
-              r[3 * k] -= base[0] * x0 + base[1] * x1 + base[2] * x2;
-              r[3 * k + 1] -= base[3] * x0 + base[4] * x1 + base[5] * x2;
-              r[3 * k + 2] -= base[6] * x0 + base[7] * x1 + base[8] * x2;
+      /* the lines of one colour read the snapshot and write only their own nodes: threads change
+         no result */
+#pragma omp parallel
+      {
+        double* D = malloc(sizeof(double) * 9 * maxLen);
+        double* E = malloc(sizeof(double) * 9 * maxLen);
+        double* F = calloc(9 * maxLen, sizeof(double));
+        double* r = malloc(sizeof(double) * 3 * maxLen);
+        double* sol = calloc(3 * maxLen, sizeof(double));
+#pragma omp for schedule(static)
+        for (int line = color; line < lines; line += colors) {
+          memset(D, 0, sizeof(double) * 9 * len);
+          memset(E, 0, sizeof(double) * 9 * len);
+          for (int k = 0; k < len; k++) {
+            int n = family == 0 ? L->offset + line * L->nx + k : L->offset + k * L->nx + line;
+            r[3 * k] = b[3 * n];
+            r[3 * k + 1] = b[3 * n + 1];
+            r[3 * k + 2] = b[3 * n + 2];
+            if (A->fixed[n]) {
+              D[9 * k] = D[9 * k + 4] = D[9 * k + 8] = 1;
+              continue;
             }
-          D[9 * k] += maj;
-          D[9 * k + 4] += maj;
-          D[9 * k + 8] += maj;
-          r[3 * k] += maj * xOld[3 * n];
-          r[3 * k + 1] += maj * xOld[3 * n + 1];
-          r[3 * k + 2] += maj * xOld[3 * n + 2];
+            double maj = 0;
+            for (int dj = -2; dj <= 2; dj++)
+              for (int di = -2; di <= 2; di++) {
+                int m = op_nbr(A, n, di, dj);
+                if (m < 0 || (A->fixed[m] && !(di == 0 && dj == 0))) continue;
+                const double* base = A->A + ((size_t)n * STENCIL + offset_index(di, dj)) * 9;
+                int along = family == 0 ? di : dj;
+                int across = family == 0 ? dj : di;
+                if (across == 0 && along == 0) {
+                  for (int q = 0; q < 9; q++) D[9 * k + q] += base[q];
+                  continue;
+                }
+                if (across == 0 && along == 1) {
+                  for (int q = 0; q < 9; q++) E[9 * k + q] = base[q];
+                  continue;
+                }
+                if (across == 0 && along == -1) continue;
+                if (across == 0 || abs(across) == colors) maj += frob(base);
+                double x0 = xOld[3 * m], x1 = xOld[3 * m + 1], x2 = xOld[3 * m + 2];
+                r[3 * k] -= base[0] * x0 + base[1] * x1 + base[2] * x2;
+                r[3 * k + 1] -= base[3] * x0 + base[4] * x1 + base[5] * x2;
+                r[3 * k + 2] -= base[6] * x0 + base[7] * x1 + base[8] * x2;
+              }
+            D[9 * k] += maj;
+            D[9 * k + 4] += maj;
+            D[9 * k + 8] += maj;
+            r[3 * k] += maj * xOld[3 * n];
+            r[3 * k + 1] += maj * xOld[3 * n + 1];
+            r[3 * k + 2] += maj * xOld[3 * n + 2];
+          }
+          or_solve_penta(len, D, E, F, r, sol, PIVOT_SHIFT);
+          for (int k = 0; k < len; k++) {
+            int n = family == 0 ? L->offset + line * L->nx + k : L->offset + k * L->nx + line;
+            if (A->fixed[n]) continue;
+            x[3 * n] = sol[3 * k];
+            x[3 * n + 1] = sol[3 * k + 1];
+            x[3 * n + 2] = sol[3 * k + 2];
+          }
         }
-        or_solve_penta(len, D, E, F, r, sol, PIVOT_SHIFT);
-        for (int k = 0; k < len; k++) {
-          int n = family == 0 ? L->offset + line * L->nx + k : L->offset + k * L->nx + line;
-          if (A->fixed[n]) continue;
-          x[3 * n] = sol[3 * k];
-          x[3 * n + 1] = sol[3 * k + 1];
-          x[3 * n + 2] = sol[3 * k + 2];
-        }
+        free(D);
+        free(E);
+        free(F);
+        free(r);
+        free(sol);
       }
     }
   }
-  free(D);
-  free(E);
-  free(F);
-  free(r);
-  free(sol);
   free(xOld);
+}
+
+/* Threads of the OpenMP loops (op_apply, smooth_lines); returns the number in use (1 without
+   OpenMP). Results do not depend on it. */
+int or_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
 }
 
 /* vcycle (newtonMultigrid.ts:495-528), pre = post = 1 */

@@ -53,6 +53,8 @@ def lib():
         vp = C.c_void_p
         P = C.POINTER
         dp = P(C.c_double)
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_threads.restype = C.c_int
         L.or_create.restype = vp
         L.or_create.argtypes = [P(bd_scene), P(bd_params)]
         L.or_destroy.argtypes = [vp]
@@ -553,3 +555,9 @@ def floor_bound_z(z0, linz, dz, half, mass, gnorm, fz):
 def fp16_round(a):
     """The oracle's FP16 rounding of one value (round to nearest even, saturated at 65504)."""
     return lib().or_fp16_round(float(a))
+
+
+def set_threads(n=0):
+    """Threads of the oracle's row and line loops (0 keeps the current count); returns the count in
+    use. Results do not depend on it."""
+    return lib().or_set_threads(int(n))
