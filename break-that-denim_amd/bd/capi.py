@@ -313,6 +313,7 @@ def load_library(path: os.PathLike | None = None):
     lib.bd_graph_substep.argtypes = [vp]
     lib.bd_graph_release.argtypes = [vp]
     lib.bd_batch_create.argtypes = [P(vp), C.c_int, P(vp)]
+    lib.bd_batch_create_concurrent.argtypes = [P(vp), C.c_int, C.c_int, P(vp)]
     for name in ("bd_batch_substep", "bd_batch_sync", "bd_batch_destroy"):
         getattr(lib, name).argtypes = [vp]
     lib.bd_batch_last_error.argtypes = [vp]
@@ -336,5 +337,5 @@ EXPORTED = (
     "bd_learn_record", "bd_learn_train", "bd_set_learned", "bd_learn_get_weights", "bd_learn_set_weights",
     "bd_learn_loss_grad", "bd_learn_read_sample", "bd_learn_info", "bd_learn_read_fields", "bd_apply",
     "bd_apply_noalloc", "bd_set_pcg_mode", "bd_graph_capture", "bd_graph_substep", "bd_graph_release", "bd_batch_create",
-    "bd_batch_substep", "bd_batch_sync", "bd_batch_destroy", "bd_batch_last_error",
+    "bd_batch_create_concurrent", "bd_batch_substep", "bd_batch_sync", "bd_batch_destroy", "bd_batch_last_error",
 )

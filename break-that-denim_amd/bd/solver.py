@@ -392,17 +392,25 @@ class GpuSolver:
 
 
 class Batch:
-    """Independent solvers (uneven patches, each its own Newton / MG-PCG system) advanced by one graph
-    launch per frame: every solver's captured substep is a branch of the batch's graph."""
+    """Independent solvers (uneven patches, each its own Newton / MG-PCG system) advanced together.
+    By default the form bd_batch_create picks: one graph launch per frame (every solver's captured
+    substep a branch of the batch's graph), or, for solvers in BD_PCG_WORKGROUP mode, each Newton
+    iteration's solves in one launch. With `threads` (an int, 0 for the library's default) the
+    concurrent form: each solver's eager substep on its own stream from a pool of host threads."""
 
-    def __init__(self, solvers):
+    def __init__(self, solvers, threads=None):
         self.lib = solvers[0].lib
         self.solvers = list(solvers)
         arr = (C.c_void_p * len(self.solvers))(*[g.h for g in self.solvers])
         out = C.c_void_p()
-        rc = self.lib.bd_batch_create(arr, len(self.solvers), C.byref(out))
+        if threads is None:
+            rc = self.lib.bd_batch_create(arr, len(self.solvers), C.byref(out))
+            what = "bd_batch_create"
+        else:
+            rc = self.lib.bd_batch_create_concurrent(arr, len(self.solvers), int(threads), C.byref(out))
+            what = "bd_batch_create_concurrent"
         if rc != 0:
-            raise RuntimeError(f"bd_batch_create failed ({rc}): {self.lib.bd_batch_last_error(None).decode()}")
+            raise RuntimeError(f"{what} failed ({rc}): {self.lib.bd_batch_last_error(None).decode()}")
         self.b = out
 
     def _check(self, rc, what):

@@ -3,8 +3,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <mutex>
+#include <thread>
 #include <cstdlib>
 #include <cstring>
 #include <stdexcept>
@@ -1746,6 +1748,12 @@ int bd_get_stats(bd_solver* h, bd_stats* out) {
 
 struct bd_batch {
   std::vector<bd_solver*> solvers;
+  // concurrent form: each solver's eager substep on its own stream, `threads` host threads taking
+  // the solvers in `order` (largest system first)
+  bool concurrent = false;
+  int threads = 0;
+  int device = 0;
+  std::vector<int> order;
   // fused form: every solver in BD_PCG_WORKGROUP mode, the PCG solves of a Newton iteration in one
   // launch over the systems' descriptors (copied back to back)
   bool fused = false;
@@ -1865,8 +1873,82 @@ int bd_batch_create(bd_solver* const* solvers, int count, bd_batch** out) {
   return BD_OK;
 }
 
+int bd_batch_create_concurrent(bd_solver* const* solvers, int count, int threads, bd_batch** out) {
+  g_create_error.clear();
+  if (!out || !solvers || count < 1 || threads < 0) {
+    g_create_error = "bd_batch_create_concurrent: need at least one solver, an output and threads >= 0";
+    return BD_ERR_INVALID;
+  }
+  *out = nullptr;
+  for (int i = 0; i < count; i++) {
+    if (!solvers[i]) {
+      g_create_error = "bd_batch_create_concurrent: null solver";
+      return BD_ERR_INVALID;
+    }
+    for (int j = 0; j < i; j++)
+      if (solvers[j] == solvers[i]) {
+        g_create_error = "bd_batch_create_concurrent: a solver appears twice";
+        return BD_ERR_INVALID;
+      }
+  }
+  bd_batch* b = new bd_batch;
+  b->concurrent = true;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  b->threads = std::min(count, threads > 0 ? threads : (int)std::min(8u, hw));
+  if (hipGetDevice(&b->device) != hipSuccess) {
+    g_create_error = "bd_batch_create_concurrent: no current device";
+    delete b;
+    return BD_ERR_HIP;
+  }
+  b->solvers.assign(solvers, solvers + count);
+  // largest systems first, so that the last ones to start are the shortest
+  b->order.resize(count);
+  for (int i = 0; i < count; i++) b->order[i] = i;
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](int u, int v) { return b->solvers[u]->s.N > b->solvers[v]->s.N; });
+  *out = b;
+  return BD_OK;
+}
+
 int bd_batch_substep(bd_batch* b) {
   return batch_guarded(b, [&](bd_batch& B) {
+    if (B.concurrent) {
+      // each worker takes the next solver and runs its whole substep (the host's convergence checks
+      // included) on the solver's own stream; the streams' kernels then share the chip
+      std::atomic<int> next{0};
+      std::mutex mu;
+      int rc = BD_OK;
+      std::string err;
+      auto work = [&](bool own_thread) {
+        if (own_thread && hipSetDevice(B.device) != hipSuccess) {
+          std::lock_guard<std::mutex> lock(mu);
+          if (rc == BD_OK) {
+            rc = BD_ERR_HIP;
+            err = "bd_batch_substep: hipSetDevice failed in a worker";
+          }
+          return;
+        }
+        for (;;) {
+          const int k = next.fetch_add(1);
+          if (k >= (int)B.order.size()) return;
+          const int i = B.order[k];
+          const int r = bd_substep(B.solvers[i], 0, 0);
+          if (r != BD_OK) {
+            std::lock_guard<std::mutex> lock(mu);
+            if (rc == BD_OK) {
+              rc = r;
+              err = "bd_batch_substep: solver " + std::to_string(i) + ": " + B.solvers[i]->s.err;
+            }
+          }
+        }
+      };
+      std::vector<std::thread> pool;
+      for (int t = 1; t < B.threads; t++) pool.emplace_back(work, true);
+      work(false);
+      for (std::thread& t : pool) t.join();
+      if (rc != BD_OK) B.err = err;
+      return rc;
+    }
     for (size_t i = 0; i < B.solvers.size(); i++) {
       const Solver& s = B.solvers[i]->s;
       const bool stale = B.fused ? (s.config_gen != B.gens[i] || s.pcg_mode != BD_PCG_WORKGROUP)
@@ -1919,7 +2001,7 @@ int bd_batch_substep(bd_batch* b) {
 
 int bd_batch_sync(bd_batch* b) {
   return batch_guarded(b, [&](bd_batch& B) {
-    BD_HIP(hipStreamSynchronize(B.stream));
+    if (B.stream) BD_HIP(hipStreamSynchronize(B.stream));
     for (bd_solver* h : B.solvers) BD_HIP(hipStreamSynchronize(h->s.stream));
     return BD_OK;
   });
@@ -1927,7 +2009,7 @@ int bd_batch_sync(bd_batch* b) {
 
 int bd_batch_destroy(bd_batch* b) {
   if (!b) return BD_OK;
-  (void)hipStreamSynchronize(b->stream);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
   batch_free(b);
   return BD_OK;
 }

@@ -106,4 +106,9 @@ def test_graph_and_batch_argument_errors():
     arr = (C.c_void_p * 1)(None)
     assert lib.bd_batch_create(arr, 1, C.byref(out)) == capi.BD_ERR_STATE
     assert b"bd_graph_capture" in lib.bd_batch_last_error(None)
+    assert lib.bd_batch_create_concurrent(None, 0, 0, C.byref(out)) == -1
+    assert b"at least one solver" in lib.bd_batch_last_error(None)
+    assert lib.bd_batch_create_concurrent(arr, 1, -1, C.byref(out)) == -1
+    assert lib.bd_batch_create_concurrent(arr, 1, 2, C.byref(out)) == -1
+    assert b"null solver" in lib.bd_batch_last_error(None)
     assert lib.bd_batch_destroy(None) == 0

@@ -9,7 +9,10 @@
 * a setter that changes what a substep launches makes the replay refuse until the next capture, and
   the capture refuses what needs the host between Newton iterations;
 * a batch of uneven patches (each its own Newton / MG-PCG system) advanced by one graph launch per
-  frame equals each patch's eager substeps bit for bit.
+  frame equals each patch's eager substeps bit for bit;
+* the concurrent batch (each patch's eager substep on its own stream from a pool of host threads)
+  equals the patches' eager substeps run one after another bit for bit, with self-contact on some
+  patches.
 """
 import sys
 
@@ -156,6 +159,41 @@ def test_batch_of_uneven_patches_matches_eager():
     batched[0].graph_capture(cfg["patches"][0]["params"].pcg_max_iterations)
     with pytest.raises(RuntimeError, match="captured again or changed"):
         batch.substep()
+    batch.close()
+    for g in eager + batched:
+        g.close()
+
+
+def test_concurrent_batch_matches_eager_substeps():
+    from bd.solver import Batch, GpuSolver
+    cfg = config_patches(count=7, side_range=(32, 120), seed=5)
+    sides = [p["side"] for p in cfg["patches"]]
+    say(f"gpu: concurrent batch of {len(sides)} patches, sides {sides}, 3 host threads")
+    eager, batched, grips = [], [], []
+    for i, p in enumerate(cfg["patches"]):
+        params = p["params"]
+        params.self_contact = i % 2 == 1
+        ga, gb = [dict(g) for g in p["grips"]], [dict(g) for g in p["grips"]]
+        eager.append(GpuSolver(p["scene"], params, ga))
+        batched.append(GpuSolver(p["scene"], params, gb))
+        grips.append((ga, gb))
+    batch = Batch(batched, threads=3)
+    for k in range(4):
+        for p, a, (ga, gb) in zip(cfg["patches"], eager, grips):
+            advance_grips(ga, p["velocity"])
+            advance_grips(gb, p["velocity"])
+        for a, b, (ga, gb) in zip(eager, batched, grips):
+            a.set_grip_goals(np.array([g["goal"] for g in ga]))
+            b.set_grip_goals(np.array([g["goal"] for g in gb]))
+            a.encode_substep()
+        batch.substep()
+        batch.sync()
+        its = []
+        for i, (a, b) in enumerate(zip(eager, batched)):
+            st = same_frame(a, b, f"frame {k} patch {i}")
+            its.append(sum(st["pcg_iterations"][:st["newton_iterations"]]))
+            assert max(st["rel_residual"][:st["newton_iterations"]]) <= 1.05e-6, (k, i, st)
+        say(f"  frame {k}: PCG iterations per patch {its}, bit-exact")
     batch.close()
     for g in eager + batched:
         g.close()
