@@ -76,7 +76,7 @@ def parse():
                     help="sheet1024: the headline workload (default); sheet64: the 64 x 64 sheet, eager against "
                          "graph-captured substeps; patches48: 48 uneven patches, one eager substep per patch "
                          "against one graph launch for the batch")
-    ap.add_argument("--batch-threads", type=int, default=8,
+    ap.add_argument("--batch-threads", type=int, default=16,
                     help="host threads of the patches' concurrent batch (grid+threads)")
     ap.add_argument("--sweeps", type=int, default=1, help="pre- and post-smoothing sweeps per level")
     ap.add_argument("--mixed-coarsest", type=int, default=MIXED_COARSEST_NODES, metavar="NODES",

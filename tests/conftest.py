@@ -1,8 +1,14 @@
 """Shared test setup: import paths, the `gpu` marker, and common scene fixtures."""
+import os
 import sys
 from pathlib import Path
 
 import pytest
+
+# Under pytest-xdist every worker would give the oracle's OpenMP loops all the host's cores, and
+# the workers' spinning threads then starve one another (a 40 s suite took 28 minutes at -n 4).
+if os.environ.get("PYTEST_XDIST_WORKER") and "OMP_NUM_THREADS" not in os.environ:
+    os.environ["OMP_NUM_THREADS"] = "1"
 
 ROOT = Path(__file__).resolve().parent.parent
 sys.path.insert(0, str(ROOT / "break-that-denim_amd"))
