@@ -227,6 +227,53 @@ __global__ __launch_bounds__(512, 4) void k_smooth_rows(SmoothArgs S, const floa
   if (!fb) store_x(S, nb, nob, xb, DUAL);
 }
 
+// Split variant with four rows per thread: the rows k_line_rows wrote (stride a multiple of 4, so
+// that a thread's four rows of a plane are one aligned float4), solved by line_solve4.
+template <bool FINE, int FAM, bool DUAL>
+__global__ __launch_bounds__(256) void k_smooth_rows4(SmoothArgs S, const float* rows, size_t cap, int stride) {
+  if (S.gate && *S.gate) return;
+  extern __shared__ __align__(16) float sm[];
+  const int slot = xcd_block(blockIdx.x, gridDim.x);
+  const LineGeo L = line_geo_at<FAM, DUAL>(S, slot);
+  const int len = L.ln.z;
+  const int t = threadIdx.x;
+  const int k0 = 4 * t;
+  const float* o = rows + (size_t)slot * stride;
+  float v[18][4];
+#pragma unroll
+  for (int q = 0; q < 18; q++) {
+    const float4 p = k0 < len ? *reinterpret_cast<const float4*>(o + q * cap + k0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    v[q][0] = p.x;
+    v[q][1] = p.y;
+    v[q][2] = p.z;
+    v[q][3] = p.w;
+  }
+  float D[4][6], U[4][9], r[4][3], x[4][3];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    if (k0 + m < len) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) D[m][q] = v[q][m];
+#pragma unroll
+      for (int q = 0; q < 9; q++) U[m][q] = v[6 + q][m];
+#pragma unroll
+      for (int q = 0; q < 3; q++) r[m][q] = v[15 + q][m];
+    } else {
+      set_identity(D[m], U[m], r[m]);
+    }
+  }
+  line_solve4(sm, blockDim.x, t, len, D, U, r, x);
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const int k = k0 + m;
+    if (k < len) {
+      int n, nb, no, i, j;
+      row_index<FAM, DUAL>(L, k, n, nb, no, i, j);
+      if (!S.fixed[n]) store_x(S, n, no, x[m], DUAL);
+    }
+  }
+}
+
 template <bool FINE, int FAM, int G>
 size_t staged_shm(int len) {
   int R2 = (len + 1) / 2;
@@ -241,12 +288,20 @@ constexpr int kMaxLds = 160 * 1024;
 // width (bd_autotune picks per level and family)
 template <bool FINE, int FAM, bool DUAL, class RDD>
 void launch_one(const SmoothArgs& S, RDD rdd, int count, int len, hipStream_t st, int variant, float* rows, size_t cap) {
-  if (variant == kSplitVariant) {
-    const int stride = (len + 1) & ~1;  // even, so that a row pair is one aligned float2
-    if (!rows || (size_t)count * stride > cap) throw std::runtime_error("smoother: split variant without room for its rows");
+  if (variant == kSplitVariant || variant == kSplit4Variant) {
+    const bool four = variant == kSplit4Variant;
+    // a thread's rows of a plane are one aligned float2 (two rows) or float4 (four)
+    const int stride = four ? (len + 3) & ~3 : (len + 1) & ~1;
+    if (!rows || (size_t)count * stride > cap || cap % 4 != 0)
+      throw std::runtime_error("smoother: split variant without room for its rows");
     k_line_rows<FINE, FAM, DUAL, RDD><<<dim3((len + 255) / 256, count), 256, 0, st>>>(S, rdd, rows, cap, stride);
-    const int block = ((len + 1) / 2 + 63) / 64 * 64;
-    k_smooth_rows<FINE, FAM, DUAL><<<count, block, (size_t)kSolveFloats * block * sizeof(float), st>>>(S, rows, cap, stride);
+    if (four) {
+      const int block = ((len + 3) / 4 + 63) / 64 * 64;
+      k_smooth_rows4<FINE, FAM, DUAL><<<count, block, (size_t)kSolveFloats * block * sizeof(float), st>>>(S, rows, cap, stride);
+    } else {
+      const int block = ((len + 1) / 2 + 63) / 64 * 64;
+      k_smooth_rows<FINE, FAM, DUAL><<<count, block, (size_t)kSolveFloats * block * sizeof(float), st>>>(S, rows, cap, stride);
+    }
     return;
   }
   if (variant == 4 || variant == 8) {
@@ -373,7 +428,10 @@ void launch_colour(Solver& s, int k, int c, int variant) {
 // a rows buffer, and the staged ones whose LDS fits
 std::vector<int> variants_for(bool fine, int len, bool has_rows) {
   std::vector<int> v{0};
-  if (has_rows) v.push_back(kSplitVariant);
+  if (has_rows) {
+    v.push_back(kSplitVariant);
+    v.push_back(kSplit4Variant);
+  }
   if ((fine ? staged_shm<true, 0, 4>(len) : staged_shm<false, 0, 4>(len)) <= (size_t)kMaxLds) v.push_back(4);
   if ((fine ? staged_shm<true, 0, 8>(len) : staged_shm<false, 0, 8>(len)) <= (size_t)kMaxLds) v.push_back(8);
   return v;
@@ -394,7 +452,7 @@ void smooth_alloc_rows(Solver& s, Level& L) {
   (void)s;
   int most = 0;
   for (int c = 0; c < 2 * L.ncol; c++) most = std::max(most, L.color_start[c + 1] - L.color_start[c]);
-  L.rows_cap = (size_t)most * ((L.max_len + 1) & ~1);
+  L.rows_cap = (size_t)most * ((L.max_len + 3) & ~3);  // room for either split variant's line stride
   BD_HIP(hipMalloc(&L.rows, 18 * L.rows_cap * sizeof(float)));
 }
 

@@ -288,8 +288,9 @@ __device__ __forceinline__ void gather_dispatch(int g, const SmoothArgs& S, cons
 constexpr int kRowF = 20;
 // LDS floats per reduced row: two record buffers
 constexpr int kSolveFloats = 2 * kRowF;
-// variant code of the split smoother (row kernel + solve kernel)
+// variant codes of the split smoother (row kernel + solve kernel, two or four rows per thread)
 constexpr int kSplitVariant = 16;
+constexpr int kSplit4Variant = 17;
 
 __device__ __forceinline__ void put_row(float* B, int t, const float Di[6], const float U[9], const float r[3]) {
   float4* p = reinterpret_cast<float4*>(B + kRowF * t);
@@ -308,16 +309,106 @@ __device__ __forceinline__ void get_row(const float* B, int j, float Di[6], floa
   r[0] = d.w; r[1] = e.x; r[2] = e.y;
 }
 
+// Block parallel cyclic reduction of R reduced rows of a line, one per thread (t < R): each row's
+// symmetric D, its forward coupling U (to row t + 1) and r in registers. At level d every row
+// eliminates both its neighbours at distance d at once, so after ceil(log2 R) levels (one barrier
+// each, the row kept in registers, LDS ping-ponged between the record buffers B0 and B1) each row
+// stands alone; there is no back-substitution sweep. The reduced system stays symmetric (row k's
+// coupling to k - d is row k - d's coupling to k, transposed), so a row carries only its forward
+// coupling, and each row publishes its own shifted pivot inverse with it (one inversion per row and
+// level). The first level writes B0 and the next one B1, so one barrier per level orders every read
+// before the overwrite; a caller may read B1 before the call. Returns the row's solution x and the
+// next row's xn (zero past the last row).
+__device__ __forceinline__ void pcr_rows(float* B0, float* B1, int t, int R, float D[6], float U[9], float r[3],
+                                         float x[3], float xn[3]) {
+  const bool row = t < R;
+  int lvl = 0;
+  for (int d = 1; d < R; d <<= 1, lvl ^= 1) {
+    float* B = lvl ? B1 : B0;
+    if (row) {
+      float Di[6];
+      pivot_inv(D, Di);
+      put_row(B, t, Di, U, r);
+    }
+    __syncthreads();
+    if (row) {
+      float Un[9];
+#pragma unroll
+      for (int q = 0; q < 9; q++) Un[q] = 0.f;
+      const int j1 = t - d, j2 = t + d;
+      if (j1 >= 0) {
+        // row j1 couples to this row by U1: D -= U1^T D1^-1 U1, r -= U1^T D1^-1 r1
+        float Di[6], U1[9], r1[3], T[9];
+        get_row(B, j1, Di, U1, r1);
+        float Dif[9];
+        sym_full(Di, Dif);
+        mm(Dif, U1, T);  // T = D1^-1 U1
+        D[0] -= U1[0] * T[0] + U1[3] * T[3] + U1[6] * T[6];
+        D[1] -= U1[0] * T[1] + U1[3] * T[4] + U1[6] * T[7];
+        D[2] -= U1[0] * T[2] + U1[3] * T[5] + U1[6] * T[8];
+        D[3] -= U1[1] * T[1] + U1[4] * T[4] + U1[7] * T[7];
+        D[4] -= U1[1] * T[2] + U1[4] * T[5] + U1[7] * T[8];
+        D[5] -= U1[2] * T[2] + U1[5] * T[5] + U1[8] * T[8];
+        float tt[3];
+        mtv(T, r1, tt);  // U1^T D1^-1 r1
+        r[0] -= tt[0];
+        r[1] -= tt[1];
+        r[2] -= tt[2];
+      }
+      if (j2 < R) {
+        // this row couples to row j2 by U, row j2 on to j2 + d by U2: D -= U D2^-1 U^T,
+        // r -= U D2^-1 r2, and the new coupling to j2 + d is -U D2^-1 U2
+        float Di[6], U2[9], r2[3], T2[9];
+        get_row(B, j2, Di, U2, r2);
+        float Dif[9];
+        sym_full(Di, Dif);
+        mm(U, Dif, T2);  // T2 = U D2^-1
+        D[0] -= T2[0] * U[0] + T2[1] * U[1] + T2[2] * U[2];
+        D[1] -= T2[0] * U[3] + T2[1] * U[4] + T2[2] * U[5];
+        D[2] -= T2[0] * U[6] + T2[1] * U[7] + T2[2] * U[8];
+        D[3] -= T2[3] * U[3] + T2[4] * U[4] + T2[5] * U[5];
+        D[4] -= T2[3] * U[6] + T2[4] * U[7] + T2[5] * U[8];
+        D[5] -= T2[6] * U[6] + T2[7] * U[7] + T2[8] * U[8];
+        float tt[3];
+        mv(T2, r2, tt);
+        r[0] -= tt[0];
+        r[1] -= tt[1];
+        r[2] -= tt[2];
+        float W[9];
+        mm(T2, U2, W);
+#pragma unroll
+        for (int q = 0; q < 9; q++) Un[q] = -W[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 9; q++) U[q] = Un[q];
+    }
+  }
+  // every reduced row now stands alone
+  float* B = lvl ? B1 : B0;
+  x[0] = x[1] = x[2] = 0.f;
+  if (row) {
+    float inv[6];
+    pivot_inv(D, inv);
+    smv(inv, r, x);
+    B[kRowF * t] = x[0];
+    B[kRowF * t + 1] = x[1];
+    B[kRowF * t + 2] = x[2];
+  }
+  __syncthreads();
+  xn[0] = xn[1] = xn[2] = 0.f;
+  if (t + 1 < R) {
+    xn[0] = B[kRowF * (t + 1)];
+    xn[1] = B[kRowF * (t + 1) + 1];
+    xn[2] = B[kRowF * (t + 1) + 2];
+  }
+}
+
 // Block tridiagonal solve of one line, two rows per thread.
 // Thread t holds rows a = 2t and b = 2t+1 (D sym, U full coupling to the next row, r) and
 // receives their solutions. The first odd-even level eliminates row b in registers (neighbours
-// exchange one 9-float term through LDS), leaving R2 = ceil(len/2) reduced rows, one per thread.
-// Those are solved by block parallel cyclic reduction: at level d every row eliminates both its
-// neighbours at distance d at once, so after ceil(log2 R2) levels (one barrier each, the row kept
-// in registers, LDS ping-ponged) each row stands alone; there is no back-substitution sweep. The
-// reduced system stays symmetric (row k's coupling to k - d is row k - d's coupling to k,
-// transposed), so a row carries only its forward coupling, and each row publishes its own shifted
-// pivot inverse with it (one inversion per row and level). cr: kSolveFloats * Lr floats of LDS.
+// exchange one 9-float term through LDS), leaving R2 = ceil(len/2) reduced rows, one per thread,
+// for pcr_rows; then b is recovered from a's and the next thread's solutions.
+// cr: kSolveFloats * Lr floats of LDS.
 __device__ __forceinline__ void line_solve(float* cr, int Lr, int t, int len, const float* Da_in, const float* Ua,
                                            const float* ra_in, const float* Db, const float* Ub, const float* rb,
                                            float xa[3], float xb[3]) {
@@ -373,93 +464,112 @@ __device__ __forceinline__ void line_solve(float* cr, int Lr, int t, int len, co
     D[0] -= e0.x; D[1] -= e0.y; D[2] -= e0.z; D[3] -= e0.w; D[4] -= e1.x; D[5] -= e1.y;
     r[0] -= e1.z; r[1] -= e1.w; r[2] -= e2.x;
   }
-  // B1's exchange terms are read before the first level's barrier; that level writes B0 and the
-  // next one B1, so one barrier per level orders every read before the overwrite.
-  const bool row = t < R2;
-  int lvl = 0;
-  for (int d = 1; d < R2; d <<= 1, lvl ^= 1) {
-    float* B = lvl ? B1 : B0;
-    if (row) {
-      float Di[6];
-      pivot_inv(D, Di);
-      put_row(B, t, Di, U, r);
-    }
-    __syncthreads();
-    if (row) {
-      float Un[9];
-#pragma unroll
-      for (int q = 0; q < 9; q++) Un[q] = 0.f;
-      const int j1 = t - d, j2 = t + d;
-      if (j1 >= 0) {
-        // row j1 couples to this row by U1: D -= U1^T D1^-1 U1, r -= U1^T D1^-1 r1
-        float Di[6], U1[9], r1[3], T[9];
-        get_row(B, j1, Di, U1, r1);
-        float Dif[9];
-        sym_full(Di, Dif);
-        mm(Dif, U1, T);  // T = D1^-1 U1
-        D[0] -= U1[0] * T[0] + U1[3] * T[3] + U1[6] * T[6];
-        D[1] -= U1[0] * T[1] + U1[3] * T[4] + U1[6] * T[7];
-        D[2] -= U1[0] * T[2] + U1[3] * T[5] + U1[6] * T[8];
-        D[3] -= U1[1] * T[1] + U1[4] * T[4] + U1[7] * T[7];
-        D[4] -= U1[1] * T[2] + U1[4] * T[5] + U1[7] * T[8];
-        D[5] -= U1[2] * T[2] + U1[5] * T[5] + U1[8] * T[8];
-        float tt[3];
-        mtv(T, r1, tt);  // U1^T D1^-1 r1
-        r[0] -= tt[0];
-        r[1] -= tt[1];
-        r[2] -= tt[2];
-      }
-      if (j2 < R2) {
-        // this row couples to row j2 by U, row j2 on to j2 + d by U2: D -= U D2^-1 U^T,
-        // r -= U D2^-1 r2, and the new coupling to j2 + d is -U D2^-1 U2
-        float Di[6], U2[9], r2[3], T2[9];
-        get_row(B, j2, Di, U2, r2);
-        float Dif[9];
-        sym_full(Di, Dif);
-        mm(U, Dif, T2);  // T2 = U D2^-1
-        D[0] -= T2[0] * U[0] + T2[1] * U[1] + T2[2] * U[2];
-        D[1] -= T2[0] * U[3] + T2[1] * U[4] + T2[2] * U[5];
-        D[2] -= T2[0] * U[6] + T2[1] * U[7] + T2[2] * U[8];
-        D[3] -= T2[3] * U[3] + T2[4] * U[4] + T2[5] * U[5];
-        D[4] -= T2[3] * U[6] + T2[4] * U[7] + T2[5] * U[8];
-        D[5] -= T2[6] * U[6] + T2[7] * U[7] + T2[8] * U[8];
-        float tt[3];
-        mv(T2, r2, tt);
-        r[0] -= tt[0];
-        r[1] -= tt[1];
-        r[2] -= tt[2];
-        float W[9];
-        mm(T2, U2, W);
-#pragma unroll
-        for (int q = 0; q < 9; q++) Un[q] = -W[q];
-      }
-#pragma unroll
-      for (int q = 0; q < 9; q++) U[q] = Un[q];
-    }
-  }
-  // every reduced row now stands alone
-  float* B = lvl ? B1 : B0;
-  xa[0] = xa[1] = xa[2] = 0.f;
-  if (row) {
-    float inv[6];
-    pivot_inv(D, inv);
-    smv(inv, r, xa);
-    B[kRowF * t] = xa[0];
-    B[kRowF * t + 1] = xa[1];
-    B[kRowF * t + 2] = xa[2];
-  }
-  __syncthreads();
-  float xn[3] = {0.f, 0.f, 0.f};
-  if (t + 1 < R2) {
-    xn[0] = B[kRowF * (t + 1)];
-    xn[1] = B[kRowF * (t + 1) + 1];
-    xn[2] = B[kRowF * (t + 1) + 2];
-  }
+  // B1's exchange terms are read before the first level's barrier (inside pcr_rows)
+  float xn[3];
+  pcr_rows(B0, B1, t, R2, D, U, r, xa, xn);
   float tt[3], q1[3];
   mtv(Ua, xa, tt);
   mv(Ub, xn, q1);
   float rr[3] = {rb[0] - tt[0] - q1[0], rb[1] - tt[1] - q1[1], rb[2] - tt[2] - q1[2]};
   smv(Dib, rr, xb);
+}
+
+// Eliminates row b (D_b, r_b; coupled to row a by U_a, to the next row by U_b) from the pair: row a
+// gains D_a -= U_a D_b^-1 U_a^T, r_a -= U_a D_b^-1 r_b and the coupling Un = -U_a D_b^-1 U_b to the
+// next row; the next row gains P = U_b^T D_b^-1 U_b (symmetric, 6) and p = U_b^T D_b^-1 r_b, returned
+// for the caller to subtract. Dib: D_b's shifted pivot inverse, kept for recovering x_b.
+__device__ __forceinline__ void eliminate_pair(float Da[6], float ra[3], const float Ua[9], const float Db[6],
+                                               const float Ub[9], const float rb[3], float Dib[6], float Un[9],
+                                               float P[6], float p[3]) {
+  float Dibf[9], yb[3], T[9], W[9], t3[3];
+  pivot_inv(Db, Dib);
+  sym_full(Dib, Dibf);
+  smv(Dib, rb, yb);
+  mm(Dibf, Ub, T);
+  mtm(Ub, T, W);
+  P[0] = W[0];
+  P[1] = 0.5f * (W[1] + W[3]);
+  P[2] = 0.5f * (W[2] + W[6]);
+  P[3] = W[4];
+  P[4] = 0.5f * (W[5] + W[7]);
+  P[5] = W[8];
+  mtv(Ub, yb, p);
+  float T2[9];
+  mm(Ua, Dibf, T2);
+  mmt(T2, Ua, W);
+  Da[0] -= W[0];
+  Da[1] -= 0.5f * (W[1] + W[3]);
+  Da[2] -= 0.5f * (W[2] + W[6]);
+  Da[3] -= W[4];
+  Da[4] -= 0.5f * (W[5] + W[7]);
+  Da[5] -= W[8];
+  mv(T2, rb, t3);
+  ra[0] -= t3[0];
+  ra[1] -= t3[1];
+  ra[2] -= t3[2];
+  mm(T2, Ub, W);
+#pragma unroll
+  for (int q = 0; q < 9; q++) Un[q] = -W[q];
+}
+
+// x_b = D_b^-1 (r_b - U_a^T x_a - U_b x_n): a row eliminated by eliminate_pair, recovered from the
+// solutions of its neighbours.
+__device__ __forceinline__ void recover_row(const float Dib[6], const float Ua[9], const float Ub[9], const float rb[3],
+                                            const float xa[3], const float xn[3], float xb[3]) {
+  float tt[3], q1[3];
+  mtv(Ua, xa, tt);
+  mv(Ub, xn, q1);
+  const float rr[3] = {rb[0] - tt[0] - q1[0], rb[1] - tt[1] - q1[1], rb[2] - tt[2] - q1[2]};
+  smv(Dib, rr, xb);
+}
+
+// Block tridiagonal solve of one line, four rows per thread: thread t holds rows 4t + m (m = 0..3,
+// a b c d; D sym, U full coupling to the next row, r; rows past the line are identity) and receives
+// their solutions. Two odd-even levels run in registers: b is eliminated into a and c, d into c and
+// the next thread's a, then c into a and the next thread's a. What a thread adds to the next
+// thread's a depends on its own rows only, so both levels' terms go through one LDS exchange. That
+// leaves R4 = ceil(len/4) reduced rows, one per thread, for pcr_rows: a quarter of the line's rows in
+// the parallel reduction, whose work grows as rows x levels, against half for line_solve. Then c, b
+// and d are recovered from a's and the next thread's solutions. With pivots shifted at each
+// elimination, the result equals line_solve's up to the shifts' different order and rounding.
+// cr: kSolveFloats * Lr floats of LDS.
+__device__ __forceinline__ void line_solve4(float* cr, int Lr, int t, int len, float (&D)[4][6], float (&U)[4][9],
+                                            float (&r)[4][3], float (&x)[4][3]) {
+  float* B0 = cr;
+  float* B1 = cr + kRowF * Lr;  // also carries the register levels' exchange (its first 9 floats)
+  const int R4 = (len + 3) >> 2;
+  // level 1: b into a (and c), d into c (and the next thread's a)
+  float Dib[6], Uac[9], Pb[6], pb[3];
+  eliminate_pair(D[0], r[0], U[0], D[1], U[1], r[1], Dib, Uac, Pb, pb);
+  float Did[6], Uce[9], Pd[6], pd[3];
+  eliminate_pair(D[2], r[2], U[2], D[3], U[3], r[3], Did, Uce, Pd, pd);
+#pragma unroll
+  for (int q = 0; q < 6; q++) D[2][q] -= Pb[q];
+#pragma unroll
+  for (int q = 0; q < 3; q++) r[2][q] -= pb[q];
+  // level 2: c into a (and the next thread's a); a then couples to the next thread's a by Uae
+  const float rc[3] = {r[2][0], r[2][1], r[2][2]};
+  float Dic[6], Uae[9], Pc[6], pc[3];
+  eliminate_pair(D[0], r[0], Uac, D[2], Uce, rc, Dic, Uae, Pc, pc);
+  if (4 * t < len && t + 1 < R4) {
+    float4* e = reinterpret_cast<float4*>(B1 + kRowF * t);
+    e[0] = make_float4(Pd[0] + Pc[0], Pd[1] + Pc[1], Pd[2] + Pc[2], Pd[3] + Pc[3]);
+    e[1] = make_float4(Pd[4] + Pc[4], Pd[5] + Pc[5], pd[0] + pc[0], pd[1] + pc[1]);
+    e[2] = make_float4(pd[2] + pc[2], 0.f, 0.f, 0.f);
+  }
+  __syncthreads();
+  if (t >= 1 && t < R4) {
+    const float4* e = reinterpret_cast<const float4*>(B1 + kRowF * (t - 1));
+    const float4 e0 = e[0], e1 = e[1], e2 = e[2];
+    D[0][0] -= e0.x; D[0][1] -= e0.y; D[0][2] -= e0.z; D[0][3] -= e0.w; D[0][4] -= e1.x; D[0][5] -= e1.y;
+    r[0][0] -= e1.z; r[0][1] -= e1.w; r[0][2] -= e2.x;
+  }
+  // B1's exchange terms are read before the first level's barrier (inside pcr_rows)
+  float xe[3];
+  pcr_rows(B0, B1, t, R4, D[0], Uae, r[0], x[0], xe);
+  recover_row(Dic, Uac, Uce, rc, x[0], xe, x[2]);
+  recover_row(Dib, U[0], U[1], r[1], x[0], x[2], x[1]);
+  recover_row(Did, U[2], U[3], r[3], x[2], xe, x[3]);
 }
 
 __device__ __forceinline__ void set_identity(float D[6], float U[9], float r[3]) {

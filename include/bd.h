@@ -341,12 +341,15 @@ int bd_set_floor(bd_solver* s, int enabled, double z);
 #define BD_SMOOTHER_STAGED8 8  /* rows gathered into LDS, 8 threads per row */
 #define BD_SMOOTHER_SPLIT 16   /* a chip-wide row kernel, then one workgroup per line solves (levels
                                   with lines longer than 257 nodes) */
+#define BD_SMOOTHER_SPLIT4 17  /* the split variant whose solve holds four rows per thread: two odd-even
+                                  levels in registers before the parallel cyclic reduction */
 
 /* Autotuner (no counterpart in the reference): for every level's v-line and u-line passes of the
  * current hierarchy, the smoother kernel variant (BD_SMOOTHER_*) whose V-cycles run fastest, chosen
  * greedily level by level with each variant timed inside whole V-cycles (zero b and x; the fastest
  * of `reps`). chosen receives 2 entries per level (u, v); may be NULL. Results of a pass differ
- * between variants only by FP32 summation order. */
+ * between variants only by FP32 summation order, and for BD_SMOOTHER_SPLIT4 by its elimination order
+ * (the same shifted-pivot line solve, rows reduced in another order). */
 int bd_autotune(bd_solver* s, int reps, int32_t* chosen);
 
 /* Pins the smoother kernel variant of one level's u-line (family 0) or v-line (family 1) passes

@@ -161,8 +161,8 @@ def test_every_smoother_variant_keeps_parity():
     """Each smoother kernel variant a level can run (bd_set_smoother), pinned on every level at once:
     one sweep, the V-cycle and PCG meet the same bars against the oracle on a 300 x 300 sheet, whose
     fine lines (300 nodes) take the direct, split and staged kernels."""
-    from bd.capi import (BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT, BD_SMOOTHER_STAGED4, BD_SMOOTHER_STAGED8,
-                         BD_SMOOTHER_DEFAULT)
+    from bd.capi import (BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT, BD_SMOOTHER_SPLIT4, BD_SMOOTHER_STAGED4,
+                         BD_SMOOTHER_STAGED8, BD_SMOOTHER_DEFAULT)
     from bd.solver import GpuSolver
     n = 300
     pitch = 1e-3
@@ -192,7 +192,7 @@ def test_every_smoother_variant_keeps_parity():
     xv_ref = cpu.vcycle(b)
     xp_ref = cpu.pcg(b, params.pcg_iterations)
     nlev = len(gpu.stats()["level_nodes"])
-    for variant in (BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT, BD_SMOOTHER_STAGED4, BD_SMOOTHER_STAGED8):
+    for variant in (BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT, BD_SMOOTHER_SPLIT4, BD_SMOOTHER_STAGED4, BD_SMOOTHER_STAGED8):
         pinned = []
         for k in range(nlev):
             for fam in (0, 1):
@@ -201,7 +201,7 @@ def test_every_smoother_variant_keeps_parity():
                     pinned.append((k, fam))
                 except RuntimeError:
                     gpu.set_smoother(k, fam, BD_SMOOTHER_DEFAULT)
-        if variant == BD_SMOOTHER_SPLIT:
+        if variant in (BD_SMOOTHER_SPLIT, BD_SMOOTHER_SPLIT4):
             assert (0, 0) in pinned and (0, 1) in pinned, "the fine level keeps a rows buffer"
         say(f"variant {variant}: pinned on {pinned}")
         gpu.write(BD_BUF_LEVEL_B, b, 0)
