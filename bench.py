@@ -78,7 +78,8 @@ def parse():
                          "against one graph launch for the batch")
     ap.add_argument("--batch-threads", type=int, default=16,
                     help="host threads of the patches' concurrent batch (grid+threads)")
-    ap.add_argument("--sweeps", type=int, default=1, help="pre- and post-smoothing sweeps per level")
+    ap.add_argument("--sweeps", type=int, default=1,
+                    help="pre- and post-smoothing sweeps per level of the mixed and learned methods (plain: 1)")
     ap.add_argument("--mixed-coarsest", type=int, default=MIXED_COARSEST_NODES, metavar="NODES",
                     help="the mixed and learned methods stop the V-cycle at the first level of at most NODES nodes "
                          "and solve it exactly")
@@ -186,7 +187,8 @@ def method_params(cfg, method, args):
     import copy
     from bd.capi import BD_COARSE_EXACT, BD_COARSE_SWEEPS, BD_PRECISION_FP16, BD_PRECISION_FP32
     params = copy.copy(cfg["params"])
-    params.smooth_sweeps = args.sweeps
+    # the reference's schedule smooths once per level; the mixed and learned methods take --sweeps
+    params.smooth_sweeps = 1 if method == "plain" else args.sweeps
     if method == "plain":
         params.coarse_solve = BD_COARSE_SWEEPS
         params.max_coarsest_nodes = 400
@@ -251,6 +253,7 @@ def run_method(args, cfg, method, profile_frames):
     t1 = time.perf_counter()
     st = gpu.stats()
     out = {"ms": 1e3 * (t1 - t0) / args.steps, "iters": float(np.mean(iters)), "newton": float(np.mean(newton)),
+           "iters_list": [int(i) for i in iters],
            "max_rel_residual": float(max(relres)), "train": train, "learned_fallbacks": st["learned_fallbacks"],
            "levels": gpu.stats()["level_nodes"], "tuned": tuned, "precision": params.precision,
            "sweeps": params.smooth_sweeps}
@@ -453,7 +456,7 @@ def main():
         "methods": {k: {"ms_per_frame": round(v["ms"], 3), "pcg_iterations_per_frame": v["iters"],
                         "smooth_sweeps": v["sweeps"], "vcycle_operators": "fp16" if v["precision"] else "fp32",
                         "max_rel_residual": v["max_rel_residual"], "learned_fallbacks": v["learned_fallbacks"],
-                        "training": v["train"],
+                        "training": v["train"], "pcg_iterations_each_frame": v["iters_list"],
                         "smoother_kernels": {str(lv): list(c) for lv, c in v["tuned"].items()}}
                     for k, v in runs.items()},
         "iteration_ratio_plain_over_learned": ratio(runs, "plain", "learned", "iters"),

@@ -266,14 +266,18 @@ def test_max_speed_counter(coupon_pair):
     assert abs(one - last1) <= 1e-5 * last1
 
 
-@pytest.mark.parametrize("max_coarsest", [400, 1100])
-def test_exact_coarse_solve(coupon_pair, max_coarsest):
+@pytest.mark.parametrize("max_coarsest,fp16", [(400, False), (1100, False), (1100, True)])
+def test_exact_coarse_solve(coupon_pair, max_coarsest, fp16):
     """BD_COARSE_EXACT: the GPU's FP64 banded Cholesky + dense inverse of the coarsest level in the
     V-cycle and PCG, against the oracle's banded Cholesky solve (bars as above). Coarsest 13 x 11
-    factors in the LDS window, 25 x 21 (bandwidth 158) in place in global memory."""
+    factors in the LDS window, 25 x 21 (bandwidth 158) in place in global memory; and the 25 x 21
+    exact solve under FP16 coarse-level operators (BD_PRECISION_FP16), the oracle rounding them the
+    same way."""
+    from bd.capi import BD_PRECISION_FP16, BD_PRECISION_FP32
     from bd.solver import GpuSolver
     S, cpu, _ = coupon_pair
-    params = make_params(coarse_solve=1, max_coarsest_nodes=max_coarsest)
+    params = make_params(coarse_solve=1, max_coarsest_nodes=max_coarsest,
+                         precision=BD_PRECISION_FP16 if fp16 else BD_PRECISION_FP32)
     gpu = GpuSolver(S, params)
     copy_state(cpu, gpu)
     ref = O.Oracle(S, params)
