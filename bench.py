@@ -281,6 +281,8 @@ def graph_bench(args):
       grid+threads    (the patches) the grid path's eager substeps on each patch's own stream, driven by
                       a pool of host threads (bd_batch_create_concurrent), so that the patches'
                       dependent launches overlap on the chip;
+      grid+graph+threads (the patches) the captured substeps replayed on each patch's own stream from the
+                      concurrent batch's host threads;
       grid+graph      the substep captured in a graph (the PCG budget per Newton solve from the warm-up:
                       iterations past convergence are launched and return on the device's flag); the
                       patches as the branches of one graph launch;
@@ -300,7 +302,7 @@ def graph_bench(args):
         cfgs = config_patches()["patches"]
         name = (f"patches48 ({len(cfgs)} uneven patches, sides {min(p['side'] for p in cfgs)}-{max(p['side'] for p in cfgs)},"
                 f" {sum(p['side'] ** 2 for p in cfgs)} nodes, each 3 Newton x MG-PCG to rel-res 1e-6, random grip paths)")
-        methods = ("grid", "grid+threads", "grid+graph", "workgroup")
+        methods = ("grid", "grid+threads", "grid+graph", "grid+graph+threads", "workgroup")
 
     def frame(group, step):
         for g, grips, vel in group:
@@ -334,15 +336,15 @@ def graph_bench(args):
                 st = g.stats()
                 most[i] = max(most[i], max(st["pcg_iterations"][:st["newton_iterations"]]))
         batch, step = None, (lambda g: g.encode_substep())
-        if m.endswith("graph"):
+        if "graph" in m:
             caps = [graph_budget(k, c["params"].pcg_max_iterations) for k, c in zip(most, cfgs)]
-            if m == "grid+graph":
+            if m.startswith("grid+graph"):
                 budgets = caps
             for (g, _, _), cap in zip(group, caps):
                 g.graph_capture(cap)
             step = (lambda g: g.graph_substep()) if single else None
         if not single and m != "grid":
-            batch = Batch([g for g, _, _ in group], threads=args.batch_threads if m == "grid+threads" else None)
+            batch = Batch([g for g, _, _ in group], threads=args.batch_threads if m.endswith("threads") else None)
             step = None
         for g, _, _ in group:
             g.sync()

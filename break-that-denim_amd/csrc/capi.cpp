@@ -1932,7 +1932,8 @@ int bd_batch_substep(bd_batch* b) {
           const int k = next.fetch_add(1);
           if (k >= (int)B.order.size()) return;
           const int i = B.order[k];
-          const int r = bd_substep(B.solvers[i], 0, 0);
+          // a solver with a captured substep replays it, the others run it eagerly
+          const int r = B.solvers[i]->s.graph_exec ? bd_graph_substep(B.solvers[i]) : bd_substep(B.solvers[i], 0, 0);
           if (r != BD_OK) {
             std::lock_guard<std::mutex> lock(mu);
             if (rc == BD_OK) {

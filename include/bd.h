@@ -291,14 +291,14 @@ int bd_graph_release(bd_solver* s);
  * BD_ERR_STATE. No counterpart in the reference, which runs one scene per solver. */
 typedef struct bd_batch bd_batch;
 int bd_batch_create(bd_solver* const* solvers, int count, bd_batch** out);
-/* A third form, for solvers on the grid path in any configuration: bd_batch_substep runs each
- * solver's eager substep (bd_substep with no extra Newton bounds, the host checking each PCG
- * iteration) on the solver's own stream, from `threads` host threads (0: min(count, 8, the host's
- * cores)) that take the solvers largest first, and returns when every substep has been issued and its
- * convergence checks have ended. The systems' dependent launches then overlap on the chip instead of
- * queueing behind one another. Results are bit-identical to calling bd_substep on each solver in
- * turn. The solvers must not be used from other threads meanwhile; the first failing solver's error
- * is reported (the others still run). */
+/* A third form, for solvers in any configuration: bd_batch_substep runs each solver's substep on the
+ * solver's own stream, from `threads` host threads (0: min(count, 8, the host's cores)) that take the
+ * solvers largest first, and returns when every substep has been issued (and, for an eager one, its
+ * convergence checks have ended). A solver with a captured substep replays it (bd_graph_substep); the
+ * others run bd_substep with no extra Newton bounds. The systems' dependent launches then overlap on
+ * the chip instead of queueing behind one another. Results are bit-identical to calling
+ * bd_graph_substep or bd_substep on each solver in turn. The solvers must not be used from other
+ * threads meanwhile; the first failing solver's error is reported (the others still run). */
 int bd_batch_create_concurrent(bd_solver* const* solvers, int count, int threads, bd_batch** out);
 int bd_batch_substep(bd_batch* b);
 int bd_batch_sync(bd_batch* b);

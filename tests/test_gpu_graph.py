@@ -10,9 +10,9 @@
   the capture refuses what needs the host between Newton iterations;
 * a batch of uneven patches (each its own Newton / MG-PCG system) advanced by one graph launch per
   frame equals each patch's eager substeps bit for bit;
-* the concurrent batch (each patch's eager substep on its own stream from a pool of host threads)
-  equals the patches' eager substeps run one after another bit for bit, with self-contact on some
-  patches.
+* the concurrent batch (each patch's substep on its own stream from a pool of host threads: eager, or
+  the replay of a captured one) equals the patches' eager substeps run one after another bit for bit,
+  with self-contact on some patches.
 """
 import sys
 
@@ -177,6 +177,14 @@ def test_concurrent_batch_matches_eager_substeps():
         eager.append(GpuSolver(p["scene"], params, ga))
         batched.append(GpuSolver(p["scene"], params, gb))
         grips.append((ga, gb))
+    # every third patch replays a captured substep (budget = the eager limit, so bit-exact)
+    for i, b in enumerate(batched):
+        if i % 3 == 2:
+            b.encode_substep()
+            b.graph_capture(cfg["patches"][i]["params"].pcg_max_iterations)
+    for i, a in enumerate(eager):
+        if i % 3 == 2:
+            a.encode_substep()
     batch = Batch(batched, threads=3)
     for k in range(4):
         for p, a, (ga, gb) in zip(cfg["patches"], eager, grips):
