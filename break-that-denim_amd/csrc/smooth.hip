@@ -281,6 +281,10 @@ size_t staged_shm(int len) {
   return sizeof(float) * ((size_t)kSolveFloats * Lr + (size_t)(6 + 9 + 3 + 3 + 4 * G) * len) + (size_t)len + 16;
 }
 
+// Threads of a staged pass: one wave per gather item (64 consecutive rows x one of G coupling parts),
+// so that the whole gather is one round of loads, up to 1024 threads.
+inline int staged_threads(int len, int G) { return std::min(1024, ((len + 63) / 64) * G * 64); }
+
 // LDS a workgroup may use (the staged kernels' attribute is raised to it for the autotuner)
 constexpr int kMaxLds = 160 * 1024;
 
@@ -307,7 +311,7 @@ void launch_one(const SmoothArgs& S, RDD rdd, int count, int len, hipStream_t st
   if (variant == 4 || variant == 8) {
     const int G = variant;
     const int R2 = (len + 1) / 2;
-    const int threads = std::max(std::min(1024, ((len * G + 63) / 64) * 64), ((R2 + 63) / 64) * 64);
+    const int threads = std::max(staged_threads(len, G), ((R2 + 63) / 64) * 64);
     if (G == 8)
       k_smooth_staged<FINE, FAM, 8, DUAL, RDD><<<count, threads, staged_shm<FINE, FAM, 8>(len), st>>>(S, rdd);
     else
@@ -317,9 +321,8 @@ void launch_one(const SmoothArgs& S, RDD rdd, int count, int len, hipStream_t st
   if (variant < 0 && len <= kStagedMaxLen) {
     // widest gather that fits 1024 threads: couplings per thread ~ (FINE ? 12 : 24) / G
     const int G = len <= 128 ? 8 : 4;
-    int threads = std::min(1024, ((len * G + 63) / 64) * 64);
     const int R2 = (len + 1) / 2;
-    threads = std::max(threads, ((R2 + 63) / 64) * 64);
+    const int threads = std::max(staged_threads(len, G), ((R2 + 63) / 64) * 64);
     if (G == 8)
       k_smooth_staged<FINE, FAM, 8, DUAL, RDD><<<count, threads, staged_shm<FINE, FAM, 8>(len), st>>>(S, rdd);
     else
