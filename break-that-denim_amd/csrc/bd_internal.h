@@ -79,6 +79,12 @@ struct Level {
   // pass, 18 planes (D 6, U 9, r 3) of rows_cap = lines per colour x longest line
   float* rows = nullptr;
   size_t rows_cap = 0;
+  // line blocks of every line of the level (BD_SMOOTHER_SPLIT4 reads them): per row D (6) and U (9)
+  // as the gather assembles them, which the iterate does not change, refreshed whenever the level's
+  // operator is final; 15 planes of du_cap = lines of both families x the line stride
+  float* du = nullptr;
+  size_t du_cap = 0;
+  int du_stride = 0;
   // BD_PRECISION_FP16: op (and opT) as FP16 pairs, comps / 2 u32 planes (oprd.h); hs[0] the scale,
   // hs[1] its inverse, hs[2] the bits of the largest free diagonal entry
   uint32_t* oph = nullptr;
@@ -332,6 +338,7 @@ void launch_prolong_add(Solver& s, int k);
 void launch_smooth(Solver& s, int k, int reverse);
 std::vector<int> smooth_variants(Solver& s, int k, int fam);
 void smooth_alloc_rows(Solver& s, Level& L);
+void smooth_line_blocks(Solver& s, int k);
 void launch_zero(Solver& s, float* p, size_t n);
 // column-major copies of a dual-layout level: operator and fixed flags, or the iterate
 void launch_colmajor_op(Solver& s, int k);

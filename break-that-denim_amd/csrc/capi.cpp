@@ -218,6 +218,7 @@ void destroy_all(Solver& s) {
     dfree(L.xT);
     dfree(L.bT);
     dfree(L.rows);
+    dfree(L.du);
     dfree(L.oph);
     dfree(L.ophT);
     dfree(L.hs);

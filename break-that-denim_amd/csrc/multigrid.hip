@@ -389,6 +389,7 @@ void launch_colmajor_op(Solver& s, int k) {
     k_pack_half<<<(unsigned)((total + 255) / 256), 256, 0, s.stream>>>(L.op, words, L.Np, L.hs, L.oph);
     if (L.dual) k_pack_half<<<(unsigned)((total + 255) / 256), 256, 0, s.stream>>>(L.opT, words, L.Np, L.hs, L.ophT);
   }
+  if (L.du) smooth_line_blocks(s, k);
 }
 
 void launch_colmajor_x(Solver& s, int k) {

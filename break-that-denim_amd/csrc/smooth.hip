@@ -227,10 +227,55 @@ __global__ __launch_bounds__(512, 4) void k_smooth_rows(SmoothArgs S, const floa
   if (!fb) store_x(S, nb, nob, xb, DUAL);
 }
 
-// Split variant with four rows per thread: the rows k_line_rows wrote (stride a multiple of 4, so
-// that a thread's four rows of a plane are one aligned float4), solved by line_solve4.
+// Split variant with four rows per thread, first kernel: as k_line_rows, but only each row's
+// right-hand side r (3 planes): the line blocks D and U do not change with the iterate, so the solve
+// kernel reads them from the operator itself and the round trip of 15 of the 18 floats is saved.
+template <bool FINE, int FAM, bool DUAL, class RDD>
+__global__ __launch_bounds__(256) void k_line_r(SmoothArgs S, RDD rdd, float* rows, size_t cap, int stride) {
+  if (S.gate && *S.gate) return;
+  const auto rd = rdd.live();
+  const int slot = blockIdx.y;
+  const LineGeo L = line_geo_at<FAM, DUAL>(S, slot);
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= L.ln.z) return;
+  int n, nb, no, i, j;
+  row_index<FAM, DUAL>(L, k, n, nb, no, i, j);
+  float D[6], U[9], r[3];
+  bool f;
+  gather_row<FINE, FAM, DUAL, decltype(rd), kGatherR>(S, rd, n, nb, i, j, L.nx, L.ny, D, U, r, f);
+  float* o = rows + (size_t)slot * stride + k;
+#pragma unroll
+  for (int q = 0; q < 3; q++) o[q * cap] = r[q];
+}
+
+// The line blocks D (6) and U (9) of every row of the lines of one family (blockIdx.y the line within
+// the family, x the 256-row chunk), as gather_row assembles them, to planes at (line0 + line) *
+// stride + row of du. Launched when the level's operator is final, outside PCG (no gate).
+template <bool FINE, int FAM, bool DUAL, class RDD>
+__global__ __launch_bounds__(256) void k_line_du(SmoothArgs S, RDD rdd, float* du, size_t cap, int stride, int line0) {
+  const auto rd = rdd.live();
+  const int slot = blockIdx.y;
+  const LineGeo L = line_geo_at<FAM, DUAL>(S, slot);
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= L.ln.z) return;
+  int n, nb, no, i, j;
+  row_index<FAM, DUAL>(L, k, n, nb, no, i, j);
+  float D[6], U[9], r[3];
+  bool f;
+  gather_row<FINE, FAM, DUAL, decltype(rd), kGatherDU>(S, rd, n, nb, i, j, L.nx, L.ny, D, U, r, f);
+  float* o = du + (size_t)(line0 + slot) * stride + k;
+#pragma unroll
+  for (int q = 0; q < 6; q++) o[q * cap] = D[q];
+#pragma unroll
+  for (int q = 0; q < 9; q++) o[(6 + q) * cap] = U[q];
+}
+
+// Split variant with four rows per thread, second kernel: r from k_line_r and D, U from the level's
+// line blocks (k_line_du), for each (stride a multiple of 4, so that a thread's four rows of a plane
+// are one aligned float4), solved by line_solve4.
 template <bool FINE, int FAM, bool DUAL>
-__global__ __launch_bounds__(256) void k_smooth_rows4(SmoothArgs S, const float* rows, size_t cap, int stride) {
+__global__ __launch_bounds__(256) void k_smooth_rows4(SmoothArgs S, const float* rows, size_t cap, int stride,
+                                                      const float* du, size_t du_cap, int du_stride, int line0) {
   if (S.gate && *S.gate) return;
   extern __shared__ __align__(16) float sm[];
   const int slot = xcd_block(blockIdx.x, gridDim.x);
@@ -239,10 +284,12 @@ __global__ __launch_bounds__(256) void k_smooth_rows4(SmoothArgs S, const float*
   const int t = threadIdx.x;
   const int k0 = 4 * t;
   const float* o = rows + (size_t)slot * stride;
+  const float* od = du + (size_t)(line0 + slot) * du_stride;
   float v[18][4];
 #pragma unroll
   for (int q = 0; q < 18; q++) {
-    const float4 p = k0 < len ? *reinterpret_cast<const float4*>(o + q * cap + k0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float* src = q < 15 ? od + q * du_cap : o + (q - 15) * cap;
+    const float4 p = k0 < len ? *reinterpret_cast<const float4*>(src + k0) : make_float4(0.f, 0.f, 0.f, 0.f);
     v[q][0] = p.x;
     v[q][1] = p.y;
     v[q][2] = p.z;
@@ -291,17 +338,23 @@ constexpr int kMaxLds = 160 * 1024;
 // variant: -1 the default rule below, 0 the direct kernel, 4 or 8 the staged kernel with that gather
 // width (bd_autotune picks per level and family)
 template <bool FINE, int FAM, bool DUAL, class RDD>
-void launch_one(const SmoothArgs& S, RDD rdd, int count, int len, hipStream_t st, int variant, float* rows, size_t cap) {
+void launch_one(const SmoothArgs& S, RDD rdd, int count, int len, hipStream_t st, int variant, const Level& Lv, int line0) {
+  float* rows = Lv.rows;
+  const size_t cap = Lv.rows_cap;
   if (variant == kSplitVariant || variant == kSplit4Variant) {
     const bool four = variant == kSplit4Variant;
     // a thread's rows of a plane are one aligned float2 (two rows) or float4 (four)
     const int stride = four ? (len + 3) & ~3 : (len + 1) & ~1;
-    if (!rows || (size_t)count * stride > cap || cap % 4 != 0)
+    if (!rows || (size_t)count * stride > cap || cap % 4 != 0 || (four && (!Lv.du || Lv.du_stride < stride)))
       throw std::runtime_error("smoother: split variant without room for its rows");
-    k_line_rows<FINE, FAM, DUAL, RDD><<<dim3((len + 255) / 256, count), 256, 0, st>>>(S, rdd, rows, cap, stride);
+    if (four)
+      k_line_r<FINE, FAM, DUAL, RDD><<<dim3((len + 255) / 256, count), 256, 0, st>>>(S, rdd, rows, cap, stride);
+    else
+      k_line_rows<FINE, FAM, DUAL, RDD><<<dim3((len + 255) / 256, count), 256, 0, st>>>(S, rdd, rows, cap, stride);
     if (four) {
       const int block = ((len + 3) / 4 + 63) / 64 * 64;
-      k_smooth_rows4<FINE, FAM, DUAL><<<count, block, (size_t)kSolveFloats * block * sizeof(float), st>>>(S, rows, cap, stride);
+      k_smooth_rows4<FINE, FAM, DUAL><<<count, block, (size_t)kSolveFloats * block * sizeof(float), st>>>(
+          S, rows, cap, stride, Lv.du, Lv.du_cap, Lv.du_stride, line0);
     } else {
       const int block = ((len + 1) / 2 + 63) / 64 * 64;
       k_smooth_rows<FINE, FAM, DUAL><<<count, block, (size_t)kSolveFloats * block * sizeof(float), st>>>(S, rows, cap, stride);
@@ -404,19 +457,19 @@ void launch_colour(Solver& s, int k, int c, int variant) {
     using R = decltype(rdd);
     if (L.dual) {
       if (L.fine) {
-        if (fam == 0) launch_one<true, 0, true, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
-        else launch_one<true, 1, true, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+        if (fam == 0) launch_one<true, 0, true, R>(S, rdd, count, maxlen, s.stream, variant, L, L.color_start[c]);
+        else launch_one<true, 1, true, R>(S, rdd, count, maxlen, s.stream, variant, L, L.color_start[c]);
       } else {
-        if (fam == 0) launch_one<false, 0, true, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
-        else launch_one<false, 1, true, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+        if (fam == 0) launch_one<false, 0, true, R>(S, rdd, count, maxlen, s.stream, variant, L, L.color_start[c]);
+        else launch_one<false, 1, true, R>(S, rdd, count, maxlen, s.stream, variant, L, L.color_start[c]);
       }
     } else {
       if (L.fine) {
-        if (fam == 0) launch_one<true, 0, false, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
-        else launch_one<true, 1, false, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+        if (fam == 0) launch_one<true, 0, false, R>(S, rdd, count, maxlen, s.stream, variant, L, L.color_start[c]);
+        else launch_one<true, 1, false, R>(S, rdd, count, maxlen, s.stream, variant, L, L.color_start[c]);
       } else {
-        if (fam == 0) launch_one<false, 0, false, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
-        else launch_one<false, 1, false, R>(S, rdd, count, maxlen, s.stream, variant, L.rows, L.rows_cap);
+        if (fam == 0) launch_one<false, 0, false, R>(S, rdd, count, maxlen, s.stream, variant, L, L.color_start[c]);
+        else launch_one<false, 1, false, R>(S, rdd, count, maxlen, s.stream, variant, L, L.color_start[c]);
       }
     }
   };
@@ -457,6 +510,52 @@ void smooth_alloc_rows(Solver& s, Level& L) {
   for (int c = 0; c < 2 * L.ncol; c++) most = std::max(most, L.color_start[c + 1] - L.color_start[c]);
   L.rows_cap = (size_t)most * ((L.max_len + 3) & ~3);  // room for either split variant's line stride
   BD_HIP(hipMalloc(&L.rows, 18 * L.rows_cap * sizeof(float)));
+  L.du_stride = (L.max_len + 3) & ~3;
+  L.du_cap = (size_t)L.color_start[2 * L.ncol] * L.du_stride;
+  BD_HIP(hipMalloc(&L.du, 15 * L.du_cap * sizeof(float)));
+}
+
+// The line blocks of every line of level k (k_line_du per family), once its operator is final.
+void smooth_line_blocks(Solver& s, int k) {
+  Level& L = s.lv[k];
+  if (!L.du) return;
+  for (int fam = 1; fam >= 0; fam--) {
+    // v-line colours come first in the table
+    const int c0 = fam == 1 ? 0 : L.ncol, c1 = c0 + L.ncol;
+    const int count = L.color_start[c1] - L.color_start[c0];
+    if (count == 0) continue;
+    int maxlen = 0;
+    for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
+    const bool colmajor = L.dual && fam == 1;
+    SmoothArgs S{colmajor ? L.opT : L.op, colmajor ? L.fixedT : L.fixed, L.Np, L.lines + L.color_start[c0],
+                 colmajor ? L.xT : L.x, colmajor ? L.bT : L.b, nullptr, nullptr, L.geo};
+    const dim3 grid((maxlen + 255) / 256, count);
+    const int line0 = L.color_start[c0];
+    auto go = [&](auto rdd) {
+      using R = decltype(rdd);
+      if (L.fine) {
+        if (fam == 0) {
+          if (L.dual) k_line_du<true, 0, true, R><<<grid, 256, 0, s.stream>>>(S, rdd, L.du, L.du_cap, L.du_stride, line0);
+          else k_line_du<true, 0, false, R><<<grid, 256, 0, s.stream>>>(S, rdd, L.du, L.du_cap, L.du_stride, line0);
+        } else {
+          if (L.dual) k_line_du<true, 1, true, R><<<grid, 256, 0, s.stream>>>(S, rdd, L.du, L.du_cap, L.du_stride, line0);
+          else k_line_du<true, 1, false, R><<<grid, 256, 0, s.stream>>>(S, rdd, L.du, L.du_cap, L.du_stride, line0);
+        }
+      } else {
+        if (fam == 0) {
+          if (L.dual) k_line_du<false, 0, true, R><<<grid, 256, 0, s.stream>>>(S, rdd, L.du, L.du_cap, L.du_stride, line0);
+          else k_line_du<false, 0, false, R><<<grid, 256, 0, s.stream>>>(S, rdd, L.du, L.du_cap, L.du_stride, line0);
+        } else {
+          if (L.dual) k_line_du<false, 1, true, R><<<grid, 256, 0, s.stream>>>(S, rdd, L.du, L.du_cap, L.du_stride, line0);
+          else k_line_du<false, 1, false, R><<<grid, 256, 0, s.stream>>>(S, rdd, L.du, L.du_cap, L.du_stride, line0);
+        }
+      }
+    };
+    if (L.oph)
+      go(OpF16Dev{colmajor ? L.ophT : L.oph, L.Np, L.hs});
+    else
+      go(OpF32{colmajor ? L.opT : L.op, L.Np});
+  }
 }
 
 // Smooth all colours of level k (v-line colours first, then u-line colours; reversed for the

@@ -96,9 +96,14 @@ __device__ constexpr Cpl kFine[12] = {
     {1, 1, 0, 0, F_NE, false},  {-1, -1, -1, -1, F_NE, false}, {-1, 1, 0, 0, F_NW, false}, {1, -1, 1, -1, F_NW, false},
     {2, 0, 0, 0, F_EE, true},   {-2, 0, -2, 0, F_EE, true}, {0, 2, 0, 0, F_NN, true},    {0, -2, 0, -2, F_NN, true}};
 
-// Gathers D (6), U (9: +1 along coupling, full), r (3) of row n on a line of family FAM.
+// What gather_row computes: the whole row, only its right-hand side r (which changes with the
+// iterate every pass), or only its line blocks D and U (which do not: the operator's own rows).
+constexpr int kGatherAll = 0, kGatherR = 1, kGatherDU = 2;
+
+// Gathers D (6), U (9: +1 along coupling, full), r (3) of row n on a line of family FAM (PART: all of
+// them, r only, or D and U only; the parts it computes are the same bits in every mode).
 // All loads use clamped in-bounds indices and are masked afterwards, so they issue together.
-template <bool FINE, int FAM, bool DUAL, class RD>
+template <bool FINE, int FAM, bool DUAL, class RD, int PART = kGatherAll>
 __device__ __forceinline__ void gather_row(const SmoothArgs& S, const RD& rd, int n, int nb, int i, int j, int nx, int ny,
                                            float D[6], float U[9], float r[3], bool& fixed_n) {
   const int Np = S.Np;
@@ -106,11 +111,16 @@ __device__ __forceinline__ void gather_row(const SmoothArgs& S, const RD& rd, in
   const int si = stride_i<FAM, DUAL>(nx, ny), sj = stride_j<FAM, DUAL>(nx, ny);
   fixed_n = S.fixed[n] != 0;
   float c[6];
-  rd.blk(F_C, n, c);
-  const float xn0 = x[n], xn1 = x[Np + n], xn2 = x[2 * Np + n];
-  r[0] = S.b[n];
-  r[1] = S.b[Np + n];
-  r[2] = S.b[2 * Np + n];
+  if (PART != kGatherR) rd.blk(F_C, n, c);
+  float xn0 = 0.f, xn1 = 0.f, xn2 = 0.f;
+  if (PART != kGatherDU) {
+    xn0 = x[n];
+    xn1 = x[Np + n];
+    xn2 = x[2 * Np + n];
+    r[0] = S.b[n];
+    r[1] = S.b[Np + n];
+    r[2] = S.b[2 * Np + n];
+  }
 #pragma unroll
   for (int q = 0; q < 9; q++) U[q] = 0.f;
   float maj = 0.f;
@@ -133,16 +143,20 @@ __device__ __forceinline__ void gather_row(const SmoothArgs& S, const RD& rd, in
     const int along = FAM == 0 ? di : dj;
     const int across = FAM == 0 ? dj : di;
     if (across == 0 && along == 1) {
-      float full[9];
-      sym_full(blk, full);
+      if (PART != kGatherR) {
+        float full[9];
+        sym_full(blk, full);
 #pragma unroll
-      for (int q = 0; q < 9; q++) U[q] = w * full[q];
+        for (int q = 0; q < 9; q++) U[q] = w * full[q];
+      }
     } else if (!(across == 0 && along == -1)) {
-      float t[3];
-      smv(blk, xm, t);
-      r[0] -= w * t[0];
-      r[1] -= w * t[1];
-      r[2] -= w * t[2];
+      if (PART != kGatherDU) {
+        float t[3];
+        smv(blk, xm, t);
+        r[0] -= w * t[0];
+        r[1] -= w * t[1];
+        r[2] -= w * t[2];
+      }
       if (across == 0) maj += w * sym_frob(blk);  // +-2 along the line
     }
   };
@@ -158,22 +172,30 @@ __device__ __forceinline__ void gather_row(const SmoothArgs& S, const RD& rd, in
     }
   }
   if (fixed_n) {
-    D[0] = 1.f; D[1] = 0.f; D[2] = 0.f; D[3] = 1.f; D[4] = 0.f; D[5] = 1.f;
+    if (PART != kGatherR) {
+      D[0] = 1.f; D[1] = 0.f; D[2] = 0.f; D[3] = 1.f; D[4] = 0.f; D[5] = 1.f;
 #pragma unroll
-    for (int q = 0; q < 9; q++) U[q] = 0.f;
-    r[0] = S.b[n];
-    r[1] = S.b[Np + n];
-    r[2] = S.b[2 * Np + n];
+      for (int q = 0; q < 9; q++) U[q] = 0.f;
+    }
+    if (PART != kGatherDU) {
+      r[0] = S.b[n];
+      r[1] = S.b[Np + n];
+      r[2] = S.b[2 * Np + n];
+    }
     return;
   }
+  if (PART != kGatherR) {
 #pragma unroll
-  for (int q = 0; q < 6; q++) D[q] = c[q];
-  D[0] += maj;
-  D[3] += maj;
-  D[5] += maj;
-  r[0] += maj * xn0;
-  r[1] += maj * xn1;
-  r[2] += maj * xn2;
+    for (int q = 0; q < 6; q++) D[q] = c[q];
+    D[0] += maj;
+    D[3] += maj;
+    D[5] += maj;
+  }
+  if (PART != kGatherDU) {
+    r[0] += maj * xn0;
+    r[1] += maj * xn1;
+    r[2] += maj * xn2;
+  }
 }
 
 // Partial row gather of rows on one wave's 64 lanes for the couplings c with c % G == GI (GI is a
