@@ -42,6 +42,11 @@ MIXED_COARSEST_NODES = 400
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_sheet1024_solver_kernels.md"
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def measured_traffic(kernel_class, size):
     """HBM bytes per launch of a kernel class from the committed rocprofv3 --pmc summary (FETCH_SIZE
     and WRITE_SIZE, scaled by a plain copy's counts), for the workload it was measured on."""
@@ -322,6 +327,7 @@ def graph_bench(args):
 
     results, budgets = {}, None
     for m in methods:
+        progress(f"method {m}: start")
         group = []
         for c in cfgs:
             grips = [dict(g) for g in c["grips"]]
@@ -408,7 +414,9 @@ def main():
     order = ("plain", "mixed", "learned") if args.method == "all" else (args.method,)
     main_name = order[-1]
     for m in order:
+        progress(f"method {m}: start")
         runs[m] = run_method(args, cfg, m, args.profile_frames if m == main_name else 0)
+        progress(f"method {m}: {runs[m]['ms']:.1f} ms/frame, {runs[m]['iters']:.1f} PCG iterations per frame")
     R = runs[main_name]
     ms = R["ms"]
     prof = R["prof"]
@@ -430,6 +438,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline and os.environ.get("BD_BENCH_NO_CPU") is None:
         P = runs.get("plain", R)
+        progress("cpu baseline: the FP64 oracle on one Newton iteration, threaded and on one core")
         cpu = cpu_baseline(cfg, P["iters"] / max(P["newton"], 1), P["newton"])
     out = {
         "metric": "ms/frame & PCG iters to rel-res 1e-6: learned+mixed vs plain MG-PCG",
