@@ -81,6 +81,9 @@ def parse():
                     help="sheet1024: the headline workload (default); sheet64: the 64 x 64 sheet, eager against "
                          "graph-captured substeps; patches48: 48 uneven patches, one eager substep per patch "
                          "against one graph launch for the batch")
+    ap.add_argument("--one-colour-max-line", type=int, default=257, metavar="NODES",
+                    help="mixed and learned methods: Galerkin levels whose lines have at most NODES nodes smooth "
+                         "with one line colour per family (bd_set_line_colors); 0 keeps the reference's 3")
     ap.add_argument("--batch-threads", type=int, default=16,
                     help="host threads of the patches' concurrent batch (grid+threads)")
     ap.add_argument("--sweeps", type=int, default=1,
@@ -227,8 +230,17 @@ def run_method(args, cfg, method, profile_frames):
         gpu.set_grip_goals(goals)
         gpu.encode_substep()
 
-    for _ in range(args.warmup):
+    colours = None
+    for w in range(args.warmup):
         frame()
+        if w == 0:
+            colours = [3] * len(gpu.stats()["level_nodes"])
+            if method != "plain" and args.one_colour_max_line > 0:
+                # the Galerkin levels whose lines are short enough smooth each family's lines at once
+                for k, nodes in enumerate(gpu.stats()["level_nodes"]):
+                    if k > 0 and round(nodes ** 0.5) <= args.one_colour_max_line:
+                        gpu.set_line_colors(k, 1)
+                        colours[k] = 1
     gpu.sync()
     if method == "learned":
         t0 = time.perf_counter()
@@ -260,7 +272,7 @@ def run_method(args, cfg, method, profile_frames):
     out = {"ms": 1e3 * (t1 - t0) / args.steps, "iters": float(np.mean(iters)), "newton": float(np.mean(newton)),
            "iters_list": [int(i) for i in iters],
            "max_rel_residual": float(max(relres)), "train": train, "learned_fallbacks": st["learned_fallbacks"],
-           "levels": gpu.stats()["level_nodes"], "tuned": tuned, "precision": params.precision,
+           "levels": gpu.stats()["level_nodes"], "tuned": tuned, "precision": params.precision, "colours": colours,
            "sweeps": params.smooth_sweeps}
     gpu.profile_enable(True)
     for _ in range(profile_frames):
@@ -468,6 +480,7 @@ def main():
                         "smooth_sweeps": v["sweeps"], "vcycle_operators": "fp16" if v["precision"] else "fp32",
                         "max_rel_residual": v["max_rel_residual"], "learned_fallbacks": v["learned_fallbacks"],
                         "training": v["train"], "pcg_iterations_each_frame": v["iters_list"],
+                        "line_colours_per_level": v["colours"],
                         "smoother_kernels": {str(lv): list(c) for lv, c in v["tuned"].items()}}
                     for k, v in runs.items()},
         "iteration_ratio_plain_over_learned": ratio(runs, "plain", "learned", "iters"),

@@ -310,6 +310,7 @@ def load_library(path: os.PathLike | None = None):
     lib.bd_apply.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     lib.bd_apply_noalloc.argtypes = [vp, C.c_int, vp, vp, vp, vp]
     lib.bd_set_pcg_mode.argtypes = [vp, C.c_int]
+    lib.bd_set_line_colors.argtypes = [vp, C.c_int, C.c_int]
     lib.bd_graph_capture.argtypes = [vp, C.c_int]
     lib.bd_graph_substep.argtypes = [vp]
     lib.bd_graph_release.argtypes = [vp]
@@ -337,6 +338,6 @@ EXPORTED = (
     "bd_grip_release", "bd_pick_ray", "bd_pick_near", "bd_autotune", "bd_set_smoother",
     "bd_learn_record", "bd_learn_train", "bd_set_learned", "bd_learn_get_weights", "bd_learn_set_weights",
     "bd_learn_loss_grad", "bd_learn_read_sample", "bd_learn_info", "bd_learn_read_fields", "bd_apply",
-    "bd_apply_noalloc", "bd_set_pcg_mode", "bd_graph_capture", "bd_graph_substep", "bd_graph_release", "bd_batch_create",
+    "bd_apply_noalloc", "bd_set_pcg_mode", "bd_set_line_colors", "bd_graph_capture", "bd_graph_substep", "bd_graph_release", "bd_batch_create",
     "bd_batch_create_concurrent", "bd_batch_substep", "bd_batch_sync", "bd_batch_destroy", "bd_batch_last_error",
 )

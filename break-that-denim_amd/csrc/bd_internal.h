@@ -72,6 +72,10 @@ struct Level {
   uint8_t* fixedT = nullptr;
   float* xT = nullptr;
   float* bT = nullptr;  // column-major copy of b, refreshed before the level is smoothed
+  // one-colour levels (bd_set_line_colors): the second buffer of the iterate in each layout; a sweep's
+  // first pass writes them, its second pass writes x and xT back (allocated by the setter)
+  float* x2 = nullptr;
+  float* xT2 = nullptr;
   // smoother kernel per family (u, v): -1 the default rule, 0 direct, 4 / 8 staged, 16 split
   // (bd_autotune)
   int variant[2] = {-1, -1};

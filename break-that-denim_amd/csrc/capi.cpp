@@ -163,8 +163,10 @@ std::vector<Geo> level_layouts(const Geo& fine, int max_coarsest) {
 }
 
 // line tables (lattice.ts:86-106 lineTable): per colour the lines of one family, v-lines (family 1)
-// first; `ncol` colours per family (kLineColors = 3 on every level: the fine level's along-line
-// bending couples lines two apart as well, so same-colour lines must be three apart)
+// first; `ncol` colours per family (kLineColors = 3 by default: the fine level's along-line bending
+// couples lines two apart as well, so lines of one colour three apart share no coupling; with 1 colour
+// (bd_set_line_colors) every line of a family is solved at once against the lagged iterate of the
+// others)
 void build_lines(Solver& s, Level& L, int ncol) {
   L.ncol = ncol;
   std::vector<std::vector<int4>> by(2 * ncol);
@@ -219,6 +221,8 @@ void destroy_all(Solver& s) {
     dfree(L.bT);
     dfree(L.rows);
     dfree(L.du);
+    dfree(L.x2);
+    dfree(L.xT2);
     dfree(L.oph);
     dfree(L.ophT);
     dfree(L.hs);
@@ -1043,6 +1047,33 @@ int bd_pick_near(bd_solver* h, const double* p, double radius, int window_radius
     for (int q = 0; q < 3; q++) P.eye[q] = (float)p[q];
     launch_pick(s, false);
     fill_pick(s, window_radius, out);
+    return BD_OK;
+  });
+}
+
+int bd_set_line_colors(bd_solver* h, int level, int colors) {
+  return guarded(h, [&](Solver& s) {
+    if (level < 0 || level >= s.nlev) return fail(h, BD_ERR_INVALID, "bd_set_line_colors: no such level");
+    if (colors != 1 && colors != kLineColors) return fail(h, BD_ERR_INVALID, "bd_set_line_colors: colours must be 1 or 3");
+    if (s.pcg_mode == BD_PCG_WORKGROUP && colors != kLineColors)
+      return fail(h, BD_ERR_STATE, "bd_set_line_colors: the one-workgroup PCG smooths with 3 colours");
+    Level& L = s.lv[level];
+    if (L.ncol == colors) return BD_OK;
+    BD_HIP(hipStreamSynchronize(s.stream));
+    dfree(L.lines);
+    L.lines = nullptr;
+    build_lines(s, L, colors);
+    if (colors == 1 && !L.x2) {
+      L.x2 = dalloc<float>(3 * (size_t)L.Np);
+      if (L.dual) L.xT2 = dalloc<float>(3 * (size_t)L.Np);
+    }
+    if (L.rows) {
+      // the split variants' buffers are laid out by colour: rebuilt, and the line blocks refilled if
+      // the level's operator is already there
+      smooth_alloc_rows(s, L);
+      if (level == 0 ? s.assembled : s.hierarchy) smooth_line_blocks(s, level);
+    }
+    s.config_gen++;
     return BD_OK;
   });
 }

@@ -436,8 +436,9 @@ void smooth_set_attributes() {
 
 namespace {
 
-// one colour pass of level k with the given variant
-void launch_colour(Solver& s, int k, int c, int variant) {
+// one colour pass of level k with the given variant; `second` is the second pass of a one-colour
+// level's sweep (it reads the pair's other buffer and writes back to x)
+void launch_colour(Solver& s, int k, int c, int variant, bool second) {
   Level& L = s.lv[k];
   int count = L.color_start[c + 1] - L.color_start[c];
   if (count == 0) return;
@@ -445,9 +446,24 @@ void launch_colour(Solver& s, int k, int c, int variant) {
   int maxlen = 0;
   for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
   const bool colmajor = L.dual && fam == 1;
+  // x in the pass's layout and in the other one; a one-colour level's first pass of a sweep writes the
+  // pair's second buffers (x2, xT2) from x, the second pass writes x back from them, so that every line
+  // reads the lagged iterate of the lines beside it
+  float* own = colmajor ? L.xT : L.x;
+  float* other = L.dual ? (colmajor ? L.x : L.xT) : nullptr;
+  const float* rd_x = own;
+  if (L.ncol == 1) {
+    float* own2 = colmajor ? L.xT2 : L.x2;
+    float* other2 = L.dual ? (colmajor ? L.x2 : L.xT2) : nullptr;
+    if (second) {
+      rd_x = own2;
+    } else {
+      own = own2;
+      other = other2;
+    }
+  }
   SmoothArgs S{colmajor ? L.opT : L.op, colmajor ? L.fixedT : L.fixed, L.Np, L.lines + L.color_start[c],
-               colmajor ? L.xT : L.x, colmajor ? L.bT : L.b, L.dual ? (colmajor ? L.x : L.xT) : nullptr, &s.scal->done,
-               L.geo};
+               own, colmajor ? L.bT : L.b, other, &s.scal->done, L.geo, rd_x};
   // each node of the colour's lines: its full row (fine 58 floats; coarse 150 values, 4 B each or 2 B
   // as FP16 pairs) + b, x read, x write (dual-layout levels write x twice)
   double nodes = (double)count * maxlen;
@@ -506,6 +522,10 @@ std::vector<int> smooth_variants(Solver& s, int k, int fam) {
 // The rows buffer of the split variant: 18 planes of (most lines of one colour) x (longest line).
 void smooth_alloc_rows(Solver& s, Level& L) {
   (void)s;
+  if (L.rows) BD_HIP(hipFree(L.rows));
+  if (L.du) BD_HIP(hipFree(L.du));
+  L.rows = nullptr;
+  L.du = nullptr;
   int most = 0;
   for (int c = 0; c < 2 * L.ncol; c++) most = std::max(most, L.color_start[c + 1] - L.color_start[c]);
   L.rows_cap = (size_t)most * ((L.max_len + 3) & ~3);  // room for either split variant's line stride
@@ -528,7 +548,7 @@ void smooth_line_blocks(Solver& s, int k) {
     for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
     const bool colmajor = L.dual && fam == 1;
     SmoothArgs S{colmajor ? L.opT : L.op, colmajor ? L.fixedT : L.fixed, L.Np, L.lines + L.color_start[c0],
-                 colmajor ? L.xT : L.x, colmajor ? L.bT : L.b, nullptr, nullptr, L.geo};
+                 colmajor ? L.xT : L.x, colmajor ? L.bT : L.b, nullptr, nullptr, L.geo, colmajor ? L.xT : L.x};
     const dim3 grid((maxlen + 255) / 256, count);
     const int line0 = L.color_start[c0];
     auto go = [&](auto rdd) {
@@ -565,7 +585,7 @@ void launch_smooth(Solver& s, int k, int reverse) {
   const int nc = 2 * L.ncol;
   for (int q = 0; q < nc; q++) {
     int c = reverse ? nc - 1 - q : q;
-    launch_colour(s, k, c, L.variant[c < L.ncol ? 1 : 0]);
+    launch_colour(s, k, c, L.variant[c < L.ncol ? 1 : 0], q == 1);
   }
 }
 

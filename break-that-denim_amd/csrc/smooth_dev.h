@@ -19,6 +19,10 @@ struct SmoothArgs {
   float* xo;             // the other layout's copy of x (dual-layout levels), or null
   const int* gate;
   Geo geo;
+  // the iterate the gather reads (same layout as x): x itself when lines of one colour share no
+  // coupling (3 colours); on a one-colour level the other buffer of a pair, so that every line reads
+  // the lagged iterate, not the one its neighbours are writing
+  const float* xr = nullptr;
 };
 
 // index strides of lattice offsets (di, dj) in the pass's layout
@@ -107,7 +111,7 @@ template <bool FINE, int FAM, bool DUAL, class RD, int PART = kGatherAll>
 __device__ __forceinline__ void gather_row(const SmoothArgs& S, const RD& rd, int n, int nb, int i, int j, int nx, int ny,
                                            float D[6], float U[9], float r[3], bool& fixed_n) {
   const int Np = S.Np;
-  const float* x = S.x;
+  const float* x = S.xr;
   const int si = stride_i<FAM, DUAL>(nx, ny), sj = stride_j<FAM, DUAL>(nx, ny);
   fixed_n = S.fixed[n] != 0;
   float c[6];
@@ -208,7 +212,7 @@ __device__ __forceinline__ void gather_part(const SmoothArgs& S, const RD& rd, i
                                             bool valid, int row, int len, float* pr, float* pm, float* gU, float* gD,
                                             float* gR, float* gX, uint8_t* gF) {
   const int Np = S.Np;
-  const float* x = S.x;
+  const float* x = S.xr;
   const int si = stride_i<FAM, DUAL>(nx, ny), sj = stride_j<FAM, DUAL>(nx, ny);
   float r0 = 0.f, r1 = 0.f, r2 = 0.f, maj = 0.f;
   float U[9];

@@ -367,6 +367,7 @@ std::string wg_unsupported(const Solver& s) {
   for (int k = 0; k < s.nlev; k++) {
     const Level& L = s.lv[k];
     if (L.oph) return "its coarse levels hold FP16 operators";
+    if (L.ncol != kLineColors) return "a level smooths with fewer than 3 line colours";
     if (std::max(L.geo.nx[0], L.geo.ny[0]) > kWgMaxLine) return "its lines are longer than 512 nodes";
   }
   return "";
@@ -392,7 +393,7 @@ void wg_prepare(Solver& s) {
   for (int k = 0; k < s.nlev; k++) {
     const Level& L = s.lv[k];
     Y.lv[k] = WgLevel{L.op, L.fixed, L.N, L.Np, L.fine ? 1 : 0, L.geo, L.x, L.b, L.r, k + 1 < s.nlev ? L.pw : nullptr};
-    Y.sa[k] = SmoothArgs{L.op, L.fixed, L.Np, nullptr, L.x, L.b, nullptr, nullptr, L.geo};
+    Y.sa[k] = SmoothArgs{L.op, L.fixed, L.Np, nullptr, L.x, L.b, nullptr, nullptr, L.geo, L.x};
     if (k + 1 < s.nlev) {
       const Level& C = s.lv[k + 1];
       Y.xf[k] = XferArgs{LvArgs{L.op, L.fixed, L.N, L.Np, L.geo}, C.geo, C.N, C.Np, L.pw, C.fixed, L.r, C.b, C.x, L.x,

@@ -295,6 +295,7 @@ typedef struct {
   int nlev;
   op_t lv[MAXLV];
   prol_t pr[MAXLV];
+  int colors[MAXLV];  /* line colours per family of each level's smoother (0 = 3) */
   int lagged;
   /* counters */
   int broken, pulled, rejections;
@@ -1536,7 +1537,9 @@ static void smooth_lines(const op_t* A, double* x, const double* b, int reverse,
                   continue;
                 }
                 if (across == 0 && along == -1) continue;
-                if (across == 0 || abs(across) == colors) maj += frob(base);
+                /* colours 2 and 3 as newtonMultigrid.ts:705; 1 colour (this build's extension)
+                   majorises only the along-line couplings */
+                if (across == 0 || (colors > 1 && abs(across) == colors)) maj += frob(base);
                 double x0 = xOld[3 * m], x1 = xOld[3 * m + 1], x2 = xOld[3 * m + 2];
                 r[3 * k] -= base[0] * x0 + base[1] * x1 + base[2] * x2;
                 r[3 * k + 1] -= base[3] * x0 + base[4] * x1 + base[5] * x2;
@@ -1623,7 +1626,7 @@ static double* half_operator(const or_solver* S, const op_t* A) {
 static void vcycle(or_solver* S, int k, double* x, const double* b) {
   op_t Ah = S->lv[k];
   const op_t* A = &S->lv[k];
-  int colors = 3;
+  int colors = S->colors[k] ? S->colors[k] : 3;
   if (k == S->nlev - 1) {
     if (S->cL) {
       band_solve(S->cn, S->cbw, S->cL, b, x);
@@ -1912,8 +1915,14 @@ void or_smooth(or_solver* S, int k, double* x, const double* b, int reverse) {
   op_t Ah = S->lv[k];
   double* h = half_operator(S, &S->lv[k]);
   if (h) Ah.A = h;
-  smooth_lines(&Ah, x, b, reverse, 3);
+  smooth_lines(&Ah, x, b, reverse, S->colors[k] ? S->colors[k] : 3);
   free(h);
+}
+/* line colours per family of level k's smoother (1, 2 or 3) */
+int or_set_line_colors(or_solver* S, int k, int colors) {
+  if (k < 0 || k >= MAXLV || colors < 1 || colors > 3) return -1;
+  S->colors[k] = colors;
+  return 0;
 }
 void or_vcycle(or_solver* S, double* x, const double* b) { vcycle(S, 0, x, b); }
 void or_pcg(or_solver* S, const double* b, double* x, int iterations, double rel_tol, int max_iter) { pcg(S, b, x, iterations, rel_tol, max_iter); }

@@ -121,6 +121,7 @@ def lib():
         L.or_fp16_round.argtypes = [C.c_double]
         L.or_fp16_round.restype = C.c_double
         L.or_smooth.argtypes = [vp, C.c_int, dp, dp, C.c_int]
+        L.or_set_line_colors.argtypes = [vp, C.c_int, C.c_int]
         L.or_vcycle.argtypes = [vp, dp, dp]
         L.or_pcg.argtypes = [vp, dp, dp, C.c_int, C.c_double, C.c_int]
         L.or_pcg_relres.argtypes = [vp]
@@ -387,6 +388,11 @@ class Oracle:
         y = np.zeros_like(x)
         lib().or_apply(self.h, k, _dp(x), _dp(y))
         return y
+
+    def set_line_colors(self, k, colors):
+        """Line colours per family of level k's smoother (newtonMultigrid.ts lineColors; 1 is an extension)."""
+        if lib().or_set_line_colors(self.h, int(k), int(colors)) != 0:
+            raise ValueError("or_set_line_colors: level or colours out of range")
 
     def smooth(self, k, x, b, reverse=False):
         x = np.ascontiguousarray(x, dtype=np.float64).ravel().copy()

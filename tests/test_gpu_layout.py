@@ -330,3 +330,81 @@ def test_set_smoother_rejects_what_a_level_cannot_run():
     gpu.encode_vcycle()
     assert np.isfinite(gpu.read(BD_BUF_LEVEL_X, 0)).all()
     gpu.close()
+
+
+def test_one_colour_coarse_levels_keep_parity():
+    """bd_set_line_colors(level, 1) on every Galerkin level of a 136 x 136 sheet (each family's lines
+    solved at once against the lagged iterate, the iterate ping-ponged between two buffers), against
+    the oracle smoothing the same levels with one colour: a level-1 sweep in both directions, the
+    V-cycle and a 2-iteration PCG at the usual bars; then a tolerance-mode PCG reaches rel-res 1e-6,
+    confirmed in FP64, and 3 colours can be set back."""
+    from bd.solver import GpuSolver
+    n = 136
+    pitch = 1e-3
+    mat = linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / 2.4)
+    S = sheet_scene(n, pitch, mat)
+    params = make_params(history=False)
+    cpu = O.Oracle(S, params)
+    for _ in range(4):
+        cpu.substep()
+    xs = cpu.x.astype(np.float32).astype(np.float64)
+    vs = cpu.v.astype(np.float32).astype(np.float64)
+    cpu.x[:] = xs
+    cpu.v[:] = vs
+    say("gpu: 136x136 sheet, one line colour on the Galerkin levels")
+    gpu = GpuSolver(S, params)
+    gpu.set_state(xs, vs)
+    cpu.predict()
+    cpu.assemble()
+    gpu.encode_predict()
+    gpu.encode_assemble(True)
+    nlev = len(gpu.stats()["level_nodes"])
+    assert nlev >= 3
+    for k in range(1, nlev):
+        gpu.set_line_colors(k, 1)
+        cpu.set_line_colors(k, 1)
+    with pytest.raises(RuntimeError, match="1 or 3"):
+        gpu.set_line_colors(1, 2)
+    cpu.load_fine_operator(gpu.stencil(0), gpu.fixed(0))
+    b = gpu.read(BD_BUF_RHS)
+    # a level-1 sweep on the Galerkin operator both sides hold
+    cn = cpu.level_nodes(1)
+    rng = np.random.default_rng(8)
+    b1 = rng.standard_normal(3 * cn) * 1e-3
+    x1 = rng.standard_normal(3 * cn) * 1e-6
+    fx1 = np.repeat(cpu.fixed(1) == 1, 3)
+    b1[fx1] = 0
+    x1[fx1] = 0
+    for reverse in (False, True):
+        gpu.write(BD_BUF_LEVEL_B, b1, 1)
+        gpu.write(BD_BUF_LEVEL_X, x1, 1)
+        gpu.encode_smooth(1, reverse)
+        e = rel_to_scale(gpu.read(BD_BUF_LEVEL_X, 1), cpu.smooth(1, x1.copy(), b1, reverse))
+        say(f"  level-1 one-colour sweep ({'reverse' if reverse else 'forward'}) maxAbs/scale = {e:.2e}")
+        assert e < 1e-3
+    gpu.write(BD_BUF_LEVEL_B, b, 0)
+    gpu.encode_vcycle()
+    e = rel_to_scale(gpu.read(BD_BUF_LEVEL_X, 0), cpu.vcycle(b))
+    say(f"V-cycle maxAbs/scale = {e:.2e}")
+    assert e < 5e-3
+    gpu.write(BD_BUF_RHS, b)
+    gpu.encode_pcg()
+    e = rel_to_scale(gpu.read(BD_BUF_DELTA), cpu.pcg(b, params.pcg_iterations))
+    say(f"PCG step maxAbs/scale = {e:.2e}")
+    assert e < 2e-2
+    gpu.set_pcg(0, 1e-6, 300)
+    gpu.write(BD_BUF_RHS, b)
+    gpu.encode_pcg()
+    st = gpu.stats()
+    say(f"tolerance PCG: {st['pcg_iterations'][0]} iterations, rel-res {st['rel_residual'][0]:.2e}")
+    assert st["rel_residual"][0] <= 1.05e-6
+    for k in range(1, nlev):
+        gpu.set_line_colors(k, 3)
+    gpu.write(BD_BUF_LEVEL_B, b, 0)
+    gpu.encode_vcycle()
+    for k in range(1, nlev):
+        cpu.set_line_colors(k, 3)
+    e = rel_to_scale(gpu.read(BD_BUF_LEVEL_X, 0), cpu.vcycle(b))
+    say(f"V-cycle with 3 colours again maxAbs/scale = {e:.2e}")
+    assert e < 5e-3
+    gpu.close()

@@ -170,7 +170,8 @@ def dense_smooth(A, fixed, nx, ny, x0, b, reverse=False, colors=3):
                             if along == 1:
                                 M[3 * (k + 1):3 * (k + 1) + 3, 3 * k:3 * k + 3] += B.T
                             continue
-                        if across == 0 or abs(across) == colors:
+                        # newtonMultigrid.ts:705 (2 and 3 colours); 1 colour majorises along the line only
+                        if across == 0 or (colors > 1 and abs(across) == colors):
                             maj += np.linalg.norm(B)
                         rhs[3 * k:3 * k + 3] -= B @ xo[m]
                 M[3 * k:3 * k + 3, 3 * k:3 * k + 3] += maj * np.eye(3)
@@ -182,14 +183,18 @@ def dense_smooth(A, fixed, nx, ny, x0, b, reverse=False, colors=3):
     return x.ravel()
 
 
+@pytest.mark.parametrize("colors", [3, 2, 1])
 @pytest.mark.parametrize("reverse", [False, True])
-def test_smoother_matches_dense_line_solves(reverse):
+def test_smoother_matches_dense_line_solves(reverse, colors):
     """Brute force on a small coupon: a full colour-ordered sweep equals dense solves of every line's
     block-tridiagonal system (lower blocks = transposes, +-2 along-line couplings majorised by their
-    Frobenius norms), up to the 2e-6 relative pivot shift."""
+    Frobenius norms; with 2 colours the couplings of lines two apart too), each line against the
+    iterate its colour started from, up to the 2e-6 relative pivot shift; for the reference's 3 and 2
+    colours per family and for 1 (every line of a family at once)."""
     S = compile_scene([mg_coupon(10, 8)])
     L = S.layers[0]
     o = O.Oracle(S, ref_params())
+    o.set_line_colors(0, colors)
     o.predict()
     o.assemble()
     A = o.stencil(0)
@@ -200,7 +205,7 @@ def test_smoother_matches_dense_line_solves(reverse):
     x0 = np.array([1e-3 * r() for _ in range(3 * N)])
     x0.reshape(N, 3)[fixed == 1] = 0
     x = o.smooth(0, x0, b, reverse)
-    ref = dense_smooth(A, fixed, L.nx, L.ny, x0, b, reverse)
+    ref = dense_smooth(A, fixed, L.nx, L.ny, x0, b, reverse, colors)
     assert np.max(np.abs(x - ref)) <= 1e-4 * np.max(np.abs(ref))
     # a plausible slip (dropping the majorisation) is caught at this tolerance
     assert np.max(np.abs(x - x0)) > 1e-3 * np.max(np.abs(ref))
