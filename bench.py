@@ -14,7 +14,9 @@ Three methods run the same frames from the same scene:
            nodes), the Galerkin hierarchy of the first Newton iteration reused by the later ones;
   mixed    the hierarchy rebuilt at every Newton iteration, the coarsest level (17 x 17, the same
            level count as plain) solved exactly in FP64 (banded Cholesky, dense inverse) instead of
-           swept, FP32 operators (FP16 coarse storage needs 4x the iterations with contact);
+           swept, FP32 operators (FP16 coarse storage needs 4x the iterations with contact), and
+           one line colour per family on the levels with lines of at most 513 nodes (every line
+           of a family solved at once against the lagged iterate; the fine level keeps 3);
   learned  mixed plus the learned coarse-space fields (BF16 message-passing net on the 65 x 65
            level): an A-optimal warm start from them (or, with --learned-mode 2, deflated flexible
            PCG over them too), kept only when it lowers the residual, plain MG-PCG otherwise; the
@@ -24,8 +26,9 @@ Three methods run the same frames from the same scene:
 against plain, are reported beside it.
 
 Prints one JSON line: the metric, the GPU roofline of the dominant kernel class (HIP-event timed in
-an instrumented pass over the same frames) and a CPU baseline (the FP64 C oracle, single thread, on
-a bounded sample of the same frame, extrapolated with the GPU's measured iteration counts).
+an instrumented pass over the same frames) and a CPU baseline (the FP64 C oracle on up to 16 host
+threads, and on one, over a bounded sample of the same frame, extrapolated with the GPU's measured
+iteration counts).
 """
 import argparse
 import json
@@ -81,9 +84,9 @@ def parse():
                     help="sheet1024: the headline workload (default); sheet64: the 64 x 64 sheet, eager against "
                          "graph-captured substeps; patches48: 48 uneven patches, one eager substep per patch "
                          "against one graph launch for the batch")
-    ap.add_argument("--one-colour-max-line", type=int, default=257, metavar="NODES",
-                    help="mixed and learned methods: Galerkin levels whose lines have at most NODES nodes smooth "
-                         "with one line colour per family (bd_set_line_colors); 0 keeps the reference's 3")
+    ap.add_argument("--one-colour-max-line", type=int, default=513, metavar="NODES",
+                    help="mixed and learned methods: levels whose lines have at most NODES nodes smooth with one "
+                         "line colour per family (bd_set_line_colors); 0 keeps the reference's 3")
     ap.add_argument("--batch-threads", type=int, default=16,
                     help="host threads of the patches' concurrent batch (grid+threads)")
     ap.add_argument("--sweeps", type=int, default=1,
@@ -236,9 +239,9 @@ def run_method(args, cfg, method, profile_frames):
         if w == 0:
             colours = [3] * len(gpu.stats()["level_nodes"])
             if method != "plain" and args.one_colour_max_line > 0:
-                # the Galerkin levels whose lines are short enough smooth each family's lines at once
+                # the levels whose lines are short enough smooth each family's lines at once
                 for k, nodes in enumerate(gpu.stats()["level_nodes"]):
-                    if k > 0 and round(nodes ** 0.5) <= args.one_colour_max_line:
+                    if round(nodes ** 0.5) <= args.one_colour_max_line:
                         gpu.set_line_colors(k, 1)
                         colours[k] = 1
     gpu.sync()
