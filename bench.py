@@ -98,7 +98,8 @@ def parse():
                     help="operator storage of the mixed method's coarse levels (fine level and plain method: fp32)")
     ap.add_argument("--smoother-probe", type=int, default=0, metavar="SWEEPS",
                     help="only run SWEEPS forward sweeps of every smoothed level of the mixed method's hierarchy "
-                         "(after one frame and the autotuner), outside PCG: the smoother passes for PMC counters")
+                         "(after one frame, the autotuner and any --pin), outside PCG: the smoother passes for PMC "
+                         "counters")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 8 plain device copies of MIB MiB (calibrates the PMC byte counters)")
     return ap.parse_args()
@@ -134,7 +135,12 @@ def smoother_probe(args, sweeps):
     goals = np.array([g["goal"] for g in cfg["grips"]], dtype=np.float64) + np.array(cfg["velocity"])
     gpu.set_grip_goals(goals)
     gpu.encode_substep()
-    tuned = gpu.autotune(3)
+    tuned = dict(gpu.autotune(3))
+    for pin in args.pin:
+        lvl, u, v = (int(q) for q in pin.split(":"))
+        gpu.set_smoother(lvl, 0, u)
+        gpu.set_smoother(lvl, 1, v)
+        tuned[lvl] = (u, v)
     levels = gpu.stats()["level_nodes"]
     for k in range(len(levels) - 1):
         for _ in range(sweeps):

@@ -31,6 +31,7 @@ BD_SMOOTHER_STAGED4 = 4
 BD_SMOOTHER_STAGED8 = 8
 BD_SMOOTHER_SPLIT = 16
 BD_SMOOTHER_SPLIT4 = 17
+BD_SMOOTHER_FACTORED = 18
 
 (BD_BUF_X, BD_BUF_V, BD_BUF_DX, BD_BUF_PRED, BD_BUF_DELTA, BD_BUF_RHS, BD_BUF_LEVEL_X, BD_BUF_LEVEL_B,
  BD_BUF_LEVEL_R, BD_BUF_STENCIL, BD_BUF_FIXED, BD_BUF_PROLONG, BD_BUF_REST, BD_BUF_INTACT, BD_BUF_CUTMASK,

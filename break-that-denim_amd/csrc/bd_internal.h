@@ -72,6 +72,10 @@ struct Level {
   uint8_t* fixedT = nullptr;
   float* xT = nullptr;
   float* bT = nullptr;  // column-major copy of b, refreshed before the level is smoothed
+  // dual levels with lines of kBatchCopyMinLine nodes or more: a pass writes x in its own layout only,
+  // and one tiled copy after each family's passes refreshes the other layout (a line's result lies
+  // strided in the other layout: written by every pass, it costs more than the pass's solve)
+  bool batch_copy = false;
   // one-colour levels (bd_set_line_colors): the second buffer of the iterate in each layout; a sweep's
   // first pass writes them, its second pass writes x and xT back (allocated by the setter)
   float* x2 = nullptr;
@@ -89,6 +93,9 @@ struct Level {
   float* du = nullptr;
   size_t du_cap = 0;
   int du_stride = 0;
+  // the lines' factorizations (BD_SMOOTHER_FACTORED): per row its pivot inverse and elimination
+  // multipliers (kFacPlanes = 24 planes of du_cap, same positions as du), refreshed with du
+  float* fac = nullptr;
   // BD_PRECISION_FP16: op (and opT) as FP16 pairs, comps / 2 u32 planes (oprd.h); hs[0] the scale,
   // hs[1] its inverse, hs[2] the bits of the largest free diagonal entry
   uint32_t* oph = nullptr;
@@ -102,6 +109,9 @@ constexpr int kStagedMaxLen = 257;
 // levels whose longest line has at least this many nodes keep the column-major copy: below it a
 // v-line pass's strided gather costs less than the copy (about twice a u-line pass at 65 nodes)
 constexpr int kDualMinLine = 33;
+// dual levels whose longest line has at least this many nodes refresh the other layout's copy of x
+// once per family's passes instead of in every pass
+constexpr int kBatchCopyMinLine = 512;
 // longest line the one-workgroup PCG solves (its LDS holds the line records of a colour round)
 constexpr int kWgMaxLine = 512;
 
@@ -347,6 +357,8 @@ void launch_zero(Solver& s, float* p, size_t n);
 // column-major copies of a dual-layout level: operator and fixed flags, or the iterate
 void launch_colmajor_op(Solver& s, int k);
 void launch_colmajor_x(Solver& s, int k);
+// x of level k in one layout copied to the other: row-major -> column-major (to_col) or back
+void launch_layout_copy(Solver& s, int k, const float* src, float* dst, bool to_col);
 void launch_colmajor_b(Solver& s, int k);
 // a vector's restriction to level k + 1 (xc of the transfer untouched) and prolongation from it,
 // both gated by `gate` (skipped while it is nonzero); `set`: the fine vector is overwritten, else added to

@@ -221,6 +221,7 @@ void destroy_all(Solver& s) {
     dfree(L.bT);
     dfree(L.rows);
     dfree(L.du);
+    dfree(L.fac);
     dfree(L.x2);
     dfree(L.xT2);
     dfree(L.oph);
@@ -605,6 +606,7 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
       int longest = 0;
       for (int l = 0; l < L.geo.nl; l++) longest = std::max(longest, std::max(L.geo.nx[l], L.geo.ny[l]));
       L.dual = longest >= kDualMinLine;
+      L.batch_copy = L.dual && longest >= kBatchCopyMinLine;
       if (L.dual) {
         L.opT = dalloc<float>((size_t)(L.fine ? kFineComps : kCoarseComps) * L.Np);
         L.fixedT = dalloc<uint8_t>(L.Np);

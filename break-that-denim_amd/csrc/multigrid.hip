@@ -350,6 +350,25 @@ __global__ __launch_bounds__(256) void k_to_colmajor(const T* in, T* out, int Np
   }
 }
 
+// column-major -> row-major copy (the inverse of k_to_colmajor), same tiling
+template <class T>
+__global__ __launch_bounds__(256) void k_to_rowmajor(const T* in, T* out, int Np, int off, int nx, int ny) {
+  __shared__ T tile[32][33];
+  const size_t q = blockIdx.z;
+  const int j0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
+  const T* src = in + q * Np + off;
+  T* dst = out + q * Np + off;
+  for (int r = threadIdx.y; r < 32; r += 8) {
+    const int j = j0 + threadIdx.x, i = i0 + r;
+    if (i < nx && j < ny) tile[r][threadIdx.x] = src[i * ny + j];
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += 8) {
+    const int i = i0 + threadIdx.x, j = j0 + r;
+    if (i < nx && j < ny) dst[j * nx + i] = tile[threadIdx.x][r];
+  }
+}
+
 template <class T>
 void to_colmajor(Solver& s, const Level& L, const T* in, T* out, int planes) {
   for (int l = 0; l < L.geo.nl; l++) {
@@ -390,6 +409,21 @@ void launch_colmajor_op(Solver& s, int k) {
     if (L.dual) k_pack_half<<<(unsigned)((total + 255) / 256), 256, 0, s.stream>>>(L.opT, words, L.Np, L.hs, L.ophT);
   }
   if (L.du) smooth_line_blocks(s, k);
+}
+
+void launch_layout_copy(Solver& s, int k, const float* src, float* dst, bool to_col) {
+  Level& L = s.lv[k];
+  if (!L.dual) return;
+  ProfScope ps(s, PC_TRANSPOSE, 2.0 * L.N * 12.0);
+  if (to_col) {
+    to_colmajor<float>(s, L, src, dst, 3);
+    return;
+  }
+  for (int l = 0; l < L.geo.nl; l++) {
+    const int nx = L.geo.nx[l], ny = L.geo.ny[l];
+    dim3 grid((ny + 31) / 32, (nx + 31) / 32, 3);
+    k_to_rowmajor<float><<<grid, dim3(32, 8), 0, s.stream>>>(src, dst, L.Np, L.geo.off[l], nx, ny);
+  }
 }
 
 void launch_colmajor_x(Solver& s, int k) {

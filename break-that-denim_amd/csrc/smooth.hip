@@ -31,10 +31,11 @@ namespace {
 // Long lines: each thread gathers its two rows straight into registers.
 template <bool FINE, int FAM, bool DUAL, class RDD>
 __global__ __launch_bounds__(512, 4) void k_smooth_direct(SmoothArgs S, RDD rdd) {
-  if (S.gate && *S.gate) return;
+  const PassHead h = pass_head(S, xcd_block(blockIdx.x, gridDim.x));
+  if (h.gated || h.ln.z <= 0) return;
   const auto rd = rdd.live();
   extern __shared__ __align__(16) float sm[];
-  const LineGeo L = line_geo<FAM, DUAL>(S);
+  const LineGeo L = line_geo_of<FAM, DUAL>(S, h.ln);
   const int len = L.ln.z;
   const int t = threadIdx.x;
   const int ka = 2 * t, kb = 2 * t + 1;
@@ -61,10 +62,11 @@ __global__ __launch_bounds__(512, 4) void k_smooth_direct(SmoothArgs S, RDD rdd)
 // one round trip of loads; rows are assembled in LDS, then solved by the first ceil(len/2) threads.
 template <bool FINE, int FAM, int G, bool DUAL, class RDD>
 __global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S, RDD rdd) {
-  if (S.gate && *S.gate) return;
+  const PassHead h = pass_head(S, xcd_block(blockIdx.x, gridDim.x));
+  if (h.gated || h.ln.z <= 0) return;
   const auto rd = rdd.live();
   extern __shared__ __align__(16) float sm[];
-  const LineGeo L = line_geo<FAM, DUAL>(S);
+  const LineGeo L = line_geo_of<FAM, DUAL>(S, h.ln);
   const int len = L.ln.z;
   const int R2 = (len + 1) >> 1;
   const int Lr = ((R2 + 63) / 64) * 64;
@@ -157,10 +159,11 @@ __global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S, RDD rdd) {
 // on another's CU; writes the row's D, U and r to the rows planes (line slot * stride + row).
 template <bool FINE, int FAM, bool DUAL, class RDD>
 __global__ __launch_bounds__(256) void k_line_rows(SmoothArgs S, RDD rdd, float* rows, size_t cap, int stride) {
-  if (S.gate && *S.gate) return;
-  const auto rd = rdd.live();
   const int slot = blockIdx.y;
-  const LineGeo L = line_geo_at<FAM, DUAL>(S, slot);
+  const PassHead h = pass_head(S, slot);
+  if (h.gated || h.ln.z <= 0) return;
+  const auto rd = rdd.live();
+  const LineGeo L = line_geo_of<FAM, DUAL>(S, h.ln);
   const int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= L.ln.z) return;
   int n, nb, no, i, j;
@@ -180,10 +183,11 @@ __global__ __launch_bounds__(256) void k_line_rows(SmoothArgs S, RDD rdd, float*
 // Split variant, second kernel: the line solve of k_smooth_direct over the rows k_line_rows wrote.
 template <bool FINE, int FAM, bool DUAL>
 __global__ __launch_bounds__(512, 4) void k_smooth_rows(SmoothArgs S, const float* rows, size_t cap, int stride) {
-  if (S.gate && *S.gate) return;
   extern __shared__ __align__(16) float sm[];
   const int slot = xcd_block(blockIdx.x, gridDim.x);
-  const LineGeo L = line_geo_at<FAM, DUAL>(S, slot);
+  const PassHead h = pass_head(S, slot);
+  if (h.gated || h.ln.z <= 0) return;
+  const LineGeo L = line_geo_of<FAM, DUAL>(S, h.ln);
   const int len = L.ln.z;
   const int t = threadIdx.x;
   const int ka = 2 * t, kb = 2 * t + 1;
@@ -232,10 +236,11 @@ __global__ __launch_bounds__(512, 4) void k_smooth_rows(SmoothArgs S, const floa
 // kernel reads them from the operator itself and the round trip of 15 of the 18 floats is saved.
 template <bool FINE, int FAM, bool DUAL, class RDD>
 __global__ __launch_bounds__(256) void k_line_r(SmoothArgs S, RDD rdd, float* rows, size_t cap, int stride) {
-  if (S.gate && *S.gate) return;
-  const auto rd = rdd.live();
   const int slot = blockIdx.y;
-  const LineGeo L = line_geo_at<FAM, DUAL>(S, slot);
+  const PassHead h = pass_head(S, slot);
+  if (h.gated || h.ln.z <= 0) return;
+  const auto rd = rdd.live();
+  const LineGeo L = line_geo_of<FAM, DUAL>(S, h.ln);
   const int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= L.ln.z) return;
   int n, nb, no, i, j;
@@ -276,25 +281,32 @@ __global__ __launch_bounds__(256) void k_line_du(SmoothArgs S, RDD rdd, float* d
 template <bool FINE, int FAM, bool DUAL>
 __global__ __launch_bounds__(256) void k_smooth_rows4(SmoothArgs S, const float* rows, size_t cap, int stride,
                                                       const float* du, size_t du_cap, int du_stride, int line0) {
-  if (S.gate && *S.gate) return;
   extern __shared__ __align__(16) float sm[];
   const int slot = xcd_block(blockIdx.x, gridDim.x);
-  const LineGeo L = line_geo_at<FAM, DUAL>(S, slot);
-  const int len = L.ln.z;
+  const PassHead h = pass_head(S, slot);
   const int t = threadIdx.x;
   const int k0 = 4 * t;
   const float* o = rows + (size_t)slot * stride;
   const float* od = du + (size_t)(line0 + slot) * du_stride;
+  // the planes' loads do not depend on the line entry: issued with the gate's, at indices clamped
+  // into the line's stride, and masked once the length is known
+  const int kd = min(k0, du_stride - 4), kr = min(k0, stride - 4);
   float v[18][4];
 #pragma unroll
   for (int q = 0; q < 18; q++) {
-    const float* src = q < 15 ? od + q * du_cap : o + (q - 15) * cap;
-    const float4 p = k0 < len ? *reinterpret_cast<const float4*>(src + k0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float* src = q < 15 ? od + q * du_cap + kd : o + (q - 15) * cap + kr;
+    const float4 p = *reinterpret_cast<const float4*>(src);
     v[q][0] = p.x;
     v[q][1] = p.y;
     v[q][2] = p.z;
     v[q][3] = p.w;
   }
+  if (h.gated || h.ln.z <= 0) return;
+  const LineGeo L = line_geo_of<FAM, DUAL>(S, h.ln);
+  const int len = L.ln.z;
+  int nn[4], no[4];
+  uint8_t fx[4];
+  line_rows_fixed<FAM, DUAL>(S, L, k0, len, nn, no, fx);
   float D[4][6], U[4][9], r[4][3], x[4][3];
 #pragma unroll
   for (int m = 0; m < 4; m++) {
@@ -311,14 +323,99 @@ __global__ __launch_bounds__(256) void k_smooth_rows4(SmoothArgs S, const float*
   }
   line_solve4(sm, blockDim.x, t, len, D, U, r, x);
 #pragma unroll
+  for (int m = 0; m < 4; m++)
+    if (k0 + m < len && !fx[m]) store_x(S, nn[m], no[m], x[m], DUAL);
+}
+
+// The factorization of every line of one family (blockIdx.x the line within the family, four rows per
+// thread) from its line blocks (k_line_du), to the factor planes at the same positions. Launched with
+// the line blocks, when the level's operator is final (no gate).
+template <int FAM, bool DUAL>
+__global__ __launch_bounds__(256) void k_line_factor(SmoothArgs S, const float* du, float* fac, size_t cap, int stride,
+                                                     int line0) {
+  extern __shared__ __align__(16) float sm[];
+  const LineGeo L = line_geo_at<FAM, DUAL>(S, blockIdx.x);
+  const int len = L.ln.z;
+  const int t = threadIdx.x;
+  const int k0 = 4 * t;
+  const float* od = du + (size_t)(line0 + blockIdx.x) * stride;
+  float* of = fac + (size_t)(line0 + blockIdx.x) * stride;
+  float v[15][4];
+#pragma unroll
+  for (int q = 0; q < 15; q++) {
+    const float4 p = k0 < len ? *reinterpret_cast<const float4*>(od + q * cap + k0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    v[q][0] = p.x;
+    v[q][1] = p.y;
+    v[q][2] = p.z;
+    v[q][3] = p.w;
+  }
+  float D[4][6], U[4][9], r[3];
+#pragma unroll
   for (int m = 0; m < 4; m++) {
-    const int k = k0 + m;
-    if (k < len) {
-      int n, nb, no, i, j;
-      row_index<FAM, DUAL>(L, k, n, nb, no, i, j);
-      if (!S.fixed[n]) store_x(S, n, no, x[m], DUAL);
+    if (k0 + m < len) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) D[m][q] = v[q][m];
+#pragma unroll
+      for (int q = 0; q < 9; q++) U[m][q] = v[6 + q][m];
+    } else {
+      set_identity(D[m], U[m], r);
     }
   }
+  line_factor4(sm, of, cap, t, len, D, U);
+}
+
+// Factored split variant, second kernel: r from k_line_r and the rows' factors (k_line_factor), four
+// rows per thread, solved by line_solve_factored: the pass moves r and x through the elimination
+// kept from the factorization instead of eliminating the blocks again.
+template <bool FINE, int FAM, bool DUAL>
+__global__ __launch_bounds__(256) void k_smooth_factored(SmoothArgs S, const float* rows, size_t cap, int stride,
+                                                         const float* fac, size_t fcap, int fstride, int line0) {
+  extern __shared__ __align__(16) float4 sm4[];
+  const int slot = xcd_block(blockIdx.x, gridDim.x);
+  const PassHead h = pass_head(S, slot);
+  const int t = threadIdx.x;
+  const int k0 = 4 * t;
+  const float* o = rows + (size_t)slot * stride;
+  const float* of = fac + (size_t)(line0 + slot) * fstride;
+  // the planes' loads do not depend on the line entry: issued with the gate's, at indices clamped
+  // into the line's stride, and masked once the length is known
+  const int kf = min(k0, fstride - 4), kr = min(k0, stride - 4);
+  float F[kFacPlanes][4], r[4][3], x[4][3];
+#pragma unroll
+  for (int q = 0; q < kFacPlanes; q++) {
+    const float4 p = *reinterpret_cast<const float4*>(of + q * fcap + kf);
+    F[q][0] = p.x;
+    F[q][1] = p.y;
+    F[q][2] = p.z;
+    F[q][3] = p.w;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const float4 p = *reinterpret_cast<const float4*>(o + c * cap + kr);
+    r[0][c] = p.x;
+    r[1][c] = p.y;
+    r[2][c] = p.z;
+    r[3][c] = p.w;
+  }
+  if (h.gated || h.ln.z <= 0) return;
+  const LineGeo L = line_geo_of<FAM, DUAL>(S, h.ln);
+  const int len = L.ln.z;
+  int nn[4], no[4];
+  uint8_t fx[4];
+  line_rows_fixed<FAM, DUAL>(S, L, k0, len, nn, no, fx);
+  // rows past the line: no factor and no right-hand side, so they contribute nothing and solve to 0
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    if (k0 + m >= len) {
+#pragma unroll
+      for (int q = 0; q < kFacPlanes; q++) F[q][m] = 0.f;
+      r[m][0] = r[m][1] = r[m][2] = 0.f;
+    }
+  }
+  line_solve_factored(sm4, blockDim.x, t, len, F, r, x);
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+    if (k0 + m < len && !fx[m]) store_x(S, nn[m], no[m], x[m], DUAL);
 }
 
 template <bool FINE, int FAM, int G>
@@ -341,6 +438,16 @@ template <bool FINE, int FAM, bool DUAL, class RDD>
 void launch_one(const SmoothArgs& S, RDD rdd, int count, int len, hipStream_t st, int variant, const Level& Lv, int line0) {
   float* rows = Lv.rows;
   const size_t cap = Lv.rows_cap;
+  if (variant == kFactorVariant) {
+    const int stride = (len + 3) & ~3;
+    if (!rows || (size_t)count * stride > cap || cap % 4 != 0 || !Lv.fac || Lv.du_stride < stride)
+      throw std::runtime_error("smoother: factored variant without its factors");
+    k_line_r<FINE, FAM, DUAL, RDD><<<dim3((len + 255) / 256, count), 256, 0, st>>>(S, rdd, rows, cap, stride);
+    const int block = ((len + 3) / 4 + 63) / 64 * 64;
+    k_smooth_factored<FINE, FAM, DUAL><<<count, block, (size_t)6 * block * sizeof(float4), st>>>(
+        S, rows, cap, stride, Lv.fac, Lv.du_cap, Lv.du_stride, line0);
+    return;
+  }
   if (variant == kSplitVariant || variant == kSplit4Variant) {
     const bool four = variant == kSplit4Variant;
     // a thread's rows of a plane are one aligned float2 (two rows) or float4 (four)
@@ -437,7 +544,8 @@ void smooth_set_attributes() {
 namespace {
 
 // one colour pass of level k with the given variant; `second` is the second pass of a one-colour
-// level's sweep (it reads the pair's other buffer and writes back to x)
+// level's sweep (it reads the pair's other buffer and writes back to x). On a batch_copy level the
+// pass writes its own layout only (launch_smooth refreshes the other one).
 void launch_colour(Solver& s, int k, int c, int variant, bool second) {
   Level& L = s.lv[k];
   int count = L.color_start[c + 1] - L.color_start[c];
@@ -462,13 +570,14 @@ void launch_colour(Solver& s, int k, int c, int variant, bool second) {
       other = other2;
     }
   }
+  if (L.batch_copy) other = nullptr;
   SmoothArgs S{colmajor ? L.opT : L.op, colmajor ? L.fixedT : L.fixed, L.Np, L.lines + L.color_start[c],
                own, colmajor ? L.bT : L.b, other, &s.scal->done, L.geo, rd_x};
   // each node of the colour's lines: its full row (fine 58 floats; coarse 150 values, 4 B each or 2 B
-  // as FP16 pairs) + b, x read, x write (dual-layout levels write x twice)
+  // as FP16 pairs) + b, x read, x write (dual-layout levels that copy x per pass write it twice)
   double nodes = (double)count * maxlen;
   const double row = L.fine ? 268.0 : (L.oph ? 336.0 : 636.0);
-  ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * (row + (L.dual ? 12.0 : 0.0)));
+  ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * (row + (other ? 12.0 : 0.0)));
   auto go = [&](auto rdd) {
     using R = decltype(rdd);
     if (L.dual) {
@@ -503,6 +612,7 @@ std::vector<int> variants_for(bool fine, int len, bool has_rows) {
   if (has_rows) {
     v.push_back(kSplitVariant);
     v.push_back(kSplit4Variant);
+    v.push_back(kFactorVariant);
   }
   if ((fine ? staged_shm<true, 0, 4>(len) : staged_shm<false, 0, 4>(len)) <= (size_t)kMaxLds) v.push_back(4);
   if ((fine ? staged_shm<true, 0, 8>(len) : staged_shm<false, 0, 8>(len)) <= (size_t)kMaxLds) v.push_back(8);
@@ -524,8 +634,10 @@ void smooth_alloc_rows(Solver& s, Level& L) {
   (void)s;
   if (L.rows) BD_HIP(hipFree(L.rows));
   if (L.du) BD_HIP(hipFree(L.du));
+  if (L.fac) BD_HIP(hipFree(L.fac));
   L.rows = nullptr;
   L.du = nullptr;
+  L.fac = nullptr;
   int most = 0;
   for (int c = 0; c < 2 * L.ncol; c++) most = std::max(most, L.color_start[c + 1] - L.color_start[c]);
   L.rows_cap = (size_t)most * ((L.max_len + 3) & ~3);  // room for either split variant's line stride
@@ -533,6 +645,7 @@ void smooth_alloc_rows(Solver& s, Level& L) {
   L.du_stride = (L.max_len + 3) & ~3;
   L.du_cap = (size_t)L.color_start[2 * L.ncol] * L.du_stride;
   BD_HIP(hipMalloc(&L.du, 15 * L.du_cap * sizeof(float)));
+  BD_HIP(hipMalloc(&L.fac, (size_t)kFacPlanes * L.du_cap * sizeof(float)));
 }
 
 // The line blocks of every line of level k (k_line_du per family), once its operator is final.
@@ -575,6 +688,16 @@ void smooth_line_blocks(Solver& s, int k) {
       go(OpF16Dev{colmajor ? L.ophT : L.oph, L.Np, L.hs});
     else
       go(OpF32{colmajor ? L.opT : L.op, L.Np});
+    // the family's factorizations from its line blocks
+    const int block = ((maxlen + 3) / 4 + 63) / 64 * 64;
+    const size_t shm = (size_t)24 * block * sizeof(float);
+    if (fam == 0) {
+      if (L.dual) k_line_factor<0, true><<<count, block, shm, s.stream>>>(S, L.du, L.fac, L.du_cap, L.du_stride, line0);
+      else k_line_factor<0, false><<<count, block, shm, s.stream>>>(S, L.du, L.fac, L.du_cap, L.du_stride, line0);
+    } else {
+      if (L.dual) k_line_factor<1, true><<<count, block, shm, s.stream>>>(S, L.du, L.fac, L.du_cap, L.du_stride, line0);
+      else k_line_factor<1, false><<<count, block, shm, s.stream>>>(S, L.du, L.fac, L.du_cap, L.du_stride, line0);
+    }
   }
 }
 
@@ -585,7 +708,18 @@ void launch_smooth(Solver& s, int k, int reverse) {
   const int nc = 2 * L.ncol;
   for (int q = 0; q < nc; q++) {
     int c = reverse ? nc - 1 - q : q;
-    launch_colour(s, k, c, L.variant[c < L.ncol ? 1 : 0], q == 1);
+    const int fam = c < L.ncol ? 1 : 0;
+    launch_colour(s, k, c, L.variant[fam], q == 1);
+    // batch_copy: after a family's last pass, its layout's x to the other layout (on a one-colour level
+    // the sweep's first pass wrote the pair's second buffers)
+    const bool last = q == nc - 1 || (((reverse ? nc - 2 - q : q + 1) < L.ncol ? 1 : 0) != fam);
+    if (L.batch_copy && last) {
+      const bool colmajor = fam == 1;
+      const bool pair = L.ncol == 1 && q == 0;
+      const float* own = colmajor ? (pair ? L.xT2 : L.xT) : (pair ? L.x2 : L.x);
+      float* other = colmajor ? (pair ? L.x2 : L.x) : (pair ? L.xT2 : L.xT);
+      launch_layout_copy(s, k, own, other, !colmajor);
+    }
   }
 }
 

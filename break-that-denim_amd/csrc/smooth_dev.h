@@ -314,9 +314,11 @@ __device__ __forceinline__ void gather_dispatch(int g, const SmoothArgs& S, cons
 constexpr int kRowF = 20;
 // LDS floats per reduced row: two record buffers
 constexpr int kSolveFloats = 2 * kRowF;
-// variant codes of the split smoother (row kernel + solve kernel, two or four rows per thread)
+// variant codes of the split smoother (row kernel + solve kernel, two or four rows per thread; the
+// factored one solves with the line's elimination kept from the operator's factorization)
 constexpr int kSplitVariant = 16;
 constexpr int kSplit4Variant = 17;
+constexpr int kFactorVariant = 18;
 
 __device__ __forceinline__ void put_row(float* B, int t, const float Di[6], const float U[9], const float r[3]) {
   float4* p = reinterpret_cast<float4*>(B + kRowF * t);
@@ -598,6 +600,254 @@ __device__ __forceinline__ void line_solve4(float* cr, int Lr, int t, int len, f
   recover_row(Did, U[2], U[3], r[3], x[2], xe, x[3]);
 }
 
+// ---- factored line solve ----
+// A line's block-tridiagonal matrix does not change between the passes over one operator, only r
+// does. So the elimination (four rows per thread: two odd-even levels in registers, then cyclic
+// reduction over the threads' first rows) runs once per operator and keeps, per row, what a pass
+// needs at the point the row is eliminated: its shifted pivot inverse Di (6), G = U_l Di (9; U_l
+// the coupling from the row's left neighbour at that point) and H = Di U_r (9; U_r its coupling to
+// the right neighbour). A pass then moves r only: eliminating row e subtracts G r_e from its left
+// neighbour's r and H^T r_e from its right one's; its solution is x_e = Di r_e - G^T x_l - H x_r.
+// kFacPlanes planes per row: Di, G, H.
+constexpr int kFacPlanes = 24;
+
+__device__ __forceinline__ void sym_part(const float W[9], float P[6]) {
+  P[0] = W[0];
+  P[1] = 0.5f * (W[1] + W[3]);
+  P[2] = 0.5f * (W[2] + W[6]);
+  P[3] = W[4];
+  P[4] = 0.5f * (W[5] + W[7]);
+  P[5] = W[8];
+}
+
+// Eliminates a row (D, coupled from its left neighbour by Ul and to its right neighbour by Ur): its
+// factor (Di, G = Ul Di, H = Di Ur), the left neighbour's update Wl = Ul Di Ul^T, the right one's
+// Pr = Ur^T Di Ur (both symmetric, to subtract) and the new coupling Un = -Ul Di Ur between them.
+__device__ __forceinline__ void factor_row(const float D[6], const float Ul[9], const float Ur[9], float Di[6],
+                                           float G[9], float H[9], float Wl[6], float Pr[6], float Un[9]) {
+  float Dif[9], W[9];
+  pivot_inv(D, Di);
+  sym_full(Di, Dif);
+  mm(Ul, Dif, G);
+  mm(Dif, Ur, H);
+  mmt(G, Ul, W);
+  sym_part(W, Wl);
+  mtm(Ur, H, W);
+  sym_part(W, Pr);
+  mm(Ul, H, W);
+#pragma unroll
+  for (int q = 0; q < 9; q++) Un[q] = -W[q];
+}
+
+// the row's factor to its planes (rows past the line are not written)
+__device__ __forceinline__ void put_factor(float* of, size_t cap, int k, int len, const float Di[6], const float G[9],
+                                           const float H[9]) {
+  if (k >= len) return;
+#pragma unroll
+  for (int q = 0; q < 6; q++) of[q * cap + k] = Di[q];
+#pragma unroll
+  for (int q = 0; q < 9; q++) of[(6 + q) * cap + k] = G[q];
+#pragma unroll
+  for (int q = 0; q < 9; q++) of[(15 + q) * cap + k] = H[q];
+}
+
+__device__ __forceinline__ void sub6(float D[6], const float P[6]) {
+#pragma unroll
+  for (int q = 0; q < 6; q++) D[q] -= P[q];
+}
+
+// Factorization of one line (thread t: rows 4t + m, D sym and U to the next row from the line blocks;
+// rows past the line are identity rows): writes every row's factor. ex: 8 floats of LDS per thread.
+// Every thread of the block calls it.
+__device__ __forceinline__ void line_factor4(float* ex, float* of, size_t cap, int t, int len, float (&D)[4][6],
+                                             const float (&U)[4][9]) {
+  const int k0 = 4 * t;
+  const int R4 = (len + 3) >> 2;
+  float Di[6], G[9], H[9], Wl[6], Pr[6];
+  // level 1: b between a and c, d between c and the next thread's a
+  float Uac[9], Uce[9], Pd[6];
+  factor_row(D[1], U[0], U[1], Di, G, H, Wl, Pr, Uac);
+  put_factor(of, cap, k0 + 1, len, Di, G, H);
+  sub6(D[0], Wl);
+  sub6(D[2], Pr);
+  factor_row(D[3], U[2], U[3], Di, G, H, Wl, Pd, Uce);
+  put_factor(of, cap, k0 + 3, len, Di, G, H);
+  sub6(D[2], Wl);
+  // level 2: c between a and the next thread's a
+  float Pc[6], Ua[9];
+  factor_row(D[2], Uac, Uce, Di, G, H, Wl, Pc, Ua);
+  put_factor(of, cap, k0 + 2, len, Di, G, H);
+  sub6(D[0], Wl);
+  float* e = ex + 8 * t;
+  if (t < R4) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) e[q] = Pd[q] + Pc[q];
+  }
+  __syncthreads();
+  if (t >= 1 && t < R4) {
+    const float* p = ex + 8 * (t - 1);
+#pragma unroll
+    for (int q = 0; q < 6; q++) D[0][q] -= p[q];
+  }
+  // cyclic reduction of the R4 first rows: at stride s the rows t = s mod 2s are eliminated between
+  // t - s and t + s. An eliminated row publishes Di (6), H (9) and Pr = U^T Di U (6); its left
+  // neighbour forms and writes its G. Two barriers a level (the buffer is reused).
+  for (int s = 1; s < R4; s <<= 1) {
+    __syncthreads();
+    const int m = t & (2 * s - 1);
+    if (t < R4 && m == s) {
+      float Dif[9], W[9], P[6];
+      pivot_inv(D[0], Di);
+      sym_full(Di, Dif);
+      mm(Dif, Ua, H);
+      mtm(Ua, H, W);
+      sym_part(W, P);
+#pragma unroll
+      for (int q = 0; q < 6; q++) of[q * cap + k0] = Di[q];
+#pragma unroll
+      for (int q = 0; q < 9; q++) of[(15 + q) * cap + k0] = H[q];
+      float* w = ex + 24 * t;
+#pragma unroll
+      for (int q = 0; q < 6; q++) w[q] = Di[q];
+#pragma unroll
+      for (int q = 0; q < 9; q++) w[6 + q] = H[q];
+#pragma unroll
+      for (int q = 0; q < 6; q++) w[15 + q] = P[q];
+    }
+    __syncthreads();
+    if (t < R4 && m == 0) {
+      if (t + s < R4) {
+        const float* w = ex + 24 * (t + s);
+        float Dif[9], W[9], Hr[9];
+        sym_full(w, Dif);
+#pragma unroll
+        for (int q = 0; q < 9; q++) Hr[q] = w[6 + q];
+        mm(Ua, Dif, G);
+        mmt(G, Ua, W);
+        sym_part(W, Wl);
+        sub6(D[0], Wl);
+        const int kr = 4 * (t + s);
+#pragma unroll
+        for (int q = 0; q < 9; q++) of[(6 + q) * cap + kr] = G[q];
+        mm(Ua, Hr, W);
+#pragma unroll
+        for (int q = 0; q < 9; q++) Ua[q] = -W[q];
+      }
+      if (t >= s) sub6(D[0], ex + 24 * (t - s) + 15);
+    }
+  }
+  // the last row standing
+  if (t == 0 && len > 0) {
+    pivot_inv(D[0], Di);
+#pragma unroll
+    for (int q = 0; q < 9; q++) G[q] = H[q] = 0.f;
+    put_factor(of, cap, 0, len, Di, G, H);
+  }
+}
+
+// One pass of a factored line (thread t: rows 4t + m): F the rows' factors (Di 0-5, G 6-14, H 15-23;
+// zero past the line), r their right-hand sides (zero past the line); returns x. lds: 6 float4 per
+// thread of the block. Every thread of the block calls it.
+__device__ __forceinline__ void line_solve_factored(float4* lds, int Lr, int t, int len, const float (&F)[kFacPlanes][4],
+                                                    float (&r)[4][3], float (&x)[4][3]) {
+  const int R4 = (len + 3) >> 2;
+  float4* Q = lds;
+  float4* CL = lds + Lr;      // two buffers: contributions to the left neighbour
+  float4* CRt = lds + 3 * Lr; // two buffers: contributions to the right neighbour
+  float4* X = lds + 5 * Lr;
+  float Fm[4][kFacPlanes];
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int q = 0; q < kFacPlanes; q++) Fm[m][q] = F[q][m];
+  x[0][0] = x[0][1] = x[0][2] = 0.f;
+  float v[3];
+  // level 1: b into a and c; d into c and the next thread's a
+  mv(Fm[1] + 6, r[1], v);
+  r[0][0] -= v[0]; r[0][1] -= v[1]; r[0][2] -= v[2];
+  mtv(Fm[1] + 15, r[1], v);
+  r[2][0] -= v[0]; r[2][1] -= v[1]; r[2][2] -= v[2];
+  mv(Fm[3] + 6, r[3], v);
+  r[2][0] -= v[0]; r[2][1] -= v[1]; r[2][2] -= v[2];
+  float q3[3], v2[3];
+  mtv(Fm[3] + 15, r[3], q3);
+  // level 2: c into a and the next thread's a
+  mv(Fm[2] + 6, r[2], v);
+  r[0][0] -= v[0]; r[0][1] -= v[1]; r[0][2] -= v[2];
+  mtv(Fm[2] + 15, r[2], v2);
+  if (t < R4) Q[t] = make_float4(q3[0] + v2[0], q3[1] + v2[1], q3[2] + v2[2], 0.f);
+  __syncthreads();
+  if (t >= 1 && t < R4) {
+    const float4 p = Q[t - 1];
+    r[0][0] -= p.x; r[0][1] -= p.y; r[0][2] -= p.z;
+  }
+  // cyclic reduction of the first rows, one barrier a level (buffers alternate)
+  int lvl = 0;
+  int top = 0;
+  for (int s = 1; s < R4; s <<= 1, lvl ^= 1) {
+    top = s;
+    const int m = t & (2 * s - 1);
+    if (t < R4 && m == s) {
+      float a[3], b[3];
+      mv(Fm[0] + 6, r[0], a);
+      mtv(Fm[0] + 15, r[0], b);
+      CL[lvl * Lr + t] = make_float4(a[0], a[1], a[2], 0.f);
+      CRt[lvl * Lr + t] = make_float4(b[0], b[1], b[2], 0.f);
+    }
+    __syncthreads();
+    if (t < R4 && m == 0) {
+      if (t + s < R4) {
+        const float4 p = CL[lvl * Lr + t + s];
+        r[0][0] -= p.x; r[0][1] -= p.y; r[0][2] -= p.z;
+      }
+      if (t >= s) {
+        const float4 p = CRt[lvl * Lr + t - s];
+        r[0][0] -= p.x; r[0][1] -= p.y; r[0][2] -= p.z;
+      }
+    }
+  }
+  // back substitution: the last row standing, then the rows of each level, widest stride first
+  if (t == 0 && R4 > 0) {
+    smv(Fm[0], r[0], x[0]);
+    X[0] = make_float4(x[0][0], x[0][1], x[0][2], 0.f);
+  }
+  __syncthreads();
+  for (int s = top; s >= 1; s >>= 1) {
+    const int m = t & (2 * s - 1);
+    if (t < R4 && m == s) {
+      const float4 xl = X[t - s];
+      const float4 xr = t + s < R4 ? X[t + s] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float l3[3] = {xl.x, xl.y, xl.z}, r3[3] = {xr.x, xr.y, xr.z};
+      float a[3], b[3];
+      smv(Fm[0], r[0], x[0]);
+      mtv(Fm[0] + 6, l3, a);
+      mv(Fm[0] + 15, r3, b);
+      x[0][0] -= a[0] + b[0]; x[0][1] -= a[1] + b[1]; x[0][2] -= a[2] + b[2];
+      X[t] = make_float4(x[0][0], x[0][1], x[0][2], 0.f);
+    }
+    __syncthreads();
+  }
+  float xe[3] = {0.f, 0.f, 0.f};
+  if (t + 1 < R4) {
+    const float4 p = X[t + 1];
+    xe[0] = p.x; xe[1] = p.y; xe[2] = p.z;
+  }
+  // the register levels' rows: c from a and e, then b from a and c, d from c and e
+  float a[3], b[3];
+  smv(Fm[2], r[2], x[2]);
+  mtv(Fm[2] + 6, x[0], a);
+  mv(Fm[2] + 15, xe, b);
+  x[2][0] -= a[0] + b[0]; x[2][1] -= a[1] + b[1]; x[2][2] -= a[2] + b[2];
+  smv(Fm[1], r[1], x[1]);
+  mtv(Fm[1] + 6, x[0], a);
+  mv(Fm[1] + 15, x[2], b);
+  x[1][0] -= a[0] + b[0]; x[1][1] -= a[1] + b[1]; x[1][2] -= a[2] + b[2];
+  smv(Fm[3], r[3], x[3]);
+  mtv(Fm[3] + 6, x[2], a);
+  mv(Fm[3] + 15, xe, b);
+  x[3][0] -= a[0] + b[0]; x[3][1] -= a[1] + b[1]; x[3][2] -= a[2] + b[2];
+}
+
 __device__ __forceinline__ void set_identity(float D[6], float U[9], float r[3]) {
   D[0] = 1.f; D[1] = 0.f; D[2] = 0.f; D[3] = 1.f; D[4] = 0.f; D[5] = 1.f;
 #pragma unroll
@@ -611,9 +861,9 @@ struct LineGeo {
 };
 
 template <int FAM, bool DUAL>
-__device__ __forceinline__ LineGeo line_geo_at(const SmoothArgs& S, int idx) {
+__device__ __forceinline__ LineGeo line_geo_of(const SmoothArgs& S, int4 ln) {
   LineGeo g;
-  g.ln = S.lines[idx];
+  g.ln = ln;
   int l = geo_layer(S.geo, g.ln.x);
   g.nx = S.geo.nx[l];
   g.ny = S.geo.ny[l];
@@ -629,10 +879,22 @@ __device__ __forceinline__ LineGeo line_geo_at(const SmoothArgs& S, int idx) {
   return g;
 }
 
-// the line of this workgroup (blocks spread over the XCDs)
 template <int FAM, bool DUAL>
-__device__ __forceinline__ LineGeo line_geo(const SmoothArgs& S) {
-  return line_geo_at<FAM, DUAL>(S, xcd_block(blockIdx.x, gridDim.x));
+__device__ __forceinline__ LineGeo line_geo_at(const SmoothArgs& S, int idx) {
+  return line_geo_of<FAM, DUAL>(S, S.lines[idx]);
+}
+
+// A pass's convergence gate and its line's table entry, loaded together at the start of a kernel (the
+// kernels test both before their first use, so neither load waits for the other's round trip).
+struct PassHead {
+  int4 ln;
+  bool gated;
+};
+__device__ __forceinline__ PassHead pass_head(const SmoothArgs& S, int idx) {
+  PassHead h;
+  h.ln = S.lines[idx];
+  h.gated = S.gate ? *S.gate != 0 : false;
+  return h;
 }
 
 // row k of the line: its index in the pass's layout, in the row-major layout (b), and in the other
@@ -647,12 +909,25 @@ __device__ __forceinline__ void row_index(const LineGeo& L, int k, int& n, int& 
   no = DUAL ? (FAM == 1 ? rm : cm) : n;
 }
 
+// rows k0 .. k0 + 3 of the line (clamped to its last row): their indices in the pass's layout and in
+// the other one, and their fixed flags, loaded before the solve so the stores need not wait for them
+template <int FAM, bool DUAL>
+__device__ __forceinline__ void line_rows_fixed(const SmoothArgs& S, const LineGeo& L, int k0, int len, int (&nn)[4],
+                                                int (&no)[4], uint8_t (&fx)[4]) {
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    int nb, i, j;
+    row_index<FAM, DUAL>(L, min(k0 + m, len - 1), nn[m], nb, no[m], i, j);
+    fx[m] = S.fixed[nn[m]];
+  }
+}
+
 __device__ __forceinline__ void store_x(const SmoothArgs& S, int n, int no, const float v[3], bool dual) {
   const int Np = S.Np;
   S.x[n] = v[0];
   S.x[Np + n] = v[1];
   S.x[2 * Np + n] = v[2];
-  if (dual) {
+  if (dual && S.xo) {
     S.xo[no] = v[0];
     S.xo[Np + no] = v[1];
     S.xo[2 * Np + no] = v[2];

@@ -343,6 +343,9 @@ int bd_set_floor(bd_solver* s, int enabled, double z);
                                   with lines longer than 257 nodes) */
 #define BD_SMOOTHER_SPLIT4 17  /* the split variant whose solve holds four rows per thread: two odd-even
                                   levels in registers before the parallel cyclic reduction */
+#define BD_SMOOTHER_FACTORED 18 /* the split variant whose solve reuses each line's elimination, factored
+                                   once per operator (four rows per thread, then cyclic reduction): a
+                                   pass moves only r and x through it */
 
 /* Line colours per family of one level's smoother (the reference uses 3: gpu/solver.ts:93-95; its CPU
  * solver also has 2, newtonMultigrid.ts:25-26). 3: lines solved together are three apart and share no

@@ -161,8 +161,8 @@ def test_every_smoother_variant_keeps_parity():
     """Each smoother kernel variant a level can run (bd_set_smoother), pinned on every level at once:
     one sweep, the V-cycle and PCG meet the same bars against the oracle on a 300 x 300 sheet, whose
     fine lines (300 nodes) take the direct, split and staged kernels."""
-    from bd.capi import (BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT, BD_SMOOTHER_SPLIT4, BD_SMOOTHER_STAGED4,
-                         BD_SMOOTHER_STAGED8, BD_SMOOTHER_DEFAULT)
+    from bd.capi import (BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT, BD_SMOOTHER_SPLIT4, BD_SMOOTHER_FACTORED,
+                         BD_SMOOTHER_STAGED4, BD_SMOOTHER_STAGED8, BD_SMOOTHER_DEFAULT)
     from bd.solver import GpuSolver
     n = 300
     pitch = 1e-3
@@ -192,7 +192,8 @@ def test_every_smoother_variant_keeps_parity():
     xv_ref = cpu.vcycle(b)
     xp_ref = cpu.pcg(b, params.pcg_iterations)
     nlev = len(gpu.stats()["level_nodes"])
-    for variant in (BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT, BD_SMOOTHER_SPLIT4, BD_SMOOTHER_STAGED4, BD_SMOOTHER_STAGED8):
+    for variant in (BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT, BD_SMOOTHER_SPLIT4, BD_SMOOTHER_FACTORED, BD_SMOOTHER_STAGED4,
+                    BD_SMOOTHER_STAGED8):
         pinned = []
         for k in range(nlev):
             for fam in (0, 1):
@@ -201,7 +202,7 @@ def test_every_smoother_variant_keeps_parity():
                     pinned.append((k, fam))
                 except RuntimeError:
                     gpu.set_smoother(k, fam, BD_SMOOTHER_DEFAULT)
-        if variant in (BD_SMOOTHER_SPLIT, BD_SMOOTHER_SPLIT4):
+        if variant in (BD_SMOOTHER_SPLIT, BD_SMOOTHER_SPLIT4, BD_SMOOTHER_FACTORED):
             assert (0, 0) in pinned and (0, 1) in pinned, "the fine level keeps a rows buffer"
         say(f"variant {variant}: pinned on {pinned}")
         gpu.write(BD_BUF_LEVEL_B, b, 0)
@@ -407,4 +408,64 @@ def test_one_colour_coarse_levels_keep_parity():
     e = rel_to_scale(gpu.read(BD_BUF_LEVEL_X, 0), cpu.vcycle(b))
     say(f"V-cycle with 3 colours again maxAbs/scale = {e:.2e}")
     assert e < 5e-3
+    gpu.close()
+
+
+def test_batch_copy_level_keeps_parity():
+    """A 520 x 520 sheet's fine level (lines of 520 nodes) keeps x in both layouts but refreshes the
+    other layout once per family's passes (one tiled copy) instead of in every pass. With the direct,
+    four-row split and factored kernels pinned, and with 3 and 1 line colours (the one-colour sweep
+    ping-pongs between the pair of buffers), a forward and a reverse sweep, the V-cycle and a
+    2-iteration PCG meet the usual bars against the oracle."""
+    from bd.capi import BD_SMOOTHER_DIRECT, BD_SMOOTHER_FACTORED, BD_SMOOTHER_SPLIT4
+    from bd.solver import GpuSolver
+    n = 520
+    pitch = 1e-3
+    mat = linear_sheet_material(pitch, 2e4, 1.6, 1e-5, 0.35, 2e4 / 2.4)
+    S = sheet_scene(n, pitch, mat)
+    params = make_params(history=False)
+    cpu = O.Oracle(S, params)
+    cpu.substep()
+    xs = cpu.x.astype(np.float32).astype(np.float64)
+    vs = cpu.v.astype(np.float32).astype(np.float64)
+    cpu.x[:] = xs
+    cpu.v[:] = vs
+    say("gpu: 520x520 sheet")
+    gpu = GpuSolver(S, params)
+    gpu.set_state(xs, vs)
+    cpu.predict()
+    cpu.assemble()
+    gpu.encode_predict()
+    gpu.encode_assemble(True)
+    cpu.load_fine_operator(gpu.stencil(0), gpu.fixed(0))
+    b = gpu.read(BD_BUF_RHS)
+    rng = np.random.default_rng(9)
+    x0 = rng.standard_normal(3 * S.node_count) * 1e-4
+    x0[np.repeat(cpu.fixed(0) == 1, 3)] = 0
+    for colours in (3, 1):
+        gpu.set_line_colors(0, colours)
+        cpu.set_line_colors(0, colours)
+        refs = {rev: cpu.smooth(0, x0.copy(), b, rev) for rev in (False, True)}
+        xv_ref = cpu.vcycle(b)
+        for variant in (BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT4, BD_SMOOTHER_FACTORED):
+            for fam in (0, 1):
+                gpu.set_smoother(0, fam, variant)
+            say(f"{colours} colour(s), variant {variant}")
+            for rev in (False, True):
+                gpu.write(BD_BUF_LEVEL_B, b, 0)
+                gpu.write(BD_BUF_LEVEL_X, x0, 0)
+                gpu.encode_smooth(0, rev)
+                e = rel_to_scale(gpu.read(BD_BUF_LEVEL_X, 0), refs[rev])
+                say(f"  sweep ({'reverse' if rev else 'forward'}) maxAbs/scale = {e:.2e}")
+                assert e < 1e-3
+            gpu.write(BD_BUF_LEVEL_B, b, 0)
+            gpu.encode_vcycle()
+            e = rel_to_scale(gpu.read(BD_BUF_LEVEL_X, 0), xv_ref)
+            say(f"  V-cycle maxAbs/scale = {e:.2e}")
+            assert e < 5e-3
+        gpu.write(BD_BUF_RHS, b)
+        gpu.encode_pcg()
+        e = rel_to_scale(gpu.read(BD_BUF_DELTA), cpu.pcg(b, params.pcg_iterations))
+        say(f"  PCG maxAbs/scale = {e:.2e}")
+        assert e < 2e-2
     gpu.close()
