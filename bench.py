@@ -87,6 +87,9 @@ def parse():
     ap.add_argument("--one-colour-max-line", type=int, default=513, metavar="NODES",
                     help="mixed and learned methods: levels whose lines have at most NODES nodes smooth with one "
                          "line colour per family (bd_set_line_colors); 0 keeps the reference's 3")
+    ap.add_argument("--fine-colours", type=int, default=3, choices=(2, 3),
+                    help="mixed and learned methods: line colours per family of the fine level (2: lines two apart "
+                         "together, their couplings majorised)")
     ap.add_argument("--batch-threads", type=int, default=16,
                     help="host threads of the patches' concurrent batch (grid+threads)")
     ap.add_argument("--sweeps", type=int, default=1,
@@ -250,6 +253,9 @@ def run_method(args, cfg, method, profile_frames):
                     if round(nodes ** 0.5) <= args.one_colour_max_line:
                         gpu.set_line_colors(k, 1)
                         colours[k] = 1
+            if method != "plain" and args.fine_colours != 3 and colours[0] == 3:
+                gpu.set_line_colors(0, args.fine_colours)
+                colours[0] = args.fine_colours
     gpu.sync()
     if method == "learned":
         t0 = time.perf_counter()

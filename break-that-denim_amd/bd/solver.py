@@ -118,8 +118,9 @@ class GpuSolver:
     substep = encode_substep
 
     def set_line_colors(self, level, colors):
-        """Line colours per family of one level's smoother: 3 (the reference) or 1 (every line of a
-        family at once against the others' lagged iterate), bd_set_line_colors."""
+        """Line colours per family of one level's smoother: 3 (the reference), 2 (lines two apart
+        together, their couplings majorised; split kernels only) or 1 (every line of a family at once
+        against the others' lagged iterate), bd_set_line_colors."""
         self._check(self.lib.bd_set_line_colors(self.h, int(level), int(colors)), "bd_set_line_colors")
 
     def set_pcg_mode(self, mode):

@@ -1056,12 +1056,18 @@ int bd_pick_near(bd_solver* h, const double* p, double radius, int window_radius
 int bd_set_line_colors(bd_solver* h, int level, int colors) {
   return guarded(h, [&](Solver& s) {
     if (level < 0 || level >= s.nlev) return fail(h, BD_ERR_INVALID, "bd_set_line_colors: no such level");
-    if (colors != 1 && colors != kLineColors) return fail(h, BD_ERR_INVALID, "bd_set_line_colors: colours must be 1 or 3");
+    if (colors < 1 || colors > kLineColors) return fail(h, BD_ERR_INVALID, "bd_set_line_colors: colours must be 1, 2 or 3");
     if (s.pcg_mode == BD_PCG_WORKGROUP && colors != kLineColors)
       return fail(h, BD_ERR_STATE, "bd_set_line_colors: the one-workgroup PCG smooths with 3 colours");
     Level& L = s.lv[level];
+    if (colors == 2 && !L.rows)
+      return fail(h, BD_ERR_INVALID, "bd_set_line_colors: 2 colours need the split kernels (lines of more than 257 nodes)");
     if (L.ncol == colors) return BD_OK;
     BD_HIP(hipStreamSynchronize(s.stream));
+    // two colours run only the split kernels
+    if (colors == 2)
+      for (int& v : L.variant)
+        if (v != BD_SMOOTHER_SPLIT && v != BD_SMOOTHER_SPLIT4 && v != BD_SMOOTHER_FACTORED) v = BD_SMOOTHER_FACTORED;
     dfree(L.lines);
     L.lines = nullptr;
     build_lines(s, L, colors);

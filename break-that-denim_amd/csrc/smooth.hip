@@ -571,8 +571,10 @@ void launch_colour(Solver& s, int k, int c, int variant, bool second) {
     }
   }
   if (L.batch_copy) other = nullptr;
+  // two colours run only the split kernels (the default rule would pick a fused one)
+  if (L.ncol == 2 && variant != kSplitVariant && variant != kSplit4Variant) variant = kFactorVariant;
   SmoothArgs S{colmajor ? L.opT : L.op, colmajor ? L.fixedT : L.fixed, L.Np, L.lines + L.color_start[c],
-               own, colmajor ? L.bT : L.b, other, &s.scal->done, L.geo, rd_x};
+               own, colmajor ? L.bT : L.b, other, &s.scal->done, L.geo, rd_x, L.ncol == 2 ? 1 : 0};
   // each node of the colour's lines: its full row (fine 58 floats; coarse 150 values, 4 B each or 2 B
   // as FP16 pairs) + b, x read, x write (dual-layout levels that copy x per pass write it twice)
   double nodes = (double)count * maxlen;
@@ -606,8 +608,14 @@ void launch_colour(Solver& s, int k, int c, int variant, bool second) {
 }
 
 // the variants a line of `len` rows can run: the direct kernel, the split one where the level keeps
-// a rows buffer, and the staged ones whose LDS fits
-std::vector<int> variants_for(bool fine, int len, bool has_rows) {
+// a rows buffer, and the staged ones whose LDS fits. With two colours only the split kernels: they
+// gather every line's r before any line of the pass is solved, so lines of one colour that couple
+// (two apart) read the lagged iterate, where a fused kernel would race.
+std::vector<int> variants_for(bool fine, int len, bool has_rows, int ncol) {
+  if (ncol == 2) {
+    if (!has_rows) return {};
+    return {kSplitVariant, kSplit4Variant, kFactorVariant};
+  }
   std::vector<int> v{0};
   if (has_rows) {
     v.push_back(kSplitVariant);
@@ -626,7 +634,7 @@ std::vector<int> smooth_variants(Solver& s, int k, int fam) {
   Level& L = s.lv[k];
   int maxlen = 0;
   for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
-  return variants_for(L.fine, maxlen, L.rows != nullptr);
+  return variants_for(L.fine, maxlen, L.rows != nullptr, L.ncol);
 }
 
 // The rows buffer of the split variant: 18 planes of (most lines of one colour) x (longest line).
@@ -661,7 +669,8 @@ void smooth_line_blocks(Solver& s, int k) {
     for (int l = 0; l < L.geo.nl; l++) maxlen = std::max(maxlen, fam == 1 ? L.geo.ny[l] : L.geo.nx[l]);
     const bool colmajor = L.dual && fam == 1;
     SmoothArgs S{colmajor ? L.opT : L.op, colmajor ? L.fixedT : L.fixed, L.Np, L.lines + L.color_start[c0],
-                 colmajor ? L.xT : L.x, colmajor ? L.bT : L.b, nullptr, nullptr, L.geo, colmajor ? L.xT : L.x};
+                 colmajor ? L.xT : L.x, colmajor ? L.bT : L.b, nullptr, nullptr, L.geo, colmajor ? L.xT : L.x,
+                 L.ncol == 2 ? 1 : 0};
     const dim3 grid((maxlen + 255) / 256, count);
     const int line0 = L.color_start[c0];
     auto go = [&](auto rdd) {

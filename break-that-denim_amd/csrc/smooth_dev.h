@@ -23,6 +23,9 @@ struct SmoothArgs {
   // coupling (3 colours); on a one-colour level the other buffer of a pair, so that every line reads
   // the lagged iterate, not the one its neighbours are writing
   const float* xr = nullptr;
+  // two line colours: lines two apart share a colour, so their couplings (the +-2 across the line)
+  // are majorised onto the diagonal like the +-2 along it (newtonMultigrid.ts:704-705)
+  int maj_across = 0;
 };
 
 // index strides of lattice offsets (di, dj) in the pass's layout
@@ -161,7 +164,8 @@ __device__ __forceinline__ void gather_row(const SmoothArgs& S, const RD& rd, in
         r[1] -= w * t[1];
         r[2] -= w * t[2];
       }
-      if (across == 0) maj += w * sym_frob(blk);  // +-2 along the line
+      // +-2 along the line; with two colours also +-2 across it
+      if (across == 0 || (S.maj_across && (across == 2 || across == -2))) maj += w * sym_frob(blk);
     }
   };
   if (FINE) {
@@ -245,7 +249,7 @@ __device__ __forceinline__ void gather_part(const SmoothArgs& S, const RD& rd, i
       r0 += w * t[0];
       r1 += w * t[1];
       r2 += w * t[2];
-      if (across == 0) maj += w * sym_frob(blk);
+      if (across == 0 || (S.maj_across && (across == 2 || across == -2))) maj += w * sym_frob(blk);
     }
   };
   if (FINE) {

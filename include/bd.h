@@ -349,11 +349,15 @@ int bd_set_floor(bd_solver* s, int enabled, double z);
 
 /* Line colours per family of one level's smoother (the reference uses 3: gpu/solver.ts:93-95; its CPU
  * solver also has 2, newtonMultigrid.ts:25-26). 3: lines solved together are three apart and share no
- * stencil coupling (the default). 1 (no counterpart in the reference): all lines of a family are solved
- * together, each against the lagged iterate of the others (line Jacobi across lines; Gauss-Seidel order
- * only between the two families), in a third of the launches, for coarse levels whose passes are bound
- * by launch latency. The V-cycle stays symmetric (the post-smoothing sweep is the pre-smoothing one
- * reversed). BD_ERR_INVALID for other counts; BD_ERR_STATE in BD_PCG_WORKGROUP mode for 1. */
+ * stencil coupling (the default). 2 (the reference CPU solver's option): lines two apart share a colour,
+ * their couplings are majorised onto the diagonal and each reads the other's lagged iterate; only the
+ * split kernels run it (levels with lines of more than 257 nodes, else BD_ERR_INVALID), and a level's
+ * other kernel choices fall back to BD_SMOOTHER_FACTORED. 1 (no counterpart in the reference): all
+ * lines of a family are solved together, each against the lagged iterate of the others (line Jacobi
+ * across lines; Gauss-Seidel order only between the two families), in a third of the launches, for
+ * coarse levels whose passes are bound by launch latency. The V-cycle stays symmetric (the
+ * post-smoothing sweep is the pre-smoothing one reversed). BD_ERR_INVALID for other counts;
+ * BD_ERR_STATE in BD_PCG_WORKGROUP mode for anything but 3. */
 int bd_set_line_colors(bd_solver* s, int level, int colors);
 
 /* Autotuner (no counterpart in the reference): for every level's v-line and u-line passes of the

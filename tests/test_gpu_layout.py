@@ -364,7 +364,9 @@ def test_one_colour_coarse_levels_keep_parity():
     for k in range(1, nlev):
         gpu.set_line_colors(k, 1)
         cpu.set_line_colors(k, 1)
-    with pytest.raises(RuntimeError, match="1 or 3"):
+    with pytest.raises(RuntimeError, match="1, 2 or 3"):
+        gpu.set_line_colors(1, 4)
+    with pytest.raises(RuntimeError, match="split kernels"):
         gpu.set_line_colors(1, 2)
     cpu.load_fine_operator(gpu.stencil(0), gpu.fixed(0))
     b = gpu.read(BD_BUF_RHS)
@@ -414,9 +416,10 @@ def test_one_colour_coarse_levels_keep_parity():
 def test_batch_copy_level_keeps_parity():
     """A 520 x 520 sheet's fine level (lines of 520 nodes) keeps x in both layouts but refreshes the
     other layout once per family's passes (one tiled copy) instead of in every pass. With the direct,
-    four-row split and factored kernels pinned, and with 3 and 1 line colours (the one-colour sweep
-    ping-pongs between the pair of buffers), a forward and a reverse sweep, the V-cycle and a
-    2-iteration PCG meet the usual bars against the oracle."""
+    four-row split and factored kernels pinned, and with 3, 2 and 1 line colours (two colours: lines
+    two apart together, their couplings majorised, split kernels only; the one-colour sweep ping-pongs
+    between the pair of buffers), a forward and a reverse sweep, the V-cycle and a 2-iteration PCG meet
+    the usual bars against the oracle."""
     from bd.capi import BD_SMOOTHER_DIRECT, BD_SMOOTHER_FACTORED, BD_SMOOTHER_SPLIT4
     from bd.solver import GpuSolver
     n = 520
@@ -442,12 +445,17 @@ def test_batch_copy_level_keeps_parity():
     rng = np.random.default_rng(9)
     x0 = rng.standard_normal(3 * S.node_count) * 1e-4
     x0[np.repeat(cpu.fixed(0) == 1, 3)] = 0
-    for colours in (3, 1):
+    for colours in (3, 2, 1):
         gpu.set_line_colors(0, colours)
         cpu.set_line_colors(0, colours)
         refs = {rev: cpu.smooth(0, x0.copy(), b, rev) for rev in (False, True)}
         xv_ref = cpu.vcycle(b)
-        for variant in (BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT4, BD_SMOOTHER_FACTORED):
+        variants = (BD_SMOOTHER_SPLIT4, BD_SMOOTHER_FACTORED) if colours == 2 else (
+            BD_SMOOTHER_DIRECT, BD_SMOOTHER_SPLIT4, BD_SMOOTHER_FACTORED)
+        if colours == 2:
+            with pytest.raises(RuntimeError, match="cannot run"):
+                gpu.set_smoother(0, 0, BD_SMOOTHER_DIRECT)
+        for variant in variants:
             for fam in (0, 1):
                 gpu.set_smoother(0, fam, variant)
             say(f"{colours} colour(s), variant {variant}")
