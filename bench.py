@@ -266,7 +266,9 @@ def run_method(args, cfg, method, profile_frames):
                  "loss_first_epoch": round(float(losses[0]), 5), "loss_last_epoch": round(float(losses[-1]), 5),
                  "seconds": round(time.perf_counter() - t0, 2)}
     # the fastest smoother kernel per level and family for this hierarchy (bd_autotune)
-    tuned = gpu.autotune(5) if not args.no_autotune else {}
+    # 15 V-cycles per candidate: the variants of a level differ by a few microseconds per V-cycle, so
+    # the fastest of fewer runs picks at random between them
+    tuned = gpu.autotune(15) if not args.no_autotune else {}
     for pin in args.pin:
         lvl, u, v = (int(q) for q in pin.split(":"))
         gpu.set_smoother(lvl, 0, u)

@@ -174,6 +174,7 @@ struct XferArgs {
   const int* gate;
   float* xcT;      // column-major coarse solution (dual-layout coarse level) or null
   float* xfT;      // column-major fine solution (dual-layout fine level) or null
+  float* bcT = nullptr;  // column-major coarse rhs (dual-layout coarse level) or null
 };
 
 // coarse correction at fine node f = (l, i, j): the up to 2 x 2 bracketing coarse nodes,

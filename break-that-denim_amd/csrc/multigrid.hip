@@ -253,6 +253,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     if (X.xc) X.xc[q * Npc + I] = 0.f;    // null for a plain restriction of a vector
     if (X.xcT) X.xcT[q * Npc + I] = 0.f;  // every entry of the permuted copy is cleared as well
   }
+  if (X.bcT) {
+    // b's column-major copy for the level's v-line passes, written here rather than by a transpose
+    // launch per visit (a strided store per node; the coarse levels are small)
+    int l, i, j;
+    node_ij(X.cgeo, I, l, i, j);
+    const int cm = X.cgeo.off[l] + i * X.cgeo.ny[l] + j;
+    for (int q = 0; q < 3; q++) X.bcT[q * Npc + cm] = s[q];
+  }
 }
 
 // levels without a column-major copy: one node per thread
@@ -484,8 +492,8 @@ void launch_restrict(Solver& s, int k) {
   Level& L = s.lv[k];
   Level& Cl = s.lv[k + 1];
   XferArgs X{lv_args(L), Cl.geo, Cl.N, Cl.Np, L.pw, Cl.fixed, L.r, Cl.b, Cl.x, L.x, pcg_gate(s),
-             Cl.dual ? Cl.xT : nullptr, nullptr};
-  ProfScope ps(s, PC_RESTRICT, L.N * 29.0 + Cl.N * (Cl.dual ? 37.0 : 25.0));
+             Cl.dual ? Cl.xT : nullptr, nullptr, Cl.dual ? Cl.bT : nullptr};
+  ProfScope ps(s, PC_RESTRICT, L.N * 29.0 + Cl.N * (Cl.dual ? 49.0 : 25.0));
   k_restrict<<<grid_for(Cl.N, 256), 256, 0, s.stream>>>(X);
 }
 
@@ -584,7 +592,8 @@ void launch_prolong_vec(Solver& s, int k, const float* xc, float* xf, bool set, 
   k_prolong_add<<<grid_for(L.N, 256), 256, 0, s.stream>>>(X);
 }
 
-// vcycle (newtonMultigrid.ts:495-528) on level k's x / b buffers, x zero on entry
+// vcycle (newtonMultigrid.ts:495-528) on level k's x / b buffers, x zero on entry (below the fine
+// level, the restriction that filled b also wrote its column-major copy)
 void vcycle(Solver& s, int k) {
   Level& L = s.lv[k];
   if (k == s.nlev - 1) {
@@ -592,11 +601,11 @@ void vcycle(Solver& s, int k) {
       launch_coarse_apply(s);
       return;
     }
-    launch_colmajor_b(s, k);
+    if (k == 0) launch_colmajor_b(s, k);
     for (int sw = 0; sw < s.P.coarse_sweeps; sw++) launch_smooth(s, k, sw & 1);
     return;
   }
-  launch_colmajor_b(s, k);
+  if (k == 0) launch_colmajor_b(s, k);
   for (int sw = 0; sw < s.P.smooth_sweeps; sw++) launch_smooth(s, k, 0);
   launch_residual(s, k, L.x, L.b, L.r);
   launch_restrict(s, k);
