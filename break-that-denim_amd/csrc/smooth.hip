@@ -159,12 +159,15 @@ __global__ __launch_bounds__(1024) void k_smooth_staged(SmoothArgs S, RDD rdd) {
 // on another's CU; writes the row's D, U and r to the rows planes (line slot * stride + row).
 template <bool FINE, int FAM, bool DUAL, class RDD>
 __global__ __launch_bounds__(256) void k_line_rows(SmoothArgs S, RDD rdd, float* rows, size_t cap, int stride) {
-  const int slot = blockIdx.y;
+  // (line, 256-row chunk) of this block, consecutive lines on one XCD so that the couplings and x
+  // a line shares with its neighbours are read through one L2
+  const int b = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int slot = b / gridDim.x;
   const PassHead h = pass_head(S, slot);
   if (h.gated || h.ln.z <= 0) return;
   const auto rd = rdd.live();
   const LineGeo L = line_geo_of<FAM, DUAL>(S, h.ln);
-  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int k = (b - slot * gridDim.x) * 256 + threadIdx.x;
   if (k >= L.ln.z) return;
   int n, nb, no, i, j;
   row_index<FAM, DUAL>(L, k, n, nb, no, i, j);
@@ -236,12 +239,15 @@ __global__ __launch_bounds__(512, 4) void k_smooth_rows(SmoothArgs S, const floa
 // kernel reads them from the operator itself and the round trip of 15 of the 18 floats is saved.
 template <bool FINE, int FAM, bool DUAL, class RDD>
 __global__ __launch_bounds__(256) void k_line_r(SmoothArgs S, RDD rdd, float* rows, size_t cap, int stride) {
-  const int slot = blockIdx.y;
+  // (line, 256-row chunk) of this block, consecutive lines on one XCD so that the couplings and x
+  // a line shares with its neighbours are read through one L2
+  const int b = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int slot = b / gridDim.x;
   const PassHead h = pass_head(S, slot);
   if (h.gated || h.ln.z <= 0) return;
   const auto rd = rdd.live();
   const LineGeo L = line_geo_of<FAM, DUAL>(S, h.ln);
-  const int k = blockIdx.x * 256 + threadIdx.x;
+  const int k = (b - slot * gridDim.x) * 256 + threadIdx.x;
   if (k >= L.ln.z) return;
   int n, nb, no, i, j;
   row_index<FAM, DUAL>(L, k, n, nb, no, i, j);
