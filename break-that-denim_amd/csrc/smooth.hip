@@ -582,9 +582,11 @@ void launch_colour(Solver& s, int k, int c, int variant, bool second) {
   SmoothArgs S{colmajor ? L.opT : L.op, colmajor ? L.fixedT : L.fixed, L.Np, L.lines + L.color_start[c],
                own, colmajor ? L.bT : L.b, other, &s.scal->done, L.geo, rd_x, L.ncol == 2 ? 1 : 0};
   // each node of the colour's lines: its full row (fine 58 floats; coarse 150 values, 4 B each or 2 B
-  // as FP16 pairs) + b, x read, x write (dual-layout levels that copy x per pass write it twice)
+  // as FP16 pairs) + b, x read, x write (dual-layout levels that copy x per pass write it twice). A
+  // one-colour pass holds every line of the family, so the mirrored couplings a row reads are other
+  // rows of the same pass: its unique bytes are the operator once (fine 32 planes, coarse 78).
   double nodes = (double)count * maxlen;
-  const double row = L.fine ? 268.0 : (L.oph ? 336.0 : 636.0);
+  const double row = L.ncol == 1 ? (L.fine ? 164.0 : (L.oph ? 192.0 : 348.0)) : (L.fine ? 268.0 : (L.oph ? 336.0 : 636.0));
   ProfScope ps(s, L.fine ? PC_SMOOTH_FINE : PC_SMOOTH_COARSE, nodes * (row + (other ? 12.0 : 0.0)));
   auto go = [&](auto rdd) {
     using R = decltype(rdd);
