@@ -376,56 +376,51 @@ __global__ __launch_bounds__(1024) void k_gram_row(GramArgs A) {
 // The pseudo-inverse of the symmetric Gram matrix G in FP64: Jacobi eigen-decomposition, then the
 // eigenvalues below kPinvTol times the largest dropped. Fields that are zero or (nearly) dependent
 // on the others then add nothing, and P Z G^+ (P Z)^T A stays an A-orthogonal projection.
+// One wave: lane l holds entry (l / 8, l % 8) of the matrix and of the rotations' product in
+// registers (a single thread kept both 8 x 8 arrays in scratch memory, 320 us a call). Each cyclic
+// rotation (p, q) reads its three pivots by lane shuffles and the partner entries of rows and columns
+// p and q by lane permutes, in the sequential algorithm's order (columns, then rows, then V).
 constexpr double kPinvTol = 1e-7;
+__device__ double wave_sum_f64(double x) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+  return __shfl(x, 0, 64);
+}
 __device__ void inverse_spd(const double* G, double* inv) {
   constexpr int K = BD_NET_FIELDS;
-  double a[K][K], v[K][K];
-  for (int i = 0; i < K; i++)
-    for (int j = 0; j < K; j++) {
-      a[i][j] = 0.5 * (G[i * K + j] + G[j * K + i]);
-      v[i][j] = i == j ? 1.0 : 0.0;
-    }
+  static_assert(K * K == 64, "one matrix entry per lane of a wave");
+  const int l = threadIdx.x, i = l / K, j = l % K;
+  double a = 0.5 * (G[i * K + j] + G[j * K + i]);
+  double v = i == j ? 1.0 : 0.0;
   for (int sweep = 0; sweep < 50; sweep++) {
-    double off = 0, tot = 0;
-    for (int i = 0; i < K; i++)
-      for (int j = 0; j < K; j++) {
-        tot += a[i][j] * a[i][j];
-        if (i != j) off += a[i][j] * a[i][j];
-      }
+    const double tot = wave_sum_f64(a * a), off = wave_sum_f64(i != j ? a * a : 0.0);
     if (!(off > 1e-30 * tot)) break;
     for (int p = 0; p < K - 1; p++)
       for (int q = p + 1; q < K; q++) {
-        if (a[p][q] == 0.0) continue;
-        const double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+        const double apq = __shfl(a, p * K + q, 64);
+        if (apq == 0.0) continue;
+        const double app = __shfl(a, p * K + p, 64), aqq = __shfl(a, q * K + q, 64);
+        const double th = (aqq - app) / (2.0 * apq);
         const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
         const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
-        for (int k = 0; k < K; k++) {
-          const double akp = a[k][p], akq = a[k][q];
-          a[k][p] = c * akp - sn * akq;
-          a[k][q] = sn * akp + c * akq;
-        }
-        for (int k = 0; k < K; k++) {
-          const double apk = a[p][k], aqk = a[q][k];
-          a[p][k] = c * apk - sn * aqk;
-          a[q][k] = sn * apk + c * aqk;
-        }
-        for (int k = 0; k < K; k++) {
-          const double vkp = v[k][p], vkq = v[k][q];
-          v[k][p] = c * vkp - sn * vkq;
-          v[k][q] = sn * vkp + c * vkq;
-        }
+        const int col = j == p ? i * K + q : (j == q ? i * K + p : l);
+        double o = __shfl(a, col, 64);
+        if (j == p) a = c * a - sn * o;
+        else if (j == q) a = sn * o + c * a;
+        o = __shfl(a, i == p ? q * K + j : (i == q ? p * K + j : l), 64);
+        if (i == p) a = c * a - sn * o;
+        else if (i == q) a = sn * o + c * a;
+        o = __shfl(v, col, 64);
+        if (j == p) v = c * v - sn * o;
+        else if (j == q) v = sn * o + c * v;
       }
   }
-  double lmax = 0;
-  for (int i = 0; i < K; i++) lmax = fmax(lmax, a[i][i]);
-  double d[K];
-  for (int i = 0; i < K; i++) d[i] = (a[i][i] > kPinvTol * lmax && a[i][i] > 0) ? 1.0 / a[i][i] : 0.0;
-  for (int i = 0; i < K; i++)
-    for (int j = 0; j < K; j++) {
-      double s = 0;
-      for (int k = 0; k < K; k++) s += v[i][k] * d[k] * v[j][k];
-      inv[i * K + j] = s;
-    }
+  double lmax = i == j ? a : 0.0;
+  for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_down(lmax, off, 64));
+  lmax = __shfl(lmax, 0, 64);
+  const double d = (i == j && a > kPinvTol * lmax && a > 0) ? 1.0 / a : 0.0;
+  double s = 0;
+  for (int k = 0; k < K; k++) s += __shfl(v, i * K + k, 64) * __shfl(d, k * K + k, 64) * __shfl(v, j * K + k, 64);
+  inv[l] = s;
 }
 
 struct LossScal {
@@ -441,9 +436,11 @@ struct LossScal {
   int pad;
 };
 
-// the inverse, the coefficients y_t and the sample's loss (one thread)
-__global__ void k_solve_small(LossScal* L, const double* s_t, int ntargets, int mask) {
+// the inverse (one wave), then the coefficients y_t and the sample's loss (its first lane)
+__global__ __launch_bounds__(64) void k_solve_small(LossScal* L, const double* s_t, int ntargets, int mask) {
   inverse_spd(L->G, L->Ginv);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   double loss = 0, wsum = 0;
   for (int t = 0; t < kTargets; t++) {
     const bool on = t < ntargets && s_t && s_t[t] > 0 && ((mask >> t) & 1);
@@ -798,13 +795,15 @@ __global__ __launch_bounds__(1024) void k_gram_col(CorrArgs A, int col) {
     for (int a = 0; a < BD_NET_FIELDS; a++) A.L->G[a * BD_NET_FIELDS + col] = tot[a];
 }
 
-// the symmetric part of G (FP32 rounding leaves it slightly asymmetric) and its inverse
-__global__ void k_gram_inverse(LossScal* L) {
-  for (int a = 0; a < BD_NET_FIELDS; a++)
-    for (int b = 0; b < a; b++) {
-      const double m = 0.5 * (L->G[a * BD_NET_FIELDS + b] + L->G[b * BD_NET_FIELDS + a]);
-      L->G[a * BD_NET_FIELDS + b] = L->G[b * BD_NET_FIELDS + a] = m;
-    }
+// the symmetric part of G (FP32 rounding leaves it slightly asymmetric) and its inverse (one wave)
+__global__ __launch_bounds__(64) void k_gram_inverse(LossScal* L) {
+  if (threadIdx.x == 0)
+    for (int a = 0; a < BD_NET_FIELDS; a++)
+      for (int b = 0; b < a; b++) {
+        const double m = 0.5 * (L->G[a * BD_NET_FIELDS + b] + L->G[b * BD_NET_FIELDS + a]);
+        L->G[a * BD_NET_FIELDS + b] = L->G[b * BD_NET_FIELDS + a] = m;
+      }
+  __syncthreads();
   inverse_spd(L->G, L->Ginv);
 }
 
@@ -952,7 +951,7 @@ void subspace(Learn& G, Solver& s, const float* op, const uint8_t* fixed, const 
                G.L->c, row};
     k_gram_row<<<std::min(kDotBlocks, grid_for(G.Nl, 1024)), 1024, 0, s.stream>>>(A);
   }
-  k_solve_small<<<1, 1, 0, s.stream>>>(G.L, s_t, b ? kTargets : 0, mask);
+  k_solve_small<<<1, 64, 0, s.stream>>>(G.L, s_t, b ? kTargets : 0, mask);
 }
 
 void backward(Learn& G, Solver& s, const uint8_t* fixed, const float* const* b) {
@@ -1128,7 +1127,7 @@ void learn_predict(Solver& s) {
     restrict_to(G, s, wk, nullptr);
     k_gram_col<<<std::min(kDotBlocks, grid_for(G.Nl, 1024)), 1024, 0, s.stream>>>(C, k);
   }
-  k_gram_inverse<<<1, 1, 0, s.stream>>>(G.L);
+  k_gram_inverse<<<1, 64, 0, s.stream>>>(G.L);
   G.have_z = true;
 }
 
