@@ -87,6 +87,9 @@ def parse():
     ap.add_argument("--one-colour-max-line", type=int, default=513, metavar="NODES",
                     help="mixed and learned methods: levels whose lines have at most NODES nodes smooth with one "
                          "line colour per family (bd_set_line_colors); 0 keeps the reference's 3")
+    ap.add_argument("--lag-coarsest", action="store_true",
+                    help="mixed and learned methods: factor the exact coarsest level once per frame, at its first "
+                         "Newton iteration (BD_COARSE_EXACT_LAGGED), instead of at every one (BD_COARSE_EXACT)")
     ap.add_argument("--fine-colours", type=int, default=3, choices=(2, 3),
                     help="mixed and learned methods: line colours per family of the fine level (2: lines two apart "
                          "together, their couplings majorised)")
@@ -205,7 +208,7 @@ def cpu_baseline(cfg, gpu_iters_per_newton, newton_per_frame):
 
 def method_params(cfg, method, args):
     import copy
-    from bd.capi import BD_COARSE_EXACT, BD_COARSE_SWEEPS, BD_PRECISION_FP16, BD_PRECISION_FP32
+    from bd.capi import BD_COARSE_EXACT, BD_COARSE_EXACT_LAGGED, BD_COARSE_SWEEPS, BD_PRECISION_FP16, BD_PRECISION_FP32
     params = copy.copy(cfg["params"])
     # the reference's schedule smooths once per level; the mixed and learned methods take --sweeps
     params.smooth_sweeps = 1 if method == "plain" else args.sweeps
@@ -215,7 +218,7 @@ def method_params(cfg, method, args):
         params.lag_coarse = 1
         params.precision = BD_PRECISION_FP32
     else:
-        params.coarse_solve = BD_COARSE_EXACT
+        params.coarse_solve = BD_COARSE_EXACT_LAGGED if args.lag_coarsest else BD_COARSE_EXACT
         params.max_coarsest_nodes = args.mixed_coarsest
         params.lag_coarse = 0
         params.precision = BD_PRECISION_FP16 if args.mixed_precision == "fp16" else BD_PRECISION_FP32

@@ -25,6 +25,7 @@ BD_PRECISION_FP32 = 0
 BD_PRECISION_FP16 = 1
 BD_COARSE_SWEEPS = 0
 BD_COARSE_EXACT = 1
+BD_COARSE_EXACT_LAGGED = 2
 BD_SMOOTHER_DEFAULT = -1
 BD_SMOOTHER_DIRECT = 0
 BD_SMOOTHER_STAGED4 = 4

@@ -318,7 +318,12 @@ struct Solver {
   uint32_t* pitems = nullptr;    // N node ids
   float ply_dist[3] = {0.f, 0.f, 0.f};  // contact distance of layer pairs 0-1, 0-2, 1-2 (m)
 
-  // exact coarsest solve (BD_COARSE_EXACT): banded Cholesky factor and dense inverse, FP64
+  // Newton iteration of the substep under way (0 outside substeps): BD_COARSE_EXACT_LAGGED factors
+  // the coarsest level only at iteration 0
+  int newton_it = 0;
+  bool coarse_factored = false;
+  // exact coarsest solve (BD_COARSE_EXACT, BD_COARSE_EXACT_LAGGED): banded Cholesky factor and dense
+  // inverse, FP64
   int cn = 0, cbw = 0;
   double* cband = nullptr;
   double* cinv = nullptr;
@@ -434,6 +439,10 @@ void launch_grip_terms(Solver& s);
 void launch_pick(Solver& s, bool ray);
 // coarse.hip
 void coarse_exact_alloc(Solver& s);
+// the V-cycle solves its coarsest level exactly (BD_COARSE_EXACT or BD_COARSE_EXACT_LAGGED)
+inline bool coarse_exact(const Solver& s) {
+  return s.P.coarse_solve == BD_COARSE_EXACT || s.P.coarse_solve == BD_COARSE_EXACT_LAGGED;
+}
 void launch_coarse_factor(Solver& s);
 void launch_coarse_apply(Solver& s);
 

@@ -262,7 +262,11 @@ void build_hierarchy(Solver& s) {
     launch_rap(s, k);
     launch_colmajor_op(s, k + 1);
   }
-  if (s.P.coarse_solve == BD_COARSE_EXACT) launch_coarse_factor(s);
+  // the lagged mode keeps the factorization of the substep's first Newton iteration
+  if (coarse_exact(s) && (s.P.coarse_solve == BD_COARSE_EXACT || s.newton_it == 0 || !s.coarse_factored)) {
+    launch_coarse_factor(s);
+    s.coarse_factored = true;
+  }
   s.hierarchy = true;
   if (s.learned) learn_predict(s);
 }
@@ -340,6 +344,7 @@ void newton_post(Solver& s, int it) {
 }
 
 void newton_iteration(Solver& s, int it, bool coarse) {
+  s.newton_it = it;
   newton_pre(s, coarse);
   pcg_solve(s);
   newton_post(s, it);
@@ -359,6 +364,7 @@ void substep_begin(Solver& s) {
 }
 
 void substep_end(Solver& s) {
+  s.newton_it = 0;
   launch_commit(s);
   if (s.P.history) {
     launch_damage(s);
@@ -431,7 +437,8 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
     g_create_error = "bd_create: node_count does not match the layers";
     return BD_ERR_INVALID;
   }
-  if (prm->coarse_solve != BD_COARSE_SWEEPS && prm->coarse_solve != BD_COARSE_EXACT) {
+  if (prm->coarse_solve != BD_COARSE_SWEEPS && prm->coarse_solve != BD_COARSE_EXACT &&
+      prm->coarse_solve != BD_COARSE_EXACT_LAGGED) {
     g_create_error = "bd_create: unknown coarse_solve";
     return BD_ERR_INVALID;
   }
@@ -626,7 +633,7 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
       }
     }
     smooth_set_attributes();
-    if (s.P.coarse_solve == BD_COARSE_EXACT) coarse_exact_alloc(s);
+    if (coarse_exact(s)) coarse_exact_alloc(s);
     s.stats.level_count = s.nlev;
     for (int k = 0; k < s.nlev && k < BD_MAX_LEVELS; k++) s.stats.level_nodes[k] = s.lv[k].N;
   } catch (const std::exception& e) {
@@ -2005,6 +2012,7 @@ int bd_batch_substep(bd_batch* b) {
       for (int it = 0; it < B.newton; it++) {
         for (size_t i = 0; i < B.solvers.size(); i++) {
           Solver& s = B.solvers[i]->s;
+          s.newton_it = it;
           newton_pre(s, it == 0 || !s.P.lag_coarse);
           launch_pcg_setup(s);
           BD_HIP(hipEventRecord(B.ready[i], s.stream));

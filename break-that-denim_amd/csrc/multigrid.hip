@@ -597,7 +597,7 @@ void launch_prolong_vec(Solver& s, int k, const float* xc, float* xf, bool set, 
 void vcycle(Solver& s, int k) {
   Level& L = s.lv[k];
   if (k == s.nlev - 1) {
-    if (s.P.coarse_solve == BD_COARSE_EXACT) {
+    if (coarse_exact(s)) {
       launch_coarse_apply(s);
       return;
     }

@@ -381,7 +381,7 @@ void wg_prepare(Solver& s) {
   Y.ncol = s.lv[0].ncol;
   Y.smooth_sweeps = std::max(1, s.P.smooth_sweeps);
   Y.coarse_sweeps = s.P.coarse_sweeps;
-  Y.exact = s.P.coarse_solve == BD_COARSE_EXACT ? 1 : 0;
+  Y.exact = coarse_exact(s) ? 1 : 0;
   Y.cn = s.cn;
   Y.cinv = s.cinv;
   Y.sc = s.scal;

@@ -146,6 +146,12 @@ typedef struct bd_params {
 
 #define BD_COARSE_SWEEPS 0     /* coarse_sweeps colour-ordered line sweeps (the reference's V-cycle) */
 #define BD_COARSE_EXACT 1      /* exact FP64 solve: banded Cholesky per hierarchy build, dense inverse applied */
+#define BD_COARSE_EXACT_LAGGED 2 /* as BD_COARSE_EXACT, but factored once per substep, at its first Newton
+                                    iteration: the later iterations' V-cycles (their Galerkin levels rebuilt)
+                                    solve the coarsest level with that factorization, an exact solve of the
+                                    substep's first coarsest operator. Outside a substep every build factors.
+                                    Saves two factorizations in three, but a later iteration whose coarsest
+                                    operator moved far from the first's (new contacts) can converge slowly */
 
 #define BD_PRECISION_FP32 0    /* FP32 operators and smoothers, FP64 dot products */
 #define BD_PRECISION_FP16 1    /* the V-cycle (smoother gathers, its residuals) reads each Galerkin coarse level's

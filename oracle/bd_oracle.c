@@ -1368,7 +1368,9 @@ static void band_solve(int n, int bw, const double* L, const double* b, double* 
   }
 }
 
-static int build_hierarchy(or_solver* S) {
+/* refactor: the coarsest level's exact factorization is rebuilt (always for BD_COARSE_EXACT; for
+   BD_COARSE_EXACT_LAGGED only at a substep's first Newton iteration or when there is none) */
+static int build_hierarchy_(or_solver* S, int refactor) {
   free_levels(S, 1);
   S->nlev = 1;
   op_t* cur = &S->lv[0];
@@ -1379,11 +1381,14 @@ static int build_hierarchy(or_solver* S) {
     S->nlev++;
     cur = &S->lv[S->nlev - 1];
   }
+  const int exact = S->P.coarse_solve == BD_COARSE_EXACT || S->P.coarse_solve == BD_COARSE_EXACT_LAGGED;
+  if (exact && S->P.coarse_solve == BD_COARSE_EXACT_LAGGED && !refactor && S->cL) return 0;
   free(S->cL);
   S->cL = NULL;
-  if (S->P.coarse_solve == BD_COARSE_EXACT && band_cholesky(cur, &S->cn, &S->cbw, &S->cL) != 0) return -1;
+  if (exact && band_cholesky(cur, &S->cn, &S->cbw, &S->cL) != 0) return -1;
   return 0;
 }
+static int build_hierarchy(or_solver* S) { return build_hierarchy_(S, 1); }
 
 /* ---------------------------------------------------------------- line smoother */
 
@@ -1938,7 +1943,7 @@ double or_newton_step(or_solver* S, int first) {
   if (S->P.lag_coarse && S->lagged && !first) {
     /* keep coarse levels and the level-0 prolongation of the first iteration */
   } else {
-    build_hierarchy(S);
+    build_hierarchy_(S, first);
     S->lagged = 1;
   }
   double* d = calloc(3 * N, sizeof(double));

@@ -303,3 +303,43 @@ def test_exact_coarse_solve(coupon_pair, max_coarsest, fp16):
     e = rel_to_scale(gpu.read(BD_BUF_DELTA), ref.pcg(b, params.pcg_iterations))
     say(f"PCG step (exact coarsest) maxAbs/scale = {e:.2e}")
     assert e < 2e-2
+
+
+def test_lagged_exact_coarse_solve(coupon_pair):
+    """BD_COARSE_EXACT_LAGGED: the coarsest level factored once per substep, at its first Newton
+    iteration, while the Galerkin levels are rebuilt at every one (lag_coarse off). The profile counts
+    one factorization per substep against one per Newton iteration for BD_COARSE_EXACT; four substeps
+    of 2 Newton iterations, each solved to rel-res 1e-6, follow the oracle's substeps of the same mode,
+    and the lagged and the exact runs end within the same bar of each other."""
+    from bd.capi import BD_COARSE_EXACT, BD_COARSE_EXACT_LAGGED
+    from bd.solver import GpuSolver
+    S = coupon_pair[0]
+    out = {}
+    for mode in (BD_COARSE_EXACT, BD_COARSE_EXACT_LAGGED):
+        params = make_params(coarse_solve=mode, lag_coarse=False, history=False, pcg_rel_tol=1e-6,
+                             pcg_max_iterations=300)
+        ref = O.Oracle(S, params)
+        gpu = GpuSolver(S, params)
+        xs, _ = copy_state(ref, gpu)
+        gpu.profile_enable(True)
+        gpu.profile_read()
+        say(f"gpu: 4 substeps, coarse_solve {mode}")
+        for _ in range(4):
+            ref.substep()
+            gpu.encode_substep()
+            st = gpu.stats()
+            n = st["newton_iterations"]
+            assert n == 2
+            assert max(st["rel_residual"][:n]) <= 1.05e-6
+        factors = gpu.profile_read().get("coarse_factor", (0, 0.0, 0.0))[0]
+        xg = gpu.read(0)
+        moved = np.max(np.abs(ref.x - xs))
+        err = np.max(np.abs(xg - ref.x))
+        say(f"  {factors} coarsest factorizations; max |x_gpu - x_oracle| = {err * 1e6:.3f} um, motion {moved * 1e6:.1f} um")
+        assert err < 0.02 * moved
+        out[mode] = (xg, factors, moved)
+        gpu.close()
+    assert out[BD_COARSE_EXACT][1] == 8 and out[BD_COARSE_EXACT_LAGGED][1] == 4
+    d = np.max(np.abs(out[BD_COARSE_EXACT][0] - out[BD_COARSE_EXACT_LAGGED][0]))
+    say(f"lagged against exact: {d * 1e6:.3f} um")
+    assert d < 0.02 * out[BD_COARSE_EXACT][2]

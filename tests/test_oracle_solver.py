@@ -365,6 +365,48 @@ def test_exact_coarse_solve_is_the_inverse():
     assert np.max(np.abs(out[1] - out[0])) <= 5e-4 * np.max(np.abs(out[0]))
 
 
+def test_lagged_exact_coarse_solve():
+    """BD_COARSE_EXACT_LAGGED (an extension): a hierarchy build that refactors (outside a substep, or a
+    substep's first Newton iteration) gives the BD_COARSE_EXACT V-cycle bit for bit; a later Newton
+    iteration's build keeps the first factorization, so its V-cycle differs from the refactored one
+    by the change of the coarsest operator, and a one-iteration substep is BD_COARSE_EXACT's."""
+    S = compile_scene([mg_coupon(40, 30)])
+    N = S.node_count
+    r = lcg(5)
+    b = np.array([r() for _ in range(3 * N)])
+    runs = {}
+    for mode in (1, 2):
+        q = ref_params()
+        q.coarse_solve = mode
+        o = O.Oracle(S, q)
+        o.predict()
+        o.newton_step(True)       # refactors in both modes
+        bb = b.copy()
+        bb.reshape(N, 3)[o.fixed(0) == 1] = 0
+        first = o.vcycle(bb)
+        o.newton_step(False)      # the lagged mode keeps the factorization of the step before
+        later = o.vcycle(bb)
+        o.build_hierarchy()       # outside a substep: refactors in both modes
+        again = o.vcycle(bb)
+        runs[mode] = (first, later, again, o.x.copy())
+    (e1, e2, e3, _), (l1, l2, l3, _) = runs[1], runs[2]
+    assert np.array_equal(e1, l1)
+    assert np.array_equal(e3, l3)
+    d = np.max(np.abs(l2 - e2)) / np.max(np.abs(e2))
+    assert 0 < d < 1e-2
+    one = {}
+    for mode in (1, 2):
+        q = ref_params()
+        q.coarse_solve = mode
+        q.newton_iterations = 1
+        q.newton_max = 1
+        o = O.Oracle(S, q)
+        for _ in range(3):
+            o.substep()
+        one[mode] = o.x.copy()
+    assert np.array_equal(one[1], one[2])
+
+
 def test_fp16_rounding_matches_ieee_half():
     """The oracle's FP16 rounding of the V-cycle operators (BD_PRECISION_FP16) is IEEE binary16 round
     to nearest even (numpy's float32 -> float16 conversion), subnormals included, saturated at the
