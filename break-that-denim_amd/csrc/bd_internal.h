@@ -204,6 +204,12 @@ struct Solver {
   DevLayer* layers = nullptr;  // device copy
   Geo geo{};
   hipStream_t stream = nullptr;
+  // side stream of the exact coarsest factorization (default priority, like `stream`): it starts when
+  // the hierarchy is built and the V-cycle waits for it only at its first coarse solve, so the
+  // factorization (one workgroup, then the inverse's 55) runs beside the fine and coarse smoothing
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_rap = nullptr, ev_fact = nullptr;
+  bool fact_pending = false;
   std::string err;
 
   // node state (3 planes each)
@@ -455,6 +461,13 @@ void launch_coarse_apply(Solver& s);
 inline int grid_for(int n, int block) { return (n + block - 1) / block; }
 
 // Brackets the launches in its scope with HIP events while profiling is on.
+// the solver's stream waits for a coarsest factorization still running on the side stream
+inline void coarse_join(Solver& s) {
+  if (!s.fact_pending) return;
+  BD_HIP(hipStreamWaitEvent(s.stream, s.ev_fact, 0));
+  s.fact_pending = false;
+}
+
 struct ProfScope {
   Solver& s;
   int rec = -1;

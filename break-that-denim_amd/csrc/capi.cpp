@@ -238,6 +238,10 @@ void destroy_all(Solver& s) {
   s.prof_pool.clear();
   release_graph(s);
   wg_free(s);
+  if (s.aux) (void)hipStreamSynchronize(s.aux);
+  if (s.aux) (void)hipStreamDestroy(s.aux);
+  if (s.ev_rap) (void)hipEventDestroy(s.ev_rap);
+  if (s.ev_fact) (void)hipEventDestroy(s.ev_fact);
   if (s.stream) (void)hipStreamDestroy(s.stream);
 }
 
@@ -264,7 +268,16 @@ void build_hierarchy(Solver& s) {
   }
   // the lagged mode keeps the factorization of the substep's first Newton iteration
   if (coarse_exact(s) && (s.P.coarse_solve == BD_COARSE_EXACT || s.newton_it == 0 || !s.coarse_factored)) {
+    // on the side stream, after everything queued so far (the RAP it reads, the V-cycles that read the
+    // previous factorization); its kernels and profile events go there while the streams are swapped
+    coarse_join(s);
+    BD_HIP(hipEventRecord(s.ev_rap, s.stream));
+    BD_HIP(hipStreamWaitEvent(s.aux, s.ev_rap, 0));
+    std::swap(s.stream, s.aux);
     launch_coarse_factor(s);
+    std::swap(s.stream, s.aux);
+    BD_HIP(hipEventRecord(s.ev_fact, s.aux));
+    s.fact_pending = true;
     s.coarse_factored = true;
   }
   s.hierarchy = true;
@@ -365,6 +378,7 @@ void substep_begin(Solver& s) {
 
 void substep_end(Solver& s) {
   s.newton_it = 0;
+  coarse_join(s);
   launch_commit(s);
   if (s.P.history) {
     launch_damage(s);
@@ -485,6 +499,9 @@ int bd_create(const bd_scene* sc, const bd_params* prm, bd_solver** out) {
     s.C = sc->crossing_count;
     s.nl = sc->layer_count;
     BD_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    BD_HIP(hipStreamCreateWithFlags(&s.aux, hipStreamNonBlocking));
+    BD_HIP(hipEventCreateWithFlags(&s.ev_rap, hipEventDisableTiming));
+    BD_HIP(hipEventCreateWithFlags(&s.ev_fact, hipEventDisableTiming));
     s.geo.nl = s.nl;
     for (int l = 0; l < s.nl; l++) {
       s.layers_h[l] = dev_layer(sc->layers[l]);
@@ -1470,6 +1487,7 @@ int bd_apply_noalloc(bd_solver* h, int level, const float* x, float* y, double* 
 
 int bd_sync(bd_solver* h) {
   return guarded(h, [&](Solver& s) {
+    coarse_join(s);
     BD_HIP(hipStreamSynchronize(s.stream));
     return BD_OK;
   });
@@ -2015,6 +2033,7 @@ int bd_batch_substep(bd_batch* b) {
           s.newton_it = it;
           newton_pre(s, it == 0 || !s.P.lag_coarse);
           launch_pcg_setup(s);
+          coarse_join(s);
           BD_HIP(hipEventRecord(B.ready[i], s.stream));
           BD_HIP(hipStreamWaitEvent(B.stream, B.ready[i], 0));
         }

@@ -598,6 +598,7 @@ void vcycle(Solver& s, int k) {
   Level& L = s.lv[k];
   if (k == s.nlev - 1) {
     if (coarse_exact(s)) {
+      coarse_join(s);
       launch_coarse_apply(s);
       return;
     }

@@ -420,6 +420,7 @@ void wg_free(Solver& s) {
 
 // One solve (k_pcg_setup has run on the stream).
 void launch_pcg_wg(Solver& s) {
+  coarse_join(s);
   ProfScope ps(s, PC_PCG_VECTOR, 0.0);
   k_pcg_wg<<<1, kWgThreads, s.wg_lds, s.stream>>>(static_cast<const WgSys*>(s.wg_sys));
 }
