@@ -1137,21 +1137,17 @@ int bd_autotune(bd_solver* h, int reps, int32_t* chosen) {
     BD_HIP(hipEventCreate(&e0));
     BD_HIP(hipEventCreate(&e1));
     Level& F = s.lv[0];
-    auto time_vcycles = [&]() {
-      float best = 0.f;
-      for (int r = 0; r < reps + 1; r++) {
-        launch_zero(s, F.b, 3 * (size_t)F.Np);
-        launch_zero(s, F.x, 3 * (size_t)F.Np);
-        if (F.dual) launch_zero(s, F.xT, 3 * (size_t)F.Np);
-        BD_HIP(hipEventRecord(e0, s.stream));
-        vcycle(s, 0);
-        BD_HIP(hipEventRecord(e1, s.stream));
-        BD_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        BD_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (r > 0 && (r == 1 || ms < best)) best = ms;  // the first run warms up
-      }
-      return best;
+    auto time_vcycle = [&]() {
+      launch_zero(s, F.b, 3 * (size_t)F.Np);
+      launch_zero(s, F.x, 3 * (size_t)F.Np);
+      if (F.dual) launch_zero(s, F.xT, 3 * (size_t)F.Np);
+      BD_HIP(hipEventRecord(e0, s.stream));
+      vcycle(s, 0);
+      BD_HIP(hipEventRecord(e1, s.stream));
+      BD_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      BD_HIP(hipEventElapsedTime(&ms, e0, e1));
+      return ms;
     };
     launch_open_gate(s);
     for (int k = 0; k < s.nlev; k++)
@@ -1161,17 +1157,20 @@ int bd_autotune(bd_solver* h, int reps, int32_t* chosen) {
           s.lv[k].variant[fam] = vs.empty() ? -1 : vs[0];
           continue;
         }
-        float best = 0.f;
-        int pick = -1;
-        for (int v : vs) {
-          s.lv[k].variant[fam] = v;
-          const float ms = time_vcycles();
-          if (pick < 0 || ms < best) {
-            best = ms;
-            pick = v;
+        // the candidates in turn, `reps` rounds after a warm-up round, so that a slow stretch of the
+        // device (clocks, other work) does not fall on one candidate's runs only; the fastest run of
+        // each candidate counts
+        std::vector<float> best(vs.size(), 0.f);
+        for (int r = 0; r < reps + 1; r++)
+          for (size_t c = 0; c < vs.size(); c++) {
+            s.lv[k].variant[fam] = vs[c];
+            const float ms = time_vcycle();
+            if (r > 0 && (r == 1 || ms < best[c])) best[c] = ms;
           }
-        }
-        s.lv[k].variant[fam] = pick;
+        size_t pick = 0;
+        for (size_t c = 1; c < vs.size(); c++)
+          if (best[c] < best[pick]) pick = c;
+        s.lv[k].variant[fam] = vs[pick];
       }
     BD_HIP(hipEventDestroy(e0));
     BD_HIP(hipEventDestroy(e1));
