@@ -198,6 +198,9 @@ void release_graph(Solver& s) {
 }
 
 void destroy_all(Solver& s) {
+  // nothing may still run on either stream while the buffers go
+  if (s.aux) (void)hipStreamSynchronize(s.aux);
+  if (s.stream) (void)hipStreamSynchronize(s.stream);
   void* ptrs[] = {s.layers, s.x, s.v, s.dx, s.pred, s.delta, s.rhs, s.p, s.ap, s.last_update, s.mass, s.cn0, s.pinned,
                   s.layer_of, s.rest, s.intact, s.fail_site, s.tails, s.softening, s.cut_mask, s.flags, s.strain,
                   s.damage, s.area, s.sep, s.pullmark, s.strength, s.grip_of, s.grip_w, s.grip_off, s.grip_goal_k,
@@ -1087,6 +1090,7 @@ int bd_set_line_colors(bd_solver* h, int level, int colors) {
     if (colors == 2 && !L.rows)
       return fail(h, BD_ERR_INVALID, "bd_set_line_colors: 2 colours need the split kernels (lines of more than 257 nodes)");
     if (L.ncol == colors) return BD_OK;
+    coarse_join(s);
     BD_HIP(hipStreamSynchronize(s.stream));
     // two colours run only the split kernels
     if (colors == 2)
