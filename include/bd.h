@@ -368,8 +368,8 @@ int bd_set_line_colors(bd_solver* s, int level, int colors);
 
 /* Autotuner (no counterpart in the reference): for every level's v-line and u-line passes of the
  * current hierarchy, the smoother kernel variant (BD_SMOOTHER_*) whose V-cycles run fastest, chosen
- * greedily level by level with each variant timed inside whole V-cycles (zero b and x; the fastest
- * of `reps`). chosen receives 2 entries per level (u, v); may be NULL. Results of a pass differ
+ * greedily level by level with each variant timed inside whole V-cycles (zero b and x; the level's
+ * candidates in turn over `reps` rounds, the fastest run of each counting). chosen receives 2 entries per level (u, v); may be NULL. Results of a pass differ
  * between variants only by FP32 summation order, and for BD_SMOOTHER_SPLIT4 by its elimination order
  * (the same shifted-pivot line solve, rows reduced in another order). */
 int bd_autotune(bd_solver* s, int reps, int32_t* chosen);
